@@ -1,0 +1,975 @@
+// fhea_engine.hip -- the C-ABI (include/fhea.h): context, device tables, arenas,
+// the op-code dispatcher and the batched composite pipelines.
+//
+// Replaces fhe_accel_top (hardware_fpga_model/hls/src/fhe_accel_top.cpp:90-328),
+// SimMemoryBus (cpp_host_model/src/sim_memory_bus.cpp) and the composite
+// sequences of FHEHostDriver (cpp_host_model/src/fhe_host_driver.cpp:248-404).
+// Ciphertexts and keys stay resident in HBM; a composite is a fixed chain of
+// batched kernels over a device workspace (no burst_load/burst_store copies,
+// fhe_accel_top.cpp:36-86).
+#include <algorithm>
+#include <cstddef>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "fhea_engine.hpp"
+
+using namespace fhea;
+
+namespace fhea {
+
+int check_hip(hipError_t e) { return e == hipSuccess ? FHEA_OK : FHEA_ERR_DEVICE; }
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        if ((x) != hipSuccess) return FHEA_ERR_DEVICE; \
+    } while (0)
+
+int ensure_ws(fhea_ctx *c, u64 words) {
+    if (words <= c->ws_words) return FHEA_OK;
+    if (c->ws) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->ws));
+        c->ws = nullptr;
+        c->ws_words = 0;
+    }
+    if (hipMalloc((void **)&c->ws, words * 8) != hipSuccess) {
+        c->ws = nullptr;
+        return FHEA_ERR_NO_MEM;
+    }
+    c->ws_words = words;
+    return FHEA_OK;
+}
+
+void *stage_args(fhea_ctx *c, const void *host, size_t bytes) {
+    size_t need = (bytes + 255) & ~(size_t)255;
+    if (c->args_used + need > c->args_bytes) {
+        // every earlier copy must have landed before the staging area is reused
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+        c->args_used = 0;
+        if (need > c->args_bytes) return nullptr;
+    }
+    char *h = (char *)c->args_host + c->args_used;
+    char *d = (char *)c->args + c->args_used;
+    memcpy(h, host, bytes);
+    if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+    c->args_used += need;
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Device table blob: all tables in one allocation; ConvDesc pointer fields are
+// stored as blob offsets and relocated once the base address is known.
+// ---------------------------------------------------------------------------
+struct Blob {
+    std::vector<uint8_t> d;
+    std::vector<size_t> fix;
+    size_t add(const void *p, size_t n) {
+        size_t off = (d.size() + 255) & ~(size_t)255;
+        d.resize(off + std::max<size_t>(n, 1));
+        if (n) memcpy(d.data() + off, p, n);
+        return off;
+    }
+    template <class T>
+    size_t addv(const std::vector<T> &v) {
+        return add(v.data(), v.size() * sizeof(T));
+    }
+    // ConvDesc whose pointer fields hold blob offsets
+    size_t add_descs(const std::vector<ConvDesc> &v) {
+        size_t off = addv(v);
+        for (size_t i = 0; i < v.size(); ++i) {
+            size_t b = off + i * sizeof(ConvDesc);
+            fix.push_back(b + offsetof(ConvDesc, in_prime));
+            fix.push_back(b + offsetof(ConvDesc, in_cst));
+            fix.push_back(b + offsetof(ConvDesc, out_prime));
+            fix.push_back(b + offsetof(ConvDesc, out_pos));
+            fix.push_back(b + offsetof(ConvDesc, hat));
+        }
+        return off;
+    }
+};
+
+template <class T>
+static T *off_ptr(size_t off) {
+    return reinterpret_cast<T *>((uintptr_t)off);
+}
+
+static u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
+
+struct LevelOffsets {
+    size_t modup_desc, modup_first, modup_ntt_prime, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
+        rs_last;
+};
+
+static int build_device_tables(fhea_ctx *c) {
+    const HostTables &h = c->h;
+    const u32 N = h.N, L = h.L, K = h.K, T = L + K, A = h.alpha;
+    Blob b;
+    std::vector<ModParams> mp(T);
+    for (u32 i = 0; i < T; ++i) mp[i] = make_mod_params(h.primes[i]);
+    size_t o_mp = b.addv(mp);
+    std::vector<u64> tw((size_t)T * N * 2), itw((size_t)T * N * 2), ninv(2 * T);
+    for (u32 i = 0; i < T; ++i) {
+        u64 q = h.primes[i];
+        for (u32 j = 0; j < N; ++j) {
+            u64 w = h.tw[(size_t)i * N + j], wi = h.itw[(size_t)i * N + j];
+            tw[((size_t)i * N + j) * 2] = w;
+            tw[((size_t)i * N + j) * 2 + 1] = shoup_companion(w, q);
+            itw[((size_t)i * N + j) * 2] = wi;
+            itw[((size_t)i * N + j) * 2 + 1] = shoup_companion(wi, q);
+        }
+        ninv[2 * i] = h.ninv[i];
+        ninv[2 * i + 1] = shoup_companion(h.ninv[i], q);
+    }
+    size_t o_tw = b.addv(tw), o_itw = b.addv(itw), o_ninv = b.addv(ninv);
+    tw.clear();
+    tw.shrink_to_fit();
+    itw.clear();
+    itw.shrink_to_fit();
+    // FP64 transform tables (w, w/q) when every prime is below 2^50
+    bool use_f64 = true;
+    for (u32 i = 0; i < T; ++i) use_f64 &= h.primes[i] < (1ull << 50);
+    size_t o_twf = 0, o_itwf = 0, o_ninvf = 0;
+    if (use_f64) {
+        std::vector<double> twf((size_t)T * N * 2), itwf((size_t)T * N * 2), ninvf(2 * T);
+        for (u32 i = 0; i < T; ++i) {
+            const double qd = (double)h.primes[i];
+            for (u32 j = 0; j < N; ++j) {
+                const double w = (double)h.tw[(size_t)i * N + j], wi = (double)h.itw[(size_t)i * N + j];
+                twf[((size_t)i * N + j) * 2] = w;
+                twf[((size_t)i * N + j) * 2 + 1] = w / qd;
+                itwf[((size_t)i * N + j) * 2] = wi;
+                itwf[((size_t)i * N + j) * 2 + 1] = wi / qd;
+            }
+            ninvf[2 * i] = (double)h.ninv[i];
+            ninvf[2 * i + 1] = (double)h.ninv[i] / qd;
+        }
+        o_twf = b.addv(twf);
+        o_itwf = b.addv(itwf);
+        o_ninvf = b.addv(ninvf);
+    }
+    std::vector<u32> ident(T);
+    for (u32 i = 0; i < T; ++i) ident[i] = i;
+    size_t o_ident = b.addv(ident);
+    u32 zero = 0;
+    size_t o_zero = b.add(&zero, 4);
+
+    std::vector<LevelOffsets> lo(L + 1);
+    c->lv.assign(L + 1, LevelTables());
+    for (u32 l = 1; l <= L; ++l) {
+        LevelTables &lt = c->lv[l];
+        lt.l = l;
+        lt.nparts = (l + A - 1) / A;
+        std::vector<ConvDesc> descs(lt.nparts);
+        std::vector<u32> first(lt.nparts), nttp((size_t)lt.nparts * (l + K), 0xFFFFFFFFu);
+        for (u32 p = 0; p < lt.nparts; ++p) {
+            u32 start = p * A, sz = std::min(A, l - start), n_out = l - sz + K;
+            lt.part_start.push_back(start);
+            lt.part_size.push_back(sz);
+            first[p] = start;
+            std::vector<u32> inp(sz), outp;
+            std::vector<u64> cst(2 * sz), hat((size_t)sz * n_out);
+            for (u32 i = 0; i < l; ++i)
+                if (i < start || i >= start + sz) outp.push_back(i);
+            for (u32 j = 0; j < K; ++j) outp.push_back(L + j);
+            for (u32 i = 0; i < sz; ++i) {
+                u32 qi_idx = start + i;
+                u64 qi = h.primes[qi_idx];
+                inp[i] = qi_idx;
+                u64 prod = 1 % qi;
+                for (u32 k = 0; k < sz; ++k)
+                    if (k != i) prod = mulm(prod, h.primes[start + k] % qi, qi);
+                u64 inv = inv_mod(prod, qi);
+                cst[2 * i] = inv;
+                cst[2 * i + 1] = shoup_companion(inv, qi);
+                for (u32 j = 0; j < n_out; ++j) {
+                    u64 pj = h.primes[outp[j]], pr = 1 % pj;
+                    for (u32 k = 0; k < sz; ++k)
+                        if (k != i) pr = mulm(pr, h.primes[start + k] % pj, pj);
+                    hat[(size_t)i * n_out + j] = pr;
+                }
+            }
+            for (u32 j = 0; j < n_out; ++j) nttp[(size_t)p * (l + K) + j] = outp[j];
+            ConvDesc &D = descs[p];
+            D.n_in = sz;
+            D.n_out = n_out;
+            D.in_prime = off_ptr<const u32>(b.addv(inp));
+            D.in_cst = off_ptr<const u64>(b.addv(cst));
+            D.out_prime = off_ptr<const u32>(b.addv(outp));
+            D.out_pos = off_ptr<const u32>(b.addv(outp));
+            D.hat = off_ptr<const u64>(b.addv(hat));
+        }
+        lo[l].modup_desc = b.add_descs(descs);
+        lo[l].modup_first = b.addv(first);
+        lo[l].modup_ntt_prime = b.addv(nttp);
+        // ModDown: P -> Q_l  (PHatInvModp, PHatModq row-major by P index)
+        {
+            std::vector<u32> inp(K), outp(l);
+            std::vector<u64> cst(2 * K), hat((size_t)K * l), pinv(2 * l);
+            for (u32 j = 0; j < K; ++j) {
+                u64 pj = h.primes[L + j], pr = 1 % pj;
+                inp[j] = L + j;
+                for (u32 k = 0; k < K; ++k)
+                    if (k != j) pr = mulm(pr, h.primes[L + k] % pj, pj);
+                u64 inv = inv_mod(pr, pj);
+                cst[2 * j] = inv;
+                cst[2 * j + 1] = shoup_companion(inv, pj);
+                for (u32 i = 0; i < l; ++i) {
+                    u64 qi = h.primes[i], hh = 1 % qi;
+                    for (u32 k = 0; k < K; ++k)
+                        if (k != j) hh = mulm(hh, h.primes[L + k] % qi, qi);
+                    hat[(size_t)j * l + i] = hh;
+                }
+            }
+            for (u32 i = 0; i < l; ++i) {
+                outp[i] = i;
+                u64 qi = h.primes[i], pr = 1 % qi;
+                for (u32 k = 0; k < K; ++k) pr = mulm(pr, h.primes[L + k] % qi, qi);
+                u64 inv = inv_mod(pr, qi);
+                pinv[2 * i] = inv;
+                pinv[2 * i + 1] = shoup_companion(inv, qi);
+            }
+            std::vector<ConvDesc> md(1);
+            md[0].n_in = K;
+            md[0].n_out = l;
+            md[0].in_prime = off_ptr<const u32>(b.addv(inp));
+            md[0].in_cst = off_ptr<const u64>(b.addv(cst));
+            md[0].out_prime = off_ptr<const u32>(b.addv(outp));
+            md[0].out_pos = off_ptr<const u32>(b.addv(outp));
+            md[0].hat = off_ptr<const u64>(b.addv(hat));
+            lo[l].moddown_desc = b.add_descs(md);
+            lo[l].pinv = b.addv(pinv);
+        }
+        // Rescale l -> l-1 (rescale.cpp:3-72; tables test_vector_generator.cpp:304-325)
+        {
+            u32 n = l >= 2 ? l - 1 : 0;
+            std::vector<u64> qlinv(2 * n + 2), qlql(2 * n + 2), sub(n + 1);
+            u64 ql = h.primes[l - 1];
+            for (u32 i = 0; i < n; ++i) {
+                u64 qi = h.primes[i];
+                u64 inv = inv_mod(ql % qi, qi);
+                u64 t = inv ? qi - inv : 0;
+                qlinv[2 * i] = inv;
+                qlinv[2 * i + 1] = shoup_companion(inv, qi);
+                qlql[2 * i] = t;
+                qlql[2 * i + 1] = shoup_companion(t, qi);
+                sub[i] = mulm(ql % qi, t, qi);
+            }
+            std::vector<u32> last = {l - 1, l - 1};
+            lo[l].rs_qlinv = b.addv(qlinv);
+            lo[l].rs_qlql = b.addv(qlql);
+            lo[l].rs_sub = b.addv(sub);
+            lo[l].rs_last = b.addv(last);
+        }
+    }
+
+    void *base = nullptr;
+    if (hipMalloc(&base, b.d.size()) != hipSuccess) return FHEA_ERR_NO_MEM;
+    for (size_t f : b.fix) *(uintptr_t *)(b.d.data() + f) += (uintptr_t)base;
+    if (hipMemcpy(base, b.d.data(), b.d.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(base);
+        return FHEA_ERR_DEVICE;
+    }
+    c->dblob = base;
+    auto P = [&](size_t off) { return (const void *)((char *)base + off); };
+    c->dt.N = N;
+    c->dt.logN = h.logN;
+    c->dt.L = L;
+    c->dt.K = K;
+    c->dt.mp = (const ModParams *)P(o_mp);
+    c->dt.tw = (const u64 *)P(o_tw);
+    c->dt.itw = (const u64 *)P(o_itw);
+    c->dt.ninv = (const u64 *)P(o_ninv);
+    c->dt.ident = (const u32 *)P(o_ident);
+    c->dt.use_f64 = use_f64 && !c->force_int;
+    if (use_f64) {
+        c->dt.twf = (const double *)P(o_twf);
+        c->dt.itwf = (const double *)P(o_itwf);
+        c->dt.ninvf = (const double *)P(o_ninvf);
+    }
+    c->zero_u32 = (const u32 *)P(o_zero);
+    c->modup_desc_dev.assign(L + 1, nullptr);
+    c->modup_first_dev.assign(L + 1, nullptr);
+    c->moddown_desc_dev.assign(L + 1, nullptr);
+    c->rs_last_map.assign(L + 1, nullptr);
+    c->rs_sub.assign(L + 1, nullptr);
+    for (u32 l = 1; l <= L; ++l) {
+        LevelTables &lt = c->lv[l];
+        c->modup_desc_dev[l] = (const ConvDesc *)P(lo[l].modup_desc);
+        c->modup_first_dev[l] = (const u32 *)P(lo[l].modup_first);
+        lt.modup_ntt_prime = (const u32 *)P(lo[l].modup_ntt_prime);
+        c->moddown_desc_dev[l] = (const ConvDesc *)P(lo[l].moddown_desc);
+        lt.pinv = (const u64 *)P(lo[l].pinv);
+        lt.rs_qlinv = (const u64 *)P(lo[l].rs_qlinv);
+        lt.rs_qlql = (const u64 *)P(lo[l].rs_qlql);
+        c->rs_sub[l] = (const u64 *)P(lo[l].rs_sub);
+        c->rs_last_map[l] = (const u32 *)P(lo[l].rs_last);
+    }
+    return FHEA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pipelines.  Word counts per batch item are computed first so the workspace
+// is sized once per call; regions are then carved from it in order.
+// ---------------------------------------------------------------------------
+static u64 ks_words(const fhea_ctx *c, u32 l) {
+    const u64 N = c->h.N, K = c->h.K, np = c->lv[l].nparts;
+    return N * (l + np * (l + K) + 2 * (l + K) + 2 * K + 2 * l);
+}
+static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2 * (l - 1)); }
+
+// Key switch of nb inputs c_z = c_base + z*c_stride (l limbs, eval domain):
+// out0_z = add0_z + ks0, out1_z = add1_z + ks1 (key_switch.cpp:167-439 followed by
+// the composite's poly_add).  keys[z] = device pointer of item z's key.
+static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
+                   const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
+                   u64 *ws) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const LevelTables &lt = c->lv[l];
+    const u32 np = lt.nparts;
+    hipStream_t s = c->stream;
+    u64 *coef = ws;
+    u64 *compact = coef + nb * l * N;
+    u64 *acc = compact + (u64)nb * np * (l + K) * N;
+    u64 *pb = acc + nb * 2 * (l + K) * N;
+    u64 *tt = pb + nb * 2 * K * N;
+    const u64 cz = (u64)np * (l + K) * N;
+    // ModUp: INTT of every input limb, convert each digit, NTT the converted limbs
+    launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
+    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, l + K, coef, l * N, compact,
+               (l + K) * N, cz, nb, s);
+    launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
+    // inner product with the evaluation key
+    std::vector<KsIpArgs> ip(nb);
+    for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
+    auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
+    if (!ip_d) return FHEA_ERR_DEVICE;
+    launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
+                    c->h.dnum, s);
+    // ModDown of both accumulators
+    launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
+    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    launch_ntt(t, false, tt, tt, l * N, l * N, 2 * nb, l, t.ident, s);
+    auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
+    if (!cb_d) return FHEA_ERR_DEVICE;
+    launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
+                           nb, l, s);
+    return FHEA_OK;
+}
+
+// Rescale of nb items of two polynomials each; poly (z, s) at a_base + z*az + s*as,
+// outputs outs[2z+s] (l-1 limbs).  rescale.cpp:3-72 applied to c0 and c1 (OP 8).
+static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u64 as,
+                        const std::vector<u64 *> &outs, u64 *ws) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N;
+    const LevelTables &lt = c->lv[l];
+    hipStream_t s = c->stream;
+    u64 *v = ws;
+    u64 *tmp = v + nb * 2 * N;
+    launch_ntt(t, true, a_base + (l - 1) * N, v, az, 2 * N, nb, 2, c->rs_last_map[l], s, as, N);
+    launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], 2 * nb, l, s);
+    launch_ntt(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, l - 1, t.ident, s);
+    std::vector<RescaleOut> ro(2 * nb);
+    for (u32 z = 0; z < nb; ++z)
+        for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {a_base + z * az + k * as, outs[2 * z + k]};
+    auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
+    if (!ro_d) return FHEA_ERR_DEVICE;
+    launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, 2 * nb, l, s);
+    return FHEA_OK;
+}
+
+struct MultItem {
+    const u64 *a0, *a1, *b0, *b1;
+    u64 *d0, *d1;
+};
+
+// eval_mult_relin_rescale (fhe_host_driver.cpp:272-306) for a batch sharing one
+// relinearisation key: tensor -> KS(d2) (+d0, +d1 fused) -> rescale.
+static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
+                              const std::vector<MultItem> &items) {
+    const u64 N = c->h.N;
+    for (size_t off = 0; off < items.size(); off += c->max_chunk) {
+        u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        u64 dwords = (u64)nb * 3 * l * N;
+        u64 need = dwords + std::max<u64>(nb * ks_words(c, l), nb * rescale_words(c, l));
+        int st = ensure_ws(c, need);
+        if (st) return st;
+        u64 *D = c->ws, *rest = D + dwords;
+        std::vector<TensorArgs> ta(nb);
+        std::vector<CombineArgs> cb(nb);
+        std::vector<const u64 *> keys(nb, key);
+        std::vector<u64 *> outs(2 * nb);
+        for (u32 z = 0; z < nb; ++z) {
+            const MultItem &it = items[off + z];
+            u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
+            ta[z] = {it.a0, it.a1, it.b0, it.b1, d0, d1, d2};
+            cb[z] = {d0, d1, d0, d1};
+            outs[2 * z] = it.d0;
+            outs[2 * z + 1] = it.d1;
+        }
+        auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
+        if (!ta_d) return FHEA_ERR_DEVICE;
+        launch_tensor(c->dt, ta_d, nb, l, c->stream);
+        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest);
+        if (st) return st;
+        st = rescale_core(c, l, nb, D, 3 * l * N, l * N, outs, rest);
+        if (st) return st;
+    }
+    return check_hip(hipGetLastError());
+}
+
+struct AutoItem {
+    const u64 *in0, *in1;
+    u64 *d0, *d1;
+    u64 galois;
+    const u64 *key;
+};
+
+// eval_rotate_single_step (fhe_host_driver.cpp:331-359) with explicit Galois
+// elements: sigma(c0), sigma(c1) -> KS(sigma(c1)) -> (sigma(c0) + ks0, ks1).
+static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<AutoItem> &items) {
+    const u64 N = c->h.N;
+    for (size_t off = 0; off < items.size(); off += c->max_chunk) {
+        u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        u64 swords = (u64)nb * 2 * l * N;
+        int st = ensure_ws(c, swords + nb * ks_words(c, l));
+        if (st) return st;
+        u64 *S = c->ws, *rest = S + swords;
+        std::vector<AutoArgs> aa(nb);
+        std::vector<CombineArgs> cb(nb);
+        std::vector<const u64 *> keys(nb);
+        for (u32 z = 0; z < nb; ++z) {
+            const AutoItem &it = items[off + z];
+            u64 *s0 = S + (u64)z * 2 * l * N, *s1 = s0 + l * N;
+            aa[z] = {it.in0, it.in1, s0, s1, it.galois};
+            cb[z] = {it.d0, it.d1, s0, nullptr};
+            keys[z] = it.key;
+        }
+        auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
+        if (!aa_d) return FHEA_ERR_DEVICE;
+        launch_automorph(c->dt, aa_d, nb, l, c->stream);
+        st = ks_core(c, l, key_limbs, nb, S + l * N, 2 * l * N, keys, cb, rest);
+        if (st) return st;
+    }
+    return check_hip(hipGetLastError());
+}
+
+}  // namespace fhea
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+static bool range_ok(u64 off, u64 words, u64 total) { return off <= total && words <= total - off; }
+
+#define CTX_CHECK(c)                               \
+    do {                                           \
+        if (!(c) || !(c)->poly) return FHEA_ERR_STATE; \
+    } while (0)
+
+#define POLY_CHECK(c, off, words)                                   \
+    do {                                                            \
+        if ((off) & 7) return FHEA_ERR_ALIGN;                       \
+        if (!range_ok((off), (words), (c)->poly_words)) return FHEA_ERR_RANGE; \
+    } while (0)
+
+static int key_check(const fhea_ctx *c, u64 evk, u32 key_limbs) {
+    if (evk & 7) return FHEA_ERR_ALIGN;
+    u64 words = 2ull * c->h.dnum * (key_limbs + c->h.K) * c->h.N;
+    return range_ok(evk, words, c->key_words) ? FHEA_OK : FHEA_ERR_RANGE;
+}
+
+extern "C" {
+
+const char *fhea_status_string(int st) {
+    switch (st) {
+    case FHEA_OK: return "ok";
+    case FHEA_ERR_BAD_OPCODE: return "unknown op code";
+    case FHEA_ERR_RANGE: return "offset or size outside the arena";
+    case FHEA_ERR_ALIGN: return "offset not a multiple of 8 words";
+    case FHEA_ERR_ARG: return "invalid argument";
+    case FHEA_ERR_NO_KEY: return "missing evaluation key";
+    case FHEA_ERR_NO_MEM: return "device allocation failed";
+    case FHEA_ERR_DEVICE: return "HIP runtime error";
+    case FHEA_ERR_STATE: return "invalid context";
+    }
+    return "unknown status";
+}
+
+int fhea_ctx_create(const fhea_params *p, fhea_ctx **out) {
+    if (!p || !out) return FHEA_ERR_ARG;
+    *out = nullptr;
+    const u32 N = p->N;
+    if (N < 1024 || N > (1u << 17) || (N & (N - 1))) return FHEA_ERR_ARG;
+    if (p->L < 1 || p->K < 1 || p->dnum < 1 || p->dnum > p->L || p->L + p->K > 255) return FHEA_ERR_ARG;
+    const u32 alpha = (p->L + p->dnum - 1) / p->dnum;
+    if (alpha > 32 || p->K > 32) return FHEA_ERR_ARG;
+    u32 bits = p->q_bits ? p->q_bits : 50;
+    if (bits < 20 || bits > 61) return FHEA_ERR_ARG;
+    fhea_ctx *c = new fhea_ctx();
+    HostTables &h = c->h;
+    h.N = N;
+    h.logN = 0;
+    while ((1u << h.logN) < N) ++h.logN;
+    h.L = p->L;
+    h.K = p->K;
+    h.dnum = p->dnum;
+    h.alpha = alpha;
+    std::vector<u64> gen;
+    if (!p->q_primes || !p->p_primes) {
+        if (gen_primes(2 * N, bits, p->L + p->K, {}, gen) != p->L + p->K) {
+            delete c;
+            return FHEA_ERR_ARG;
+        }
+    }
+    for (u32 i = 0; i < p->L; ++i) h.primes.push_back(p->q_primes ? p->q_primes[i] : gen[i]);
+    for (u32 j = 0; j < p->K; ++j) h.primes.push_back(p->p_primes ? p->p_primes[j] : gen[p->L + j]);
+    for (u64 q : h.primes) {
+        if (q < 3 || q >= (1ull << 61) || (q - 1) % (2ull * N) != 0 || !is_prime_u64(q)) {
+            delete c;
+            return FHEA_ERR_ARG;
+        }
+    }
+    for (size_t i = 0; i < h.primes.size(); ++i)
+        for (size_t j = i + 1; j < h.primes.size(); ++j)
+            if (h.primes[i] == h.primes[j]) {
+                delete c;
+                return FHEA_ERR_ARG;
+            }
+    build_host_tables(h);
+    {
+        const char *fi = getenv("FHEA_FORCE_INT");
+        c->force_int = fi && atoi(fi) != 0;
+    }
+    c->device = (int)p->device;
+    int st = FHEA_OK;
+    if (hipSetDevice(c->device) != hipSuccess) st = FHEA_ERR_DEVICE;
+    if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = FHEA_ERR_DEVICE;
+    c->own_stream = true;
+    if (!st && (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess))
+        st = FHEA_ERR_DEVICE;
+    c->poly_words = p->poly_words ? (p->poly_words + 7) & ~7ull : 8ull * 1024 * 1024;
+    c->key_words = p->key_words ? (p->key_words + 7) & ~7ull : 8ull * 1024 * 1024;
+    if (!st && hipMalloc((void **)&c->poly, c->poly_words * 8) != hipSuccess) st = FHEA_ERR_NO_MEM;
+    if (!st && hipMalloc((void **)&c->key, c->key_words * 8) != hipSuccess) st = FHEA_ERR_NO_MEM;
+    if (!st && (hipMemset(c->poly, 0, c->poly_words * 8) != hipSuccess ||
+                hipMemset(c->key, 0, c->key_words * 8) != hipSuccess))
+        st = FHEA_ERR_DEVICE;
+    c->args_bytes = 4u << 20;
+    if (!st && hipMalloc(&c->args, c->args_bytes) != hipSuccess) st = FHEA_ERR_NO_MEM;
+    if (!st && hipHostMalloc(&c->args_host, c->args_bytes, hipHostMallocDefault) != hipSuccess)
+        st = FHEA_ERR_NO_MEM;
+    if (!st) st = build_device_tables(c);
+    if (st) {
+        fhea_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return FHEA_OK;
+}
+
+int fhea_ctx_destroy(fhea_ctx *c) {
+    if (!c) return FHEA_ERR_STATE;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->poly) hipFree(c->poly);
+    if (c->key) hipFree(c->key);
+    if (c->ws) hipFree(c->ws);
+    if (c->dblob) hipFree(c->dblob);
+    if (c->args) hipFree(c->args);
+    if (c->args_host) hipHostFree(c->args_host);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return FHEA_OK;
+}
+
+int fhea_ctx_info(const fhea_ctx *c, uint32_t *info) {
+    if (!c || !info) return FHEA_ERR_STATE;
+    info[0] = c->h.N;
+    info[1] = c->h.L;
+    info[2] = c->h.K;
+    info[3] = c->h.dnum;
+    info[4] = c->h.alpha;
+    return FHEA_OK;
+}
+
+int fhea_ctx_primes(const fhea_ctx *c, uint64_t *out) {
+    if (!c || !out) return FHEA_ERR_STATE;
+    std::copy(c->h.primes.begin(), c->h.primes.end(), out);
+    return FHEA_OK;
+}
+
+int fhea_ctx_table(const fhea_ctx *c, int which, uint64_t *out, uint64_t count) {
+    if (!c || !out) return FHEA_ERR_STATE;
+    const std::vector<u64> *v = which == 0 ? &c->h.psi : which == 1 ? &c->h.ninv
+                              : which == 2 ? &c->h.tw : which == 3 ? &c->h.itw : nullptr;
+    if (!v) return FHEA_ERR_ARG;
+    if (count != v->size()) return FHEA_ERR_RANGE;
+    std::copy(v->begin(), v->end(), out);
+    return FHEA_OK;
+}
+
+int fhea_ctx_set_stream(fhea_ctx *c, void *stream) {
+    CTX_CHECK(c);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    if (stream) {
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else if (!c->own_stream) {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FHEA_ERR_DEVICE;
+        c->own_stream = true;
+    }
+    return FHEA_OK;
+}
+
+void *fhea_ctx_stream(const fhea_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+uint64_t *fhea_poly_arena(fhea_ctx *c) { return c ? c->poly : nullptr; }
+uint64_t *fhea_key_arena(fhea_ctx *c) { return c ? c->key : nullptr; }
+uint64_t fhea_poly_words(const fhea_ctx *c) { return c ? c->poly_words : 0; }
+uint64_t fhea_key_words(const fhea_ctx *c) { return c ? c->key_words : 0; }
+
+static int arena_copy(fhea_ctx *c, u64 *base, u64 total, u64 off, void *host, u64 count, bool write) {
+    CTX_CHECK(c);
+    if (!host && count) return FHEA_ERR_ARG;
+    if (!range_ok(off, count, total)) return FHEA_ERR_RANGE;
+    if (!count) return FHEA_OK;
+    hipError_t e;
+    if (write)
+        e = hipMemcpyAsync(base + off, host, count * 8, hipMemcpyHostToDevice, c->stream);
+    else
+        e = hipMemcpyAsync(host, base + off, count * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return FHEA_ERR_DEVICE;
+    return check_hip(hipStreamSynchronize(c->stream));
+}
+
+int fhea_write_poly(fhea_ctx *c, uint64_t off, const uint64_t *h, uint64_t n) {
+    return c ? arena_copy(c, c->poly, c->poly_words, off, (void *)h, n, true) : FHEA_ERR_STATE;
+}
+int fhea_read_poly(fhea_ctx *c, uint64_t off, uint64_t *h, uint64_t n) {
+    return c ? arena_copy(c, c->poly, c->poly_words, off, h, n, false) : FHEA_ERR_STATE;
+}
+int fhea_write_key(fhea_ctx *c, uint64_t off, const uint64_t *h, uint64_t n) {
+    return c ? arena_copy(c, c->key, c->key_words, off, (void *)h, n, true) : FHEA_ERR_STATE;
+}
+int fhea_read_key(fhea_ctx *c, uint64_t off, uint64_t *h, uint64_t n) {
+    return c ? arena_copy(c, c->key, c->key_words, off, h, n, false) : FHEA_ERR_STATE;
+}
+
+int fhea_copy_poly(fhea_ctx *c, uint64_t dst, uint64_t src, uint64_t n) {
+    CTX_CHECK(c);
+    if (!range_ok(dst, n, c->poly_words) || !range_ok(src, n, c->poly_words)) return FHEA_ERR_RANGE;
+    if (!n || dst == src) return FHEA_OK;
+    return check_hip(hipMemcpyAsync(c->poly + dst, c->poly + src, n * 8, hipMemcpyDeviceToDevice, c->stream));
+}
+
+int fhea_op(fhea_ctx *c, uint32_t op, uint64_t src_a, uint64_t src_b, uint64_t dst, uint64_t evk,
+            uint32_t limbs, uint32_t key_limbs) {
+    CTX_CHECK(c);
+    if (op == FHEA_OP_LOAD_CONTEXT) return FHEA_OK;
+    if (op < FHEA_OP_NTT || op > FHEA_OP_RESCALE) return FHEA_ERR_BAD_OPCODE;
+    if ((src_a | src_b | dst | evk) & 7) return FHEA_ERR_ALIGN;
+    if (limbs == 0 || limbs > c->h.L) return FHEA_ERR_ARG;
+    const u64 N = c->h.N, W = (u64)limbs * N;
+    const DevTables &t = c->dt;
+    POLY_CHECK(c, src_a, W);
+    u64 *P = c->poly;
+    int st = FHEA_OK;
+    switch (op) {
+    case FHEA_OP_NTT:
+    case FHEA_OP_INTT:
+        POLY_CHECK(c, dst, W);
+        launch_ntt(t, op == FHEA_OP_INTT, P + src_a, P + dst, W, W, 1, limbs, t.ident, c->stream);
+        break;
+    case FHEA_OP_POLY_ADD:
+    case FHEA_OP_POLY_SUB:
+    case FHEA_OP_POLY_MUL_NTT:
+        POLY_CHECK(c, src_b, W);
+        POLY_CHECK(c, dst, W);
+        launch_binop(t, (int)op - FHEA_OP_POLY_ADD, P + src_a, P + src_b, P + dst, limbs, c->stream);
+        break;
+    case FHEA_OP_AUTOMORPHISM: {
+        POLY_CHECK(c, src_b, N / 2);
+        POLY_CHECK(c, dst, W);
+        // the gather needs distinct input and output: stage through the workspace
+        if ((st = ensure_ws(c, W))) return st;
+        launch_automorph_map(t, P + src_a, c->ws, P + src_b, limbs, c->stream);
+        if (hipMemcpyAsync(P + dst, c->ws, W * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        break;
+    }
+    case FHEA_OP_KEY_SWITCH: {
+        u32 kl = key_limbs ? key_limbs : limbs;
+        if (kl < limbs || kl > c->h.L) return FHEA_ERR_ARG;
+        POLY_CHECK(c, dst, 2 * W);
+        if ((st = key_check(c, evk, kl))) return st;
+        if ((st = ensure_ws(c, ks_words(c, limbs)))) return st;
+        std::vector<const u64 *> keys = {c->key + evk};
+        std::vector<CombineArgs> cb = {{P + dst, P + dst + W, nullptr, nullptr}};
+        st = ks_core(c, limbs, kl, 1, P + src_a, 0, keys, cb, c->ws);
+        break;
+    }
+    case FHEA_OP_RESCALE: {
+        if (limbs < 2) return FHEA_ERR_ARG;
+        const u64 W1 = (u64)(limbs - 1) * N;
+        POLY_CHECK(c, src_b, W);
+        POLY_CHECK(c, dst, 2 * W1);
+        // inputs are staged so outputs may overlap them (fhe_accel_top.cpp:282-296
+        // loads both operands into on-chip banks first)
+        if ((st = ensure_ws(c, 2 * W + rescale_words(c, limbs)))) return st;
+        u64 *A = c->ws;
+        if (hipMemcpyAsync(A, P + src_a, W * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(A + W, P + src_b, W * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        std::vector<u64 *> outs = {P + dst, P + dst + W1};
+        st = rescale_core(c, limbs, 1, A, 2 * W, W, outs, A + 2 * W);
+        break;
+    }
+    }
+    if (st) return st;
+    return check_hip(hipGetLastError());
+}
+
+int fhea_eval_add(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1, uint64_t d0,
+                  uint64_t d1, uint32_t limbs) {
+    int st = fhea_op(c, FHEA_OP_POLY_ADD, a0, b0, d0, 0, limbs, 0);
+    if (st) return st;
+    return fhea_op(c, FHEA_OP_POLY_ADD, a1, b1, d1, 0, limbs, 0);
+}
+
+int fhea_eval_sub_plain(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t pt, uint64_t d0, uint64_t d1,
+                        uint32_t limbs) {
+    CTX_CHECK(c);
+    if (limbs == 0 || limbs > c->h.L) return FHEA_ERR_ARG;
+    const u64 W = (u64)limbs * c->h.N;
+    POLY_CHECK(c, a1, W);
+    POLY_CHECK(c, d1, W);
+    int st = fhea_op(c, FHEA_OP_POLY_SUB, a0, pt, d0, 0, limbs, 0);
+    if (st) return st;
+    return fhea_copy_poly(c, d1, a1, W);  // fhe_host_driver.cpp:264-269
+}
+
+int fhea_set_relin_key(fhea_ctx *c, uint64_t evk, uint32_t key_limbs) {
+    CTX_CHECK(c);
+    u32 kl = key_limbs ? key_limbs : c->h.L;
+    if (kl > c->h.L) return FHEA_ERR_ARG;
+    int st = key_check(c, evk, kl);
+    if (st) return st;
+    c->has_relin = true;
+    c->relin_off = evk;
+    c->relin_limbs = kl;
+    return FHEA_OK;
+}
+
+static int check_mult_args(fhea_ctx *c, u64 a0, u64 a1, u64 b0, u64 b1, u64 d0, u64 d1, u32 l) {
+    if (l < 2 || l > c->h.L) return FHEA_ERR_ARG;
+    const u64 W = (u64)l * c->h.N, W1 = (u64)(l - 1) * c->h.N;
+    POLY_CHECK(c, a0, W);
+    POLY_CHECK(c, a1, W);
+    POLY_CHECK(c, b0, W);
+    POLY_CHECK(c, b1, W);
+    POLY_CHECK(c, d0, W1);
+    POLY_CHECK(c, d1, W1);
+    return FHEA_OK;
+}
+
+int fhea_eval_mult_relin_rescale(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                                 uint64_t d0, uint64_t d1, uint32_t l) {
+    CTX_CHECK(c);
+    int st = check_mult_args(c, a0, a1, b0, b1, d0, d1, l);
+    if (st) return st;
+    if (!c->has_relin) return FHEA_ERR_NO_KEY;
+    u32 kl = c->relin_limbs ? c->relin_limbs : l;
+    if (kl < l) return FHEA_ERR_ARG;
+    u64 *P = c->poly;
+    std::vector<MultItem> it = {{P + a0, P + a1, P + b0, P + b1, P + d0, P + d1}};
+    return mult_relin_rescale(c, l, c->key + c->relin_off, kl, it);
+}
+
+int fhea_eval_mult_plain_rescale(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t pt, uint64_t d0,
+                                 uint64_t d1, uint32_t l) {
+    CTX_CHECK(c);
+    int st = check_mult_args(c, a0, a1, pt, pt, d0, d1, l);
+    if (st) return st;
+    const u64 W = (u64)l * c->h.N;
+    if ((st = ensure_ws(c, 2 * W + rescale_words(c, l)))) return st;
+    u64 *A = c->ws, *P = c->poly;
+    launch_binop(c->dt, 2, P + a0, P + pt, A, l, c->stream);
+    launch_binop(c->dt, 2, P + a1, P + pt, A + W, l, c->stream);
+    std::vector<u64 *> outs = {P + d0, P + d1};
+    st = rescale_core(c, l, 1, A, 2 * W, W, outs, A + 2 * W);
+    if (st) return st;
+    return check_hip(hipGetLastError());
+}
+
+int fhea_add_rotation_key(fhea_ctx *c, int32_t step, uint64_t galois, uint64_t evk, uint32_t key_limbs) {
+    CTX_CHECK(c);
+    u32 kl = key_limbs ? key_limbs : c->h.L;
+    if (kl > c->h.L || (galois & 1) == 0 || galois >= 2ull * c->h.N) return FHEA_ERR_ARG;
+    int st = key_check(c, evk, kl);
+    if (st) return st;
+    c->rot[step] = RotKey{step, galois, evk, kl};
+    return FHEA_OK;
+}
+
+int fhea_eval_automorphism_ks(fhea_ctx *c, uint64_t galois, uint64_t evk, uint32_t key_limbs,
+                              uint64_t in0, uint64_t in1, uint64_t d0, uint64_t d1, uint32_t l) {
+    CTX_CHECK(c);
+    if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+    if ((galois & 1) == 0 || galois >= 2ull * c->h.N) return FHEA_ERR_ARG;
+    u32 kl = key_limbs ? key_limbs : l;
+    if (kl < l || kl > c->h.L) return FHEA_ERR_ARG;
+    const u64 W = (u64)l * c->h.N;
+    POLY_CHECK(c, in0, W);
+    POLY_CHECK(c, in1, W);
+    POLY_CHECK(c, d0, W);
+    POLY_CHECK(c, d1, W);
+    int st = key_check(c, evk, kl);
+    if (st) return st;
+    u64 *P = c->poly;
+    std::vector<AutoItem> it = {{P + in0, P + in1, P + d0, P + d1, galois, c->key + evk}};
+    return automorphism_ks(c, l, kl, it);
+}
+
+int fhea_eval_rotate(fhea_ctx *c, uint64_t in0, uint64_t in1, uint64_t d0, uint64_t d1,
+                     int32_t total_rot, uint32_t l) {
+    CTX_CHECK(c);
+    if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+    const u64 W = (u64)l * c->h.N;
+    POLY_CHECK(c, in0, W);
+    POLY_CHECK(c, in1, W);
+    POLY_CHECK(c, d0, W);
+    POLY_CHECK(c, d1, W);
+    // decomposition and key lookup first, so a missing key leaves dst untouched
+    std::vector<fhea::RotKey> steps;
+    int64_t abs_rot = total_rot < 0 ? -(int64_t)total_rot : total_rot;
+    int sign = total_rot > 0 ? 1 : -1;
+    while (abs_rot > 0) {
+        int64_t step_size = 1;
+        while ((step_size << 1) <= abs_rot) step_size <<= 1;
+        auto it = c->rot.find((int32_t)(sign * step_size));
+        if (it == c->rot.end()) return FHEA_ERR_NO_KEY;
+        if (it->second.key_limbs < l) return FHEA_ERR_ARG;
+        steps.push_back(it->second);
+        abs_rot -= step_size;
+    }
+    u64 *P = c->poly;
+    if (steps.empty()) {
+        int st = fhea_copy_poly(c, d0, in0, W);
+        return st ? st : fhea_copy_poly(c, d1, in1, W);
+    }
+    // ping-pong ciphertext buffers live in a separate allocation from the
+    // pipeline workspace so that a workspace resize cannot invalidate them
+    u64 *pp = nullptr;
+    if (steps.size() > 1 && hipMallocAsync((void **)&pp, 4 * W * 8, c->stream) != hipSuccess)
+        return FHEA_ERR_NO_MEM;
+    const u64 *cur0 = P + in0, *cur1 = P + in1;
+    int st = FHEA_OK;
+    for (size_t k = 0; k < steps.size() && !st; ++k) {
+        bool last = k + 1 == steps.size();
+        u64 *n0 = last ? P + d0 : pp + (k & 1) * 2 * W;
+        u64 *n1 = last ? P + d1 : n0 + W;
+        std::vector<AutoItem> it = {{cur0, cur1, n0, n1, steps[k].galois, c->key + steps[k].evk_off}};
+        st = automorphism_ks(c, l, steps[k].key_limbs, it);
+        cur0 = n0;
+        cur1 = n1;
+    }
+    if (pp) hipFreeAsync(pp, c->stream);
+    return st;
+}
+
+int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
+    CTX_CHECK(c);
+    if (n && !items) return FHEA_ERR_ARG;
+    u64 *P = c->poly;
+    // group consecutive items of the same kind / level / key level
+    uint32_t i = 0;
+    while (i < n) {
+        const fhea_work &w0 = items[i];
+        uint32_t j = i;
+        while (j < n && items[j].kind == w0.kind && items[j].limbs == w0.limbs &&
+               items[j].key_limbs == w0.key_limbs)
+            ++j;
+        const u32 l = w0.limbs;
+        int st = FHEA_OK;
+        if (w0.kind == FHEA_WORK_MULT_RELIN_RESCALE) {
+            if (!c->has_relin) return FHEA_ERR_NO_KEY;
+            u32 kl = c->relin_limbs ? c->relin_limbs : l;
+            if (kl < l) return FHEA_ERR_ARG;
+            std::vector<MultItem> v;
+            for (uint32_t k = i; k < j; ++k) {
+                const fhea_work &w = items[k];
+                if ((st = check_mult_args(c, w.a0, w.a1, w.b0, w.b1, w.d0, w.d1, l))) return st;
+                v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+            }
+            st = mult_relin_rescale(c, l, c->key + c->relin_off, kl, v);
+        } else if (w0.kind == FHEA_WORK_AUTOMORPHISM_KS) {
+            if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+            u32 kl = w0.key_limbs ? w0.key_limbs : l;
+            if (kl < l || kl > c->h.L) return FHEA_ERR_ARG;
+            const u64 W = (u64)l * c->h.N;
+            std::vector<AutoItem> v;
+            for (uint32_t k = i; k < j; ++k) {
+                const fhea_work &w = items[k];
+                POLY_CHECK(c, w.a0, W);
+                POLY_CHECK(c, w.a1, W);
+                POLY_CHECK(c, w.d0, W);
+                POLY_CHECK(c, w.d1, W);
+                if ((w.galois & 1) == 0 || w.galois >= 2ull * c->h.N) return FHEA_ERR_ARG;
+                if ((st = key_check(c, w.evk, kl))) return st;
+                v.push_back({P + w.a0, P + w.a1, P + w.d0, P + w.d1, w.galois, c->key + w.evk});
+            }
+            st = automorphism_ks(c, l, kl, v);
+        } else if (w0.kind == FHEA_WORK_NTT || w0.kind == FHEA_WORK_INTT) {
+            for (uint32_t k = i; k < j && !st; ++k)
+                st = fhea_op(c, w0.kind == FHEA_WORK_NTT ? FHEA_OP_NTT : FHEA_OP_INTT, items[k].a0, 0,
+                             items[k].d0, 0, l, 0);
+        } else {
+            return FHEA_ERR_ARG;
+        }
+        if (st) return st;
+        i = j;
+    }
+    return FHEA_OK;
+}
+
+int fhea_ntt_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t npolys, uint32_t limbs,
+                   int inverse) {
+    CTX_CHECK(c);
+    if (limbs == 0 || limbs > c->h.L || npolys == 0) return FHEA_ERR_ARG;
+    const u64 W = (u64)limbs * c->h.N;
+    POLY_CHECK(c, src, W * npolys);
+    POLY_CHECK(c, dst, W * npolys);
+    launch_ntt(c->dt, inverse != 0, c->poly + src, c->poly + dst, W, W, npolys, limbs, c->dt.ident,
+               c->stream);
+    return check_hip(hipGetLastError());
+}
+
+int fhea_sync(fhea_ctx *c) {
+    CTX_CHECK(c);
+    int st = check_hip(hipStreamSynchronize(c->stream));
+    if (!st) c->args_used = 0;
+    return st;
+}
+
+int fhea_timer_start(fhea_ctx *c) {
+    CTX_CHECK(c);
+    return check_hip(hipEventRecord(c->ev0, c->stream));
+}
+
+int fhea_timer_stop(fhea_ctx *c, float *ms) {
+    CTX_CHECK(c);
+    if (hipEventRecord(c->ev1, c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    if (hipEventSynchronize(c->ev1) != hipSuccess) return FHEA_ERR_DEVICE;
+    float v = 0;
+    if (hipEventElapsedTime(&v, c->ev0, c->ev1) != hipSuccess) return FHEA_ERR_DEVICE;
+    if (ms) *ms = v;
+    return FHEA_OK;
+}
+
+}  // extern "C"

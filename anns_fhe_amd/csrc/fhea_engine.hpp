@@ -1,0 +1,56 @@
+// fhea_engine.hpp -- the context object behind the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "../../include/fhea.h"
+#include "fhea_internal.hpp"
+#include "fhea_kernels.hpp"
+
+struct fhea_ctx {
+    fhea::HostTables h;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    u64 *poly = nullptr, *key = nullptr;
+    u64 poly_words = 0, key_words = 0;
+
+    void *dblob = nullptr;  // device tables
+    fhea::DevTables dt;
+    std::vector<fhea::LevelTables> lv;       // index l = 1..L
+    std::vector<const fhea::ConvDesc *> modup_desc_dev;  // per level
+    std::vector<const u32 *> modup_first_dev;            // per level
+    std::vector<const fhea::ConvDesc *> moddown_desc_dev;
+    std::vector<const u32 *> rs_last_map;                // per level: {l-1, l-1}
+    std::vector<const u64 *> rs_sub;                     // per level: q_l * T_i mod q_i
+    const u32 *zero_u32 = nullptr;                       // {0}
+
+    // workspace
+    u64 *ws = nullptr;
+    u64 ws_words = 0;
+    // per-item pointer lists: bump-allocated in a pinned host buffer and copied
+    // to the same offset of a device buffer; reset only after a stream sync
+    void *args = nullptr;
+    void *args_host = nullptr;
+    size_t args_bytes = 0;
+    size_t args_used = 0;
+
+    bool has_relin = false;
+    u64 relin_off = 0;
+    u32 relin_limbs = 0;
+    std::map<int32_t, fhea::RotKey> rot;
+
+    u32 max_chunk = 16;
+    bool force_int = false;  // FHEA_FORCE_INT=1: integer transforms even for < 2^50 primes
+};
+
+namespace fhea {
+int check_hip(hipError_t e);
+int ensure_ws(fhea_ctx *c, u64 words);
+// stage a host array into the device args buffer, returns device pointer
+void *stage_args(fhea_ctx *c, const void *host, size_t bytes);
+}  // namespace fhea

@@ -1,0 +1,43 @@
+// fhea_f64.hpp -- exact modular arithmetic on FP64 FMA for primes q < 2^50.
+//
+// gfx950 issues FP64 FMA at the full VALU rate (64 lanes/CU/clock) but 32-bit
+// integer multiplies at a quarter of it, and a 64x64-bit modular product needs
+// about ten of those.  For q < 2^50 every residue is an exact double, and the
+// product v*w splits exactly into hi = fl(v*w) and lo = fma(v, w, -hi)
+// (error-free transformation: |v*w| < 2^106, both integers).  With a quotient
+// estimate k = rint(v * (w/q)), the value v*w - k*q is an integer of magnitude
+// below 2^51 and is computed exactly as fma(-k, q, hi) + lo.  Residues are kept
+// signed ("centred"), |x| < 8q < 2^53, and brought back to [0, q) only when
+// they leave a transform, so the result words equal the integer path's.
+//
+// Bounds used by the callers (q < 2^50):
+//   mulmod_f64(v, w, wq): |v| <= 8q, 0 <= w < q  ->  |result| <= 1.5q (< 2^51),
+//                         result == v*w (mod q) exactly.
+//   reduce_f64(x):        |x| <= 8q  ->  |result| <= q/2 + 1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double mulmod_f64(double v, double w, double wq, double q) {
+    const double k = __builtin_rint(v * wq);
+    const double hi = v * w;
+    const double lo = __builtin_fma(v, w, -hi);
+    return __builtin_fma(-k, q, hi) + lo;
+}
+
+__device__ __forceinline__ double reduce_f64(double x, double q, double qinv) {
+    return __builtin_fma(-__builtin_rint(x * qinv), q, x);
+}
+
+// x in [0, 2^52) integer <-> double, exact (exponent-bias trick).
+__device__ __forceinline__ double u64_to_f64(uint64_t x) {
+    return __longlong_as_double((long long)(x | 0x4330000000000000ull)) - 4503599627370496.0;
+}
+
+// |x| <= q/2 + 1 (after reduce_f64)  ->  canonical residue in [0, q).
+__device__ __forceinline__ uint64_t f64_to_canon(double x, double q) {
+    double y = x < 0.0 ? x + q : x;
+    return (uint64_t)__double_as_longlong(y + 4503599627370496.0) & 0x000FFFFFFFFFFFFFull;
+}

@@ -1,0 +1,71 @@
+// fhea_internal.hpp -- engine-internal structures shared by the host code and
+// the kernel launchers.  Not part of the C-ABI (include/fhea.h is).
+#pragma once
+#include <stdint.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fhea_arith.hpp"
+
+namespace fhea {
+
+// Per-prime device tables.  Index 0..L-1 = Q primes, L..L+K-1 = P primes.
+struct PrimeTables {
+    const ModParams *mp = nullptr;  // [T]
+    const u64 *tw = nullptr;        // [T*N] psi^br(j)                (Shoup pairs interleaved:
+    const u64 *itw = nullptr;       // [T*N] psi^-br(j)                 see NttTw below)
+    const u64 *ninv = nullptr;      // [T*2] (N^-1, shoup)
+};
+
+// Twiddle layout on the device: for prime i, 2*N words: tw[2j] = w_j,
+// tw[2j+1] = floor(w_j 2^64 / q) so one 16-byte load fetches a Shoup pair.
+
+// Basis conversion descriptor (fast_bases_conv, key_switch.cpp:7-60).
+struct ConvDesc {
+    u32 n_in = 0, n_out = 0;
+    const u32 *in_prime = nullptr;   // [n_in]  prime index of each input limb
+    const u64 *in_cst = nullptr;     // [2*n_in] (qhat_i^-1 mod q_i, shoup)
+    const u32 *out_prime = nullptr;  // [n_out] prime index of each output limb
+    const u32 *out_pos = nullptr;    // [n_out] limb slot in the destination
+    const u64 *hat = nullptr;        // [n_in*n_out] qhat_i mod p_j, row-major by input
+};
+
+// Per-level tables (sizeQl = l).
+struct LevelTables {
+    u32 l = 0, nparts = 0;
+    std::vector<ConvDesc> modup;           // [nparts]
+    std::vector<u32> part_start, part_size;
+    const u32 *qlp_prime = nullptr;        // [l+K] prime index of each QlP slot
+    const u32 *q_prime = nullptr;          // [l] identity map 0..l-1
+    const u32 *p_prime = nullptr;          // [K] L..L+K-1
+    const u32 *modup_ntt_prime = nullptr;  // [nparts*(l+K)] prime index per digit slot
+    ConvDesc moddown;                      // P -> Q_l
+    const u64 *pinv = nullptr;             // [2*l] (P^-1 mod q_i, shoup)
+    const u64 *rs_qlinv = nullptr;         // [2*(l-1)] (q_l^-1 mod q_i, shoup)
+    const u64 *rs_qlql = nullptr;          // [2*(l-1)] (-q_l^-1 mod q_i, shoup)
+};
+
+struct HostTables {
+    u32 N = 0, logN = 0, L = 0, K = 0, dnum = 0, alpha = 0;
+    std::vector<u64> primes, psi, ninv;
+    std::vector<u64> tw, itw;  // [T*N] plain (for export / CPU client)
+};
+
+struct RotKey {
+    int32_t step;
+    u64 galois;
+    u64 evk_off;
+    u32 key_limbs;
+};
+
+// Host-side table generation (fhea_context.cpp).
+bool is_prime_u64(u64 n);
+u32 gen_primes(u32 two_n, u32 bits, u32 count, const std::vector<u64> &skip, std::vector<u64> &out);
+u64 min_primitive_root(u64 q, u32 two_n);
+u64 inv_mod(u64 a, u64 q);
+u64 pow_mod(u64 a, u64 e, u64 q);
+u32 bit_reverse(u32 x, u32 bits);
+void build_host_tables(HostTables &h);
+
+}  // namespace fhea

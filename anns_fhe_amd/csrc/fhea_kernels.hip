@@ -1,0 +1,284 @@
+// fhea_kernels.hip -- elementwise, permutation, basis-conversion, key-switch
+// inner-product, ModDown and rescale kernels for gfx950.
+//
+// All of these are streaming kernels over (limb x coefficient): one thread per
+// 64-bit word, consecutive threads on consecutive coefficients, so every global
+// access is coalesced (the automorphism gather aside).  Per-limb constants are
+// wave-uniform and come through the scalar cache.
+#include "fhea_kernels.hpp"
+
+namespace fhea {
+
+static constexpr u32 TPB = 256;
+
+// ---------------------------------------------------------------------------
+// poly_add / poly_sub / poly_mul (poly_arith.cpp:3-42), limb i mod prime i.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_binop(int op, const u64 *__restrict__ a,
+                                               const u64 *__restrict__ b, u64 *__restrict__ o,
+                                               const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y;
+    const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const ModParams m = mp[limb];
+    u64 x = a[i], y = b[i], r;
+    if (op == 0) r = add_mod(x, y, m.q);
+    else if (op == 1) r = sub_mod(x, y, m.q);
+    else r = mulmod(x, y, m);
+    o[i] = r;
+}
+
+void launch_binop(const DevTables &t, int op, const u64 *a, const u64 *b, u64 *o, u32 limbs,
+                  hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs);
+    hipLaunchKernelGGL(k_binop, grid, dim3(TPB), 0, s, op, a, b, o, t.mp, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// Tensor product (fhe_host_driver.cpp:280-286, four OP 5 + one OP 3 fused).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_tensor(const TensorArgs *__restrict__ args,
+                                                const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y;
+    const TensorArgs A = args[blockIdx.z];
+    const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const ModParams m = mp[limb];
+    u64 a0 = A.a0[i], a1 = A.a1[i], b0 = A.b0[i], b1 = A.b1[i];
+    A.d0[i] = mulmod(a0, b0, m);
+    A.d2[i] = mulmod(a1, b1, m);
+    A.d1[i] = reduce128((u128)a0 * b1 + (u128)a1 * b0, m);
+}
+
+void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u32 limbs,
+                   hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs, nbatch);
+    hipLaunchKernelGGL(k_tensor, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// Automorphism with an explicit packed map (automorphism.cpp:3-12, OP 6).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_automorph_map(const u64 *__restrict__ in,
+                                                       u64 *__restrict__ out,
+                                                       const u64 *__restrict__ pmap, u32 logN) {
+    const u32 limb = blockIdx.y;
+    const u32 i = blockIdx.x * TPB + threadIdx.x;
+    u64 p = pmap[i >> 1];
+    u32 idx = (i & 1) ? (u32)(p >> 32) : (u32)p;
+    out[((size_t)limb << logN) + i] = in[((size_t)limb << logN) + idx];
+}
+
+void launch_automorph_map(const DevTables &t, const u64 *in, u64 *out, const u64 *packed_map,
+                          u32 limbs, hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs);
+    hipLaunchKernelGGL(k_automorph_map, grid, dim3(TPB), 0, s, in, out, packed_map, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// sigma_g on c0 and c1 with the closed-form map
+//   map[i] = br(((g (2 br(i) + 1)) mod 2N - 1) / 2)
+// (the evaluation-domain image of a(X) -> a(X^g) in bit-reversed slot order).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u32 galois_src(u32 i, u64 g, u32 logN) {
+    const u32 br = __brev(i) >> (32 - logN);
+    const u64 e = (g * (2ull * br + 1)) & ((2ull << logN) - 1);
+    const u32 k = (u32)((e - 1) >> 1);
+    return __brev(k) >> (32 - logN);
+}
+
+__global__ __launch_bounds__(TPB) void k_automorph(const AutoArgs *__restrict__ args, u32 limbs,
+                                                   u32 logN) {
+    const AutoArgs A = args[blockIdx.z];
+    const u32 y = blockIdx.y;  // [0, 2*limbs): c0 limbs then c1 limbs
+    const u32 limb = y % limbs;
+    const u32 i = blockIdx.x * TPB + threadIdx.x;
+    const u32 src = galois_src(i, A.galois, logN);
+    const u64 *in = (y < limbs) ? A.in0 : A.in1;
+    u64 *out = (y < limbs) ? A.out0 : A.out1;
+    out[((size_t)limb << logN) + i] = in[((size_t)limb << logN) + src];
+}
+
+void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, u32 limbs,
+                      hipStream_t s) {
+    dim3 grid(t.N / TPB, 2 * limbs, nbatch);
+    hipLaunchKernelGGL(k_automorph, grid, dim3(TPB), 0, s, args_dev, limbs, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// Fast basis conversion (key_switch.cpp:7-60):
+//   y_i   = x_i * qhat_i^-1 mod q_i
+//   out_j = sum_i y_i * (qhat_i mod p_j) mod p_j
+// The reference reduces y_i mod p_j before each product; the residue of the
+// sum mod p_j is the same, so the products are accumulated in 128 bits and
+// reduced once per output.
+// ---------------------------------------------------------------------------
+static constexpr u32 FBC_MAX_IN = 32;
+
+__global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
+                                             const u32 *__restrict__ in_first,
+                                             const ModParams *__restrict__ mp,
+                                             const u64 *__restrict__ in, u64 in_zstride,
+                                             u64 *__restrict__ out, u64 out_pstride,
+                                             u64 out_zstride, u32 logN) {
+    const u32 p = blockIdx.y, z = blockIdx.z;
+    const ConvDesc D = descs[p];
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u64 *x = in + (size_t)z * in_zstride + ((size_t)in_first[p] << logN) + r;
+    u64 *o = out + (size_t)z * out_zstride + (size_t)p * out_pstride + r;
+    u64 y[FBC_MAX_IN];
+#pragma unroll 4
+    for (u32 i = 0; i < D.n_in; ++i) {
+        const u64 q = mp[D.in_prime[i]].q;
+        y[i] = mul_shoup(x[(size_t)i << logN], D.in_cst[2 * i], D.in_cst[2 * i + 1], q);
+    }
+    for (u32 j = 0; j < D.n_out; ++j) {
+        u128 acc = 0;
+        const u64 *h = D.hat + j;
+#pragma unroll 4
+        for (u32 i = 0; i < D.n_in; ++i) acc += (u128)y[i] * h[(size_t)i * D.n_out];
+        o[(size_t)j << logN] = reduce128(acc, mp[D.out_prime[j]]);
+    }
+}
+
+void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev, u32 ndesc,
+                u32 max_nout, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
+                u64 out_zstride, u32 nbatch, hipStream_t s) {
+    (void)max_nout;
+    dim3 grid(t.N / TPB, ndesc, nbatch);
+    hipLaunchKernelGGL(k_fbc, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in, in_zstride,
+                       out, out_pstride, out_zstride, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// Key-switch inner product (key_switch.cpp:332-381).  For QlP slot i of the
+// extended basis: acc0_i = sum_p digit_p[i] * evk_b[p][i],
+// acc1_i = sum_p digit_p[i] * evk_a[p][i], where digit_p[i] is the input limb
+// itself when i lies in digit p and the converted limb otherwise.  Products
+// accumulate in 128 bits (< nparts * q^2) and are reduced once.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_ks_inner(const KsIpArgs *__restrict__ args,
+                                                  const u64 *__restrict__ compact, u64 cz,
+                                                  u64 *__restrict__ acc, u64 az,
+                                                  const ModParams *__restrict__ mp, u32 logN, u32 l,
+                                                  u32 K, u32 L, u32 alpha, u32 nparts,
+                                                  u32 key_limbs, u32 dnum) {
+    const u32 i = blockIdx.y;  // QlP slot
+    const u32 z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const KsIpArgs A = args[z];
+    const u32 pidx = (i < l) ? i : (L + i - l);
+    const u32 kl = (i < l) ? i : (key_limbs + i - l);
+    const u64 kstride = (u64)(key_limbs + K) << logN;
+    const u64 *ka = A.key + ((u64)kl << logN) + r;
+    const u64 *kb = ka + (u64)dnum * kstride;
+    const u64 *cz_base = compact + (size_t)z * cz + r;
+    u128 s0 = 0, s1 = 0;
+    for (u32 p = 0; p < nparts; ++p) {
+        const u32 start = p * alpha;
+        const u32 sz = (p == nparts - 1) ? (l - start) : alpha;
+        u64 d;
+        if (i >= start && i < start + sz) {
+            d = A.c[((size_t)i << logN) + r];
+        } else {
+            const u32 pos = (i < start) ? i : (i - sz);
+            d = cz_base[((size_t)p * (l + K) + pos) << logN];
+        }
+        const u64 b = kb[(size_t)p * kstride], a = ka[(size_t)p * kstride];
+        s0 += (u128)d * b;
+        s1 += (u128)d * a;
+    }
+    const ModParams m = mp[pidx];
+    u64 *o = acc + (size_t)z * az + ((size_t)i << logN) + r;
+    o[0] = reduce128(s0, m);
+    o[(size_t)(l + K) << logN] = reduce128(s1, m);
+}
+
+void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
+                     u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
+                     u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s) {
+    dim3 grid(t.N / TPB, l + t.K, nbatch);
+    hipLaunchKernelGGL(k_ks_inner, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride, acc,
+                       acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+}
+
+// ---------------------------------------------------------------------------
+// ModDown combine (key_switch.cpp:253-273), optionally fused with the
+// composite's following poly_add (fhe_host_driver.cpp:289-290).
+// acc layout per batch item: [2][l+K][N] (pstride between the two polys).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_moddown_combine(
+    const CombineArgs *__restrict__ args, const u64 *__restrict__ acc, u64 az, u64 ap,
+    const u64 *__restrict__ tt, u64 tz, u64 tp, const u64 *__restrict__ pinv,
+    const ModParams *__restrict__ mp, u32 logN, u32 l) {
+    const u32 y = blockIdx.y;  // [0, 2l)
+    const u32 poly = y / l, i = y % l, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const CombineArgs A = args[z];
+    const u64 q = mp[i].q;
+    const size_t off = ((size_t)i << logN) + r;
+    u64 x = acc[(size_t)z * az + poly * ap + off];
+    u64 v = tt[(size_t)z * tz + poly * tp + off];
+    u64 res = mul_shoup(sub_mod(x, v, q), pinv[2 * i], pinv[2 * i + 1], q);
+    const u64 *add = poly ? A.add1 : A.add0;
+    if (add) res = add_mod(add[off], res, q);
+    (poly ? A.out1 : A.out0)[off] = res;
+}
+
+void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, const u64 *acc,
+                            u64 acc_zstride, u64 acc_pstride, const u64 *tt, u64 tt_zstride,
+                            u64 tt_pstride, const u64 *pinv, u32 nbatch, u32 l, hipStream_t s) {
+    dim3 grid(t.N / TPB, 2 * l, nbatch);
+    hipLaunchKernelGGL(k_moddown_combine, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride,
+                       acc_pstride, tt, tt_zstride, tt_pstride, pinv, t.mp, t.logN, l);
+}
+
+// ---------------------------------------------------------------------------
+// Rescale (rescale.cpp:43-70).  With v the coefficient-domain last limb and
+// T_i = QlQlInvModqlDivqlModq[i] (== -q_l^-1 mod q_i):
+//   prep:    tmp_i = centred(v) * T_i mod q_i
+//          = v * T_i - [v > floor(q_l/2)] * (q_l * T_i mod q_i)
+//   combine: out_i = a_i * q_l^-1 + NTT_i(tmp_i)  mod q_i
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_rescale_prep(const u64 *__restrict__ v, u64 vz,
+                                                      u64 *__restrict__ tmp, u64 tz,
+                                                      const u64 *__restrict__ qlql,
+                                                      const u64 *__restrict__ sub,
+                                                      const ModParams *__restrict__ mp, u32 logN,
+                                                      u32 l) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u64 q = mp[i].q, ql = mp[l - 1].q;
+    const u64 x = v[(size_t)z * vz + r];
+    u64 t = mul_shoup(x, qlql[2 * i], qlql[2 * i + 1], q);
+    if (x > (ql >> 1)) t = sub_mod(t, sub[i], q);
+    tmp[(size_t)z * tz + ((size_t)i << logN) + r] = t;
+}
+
+void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp, u64 tmp_stride,
+                         const u64 *qlql, const u64 *sub, u32 npolys, u32 l, hipStream_t s) {
+    dim3 grid(t.N / TPB, l - 1, npolys);
+    hipLaunchKernelGGL(k_rescale_prep, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride, qlql, sub,
+                       t.mp, t.logN, l);
+}
+
+__global__ __launch_bounds__(TPB) void k_rescale_combine(const RescaleOut *__restrict__ args,
+                                                         const u64 *__restrict__ tmp, u64 tz,
+                                                         const u64 *__restrict__ qlinv,
+                                                         const ModParams *__restrict__ mp,
+                                                         u32 logN) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const RescaleOut A = args[z];
+    const u64 q = mp[i].q;
+    const size_t off = ((size_t)i << logN) + r;
+    u64 a = mul_shoup(A.a[off], qlinv[2 * i], qlinv[2 * i + 1], q);
+    A.out[off] = add_mod(a, tmp[(size_t)z * tz + off], q);
+}
+
+void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
+                            u64 tmp_stride, const u64 *qlinv, u32 npolys, u32 l, hipStream_t s) {
+    dim3 grid(t.N / TPB, l - 1, npolys);
+    hipLaunchKernelGGL(k_rescale_combine, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride, qlinv,
+                       t.mp, t.logN);
+}
+
+}  // namespace fhea

@@ -1,0 +1,92 @@
+// fhea_kernels.hpp -- launchers for the gfx950 kernels (fhea_ntt.hip, fhea_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fhea_internal.hpp"
+
+namespace fhea {
+
+// Device-resident tables every launcher needs.
+struct DevTables {
+    u32 N = 0, logN = 0, L = 0, K = 0;
+    const ModParams *mp = nullptr;  // [T]
+    const u64 *tw = nullptr;        // [T][N][2]  (psi^br(j), shoup)
+    const u64 *itw = nullptr;       // [T][N][2]  (psi^-br(j), shoup)
+    const u64 *ninv = nullptr;      // [T][2]
+    const u32 *ident = nullptr;     // [T] identity prime map 0..T-1
+    // FP64 transform path, used when every prime is below 2^50
+    bool use_f64 = false;
+    const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
+    const double *itwf = nullptr;   // [T][N][2]
+    const double *ninvf = nullptr;  // [T][2]
+};
+
+// Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
+// has limb slot s at src + z*src_stride + s*src_slot (src_slot 0 = N); slot s
+// uses prime prime_map[s] (0xFFFFFFFF = skip).  Output order is the
+// reference's bit-reversed evaluation order.  In place is allowed (src == dst).
+void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
+                u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
+                u64 src_slot = 0, u64 dst_slot = 0);
+
+// Elementwise ops on `limbs` limbs, limb i mod prime i (poly_arith.cpp:3-42).
+// op: 0 add, 1 sub, 2 mul.
+void launch_binop(const DevTables &t, int op, const u64 *a, const u64 *b, u64 *o, u32 limbs,
+                  hipStream_t s);
+// Tensor product of two ciphertexts (fhe_host_driver.cpp:280-286):
+// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1.  Pointer arrays of npolys.
+struct TensorArgs {
+    const u64 *a0, *a1, *b0, *b1;
+    u64 *d0, *d1, *d2;
+};
+void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u32 limbs,
+                   hipStream_t s);
+// out[i] = in[map[i]] with map packed two u32 per word (automorphism.cpp:3-12).
+void launch_automorph_map(const DevTables &t, const u64 *in, u64 *out, const u64 *packed_map,
+                          u32 limbs, hipStream_t s);
+// out = sigma_g(in) for c0 and c1 of nbatch ciphertexts (closed-form map).
+struct AutoArgs {
+    const u64 *in0, *in1;
+    u64 *out0, *out1;
+    u64 galois;
+};
+void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, u32 limbs,
+                      hipStream_t s);
+
+// Basis conversion (key_switch.cpp:7-60) for a batch: block (x, p, z) converts
+// descriptor descs[p] reading in + z*in_zstride + (in_first_p)*N and writing
+// out + z*out_zstride + p*out_pstride + j*N.
+void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev,
+                u32 ndesc, u32 max_nout, const u64 *in, u64 in_zstride, u64 *out,
+                u64 out_pstride, u64 out_zstride, u32 nbatch, hipStream_t s);
+
+// Key-switch inner product (key_switch.cpp:352-381) for nbatch inputs at level l.
+struct KsIpArgs {
+    const u64 *c;      // input polynomial (eval), l limbs
+    const u64 *key;    // evk base (a then b)
+};
+void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
+                     u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
+                     u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s);
+
+// ModDown combine (key_switch.cpp:253-273): out_i = (x_i - t_i) * P^-1 [+ add_i].
+struct CombineArgs {
+    u64 *out0, *out1;
+    const u64 *add0, *add1;  // may be null
+};
+void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, const u64 *acc,
+                            u64 acc_zstride, u64 acc_pstride, const u64 *tt, u64 tt_zstride,
+                            u64 tt_pstride, const u64 *pinv, u32 nbatch, u32 l, hipStream_t s);
+
+// Rescale (rescale.cpp:3-72) helpers.  v: INTT of the last limb, [z][2][N].
+void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp,
+                         u64 tmp_stride, const u64 *qlql, const u64 *sub, u32 npolys, u32 l,
+                         hipStream_t s);
+struct RescaleOut {
+    const u64 *a;  // input polynomial (l limbs)
+    u64 *out;      // output polynomial (l-1 limbs)
+};
+void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
+                            u64 tmp_stride, const u64 *qlinv, u32 npolys, u32 l, hipStream_t s);
+
+}  // namespace fhea

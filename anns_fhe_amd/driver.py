@@ -1,0 +1,203 @@
+"""Host-side mirror of the reference's FHEHostDriver over the C-ABI.
+
+FheContext owns a libfhea context (parameters, device tables, the poly and key
+arenas in HBM).  FHEHostDriver keeps the reference's method names, argument
+order and meaning (cpp_host_model/src/fhe_host_driver.h:71-111): offsets are
+64-bit word offsets into the poly arena ("poly_gmem") or key arena
+("key_gmem").  Errors raise FheError, as the reference raises runtime_error /
+out_of_range (fhe_host_driver.cpp:336-339, sim_memory_bus.cpp:13-15).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FheError, check  # noqa: F401  (re-exported)
+
+
+def _p64(a):
+    return a.ctypes.data_as(_lib.u64p)
+
+
+class FheContext:
+    """Parameters + device-resident tables + arenas (replaces OP 99 and SimMemoryBus)."""
+
+    def __init__(self, N, L, K, dnum, q_bits=50, q_primes=None, p_primes=None,
+                 poly_words=1 << 23, key_words=1 << 23, device=0):
+        self.lib = _lib.load()
+        prm = _lib.fhea_params()
+        prm.N, prm.L, prm.K, prm.dnum, prm.q_bits, prm.device = N, L, K, dnum, q_bits, device
+        self._qa = np.array(q_primes, np.uint64) if q_primes is not None else None
+        self._pa = np.array(p_primes, np.uint64) if p_primes is not None else None
+        prm.q_primes = _p64(self._qa) if self._qa is not None else C.cast(None, _lib.u64p)
+        prm.p_primes = _p64(self._pa) if self._pa is not None else C.cast(None, _lib.u64p)
+        prm.poly_words, prm.key_words = poly_words, key_words
+        h = C.c_void_p()
+        check(self.lib.fhea_ctx_create(C.byref(prm), C.byref(h)), "fhea_ctx_create")
+        self.h = h
+        info = (C.c_uint32 * 5)()
+        check(self.lib.fhea_ctx_info(self.h, info), "fhea_ctx_info")
+        self.N, self.L, self.K, self.dnum, self.alpha = list(info)
+        pr = np.zeros(self.L + self.K, np.uint64)
+        check(self.lib.fhea_ctx_primes(self.h, _p64(pr)), "fhea_ctx_primes")
+        self.primes = [int(x) for x in pr]
+        self.q = self.primes[: self.L]
+        self.p = self.primes[self.L:]
+        self.poly_words = int(self.lib.fhea_poly_words(self.h))
+        self.key_words = int(self.lib.fhea_key_words(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fhea_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def table(self, which):
+        T = self.L + self.K
+        n = T if which in (0, 1) else T * self.N
+        out = np.zeros(n, np.uint64)
+        check(self.lib.fhea_ctx_table(self.h, which, _p64(out), n), "fhea_ctx_table")
+        return out if which in (0, 1) else out.reshape(T, self.N)
+
+    # ---- arenas (SimMemoryBus::write_poly / read_poly / write_key / read_key)
+    def write_poly(self, off, data):
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        check(self.lib.fhea_write_poly(self.h, off, _p64(a), a.size), "write_poly")
+
+    def read_poly(self, off, count):
+        a = np.zeros(count, np.uint64)
+        check(self.lib.fhea_read_poly(self.h, off, _p64(a), count), "read_poly")
+        return a
+
+    def write_key(self, off, data):
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        check(self.lib.fhea_write_key(self.h, off, _p64(a), a.size), "write_key")
+
+    def read_key(self, off, count):
+        a = np.zeros(count, np.uint64)
+        check(self.lib.fhea_read_key(self.h, off, _p64(a), count), "read_key")
+        return a
+
+    # ---- raw op code (fhe_accel_top)
+    def op(self, op_code, src_a, src_b, dst, evk=0, limbs=1, key_limbs=0):
+        return self.lib.fhea_op(self.h, op_code, src_a, src_b, dst, evk, limbs, key_limbs)
+
+    def submit(self, works):
+        arr = (_lib.fhea_work * len(works))()
+        for i, w in enumerate(works):
+            for k, v in w.items():
+                setattr(arr[i], k, v)
+        check(self.lib.fhea_submit(self.h, arr, len(works)), "fhea_submit")
+
+    def ntt_batch(self, src, dst, npolys, limbs, inverse=False):
+        check(self.lib.fhea_ntt_batch(self.h, src, dst, npolys, limbs, 1 if inverse else 0), "ntt_batch")
+
+    def sync(self):
+        check(self.lib.fhea_sync(self.h), "fhea_sync")
+
+    def timer_start(self):
+        check(self.lib.fhea_timer_start(self.h), "fhea_timer_start")
+
+    def timer_stop(self):
+        ms = C.c_float()
+        check(self.lib.fhea_timer_stop(self.h, C.byref(ms)), "fhea_timer_stop")
+        return ms.value
+
+    # ---- client key generation (CPU)
+    def keygen_secret(self, seed):
+        s = np.zeros(self.N, np.int8)
+        check(self.lib.fhea_keygen_secret(self.h, seed, s.ctypes.data_as(_lib.i8p)), "keygen_secret")
+        return s
+
+    def key_size_words(self, key_limbs):
+        return int(self.lib.fhea_key_size_words(self.h, key_limbs))
+
+    def galois_for_step(self, step):
+        return int(self.lib.fhea_galois_for_step(self.h, step))
+
+    def keygen_relin_key(self, s, key_limbs, seed, evk):
+        check(self.lib.fhea_keygen_relin_key(self.h, s.ctypes.data_as(_lib.i8p), key_limbs, seed, evk),
+              "keygen_relin_key")
+
+    def keygen_rotation_key(self, s, galois, key_limbs, seed, evk):
+        check(self.lib.fhea_keygen_rotation_key(self.h, s.ctypes.data_as(_lib.i8p), galois, key_limbs, seed,
+                                                evk), "keygen_rotation_key")
+
+    def keygen_switch_key(self, s, s_new, key_limbs, seed, evk):
+        sn = np.ascontiguousarray(s_new, dtype=np.int64)
+        check(self.lib.fhea_keygen_switch_key(self.h, s.ctypes.data_as(_lib.i8p), sn.ctypes.data_as(_lib.i64p),
+                                              key_limbs, seed, evk), "keygen_switch_key")
+
+
+class FHEHostDriver:
+    """cpp_host_model/src/fhe_host_driver.h:55-123 over libfhea."""
+
+    def __init__(self, ctx: FheContext):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = ctx.h
+        self.N = ctx.N
+
+    # ---- key material (initialize_context_from_directory, fhe_host_driver.cpp:105-172)
+    def set_relin_key(self, evk, key_limbs=0):
+        check(self.lib.fhea_set_relin_key(self.h, evk, key_limbs), "set_relin_key")
+
+    def add_rotation_key(self, step, galois, evk, key_limbs=0):
+        check(self.lib.fhea_add_rotation_key(self.h, step, galois, evk, key_limbs), "add_rotation_key")
+
+    def load_context_to_hardware(self):
+        check(self.ctx.op(_lib.OP_LOAD_CONTEXT, 0, 0, 0, 0, self.ctx.L), "load_context_to_hardware")
+
+    # ---- primitive wrappers (fhe_host_driver.cpp:210-242)
+    def ntt(self, src, dst, active_limbs):
+        check(self.ctx.op(_lib.OP_NTT, src, 0, dst, 0, active_limbs), "ntt")
+
+    def intt(self, src, dst, active_limbs):
+        check(self.ctx.op(_lib.OP_INTT, src, 0, dst, 0, active_limbs), "intt")
+
+    def poly_add(self, a, b, dst, active_limbs):
+        check(self.ctx.op(_lib.OP_POLY_ADD, a, b, dst, 0, active_limbs), "poly_add")
+
+    def poly_sub(self, a, b, dst, active_limbs):
+        check(self.ctx.op(_lib.OP_POLY_SUB, a, b, dst, 0, active_limbs), "poly_sub")
+
+    def poly_mul_ntt(self, a, b, dst, active_limbs):
+        check(self.ctx.op(_lib.OP_POLY_MUL_NTT, a, b, dst, 0, active_limbs), "poly_mul_ntt")
+
+    def automorphism(self, src, auto_map, dst, active_limbs, galois_elt=0):
+        check(self.ctx.op(_lib.OP_AUTOMORPHISM, src, auto_map, dst, 0, active_limbs), "automorphism")
+
+    def rescale(self, in_c0, in_c1, out_c0, out_c1, active_limbs):
+        # OP 8 writes c1' at out_c0 + (L-1)N, as the reference does (out_c1 unused)
+        check(self.ctx.op(_lib.OP_RESCALE, in_c0, in_c1, out_c0, 0, active_limbs), "rescale")
+
+    def key_switch(self, src_c1, evk, dst_c0, dst_c1, size_ql, key_limbs=0):
+        # OP 7 writes ks1 at dst_c0 + L*N (dst_c1 unused, fhe_host_driver.cpp:236-239)
+        check(self.ctx.op(_lib.OP_KEY_SWITCH, src_c1, 0, dst_c0, evk, size_ql, key_limbs), "key_switch")
+
+    # ---- composites (fhe_host_driver.cpp:248-404)
+    def eval_add(self, a0, a1, b0, b1, d0, d1, limbs):
+        check(self.lib.fhea_eval_add(self.h, a0, a1, b0, b1, d0, d1, limbs), "eval_add")
+
+    def eval_sub_plain(self, a0, a1, pt, d0, d1, limbs):
+        check(self.lib.fhea_eval_sub_plain(self.h, a0, a1, pt, d0, d1, limbs), "eval_sub_plain")
+
+    def eval_mult_relin_rescale(self, a0, a1, b0, b1, d0, d1, in_limbs):
+        check(self.lib.fhea_eval_mult_relin_rescale(self.h, a0, a1, b0, b1, d0, d1, in_limbs),
+              "eval_mult_relin_rescale")
+
+    def eval_mult_plain_rescale(self, a0, a1, pt, d0, d1, in_limbs):
+        check(self.lib.fhea_eval_mult_plain_rescale(self.h, a0, a1, pt, d0, d1, in_limbs),
+              "eval_mult_plain_rescale")
+
+    def eval_rotate(self, in0, in1, d0, d1, total_rot, limbs):
+        check(self.lib.fhea_eval_rotate(self.h, in0, in1, d0, d1, total_rot, limbs), "eval_rotate")
+
+    def eval_automorphism_ks(self, galois, evk, key_limbs, in0, in1, d0, d1, limbs):
+        check(self.lib.fhea_eval_automorphism_ks(self.h, galois, evk, key_limbs, in0, in1, d0, d1, limbs),
+              "eval_automorphism_ks")

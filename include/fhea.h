@@ -1,0 +1,201 @@
+/*
+ * fhea.h -- C-ABI of the MI355X CKKS evaluation engine (libfhea.so).
+ *
+ * Drop-in boundary for the reference's accelerator interface:
+ *   - the op-code kernel  fhe_accel_top(...)
+ *       hardware_fpga_model/hls/src/fhe_accel_top.h:8-55, op codes hls_params.h:18-26
+ *   - the emulated device memories  SimMemoryBus
+ *       cpp_host_model/src/sim_memory_bus.h:10-34
+ *   - the composite calls of  FHEHostDriver
+ *       cpp_host_model/src/fhe_host_driver.h:74-111
+ *
+ * Conventions kept from the reference:
+ *   - polynomials are limb-major: limb i of an L-limb polynomial occupies words
+ *     [i*N, (i+1)*N) from its offset (docs/data_format.md:51-73);
+ *   - all offsets are 64-bit WORD offsets into the poly or key arena and must be
+ *     multiples of 8 (one 512-bit beat, fhe_accel_top.cpp:39);
+ *   - op-code output placement: OP 7 writes ks0 at dst and ks1 at dst+L*N,
+ *     OP 8 writes c0' at dst and c1' at dst+(L-1)*N (fhe_accel_top.cpp:282-326);
+ *   - evaluation keys (OP 7): a-part at evk as [dnum][key_limbs+K][N], b-part at
+ *     evk + dnum*(key_limbs+K)*N (fhe_accel_top.cpp:305).  Q limb i uses key limb
+ *     i and P limb j uses key limb key_limbs+j; key_limbs == L (or 0) is exactly
+ *     the reference layout, key_limbs > L lets one full-level key serve every level.
+ *
+ * Differences (deliberate, see the call-by-call comments):
+ *   - N, L, K and dnum are runtime parameters (the reference compiles in N=16384,
+ *     MAX_LIMBS=15, K<=4, 3 digits; hls_params.h:6-8, key_switch.cpp:271-272);
+ *   - the context tables are generated at fhea_ctx_create (replaces OP 99 and the
+ *     table loading of fhe_host_driver.cpp:18-175) for EVERY level, not one;
+ *   - every call returns an int status (FHEA_*), never throws, and rejects
+ *     unknown op codes, unaligned or out-of-range offsets and missing keys
+ *     (the reference ignores, truncates or throws; fhe_accel_top.cpp:220-327,
+ *     fhe_host_driver.cpp:336-339, sim_memory_bus.cpp:13-15).
+ *
+ * Threading: a context is externally synchronised (one host thread at a time);
+ * all work is issued on the context's HIP stream in call order.
+ */
+#ifndef FHEA_H
+#define FHEA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FHEA_OK 0
+#define FHEA_ERR_BAD_OPCODE (-1) /* unknown op code                              */
+#define FHEA_ERR_RANGE (-2)      /* offset/size outside an arena                 */
+#define FHEA_ERR_ALIGN (-3)      /* offset not a multiple of 8 words             */
+#define FHEA_ERR_ARG (-4)        /* bad limb count / parameter                   */
+#define FHEA_ERR_NO_KEY (-5)     /* missing relinearisation / rotation key       */
+#define FHEA_ERR_NO_MEM (-6)     /* device allocation failed / arena exhausted   */
+#define FHEA_ERR_DEVICE (-7)     /* HIP runtime error                            */
+#define FHEA_ERR_STATE (-8)      /* null context / not initialised               */
+
+/* ---- op codes (hls_params.h:18-26) -------------------------------------- */
+#define FHEA_OP_NTT 1
+#define FHEA_OP_INTT 2
+#define FHEA_OP_POLY_ADD 3
+#define FHEA_OP_POLY_SUB 4
+#define FHEA_OP_POLY_MUL_NTT 5
+#define FHEA_OP_AUTOMORPHISM 6
+#define FHEA_OP_KEY_SWITCH 7
+#define FHEA_OP_RESCALE 8
+#define FHEA_OP_LOAD_CONTEXT 99
+
+typedef struct fhea_ctx fhea_ctx;
+
+/* Parameter block: replaces FHEContextConfig (fhe_host_driver.h:17-53). */
+typedef struct {
+    uint32_t N;                /* ring dimension, power of two, 2^10..2^17        */
+    uint32_t L;                /* number of Q primes (top level)                  */
+    uint32_t K;                /* number of special P primes                      */
+    uint32_t dnum;             /* key-switching digits at level L (alpha=ceil(L/dnum)) */
+    uint32_t q_bits;           /* bit size for generated primes (<= 61), 0 = 50   */
+    uint32_t device;           /* HIP device ordinal                              */
+    const uint64_t *q_primes;  /* optional explicit Q primes (L), NULL = generate */
+    const uint64_t *p_primes;  /* optional explicit P primes (K), NULL = generate */
+    uint64_t poly_words;       /* poly arena size in 64-bit words (poly_gmem)      */
+    uint64_t key_words;        /* key arena size in 64-bit words (key_gmem)        */
+} fhea_params;
+
+/* ---- context (replaces OP 99 + initialize_context_from_directory,
+ *      fhe_host_driver.cpp:18-175,206-208; fhe_accel_top.cpp:194-218) ------- */
+int fhea_ctx_create(const fhea_params *params, fhea_ctx **out);
+int fhea_ctx_destroy(fhea_ctx *ctx);
+/* N, L, K, dnum, alpha */
+int fhea_ctx_info(const fhea_ctx *ctx, uint32_t *info5);
+/* L+K primes, Q first (rns_primes_Q.bin / rns_primes_P.bin). */
+int fhea_ctx_primes(const fhea_ctx *ctx, uint64_t *out);
+/* Exported tables for verification: which = 0 psi[T], 1 n_inv[T], 2 tw[T*N],
+ * 3 itw[T*N] (twiddles.bin / inv_twiddles.bin layout). */
+int fhea_ctx_table(const fhea_ctx *ctx, int which, uint64_t *out, uint64_t count);
+/* Use an external HIP stream (hipStream_t) for all subsequent work; NULL = own. */
+int fhea_ctx_set_stream(fhea_ctx *ctx, void *hip_stream);
+void *fhea_ctx_stream(const fhea_ctx *ctx);
+const char *fhea_status_string(int status);
+
+/* ---- arenas (SimMemoryBus::write_poly/read_poly/write_key/read_key,
+ *      sim_memory_bus.cpp:12-64) ------------------------------------------- */
+int fhea_write_poly(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint64_t count);
+int fhea_read_poly(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
+int fhea_write_key(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint64_t count);
+int fhea_read_key(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
+int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t count);
+/* Device base pointers of the arenas (poly_gmem_ptr / key_gmem_ptr, sim_memory_bus.h:16-17). */
+uint64_t *fhea_poly_arena(fhea_ctx *ctx);
+uint64_t *fhea_key_arena(fhea_ctx *ctx);
+uint64_t fhea_poly_words(const fhea_ctx *ctx);
+uint64_t fhea_key_words(const fhea_ctx *ctx);
+
+/* ---- L3: one op code (fhe_accel_top(...) / FHEHostDriver::dispatch,
+ *      fhe_accel_top.cpp:220-327, fhe_host_driver.cpp:177-242) --------------
+ * src_a, src_b, dst: poly-arena word offsets; evk: key-arena word offset;
+ * active_limbs: L of the operands; key_limbs: stored Q limbs of the key (OP 7),
+ * 0 = active_limbs.  OP 6 reads the packed Galois map (N/2 words, two u32 per
+ * word, low half = even index) at src_b.  OP 99 is accepted as a no-op (tables
+ * are built at context creation).  In-place operation (dst == src) is allowed. */
+int fhea_op(fhea_ctx *ctx, uint32_t op_code, uint64_t src_a, uint64_t src_b, uint64_t dst,
+            uint64_t evk, uint32_t active_limbs, uint32_t key_limbs);
+
+/* ---- L5: composites (FHEHostDriver, fhe_host_driver.cpp:248-404) --------- */
+int fhea_eval_add(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                  uint64_t d0, uint64_t d1, uint32_t limbs);
+int fhea_eval_sub_plain(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t pt,
+                        uint64_t d0, uint64_t d1, uint32_t limbs);
+/* relin key location (FHEContextConfig::relin_key_offset, fhe_host_driver.h:52). */
+int fhea_set_relin_key(fhea_ctx *ctx, uint64_t evk, uint32_t key_limbs);
+/* d = Rescale(Relin(a * b)), in_limbs -> in_limbs-1 (fhe_host_driver.cpp:272-306). */
+int fhea_eval_mult_relin_rescale(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t b0,
+                                 uint64_t b1, uint64_t d0, uint64_t d1, uint32_t in_limbs);
+/* d = Rescale(a * pt) (fhe_host_driver.cpp:308-329). */
+int fhea_eval_mult_plain_rescale(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t pt,
+                                 uint64_t d0, uint64_t d1, uint32_t in_limbs);
+/* Rotation-key directory entry (RotationKeyMeta, fhe_host_driver.h:10-15):
+ * step -> Galois element + key location. */
+int fhea_add_rotation_key(fhea_ctx *ctx, int32_t step, uint64_t galois, uint64_t evk,
+                          uint32_t key_limbs);
+/* Rotation by total_rot via its signed power-of-two decomposition, largest step
+ * first (fhe_host_driver.cpp:361-404); FHEA_ERR_NO_KEY if a step key is missing. */
+int fhea_eval_rotate(fhea_ctx *ctx, uint64_t in0, uint64_t in1, uint64_t d0, uint64_t d1,
+                     int32_t total_rot, uint32_t limbs);
+/* One step of eval_rotate_single_step (fhe_host_driver.cpp:331-359) with an
+ * explicit Galois element and key: d = (sigma_g(c0) + ks0, ks1), ks = KS(sigma_g(c1)). */
+int fhea_eval_automorphism_ks(fhea_ctx *ctx, uint64_t galois, uint64_t evk, uint32_t key_limbs,
+                              uint64_t in0, uint64_t in1, uint64_t d0, uint64_t d1,
+                              uint32_t limbs);
+
+/* ---- batched submission (no reference counterpart: the reference issues one
+ *      ciphertext at a time).  Items of the same kind and level are fused into
+ *      one set of launches over the whole batch. ---------------------------- */
+#define FHEA_WORK_MULT_RELIN_RESCALE 1 /* a0,a1,b0,b1 -> d0,d1                    */
+#define FHEA_WORK_AUTOMORPHISM_KS 2    /* a0,a1 -> d0,d1 with galois, evk         */
+#define FHEA_WORK_NTT 3                /* a0 -> d0 (limbs), forward               */
+#define FHEA_WORK_INTT 4               /* a0 -> d0 (limbs), inverse               */
+typedef struct {
+    uint32_t kind;
+    uint32_t limbs;
+    uint64_t a0, a1, b0, b1;
+    uint64_t d0, d1;
+    uint64_t galois;
+    uint64_t evk;
+    uint32_t key_limbs;
+    uint32_t reserved;
+} fhea_work;
+int fhea_submit(fhea_ctx *ctx, const fhea_work *items, uint32_t n_items);
+/* Batched forward (inverse != 0: inverse) NTT of npolys contiguous polynomials
+ * of `limbs` limbs each (poly z at src + z*limbs*N), limb i mod prime i
+ * (OP 1 / OP 2 over a whole batch in one set of launches). */
+int fhea_ntt_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t npolys, uint32_t limbs,
+                   int inverse);
+/* Wait for all work issued on the context's stream. */
+int fhea_sync(fhea_ctx *ctx);
+
+/* ---- timing on the context's stream (HIP events) ----------------------- */
+int fhea_timer_start(fhea_ctx *ctx);
+/* milliseconds since fhea_timer_start, synchronising the stream */
+int fhea_timer_stop(fhea_ctx *ctx, float *ms);
+
+/* ---- CPU client (key generation), the role of OpenFHE KeyGen /
+ *      EvalMultKeyGen / EvalRotateKeyGen in
+ *      integration_tools/generate_interactive_test_vectors.cpp:223-224 -------
+ * s: N ternary coefficients in {-1,0,1} (int8).  Keys are written into the key
+ * arena at evk in the layout described above, for level key_limbs. */
+int fhea_keygen_secret(const fhea_ctx *ctx, uint64_t seed, int8_t *s_out);
+/* hybrid key switching s_new -> s; s_new given by int64 coefficients. */
+int fhea_keygen_switch_key(fhea_ctx *ctx, const int8_t *s, const int64_t *s_new,
+                           uint32_t key_limbs, uint64_t seed, uint64_t evk);
+int fhea_keygen_relin_key(fhea_ctx *ctx, const int8_t *s, uint32_t key_limbs, uint64_t seed,
+                          uint64_t evk);
+int fhea_keygen_rotation_key(fhea_ctx *ctx, const int8_t *s, uint64_t galois, uint32_t key_limbs,
+                             uint64_t seed, uint64_t evk);
+/* Key size in words for one key at level key_limbs. */
+uint64_t fhea_key_size_words(const fhea_ctx *ctx, uint32_t key_limbs);
+/* Galois element for a slot rotation by step: 5^step mod 2N (step may be negative). */
+uint64_t fhea_galois_for_step(const fhea_ctx *ctx, int32_t step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

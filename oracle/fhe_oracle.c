@@ -25,7 +25,8 @@
  *   orc_fast_bases_conv      CRT closed form sum_i [x_i qhat_i^-1]_qi qhat_i mod p
  *   orc_approx_mod_down      CRT closed form with exact big-int P division bound
  *   orc_key_switch           decryption invariant with keys made in the test
- *                            (|Dec(KS(d)) - d s'| small) + linearity
+ *                            (|Dec(KS(d)) - d s'| small) and the exact noiseless
+ *                            case (s = 0, e = 0: KS(d) = (d s', .) bit for bit)
  *   composites               composition of the above + decryption invariants
  */
 #include "fhe_oracle.h"
@@ -210,11 +211,19 @@ static uint32_t log2u(uint32_t n) { uint32_t l = 0; while ((1u << l) < n) l++; r
 void orc_twiddles(uint64_t q, uint64_t psi, uint32_t N, uint64_t *tw, uint64_t *itw) {
     uint32_t lg = log2u(N);
     uint64_t psi_inv = orc_inv_mod(psi, q);
+    uint64_t *pw = (uint64_t *)malloc((size_t)N * 8), *pwi = (uint64_t *)malloc((size_t)N * 8);
+    uint64_t c = 1, ci = 1;
+    for (uint32_t e = 0; e < N; e++) {
+        pw[e] = c; pwi[e] = ci;
+        c = (uint64_t)(((u128)c * psi) % q);
+        ci = (uint64_t)(((u128)ci * psi_inv) % q);
+    }
     for (uint32_t j = 0; j < N; j++) {
         uint32_t br = orc_bit_reverse(j, lg);
-        tw[j] = orc_pow_mod(psi, br, q);
-        itw[j] = orc_pow_mod(psi_inv, br, q);
+        tw[j] = pw[br];
+        itw[j] = pwi[br];
     }
+    free(pw); free(pwi);
 }
 
 /* ===================================================================== */

@@ -1,0 +1,356 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle,
+bit-exact on every RNS residue.  Mirrors the reference testbenches
+(tb_ntt.cpp, tb_intt.cpp, tb_poly_arith.cpp, tb_automorphism.cpp,
+tb_rescale.cpp, tb_keyswitch.cpp, tb_fhe_accel.cpp) and the main_sim.cpp
+sequence, at runtime N/L/K/dnum instead of the compiled-in N=16384."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import ckks_ref as ck
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-parity]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_poly(rng, primes, N):
+    return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+
+
+_CTX = {}
+
+
+def get_ctx(N, L, K, dnum, q_bits=50, poly_words=1 << 24, key_words=1 << 24):
+    from anns_fhe_amd.driver import FheContext
+    key = (N, L, K, dnum, q_bits, poly_words, key_words)
+    if key not in _CTX:
+        say("ctx create", key)
+        _CTX[key] = FheContext(N, L, K, dnum, q_bits=q_bits, poly_words=poly_words, key_words=key_words)
+    return _CTX[key]
+
+
+def get_orc(orc, ctx):
+    return orc.OracleCtx(ctx.N, ctx.q, ctx.p, ctx.dnum)
+
+
+def test_tables_match_oracle(orc):
+    ctx = get_ctx(8192, 4, 1, 2)
+    assert ctx.primes == orc.gen_primes(2 * 8192, 50, 5)
+    o = get_orc(orc, ctx)
+    assert np.array_equal(ctx.table(0), o.psi)
+    assert np.array_equal(ctx.table(1), o.n_inv)
+    assert np.array_equal(ctx.table(2), o.tw)
+    assert np.array_equal(ctx.table(3), o.itw)
+
+
+@pytest.mark.parametrize("N,L", [(1024, 3), (4096, 2), (8192, 4), (16384, 3), (65536, 3)])
+def test_ntt_intt_parity(orc, N, L):
+    say("ntt", N, L)
+    ctx = get_ctx(N, L, 1, 1)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(42)
+    a = rand_poly(rng, ctx.q, N)
+    a[0, :4] = [0, 1, ctx.q[0] - 1, ctx.q[0] - 2]
+    W = L * N
+    ctx.write_poly(0, a)
+    assert ctx.op(1, 0, 0, W, 0, L) == 0
+    got = ctx.read_poly(W, W).reshape(L, N)
+    assert np.array_equal(got, o.ntt_poly(a))
+    # INTT back (in place) is the identity, INTT of random data matches
+    assert ctx.op(2, W, 0, W, 0, L) == 0
+    assert np.array_equal(ctx.read_poly(W, W).reshape(L, N), a)
+    assert ctx.op(2, 0, 0, 2 * W, 0, L) == 0
+    assert np.array_equal(ctx.read_poly(2 * W, W).reshape(L, N), o.intt_poly(a))
+
+
+def test_elementwise_parity(orc):
+    N, L = 8192, 4
+    ctx = get_ctx(N, L, 1, 2)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(1)
+    W = L * N
+    a, b = rand_poly(rng, ctx.q, N), rand_poly(rng, ctx.q, N)
+    a[:, 0] = [q - 1 for q in ctx.q]
+    b[:, 0] = [q - 1 for q in ctx.q]
+    pm = np.zeros(8 * W, np.uint64)
+    pm[:W], pm[W:2 * W] = a.reshape(-1), b.reshape(-1)
+    ctx.write_poly(0, pm[:2 * W])
+    for op in (3, 4, 5):
+        say("binop", op)
+        assert ctx.op(op, 0, W, 2 * W, 0, L) == 0
+        ref = pm.copy()
+        assert o.accel_op(ref, None, op, 0, W, 2 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(2 * W, W), ref[2 * W:3 * W])
+    # in place (dst == src_a)
+    assert ctx.op(5, 0, W, 0, 0, L) == 0
+    ref = pm.copy()
+    o.accel_op(ref, None, 5, 0, W, 0, 0, L)
+    assert np.array_equal(ctx.read_poly(0, W), ref[:W])
+
+
+def test_automorphism_op6_parity(orc):
+    N, L = 8192, 3
+    ctx = get_ctx(N, L, 1, 1)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(2)
+    W = L * N
+    a = rand_poly(rng, ctx.q, N)
+    for g in (5, pow(5, 7, 2 * N), 2 * N - 1):
+        say("automorphism", g)
+        packed = orc.pack_map(orc.galois_map(N, g))
+        pm = np.zeros(4 * W, np.uint64)
+        pm[:W] = a.reshape(-1)
+        pm[W:W + N // 2] = packed
+        ctx.write_poly(0, pm[:2 * W])
+        assert ctx.op(6, 0, W, 2 * W, 0, L) == 0
+        ref = pm.copy()
+        assert o.accel_op(ref, None, 6, 0, W, 2 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(2 * W, W), ref[2 * W:3 * W])
+        # in place
+        assert ctx.op(6, 0, W, 0, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(0, W), ref[2 * W:3 * W])
+
+
+@pytest.mark.parametrize("L", [4, 2])
+def test_rescale_op8_parity(orc, L):
+    N = 8192
+    ctx = get_ctx(N, 4, 1, 2)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(3 + L)
+    W = L * N
+    c0, c1 = rand_poly(rng, ctx.q[:L], N), rand_poly(rng, ctx.q[:L], N)
+    pm = np.zeros(8 * W, np.uint64)
+    pm[:W], pm[W:2 * W] = c0.reshape(-1), c1.reshape(-1)
+    ctx.write_poly(0, pm[:2 * W])
+    say("rescale", L)
+    assert ctx.op(8, 0, W, 4 * W, 0, L) == 0
+    ref = pm.copy()
+    assert o.accel_op(ref, None, 8, 0, W, 4 * W, 0, L) == 0
+    W1 = (L - 1) * N
+    assert np.array_equal(ctx.read_poly(4 * W, 2 * W1), ref[4 * W:4 * W + 2 * W1])
+    # output overlapping the inputs (dst == src_a)
+    assert ctx.op(8, 0, W, 0, 0, L) == 0
+    assert np.array_equal(ctx.read_poly(0, 2 * W1), ref[4 * W:4 * W + 2 * W1])
+
+
+@pytest.mark.parametrize("sizeQl,key_limbs", [(6, 6), (5, 5), (3, 6), (1, 2)])
+def test_keyswitch_op7_parity(orc, sizeQl, key_limbs):
+    N, L, K, dnum = 4096, 6, 2, 3  # alpha = 2
+    ctx = get_ctx(N, L, K, dnum)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(10 * sizeQl + key_limbs)
+    W = sizeQl * N
+    kidx = list(range(key_limbs)) + [L + j for j in range(K)]
+    key = np.concatenate([rand_poly(rng, [ctx.primes[i] for i in kidx], N).reshape(-1)
+                          for _ in range(2 * dnum)])
+    evk = 1024
+    ctx.write_key(evk, key)
+    c = rand_poly(rng, ctx.q[:sizeQl], N)
+    ctx.write_poly(0, c)
+    say("keyswitch", sizeQl, key_limbs)
+    assert ctx.op(7, 0, 0, 8 * W, evk, sizeQl, key_limbs) == 0
+    km = np.zeros(evk + key.size, np.uint64)
+    km[evk:] = key
+    pm = np.zeros(12 * W, np.uint64)
+    pm[:W] = c.reshape(-1)
+    assert o.accel_op(pm, km, 7, 0, 0, 8 * W, evk, sizeQl, key_limbs) == 0
+    assert np.array_equal(ctx.read_poly(8 * W, 2 * W), pm[8 * W:10 * W])
+    # in place: ks0 over the input
+    assert ctx.op(7, 0, 0, 0, evk, sizeQl, key_limbs) == 0
+    assert np.array_equal(ctx.read_poly(0, 2 * W), pm[8 * W:10 * W])
+
+
+def test_status_codes_gpu():
+    ctx = get_ctx(8192, 4, 1, 2)
+    N = ctx.N
+    assert ctx.op(42, 0, 0, 0, 0, 2) == -1
+    assert ctx.op(1, 3, 0, 0, 0, 2) == -3
+    assert ctx.op(1, ctx.poly_words - N, 0, 0, 0, 2) == -2
+    assert ctx.op(1, 0, 0, 0, 0, 0) == -4
+    assert ctx.op(8, 0, 8 * N, 16 * N, 0, 1) == -4
+    assert ctx.op(7, 0, 0, 8 * N, ctx.key_words - 8, 2) == -2
+    assert ctx.op(99, 0, 0, 0, 0, 4) == 0
+
+
+def _c1_setup(ctx, seed=42):
+    """C1 keys: seeded ternary secret, relin key, rotation key for +1."""
+    N, L = ctx.N, ctx.L
+    s = ctx.keygen_secret(seed)
+    relin = 0
+    ctx.keygen_relin_key(s, L, seed + 1, relin)
+    rot = ctx.key_size_words(L)
+    g1 = ctx.galois_for_step(1)
+    ctx.keygen_rotation_key(s, g1, L, seed + 2, rot)
+    return s, relin, rot, g1
+
+
+def test_c1_sequence_bit_exact(orc):
+    """C1: N=2^13, L=4, K=1, dnum=2; eval_sub_plain -> eval_mult_relin_rescale
+    -> eval_rotate(+1), uniform residues, bit-exact (main_sim.cpp CP2-CP5 order)."""
+    from anns_fhe_amd.driver import FHEHostDriver
+    N, L = 8192, 4
+    ctx = get_ctx(N, L, 1, 2)
+    o = get_orc(orc, ctx)
+    drv = FHEHostDriver(ctx)
+    s, relin, rot, g1 = _c1_setup(ctx)
+    drv.set_relin_key(relin, L)
+    drv.add_rotation_key(1, g1, rot, L)
+    km = ctx.read_key(0, 2 * rot)
+    rng = np.random.default_rng(42)
+    W = L * N
+    ct0, ct1, pt = rand_poly(rng, ctx.q, N), rand_poly(rng, ctx.q, N), rand_poly(rng, ctx.q, N)
+    pm = np.zeros(12 * W, np.uint64)
+    pm[:W], pm[W:2 * W], pm[2 * W:3 * W] = ct0.reshape(-1), ct1.reshape(-1), pt.reshape(-1)
+    ctx.write_poly(0, pm[:3 * W])
+    say("c1 sequence")
+    drv.eval_sub_plain(0, W, 2 * W, 3 * W, 4 * W, L)
+    drv.eval_mult_relin_rescale(3 * W, 4 * W, 3 * W, 4 * W, 5 * W, 6 * W, L)
+    drv.eval_rotate(5 * W, 6 * W, 7 * W, 8 * W, 1, L - 1)
+    got = ctx.read_poly(0, 9 * W)
+    ref = pm.copy()
+    assert o.eval_sub_plain(ref, 0, W, 2 * W, 3 * W, 4 * W, L) == 0
+    assert np.array_equal(got[3 * W:5 * W], ref[3 * W:5 * W])
+    assert o.eval_mult_relin_rescale(ref, km, relin, L, 3 * W, 4 * W, 3 * W, 4 * W, 5 * W, 6 * W, L) == 0
+    W1 = (L - 1) * N
+    assert np.array_equal(got[5 * W:5 * W + W1], ref[5 * W:5 * W + W1])
+    assert np.array_equal(got[6 * W:6 * W + W1], ref[6 * W:6 * W + W1])
+    assert o.eval_rotate(ref, km, [(1, g1, rot, L)], 5 * W, 6 * W, 7 * W, 8 * W, 1, L - 1) == 0
+    assert np.array_equal(got[7 * W:7 * W + W1], ref[7 * W:7 * W + W1])
+    assert np.array_equal(got[8 * W:8 * W + W1], ref[8 * W:8 * W + W1])
+
+
+def test_rotate_decomposition_and_missing_key(orc):
+    from anns_fhe_amd.driver import FHEHostDriver, FheError
+    N, L = 4096, 3
+    ctx = get_ctx(N, L, 1, 3)
+    o = get_orc(orc, ctx)
+    drv = FHEHostDriver(ctx)
+    s = ctx.keygen_secret(5)
+    ksz = ctx.key_size_words(L)
+    keys = []
+    for i, step in enumerate([1, 2, -1, -4]):
+        g = ctx.galois_for_step(step)
+        ctx.keygen_rotation_key(s, g, L, 100 + i, i * ksz)
+        drv.add_rotation_key(step, g, i * ksz, L)
+        keys.append((step, g, i * ksz, L))
+    km = ctx.read_key(0, 4 * ksz)
+    rng = np.random.default_rng(8)
+    W = L * N
+    c0, c1 = rand_poly(rng, ctx.q, N), rand_poly(rng, ctx.q, N)
+    pm = np.zeros(8 * W, np.uint64)
+    pm[:W], pm[W:2 * W] = c0.reshape(-1), c1.reshape(-1)
+    for rot in (3, -5, 0):
+        say("rotate", rot)
+        ctx.write_poly(0, pm[:2 * W])
+        drv.eval_rotate(0, W, 2 * W, 3 * W, rot, L)
+        ref = pm.copy()
+        assert o.eval_rotate(ref, km, keys, 0, W, 2 * W, 3 * W, rot, L) == 0
+        assert np.array_equal(ctx.read_poly(2 * W, 2 * W), ref[2 * W:4 * W])
+    with pytest.raises(FheError) as ei:
+        drv.eval_rotate(0, W, 2 * W, 3 * W, 8, L)
+    assert ei.value.status == -5
+
+
+def test_batched_submit_matches_single_calls(orc):
+    from anns_fhe_amd.driver import FHEHostDriver
+    N, L = 4096, 4
+    ctx = get_ctx(N, L, 2, 2)
+    drv = FHEHostDriver(ctx)
+    s = ctx.keygen_secret(9)
+    ctx.keygen_relin_key(s, L, 10, 0)
+    drv.set_relin_key(0, L)
+    ksz = ctx.key_size_words(L)
+    gs = [ctx.galois_for_step(k) for k in (1, 3, 7)]
+    for i, g in enumerate(gs):
+        ctx.keygen_rotation_key(s, g, L, 20 + i, (i + 1) * ksz)
+    rng = np.random.default_rng(11)
+    W = L * N
+    n = 5
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(2 * n)])
+    ctx.write_poly(0, data)
+    say("batched mult")
+    works = [dict(kind=1, limbs=L, a0=2 * i * W, a1=(2 * i + 1) * W, b0=2 * ((i + 1) % n) * W,
+                  b1=(2 * ((i + 1) % n) + 1) * W, d0=(20 + 2 * i) * W, d1=(21 + 2 * i) * W) for i in range(n)]
+    ctx.submit(works)
+    batched = ctx.read_poly(20 * W, 2 * n * W)
+    for w in works:
+        drv.eval_mult_relin_rescale(w["a0"], w["a1"], w["b0"], w["b1"], w["d0"], w["d1"], L)
+    single = ctx.read_poly(20 * W, 2 * n * W)
+    assert np.array_equal(batched, single)
+    say("batched automorphism-ks")
+    works = [dict(kind=2, limbs=L, key_limbs=L, a0=2 * i * W, a1=(2 * i + 1) * W, d0=(40 + 2 * i) * W,
+                  d1=(41 + 2 * i) * W, galois=gs[i % 3], evk=((i % 3) + 1) * ksz) for i in range(n)]
+    ctx.submit(works)
+    batched = ctx.read_poly(40 * W, 2 * n * W)
+    for w in works:
+        drv.eval_automorphism_ks(w["galois"], w["evk"], L, w["a0"], w["a1"], w["d0"], w["d1"], L)
+    single = ctx.read_poly(40 * W, 2 * n * W)
+    assert np.array_equal(batched, single)
+    # and one item against the oracle
+    o = get_orc(orc, ctx)
+    km = ctx.read_key(0, 4 * ksz)
+    pm = np.zeros(50 * W, np.uint64)
+    pm[:2 * n * W] = data
+    w = works[1]
+    assert o.eval_automorphism_ks(pm, km, w["galois"], w["evk"], L, w["a0"], w["a1"], w["d0"], w["d1"], L) == 0
+    assert np.array_equal(single[2 * W:4 * W], pm[w["d0"]:w["d0"] + 2 * W])
+
+
+def test_keygen_keys_decrypt(orc):
+    """The engine's own keys: Dec(KS_relin(d)) ~= d * s^2 (product client pinned
+    by the decryption invariant, evaluated in Python big integers)."""
+    N, L = 1024, 3
+    ctx = get_ctx(N, L, 1, 3)
+    o = get_orc(orc, ctx)
+    s = ctx.keygen_secret(77)
+    assert set(np.unique(s)) <= {-1, 0, 1}
+    ctx.keygen_relin_key(s, L, 78, 0)
+    rng = np.random.default_rng(12)
+    W = L * N
+    d = rand_poly(rng, ctx.q, N)
+    ctx.write_poly(0, d)
+    assert ctx.op(7, 0, 0, 4 * W, 0, L, L) == 0
+    ks = ctx.read_poly(4 * W, 2 * W).reshape(2, L, N)
+    got = ck.decrypt_coeffs(o, s.astype(np.int64), ks[0], ks[1])
+    from tests.bigint_ref import centered, crt, prod
+    dc = o.intt_poly(d)
+    Q = prod(ctx.q)
+    dint = [crt([dc[i][j] for i in range(L)], ctx.q)[0] for j in range(N)]
+    s2 = ck.negacyclic(s.astype(np.int64), s.astype(np.int64))
+    # d * s^2 via two negacyclic products with exact integers
+    exp = [centered(v, Q) for v in ck.negacyclic(dint, s2)]
+    err = max(abs(centered(g - e, Q)) for g, e in zip(got, exp))
+    assert err < 2 ** 24, err
+
+
+def test_full_size_hmult_bit_exact(orc):
+    """One eval_mult_relin_rescale at N=2^16, L=24, K=8, dnum=3 against the oracle."""
+    N, L, K, dnum = 65536, 24, 8, 3
+    W = L * N
+    ctx = get_ctx(N, L, K, dnum, poly_words=8 * W, key_words=2 * dnum * (L + K) * N + 64)
+    from anns_fhe_amd.driver import FHEHostDriver
+    drv = FHEHostDriver(ctx)
+    rng = np.random.default_rng(2024)
+    kidx = list(range(L)) + [L + j for j in range(K)]
+    key = np.concatenate([rand_poly(rng, [ctx.primes[i] for i in kidx], N).reshape(-1) for _ in range(2 * dnum)])
+    ctx.write_key(0, key)
+    drv.set_relin_key(0, L)
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+    ctx.write_poly(0, data)
+    say("full-size hmult gpu")
+    drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 4 * W, 5 * W, L)
+    W1 = (L - 1) * N
+    got0 = ctx.read_poly(4 * W, W1)
+    got1 = ctx.read_poly(5 * W, W1)
+    say("full-size hmult oracle")
+    o = get_orc(orc, ctx)
+    pm = np.zeros(6 * W, np.uint64)
+    pm[:4 * W] = data
+    assert o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+    assert np.array_equal(got0, pm[4 * W:4 * W + W1])
+    assert np.array_equal(got1, pm[5 * W:5 * W + W1])

@@ -85,6 +85,8 @@ struct Blob {
             fix.push_back(b + offsetof(ConvDesc, out_prime));
             fix.push_back(b + offsetof(ConvDesc, out_pos));
             fix.push_back(b + offsetof(ConvDesc, hat));
+            if (v[i].in_cstf) fix.push_back(b + offsetof(ConvDesc, in_cstf));
+            if (v[i].hatf) fix.push_back(b + offsetof(ConvDesc, hatf));
         }
         return off;
     }
@@ -99,8 +101,18 @@ static u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
 
 struct LevelOffsets {
     size_t modup_desc, modup_first, modup_ntt_prime, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
-        rs_last;
+        rs_last, pinvf, rs_qlinvf, rs_qlqlf;
 };
+
+// (value, value/modulus) pairs for the FP64 kernels, from (value, shoup) pairs
+static std::vector<double> f64_pairs(const std::vector<u64> &vs, const std::vector<u64> &mods) {
+    std::vector<double> out(2 * mods.size());
+    for (size_t i = 0; i < mods.size(); ++i) {
+        out[2 * i] = (double)vs[2 * i];
+        out[2 * i + 1] = (double)vs[2 * i] / (double)mods[i];
+    }
+    return out;
+}
 
 static int build_device_tables(fhea_ctx *c) {
     const HostTables &h = c->h;
@@ -128,8 +140,16 @@ static int build_device_tables(fhea_ctx *c) {
     itw.clear();
     itw.shrink_to_fit();
     // FP64 transform tables (w, w/q) when every prime is below 2^50
+    // The FP64 bounds need every q < 2^50 and max(q)/min(q) < 4 (a residue
+    // reduced mod one prime must stay below 8x any other prime).
     bool use_f64 = true;
-    for (u32 i = 0; i < T; ++i) use_f64 &= h.primes[i] < (1ull << 50);
+    u64 qmin = ~0ull, qmax = 0;
+    for (u32 i = 0; i < T; ++i) {
+        use_f64 &= h.primes[i] < (1ull << 50);
+        qmin = std::min(qmin, h.primes[i]);
+        qmax = std::max(qmax, h.primes[i]);
+    }
+    use_f64 &= qmax / qmin < 4;
     size_t o_twf = 0, o_itwf = 0, o_ninvf = 0;
     if (use_f64) {
         std::vector<double> twf((size_t)T * N * 2), itwf((size_t)T * N * 2), ninvf(2 * T);
@@ -199,6 +219,19 @@ static int build_device_tables(fhea_ctx *c) {
             D.out_prime = off_ptr<const u32>(b.addv(outp));
             D.out_pos = off_ptr<const u32>(b.addv(outp));
             D.hat = off_ptr<const u64>(b.addv(hat));
+            if (use_f64) {
+                std::vector<u64> inm(sz);
+                for (u32 i = 0; i < sz; ++i) inm[i] = h.primes[inp[i]];
+                std::vector<double> hf(2 * (size_t)sz * n_out);
+                for (u32 i = 0; i < sz; ++i)
+                    for (u32 j = 0; j < n_out; ++j) {
+                        hf[2 * ((size_t)i * n_out + j)] = (double)hat[(size_t)i * n_out + j];
+                        hf[2 * ((size_t)i * n_out + j) + 1] =
+                            (double)hat[(size_t)i * n_out + j] / (double)h.primes[outp[j]];
+                    }
+                D.in_cstf = off_ptr<const double>(b.addv(f64_pairs(cst, inm)));
+                D.hatf = off_ptr<const double>(b.addv(hf));
+            }
         }
         lo[l].modup_desc = b.add_descs(descs);
         lo[l].modup_first = b.addv(first);
@@ -238,6 +271,20 @@ static int build_device_tables(fhea_ctx *c) {
             md[0].out_prime = off_ptr<const u32>(b.addv(outp));
             md[0].out_pos = off_ptr<const u32>(b.addv(outp));
             md[0].hat = off_ptr<const u64>(b.addv(hat));
+            std::vector<u64> qm(l), pm(K);
+            for (u32 i = 0; i < l; ++i) qm[i] = h.primes[i];
+            for (u32 j = 0; j < K; ++j) pm[j] = h.primes[L + j];
+            if (use_f64) {
+                std::vector<double> hf(2 * (size_t)K * l);
+                for (u32 j = 0; j < K; ++j)
+                    for (u32 i = 0; i < l; ++i) {
+                        hf[2 * ((size_t)j * l + i)] = (double)hat[(size_t)j * l + i];
+                        hf[2 * ((size_t)j * l + i) + 1] = (double)hat[(size_t)j * l + i] / (double)qm[i];
+                    }
+                md[0].in_cstf = off_ptr<const double>(b.addv(f64_pairs(cst, pm)));
+                md[0].hatf = off_ptr<const double>(b.addv(hf));
+                lo[l].pinvf = b.addv(f64_pairs(pinv, qm));
+            }
             lo[l].moddown_desc = b.add_descs(md);
             lo[l].pinv = b.addv(pinv);
         }
@@ -257,6 +304,12 @@ static int build_device_tables(fhea_ctx *c) {
                 sub[i] = mulm(ql % qi, t, qi);
             }
             std::vector<u32> last = {l - 1, l - 1};
+            if (use_f64) {
+                std::vector<u64> qm(n + 1, 1);
+                for (u32 i = 0; i < n; ++i) qm[i] = h.primes[i];
+                lo[l].rs_qlinvf = b.addv(f64_pairs(qlinv, qm));
+                lo[l].rs_qlqlf = b.addv(f64_pairs(qlql, qm));
+            }
             lo[l].rs_qlinv = b.addv(qlinv);
             lo[l].rs_qlql = b.addv(qlql);
             lo[l].rs_sub = b.addv(sub);
@@ -305,6 +358,11 @@ static int build_device_tables(fhea_ctx *c) {
         lt.rs_qlql = (const u64 *)P(lo[l].rs_qlql);
         c->rs_sub[l] = (const u64 *)P(lo[l].rs_sub);
         c->rs_last_map[l] = (const u32 *)P(lo[l].rs_last);
+        if (use_f64) {
+            lt.pinvf = (const double *)P(lo[l].pinvf);
+            lt.rs_qlinvf = (const double *)P(lo[l].rs_qlinvf);
+            lt.rs_qlqlf = (const double *)P(lo[l].rs_qlqlf);
+        }
     }
     return FHEA_OK;
 }
@@ -338,24 +396,26 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     const u64 cz = (u64)np * (l + K) * N;
     // ModUp: INTT of every input limb, convert each digit, NTT the converted limbs
     launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
-    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, l + K, coef, l * N, compact,
-               (l + K) * N, cz, nb, s);
+    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), coef,
+               l * N, compact, (l + K) * N, cz, nb, s);
     launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
     // inner product with the evaluation key
     std::vector<KsIpArgs> ip(nb);
     for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
     auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
     if (!ip_d) return FHEA_ERR_DEVICE;
+    bool shared = true;
+    for (u32 z = 1; z < nb; ++z) shared &= keys[z] == keys[0];
     launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
-                    c->h.dnum, s);
+                    c->h.dnum, shared, s);
     // ModDown of both accumulators
     launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
-    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, pb, K * N, tt, 0, l * N, 2 * nb, s);
     launch_ntt(t, false, tt, tt, l * N, l * N, 2 * nb, l, t.ident, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
-                           nb, l, s);
+                           lt.pinvf, nb, l, s);
     return FHEA_OK;
 }
 
@@ -370,14 +430,14 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
     u64 *v = ws;
     u64 *tmp = v + nb * 2 * N;
     launch_ntt(t, true, a_base + (l - 1) * N, v, az, 2 * N, nb, 2, c->rs_last_map[l], s, as, N);
-    launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], 2 * nb, l, s);
+    launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s);
     launch_ntt(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, l - 1, t.ident, s);
     std::vector<RescaleOut> ro(2 * nb);
     for (u32 z = 0; z < nb; ++z)
         for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {a_base + z * az + k * as, outs[2 * z + k]};
     auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
     if (!ro_d) return FHEA_ERR_DEVICE;
-    launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, 2 * nb, l, s);
+    launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, lt.rs_qlinvf, 2 * nb, l, s);
     return FHEA_OK;
 }
 

@@ -27,6 +27,14 @@ __device__ __forceinline__ double mulmod_f64(double v, double w, double wq, doub
     return __builtin_fma(-k, q, hi) + lo;
 }
 
+// a*b mod q for two canonical residues a, b in [0, q): |result| <= 0.9q.
+__device__ __forceinline__ double mulmod_vv_f64(double a, double b, double q, double qinv) {
+    const double hi = a * b;
+    const double lo = __builtin_fma(a, b, -hi);
+    const double k = __builtin_rint(hi * qinv);
+    return __builtin_fma(-k, q, hi) + lo;
+}
+
 __device__ __forceinline__ double reduce_f64(double x, double q, double qinv) {
     return __builtin_fma(-__builtin_rint(x * qinv), q, x);
 }

@@ -29,6 +29,9 @@ struct ConvDesc {
     const u32 *out_prime = nullptr;  // [n_out] prime index of each output limb
     const u32 *out_pos = nullptr;    // [n_out] limb slot in the destination
     const u64 *hat = nullptr;        // [n_in*n_out] qhat_i mod p_j, row-major by input
+    // FP64 forms (only when every prime is below 2^50): (value, value / modulus)
+    const double *in_cstf = nullptr; // [2*n_in]
+    const double *hatf = nullptr;    // [2*n_in*n_out]
 };
 
 // Per-level tables (sizeQl = l).
@@ -44,6 +47,8 @@ struct LevelTables {
     const u64 *pinv = nullptr;             // [2*l] (P^-1 mod q_i, shoup)
     const u64 *rs_qlinv = nullptr;         // [2*(l-1)] (q_l^-1 mod q_i, shoup)
     const u64 *rs_qlql = nullptr;          // [2*(l-1)] (-q_l^-1 mod q_i, shoup)
+    // FP64 forms (value, value / q_i)
+    const double *pinvf = nullptr, *rs_qlinvf = nullptr, *rs_qlqlf = nullptr;
 };
 
 struct HostTables {

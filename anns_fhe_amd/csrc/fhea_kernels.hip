@@ -5,6 +5,7 @@
 // 64-bit word, consecutive threads on consecutive coefficients, so every global
 // access is coalesced (the automorphism gather aside).  Per-limb constants are
 // wave-uniform and come through the scalar cache.
+#include "fhea_f64.hpp"
 #include "fhea_kernels.hpp"
 
 namespace fhea {
@@ -48,10 +49,28 @@ __global__ __launch_bounds__(TPB) void k_tensor(const TensorArgs *__restrict__ a
     A.d1[i] = reduce128((u128)a0 * b1 + (u128)a1 * b0, m);
 }
 
+// FP64 form (all primes < 2^50): products by mulmod_vv_f64, |a*b mod q| <= 0.9q.
+__global__ __launch_bounds__(TPB) void k_tensor_f64(const TensorArgs *__restrict__ args,
+                                                    const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y;
+    const TensorArgs A = args[blockIdx.z];
+    const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const double q = (double)mp[limb].q, qinv = 1.0 / q;
+    const double a0 = u64_to_f64(A.a0[i]), a1 = u64_to_f64(A.a1[i]);
+    const double b0 = u64_to_f64(A.b0[i]), b1 = u64_to_f64(A.b1[i]);
+    A.d0[i] = f64_to_canon(mulmod_vv_f64(a0, b0, q, qinv), q);
+    A.d2[i] = f64_to_canon(mulmod_vv_f64(a1, b1, q, qinv), q);
+    const double x = mulmod_vv_f64(a0, b1, q, qinv) + mulmod_vv_f64(a1, b0, q, qinv);
+    A.d1[i] = f64_to_canon(reduce_f64(x, q, qinv), q);
+}
+
 void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u32 limbs,
                    hipStream_t s) {
     dim3 grid(t.N / TPB, limbs, nbatch);
-    hipLaunchKernelGGL(k_tensor, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
+    if (t.use_f64)
+        hipLaunchKernelGGL(k_tensor_f64, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
+    else
+        hipLaunchKernelGGL(k_tensor, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,13 +158,57 @@ __global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
     }
 }
 
+// FP64 form: y_i = x_i * qhat_i^-1 (|y_i| <= 1.5 q_i <= 3 p_j since every prime
+// lies in [2^49, 2^50)), each term y_i * (qhat_i mod p_j) by mulmod_f64
+// (|term| <= 1.5 p_j), the running sum centred every four terms (<= 6.5 p_j).
+static constexpr u32 FBC_F64_MAX_IN = 8;
+
+__global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ descs,
+                                                 const u32 *__restrict__ in_first,
+                                                 const ModParams *__restrict__ mp,
+                                                 const u64 *__restrict__ in, u64 in_zstride,
+                                                 u64 *__restrict__ out, u64 out_pstride,
+                                                 u64 out_zstride, u32 logN) {
+    const u32 p = blockIdx.y, z = blockIdx.z;
+    const ConvDesc D = descs[p];
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u64 *x = in + (size_t)z * in_zstride + ((size_t)in_first[p] << logN) + r;
+    u64 *o = out + (size_t)z * out_zstride + (size_t)p * out_pstride + r;
+    double y[FBC_F64_MAX_IN];
+#pragma unroll
+    for (u32 i = 0; i < FBC_F64_MAX_IN; ++i) {
+        if (i < D.n_in) {
+            const double q = (double)mp[D.in_prime[i]].q;
+            y[i] = mulmod_f64(u64_to_f64(x[(size_t)i << logN]), D.in_cstf[2 * i], D.in_cstf[2 * i + 1], q);
+        } else {
+            y[i] = 0.0;
+        }
+    }
+    for (u32 j = 0; j < D.n_out; ++j) {
+        const double pj = (double)mp[D.out_prime[j]].q, pinv = 1.0 / pj;
+        const double *h = D.hatf + 2 * j;
+        double acc = 0.0;
+#pragma unroll
+        for (u32 i = 0; i < FBC_F64_MAX_IN; ++i) {
+            if (i < D.n_in) {
+                acc += mulmod_f64(y[i], h[2 * (size_t)i * D.n_out], h[2 * (size_t)i * D.n_out + 1], pj);
+                if ((i & 3) == 3) acc = reduce_f64(acc, pj, pinv);
+            }
+        }
+        o[(size_t)j << logN] = f64_to_canon(reduce_f64(acc, pj, pinv), pj);
+    }
+}
+
 void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev, u32 ndesc,
-                u32 max_nout, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
+                u32 max_nin, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
                 u64 out_zstride, u32 nbatch, hipStream_t s) {
-    (void)max_nout;
     dim3 grid(t.N / TPB, ndesc, nbatch);
-    hipLaunchKernelGGL(k_fbc, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in, in_zstride,
-                       out, out_pstride, out_zstride, t.logN);
+    if (t.use_f64 && max_nin <= FBC_F64_MAX_IN)
+        hipLaunchKernelGGL(k_fbc_f64, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in,
+                           in_zstride, out, out_pstride, out_zstride, t.logN);
+    else
+        hipLaunchKernelGGL(k_fbc, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in, in_zstride,
+                           out, out_pstride, out_zstride, t.logN);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,9 +255,68 @@ __global__ __launch_bounds__(TPB) void k_ks_inner(const KsIpArgs *__restrict__ a
     o[(size_t)(l + K) << logN] = reduce128(s1, m);
 }
 
+// FP64 form with a stationary key: block (x, i, g) handles key slot i for the
+// `gs` consecutive items of group g, which share one key (a relinearisation
+// batch: one group; per-item rotation keys: gs = 1).  The key words are loaded
+// once per group.  Terms are mulmod_vv_f64 products (<= 0.9q each), at most
+// KS_F64_MAX_PARTS of them per sum (<= 7.2q), centred once.
+static constexpr u32 KS_F64_MAX_PARTS = 8;
+
+__global__ __launch_bounds__(TPB) void k_ks_inner_f64(const KsIpArgs *__restrict__ args,
+                                                      const u64 *__restrict__ compact, u64 cz,
+                                                      u64 *__restrict__ acc, u64 az,
+                                                      const ModParams *__restrict__ mp, u32 logN,
+                                                      u32 l, u32 K, u32 L, u32 alpha, u32 nparts,
+                                                      u32 key_limbs, u32 dnum, u32 gs, u32 nb) {
+    const u32 i = blockIdx.y;
+    const u32 z0 = blockIdx.z * gs, z1 = min(z0 + gs, nb);
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u32 pidx = (i < l) ? i : (L + i - l);
+    const u32 kl = (i < l) ? i : (key_limbs + i - l);
+    const double q = (double)mp[pidx].q, qinv = 1.0 / q;
+    const u64 kstride = (u64)(key_limbs + K) << logN;
+    const u64 *ka = args[z0].key + ((u64)kl << logN) + r;
+    const u64 *kb = ka + (u64)dnum * kstride;
+    double va[KS_F64_MAX_PARTS], vb[KS_F64_MAX_PARTS];
+#pragma unroll
+    for (u32 p = 0; p < KS_F64_MAX_PARTS; ++p) {
+        va[p] = p < nparts ? u64_to_f64(ka[(size_t)p * kstride]) : 0.0;
+        vb[p] = p < nparts ? u64_to_f64(kb[(size_t)p * kstride]) : 0.0;
+    }
+    for (u32 z = z0; z < z1; ++z) {
+        const u64 *c = args[z].c;
+        const u64 *cb = compact + (size_t)z * cz + r;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (u32 p = 0; p < KS_F64_MAX_PARTS; ++p) {
+            if (p < nparts) {
+                const u32 start = p * alpha;
+                const u32 sz = (p == nparts - 1) ? (l - start) : alpha;
+                u64 dw;
+                if (i >= start && i < start + sz) dw = c[((size_t)i << logN) + r];
+                else dw = cb[((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN];
+                const double d = u64_to_f64(dw);
+                s0 += mulmod_vv_f64(d, vb[p], q, qinv);
+                s1 += mulmod_vv_f64(d, va[p], q, qinv);
+            }
+        }
+        u64 *o = acc + (size_t)z * az + ((size_t)i << logN) + r;
+        o[0] = f64_to_canon(reduce_f64(s0, q, qinv), q);
+        o[(size_t)(l + K) << logN] = f64_to_canon(reduce_f64(s1, q, qinv), q);
+    }
+}
+
 void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
                      u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
-                     u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s) {
+                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s) {
+    if (t.use_f64 && nparts <= KS_F64_MAX_PARTS) {
+        const u32 gs = shared_key ? nbatch : 1;
+        dim3 grid(t.N / TPB, l + t.K, (nbatch + gs - 1) / gs);
+        hipLaunchKernelGGL(k_ks_inner_f64, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride,
+                           acc, acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum,
+                           gs, nbatch);
+        return;
+    }
     dim3 grid(t.N / TPB, l + t.K, nbatch);
     hipLaunchKernelGGL(k_ks_inner, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride, acc,
                        acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
@@ -223,12 +345,35 @@ __global__ __launch_bounds__(TPB) void k_moddown_combine(
     (poly ? A.out1 : A.out0)[off] = res;
 }
 
+__global__ __launch_bounds__(TPB) void k_moddown_combine_f64(
+    const CombineArgs *__restrict__ args, const u64 *__restrict__ acc, u64 az, u64 ap,
+    const u64 *__restrict__ tt, u64 tz, u64 tp, const double *__restrict__ pinvf,
+    const ModParams *__restrict__ mp, u32 logN, u32 l) {
+    const u32 y = blockIdx.y;
+    const u32 poly = y / l, i = y % l, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const CombineArgs A = args[z];
+    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const size_t off = ((size_t)i << logN) + r;
+    const double x = u64_to_f64(acc[(size_t)z * az + poly * ap + off]);
+    const double v = u64_to_f64(tt[(size_t)z * tz + poly * tp + off]);
+    double res = mulmod_f64(x - v, pinvf[2 * i], pinvf[2 * i + 1], q);  // |x-v| < q
+    const u64 *add = poly ? A.add1 : A.add0;
+    if (add) res += u64_to_f64(add[off]);
+    (poly ? A.out1 : A.out0)[off] = f64_to_canon(reduce_f64(res, q, qinv), q);
+}
+
 void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, const u64 *acc,
                             u64 acc_zstride, u64 acc_pstride, const u64 *tt, u64 tt_zstride,
-                            u64 tt_pstride, const u64 *pinv, u32 nbatch, u32 l, hipStream_t s) {
+                            u64 tt_pstride, const u64 *pinv, const double *pinvf, u32 nbatch, u32 l,
+                            hipStream_t s) {
     dim3 grid(t.N / TPB, 2 * l, nbatch);
-    hipLaunchKernelGGL(k_moddown_combine, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride,
-                       acc_pstride, tt, tt_zstride, tt_pstride, pinv, t.mp, t.logN, l);
+    if (t.use_f64)
+        hipLaunchKernelGGL(k_moddown_combine_f64, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride,
+                           acc_pstride, tt, tt_zstride, tt_pstride, pinvf, t.mp, t.logN, l);
+    else
+        hipLaunchKernelGGL(k_moddown_combine, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride,
+                           acc_pstride, tt, tt_zstride, tt_pstride, pinv, t.mp, t.logN, l);
 }
 
 // ---------------------------------------------------------------------------
@@ -253,11 +398,32 @@ __global__ __launch_bounds__(TPB) void k_rescale_prep(const u64 *__restrict__ v,
     tmp[(size_t)z * tz + ((size_t)i << logN) + r] = t;
 }
 
+// FP64 form: c = centred(v) with |c| <= q_l/2 < q_i, tmp = c * T_i by mulmod_f64.
+__global__ __launch_bounds__(TPB) void k_rescale_prep_f64(const u64 *__restrict__ v, u64 vz,
+                                                          u64 *__restrict__ tmp, u64 tz,
+                                                          const double *__restrict__ qlqlf,
+                                                          const ModParams *__restrict__ mp,
+                                                          u32 logN, u32 l) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u64 qlw = mp[l - 1].q;
+    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const u64 x = v[(size_t)z * vz + r];
+    const double c = (x > (qlw >> 1)) ? u64_to_f64(x) - (double)qlw : u64_to_f64(x);
+    const double t = mulmod_f64(c, qlqlf[2 * i], qlqlf[2 * i + 1], q);
+    tmp[(size_t)z * tz + ((size_t)i << logN) + r] = f64_to_canon(reduce_f64(t, q, qinv), q);
+}
+
 void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp, u64 tmp_stride,
-                         const u64 *qlql, const u64 *sub, u32 npolys, u32 l, hipStream_t s) {
+                         const u64 *qlql, const u64 *sub, const double *qlqlf, u32 npolys, u32 l,
+                         hipStream_t s) {
     dim3 grid(t.N / TPB, l - 1, npolys);
-    hipLaunchKernelGGL(k_rescale_prep, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride, qlql, sub,
-                       t.mp, t.logN, l);
+    if (t.use_f64)
+        hipLaunchKernelGGL(k_rescale_prep_f64, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride,
+                           qlqlf, t.mp, t.logN, l);
+    else
+        hipLaunchKernelGGL(k_rescale_prep, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride, qlql,
+                           sub, t.mp, t.logN, l);
 }
 
 __global__ __launch_bounds__(TPB) void k_rescale_combine(const RescaleOut *__restrict__ args,
@@ -274,11 +440,31 @@ __global__ __launch_bounds__(TPB) void k_rescale_combine(const RescaleOut *__res
     A.out[off] = add_mod(a, tmp[(size_t)z * tz + off], q);
 }
 
+__global__ __launch_bounds__(TPB) void k_rescale_combine_f64(const RescaleOut *__restrict__ args,
+                                                             const u64 *__restrict__ tmp, u64 tz,
+                                                             const double *__restrict__ qlinvf,
+                                                             const ModParams *__restrict__ mp,
+                                                             u32 logN) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const RescaleOut A = args[z];
+    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const size_t off = ((size_t)i << logN) + r;
+    const double a = mulmod_f64(u64_to_f64(A.a[off]), qlinvf[2 * i], qlinvf[2 * i + 1], q);
+    const double x = a + u64_to_f64(tmp[(size_t)z * tz + off]);
+    A.out[off] = f64_to_canon(reduce_f64(x, q, qinv), q);
+}
+
 void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
-                            u64 tmp_stride, const u64 *qlinv, u32 npolys, u32 l, hipStream_t s) {
+                            u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys, u32 l,
+                            hipStream_t s) {
     dim3 grid(t.N / TPB, l - 1, npolys);
-    hipLaunchKernelGGL(k_rescale_combine, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride, qlinv,
-                       t.mp, t.logN);
+    if (t.use_f64)
+        hipLaunchKernelGGL(k_rescale_combine_f64, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride,
+                           qlinvf, t.mp, t.logN);
+    else
+        hipLaunchKernelGGL(k_rescale_combine, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride, qlinv,
+                           t.mp, t.logN);
 }
 
 }  // namespace fhea

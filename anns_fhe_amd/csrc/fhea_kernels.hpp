@@ -56,8 +56,9 @@ void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, 
 // Basis conversion (key_switch.cpp:7-60) for a batch: block (x, p, z) converts
 // descriptor descs[p] reading in + z*in_zstride + (in_first_p)*N and writing
 // out + z*out_zstride + p*out_pstride + j*N.
+// max_nin: the largest n_in among the descriptors.
 void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev,
-                u32 ndesc, u32 max_nout, const u64 *in, u64 in_zstride, u64 *out,
+                u32 ndesc, u32 max_nin, const u64 *in, u64 in_zstride, u64 *out,
                 u64 out_pstride, u64 out_zstride, u32 nbatch, hipStream_t s);
 
 // Key-switch inner product (key_switch.cpp:352-381) for nbatch inputs at level l.
@@ -65,9 +66,10 @@ struct KsIpArgs {
     const u64 *c;      // input polynomial (eval), l limbs
     const u64 *key;    // evk base (a then b)
 };
+// shared_key: every item uses args_dev[0].key (read once per batch).
 void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
                      u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
-                     u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s);
+                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s);
 
 // ModDown combine (key_switch.cpp:253-273): out_i = (x_i - t_i) * P^-1 [+ add_i].
 struct CombineArgs {
@@ -76,17 +78,19 @@ struct CombineArgs {
 };
 void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, const u64 *acc,
                             u64 acc_zstride, u64 acc_pstride, const u64 *tt, u64 tt_zstride,
-                            u64 tt_pstride, const u64 *pinv, u32 nbatch, u32 l, hipStream_t s);
+                            u64 tt_pstride, const u64 *pinv, const double *pinvf, u32 nbatch,
+                            u32 l, hipStream_t s);
 
 // Rescale (rescale.cpp:3-72) helpers.  v: INTT of the last limb, [z][2][N].
 void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp,
-                         u64 tmp_stride, const u64 *qlql, const u64 *sub, u32 npolys, u32 l,
-                         hipStream_t s);
+                         u64 tmp_stride, const u64 *qlql, const u64 *sub, const double *qlqlf,
+                         u32 npolys, u32 l, hipStream_t s);
 struct RescaleOut {
     const u64 *a;  // input polynomial (l limbs)
     u64 *out;      // output polynomial (l-1 limbs)
 };
 void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
-                            u64 tmp_stride, const u64 *qlinv, u32 npolys, u32 l, hipStream_t s);
+                            u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys,
+                            u32 l, hipStream_t s);
 
 }  // namespace fhea

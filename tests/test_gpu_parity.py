@@ -354,3 +354,40 @@ def test_full_size_hmult_bit_exact(orc):
     assert o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
     assert np.array_equal(got0, pm[4 * W:4 * W + W1])
     assert np.array_equal(got1, pm[5 * W:5 * W + W1])
+
+
+def test_cpp_host_driver_c1_sequence(orc, tmp_path):
+    """The C++ FHEHostDriver over the C-ABI (anns_fhe_amd/c1_sequence) runs the
+    main_sim flow; every output word must equal the oracle's."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "anns_fhe_amd", "c1_sequence")
+    out = str(tmp_path / "c1.bin")
+    say("c++ host driver")
+    r = subprocess.run([exe, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint64)
+    N, L, K, dnum = (int(x) for x in raw[:4])
+    pos = 4
+    primes = [int(x) for x in raw[pos:pos + L + K]]
+    pos += L + K
+    W, W1 = L * N, (L - 1) * N
+    inp = raw[pos:pos + 3 * W]
+    pos += 3 * W
+    ksz = 2 * dnum * (L + K) * N
+    km = raw[pos:pos + 2 * ksz].copy()
+    pos += 2 * ksz
+    sub, mult, rot = raw[pos:pos + 2 * W], raw[pos + 2 * W:pos + 2 * W + 2 * W1], raw[pos + 2 * W + 2 * W1:]
+    assert primes == orc.gen_primes(2 * N, 50, L + K)
+    o = orc.OracleCtx(N, primes[:L], primes[L:], dnum)
+    pm = np.zeros(12 * W, np.uint64)
+    pm[:3 * W] = inp
+    assert o.eval_sub_plain(pm, 0, W, 2 * W, 3 * W, 4 * W, L) == 0
+    assert np.array_equal(sub, pm[3 * W:5 * W])
+    assert o.eval_mult_relin_rescale(pm, km, 0, L, 3 * W, 4 * W, 3 * W, 4 * W, 5 * W, 6 * W, L) == 0
+    assert np.array_equal(mult[:W1], pm[5 * W:5 * W + W1])
+    assert np.array_equal(mult[W1:], pm[6 * W:6 * W + W1])
+    g1 = pow(5, 1, 2 * N)
+    assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 5 * W, 6 * W, 7 * W, 8 * W, 1, L - 1) == 0
+    assert np.array_equal(rot[:W1], pm[7 * W:7 * W + W1])
+    assert np.array_equal(rot[W1:], pm[8 * W:8 * W + W1])

@@ -54,6 +54,8 @@ struct ModParams {
     u64 r_hi;    // floor(2^128 / q) high word
     u32 bq;      // bit length of q
     u32 pad;
+    double qd;   // q as a double (exact for q < 2^53)
+    double qinv; // 1/q rounded to double (FP64 kernels)
 };
 
 // x mod q for x < 2^(2*bq) (in particular x = a*b with a, b < q).
@@ -102,5 +104,7 @@ inline ModParams make_mod_params(u64 q) {
     m.r_lo = (u64)r;
     m.r_hi = (u64)(r >> 64);
     m.pad = 0;
+    m.qd = (double)q;
+    m.qinv = 1.0 / (double)q;
     return m;
 }

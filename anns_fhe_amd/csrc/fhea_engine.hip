@@ -336,6 +336,7 @@ static int build_device_tables(fhea_ctx *c) {
     c->dt.ninv = (const u64 *)P(o_ninv);
     c->dt.ident = (const u32 *)P(o_ident);
     c->dt.use_f64 = use_f64 && !c->force_int;
+    c->dt.f64_off = c->f64_off;
     if (use_f64) {
         c->dt.twf = (const double *)P(o_twf);
         c->dt.itwf = (const double *)P(o_itwf);
@@ -411,11 +412,22 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // ModDown of both accumulators
     launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
     launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, pb, K * N, tt, 0, l * N, 2 * nb, s);
-    launch_ntt(t, false, tt, tt, l * N, l * N, 2 * nb, l, t.ident, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
-    launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
-                           lt.pinvf, nb, l, s);
+    if (t.use_f64 && !(t.f64_off & 17)) {
+        // NTT of the converted P part with (acc - t) * P^-1 [+ add] fused into its last pass
+        NttFuse fz;
+        fz.epi = FUSE_MODDOWN_OUT;
+        fz.cst_out = lt.pinvf;
+        fz.cb = cb_d;
+        fz.acc = acc;
+        fz.acc_stride = (l + K) * N;
+        launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, fz, s);
+    } else {
+        launch_ntt(t, false, tt, tt, l * N, l * N, 2 * nb, l, t.ident, s);
+        launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
+                               lt.pinvf, nb, l, s);
+    }
     return FHEA_OK;
 }
 
@@ -430,13 +442,27 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
     u64 *v = ws;
     u64 *tmp = v + nb * 2 * N;
     launch_ntt(t, true, a_base + (l - 1) * N, v, az, 2 * N, nb, 2, c->rs_last_map[l], s, as, N);
-    launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s);
-    launch_ntt(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, l - 1, t.ident, s);
     std::vector<RescaleOut> ro(2 * nb);
     for (u32 z = 0; z < nb; ++z)
         for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {a_base + z * az + k * as, outs[2 * z + k]};
     auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
     if (!ro_d) return FHEA_ERR_DEVICE;
+    if (t.use_f64 && !(t.f64_off & 17)) {
+        // centred lift * T_i fused into the NTT's first pass, a*q_l^-1 + x into its last
+        NttFuse fz;
+        fz.pro = FUSE_RESCALE_IN;
+        fz.v = v;
+        fz.v_stride = N;
+        fz.ql = c->h.primes[l - 1];
+        fz.cst_in = lt.rs_qlqlf;
+        fz.epi = FUSE_RESCALE_OUT;
+        fz.cst_out = lt.rs_qlinvf;
+        fz.ro = ro_d;
+        launch_ntt_f64(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, N, N, 2 * nb, l - 1, t.ident, fz, s);
+        return FHEA_OK;
+    }
+    launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s);
+    launch_ntt(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, l - 1, t.ident, s);
     launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, lt.rs_qlinvf, 2 * nb, l, s);
     return FHEA_OK;
 }
@@ -602,6 +628,8 @@ int fhea_ctx_create(const fhea_params *p, fhea_ctx **out) {
     {
         const char *fi = getenv("FHEA_FORCE_INT");
         c->force_int = fi && atoi(fi) != 0;
+        const char *fo = getenv("FHEA_F64_OFF");
+        c->f64_off = fo ? (u32)strtoul(fo, nullptr, 0) : 0;
     }
     c->device = (int)p->device;
     int st = FHEA_OK;

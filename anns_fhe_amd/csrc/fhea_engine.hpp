@@ -45,7 +45,8 @@ struct fhea_ctx {
     std::map<int32_t, fhea::RotKey> rot;
 
     u32 max_chunk = 16;
-    bool force_int = false;  // FHEA_FORCE_INT=1: integer transforms even for < 2^50 primes
+    bool force_int = false;  // FHEA_FORCE_INT=1: integer kernels even for < 2^50 primes
+    u32 f64_off = 0;         // FHEA_F64_OFF: per-component switch (DevTables::f64_off)
 };
 
 namespace fhea {

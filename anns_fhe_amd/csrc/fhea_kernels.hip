@@ -55,7 +55,7 @@ __global__ __launch_bounds__(TPB) void k_tensor_f64(const TensorArgs *__restrict
     const u32 limb = blockIdx.y;
     const TensorArgs A = args[blockIdx.z];
     const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
-    const double q = (double)mp[limb].q, qinv = 1.0 / q;
+    const double q = mp[limb].qd, qinv = mp[limb].qinv;
     const double a0 = u64_to_f64(A.a0[i]), a1 = u64_to_f64(A.a1[i]);
     const double b0 = u64_to_f64(A.b0[i]), b1 = u64_to_f64(A.b1[i]);
     A.d0[i] = f64_to_canon(mulmod_vv_f64(a0, b0, q, qinv), q);
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(TPB) void k_tensor_f64(const TensorArgs *__restrict
 void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u32 limbs,
                    hipStream_t s) {
     dim3 grid(t.N / TPB, limbs, nbatch);
-    if (t.use_f64)
+    if (t.use_f64 && !(t.f64_off & 8))
         hipLaunchKernelGGL(k_tensor_f64, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
     else
         hipLaunchKernelGGL(k_tensor, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
@@ -158,9 +158,11 @@ __global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
     }
 }
 
-// FP64 form: y_i = x_i * qhat_i^-1 (|y_i| <= 1.5 q_i <= 3 p_j since every prime
-// lies in [2^49, 2^50)), each term y_i * (qhat_i mod p_j) by mulmod_f64
-// (|term| <= 1.5 p_j), the running sum centred every four terms (<= 6.5 p_j).
+// FP64 form: y_i = [x_i * qhat_i^-1]_{q_i} in [0, q_i) (x_i, qhat_i^-1 < q_i, so
+// the centred product lies in (-q_i, q_i) and one conditional add makes it
+// canonical; y_i < q_i < 4 p_j because max(q)/min(q) < 4), each term
+// y_i * (qhat_i mod p_j) by mulmod_f64 (|term| <= 1.5 p_j), the running sum
+// centred every four terms (<= 6.5 p_j).
 static constexpr u32 FBC_F64_MAX_IN = 8;
 
 __global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ descs,
@@ -178,14 +180,17 @@ __global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ de
 #pragma unroll
     for (u32 i = 0; i < FBC_F64_MAX_IN; ++i) {
         if (i < D.n_in) {
-            const double q = (double)mp[D.in_prime[i]].q;
-            y[i] = mulmod_f64(u64_to_f64(x[(size_t)i << logN]), D.in_cstf[2 * i], D.in_cstf[2 * i + 1], q);
+            // the conversion needs the integer [x_i qhat_i^-1]_{q_i} in [0, q_i), not just
+            // its residue: the centred product (|.| < q_i) is moved into [0, q_i)
+            const double q = mp[D.in_prime[i]].qd;
+            const double t = mulmod_f64(u64_to_f64(x[(size_t)i << logN]), D.in_cstf[2 * i], D.in_cstf[2 * i + 1], q);
+            y[i] = t < 0.0 ? t + q : t;
         } else {
             y[i] = 0.0;
         }
     }
     for (u32 j = 0; j < D.n_out; ++j) {
-        const double pj = (double)mp[D.out_prime[j]].q, pinv = 1.0 / pj;
+        const double pj = mp[D.out_prime[j]].qd, pinv = mp[D.out_prime[j]].qinv;
         const double *h = D.hatf + 2 * j;
         double acc = 0.0;
 #pragma unroll
@@ -203,7 +208,7 @@ void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_fir
                 u32 max_nin, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
                 u64 out_zstride, u32 nbatch, hipStream_t s) {
     dim3 grid(t.N / TPB, ndesc, nbatch);
-    if (t.use_f64 && max_nin <= FBC_F64_MAX_IN)
+    if (t.use_f64 && !(t.f64_off & 2) && max_nin <= FBC_F64_MAX_IN)
         hipLaunchKernelGGL(k_fbc_f64, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in,
                            in_zstride, out, out_pstride, out_zstride, t.logN);
     else
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_f64(const KsIpArgs *__restrict
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const u32 pidx = (i < l) ? i : (L + i - l);
     const u32 kl = (i < l) ? i : (key_limbs + i - l);
-    const double q = (double)mp[pidx].q, qinv = 1.0 / q;
+    const double q = mp[pidx].qd, qinv = mp[pidx].qinv;
     const u64 kstride = (u64)(key_limbs + K) << logN;
     const u64 *ka = args[z0].key + ((u64)kl << logN) + r;
     const u64 *kb = ka + (u64)dnum * kstride;
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_f64(const KsIpArgs *__restrict
 void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
                      u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
                      u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s) {
-    if (t.use_f64 && nparts <= KS_F64_MAX_PARTS) {
+    if (t.use_f64 && !(t.f64_off & 4) && nparts <= KS_F64_MAX_PARTS) {
         const u32 gs = shared_key ? nbatch : 1;
         dim3 grid(t.N / TPB, l + t.K, (nbatch + gs - 1) / gs);
         hipLaunchKernelGGL(k_ks_inner_f64, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride,
@@ -353,7 +358,7 @@ __global__ __launch_bounds__(TPB) void k_moddown_combine_f64(
     const u32 poly = y / l, i = y % l, z = blockIdx.z;
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const CombineArgs A = args[z];
-    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const double q = mp[i].qd, qinv = mp[i].qinv;
     const size_t off = ((size_t)i << logN) + r;
     const double x = u64_to_f64(acc[(size_t)z * az + poly * ap + off]);
     const double v = u64_to_f64(tt[(size_t)z * tz + poly * tp + off]);
@@ -368,7 +373,7 @@ void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, con
                             u64 tt_pstride, const u64 *pinv, const double *pinvf, u32 nbatch, u32 l,
                             hipStream_t s) {
     dim3 grid(t.N / TPB, 2 * l, nbatch);
-    if (t.use_f64)
+    if (t.use_f64 && !(t.f64_off & 32))
         hipLaunchKernelGGL(k_moddown_combine_f64, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride,
                            acc_pstride, tt, tt_zstride, tt_pstride, pinvf, t.mp, t.logN, l);
     else
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(TPB) void k_rescale_prep_f64(const u64 *__restrict_
     const u32 i = blockIdx.y, z = blockIdx.z;
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const u64 qlw = mp[l - 1].q;
-    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const double q = mp[i].qd, qinv = mp[i].qinv;
     const u64 x = v[(size_t)z * vz + r];
     const double c = (x > (qlw >> 1)) ? u64_to_f64(x) - (double)qlw : u64_to_f64(x);
     const double t = mulmod_f64(c, qlqlf[2 * i], qlqlf[2 * i + 1], q);
@@ -418,7 +423,7 @@ void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tm
                          const u64 *qlql, const u64 *sub, const double *qlqlf, u32 npolys, u32 l,
                          hipStream_t s) {
     dim3 grid(t.N / TPB, l - 1, npolys);
-    if (t.use_f64)
+    if (t.use_f64 && !(t.f64_off & 32))
         hipLaunchKernelGGL(k_rescale_prep_f64, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride,
                            qlqlf, t.mp, t.logN, l);
     else
@@ -448,7 +453,7 @@ __global__ __launch_bounds__(TPB) void k_rescale_combine_f64(const RescaleOut *_
     const u32 i = blockIdx.y, z = blockIdx.z;
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const RescaleOut A = args[z];
-    const double q = (double)mp[i].q, qinv = 1.0 / q;
+    const double q = mp[i].qd, qinv = mp[i].qinv;
     const size_t off = ((size_t)i << logN) + r;
     const double a = mulmod_f64(u64_to_f64(A.a[off]), qlinvf[2 * i], qlinvf[2 * i + 1], q);
     const double x = a + u64_to_f64(tmp[(size_t)z * tz + off]);
@@ -459,7 +464,7 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
                             u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys, u32 l,
                             hipStream_t s) {
     dim3 grid(t.N / TPB, l - 1, npolys);
-    if (t.use_f64)
+    if (t.use_f64 && !(t.f64_off & 32))
         hipLaunchKernelGGL(k_rescale_combine_f64, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride,
                            qlinvf, t.mp, t.logN);
     else

@@ -14,11 +14,33 @@ struct DevTables {
     const u64 *itw = nullptr;       // [T][N][2]  (psi^-br(j), shoup)
     const u64 *ninv = nullptr;      // [T][2]
     const u32 *ident = nullptr;     // [T] identity prime map 0..T-1
-    // FP64 transform path, used when every prime is below 2^50
+    // FP64 path, used when every prime is below 2^50; f64_off (FHEA_F64_OFF
+    // bitmask) turns single components back to the integer kernels:
+    // 1 transforms, 2 basis conversion, 4 inner product, 8 tensor,
+    // 16 fused transform ends, 32 rescale / ModDown combines
     bool use_f64 = false;
+    u32 f64_off = 0;
     const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
     const double *itwf = nullptr;   // [T][N][2]
     const double *ninvf = nullptr;  // [T][2]
+};
+
+struct RescaleOut;
+struct CombineArgs;
+enum : int { FUSE_NONE = 0, FUSE_RESCALE_IN = 1, FUSE_RESCALE_OUT = 2, FUSE_MODDOWN_OUT = 3 };
+// Fused prologue (first forward pass) / epilogue (last forward pass) of the
+// FP64 transform; poly = batch index of the NTT launch.
+struct NttFuse {
+    int pro = FUSE_NONE, epi = FUSE_NONE;
+    const u64 *v = nullptr;            // RESCALE_IN: INTT'd last limb, poly at v + poly*v_stride
+    u64 v_stride = 0;
+    u64 ql = 0;                        // RESCALE_IN: the dropped prime
+    const double *cst_in = nullptr;    // RESCALE_IN: (T_i, T_i/q_i) per limb slot
+    const double *cst_out = nullptr;   // RESCALE_OUT: q_l^-1 / MODDOWN_OUT: P^-1, per slot
+    const RescaleOut *ro = nullptr;    // RESCALE_OUT: per poly (a, out)
+    const CombineArgs *cb = nullptr;   // MODDOWN_OUT: per item (poly>>1), half = poly&1
+    const u64 *acc = nullptr;          // MODDOWN_OUT: x operand, poly at acc + poly*acc_stride
+    u64 acc_stride = 0;
 };
 
 // Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
@@ -28,6 +50,11 @@ struct DevTables {
 void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
                 u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
                 u64 src_slot = 0, u64 dst_slot = 0);
+
+// FP64 transform (t.use_f64) with optional fused ends (forward only).
+void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
+                    u64 dst_stride, u64 src_slot, u64 dst_slot, u32 npolys, u32 limbs,
+                    const u32 *prime_map, const NttFuse &fz, hipStream_t s);
 
 // Elementwise ops on `limbs` limbs, limb i mod prime i (poly_arith.cpp:3-42).
 // op: 0 add, 1 sub, 2 mul.

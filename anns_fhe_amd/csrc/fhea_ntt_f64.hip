@@ -1,0 +1,333 @@
+// fhea_ntt_f64.hip -- negacyclic NTT / INTT on FP64 FMA (every prime < 2^50).
+//
+// Same transform as ntt_forward / intt_inverse (hardware_fpga_model/hls/src/
+// ntt.cpp:4-80), bit for bit: CT with W = tw[m+i] (natural in, bit-reversed
+// out), GS with W = itw[m+i] and the N^-1 scaling.  Arithmetic is exact
+// (fhea_f64.hpp); values are signed residues between the stages and canonical
+// [0, q) words at the transform's ends.
+//
+// Decomposition.  N <= 4096: one pass, the whole limb in one workgroup.  Larger
+// N: two passes over a limb viewed as [2^k1 rows][2^k2 columns]; the forward
+// transform runs its first k1 stages down the columns ("COLS" pass, 16..64
+// adjacent columns per workgroup so that every global access is a run of
+// consecutive words) and the last k2 stages along rows ("ROWS" pass, several
+// rows per workgroup); the inverse runs ROWS then COLS.  A workgroup owns a
+// 4096-word tile.  Each of its 256 threads keeps 16 tile elements in
+// registers — the elements whose row index differs only in a 4-bit window —
+// and runs up to four radix-2 stages on them ("round"); rounds exchange
+// through LDS.  All round geometry is compile-time (template LOGR = log2 of
+// the sub-transform length).  The twiddles of a round (15 (w, w/q) pairs per
+// thread) are loaded before its arithmetic.  Tile loads and stores go straight
+// between global memory and registers when the thread's window makes them runs
+// of >= 8 consecutive words (every COLS round, ROWS rounds whose window starts
+// at bit >= 3); otherwise they are staged through LDS.
+//
+// Value bounds (q < 2^50, see fhea_f64.hpp):
+//   forward: a round starts at |x| <= q, grows <= 1.5q per stage to <= 7q < 8q,
+//            then every value is centred (|x| <= q/2 + 1);
+//   inverse: the sum branch is centred every stage, products stay <= 1.5q.
+//
+// Fused ends (used by the rescale and ModDown pipelines to save HBM passes):
+//   FUSE_RESCALE_IN   forward COLS prologue: x = centred(v[j]) * T_i (rescale.cpp:43-56)
+//   FUSE_RESCALE_OUT  forward ROWS epilogue: out = a*q_l^-1 + x    (rescale.cpp:62-69)
+//   FUSE_MODDOWN_OUT  forward ROWS epilogue: out = (acc - x)*P^-1 [+ add]
+//                     (key_switch.cpp:253-273 with the composite's poly_add)
+#include "fhea_f64.hpp"
+#include "fhea_kernels.hpp"
+
+namespace fhea {
+
+static constexpr u32 F_TPB = 256;
+static constexpr u32 F_LOG_TILE = 12;
+static constexpr u32 F_TILE = 1u << F_LOG_TILE;
+
+__device__ __forceinline__ u32 lds_pad(u32 e) { return e + (e >> 4); }
+
+template <bool INV, int LOGR, int RD>
+struct Round {
+    static constexpr int NR = (LOGR + 3) / 4;
+    static constexpr int NS = (RD + 1 < NR) ? 4 : (LOGR - 4 * RD);  // stages in this round
+    static constexpr int TOP = LOGR - 1 - 4 * RD;
+    // window [P, P+4) of row-index bits; BIT0 = bit of the round's first stage
+    static constexpr int P = INV ? ((4 * RD + 4 <= LOGR) ? 4 * RD : LOGR - 4) : (TOP >= 3 ? TOP - 3 : 0);
+    static constexpr int BIT0 = INV ? 4 * RD : TOP;
+};
+
+struct TileGeom {
+    u32 logN, logS, logC, logT, b0, c0;
+    const double *tw;
+    double q, qinv;
+    // global word index of tile element e
+    __device__ __forceinline__ u32 gidx(u32 e, u32 logR) const {
+        const u32 c = e & ((1u << logC) - 1), r = (e >> logC) & ((1u << logR) - 1), ob = e >> (logC + logR);
+        return ((b0 + ob) << (logR + logS)) + (r << logS) + c0 + c;
+    }
+};
+
+template <bool INV, int LOGR, int RD>
+__device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, u32 j0) {
+    using R = Round<INV, LOGR, RD>;
+    // preload this round's twiddles: stage st uses 2^(3-wb) distinct pairs
+    double w[4][8], wq[4][8];
+#pragma unroll
+    for (int st = 0; st < R::NS; ++st) {
+        const int b = INV ? R::BIT0 + st : R::BIT0 - st;
+        const int wb = b - R::P;
+        const u32 k = (u32)b + g.logS + 1;
+        const u32 wbase = ((1u << g.logN) >> k) | (j0 >> k);
+#pragma unroll
+        for (int t = 0; t < (8 >> wb); ++t) {
+            const double2 pr = reinterpret_cast<const double2 *>(g.tw)[wbase | (u32)t];
+            w[st][t] = pr.x;
+            wq[st][t] = pr.y;
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < R::NS; ++st) {
+        const int b = INV ? R::BIT0 + st : R::BIT0 - st;
+        const int wb = b - R::P;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            if (u & (1 << wb)) continue;
+            const int u2 = u | (1 << wb);
+            const double ww = w[st][u >> (wb + 1)], wwq = wq[st][u >> (wb + 1)];
+            if (!INV) {
+                const double rr = mulmod_f64(x[u2], ww, wwq, g.q);
+                const double a = x[u];
+                x[u] = a + rr;
+                x[u2] = a - rr;
+            } else {
+                const double a = x[u], bb = x[u2];
+                x[u] = reduce_f64(a + bb, g.q, g.qinv);
+                x[u2] = mulmod_f64(a - bb, ww, wwq, g.q);
+            }
+        }
+    }
+    if (!INV) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = reduce_f64(x[u], g.q, g.qinv);
+    }
+}
+
+// loads word j of the pass input as a double (with the fused prologue)
+__device__ __forceinline__ double load_in(const char *sbase, u32 j, int in_canon, const NttFuse &fz,
+                                          u32 poly, u32 slot, double q) {
+    if (fz.pro == FUSE_RESCALE_IN) {
+        const u64 v = fz.v[(size_t)poly * fz.v_stride + j];
+        const double c = (v > (fz.ql >> 1)) ? u64_to_f64(v) - (double)fz.ql : u64_to_f64(v);
+        return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q);
+    }
+    if (in_canon) return u64_to_f64(((const u64 *)sbase)[j]);
+    return ((const double *)sbase)[j];
+}
+
+__device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_canon, int scale,
+                                          double ni, double niq, const NttFuse &fz, u32 poly, u32 slot,
+                                          u32 logN, double q, double qinv, bool inv) {
+    if (scale) v = mulmod_f64(v, ni, niq, q);
+    if (inv) v = reduce_f64(v, q, qinv);
+    const size_t off = ((size_t)slot << logN) + j;
+    if (fz.epi == FUSE_RESCALE_OUT) {
+        const RescaleOut A = fz.ro[poly];
+        const double a = mulmod_f64(u64_to_f64(A.a[off]), fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
+        A.out[off] = f64_to_canon(reduce_f64(a + v, q, qinv), q);
+        return;
+    }
+    if (fz.epi == FUSE_MODDOWN_OUT) {
+        const CombineArgs A = fz.cb[poly >> 1];
+        const u32 half = poly & 1;
+        const double x = u64_to_f64(fz.acc[(size_t)poly * fz.acc_stride + off]);
+        // x - v: |x| < q, |v| <= q/2 + 1  ->  |x - v| < 2q
+        double r = mulmod_f64(x - v, fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
+        const u64 *add = half ? A.add1 : A.add0;
+        if (add) r += u64_to_f64(add[off]);
+        (half ? A.out1 : A.out0)[off] = f64_to_canon(reduce_f64(r, q, qinv), q);
+        return;
+    }
+    if (out_canon) ((u64 *)dbase)[j] = f64_to_canon(v, q);
+    else ((double *)dbase)[j] = v;
+}
+
+template <bool INV, int LOGR, bool COLS>
+__global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u64 src_stride,
+                                                  u64 dst_stride, u64 src_slot, u64 dst_slot,
+                                                  u32 limbs, const u32 *__restrict__ prime_map,
+                                                  const double *__restrict__ twf,
+                                                  const double *__restrict__ ninvf,
+                                                  const ModParams *__restrict__ mp, u32 logN,
+                                                  u32 logT, int in_canon, int out_canon, int scale,
+                                                  NttFuse fz) {
+    __shared__ double lds[F_TILE + F_TILE / 16];
+    constexpr int NR = (LOGR + 3) / 4;
+    const u32 slot = blockIdx.y % limbs;
+    const u32 poly = blockIdx.y / limbs;
+    const u32 pidx = prime_map[slot];
+    if (pidx == 0xFFFFFFFFu) return;
+    TileGeom g;
+    g.logN = logN;
+    g.logT = logT;
+    g.logC = COLS ? (logT - LOGR) : 0;
+    g.logS = COLS ? (logN - LOGR) : 0;
+    const u32 logNB = logT - LOGR - g.logC;
+    const u32 ncg = 1u << (g.logS - g.logC);
+    g.c0 = (blockIdx.x & (ncg - 1)) << g.logC;
+    g.b0 = (blockIdx.x >> (g.logS - g.logC)) << logNB;
+    g.q = mp[pidx].qd;
+    g.qinv = mp[pidx].qinv;
+    g.tw = twf + ((size_t)pidx << (logN + 1));
+    const char *sbase = (const char *)src + 8 * ((size_t)poly * src_stride + (size_t)slot * src_slot);
+    char *dbase = (char *)dst + 8 * ((size_t)poly * dst_stride + (size_t)slot * dst_slot);
+    const u32 T = 1u << logT;
+    const u32 tid = threadIdx.x;
+    const bool active = tid < (T >> 4);
+    double x[16];
+
+    // ---- round 0 input
+    {
+        using R0 = Round<INV, LOGR, 0>;
+        const u32 P = g.logC + R0::P;
+        const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
+        const bool direct = COLS || R0::P >= 3;
+        if (direct) {
+            if (active) {
+                const u32 jb = g.gidx(eb, LOGR);
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    x[u] = load_in(sbase, jb + ((u32)u << (R0::P + g.logS)), in_canon, fz, poly, slot, g.q);
+            }
+        } else {
+            for (u32 e = tid; e < T; e += F_TPB)
+                lds[lds_pad(e)] = load_in(sbase, g.gidx(e, LOGR), in_canon, fz, poly, slot, g.q);
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb | ((u32)u << P))];
+            }
+        }
+        if (active) round_arith<INV, LOGR, 0>(x, g, g.gidx(eb, LOGR));
+    }
+    // ---- later rounds (exchange through LDS)
+    if constexpr (NR > 1) {
+        {
+            using Rp = Round<INV, LOGR, 0>;
+            using R1 = Round<INV, LOGR, 1>;
+            const u32 Pp = g.logC + Rp::P, P1 = g.logC + R1::P;
+            const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+            const u32 eb1 = (tid & ((1u << P1) - 1)) | ((tid >> P1) << (P1 + 4));
+            __syncthreads();  // every round-0 LDS read is done (staged path)
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+            }
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1 | ((u32)u << P1))];
+                round_arith<INV, LOGR, 1>(x, g, g.gidx(eb1, LOGR));
+            }
+        }
+        if constexpr (NR > 2) {
+            using Rp = Round<INV, LOGR, 1>;
+            using R2 = Round<INV, LOGR, 2>;
+            const u32 Pp = g.logC + Rp::P, P2 = g.logC + R2::P;
+            const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+            const u32 eb2 = (tid & ((1u << P2) - 1)) | ((tid >> P2) << (P2 + 4));
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+            }
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2 | ((u32)u << P2))];
+                round_arith<INV, LOGR, 2>(x, g, g.gidx(eb2, LOGR));
+            }
+        }
+    }
+    // ---- output of the last round
+    {
+        using RL = Round<INV, LOGR, NR - 1>;
+        const u32 P = g.logC + RL::P;
+        const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
+        const double ni = scale ? ninvf[2 * pidx] : 0.0, niq = scale ? ninvf[2 * pidx + 1] : 0.0;
+        const bool direct = (COLS || RL::P >= 3) && fz.epi == FUSE_NONE;
+        if (direct) {
+            if (active) {
+                const u32 jb = g.gidx(eb, LOGR);
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    store_out(dbase, jb + ((u32)u << (RL::P + g.logS)), x[u], out_canon, scale, ni, niq, fz,
+                              poly, slot, logN, g.q, g.qinv, INV);
+            }
+        } else {
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
+            }
+            __syncthreads();
+            for (u32 e = tid; e < T; e += F_TPB)
+                store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, poly, slot,
+                          logN, g.q, g.qinv, INV);
+        }
+    }
+}
+
+template <bool INV, bool COLS>
+static void launch_pass(int logR, dim3 grid, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+                        u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
+                        const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
+#define FHEA_NTT_CASE(LR)                                                                            \
+    case LR:                                                                                         \
+        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), grid, dim3(F_TPB), 0, s, src, dst, ss, ds, sl, dl, \
+                           limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz);                   \
+        break;
+    switch (logR) {
+        FHEA_NTT_CASE(6)
+        FHEA_NTT_CASE(7)
+        FHEA_NTT_CASE(8)
+        FHEA_NTT_CASE(9)
+        FHEA_NTT_CASE(10)
+        FHEA_NTT_CASE(11)
+        FHEA_NTT_CASE(12)
+    default:
+        break;
+    }
+#undef FHEA_NTT_CASE
+}
+
+void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 ss, u64 ds, u64 sl,
+                    u64 dl, u32 npolys, u32 limbs, const u32 *pm, const NttFuse &fz, hipStream_t s) {
+    const u32 lg = t.logN;
+    const double *tw = inverse ? t.itwf : t.twf;
+    NttFuse none;
+    if (lg <= F_LOG_TILE) {
+        dim3 grid(1, npolys * limbs);
+        if (inverse)
+            launch_pass<true, false>(lg, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
+                                     1, 1, 1, none);
+        else
+            launch_pass<false, false>(lg, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+                                      lg, 1, 1, 0, fz);
+        return;
+    }
+    const u32 k1 = lg / 2, k2 = lg - k1;
+    dim3 grid(1u << (lg - F_LOG_TILE), npolys * limbs);
+    if (!inverse) {
+        NttFuse pro = fz, epi = fz;
+        pro.epi = FUSE_NONE;
+        epi.pro = FUSE_NONE;
+        launch_pass<false, true>(k1, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+                                 F_LOG_TILE, 1, 0, 0, pro);
+        launch_pass<false, false>(k2, grid, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+                                  F_LOG_TILE, 0, 1, 0, epi);
+    } else {
+        launch_pass<true, false>(k2, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+                                 F_LOG_TILE, 1, 0, 0, none);
+        launch_pass<true, true>(k1, grid, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+                                F_LOG_TILE, 0, 1, 1, none);
+    }
+}
+
+}  // namespace fhea

@@ -145,6 +145,14 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},
     }
+    # calibration copy of a known size (64 MiB, D2D blit) for rocprof FETCH_SIZE / WRITE_SIZE
+    # scaling; it runs after every measurement and overwrites the first input only
+    calib_words = 8 << 20
+    if B * per_item >= 2 * calib_words:
+        lib = ctx.lib
+        assert lib.fhea_copy_poly(ctx.h, calib_words, 0, calib_words) == 0
+        ctx.sync()
+        result["calibration_copy_bytes"] = calib_words * 8
     if args.cpu_sample > 0:
         log("cpu baseline")
         key = ctx.read_key(0, key_words)

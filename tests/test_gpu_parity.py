@@ -48,7 +48,8 @@ def test_tables_match_oracle(orc):
     assert np.array_equal(ctx.table(3), o.itw)
 
 
-@pytest.mark.parametrize("N,L", [(1024, 3), (4096, 2), (8192, 4), (16384, 3), (65536, 3)])
+@pytest.mark.parametrize("N,L", [(1024, 3), (2048, 2), (4096, 2), (8192, 4), (16384, 3), (32768, 2),
+                                 (65536, 3), (131072, 2)])
 def test_ntt_intt_parity(orc, N, L):
     say("ntt", N, L)
     ctx = get_ctx(N, L, 1, 1)

@@ -100,7 +100,7 @@ static T *off_ptr(size_t off) {
 static u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
 
 struct LevelOffsets {
-    size_t modup_desc, modup_first, modup_ntt_prime, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
+    size_t modup_desc, modup_first, modup_ntt_prime, modup_slots, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
         rs_last, pinvf, rs_qlinvf, rs_qlqlf;
 };
 
@@ -236,6 +236,13 @@ static int build_device_tables(fhea_ctx *c) {
         lo[l].modup_desc = b.add_descs(descs);
         lo[l].modup_first = b.addv(first);
         lo[l].modup_ntt_prime = b.addv(nttp);
+        {
+            std::vector<u32> sl;
+            for (u32 k = 0; k < nttp.size(); ++k)
+                if (nttp[k] != 0xFFFFFFFFu) sl.push_back(k);
+            lt.n_modup_slots = (u32)sl.size();
+            lo[l].modup_slots = b.addv(sl);
+        }
         // ModDown: P -> Q_l  (PHatInvModp, PHatModq row-major by P index)
         {
             std::vector<u32> inp(K), outp(l);
@@ -336,6 +343,11 @@ static int build_device_tables(fhea_ctx *c) {
     c->dt.ninv = (const u64 *)P(o_ninv);
     c->dt.ident = (const u32 *)P(o_ident);
     c->dt.use_f64 = use_f64 && !c->force_int;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+            c->dt.max_wg = 4u * (u32)cus;
+    }
     c->dt.f64_off = c->f64_off;
     if (use_f64) {
         c->dt.twf = (const double *)P(o_twf);
@@ -353,6 +365,7 @@ static int build_device_tables(fhea_ctx *c) {
         c->modup_desc_dev[l] = (const ConvDesc *)P(lo[l].modup_desc);
         c->modup_first_dev[l] = (const u32 *)P(lo[l].modup_first);
         lt.modup_ntt_prime = (const u32 *)P(lo[l].modup_ntt_prime);
+        lt.modup_slots = (const u32 *)P(lo[l].modup_slots);
         c->moddown_desc_dev[l] = (const ConvDesc *)P(lo[l].moddown_desc);
         lt.pinv = (const u64 *)P(lo[l].pinv);
         lt.rs_qlinv = (const u64 *)P(lo[l].rs_qlinv);
@@ -397,9 +410,16 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     const u64 cz = (u64)np * (l + K) * N;
     // ModUp: INTT of every input limb, convert each digit, NTT the converted limbs
     launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
-    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), coef,
-               l * N, compact, (l + K) * N, cz, nb, s);
-    launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
+    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K,
+               coef, l * N, compact, (l + K) * N, cz, nb, s);
+    if (t.use_f64 && !(t.f64_off & 1)) {
+        NttFuse fz;  // only the converted slots of each digit
+        fz.slots = lt.modup_slots;
+        fz.nslots = lt.n_modup_slots;
+        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+    } else {
+        launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
+    }
     // inner product with the evaluation key
     std::vector<KsIpArgs> ip(nb);
     for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
@@ -411,7 +431,7 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
                     c->h.dnum, shared, s);
     // ModDown of both accumulators
     launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
-    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     if (t.use_f64 && !(t.f64_off & 17)) {

@@ -43,6 +43,8 @@ struct LevelTables {
     const u32 *q_prime = nullptr;          // [l] identity map 0..l-1
     const u32 *p_prime = nullptr;          // [K] L..L+K-1
     const u32 *modup_ntt_prime = nullptr;  // [nparts*(l+K)] prime index per digit slot
+    const u32 *modup_slots = nullptr;      // the valid (converted) digit slots, in order
+    u32 n_modup_slots = 0;
     ConvDesc moddown;                      // P -> Q_l
     const u64 *pinv = nullptr;             // [2*l] (P^-1 mod q_i, shoup)
     const u64 *rs_qlinv = nullptr;         // [2*(l-1)] (q_l^-1 mod q_i, shoup)

@@ -164,6 +164,7 @@ __global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
 // y_i * (qhat_i mod p_j) by mulmod_f64 (|term| <= 1.5 p_j), the running sum
 // centred every four terms (<= 6.5 p_j).
 static constexpr u32 FBC_F64_MAX_IN = 8;
+static constexpr u32 FBC_F64_MAX_PAIRS = 1024;  // n_in * n_out staged in LDS (16 KiB)
 
 __global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ descs,
                                                  const u32 *__restrict__ in_first,
@@ -171,15 +172,24 @@ __global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ de
                                                  const u64 *__restrict__ in, u64 in_zstride,
                                                  u64 *__restrict__ out, u64 out_pstride,
                                                  u64 out_zstride, u32 logN) {
+    __shared__ double2 s_hat[FBC_F64_MAX_PAIRS];  // (qhat_i mod p_j, same / p_j), [i][j]
+    __shared__ double2 s_p[FBC_F64_MAX_PAIRS / FBC_F64_MAX_IN];  // (p_j, 1/p_j)
     const u32 p = blockIdx.y, z = blockIdx.z;
     const ConvDesc D = descs[p];
+    const u32 n_in = D.n_in, n_out = D.n_out;
+    for (u32 k = threadIdx.x; k < n_in * n_out; k += TPB)
+        s_hat[k] = make_double2(D.hatf[2 * k], D.hatf[2 * k + 1]);
+    for (u32 j = threadIdx.x; j < n_out; j += TPB) {
+        const ModParams &m = mp[D.out_prime[j]];
+        s_p[j] = make_double2(m.qd, m.qinv);
+    }
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const u64 *x = in + (size_t)z * in_zstride + ((size_t)in_first[p] << logN) + r;
     u64 *o = out + (size_t)z * out_zstride + (size_t)p * out_pstride + r;
     double y[FBC_F64_MAX_IN];
 #pragma unroll
     for (u32 i = 0; i < FBC_F64_MAX_IN; ++i) {
-        if (i < D.n_in) {
+        if (i < n_in) {
             // the conversion needs the integer [x_i qhat_i^-1]_{q_i} in [0, q_i), not just
             // its residue: the centred product (|.| < q_i) is moved into [0, q_i)
             const double q = mp[D.in_prime[i]].qd;
@@ -189,26 +199,28 @@ __global__ __launch_bounds__(TPB) void k_fbc_f64(const ConvDesc *__restrict__ de
             y[i] = 0.0;
         }
     }
-    for (u32 j = 0; j < D.n_out; ++j) {
-        const double pj = mp[D.out_prime[j]].qd, pinv = mp[D.out_prime[j]].qinv;
-        const double *h = D.hatf + 2 * j;
+    __syncthreads();
+    for (u32 j = 0; j < n_out; ++j) {
+        const double2 pp = s_p[j];
         double acc = 0.0;
 #pragma unroll
         for (u32 i = 0; i < FBC_F64_MAX_IN; ++i) {
-            if (i < D.n_in) {
-                acc += mulmod_f64(y[i], h[2 * (size_t)i * D.n_out], h[2 * (size_t)i * D.n_out + 1], pj);
-                if ((i & 3) == 3) acc = reduce_f64(acc, pj, pinv);
+            if (i < n_in) {
+                const double2 h = s_hat[i * n_out + j];
+                acc += mulmod_f64(y[i], h.x, h.y, pp.x);
+                if ((i & 3) == 3) acc = reduce_f64(acc, pp.x, pp.y);
             }
         }
-        o[(size_t)j << logN] = f64_to_canon(reduce_f64(acc, pj, pinv), pj);
+        o[(size_t)j << logN] = f64_to_canon(reduce_f64(acc, pp.x, pp.y), pp.x);
     }
 }
 
 void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev, u32 ndesc,
-                u32 max_nin, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
+                u32 max_nin, u32 max_nout, const u64 *in, u64 in_zstride, u64 *out, u64 out_pstride,
                 u64 out_zstride, u32 nbatch, hipStream_t s) {
     dim3 grid(t.N / TPB, ndesc, nbatch);
-    if (t.use_f64 && !(t.f64_off & 2) && max_nin <= FBC_F64_MAX_IN)
+    if (t.use_f64 && !(t.f64_off & 2) && max_nin <= FBC_F64_MAX_IN && max_nin * max_nout <= FBC_F64_MAX_PAIRS &&
+        max_nout <= FBC_F64_MAX_PAIRS / FBC_F64_MAX_IN)
         hipLaunchKernelGGL(k_fbc_f64, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in,
                            in_zstride, out, out_pstride, out_zstride, t.logN);
     else

@@ -20,6 +20,7 @@ struct DevTables {
     // 16 fused transform ends, 32 rescale / ModDown combines
     bool use_f64 = false;
     u32 f64_off = 0;
+    u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
     const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
     const double *itwf = nullptr;   // [T][N][2]
     const double *ninvf = nullptr;  // [T][2]
@@ -41,6 +42,9 @@ struct NttFuse {
     const CombineArgs *cb = nullptr;   // MODDOWN_OUT: per item (poly>>1), half = poly&1
     const u64 *acc = nullptr;          // MODDOWN_OUT: x operand, poly at acc + poly*acc_stride
     u64 acc_stride = 0;
+    // optional list of the limb slots to transform (instead of all `limbs` slots)
+    const u32 *slots = nullptr;
+    u32 nslots = 0;
 };
 
 // Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
@@ -83,9 +87,9 @@ void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, 
 // Basis conversion (key_switch.cpp:7-60) for a batch: block (x, p, z) converts
 // descriptor descs[p] reading in + z*in_zstride + (in_first_p)*N and writing
 // out + z*out_zstride + p*out_pstride + j*N.
-// max_nin: the largest n_in among the descriptors.
+// max_nin / max_nout: the largest n_in / n_out among the descriptors.
 void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_first_dev,
-                u32 ndesc, u32 max_nin, const u64 *in, u64 in_zstride, u64 *out,
+                u32 ndesc, u32 max_nin, u32 max_nout, const u64 *in, u64 in_zstride, u64 *out,
                 u64 out_pstride, u64 out_zstride, u32 nbatch, hipStream_t s);
 
 // Key-switch inner product (key_switch.cpp:352-381) for nbatch inputs at level l.

@@ -67,30 +67,25 @@ struct TileGeom {
 template <bool INV, int LOGR, int RD>
 __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, u32 j0) {
     using R = Round<INV, LOGR, RD>;
-    // preload this round's twiddles: stage st uses 2^(3-wb) distinct pairs
-    double w[4][8], wq[4][8];
+    // stage st of the round uses 2^(3-wb) distinct (w, w/q) pairs, wb = its window bit
 #pragma unroll
     for (int st = 0; st < R::NS; ++st) {
         const int b = INV ? R::BIT0 + st : R::BIT0 - st;
         const int wb = b - R::P;
         const u32 k = (u32)b + g.logS + 1;
         const u32 wbase = ((1u << g.logN) >> k) | (j0 >> k);
+        double w[8], wq[8];
 #pragma unroll
         for (int t = 0; t < (8 >> wb); ++t) {
             const double2 pr = reinterpret_cast<const double2 *>(g.tw)[wbase | (u32)t];
-            w[st][t] = pr.x;
-            wq[st][t] = pr.y;
+            w[t] = pr.x;
+            wq[t] = pr.y;
         }
-    }
-#pragma unroll
-    for (int st = 0; st < R::NS; ++st) {
-        const int b = INV ? R::BIT0 + st : R::BIT0 - st;
-        const int wb = b - R::P;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             if (u & (1 << wb)) continue;
             const int u2 = u | (1 << wb);
-            const double ww = w[st][u >> (wb + 1)], wwq = wq[st][u >> (wb + 1)];
+            const double ww = w[u >> (wb + 1)], wwq = wq[u >> (wb + 1)];
             if (!INV) {
                 const double rr = mulmod_f64(x[u2], ww, wwq, g.q);
                 const double a = x[u];
@@ -109,16 +104,20 @@ __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, 
     }
 }
 
-// loads word j of the pass input as a double (with the fused prologue)
-__device__ __forceinline__ double load_in(const char *sbase, u32 j, int in_canon, const NttFuse &fz,
-                                          u32 poly, u32 slot, double q) {
+// raw input word j of the pass (the RESCALE_IN prologue reads the INTT'd last limb)
+__device__ __forceinline__ u64 load_raw(const char *sbase, u32 j, const NttFuse &fz, u32 poly) {
+    if (fz.pro == FUSE_RESCALE_IN) return fz.v[(size_t)poly * fz.v_stride + j];
+    return ((const u64 *)sbase)[j];
+}
+
+// raw word -> signed residue (double)
+__device__ __forceinline__ double raw_to_f64(u64 w, int in_canon, const NttFuse &fz, u32 slot, double q) {
     if (fz.pro == FUSE_RESCALE_IN) {
-        const u64 v = fz.v[(size_t)poly * fz.v_stride + j];
-        const double c = (v > (fz.ql >> 1)) ? u64_to_f64(v) - (double)fz.ql : u64_to_f64(v);
+        const double c = (w > (fz.ql >> 1)) ? u64_to_f64(w) - (double)fz.ql : u64_to_f64(w);
         return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q);
     }
-    if (in_canon) return u64_to_f64(((const u64 *)sbase)[j]);
-    return ((const double *)sbase)[j];
+    if (in_canon) return u64_to_f64(w);
+    return __longlong_as_double((long long)w);
 }
 
 __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_canon, int scale,
@@ -148,6 +147,35 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
     else ((double *)dbase)[j] = v;
 }
 
+// Work item = one 4096-word tile (tiles_per_limb per limb slot, n_tiles in all).
+// The kernel is persistent: workgroup w takes tiles w, w + G, w + 2G, ... and
+// issues the next tile's global loads before the current tile's arithmetic, so
+// loads overlap FP64 work.  Tiles of skipped slots (prime map 0xFFFFFFFF) are
+// passed over.
+struct TileId {
+    u32 tile, poly, slot, pidx, tI;
+};
+
+__device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 limbs,
+                                            const u32 *prime_map, const u32 *slots, u32 nslots) {
+    TileId id{n_tiles, 0, 0, 0, 0};
+    const u32 per_poly = slots ? nslots : limbs;
+    for (; t < n_tiles; t += gridDim.x) {
+        const u32 yl = t / tiles_per_limb;
+        const u32 slot = slots ? slots[yl % nslots] : yl % limbs;
+        const u32 p = prime_map[slot];
+        if (p != 0xFFFFFFFFu) {
+            id.tile = t;
+            id.poly = yl / per_poly;
+            id.slot = slot;
+            id.pidx = p;
+            id.tI = t % tiles_per_limb;
+            return id;
+        }
+    }
+    return id;
+}
+
 template <bool INV, int LOGR, bool COLS>
 __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
@@ -156,133 +184,176 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
                                                   const double *__restrict__ ninvf,
                                                   const ModParams *__restrict__ mp, u32 logN,
                                                   u32 logT, int in_canon, int out_canon, int scale,
-                                                  NttFuse fz) {
+                                                  NttFuse fz, u32 tiles_per_limb, u32 n_tiles) {
     __shared__ double lds[F_TILE + F_TILE / 16];
     constexpr int NR = (LOGR + 3) / 4;
-    const u32 slot = blockIdx.y % limbs;
-    const u32 poly = blockIdx.y / limbs;
-    const u32 pidx = prime_map[slot];
-    if (pidx == 0xFFFFFFFFu) return;
+    using R0 = Round<INV, LOGR, 0>;
+    using RL = Round<INV, LOGR, NR - 1>;
+    constexpr bool DIRECT_IN = COLS || R0::P >= 3;
+    const u32 tid = threadIdx.x;
+    const u32 T = 1u << logT;
+    const bool active = tid < (T >> 4);
     TileGeom g;
     g.logN = logN;
     g.logT = logT;
     g.logC = COLS ? (logT - LOGR) : 0;
     g.logS = COLS ? (logN - LOGR) : 0;
     const u32 logNB = logT - LOGR - g.logC;
-    const u32 ncg = 1u << (g.logS - g.logC);
-    g.c0 = (blockIdx.x & (ncg - 1)) << g.logC;
-    g.b0 = (blockIdx.x >> (g.logS - g.logC)) << logNB;
-    g.q = mp[pidx].qd;
-    g.qinv = mp[pidx].qinv;
-    g.tw = twf + ((size_t)pidx << (logN + 1));
-    const char *sbase = (const char *)src + 8 * ((size_t)poly * src_stride + (size_t)slot * src_slot);
-    char *dbase = (char *)dst + 8 * ((size_t)poly * dst_stride + (size_t)slot * dst_slot);
-    const u32 T = 1u << logT;
-    const u32 tid = threadIdx.x;
-    const bool active = tid < (T >> 4);
-    double x[16];
+    const u32 ncgl = g.logS - g.logC;  // log2 of column groups per limb
+    const u32 P0 = g.logC + R0::P;
+    const u32 eb0 = (tid & ((1u << P0) - 1)) | ((tid >> P0) << (P0 + 4));
+    const bool direct_out_possible = (COLS || RL::P >= 3);
 
-    // ---- round 0 input
-    {
-        using R0 = Round<INV, LOGR, 0>;
-        const u32 P = g.logC + R0::P;
-        const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
-        const bool direct = COLS || R0::P >= 3;
-        if (direct) {
+    auto geom = [&](const TileId &id, TileGeom &gg) {
+        gg.c0 = (id.tI & ((1u << ncgl) - 1)) << gg.logC;
+        gg.b0 = (id.tI >> ncgl) << logNB;
+    };
+    auto issue = [&](const TileId &id, u64 (&r)[16]) {
+        TileGeom gg = g;
+        geom(id, gg);
+        const char *sbase = (const char *)src + 8 * ((size_t)id.poly * src_stride + (size_t)id.slot * src_slot);
+        if (DIRECT_IN) {
             if (active) {
-                const u32 jb = g.gidx(eb, LOGR);
+                const u32 jb = gg.gidx(eb0, LOGR);
 #pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    x[u] = load_in(sbase, jb + ((u32)u << (R0::P + g.logS)), in_canon, fz, poly, slot, g.q);
+                for (int u = 0; u < 16; ++u) r[u] = load_raw(sbase, jb + ((u32)u << (R0::P + gg.logS)), fz, id.poly);
             }
         } else {
-            for (u32 e = tid; e < T; e += F_TPB)
-                lds[lds_pad(e)] = load_in(sbase, g.gidx(e, LOGR), in_canon, fz, poly, slot, g.q);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const u32 e = tid + (u32)k * F_TPB;
+                if (e < T) r[k] = load_raw(sbase, gg.gidx(e, LOGR), fz, id.poly);
+            }
+        }
+    };
+
+    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, limbs, prime_map, fz.slots, fz.nslots);
+    u64 raw[16];
+    if (cur.tile < n_tiles) issue(cur, raw);
+    while (cur.tile < n_tiles) {
+        const TileId nxt =
+            find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, limbs, prime_map, fz.slots, fz.nslots);
+        geom(cur, g);
+        g.q = mp[cur.pidx].qd;
+        g.qinv = mp[cur.pidx].qinv;
+        g.tw = twf + ((size_t)cur.pidx << (logN + 1));
+        char *dbase = (char *)dst + 8 * ((size_t)cur.poly * dst_stride + (size_t)cur.slot * dst_slot);
+        double x[16];
+        // ---- round 0 input (this tile's words were loaded one iteration ago)
+        if (DIRECT_IN) {
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = raw_to_f64(raw[u], in_canon, fz, cur.slot, g.q);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const u32 e = tid + (u32)k * F_TPB;
+                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q);
+            }
+        }
+        // prefetch the next tile
+        if (nxt.tile < n_tiles) issue(nxt, raw);
+        if (!DIRECT_IN) {
             __syncthreads();
             if (active) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb | ((u32)u << P))];
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << P0))];
             }
         }
-        if (active) round_arith<INV, LOGR, 0>(x, g, g.gidx(eb, LOGR));
-    }
-    // ---- later rounds (exchange through LDS)
-    if constexpr (NR > 1) {
+        if (active) round_arith<INV, LOGR, 0>(x, g, g.gidx(eb0, LOGR));
+        // ---- later rounds (exchange through LDS)
+        if constexpr (NR > 1) {
+            {
+                using Rp = Round<INV, LOGR, 0>;
+                using R1 = Round<INV, LOGR, 1>;
+                const u32 Pp = g.logC + Rp::P, P1 = g.logC + R1::P;
+                const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+                const u32 eb1 = (tid & ((1u << P1) - 1)) | ((tid >> P1) << (P1 + 4));
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                }
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1 | ((u32)u << P1))];
+                    round_arith<INV, LOGR, 1>(x, g, g.gidx(eb1, LOGR));
+                }
+            }
+            if constexpr (NR > 2) {
+                using Rp = Round<INV, LOGR, 1>;
+                using R2 = Round<INV, LOGR, 2>;
+                const u32 Pp = g.logC + Rp::P, P2 = g.logC + R2::P;
+                const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+                const u32 eb2 = (tid & ((1u << P2) - 1)) | ((tid >> P2) << (P2 + 4));
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                }
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2 | ((u32)u << P2))];
+                    round_arith<INV, LOGR, 2>(x, g, g.gidx(eb2, LOGR));
+                }
+            }
+        }
+        // ---- output of the last round
         {
-            using Rp = Round<INV, LOGR, 0>;
-            using R1 = Round<INV, LOGR, 1>;
-            const u32 Pp = g.logC + Rp::P, P1 = g.logC + R1::P;
-            const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
-            const u32 eb1 = (tid & ((1u << P1) - 1)) | ((tid >> P1) << (P1 + 4));
-            __syncthreads();  // every round-0 LDS read is done (staged path)
-            if (active) {
+            const u32 P = g.logC + RL::P;
+            const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
+            const double ni = scale ? ninvf[2 * cur.pidx] : 0.0, niq = scale ? ninvf[2 * cur.pidx + 1] : 0.0;
+            if (direct_out_possible && fz.epi == FUSE_NONE) {
+                if (active) {
+                    const u32 jb = g.gidx(eb, LOGR);
 #pragma unroll
-                for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
-            }
-            __syncthreads();
-            if (active) {
+                    for (int u = 0; u < 16; ++u)
+                        store_out(dbase, jb + ((u32)u << (RL::P + g.logS)), x[u], out_canon, scale, ni, niq, fz,
+                                  cur.poly, cur.slot, logN, g.q, g.qinv, INV);
+                }
+            } else {
+                __syncthreads();
+                if (active) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1 | ((u32)u << P1))];
-                round_arith<INV, LOGR, 1>(x, g, g.gidx(eb1, LOGR));
-            }
-        }
-        if constexpr (NR > 2) {
-            using Rp = Round<INV, LOGR, 1>;
-            using R2 = Round<INV, LOGR, 2>;
-            const u32 Pp = g.logC + Rp::P, P2 = g.logC + R2::P;
-            const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
-            const u32 eb2 = (tid & ((1u << P2) - 1)) | ((tid >> P2) << (P2 + 4));
-            __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
-            }
-            __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2 | ((u32)u << P2))];
-                round_arith<INV, LOGR, 2>(x, g, g.gidx(eb2, LOGR));
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
+                }
+                __syncthreads();
+                for (u32 e = tid; e < T; e += F_TPB)
+                    store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, cur.poly,
+                              cur.slot, logN, g.q, g.qinv, INV);
             }
         }
-    }
-    // ---- output of the last round
-    {
-        using RL = Round<INV, LOGR, NR - 1>;
-        const u32 P = g.logC + RL::P;
-        const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
-        const double ni = scale ? ninvf[2 * pidx] : 0.0, niq = scale ? ninvf[2 * pidx + 1] : 0.0;
-        const bool direct = (COLS || RL::P >= 3) && fz.epi == FUSE_NONE;
-        if (direct) {
-            if (active) {
-                const u32 jb = g.gidx(eb, LOGR);
-#pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    store_out(dbase, jb + ((u32)u << (RL::P + g.logS)), x[u], out_canon, scale, ni, niq, fz,
-                              poly, slot, logN, g.q, g.qinv, INV);
-            }
-        } else {
-            __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
-            }
-            __syncthreads();
-            for (u32 e = tid; e < T; e += F_TPB)
-                store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, poly, slot,
-                          logN, g.q, g.qinv, INV);
-        }
+        __syncthreads();  // LDS reads of this tile are done before the next tile writes
+        cur = nxt;
     }
 }
 
 template <bool INV, bool COLS>
-static void launch_pass(int logR, dim3 grid, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+static void launch_pass(int logR, dim3 grid, u32 max_wg, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
+    const u32 tpl = grid.x, ntiles = grid.x * grid.y;
+    // persistent grid: as many workgroups as can be resident at once (max_wg is
+    // 4 per CU; a kernel instance that fits fewer per CU gets fewer)
 #define FHEA_NTT_CASE(LR)                                                                            \
-    case LR:                                                                                         \
-        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), grid, dim3(F_TPB), 0, s, src, dst, ss, ds, sl, dl, \
-                           limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz);                   \
-        break;
+    case LR: {                                                                                       \
+        static int per_cu = 0;                                                                       \
+        if (per_cu == 0) {                                                                           \
+            int n = 0;                                                                               \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS>, F_TPB, 0) != \
+                    hipSuccess || n <= 0)                                                            \
+                n = 4;                                                                               \
+            per_cu = n < 4 ? n : 4;                                                                  \
+        }                                                                                            \
+        const u32 cap = max_wg / 4 * (u32)per_cu;                                                    \
+        const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
+        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
+                           dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles);  \
+        break;                                                                                       \
+    }
     switch (logR) {
         FHEA_NTT_CASE(6)
         FHEA_NTT_CASE(7)
@@ -302,30 +373,33 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
     const u32 lg = t.logN;
     const double *tw = inverse ? t.itwf : t.twf;
     NttFuse none;
+    none.slots = fz.slots;
+    none.nslots = fz.nslots;
+    const u32 ylimbs = fz.slots ? fz.nslots : limbs;
     if (lg <= F_LOG_TILE) {
-        dim3 grid(1, npolys * limbs);
+        dim3 grid(1, npolys * ylimbs);
         if (inverse)
-            launch_pass<true, false>(lg, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
+            launch_pass<true, false>(lg, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
                                      1, 1, 1, none);
         else
-            launch_pass<false, false>(lg, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+            launch_pass<false, false>(lg, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                       lg, 1, 1, 0, fz);
         return;
     }
     const u32 k1 = lg / 2, k2 = lg - k1;
-    dim3 grid(1u << (lg - F_LOG_TILE), npolys * limbs);
+    dim3 grid(1u << (lg - F_LOG_TILE), npolys * ylimbs);
     if (!inverse) {
         NttFuse pro = fz, epi = fz;
         pro.epi = FUSE_NONE;
         epi.pro = FUSE_NONE;
-        launch_pass<false, true>(k1, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, true>(k1, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, pro);
-        launch_pass<false, false>(k2, grid, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, false>(k2, grid, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                   F_LOG_TILE, 0, 1, 0, epi);
     } else {
-        launch_pass<true, false>(k2, grid, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, false>(k2, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, none);
-        launch_pass<true, true>(k1, grid, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, true>(k1, grid, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                 F_LOG_TILE, 0, 1, 1, none);
     }
 }

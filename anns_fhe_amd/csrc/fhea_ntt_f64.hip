@@ -156,17 +156,20 @@ struct TileId {
     u32 tile, poly, slot, pidx, tI;
 };
 
-__device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 limbs,
-                                            const u32 *prime_map, const u32 *slots, u32 nslots) {
+// Tiles are numbered (limb slot, polynomial, tile-in-limb) with the polynomial
+// varying fastest after the tile: the same prime's limbs of a whole batch are
+// in flight together and share its twiddle table in L2.
+__device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 npolys,
+                                            const u32 *prime_map, const u32 *slots) {
     TileId id{n_tiles, 0, 0, 0, 0};
-    const u32 per_poly = slots ? nslots : limbs;
     for (; t < n_tiles; t += gridDim.x) {
         const u32 yl = t / tiles_per_limb;
-        const u32 slot = slots ? slots[yl % nslots] : yl % limbs;
+        const u32 k = yl / npolys;
+        const u32 slot = slots ? slots[k] : k;
         const u32 p = prime_map[slot];
         if (p != 0xFFFFFFFFu) {
             id.tile = t;
-            id.poly = yl / per_poly;
+            id.poly = yl % npolys;
             id.slot = slot;
             id.pidx = p;
             id.tI = t % tiles_per_limb;
@@ -184,7 +187,8 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
                                                   const double *__restrict__ ninvf,
                                                   const ModParams *__restrict__ mp, u32 logN,
                                                   u32 logT, int in_canon, int out_canon, int scale,
-                                                  NttFuse fz, u32 tiles_per_limb, u32 n_tiles) {
+                                                  NttFuse fz, u32 tiles_per_limb, u32 n_tiles,
+                                                  u32 npolys) {
     __shared__ double lds[F_TILE + F_TILE / 16];
     constexpr int NR = (LOGR + 3) / 4;
     using R0 = Round<INV, LOGR, 0>;
@@ -227,12 +231,11 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
         }
     };
 
-    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, limbs, prime_map, fz.slots, fz.nslots);
+    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
     u64 raw[16];
     if (cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
-        const TileId nxt =
-            find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, limbs, prime_map, fz.slots, fz.nslots);
+        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
         g.qinv = mp[cur.pidx].qinv;
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
 }
 
 template <bool INV, bool COLS>
-static void launch_pass(int logR, dim3 grid, u32 max_wg, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
     const u32 tpl = grid.x, ntiles = grid.x * grid.y;
@@ -351,7 +354,8 @@ static void launch_pass(int logR, dim3 grid, u32 max_wg, hipStream_t s, const vo
         const u32 cap = max_wg / 4 * (u32)per_cu;                                                    \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
         hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
-                           dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles);  \
+                           dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles,   \
+                           npolys);                                                                  \
         break;                                                                                       \
     }
     switch (logR) {
@@ -379,10 +383,10 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
     if (lg <= F_LOG_TILE) {
         dim3 grid(1, npolys * ylimbs);
         if (inverse)
-            launch_pass<true, false>(lg, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
+            launch_pass<true, false>(lg, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
                                      1, 1, 1, none);
         else
-            launch_pass<false, false>(lg, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+            launch_pass<false, false>(lg, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                       lg, 1, 1, 0, fz);
         return;
     }
@@ -392,14 +396,14 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
         NttFuse pro = fz, epi = fz;
         pro.epi = FUSE_NONE;
         epi.pro = FUSE_NONE;
-        launch_pass<false, true>(k1, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, true>(k1, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, pro);
-        launch_pass<false, false>(k2, grid, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, false>(k2, grid, npolys, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                   F_LOG_TILE, 0, 1, 0, epi);
     } else {
-        launch_pass<true, false>(k2, grid, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, false>(k2, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, none);
-        launch_pass<true, true>(k1, grid, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, true>(k1, grid, npolys, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                 F_LOG_TILE, 0, 1, 1, none);
     }
 }

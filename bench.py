@@ -47,6 +47,17 @@ def rand_limbs(rng, primes, n):
     return np.stack([rng.integers(0, q, n, dtype=np.uint64) for q in primes])
 
 
+def traffic_from_profile():
+    """HBM bytes per roofline launch from the committed rocprof PMC summary
+    (FETCH_SIZE x its copy-kernel scale + WRITE_SIZE), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "ntt_N65536_L24_96limbs_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(ctx, key, sample):
     """Oracle eval_mult_relin_rescale, single thread, on `sample` ops."""
     from oracle import oracle as O
@@ -140,9 +151,10 @@ def main():
         "config": {"workload": "batched eval_mult_relin_rescale, N=2^16, L=24->23, K=8, dnum=3, "
                                "shared relin key, inputs resident in HBM",
                    "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B, "q_bits": 50},
-        "roofline": {"bound": "hbm", "kernel": "batched forward NTT (2 k_ntt_pass dispatches per launch)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "batched forward NTT, 4 polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},
     }
     # calibration copy of a known size (64 MiB, D2D blit) for rocprof FETCH_SIZE / WRITE_SIZE

@@ -11,9 +11,16 @@
 // they leave a transform, so the result words equal the integer path's.
 //
 // Bounds used by the callers (q < 2^50):
-//   mulmod_f64(v, w, wq): |v| <= 8q, 0 <= w < q  ->  |result| <= 1.5q (< 2^51),
-//                         result == v*w (mod q) exactly.
-//   reduce_f64(x):        |x| <= 8q  ->  |result| <= q/2 + 1.
+//   mulmod_f64(v, w, wq): |v| <= 8q, 0 <= w < q  ->  result == v*w (mod q) exactly and
+//                         |result| <= q (1/2 + |v| 2^-52), i.e. <= q/2 + |v|/4:
+//                         k is within 1/2 + |v w/q| 2^-52 of v w / q (one rounding of
+//                         w/q, one of the product).  So |v| <= q -> 0.75q,
+//                         |v| <= 4q -> 1.5q, |v| <= 8q -> 2.5q.
+//   mulmod_vv_f64(a, b):  canonical a, b -> |result| <= 0.875q.
+//   reduce_f64(x):        |x| <= 8q  ->  |result| <= q/2 + 2.
+// A forward radix-16 round from |x| <= q grows at most to 1.75q, 2.69q, 3.86q, 5.33q
+// over its four stages (< 8q < 2^53); tests/test_f64_arith.py checks these in exact
+// binary64 emulation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,7 +34,8 @@ __device__ __forceinline__ double mulmod_f64(double v, double w, double wq, doub
     return __builtin_fma(-k, q, hi) + lo;
 }
 
-// a*b mod q for two canonical residues a, b in [0, q): |result| <= 0.9q.
+// a*b mod q for two canonical residues a, b in [0, q): |result| <= 0.875q
+// (k is within 3 * 2^-53 * ab/q < 3/8 of ab/q).
 __device__ __forceinline__ double mulmod_vv_f64(double a, double b, double q, double qinv) {
     const double hi = a * b;
     const double lo = __builtin_fma(a, b, -hi);
@@ -44,7 +52,7 @@ __device__ __forceinline__ double u64_to_f64(uint64_t x) {
     return __longlong_as_double((long long)(x | 0x4330000000000000ull)) - 4503599627370496.0;
 }
 
-// |x| <= q/2 + 1 (after reduce_f64)  ->  canonical residue in [0, q).
+// |x| <= q/2 + 2 (after reduce_f64)  ->  canonical residue in [0, q).
 __device__ __forceinline__ uint64_t f64_to_canon(double x, double q) {
     double y = x < 0.0 ? x + q : x;
     return (uint64_t)__double_as_longlong(y + 4503599627370496.0) & 0x000FFFFFFFFFFFFFull;

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(TPB) void k_tensor(const TensorArgs *__restrict__ a
     A.d1[i] = reduce128((u128)a0 * b1 + (u128)a1 * b0, m);
 }
 
-// FP64 form (all primes < 2^50): products by mulmod_vv_f64, |a*b mod q| <= 0.9q.
+// FP64 form (all primes < 2^50): products by mulmod_vv_f64, |a*b mod q| <= 0.875q.
 __global__ __launch_bounds__(TPB) void k_tensor_f64(const TensorArgs *__restrict__ args,
                                                     const ModParams *__restrict__ mp, u32 logN) {
     const u32 limb = blockIdx.y;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
 // FP64 form: y_i = [x_i * qhat_i^-1]_{q_i} in [0, q_i) (x_i, qhat_i^-1 < q_i, so
 // the centred product lies in (-q_i, q_i) and one conditional add makes it
 // canonical; y_i < q_i < 4 p_j because max(q)/min(q) < 4), each term
-// y_i * (qhat_i mod p_j) by mulmod_f64 (|term| <= 1.5 p_j), the running sum
+// y_i * (qhat_i mod p_j) by mulmod_f64 (|term| <= p_j/2 + y_i/4 < 1.5 p_j), the running sum
 // centred every four terms (<= 6.5 p_j).
 static constexpr u32 FBC_F64_MAX_IN = 8;
 static constexpr u32 FBC_F64_MAX_PAIRS = 1024;  // n_in * n_out staged in LDS (16 KiB)
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(TPB) void k_ks_inner(const KsIpArgs *__restrict__ a
 // FP64 form with a stationary key: block (x, i, g) handles key slot i for the
 // `gs` consecutive items of group g, which share one key (a relinearisation
 // batch: one group; per-item rotation keys: gs = 1).  The key words are loaded
-// once per group.  Terms are mulmod_vv_f64 products (<= 0.9q each), at most
+// once per group.  Terms are mulmod_vv_f64 products (<= 0.875q each), at most
 // KS_F64_MAX_PARTS of them per sum (<= 7.2q), centred once.
 static constexpr u32 KS_F64_MAX_PARTS = 8;
 

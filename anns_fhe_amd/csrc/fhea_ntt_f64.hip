@@ -23,9 +23,13 @@
 // at bit >= 3); otherwise they are staged through LDS.
 //
 // Value bounds (q < 2^50, see fhea_f64.hpp):
-//   forward: a round starts at |x| <= q, grows <= 1.5q per stage to <= 7q < 8q,
-//            then every value is centred (|x| <= q/2 + 1);
-//   inverse: the sum branch is centred every stage, products stay <= 1.5q.
+//   forward: a round starts at |x| <= 1.5q (canonical input, a centred value, or
+//            the RESCALE_IN product of |c| <= 2q, which is <= q/2 + |c|/4);
+//            stage s adds a product of magnitude <= q/2 + B_s/4, so the four
+//            stages reach at most 6.6q < 8q; then every value is centred
+//            (|x| <= q/2 + 2);
+//   inverse: the sum branch is centred every stage, the difference (<= 3q) is
+//            multiplied back to <= 1.25q.
 //
 // Fused ends (used by the rescale and ModDown pipelines to save HBM passes):
 //   FUSE_RESCALE_IN   forward COLS prologue: x = centred(v[j]) * T_i (rescale.cpp:43-56)

@@ -1,0 +1,152 @@
+"""Exact IEEE-754 binary64 emulation of the engine's FP64 modular arithmetic.
+
+Mirrors anns_fhe_amd/csrc/fhea_f64.hpp operation by operation so that the FP64
+kernels' arithmetic can be checked on the CPU:
+
+* a * b, a + b, a - b, a / b on Python floats are correctly rounded binary64
+  operations (round to nearest, ties to even), as on the GPU with contraction off;
+* fma(a, b, c) is computed exactly with integers/fractions and rounded once;
+* rint is round-half-to-even (the default rounding mode's rint).
+
+`trace` collects the largest |value| seen at each checked point so that tests can
+assert the magnitude bounds the kernels rely on (|x| < 2^53 keeps integers exact).
+"""
+from fractions import Fraction
+
+TWO53 = float(2 ** 53)
+
+
+def fma(a, b, c):
+    if a == int(a) and b == int(b) and c == int(c):
+        return float(int(a) * int(b) + int(c))  # int -> float rounds to nearest even
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def rint(x):
+    return float(round(x))
+
+
+class Trace:
+    def __init__(self):
+        self.max_ratio = {}
+
+    def see(self, key, x, q):
+        assert abs(x) < TWO53, (key, x)
+        assert x == int(x), (key, x)
+        r = abs(x) / q
+        if r > self.max_ratio.get(key, 0.0):
+            self.max_ratio[key] = r
+
+
+NULL = Trace()
+
+
+def mulmod_f64(v, w, wq, q, tr=NULL):
+    tr.see("mulmod_in", v, q)
+    k = rint(v * wq)
+    hi = v * w
+    lo = fma(v, w, -hi)
+    r = fma(-k, q, hi) + lo
+    tr.see("mulmod_out", r, q)
+    return r
+
+
+def mulmod_vv_f64(a, b, q, qinv, tr=NULL):
+    hi = a * b
+    lo = fma(a, b, -hi)
+    k = rint(hi * qinv)
+    r = fma(-k, q, hi) + lo
+    tr.see("mulmod_vv_out", r, q)
+    return r
+
+
+def reduce_f64(x, q, qinv, tr=NULL):
+    tr.see("reduce_in", x, q)
+    r = fma(-rint(x * qinv), q, x)
+    tr.see("reduce_out", r, q)
+    return r
+
+
+def f64_to_canon(x, q):
+    y = x + q if x < 0.0 else x
+    assert 0.0 <= y < q and y == int(y), (x, q)
+    return int(y)
+
+
+def bit_reverse(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def ntt_forward_f64(a, tw, q, round_stages=4, tr=NULL):
+    """Forward negacyclic NTT as the FP64 kernel computes it: Cooley-Tukey stages
+    (W = tw[m + i]) in radix-2^round_stages rounds, signed residues, one
+    reduce_f64 per element at the end of every round, canonical output."""
+    n = len(a)
+    qd, qinv = float(q), 1.0 / float(q)
+    x = [float(v) for v in a]
+    logn = n.bit_length() - 1
+    m, t, st = 1, n // 2, 0
+    while m < n:
+        for i in range(m):
+            w = float(tw[m + i])
+            wq = w / qd
+            for j in range(2 * i * t, 2 * i * t + t):
+                rr = mulmod_f64(x[j + t], w, wq, qd, tr)
+                u = x[j]
+                x[j], x[j + t] = u + rr, u - rr
+                tr.see("butterfly_out", x[j], qd)
+                tr.see("butterfly_out", x[j + t], qd)
+        st += 1
+        if st % round_stages == 0 or st == logn:
+            x = [reduce_f64(v, qd, qinv, tr) for v in x]
+        m, t = 2 * m, t // 2
+    return [f64_to_canon(v, qd) for v in x]
+
+
+def ntt_inverse_f64(a, itw, n_inv, q, tr=NULL):
+    """Inverse as the FP64 kernel computes it: Gentleman-Sande stages with the
+    sum reduced every stage and the difference multiplied by W = itw[m + i],
+    then the N^-1 scaling and a final reduction."""
+    n = len(a)
+    qd, qinv = float(q), 1.0 / float(q)
+    x = [float(v) for v in a]
+    m, t = n // 2, 1
+    while m >= 1:
+        for i in range(m):
+            w = float(itw[m + i])
+            wq = w / qd
+            for j in range(2 * i * t, 2 * i * t + t):
+                u, v = x[j], x[j + t]
+                x[j] = reduce_f64(u + v, qd, qinv, tr)
+                x[j + t] = mulmod_f64(u - v, w, wq, qd, tr)
+        m, t = m // 2, 2 * t
+    ni = float(n_inv)
+    niq = ni / qd
+    out = []
+    for v in x:
+        v = mulmod_f64(v, ni, niq, qd, tr)
+        out.append(f64_to_canon(reduce_f64(v, qd, qinv, tr), qd))
+    return out
+
+
+def fast_bases_conv_f64(x, in_primes, qhat_inv, qhat_mod_p, out_primes, tr=NULL):
+    """One coefficient of the FP64 basis conversion: y_i = [x_i qhat_i^-1]_{q_i}
+    made canonical, then sum_i y_i (qhat_i mod p_j) with a reduction every four
+    terms (the kernel's k_fbc_f64)."""
+    ys = []
+    for i, qi in enumerate(in_primes):
+        w = float(qhat_inv[i])
+        t = mulmod_f64(float(x[i]), w, w / float(qi), float(qi), tr)
+        ys.append(t + float(qi) if t < 0.0 else t)
+    out = []
+    for j, pj in enumerate(out_primes):
+        pd, pinv = float(pj), 1.0 / float(pj)
+        acc = 0.0
+        for i in range(len(in_primes)):
+            h = float(qhat_mod_p[i][j])
+            acc += mulmod_f64(ys[i], h, h / pd, pd, tr)
+            tr.see("fbc_acc", acc, pd)
+            if (i & 3) == 3:
+                acc = reduce_f64(acc, pd, pinv, tr)
+        out.append(f64_to_canon(reduce_f64(acc, pd, pinv, tr), pd))
+    return out

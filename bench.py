@@ -36,6 +36,11 @@ def parse():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle ops timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ntt-reps", type=int, default=5)
+    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64"],
+                    help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
+                         "ks64: 64 rotations with distinct resident keys")
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="single-ciphertext op-by-op HMult runs timed for plain_path (0 = skip)")
     return ap.parse_args()
 
 
@@ -76,12 +81,134 @@ def cpu_baseline(ctx, key, sample):
             "sample": f"{sample} x eval_mult_relin_rescale N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread)"}
 
 
+def plain_hmult(ctx, drv, base, steps):
+    """The reference's eval_mult_relin_rescale op by op (fhe_host_driver.cpp:272-306):
+    OP 5 x4, OP 3, OP 7, OP 3 x2, OP 8 and the two copies, one ciphertext at a time,
+    no fusion or batching.  Scratch follows the inputs at `base`."""
+    W, W1 = L * N, (L - 1) * N
+    a0, a1, b0, b1 = 0, W, 2 * W, 3 * W
+    d0, d1, d2, tmp = base, base + W, base + 2 * W, base + 3 * W
+    ks, out = base + 4 * W, base + 6 * W
+
+    def one():
+        drv.poly_mul_ntt(a0, b0, d0, L)
+        drv.poly_mul_ntt(a1, b1, d2, L)
+        drv.poly_mul_ntt(a0, b1, d1, L)
+        drv.poly_mul_ntt(a1, b0, tmp, L)
+        drv.poly_add(d1, tmp, d1, L)
+        drv.key_switch(d2, 0, ks, ks + W, L)
+        drv.poly_add(d0, ks, d0, L)
+        drv.poly_add(d1, ks + W, d1, L)
+        drv.rescale(d0, d1, tmp, tmp + W1, L)
+        assert ctx.lib.fhea_copy_poly(ctx.h, out, tmp, W1) == 0
+        assert ctx.lib.fhea_copy_poly(ctx.h, out + W, tmp + W1, W1) == 0
+
+    one()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(steps):
+        one()
+        ctx.sync()  # the reference's composite is synchronous (host read/write of the result)
+    ms = ctx.timer_stop()
+    return {"value": round(steps / (ms / 1e3), 2), "unit": "HMult+relin+rescale ops/s",
+            "ms_per_op": round(ms / steps, 3),
+            "path": "one ciphertext, reference op codes one launch sequence each (OP 5,3,7,3,8), "
+                    "synchronised per op"}
+
+
+def bench_ntt256(args):
+    """256 polynomials x 24 limbs, forward then inverse (in place), 50-bit primes."""
+    from anns_fhe_amd.driver import FheContext
+    P = 256
+    T = P * L * N
+    ctx = FheContext(N, L, 1, 1, q_bits=50, poly_words=T + 8, key_words=8)
+    rng = np.random.default_rng(42)
+    for p in range(P):
+        ctx.write_poly(p * L * N, rand_limbs(rng, ctx.q, N))
+    for _ in range(args.warmup):
+        ctx.ntt_batch(0, 0, P, L, inverse=False)
+        ctx.ntt_batch(0, 0, P, L, inverse=True)
+    ctx.sync()
+    fwd_ms = inv_ms = 0.0
+    for _ in range(args.steps):
+        ctx.timer_start()
+        ctx.ntt_batch(0, 0, P, L, inverse=False)
+        fwd_ms += ctx.timer_stop()
+        ctx.timer_start()
+        ctx.ntt_batch(0, 0, P, L, inverse=True)
+        inv_ms += ctx.timer_stop()
+    ms = (fwd_ms + inv_ms) / args.steps
+    limbs = 2 * P * L
+    alg = limbs * 2 * N * 8
+    gbs = alg / (ms / 1e3) / 1e9
+    log(f"ntt256: {ms:.2f} ms per fwd+inv -> {gbs:.0f} GB/s algorithmic")
+    return {
+        "metric": "NTT+INTT limb-transforms/s at N=2^16 (256 polys x 24 limbs)",
+        "value": round(limbs / (ms / 1e3), 1), "unit": "limb-transforms/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (uniform residues)",
+        "config": {"workload": "batched forward then inverse negacyclic NTT, 256 polys x 24 limbs, N=2^16, "
+                               "50-bit primes, in place in HBM", "N": N, "L": L, "polys": P},
+        "roofline": {"bound": "hbm", "kernel": "k_ntt64f (forward and inverse, COLS + ROWS passes)",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "fwd_ms": round(fwd_ms / args.steps, 3), "inv_ms": round(inv_ms / args.steps, 3),
+                     "alg_bytes_per_step": alg},
+    }
+
+
+def bench_ks64(args):
+    """64 ciphertexts, each sigma_g then key switching with its own resident key
+    (g = 5^k mod 2N, k seeded uniform in [1, N/2)); keys are uniform residues."""
+    from anns_fhe_amd.driver import FheContext
+    n_ct = 64
+    W = L * N
+    ksz = 2 * DNUM * (L + K) * N
+    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=4 * n_ct * W + 8, key_words=n_ct * ksz + 8)
+    rng = np.random.default_rng(42)
+    kprimes = ctx.q + ctx.p
+    blob = np.concatenate([rand_limbs(rng, kprimes, N).reshape(-1) for _ in range(2 * DNUM)])
+    for z in range(n_ct):
+        ctx.write_key(z * ksz, blob)
+    for z in range(2 * n_ct):
+        ctx.write_poly(z * W, rand_limbs(rng, ctx.q, N))
+    gal = [pow(5, int(k), 2 * N) for k in rng.integers(1, N // 2, n_ct)]
+    works = [dict(kind=2, limbs=L, key_limbs=L, a0=2 * z * W, a1=(2 * z + 1) * W, d0=(2 * n_ct + 2 * z) * W,
+                  d1=(2 * n_ct + 2 * z + 1) * W, galois=gal[z], evk=z * ksz) for z in range(n_ct)]
+    for _ in range(args.warmup):
+        ctx.submit(works)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.submit(works)
+    ms = ctx.timer_stop() / args.steps
+    alg = n_ct * (2 * W + ksz + 2 * W) * 8  # ct in, own key, ct out
+    gbs = alg / (ms / 1e3) / 1e9
+    log(f"ks64: {ms:.2f} ms per 64 rotations -> {gbs:.0f} GB/s algorithmic")
+    return {
+        "metric": "rotations (automorphism + hybrid key switch) /s at N=2^16,L=24,K=8,dnum=3, distinct keys",
+        "value": round(n_ct / (ms / 1e3), 1), "unit": "rotations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (uniform residues and key words)",
+        "config": {"workload": "64 ciphertexts, each rotated by its own seeded Galois element 5^k mod 2N "
+                               "with its own resident 96 MiB key (6 GiB of keys)",
+                   "N": N, "L": L, "K": K, "dnum": DNUM, "cts": n_ct, "key_bytes": ksz * 8},
+        "roofline": {"bound": "hbm", "kernel": "whole rotation (ct in + key + ct out per rotation)",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(data-parallel sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "hmult":
+        res = bench_ntt256(args) if args.workload == "ntt256" else bench_ks64(args)
+        print(json.dumps(res), flush=True)
+        return
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
 
     B = args.batch
@@ -89,7 +216,9 @@ def main():
     per_item = 4 * W + 2 * W1
     key_words = 2 * DNUM * (L + K) * N
     log(f"context N={N} L={L} K={K} dnum={DNUM} batch={B}")
-    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=B * per_item + 8, key_words=key_words + 8)
+    plain_words = 8 * W  # scratch of the op-by-op path
+    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=B * per_item + plain_words + 8,
+                     key_words=key_words + 8)
     drv = FHEHostDriver(ctx)
     s = ctx.keygen_secret(42)
     ctx.keygen_relin_key(s, L, 43, 0)
@@ -165,6 +294,9 @@ def main():
         assert lib.fhea_copy_poly(ctx.h, calib_words, 0, calib_words) == 0
         ctx.sync()
         result["calibration_copy_bytes"] = calib_words * 8
+    if args.plain_steps > 0:
+        log("plain per-op path")
+        result["plain_path"] = plain_hmult(ctx, drv, B * per_item, args.plain_steps)
     if args.cpu_sample > 0:
         log("cpu baseline")
         key = ctx.read_key(0, key_words)

@@ -14,6 +14,7 @@ u64p = C.POINTER(C.c_uint64)
 u32p = C.POINTER(C.c_uint32)
 i8p = C.POINTER(C.c_int8)
 i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
 vp = C.c_void_p
 
 STATUS = {
@@ -90,6 +91,20 @@ SIGNATURES = {
     "fhea_keygen_rotation_key": (C.c_int, [vp, i8p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
     "fhea_key_size_words": (C.c_uint64, [vp, C.c_uint32]),
     "fhea_galois_for_step": (C.c_uint64, [vp, C.c_int32]),
+    "fhea_client_create": (C.c_int, [C.POINTER(fhea_params), C.POINTER(vp)]),
+    "fhea_client_from_ctx": (C.c_int, [vp, C.POINTER(vp)]),
+    "fhea_client_destroy": (C.c_int, [vp]),
+    "fhea_client_info": (C.c_int, [vp, u32p]),
+    "fhea_client_primes": (C.c_int, [vp, u64p]),
+    "fhea_client_keygen_secret": (C.c_int, [vp, C.c_uint64, i8p]),
+    "fhea_client_switch_key": (C.c_int, [vp, i8p, i64p, C.c_uint32, C.c_uint64, u64p]),
+    "fhea_client_relin_key": (C.c_int, [vp, i8p, C.c_uint32, C.c_uint64, u64p]),
+    "fhea_client_rotation_key": (C.c_int, [vp, i8p, C.c_uint64, C.c_uint32, C.c_uint64, u64p]),
+    "fhea_encode": (C.c_int, [vp, f64p, f64p, C.c_uint32, C.c_uint32, C.c_double, u64p]),
+    "fhea_decode": (C.c_int, [vp, u64p, C.c_uint32, C.c_double, C.c_uint32, f64p, f64p]),
+    "fhea_keygen_public": (C.c_int, [vp, i8p, C.c_uint64, u64p]),
+    "fhea_encrypt": (C.c_int, [vp, u64p, u64p, C.c_uint32, C.c_uint64, u64p]),
+    "fhea_decrypt": (C.c_int, [vp, i8p, u64p, C.c_uint32, u64p]),
 }
 
 _lib = None

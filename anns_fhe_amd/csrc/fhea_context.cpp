@@ -121,4 +121,35 @@ void build_host_tables(HostTables &h) {
     }
 }
 
+int host_tables_from_params(const fhea_params *p, u32 min_n, HostTables &h) {
+    if (!p) return FHEA_ERR_ARG;
+    const u32 N = p->N;
+    if (N < min_n || N > (1u << 17) || (N & (N - 1))) return FHEA_ERR_ARG;
+    if (p->L < 1 || p->K < 1 || p->dnum < 1 || p->dnum > p->L || p->L + p->K > 255) return FHEA_ERR_ARG;
+    const u32 alpha = (p->L + p->dnum - 1) / p->dnum;
+    if (alpha > 32 || p->K > 32) return FHEA_ERR_ARG;
+    const u32 bits = p->q_bits ? p->q_bits : 50;
+    if (bits < 20 || bits > 61) return FHEA_ERR_ARG;
+    h = HostTables();
+    h.N = N;
+    while ((1u << h.logN) < N) ++h.logN;
+    h.L = p->L;
+    h.K = p->K;
+    h.dnum = p->dnum;
+    h.alpha = alpha;
+    std::vector<u64> gen;
+    if (!p->q_primes || !p->p_primes) {
+        if (gen_primes(2 * N, bits, p->L + p->K, {}, gen) != p->L + p->K) return FHEA_ERR_ARG;
+    }
+    for (u32 i = 0; i < p->L; ++i) h.primes.push_back(p->q_primes ? p->q_primes[i] : gen[i]);
+    for (u32 j = 0; j < p->K; ++j) h.primes.push_back(p->p_primes ? p->p_primes[j] : gen[p->L + j]);
+    for (u64 q : h.primes)
+        if (q < 3 || q >= (1ull << 61) || (q - 1) % (2ull * N) != 0 || !is_prime_u64(q)) return FHEA_ERR_ARG;
+    for (size_t i = 0; i < h.primes.size(); ++i)
+        for (size_t j = i + 1; j < h.primes.size(); ++j)
+            if (h.primes[i] == h.primes[j]) return FHEA_ERR_ARG;
+    build_host_tables(h);
+    return FHEA_OK;
+}
+
 }  // namespace fhea

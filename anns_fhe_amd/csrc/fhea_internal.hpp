@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fhea.h"
 #include "fhea_arith.hpp"
 
 namespace fhea {
@@ -74,5 +75,8 @@ u64 inv_mod(u64 a, u64 q);
 u64 pow_mod(u64 a, u64 e, u64 q);
 u32 bit_reverse(u32 x, u32 bits);
 void build_host_tables(HostTables &h);
+// Validates a parameter block (N in [min_n, 2^17], L, K, dnum, primes) and fills h.
+// Returns an FHEA_* status.
+int host_tables_from_params(const fhea_params *p, u32 min_n, HostTables &h);
 
 }  // namespace fhea

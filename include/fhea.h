@@ -195,6 +195,43 @@ uint64_t fhea_key_size_words(const fhea_ctx *ctx, uint32_t key_limbs);
 /* Galois element for a slot rotation by step: 5^step mod 2N (step may be negative). */
 uint64_t fhea_galois_for_step(const fhea_ctx *ctx, int32_t step);
 
+/* ---- Host-only CKKS client: the role OpenFHE plays for the reference's client
+ *      (MakeCKKSPackedPlaintext / Encrypt / Decrypt / GetRealPackedValue and the
+ *      key generators, software_golden_model/fhe_core/fhe_context_manager.cpp:81-128,
+ *      457-512; integration_tools/generate_interactive_test_vectors.cpp:161-224).
+ *      No GPU is touched; the same parameter block gives the same primes as
+ *      fhea_ctx_create.  Polynomials are limb-major, evaluation domain.
+ *      Slot j of a plaintext is m(zeta^(5^j mod 2N)) / scale, zeta = exp(i pi / N);
+ *      n values (n a power of two, n <= N/2) are replicated with period n.       */
+typedef struct fhea_client fhea_client;
+int fhea_client_create(const fhea_params *params, fhea_client **out); /* N >= 16 allowed */
+int fhea_client_from_ctx(const fhea_ctx *ctx, fhea_client **out);
+int fhea_client_destroy(fhea_client *cl);
+int fhea_client_info(const fhea_client *cl, uint32_t *info5);  /* N, L, K, dnum, alpha */
+int fhea_client_primes(const fhea_client *cl, uint64_t *out);   /* L+K primes        */
+int fhea_client_keygen_secret(const fhea_client *cl, uint64_t seed, int8_t *s_out);
+/* evaluation keys into a host buffer of fhea_key_size_words words, same layout as the arena */
+int fhea_client_switch_key(const fhea_client *cl, const int8_t *s, const int64_t *s_new,
+                           uint32_t key_limbs, uint64_t seed, uint64_t *key_out);
+int fhea_client_relin_key(const fhea_client *cl, const int8_t *s, uint32_t key_limbs, uint64_t seed,
+                          uint64_t *key_out);
+int fhea_client_rotation_key(const fhea_client *cl, const int8_t *s, uint64_t galois,
+                             uint32_t key_limbs, uint64_t seed, uint64_t *key_out);
+/* re/im: n values (im may be NULL); pt: limbs*N words; |coefficients| < 4e18 or FHEA_ERR_ARG */
+int fhea_encode(const fhea_client *cl, const double *re, const double *im, uint32_t n,
+                uint32_t limbs, double scale, uint64_t *pt);
+/* first n slots of pt (limbs*N words); im may be NULL */
+int fhea_decode(const fhea_client *cl, const uint64_t *pt, uint32_t limbs, double scale,
+                uint32_t n, double *re, double *im);
+/* pk: 2*L*N words, b = -a s + e then a (all L Q limbs) */
+int fhea_keygen_public(const fhea_client *cl, const int8_t *s, uint64_t seed, uint64_t *pk);
+/* ct = (v b + e0 + pt, v a + e1) on the first `limbs` Q limbs: 2*limbs*N words */
+int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, uint32_t limbs,
+                 uint64_t seed, uint64_t *ct);
+/* pt = c0 + c1 s (limbs*N words) */
+int fhea_decrypt(const fhea_client *cl, const int8_t *s, const uint64_t *ct, uint32_t limbs,
+                 uint64_t *pt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,3 +111,47 @@ def test_batched_keyswitch_64_cts_distinct_keys(orc):
             assert np.array_equal(batched[2 * z * W:2 * (z + 1) * W], pm[2 * W:4 * W])
     finally:
         ctx.close()
+
+
+def test_full_size_ckks_semantics_on_gpu():
+    """Client-encrypted vectors at N=2^16, L=24, K=8, dnum=3 through the GPU:
+    Dec(HMult(Enc(x), Enc(y))) = x*y and Dec(rotate(Enc(x), r)) = x shifted by r."""
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    W = L * N
+    ksz = 2 * DNUM * (L + K) * N
+    steps = [1, 2, 4, -1]
+    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=8 * W + 8, key_words=(1 + len(steps)) * ksz + 8)
+    try:
+        cl = CkksClient(0, 0, 0, 0, ctx=ctx)
+        assert cl.primes == ctx.primes
+        drv = FHEHostDriver(ctx)
+        s = cl.keygen_secret(11)
+        pk = cl.keygen_public(s, 12)
+        say("keys")
+        ctx.write_key(0, cl.relin_key(s, L, 13))
+        drv.set_relin_key(0, L)
+        for i, st in enumerate(steps):
+            g = cl.galois_for_step(st)
+            ctx.write_key((1 + i) * ksz, cl.rotation_key(s, g, L, 20 + i))
+            drv.add_rotation_key(st, g, (1 + i) * ksz, L)
+        rng = np.random.default_rng(5)
+        n = N // 2
+        x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        scale = float(ctx.q[L - 1])
+        say("encrypt")
+        ctx.write_poly(0, cl.encrypt_values(pk, x, L, scale, 30))
+        ctx.write_poly(2 * W, cl.encrypt_values(pk, y, L, scale, 31))
+        drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 4 * W, 5 * W, L)
+        W1 = (L - 1) * N
+        ct = np.concatenate([ctx.read_poly(4 * W, W1), ctx.read_poly(5 * W, W1)])
+        say("decrypt product")
+        got = cl.decrypt_values(s, ct, L - 1, scale * scale / ctx.q[L - 1], n)
+        assert np.max(np.abs(got - x * y)) < 1e-6
+        for r in (1, 7, -1):
+            drv.eval_rotate(0, W, 6 * W, 7 * W, r, L)
+            rot = np.concatenate([ctx.read_poly(6 * W, W), ctx.read_poly(7 * W, W)])
+            got = cl.decrypt_values(s, rot, L, scale, n)
+            assert np.max(np.abs(got - np.roll(x, -r))) < 1e-6, r
+    finally:
+        ctx.close()

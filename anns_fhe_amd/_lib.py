@@ -26,6 +26,8 @@ STATUS = {
 OP_NTT, OP_INTT, OP_POLY_ADD, OP_POLY_SUB, OP_POLY_MUL_NTT = 1, 2, 3, 4, 5
 OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
 WORK_MULT_RELIN_RESCALE, WORK_AUTOMORPHISM_KS, WORK_NTT, WORK_INTT = 1, 2, 3, 4
+WORK_MULT_RELIN, WORK_ADD = 5, 6
+NONE = (1 << 64) - 1
 
 
 class FheLibraryError(RuntimeError):
@@ -83,6 +85,8 @@ SIGNATURES = {
     "fhea_submit": (C.c_int, [vp, C.POINTER(fhea_work), C.c_uint32]),
     "fhea_ntt_batch": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
     "fhea_sync": (C.c_int, [vp]),
+    "fhea_eval_linear_sum": (C.c_int, [vp, u64p, u64p, C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64,
+                                       C.c_uint64, u64p, u64p, C.c_uint32]),
     "fhea_timer_start": (C.c_int, [vp]),
     "fhea_timer_stop": (C.c_int, [vp, C.POINTER(C.c_float)]),
     "fhea_keygen_secret": (C.c_int, [vp, C.c_uint64, i8p]),

@@ -527,6 +527,34 @@ static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
     return check_hip(hipGetLastError());
 }
 
+// mult + relinearise without the rescale (the first half of
+// fhe_host_driver.cpp:272-306): tensor -> KS(d2) with d0, d1 added in the ModDown.
+static int mult_relin(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs, const std::vector<MultItem> &items) {
+    const u64 N = c->h.N;
+    for (size_t off = 0; off < items.size(); off += c->max_chunk) {
+        u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        u64 dwords = (u64)nb * 3 * l * N;
+        int st = ensure_ws(c, dwords + nb * ks_words(c, l));
+        if (st) return st;
+        u64 *D = c->ws, *rest = D + dwords;
+        std::vector<TensorArgs> ta(nb);
+        std::vector<CombineArgs> cb(nb);
+        std::vector<const u64 *> keys(nb, key);
+        for (u32 z = 0; z < nb; ++z) {
+            const MultItem &it = items[off + z];
+            u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
+            ta[z] = {it.a0, it.a1, it.b0, it.b1, d0, d1, d2};
+            cb[z] = {it.d0, it.d1, d0, d1};
+        }
+        auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
+        if (!ta_d) return FHEA_ERR_DEVICE;
+        launch_tensor(c->dt, ta_d, nb, l, c->stream);
+        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest);
+        if (st) return st;
+    }
+    return check_hip(hipGetLastError());
+}
+
 struct AutoItem {
     const u64 *in0, *in1;
     u64 *d0, *d1;
@@ -1003,6 +1031,45 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                 v.push_back({P + w.a0, P + w.a1, P + w.d0, P + w.d1, w.galois, c->key + w.evk});
             }
             st = automorphism_ks(c, l, kl, v);
+        } else if (w0.kind == FHEA_WORK_MULT_RELIN) {
+            if (!c->has_relin) return FHEA_ERR_NO_KEY;
+            u32 kl = c->relin_limbs ? c->relin_limbs : l;
+            if (kl < l) return FHEA_ERR_ARG;
+            const u64 W = (u64)l * c->h.N;
+            std::vector<MultItem> v;
+            for (uint32_t k = i; k < j; ++k) {
+                const fhea_work &w = items[k];
+                if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+                POLY_CHECK(c, w.a0, W);
+                POLY_CHECK(c, w.a1, W);
+                POLY_CHECK(c, w.b0, W);
+                POLY_CHECK(c, w.b1, W);
+                POLY_CHECK(c, w.d0, W);
+                POLY_CHECK(c, w.d1, W);
+                v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+            }
+            st = mult_relin(c, l, c->key + c->relin_off, kl, v);
+        } else if (w0.kind == FHEA_WORK_ADD) {
+            if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+            const u64 W = (u64)l * c->h.N;
+            std::vector<AddArgs> v;
+            for (uint32_t k = i; k < j; ++k) {
+                const fhea_work &w = items[k];
+                POLY_CHECK(c, w.a0, W);
+                POLY_CHECK(c, w.a1, W);
+                POLY_CHECK(c, w.b0, W);
+                POLY_CHECK(c, w.b1, W);
+                POLY_CHECK(c, w.d0, W);
+                POLY_CHECK(c, w.d1, W);
+                v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+            }
+            for (size_t o = 0; o < v.size(); o += 65535) {
+                const u32 nb = (u32)std::min<size_t>(65535, v.size() - o);
+                auto *ad = (const AddArgs *)stage_args(c, v.data() + o, nb * sizeof(AddArgs));
+                if (!ad) return FHEA_ERR_DEVICE;
+                launch_add_batch(c->dt, ad, nb, l, c->stream);
+            }
+            st = check_hip(hipGetLastError());
         } else if (w0.kind == FHEA_WORK_NTT || w0.kind == FHEA_WORK_INTT) {
             for (uint32_t k = i; k < j && !st; ++k)
                 st = fhea_op(c, w0.kind == FHEA_WORK_NTT ? FHEA_OP_NTT : FHEA_OP_INTT, items[k].a0, 0,
@@ -1014,6 +1081,66 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
         i = j;
     }
     return FHEA_OK;
+}
+
+static bool overlaps(u64 a, u64 b, u64 W) { return a < b + W && b < a + W; }
+
+int fhea_eval_linear_sum(fhea_ctx *c, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,
+                         const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,
+                         uint64_t bias1, const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (!ct0 || !ct1 || !pt || !out0 || !out1 || R == 0 || G == 0 || limbs == 0 || limbs > c->h.L)
+        return FHEA_ERR_ARG;
+    if ((bias0 == FHEA_NONE) != (bias1 == FHEA_NONE)) return FHEA_ERR_ARG;
+    if (G > 65535u * 8u) return FHEA_ERR_ARG;
+    const u64 W = (u64)limbs * c->h.N;
+    std::vector<u64> offs;
+    offs.reserve(2 * R + (size_t)G * R + 3 * G);
+    for (u32 r = 0; r < R; ++r) {
+        POLY_CHECK(c, ct0[r], W);
+        offs.push_back(ct0[r]);
+    }
+    for (u32 r = 0; r < R; ++r) {
+        POLY_CHECK(c, ct1[r], W);
+        offs.push_back(ct1[r]);
+    }
+    for (size_t k = 0; k < (size_t)G * R; ++k) {
+        POLY_CHECK(c, pt[k], W);
+        offs.push_back(pt[k]);
+    }
+    for (u32 g = 0; g < G; ++g) {
+        const u64 o = ptb ? ptb[g] : FHEA_NONE;
+        if (o != FHEA_NONE) POLY_CHECK(c, o, W);
+        offs.push_back(o);
+    }
+    if (bias0 != FHEA_NONE) {
+        POLY_CHECK(c, bias0, W);
+        POLY_CHECK(c, bias1, W);
+    }
+    const size_t n_in = offs.size();
+    for (u32 g = 0; g < G; ++g) {
+        POLY_CHECK(c, out0[g], W);
+        POLY_CHECK(c, out1[g], W);
+    }
+    // outputs must not overlap any input or each other (blocks of other outputs read inputs late)
+    for (u32 g = 0; g < G; ++g) {
+        for (u64 o : {out0[g], out1[g]}) {
+            for (size_t k = 0; k < n_in; ++k)
+                if (offs[k] != FHEA_NONE && overlaps(o, offs[k], W)) return FHEA_ERR_ARG;
+            if (bias0 != FHEA_NONE && (overlaps(o, bias0, W) || overlaps(o, bias1, W))) return FHEA_ERR_ARG;
+        }
+        if (overlaps(out0[g], out1[g], W)) return FHEA_ERR_ARG;
+        for (u32 h2 = g + 1; h2 < G; ++h2)
+            if (overlaps(out0[g], out0[h2], W) || overlaps(out0[g], out1[h2], W) ||
+                overlaps(out1[g], out0[h2], W) || overlaps(out1[g], out1[h2], W))
+                return FHEA_ERR_ARG;
+    }
+    for (u32 g = 0; g < G; ++g) offs.push_back(out0[g]);
+    for (u32 g = 0; g < G; ++g) offs.push_back(out1[g]);
+    auto *od = (const u64 *)stage_args(c, offs.data(), offs.size() * 8);
+    if (!od) return FHEA_ERR_NO_MEM;
+    launch_linear_sum(c->dt, c->poly, od, R, G, bias0, bias1, limbs, c->stream);
+    return check_hip(hipGetLastError());
 }
 
 int fhea_ntt_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t npolys, uint32_t limbs,

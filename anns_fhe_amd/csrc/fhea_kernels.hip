@@ -484,4 +484,133 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
                            t.mp, t.logN);
 }
 
+// ---------------------------------------------------------------------------
+// Batched ciphertext addition (OP 3 on both polynomials, fhe_host_driver.cpp:248-256).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_add_batch(const AddArgs *__restrict__ args,
+                                                   const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y;
+    const AddArgs A = args[blockIdx.z];
+    const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const u64 q = mp[limb].q;
+    A.d0[i] = add_mod(A.a0[i], A.b0[i], q);
+    A.d1[i] = add_mod(A.a1[i], A.b1[i], q);
+}
+
+void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u32 limbs, hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs, nbatch);
+    hipLaunchKernelGGL(k_add_batch, grid, dim3(TPB), 0, s, args_dev, t.mp, t.logN);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-method inner products: out_g = bias + ptb_g + sum_r ct_r (.) pt_{g,r}.
+// The reference evaluates each candidate batch with its own rotate-and-sum
+// (interactive_query_manager.cpp:190-274); with the query's rotations computed
+// once, every candidate block is one plaintext multiply-accumulate over r, which
+// is this kernel: thread (j, limb) holds LIN_G outputs, streams the R rotated
+// ciphertext words once and the G*R plaintext words once.
+// ---------------------------------------------------------------------------
+static constexpr u32 LIN_G = 8;
+
+__device__ __forceinline__ u64 lin_word(const u64 *P, u64 off, size_t idx) {
+    return off == LIN_NONE ? 0 : P[off + idx];
+}
+
+// Integer form: 128-bit lazy sums, reduced every 32 products (< 2^127).
+__global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const u64 *__restrict__ offs,
+                                                    u32 R, u32 G, u64 bias0, u64 bias1,
+                                                    const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y, g0 = blockIdx.z * LIN_G;
+    const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const ModParams m = mp[limb];
+    const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
+    const u64 *out0 = ptb + G, *out1 = out0 + G;
+    const u64 b0 = lin_word(P, bias0, idx), b1 = lin_word(P, bias1, idx);
+    u128 a0[LIN_G], a1[LIN_G];
+#pragma unroll
+    for (u32 g = 0; g < LIN_G; ++g) {
+        a0[g] = (u128)b0 + (g0 + g < G ? lin_word(P, ptb[g0 + g], idx) : 0);
+        a1[g] = b1;
+    }
+    for (u32 r = 0; r < R; ++r) {
+        const u64 x0 = P[ct0[r] + idx], x1 = P[ct1[r] + idx];
+#pragma unroll
+        for (u32 g = 0; g < LIN_G; ++g) {
+            if (g0 + g < G) {
+                const u64 p = P[pt[(size_t)(g0 + g) * R + r] + idx];
+                a0[g] += (u128)x0 * p;
+                a1[g] += (u128)x1 * p;
+            }
+        }
+        if ((r & 31) == 31) {
+#pragma unroll
+            for (u32 g = 0; g < LIN_G; ++g) {
+                a0[g] = reduce128(a0[g], m);
+                a1[g] = reduce128(a1[g], m);
+            }
+        }
+    }
+#pragma unroll
+    for (u32 g = 0; g < LIN_G; ++g) {
+        if (g0 + g < G) {
+            P[out0[g0 + g] + idx] = reduce128(a0[g], m);
+            P[out1[g0 + g] + idx] = reduce128(a1[g], m);
+        }
+    }
+}
+
+// FP64 form (primes < 2^50): mulmod_vv_f64 terms (<= 0.875q), sums centred every
+// 8 products (<= q/2 + 2 + 7q).
+__global__ __launch_bounds__(TPB) void k_linear_sum_f64(u64 *__restrict__ P, const u64 *__restrict__ offs,
+                                                        u32 R, u32 G, u64 bias0, u64 bias1,
+                                                        const ModParams *__restrict__ mp, u32 logN) {
+    const u32 limb = blockIdx.y, g0 = blockIdx.z * LIN_G;
+    const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const double q = mp[limb].qd, qinv = mp[limb].qinv;
+    const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
+    const u64 *out0 = ptb + G, *out1 = out0 + G;
+    const double b0 = u64_to_f64(lin_word(P, bias0, idx)), b1 = u64_to_f64(lin_word(P, bias1, idx));
+    double a0[LIN_G], a1[LIN_G];
+#pragma unroll
+    for (u32 g = 0; g < LIN_G; ++g) {
+        const double pb = g0 + g < G ? u64_to_f64(lin_word(P, ptb[g0 + g], idx)) : 0.0;
+        a0[g] = reduce_f64(b0 + pb, q, qinv);
+        a1[g] = reduce_f64(b1, q, qinv);
+    }
+    for (u32 r = 0; r < R; ++r) {
+        const double x0 = u64_to_f64(P[ct0[r] + idx]), x1 = u64_to_f64(P[ct1[r] + idx]);
+#pragma unroll
+        for (u32 g = 0; g < LIN_G; ++g) {
+            if (g0 + g < G) {
+                const double p = u64_to_f64(P[pt[(size_t)(g0 + g) * R + r] + idx]);
+                a0[g] += mulmod_vv_f64(x0, p, q, qinv);
+                a1[g] += mulmod_vv_f64(x1, p, q, qinv);
+            }
+        }
+        if ((r & 7) == 7) {
+#pragma unroll
+            for (u32 g = 0; g < LIN_G; ++g) {
+                a0[g] = reduce_f64(a0[g], q, qinv);
+                a1[g] = reduce_f64(a1[g], q, qinv);
+            }
+        }
+    }
+#pragma unroll
+    for (u32 g = 0; g < LIN_G; ++g) {
+        if (g0 + g < G) {
+            P[out0[g0 + g] + idx] = f64_to_canon(reduce_f64(a0[g], q, qinv), q);
+            P[out1[g0 + g] + idx] = f64_to_canon(reduce_f64(a1[g], q, qinv), q);
+        }
+    }
+}
+
+void launch_linear_sum(const DevTables &t, u64 *P, const u64 *offs_dev, u32 R, u32 G, u64 bias0, u64 bias1,
+                       u32 limbs, hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs, (G + LIN_G - 1) / LIN_G);
+    if (t.use_f64 && !(t.f64_off & 8))
+        hipLaunchKernelGGL(k_linear_sum_f64, grid, dim3(TPB), 0, s, P, offs_dev, R, G, bias0, bias1, t.mp, t.logN);
+    else
+        hipLaunchKernelGGL(k_linear_sum, grid, dim3(TPB), 0, s, P, offs_dev, R, G, bias0, bias1, t.mp, t.logN);
+}
+
 }  // namespace fhea

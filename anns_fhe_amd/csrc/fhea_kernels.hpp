@@ -124,4 +124,19 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
                             u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys,
                             u32 l, hipStream_t s);
 
+// Batched ciphertext addition: d = a + b (both polynomials), items of `limbs` limbs.
+struct AddArgs {
+    const u64 *a0, *a1, *b0, *b1;
+    u64 *d0, *d1;
+};
+void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u32 limbs, hipStream_t s);
+
+// Plaintext-matrix x encrypted-vector inner products (diagonal method):
+//   out_g = bias + ptb_g + sum_{r<R} ct_r (.) pt_{g,r}
+// offs (device, u64 word offsets into P): [ct0: R][ct1: R][pt: G*R][ptb: G][out0: G][out1: G];
+// ptb entries and bias0/bias1 may be LIN_NONE.
+static constexpr u64 LIN_NONE = ~0ull;
+void launch_linear_sum(const DevTables &t, u64 *P, const u64 *offs_dev, u32 R, u32 G, u64 bias0, u64 bias1,
+                       u32 limbs, hipStream_t s);
+
 }  // namespace fhea

@@ -97,6 +97,21 @@ class FheContext:
     def ntt_batch(self, src, dst, npolys, limbs, inverse=False):
         check(self.lib.fhea_ntt_batch(self.h, src, dst, npolys, limbs, 1 if inverse else 0), "ntt_batch")
 
+    def linear_sum(self, ct0, ct1, pt, out0, out1, limbs, ptb=None, bias=None):
+        """out_g = bias + ptb[g] + sum_r ct_r (.) pt[g][r]; pt is a G x R array of offsets."""
+        pt = np.ascontiguousarray(pt, dtype=np.uint64)
+        G, R = pt.shape
+        c0 = np.ascontiguousarray(ct0, dtype=np.uint64)
+        c1 = np.ascontiguousarray(ct1, dtype=np.uint64)
+        o0 = np.ascontiguousarray(out0, dtype=np.uint64)
+        o1 = np.ascontiguousarray(out1, dtype=np.uint64)
+        assert c0.size == R and c1.size == R and o0.size == G and o1.size == G
+        pb = np.ascontiguousarray(ptb, dtype=np.uint64) if ptb is not None else None
+        b0, b1 = bias if bias is not None else (_lib.NONE, _lib.NONE)
+        check(self.lib.fhea_eval_linear_sum(self.h, _p64(c0), _p64(c1), R, _p64(pt),
+                                            _p64(pb) if pb is not None else C.cast(None, _lib.u64p), G, b0, b1,
+                                            _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum")
+
     def sync(self):
         check(self.lib.fhea_sync(self.h), "fhea_sync")
 

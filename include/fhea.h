@@ -153,6 +153,8 @@ int fhea_eval_automorphism_ks(fhea_ctx *ctx, uint64_t galois, uint64_t evk, uint
 #define FHEA_WORK_AUTOMORPHISM_KS 2    /* a0,a1 -> d0,d1 with galois, evk         */
 #define FHEA_WORK_NTT 3                /* a0 -> d0 (limbs), forward               */
 #define FHEA_WORK_INTT 4               /* a0 -> d0 (limbs), inverse               */
+#define FHEA_WORK_MULT_RELIN 5         /* a0,a1,b0,b1 -> d0,d1, no rescale        */
+#define FHEA_WORK_ADD 6                /* (a0,a1) + (b0,b1) -> (d0,d1)            */
 typedef struct {
     uint32_t kind;
     uint32_t limbs;
@@ -171,6 +173,22 @@ int fhea_ntt_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t npolys, u
                    int inverse);
 /* Wait for all work issued on the context's stream. */
 int fhea_sync(fhea_ctx *ctx);
+
+/* Sentinel for an absent optional offset. */
+#define FHEA_NONE (~(uint64_t)0)
+
+/* Plaintext-matrix x encrypted-vector inner products by the diagonal method:
+ *   out_g = bias + ptb[g] + sum_{r<R} ct_r (.) pt[g*R + r]     (g < G, on `limbs` limbs)
+ * ct_r = (ct0[r], ct1[r]) (typically the rotations of one query by r), pt and ptb
+ * plaintext offsets (ptb added to c0 only; ptb may be NULL or hold FHEA_NONE),
+ * bias = (bias0, bias1) a ciphertext or FHEA_NONE twice.  Outputs (out0[g], out1[g])
+ * must not overlap any input or each other (FHEA_ERR_ARG).  No reference
+ * counterpart: it replaces the per-batch rotate-and-sum loop of
+ * cpp_host_model/src/interactive_query_manager.cpp:190-274 with one pass per
+ * candidate block. */
+int fhea_eval_linear_sum(fhea_ctx *ctx, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,
+                         const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,
+                         uint64_t bias1, const uint64_t *out0, const uint64_t *out1, uint32_t limbs);
 
 /* ---- timing on the context's stream (HIP events) ----------------------- */
 int fhea_timer_start(fhea_ctx *ctx);

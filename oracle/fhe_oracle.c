@@ -872,3 +872,65 @@ int orc_eval_automorphism_ks(const orc_ctx *c, uint64_t *pm, uint64_t pw, const 
     free(o0); free(o1);
     return ST_OK;
 }
+
+int orc_eval_mult_relin(const orc_ctx *c, uint64_t *pm, uint64_t pw, const uint64_t *km, uint64_t kw,
+                        uint64_t relin_evk, uint32_t key_limbs, uint64_t a0, uint64_t a1, uint64_t b0,
+                        uint64_t b1, uint64_t d0, uint64_t d1, uint32_t L) {
+    if (L < 1 || L > c->L) return ST_ARG;
+    uint32_t N = c->N;
+    uint32_t kl = key_limbs ? key_limbs : L;
+    if (kl < L) return ST_ARG;
+    uint64_t W = (uint64_t)L * N;
+    CHECK_RANGE(a0, W); CHECK_RANGE(a1, W); CHECK_RANGE(b0, W); CHECK_RANGE(b1, W);
+    CHECK_RANGE(d0, W); CHECK_RANGE(d1, W);
+    if ((relin_evk & 7) || !in_range(relin_evk, 2ull * c->dnum * (kl + c->K) * N, kw)) return ST_RANGE;
+    uint64_t *D0 = (uint64_t *)malloc(W * 8), *D1 = (uint64_t *)malloc(W * 8);
+    uint64_t *D2 = (uint64_t *)malloc(W * 8), *T = (uint64_t *)malloc(W * 8);
+    uint64_t *KS = (uint64_t *)malloc(2 * W * 8);
+    poly_binop(c, 2, pm + a0, pm + b0, D0, L);
+    poly_binop(c, 2, pm + a1, pm + b1, D2, L);
+    poly_binop(c, 2, pm + a0, pm + b1, D1, L);
+    poly_binop(c, 2, pm + a1, pm + b0, T, L);
+    poly_binop(c, 0, D1, T, D1, L);
+    orc_key_switch(c, D2, km, relin_evk, kl, KS, KS + W, L);
+    poly_binop(c, 0, D0, KS, pm + d0, L);
+    poly_binop(c, 0, D1, KS + W, pm + d1, L);
+    free(D0); free(D1); free(D2); free(T); free(KS);
+    return ST_OK;
+}
+
+int orc_linear_sum(const orc_ctx *c, uint64_t *pm, uint64_t pw, const uint64_t *ct0, const uint64_t *ct1,
+                   uint32_t R, const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,
+                   uint64_t bias1, const uint64_t *out0, const uint64_t *out1, uint32_t L) {
+    const uint64_t NONE = ~(uint64_t)0;
+    if (L < 1 || L > c->L || R == 0 || G == 0) return ST_ARG;
+    uint64_t W = (uint64_t)L * c->N;
+    if (bias0 != NONE) { CHECK_RANGE(bias0, W); CHECK_RANGE(bias1, W); }
+    for (uint32_t r = 0; r < R; ++r) { CHECK_RANGE(ct0[r], W); CHECK_RANGE(ct1[r], W); }
+    for (uint32_t g = 0; g < G; ++g) {
+        CHECK_RANGE(out0[g], W); CHECK_RANGE(out1[g], W);
+        if (ptb && ptb[g] != NONE) CHECK_RANGE(ptb[g], W);
+        for (uint32_t r = 0; r < R; ++r) CHECK_RANGE(pt[(uint64_t)g * R + r], W);
+    }
+    uint64_t *A0 = (uint64_t *)malloc(W * 8), *A1 = (uint64_t *)malloc(W * 8), *T = (uint64_t *)malloc(W * 8);
+    for (uint32_t g = 0; g < G; ++g) {
+        memset(A0, 0, W * 8);
+        memset(A1, 0, W * 8);
+        if (bias0 != NONE) {
+            memcpy(A0, pm + bias0, W * 8);
+            memcpy(A1, pm + bias1, W * 8);
+        }
+        if (ptb && ptb[g] != NONE) poly_binop(c, 0, A0, pm + ptb[g], A0, L);
+        for (uint32_t r = 0; r < R; ++r) {
+            uint64_t p = pt[(uint64_t)g * R + r];
+            poly_binop(c, 2, pm + ct0[r], pm + p, T, L);
+            poly_binop(c, 0, A0, T, A0, L);
+            poly_binop(c, 2, pm + ct1[r], pm + p, T, L);
+            poly_binop(c, 0, A1, T, A1, L);
+        }
+        memcpy(pm + out0[g], A0, W * 8);
+        memcpy(pm + out1[g], A1, W * 8);
+    }
+    free(A0); free(A1); free(T);
+    return ST_OK;
+}

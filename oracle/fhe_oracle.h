@@ -125,6 +125,20 @@ int orc_eval_automorphism_ks(const orc_ctx *c, uint64_t *poly_mem, uint64_t poly
                              uint64_t evk_off, uint32_t key_limbs, uint64_t in0,
                              uint64_t in1, uint64_t d0, uint64_t d1, uint32_t limbs);
 
+/* The multiply + relinearise half of eval_mult_relin_rescale
+ * (fhe_host_driver.cpp:280-294): d = (a0 b0 + ks0(a1 b1), a0 b1 + a1 b0 + ks1(a1 b1)),
+ * at `limbs` limbs, no rescale. */
+int orc_eval_mult_relin(const orc_ctx *c, uint64_t *poly_mem, uint64_t poly_words,
+                        const uint64_t *key_mem, uint64_t key_words, uint64_t relin_evk,
+                        uint32_t key_limbs, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                        uint64_t d0, uint64_t d1, uint32_t limbs);
+/* out_g = bias + ptb[g] + sum_r ct_r (.) pt[g*R + r] by OP 5 / OP 3 per term
+ * (poly_arith.cpp:3-42); FHEA_NONE (all ones) marks an absent bias / ptb entry. */
+int orc_linear_sum(const orc_ctx *c, uint64_t *poly_mem, uint64_t poly_words, const uint64_t *ct0,
+                   const uint64_t *ct1, uint32_t R, const uint64_t *pt, const uint64_t *ptb,
+                   uint32_t G, uint64_t bias0, uint64_t bias1, const uint64_t *out0,
+                   const uint64_t *out1, uint32_t limbs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,10 @@ def lib():
                                           u64p, u32p] + [C.c_uint64] * 4 + [C.c_int32, C.c_uint32]),
             "orc_eval_automorphism_ks": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
                                                    C.c_uint64, C.c_uint32] + [C.c_uint64] * 4 + [C.c_uint32]),
+            "orc_eval_mult_relin": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
+                                              C.c_uint32] + [C.c_uint64] * 6 + [C.c_uint32]),
+            "orc_linear_sum": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, u64p, C.c_uint32, u64p, u64p,
+                                         C.c_uint32, C.c_uint64, C.c_uint64, u64p, u64p, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -254,3 +258,21 @@ class OracleCtx:
     def eval_automorphism_ks(self, pm, km, galois, evk, key_limbs, in0, in1, d0, d1, limbs):
         return lib().orc_eval_automorphism_ks(self.ptr, P64(pm), pm.size, P64(km), km.size, galois, evk,
                                               key_limbs, in0, in1, d0, d1, limbs)
+
+    def eval_mult_relin(self, pm, km, relin_evk, key_limbs, a0, a1, b0, b1, d0, d1, limbs):
+        return lib().orc_eval_mult_relin(self.ptr, P64(pm), pm.size, P64(km), km.size, relin_evk, key_limbs,
+                                         a0, a1, b0, b1, d0, d1, limbs)
+
+    def linear_sum(self, pm, ct0, ct1, pt, out0, out1, limbs, ptb=None, bias=None):
+        """pt: G x R offsets; ptb: G offsets or None; bias: (b0, b1) or None."""
+        none = (1 << 64) - 1
+        pt = np.ascontiguousarray(pt, dtype=np.uint64)
+        G, R = pt.shape
+        c0 = np.ascontiguousarray(ct0, dtype=np.uint64)
+        c1 = np.ascontiguousarray(ct1, dtype=np.uint64)
+        o0 = np.ascontiguousarray(out0, dtype=np.uint64)
+        o1 = np.ascontiguousarray(out1, dtype=np.uint64)
+        pb = np.ascontiguousarray(ptb if ptb is not None else [none] * G, dtype=np.uint64)
+        b0, b1 = bias if bias is not None else (none, none)
+        return lib().orc_linear_sum(self.ptr, P64(pm), pm.size, P64(c0), P64(c1), R, P64(pt), P64(pb), G, b0, b1,
+                                    P64(o0), P64(o1), limbs)

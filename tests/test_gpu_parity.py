@@ -392,3 +392,72 @@ def test_cpp_host_driver_c1_sequence(orc, tmp_path):
     assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 5 * W, 6 * W, 7 * W, 8 * W, 1, L - 1) == 0
     assert np.array_equal(rot[:W1], pm[7 * W:7 * W + W1])
     assert np.array_equal(rot[W1:], pm[8 * W:8 * W + W1])
+
+
+@pytest.mark.parametrize("q_bits", [50, 60])
+def test_linear_sum_parity(orc, q_bits):
+    """fhea_eval_linear_sum (FP64 path at 50-bit primes, integer path at 60-bit)
+    against the oracle's OP 5 / OP 3 composition, with bias and per-output plaintexts."""
+    from anns_fhe_amd._lib import NONE, FheError
+    N, L = 4096, 3
+    ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(q_bits)
+    W = L * N
+    R, G = 20, 11
+    n_slots = 2 * R + G * R + G + 2 + 2 * G
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(2 * R + G * R + G + 2)])
+    data[:N] = ctx.q[0] - 1
+    assert n_slots * W <= ctx.poly_words
+    ctx.write_poly(0, data)
+    ct0 = [2 * r * W for r in range(R)]
+    ct1 = [(2 * r + 1) * W for r in range(R)]
+    pt = np.array([[(2 * R + g * R + r) * W for r in range(R)] for g in range(G)], np.uint64)
+    base = 2 * R + G * R
+    ptb = [(base + g) * W if g % 3 else NONE for g in range(G)]
+    bias = ((base + G) * W, (base + G + 1) * W)
+    out0 = [(base + G + 2 + 2 * g) * W for g in range(G)]
+    out1 = [(base + G + 3 + 2 * g) * W for g in range(G)]
+    say("linear sum", q_bits)
+    ctx.linear_sum(ct0, ct1, pt, out0, out1, L, ptb=ptb, bias=bias)
+    got = ctx.read_poly(out0[0], 2 * G * W)
+    pm = np.zeros(n_slots * W, np.uint64)
+    pm[:data.size] = data
+    assert o.linear_sum(pm, ct0, ct1, pt, out0, out1, L, ptb=ptb, bias=bias) == 0
+    assert np.array_equal(got, pm[out0[0]:out0[0] + 2 * G * W])
+    # no bias, one output, two limbs of the same words
+    ctx.linear_sum(ct0, ct1, pt[:1], out0[:1], out1[:1], 2)
+    assert o.linear_sum(pm, ct0, ct1, pt[:1], out0[:1], out1[:1], 2) == 0
+    assert np.array_equal(ctx.read_poly(out0[0], 2 * N), pm[out0[0]:out0[0] + 2 * N])
+    # an output overlapping an input is refused
+    with pytest.raises(FheError) as ei:
+        ctx.linear_sum(ct0, ct1, pt[:1], [ct0[3]], out1[:1], L)
+    assert ei.value.status == -4
+
+
+def test_submit_mult_relin_and_add_parity(orc):
+    from anns_fhe_amd.driver import FHEHostDriver
+    N, L = 4096, 4
+    ctx = get_ctx(N, L, 2, 2)
+    drv = FHEHostDriver(ctx)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(21)
+    kidx = list(range(L)) + [L + j for j in range(ctx.K)]
+    key = np.concatenate([rand_poly(rng, [ctx.primes[i] for i in kidx], N).reshape(-1) for _ in range(4)])
+    ctx.write_key(0, key)
+    drv.set_relin_key(0, L)
+    W = L * N
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(8)])
+    ctx.write_poly(0, data)
+    for limbs in (L, 2):
+        works = [dict(kind=5, limbs=limbs, a0=0, a1=W, b0=2 * W, b1=3 * W, d0=8 * W, d1=9 * W),
+                 dict(kind=5, limbs=limbs, a0=4 * W, a1=5 * W, b0=6 * W, b1=7 * W, d0=10 * W, d1=11 * W),
+                 dict(kind=6, limbs=limbs, a0=8 * W, a1=9 * W, b0=10 * W, b1=11 * W, d0=12 * W, d1=13 * W)]
+        ctx.submit(works)
+        pm = np.zeros(14 * W, np.uint64)
+        pm[:8 * W] = data
+        assert o.eval_mult_relin(pm, key, 0, L, 0, W, 2 * W, 3 * W, 8 * W, 9 * W, limbs) == 0
+        assert o.eval_mult_relin(pm, key, 0, L, 4 * W, 5 * W, 6 * W, 7 * W, 10 * W, 11 * W, limbs) == 0
+        assert o.eval_add(pm, 8 * W, 9 * W, 10 * W, 11 * W, 12 * W, 13 * W, limbs) == 0
+        for off in (8, 9, 10, 11, 12, 13):
+            assert np.array_equal(ctx.read_poly(off * W, limbs * N), pm[off * W:off * W + limbs * N]), (limbs, off)

@@ -217,3 +217,56 @@ def test_composites_decrypt(ctx):
     assert max(abs(g - e) for g, e in zip(got_r, exp_r)) < 2 ** 12
     # a missing step key is an error (fhe_host_driver.cpp:336-339)
     assert ctx.eval_rotate(pm, km, keys, 5 * W, 6 * W, 7 * W, 8 * W, 4, L - 1) == -5
+
+
+def test_linear_sum_is_op5_op3_composition(orc):
+    """orc_linear_sum equals OP 5 (poly_mul) and OP 3 (poly_add) applied term by term
+    through the op dispatcher, with bias and per-output plaintexts."""
+    N, L = 1024, 2
+    q = orc.gen_primes(2 * N, 50, 3)
+    o = orc.OracleCtx(N, q[:L], q[L:], 1)
+    rng = np.random.default_rng(8)
+    W = L * N
+    R, G = 3, 2
+    pm = np.zeros(40 * W, np.uint64)
+    for k in range(20):
+        pm[k * W:(k + 1) * W] = np.concatenate([rng.integers(0, p, N, dtype=np.uint64) for p in q[:L]])
+    ct0, ct1 = [0, 2 * W, 4 * W], [W, 3 * W, 5 * W]
+    pt = np.array([[6 * W, 7 * W, 8 * W], [9 * W, 10 * W, 11 * W]], np.uint64)
+    ptb = [12 * W, (1 << 64) - 1]
+    bias = (13 * W, 14 * W)
+    out0, out1 = [20 * W, 22 * W], [21 * W, 23 * W]
+    assert o.linear_sum(pm, ct0, ct1, pt, out0, out1, L, ptb=ptb, bias=bias) == 0
+    ref = pm.copy()
+    T = 30 * W
+    for g in range(G):
+        a0, a1 = 24 * W, 25 * W
+        ref[a0:a0 + W] = ref[bias[0]:bias[0] + W]
+        ref[a1:a1 + W] = ref[bias[1]:bias[1] + W]
+        if ptb[g] != (1 << 64) - 1:
+            assert o.accel_op(ref, None, 3, a0, ptb[g], a0, 0, L) == 0
+        for r in range(R):
+            for c, a in ((ct0[r], a0), (ct1[r], a1)):
+                assert o.accel_op(ref, None, 5, c, int(pt[g, r]), T, 0, L) == 0
+                assert o.accel_op(ref, None, 3, a, T, a, 0, L) == 0
+        assert np.array_equal(pm[out0[g]:out0[g] + W], ref[a0:a0 + W])
+        assert np.array_equal(pm[out1[g]:out1[g] + W], ref[a1:a1 + W])
+
+
+def test_mult_relin_then_rescale_is_mult_relin_rescale(orc):
+    N, L, K = 1024, 3, 2
+    q = orc.gen_primes(2 * N, 50, L + K)
+    o = orc.OracleCtx(N, q[:L], q[L:], 3)
+    rng = np.random.default_rng(12)
+    W = L * N
+    km = np.concatenate([rng.integers(0, p, N, dtype=np.uint64) for _ in range(2 * 3) for p in q])
+    pm = np.zeros(12 * W, np.uint64)
+    for k in range(4):
+        pm[k * W:(k + 1) * W] = np.concatenate([rng.integers(0, p, N, dtype=np.uint64) for p in q[:L]])
+    assert o.eval_mult_relin(pm, km, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+    r0 = o.rescale(pm[4 * W:5 * W]).reshape(-1)
+    r1 = o.rescale(pm[5 * W:6 * W]).reshape(-1)
+    assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 6 * W, 7 * W, L) == 0
+    W1 = (L - 1) * N
+    assert np.array_equal(r0, pm[6 * W:6 * W + W1])
+    assert np.array_equal(r1, pm[7 * W:7 * W + W1])

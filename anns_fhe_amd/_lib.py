@@ -105,6 +105,8 @@ SIGNATURES = {
     "fhea_client_relin_key": (C.c_int, [vp, i8p, C.c_uint32, C.c_uint64, u64p]),
     "fhea_client_rotation_key": (C.c_int, [vp, i8p, C.c_uint64, C.c_uint32, C.c_uint64, u64p]),
     "fhea_encode": (C.c_int, [vp, f64p, f64p, C.c_uint32, C.c_uint32, C.c_double, u64p]),
+    "fhea_encode_batch": (C.c_int, [vp, f64p, f64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, u64p,
+                                    C.c_uint32]),
     "fhea_decode": (C.c_int, [vp, u64p, C.c_uint32, C.c_double, C.c_uint32, f64p, f64p]),
     "fhea_keygen_public": (C.c_int, [vp, i8p, C.c_uint64, u64p]),
     "fhea_encrypt": (C.c_int, [vp, u64p, u64p, C.c_uint32, C.c_uint64, u64p]),

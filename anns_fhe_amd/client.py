@@ -104,6 +104,15 @@ class CkksClient:
                                    n, limbs, scale, _p64(pt)), "fhea_encode")
         return pt
 
+    def encode_batch(self, values, limbs, scale, threads=0):
+        """values: (count, n) real array -> (count, limbs*N) plaintexts, encoded on host threads."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        count, n = v.shape
+        pt = np.zeros((count, limbs * self.N), np.uint64)
+        check(self.lib.fhea_encode_batch(self.h, _pf(v), C.cast(None, _lib.f64p), count, n, limbs, scale,
+                                         _p64(pt), threads), "fhea_encode_batch")
+        return pt
+
     def decode(self, pt, limbs, scale, n, complex_out=False):
         pt = np.ascontiguousarray(pt, dtype=np.uint64)
         re = np.zeros(n, np.float64)

@@ -29,7 +29,9 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <algorithm>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "fhea_engine.hpp"
@@ -255,6 +257,51 @@ void slot_index(u32 N, u32 j, u32 &t, u32 &tc) {
 
 bool valid_slots(u32 N, u32 n) { return n >= 1 && n <= N / 2 && (n & (n - 1)) == 0; }
 
+// Integer-valued double -> residue mod q, exact for any magnitude: |v| = mant * 2^e
+// with a 53-bit integer mantissa.
+u64 double_to_residue(double v, u64 q) {
+    const double a = std::fabs(v);
+    u64 r;
+    if (a < 4.0e18) {
+        r = (u64)(int64_t)a % q;
+    } else {
+        int e = 0;
+        const double m = std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+        const u64 mant = (u64)std::ldexp(m, 53);
+        r = (u64)((u128)(mant % q) * pow_mod(2, (u64)(e - 53), q) % q);
+    }
+    return (v < 0 && r) ? q - r : r;
+}
+
+// Encodes n values (replicated to N/2 slots) into `limbs` evaluation-domain limbs.
+int encode_one(const HostTables &h, const double *re, const double *im, u32 n, u32 limbs, double scale, u64 *pt) {
+    const u32 N = h.N;
+    std::vector<cplx> Y(N);
+    for (u32 j = 0; j < N / 2; ++j) {
+        const cplx z(re[j % n] * scale, im ? im[j % n] * scale : 0.0);
+        u32 t, tc;
+        slot_index(N, j, t, tc);
+        Y[t] = z;
+        Y[tc] = std::conj(z);
+    }
+    dft(Y, -1);
+    std::vector<double> m(N);
+    for (u32 k = 0; k < N; ++k) {
+        const double ang = -kPi * (double)k / (double)N;  // zeta^-k
+        const double v = std::nearbyint((Y[k] * cplx(std::cos(ang), std::sin(ang))).real() / (double)N);
+        if (!std::isfinite(v) || std::fabs(v) > 0x1p1000) return FHEA_ERR_ARG;
+        m[k] = v;
+    }
+    HostNtt ntt(h);
+    for (u32 i = 0; i < limbs; ++i) {
+        const u64 q = h.primes[i];
+        u64 *row = pt + (size_t)i * N;
+        for (u32 k = 0; k < N; ++k) row[k] = double_to_residue(m[k], q);
+        ntt.forward(row, i);
+    }
+    return FHEA_OK;
+}
+
 // centred integer value of residues x[0..l) (Garner), as a double
 struct Crt {
     std::vector<u64> q;
@@ -417,26 +464,35 @@ int fhea_encode(const fhea_client *cl, const double *re, const double *im, uint3
                 double scale, uint64_t *pt) {
     if (!cl) return FHEA_ERR_STATE;
     const HostTables &h = cl->h;
-    const u32 N = h.N;
-    if (!re || !pt || !valid_slots(N, n) || limbs == 0 || limbs > h.L || !(scale > 0.0)) return FHEA_ERR_ARG;
-    std::vector<cplx> Y(N);
-    for (u32 j = 0; j < N / 2; ++j) {
-        const cplx z(re[j % n] * scale, im ? im[j % n] * scale : 0.0);
-        u32 t, tc;
-        slot_index(N, j, t, tc);
-        Y[t] = z;
-        Y[tc] = std::conj(z);
-    }
-    dft(Y, -1);
-    std::vector<int64_t> m(N);
-    for (u32 k = 0; k < N; ++k) {
-        const double ang = -kPi * (double)k / (double)N;  // zeta^-k
-        const double v = (Y[k] * cplx(std::cos(ang), std::sin(ang))).real() / (double)N;
-        if (!(std::fabs(v) < 4.0e18)) return FHEA_ERR_ARG;  // |m_k| must fit in int64
-        m[k] = (int64_t)std::llround(v);
-    }
-    std::vector<u64> e = to_eval(h, m.data(), q_primes(limbs));
-    std::memcpy(pt, e.data(), e.size() * 8);
+    if (!re || !pt || !valid_slots(h.N, n) || limbs == 0 || limbs > h.L || !(scale > 0.0)) return FHEA_ERR_ARG;
+    return encode_one(h, re, im, n, limbs, scale, pt);
+}
+
+int fhea_encode_batch(const fhea_client *cl, const double *re, const double *im, uint32_t count, uint32_t n,
+                      uint32_t limbs, double scale, uint64_t *pt, uint32_t threads) {
+    if (!cl) return FHEA_ERR_STATE;
+    const HostTables &h = cl->h;
+    if (!re || !pt || !valid_slots(h.N, n) || limbs == 0 || limbs > h.L || !(scale > 0.0)) return FHEA_ERR_ARG;
+    if (count == 0) return FHEA_OK;
+    u32 T = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    T = std::min<u32>(T, count);
+    std::vector<int> st(T, FHEA_OK);
+    std::vector<std::thread> pool;
+    const size_t W = (size_t)limbs * h.N;
+    for (u32 t = 0; t < T; ++t)
+        pool.emplace_back([&, t]() {
+            for (u32 k = t; k < count; k += T) {
+                const int r = encode_one(h, re + (size_t)k * n, im ? im + (size_t)k * n : nullptr, n, limbs, scale,
+                                         pt + k * W);
+                if (r != FHEA_OK) {
+                    st[t] = r;
+                    return;
+                }
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (int r : st)
+        if (r != FHEA_OK) return r;
     return FHEA_OK;
 }
 

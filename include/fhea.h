@@ -235,9 +235,14 @@ int fhea_client_relin_key(const fhea_client *cl, const int8_t *s, uint32_t key_l
                           uint64_t *key_out);
 int fhea_client_rotation_key(const fhea_client *cl, const int8_t *s, uint64_t galois,
                              uint32_t key_limbs, uint64_t seed, uint64_t *key_out);
-/* re/im: n values (im may be NULL); pt: limbs*N words; |coefficients| < 4e18 or FHEA_ERR_ARG */
+/* re/im: n values (im may be NULL); pt: limbs*N words.  Coefficients are rounded
+ * to integers and reduced exactly (any scale up to 2^1000). */
 int fhea_encode(const fhea_client *cl, const double *re, const double *im, uint32_t n,
                 uint32_t limbs, double scale, uint64_t *pt);
+/* count vectors of n values (re[k*n..], im likewise or NULL) -> count plaintexts of
+ * limbs*N words, on `threads` host threads (0 = all cores) */
+int fhea_encode_batch(const fhea_client *cl, const double *re, const double *im, uint32_t count,
+                      uint32_t n, uint32_t limbs, double scale, uint64_t *pt, uint32_t threads);
 /* first n slots of pt (limbs*N words); im may be NULL */
 int fhea_decode(const fhea_client *cl, const uint64_t *pt, uint32_t limbs, double scale,
                 uint32_t n, double *re, double *im);

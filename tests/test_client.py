@@ -146,4 +146,20 @@ def test_client_errors():
     with pytest.raises(FheError):
         cl.encode(np.ones(4), 3, 2.0 ** 20)  # more limbs than L
     with pytest.raises(FheError):
-        cl.encode(np.full(4, 1e10), 2, 2.0 ** 60)  # coefficients beyond int64
+        cl.encode(np.full(4, np.inf), 2, 2.0 ** 20)  # not finite
+
+
+def test_large_scale_encoding_and_batch():
+    """Scales beyond 2^62 (a product scale of two 2^45 factors) encode exactly;
+    the threaded batch encoder equals one-by-one encoding."""
+    N, L = 1024, 3
+    cl = _client(N, L, 1, 3)
+    rng = np.random.default_rng(2)
+    z = rng.uniform(0, 500, 64)
+    scale = 2.0 ** 90
+    pt = cl.encode(z, L, scale)
+    assert np.max(np.abs(cl.decode(pt, L, scale, 64) - z)) < 1e-9
+    vals = rng.uniform(-2, 2, (7, 128))
+    batch = cl.encode_batch(vals, 2, 2.0 ** 45, threads=3)
+    for k in range(7):
+        assert np.array_equal(batch[k], cl.encode(vals[k], 2, 2.0 ** 45))

@@ -1143,6 +1143,28 @@ int fhea_eval_linear_sum(fhea_ctx *c, const uint64_t *ct0, const uint64_t *ct1, 
     return check_hip(hipGetLastError());
 }
 
+int fhea_rescale_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (limbs < 2 || limbs > c->h.L || n == 0) return FHEA_ERR_ARG;
+    const u64 N = c->h.N, W = (u64)limbs * N, W1 = (u64)(limbs - 1) * N;
+    POLY_CHECK(c, src, 2 * W * n);
+    POLY_CHECK(c, dst, 2 * W1 * n);
+    if (src < dst + 2 * W1 * n && dst < src + 2 * W * n) return FHEA_ERR_ARG;
+    for (u32 off = 0; off < n; off += 64) {
+        const u32 nb = std::min<u32>(64, n - off);
+        int st = ensure_ws(c, nb * rescale_words(c, limbs));
+        if (st) return st;
+        std::vector<u64 *> outs(2 * nb);
+        for (u32 z = 0; z < nb; ++z) {
+            outs[2 * z] = c->poly + dst + (u64)(off + z) * 2 * W1;
+            outs[2 * z + 1] = outs[2 * z] + W1;
+        }
+        st = rescale_core(c, limbs, nb, c->poly + src + (u64)off * 2 * W, 2 * W, W, outs, c->ws);
+        if (st) return st;
+    }
+    return check_hip(hipGetLastError());
+}
+
 int fhea_ntt_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t npolys, uint32_t limbs,
                    int inverse) {
     CTX_CHECK(c);

@@ -112,6 +112,9 @@ class FheContext:
                                             _p64(pb) if pb is not None else C.cast(None, _lib.u64p), G, b0, b1,
                                             _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum")
 
+    def rescale_batch(self, src, dst, n, limbs):
+        check(self.lib.fhea_rescale_batch(self.h, src, dst, n, limbs), "fhea_rescale_batch")
+
     def sync(self):
         check(self.lib.fhea_sync(self.h), "fhea_sync")
 

@@ -1,0 +1,389 @@
+"""IVF-PQ approximate nearest-neighbour search, in the clear and encrypted.
+
+Plaintext side (the reference's software_golden_model: data_loader.py,
+ivf_quantizer.py, product_quantizer.py, searcher.py:26-87, and its C++ twin
+fhe_core/plaintext_searcher.cpp): a seeded Gaussian-mixture dataset, IVF
+(k-means coarse quantiser) + PQ (per-subspace k-means on residuals) training,
+encoding to uint8 codes, and ADC search.  This is also the recall baseline.
+
+Encrypted side, interactive mode (cpp_host_model/src/interactive_query_manager.cpp:85-274,
+software_golden_model/fhe_core/fhe_searcher.cpp:129-230,377-409): the client
+encrypts its query, the server evaluates squared distances to the coarse
+centroids, the client decrypts them and picks n_probe clusters, the server
+evaluates the distances of every candidate in those clusters, the client
+decrypts them and keeps the top k.
+
+The distance of candidate x_s (the PQ reconstruction c + r_hat, so that
+||q - x_s||^2 is exactly the ADC distance sum_m ||(q - c)_m - r_hat_m||^2) is
+evaluated as
+    ||q||^2 - 2 <q, x_s> + ||x_s||^2
+with the query replicated over the slots with period D:
+  * ||q||^2 in every slot: relinearised q*q, then rotate-and-sum with strides
+    1, 2, ..., D/2 (the reference's coarse tree sum, interactive_query_manager.cpp:111-124);
+  * -2 <q, x_s> for S = N/2 candidates at once by the diagonal method:
+    sum_{r<D} rot_r(q) (.) pt_r with pt_r[s] = -2 x_s[(s + r) mod D];
+    the D-1 rotations of the query are computed once per query and shared by
+    every candidate block (where the reference runs a 7-rotation tree sum, a
+    mask multiply and a position rotation per block of B = S/D candidates);
+  * ||x_s||^2 as a plaintext.
+Everything runs at `lt` limbs (the query is encrypted at L limbs and its upper
+limbs dropped); the result is rescaled once and returned at lt-1 limbs.
+"""
+import time
+
+import numpy as np
+
+from ._lib import NONE
+
+
+# ---------------------------------------------------------------------------
+# data and index
+# ---------------------------------------------------------------------------
+def make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42):
+    """n float32 vectors from n_centres isotropic Gaussians (centres uniform in [-1, 1]^dim)."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-1.0, 1.0, (n_centres, dim)).astype(np.float32)
+    labels = rng.integers(0, n_centres, n)
+    x = centres[labels] + rng.normal(0.0, sigma, (n, dim)).astype(np.float32)
+    return x.astype(np.float32), centres
+
+
+def _nearest(x, c, chunk=65536):
+    """Index of the nearest row of c for every row of x (squared L2)."""
+    cn = (c.astype(np.float64) ** 2).sum(1)
+    out = np.empty(len(x), np.int64)
+    for i in range(0, len(x), chunk):
+        xb = x[i:i + chunk].astype(np.float64)
+        d = cn[None, :] - 2.0 * xb @ c.T.astype(np.float64)
+        out[i:i + chunk] = np.argmin(d, axis=1)
+    return out
+
+
+def kmeans(x, k, iters=20, seed=0):
+    """Lloyd's k-means, seeded initialisation from distinct data points."""
+    rng = np.random.default_rng(seed)
+    c = x[rng.choice(len(x), k, replace=False)].astype(np.float64)
+    for _ in range(iters):
+        a = _nearest(x, c)
+        sums = np.zeros_like(c)
+        np.add.at(sums, a, x.astype(np.float64))
+        cnt = np.bincount(a, minlength=k)
+        empty = cnt == 0
+        c[~empty] = sums[~empty] / cnt[~empty, None]
+        if empty.any():
+            c[empty] = x[rng.choice(len(x), int(empty.sum()), replace=False)]
+    return c.astype(np.float32)
+
+
+class IvfPqIndex:
+    """IVF (n_list centroids) + PQ (M subquantisers of K centroids) over residuals."""
+
+    def __init__(self, n_list=32, M=8, K=256):
+        self.n_list, self.M, self.K = n_list, M, K
+
+    def train(self, x, train_size=65536, iters=20, seed=0):
+        rng = np.random.default_rng(seed)
+        t = x[rng.choice(len(x), min(train_size, len(x)), replace=False)]
+        self.dim = x.shape[1]
+        self.sub = self.dim // self.M
+        self.centroids = kmeans(t, self.n_list, iters, seed)
+        res = t - self.centroids[_nearest(t, self.centroids)]
+        self.codebooks = np.stack([kmeans(res[:, m * self.sub:(m + 1) * self.sub], self.K, iters, seed + 1 + m)
+                                   for m in range(self.M)])
+        return self
+
+    def add(self, x):
+        self.assign = _nearest(x, self.centroids).astype(np.int32)
+        res = x - self.centroids[self.assign]
+        self.codes = np.stack([_nearest(res[:, m * self.sub:(m + 1) * self.sub], self.codebooks[m])
+                               for m in range(self.M)], axis=1).astype(np.uint8)
+        self.lists = [np.flatnonzero(self.assign == c) for c in range(self.n_list)]
+        return self
+
+    def reconstruct(self, ids):
+        ids = np.asarray(ids)
+        r = np.concatenate([self.codebooks[m][self.codes[ids, m]] for m in range(self.M)], axis=1)
+        return self.centroids[self.assign[ids]] + r
+
+
+class PlaintextSearcher:
+    """ADC search over an IvfPqIndex (searcher.py:26-87, plaintext_searcher.cpp)."""
+
+    def __init__(self, index):
+        self.ix = index
+
+    def coarse(self, q):
+        return ((self.ix.centroids - q) ** 2).sum(1)
+
+    def search(self, q, n_probe, top_k):
+        ix = self.ix
+        probes = np.argsort(self.coarse(q), kind="stable")[:n_probe]
+        cand, dist = [], []
+        for c in probes:
+            ids = ix.lists[c]
+            qr = q - ix.centroids[c]
+            lut = np.stack([((ix.codebooks[m] - qr[m * ix.sub:(m + 1) * ix.sub]) ** 2).sum(1)
+                            for m in range(ix.M)])  # [M][K]
+            d = np.zeros(len(ids), np.float64)
+            for m in range(ix.M):
+                d += lut[m][ix.codes[ids, m]]
+            cand.append(ids)
+            dist.append(d)
+        cand = np.concatenate(cand) if cand else np.zeros(0, np.int64)
+        dist = np.concatenate(dist) if dist else np.zeros(0)
+        order = np.argsort(dist, kind="stable")[:top_k]
+        return cand[order], dist[order], probes
+
+
+def exact_knn(x, q, k):
+    d = ((x.astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+    return np.argsort(d, kind="stable")[:k]
+
+
+def recall_at(found, truth):
+    return len(set(int(i) for i in found) & set(int(i) for i in truth)) / max(1, len(truth))
+
+
+# ---------------------------------------------------------------------------
+# encrypted, interactive
+# ---------------------------------------------------------------------------
+class FheIvfPqClient:
+    """Client of the interactive protocol: keys, query encryption, decryption of
+    distances, cluster selection and top-k (fhe_context_manager.cpp:457-512,
+    interactive_query_manager.cpp:166-174,276-290)."""
+
+    def __init__(self, ckks, dim=128, log_scale=40, lt=3, seed=1):
+        self.ck = ckks
+        self.D = dim
+        self.scale = 2.0 ** log_scale
+        self.lt = lt
+        self.s = ckks.keygen_secret(seed)
+        self.pk = ckks.keygen_public(self.s, seed + 1)
+        self.seed = seed + 100
+
+    def evaluation_keys(self):
+        """Relinearisation key and rotation keys for r = 1..D-1, all at lt limbs."""
+        rel = self.ck.relin_key(self.s, self.lt, self.seed)
+        rots = {r: self.ck.rotation_key(self.s, self.ck.galois_for_step(r), self.lt, self.seed + r)
+                for r in range(1, self.D)}
+        return rel, rots
+
+    def encrypt_query(self, q, limbs=None):
+        limbs = limbs or self.ck.L
+        self.seed += 1
+        return self.ck.encrypt_values(self.pk, np.asarray(q, np.float64), limbs, self.scale, self.seed)
+
+    def result_scale(self, q_last):
+        return self.scale * self.scale / q_last
+
+    def decrypt_distances(self, ct, limbs, scale, n):
+        return self.ck.decrypt_values(self.s, ct, limbs, scale, n)
+
+
+class FheIvfPqServer:
+    """Server side: the encoded database resident in HBM, keys, and the batched
+    homomorphic stages.  Word layout of the poly arena:
+        [plaintext blocks | query slots]
+    block b: D plaintexts pt_r (lt limbs each) then ptb (||x_s||^2);
+    query slot: ct (L limbs x 2), D-1 rotations, norm, tmp, outputs, rescaled outputs."""
+
+    def __init__(self, ctx, ckks, index, lt=3, log_scale=40, max_queries=64, max_blocks_per_query=None,
+                 threads=0, log=None):
+        self.ctx, self.ck, self.ix = ctx, ckks, index
+        self.N, self.L, self.lt = ctx.N, ctx.L, lt
+        self.S = ctx.N // 2
+        self.D = index.dim
+        assert self.S % self.D == 0 and (self.D & (self.D - 1)) == 0
+        self.scale = 2.0 ** log_scale
+        self.log = log or (lambda *a: None)
+        Wl = lt * self.N
+        # ---- blocks: 0 = coarse centroids, then every cluster in chunks of S candidates
+        self.blocks = [("coarse", np.arange(index.n_list))]
+        self.cluster_blocks = []
+        for c in range(index.n_list):
+            ids = index.lists[c]
+            bl = []
+            for o in range(0, max(1, len(ids)), self.S):
+                if len(ids) == 0:
+                    break
+                bl.append(len(self.blocks))
+                self.blocks.append((c, ids[o:o + self.S]))
+            self.cluster_blocks.append(bl)
+        self.block_words = (self.D + 1) * Wl
+        if max_blocks_per_query is None:
+            per = sorted((len(b) for b in self.cluster_blocks), reverse=True)
+            max_blocks_per_query = max(1, sum(per[:8]))
+        self.max_g = max_blocks_per_query
+        self.pt_words = len(self.blocks) * self.block_words
+        self.ct_words = 2 * self.L * self.N
+        self.slot_words = (self.ct_words + (self.D - 1) * 2 * Wl + 4 * Wl + self.max_g * 2 * Wl
+                           + self.max_g * 2 * (lt - 1) * self.N)
+        self.max_queries = max_queries
+        need = self.pt_words + max_queries * self.slot_words
+        assert need <= ctx.poly_words, (need, ctx.poly_words)
+        self._encode_blocks(threads)
+
+    @staticmethod
+    def arena_words(N, L, lt, D, n_blocks, max_queries, max_g):
+        Wl = lt * N
+        slot = 2 * L * N + (D - 1) * 2 * Wl + 4 * Wl + max_g * 2 * Wl + max_g * 2 * (lt - 1) * N
+        return n_blocks * (D + 1) * Wl + max_queries * slot + 64
+
+    def _encode_blocks(self, threads):
+        D, S, lt = self.D, self.S, self.lt
+        t0 = time.perf_counter()
+        s = np.arange(S)
+        for b, (_, ids) in enumerate(self.blocks):
+            if b == 0:
+                x = self.ix.centroids.astype(np.float64)
+            else:
+                x = self.ix.reconstruct(ids).astype(np.float64)
+            n = len(x)
+            vals = np.zeros((D + 1, S), np.float64)
+            for r in range(D):
+                vals[r, :n] = -2.0 * x[s[:n], (s[:n] + r) % D]
+            vals[D, :n] = (x ** 2).sum(1)
+            pts = self.ck.encode_batch(vals[:D], lt, self.scale, threads)
+            ptb = self.ck.encode_batch(vals[D:], lt, self.scale * self.scale, threads)
+            base = b * self.block_words
+            self.ctx.write_poly(base, pts)
+            self.ctx.write_poly(base + D * lt * self.N, ptb)
+        self.log(f"encoded {len(self.blocks)} blocks in {time.perf_counter() - t0:.1f} s")
+
+    # ---- keys
+    def load_keys(self, relin_key, rot_keys, key_base=0):
+        ksz = relin_key.size
+        self.ctx.write_key(key_base, relin_key)
+        self.ctx.lib.fhea_set_relin_key(self.ctx.h, key_base, self.lt)
+        self.rot_off = {}
+        for k, (r, key) in enumerate(sorted(rot_keys.items())):
+            off = key_base + (1 + k) * ksz
+            self.ctx.write_key(off, key)
+            self.rot_off[r] = off
+
+    # ---- offsets inside a query slot
+    def slot_base(self, z):
+        return self.pt_words + z * self.slot_words
+
+    def _offs(self, z):
+        N, L, lt, D = self.N, self.L, self.lt, self.D
+        Wl = lt * N
+        b = self.slot_base(z)
+        o = {"q0": b, "q1": b + L * N}
+        o["rot"] = b + self.ct_words  # rotation r (1..D-1) at rot + (r-1)*2*Wl
+        o["norm"] = o["rot"] + (D - 1) * 2 * Wl
+        o["tmp"] = o["norm"] + 2 * Wl
+        o["out"] = o["tmp"] + 2 * Wl
+        o["res"] = o["out"] + self.max_g * 2 * Wl
+        return o
+
+    def upload(self, z, ct):
+        self.ctx.write_poly(self.slot_base(z), ct)
+
+    def prepare(self, slots):
+        """Rotations of every query by 1..D-1 and the encrypted ||q||^2 (all at lt limbs)."""
+        N, lt, D = self.N, self.lt, self.D
+        Wl = lt * N
+        works = []
+        for z in slots:
+            o = self._offs(z)
+            for r in range(1, D):
+                d = o["rot"] + (r - 1) * 2 * Wl
+                works.append(dict(kind=2, limbs=lt, key_limbs=lt, a0=o["q0"], a1=o["q1"], d0=d, d1=d + Wl,
+                                  galois=self.ck.galois_for_step(r), evk=self.rot_off[r]))
+        self.ctx.submit(works)
+        self.ctx.submit([dict(kind=5, limbs=lt, a0=self._offs(z)["q0"], a1=self._offs(z)["q1"],
+                              b0=self._offs(z)["q0"], b1=self._offs(z)["q1"], d0=self._offs(z)["norm"],
+                              d1=self._offs(z)["norm"] + Wl) for z in slots])
+        stride = 1
+        while stride < D:
+            rot, add = [], []
+            for z in slots:
+                o = self._offs(z)
+                rot.append(dict(kind=2, limbs=lt, key_limbs=lt, a0=o["norm"], a1=o["norm"] + Wl, d0=o["tmp"],
+                                d1=o["tmp"] + Wl, galois=self.ck.galois_for_step(stride), evk=self.rot_off[stride]))
+                add.append(dict(kind=6, limbs=lt, a0=o["norm"], a1=o["norm"] + Wl, b0=o["tmp"], b1=o["tmp"] + Wl,
+                                d0=o["norm"], d1=o["norm"] + Wl))
+            self.ctx.submit(rot)
+            self.ctx.submit(add)
+            stride *= 2
+
+    def distances(self, z, block_ids):
+        """Encrypted squared distances of the candidates of `block_ids` for query slot z;
+        leaves the rescaled ciphertexts (lt-1 limbs) at o['res'] + g*2*(lt-1)*N."""
+        N, lt, D = self.N, self.lt, self.D
+        Wl = lt * N
+        G = len(block_ids)
+        assert 0 < G <= self.max_g
+        o = self._offs(z)
+        ct0 = [o["q0"]] + [o["rot"] + (r - 1) * 2 * Wl for r in range(1, D)]
+        ct1 = [o["q1"]] + [o["rot"] + (r - 1) * 2 * Wl + Wl for r in range(1, D)]
+        pt = np.array([[b * self.block_words + r * Wl for r in range(D)] for b in block_ids], np.uint64)
+        ptb = [b * self.block_words + D * Wl for b in block_ids]
+        out0 = [o["out"] + g * 2 * Wl for g in range(G)]
+        out1 = [x + Wl for x in out0]
+        self.ctx.linear_sum(ct0, ct1, pt, out0, out1, lt, ptb=ptb, bias=(o["norm"], o["norm"] + Wl))
+        self.ctx.rescale_batch(o["out"], o["res"], G, lt)
+
+    def read_distances(self, z, G):
+        n = 2 * (self.lt - 1) * self.N
+        return self.ctx.read_poly(self._offs(z)["res"], G * n).reshape(G, n)
+
+    # ---- stages (interactive_query_manager.cpp:85-274)
+    def stage1_coarse_centroid_distance(self, slots):
+        self.prepare(slots)
+        for z in slots:
+            self.distances(z, [0])
+
+    def stage3_fine_candidate_batch_distance(self, z, clusters):
+        blocks = [b for c in clusters for b in self.cluster_blocks[c]]
+        if blocks:
+            self.distances(z, blocks)
+        return blocks
+
+
+def execute_interactive_queries(server, client, queries, n_probe, top_k):
+    """Runs the whole protocol for a batch of queries; returns per query
+    (ids, distances, probes) and a timing record."""
+    ck, N = client.ck, server.N
+    B = len(queries)
+    assert B <= server.max_queries
+    lt1 = server.lt - 1
+    q_last = ck.q[server.lt - 1]
+    scale = client.result_scale(q_last)
+    t = {}
+    t0 = time.perf_counter()
+    for z, q in enumerate(queries):
+        server.upload(z, client.encrypt_query(q))
+    t["client_encrypt_upload"] = time.perf_counter() - t0
+    slots = list(range(B))
+    t1 = time.perf_counter()
+    server.stage1_coarse_centroid_distance(slots)
+    coarse = [server.read_distances(z, 1)[0] for z in slots]
+    t["server_stage1"] = time.perf_counter() - t1
+    # stage 2 (client): decrypt the coarse distances, pick n_probe clusters
+    t2 = time.perf_counter()
+    probes = []
+    for z in slots:
+        d = client.decrypt_distances(coarse[z], lt1, scale, server.S)[:server.ix.n_list]
+        probes.append(np.argsort(d, kind="stable")[:n_probe])
+    t["client_stage2"] = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    blocks = [server.stage3_fine_candidate_batch_distance(z, probes[z]) for z in slots]
+    fine = [server.read_distances(z, len(blocks[z])) for z in slots]
+    t["server_stage3"] = time.perf_counter() - t3
+    t4 = time.perf_counter()
+    out = []
+    for z in slots:
+        ids, dist = [], []
+        for g, b in enumerate(blocks[z]):
+            cand = server.blocks[b][1]
+            d = client.decrypt_distances(fine[z][g], lt1, scale, server.S)[:len(cand)]
+            ids.append(cand)
+            dist.append(d)
+        ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
+        dist = np.concatenate(dist) if dist else np.zeros(0)
+        order = np.argsort(dist, kind="stable")[:top_k]
+        out.append((ids[order], dist[order], probes[z]))
+    t["client_stage4"] = time.perf_counter() - t4
+    return out, t

@@ -36,9 +36,12 @@ def parse():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle ops timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ntt-reps", type=int, default=5)
-    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64"],
+    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
-                         "ks64: 64 rotations with distinct resident keys")
+                         "ks64: 64 rotations with distinct resident keys; ivfpq: encrypted IVF-PQ "
+                         "search (1M x 128, n_list=32, M=8, K=256, B=64 queries, N=2^14, L=8, n_probe=4)")
+    ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
+    ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="single-ciphertext op-by-op HMult runs timed for plain_path (0 = skip)")
     return ap.parse_args()
@@ -199,6 +202,76 @@ def bench_ks64(args):
     }
 
 
+def bench_ivfpq(args):
+    """Encrypted IVF-PQ, interactive mode.  A step is one batch of B queries through
+    the whole protocol; the server's share (stages 1 and 3 with result readback) is
+    timed apart from the client's (encrypt, decrypt, cluster choice, top-k)."""
+    from anns_fhe_amd import ivfpq as IV
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.driver import FheContext
+    n, B, n_probe = args.db_size, args.queries, 4
+    Nq, Lq, Kq, dq, lt = 1 << 14, 8, 4, 2, 3
+    t0 = time.perf_counter()
+    x, _ = IV.make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42)
+    ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
+    t_index = time.perf_counter() - t0
+    log(f"ivfpq: dataset + index {t_index:.1f} s")
+    n_blocks = 1 + sum(-(-len(l) // (Nq // 2)) for l in ix.lists)
+    per = sorted((-(-len(l) // (Nq // 2)) for l in ix.lists), reverse=True)
+    max_g = sum(per[:n_probe])
+    words = IV.FheIvfPqServer.arena_words(Nq, Lq, lt, 128, n_blocks, B, max_g)
+    ctx = FheContext(Nq, Lq, Kq, dq, poly_words=words, key_words=128 * 2 * dq * (lt + Kq) * Nq + 8)
+    ck = CkksClient(0, 0, 0, 0, ctx=ctx)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
+    t1 = time.perf_counter()
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=max_g,
+                               log=log)
+    t_encode = time.perf_counter() - t1
+    rel, rots = client.evaluation_keys()
+    server.load_keys(rel, rots)
+    rng = np.random.default_rng(5)
+
+    def batch():
+        return [x[rng.integers(n)] + rng.normal(0, 0.05, 128).astype(np.float32) for _ in range(B)]
+
+    for _ in range(args.warmup):
+        IV.execute_interactive_queries(server, client, batch(), n_probe, 10)
+    ps = IV.PlaintextSearcher(ix)
+    server_s, lat, rec_f, rec_p, same = [], [], [], [], 0
+    for _ in range(args.steps):
+        qs = batch()
+        w0 = time.perf_counter()
+        res, t = IV.execute_interactive_queries(server, client, qs, n_probe, 10)
+        lat.append(time.perf_counter() - w0)
+        server_s.append(t["server_stage1"] + t["server_stage3"])
+        for q, (ids, dist, probes) in zip(qs, res):
+            pids, _, _ = ps.search(q, n_probe, 10)
+            truth = IV.exact_knn(x, q, 10)
+            rec_f.append(IV.recall_at(ids, truth))
+            rec_p.append(IV.recall_at(pids, truth))
+            same += set(ids) == set(pids)
+    srv = float(np.sum(server_s))
+    qps = B * args.steps / srv
+    lat_ms = np.array(lat) * 1e3
+    log(f"ivfpq: {qps:.1f} queries/s (server), batch latency p50 {np.percentile(lat_ms, 50):.1f} ms")
+    return {
+        "metric": "encrypted IVF-PQ queries/s (interactive, N=2^14, L=8, n_probe=4, 1M x 128)",
+        "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(srv / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded 32-Gaussian mixture, sigma 0.1)",
+        "config": {"workload": f"batches of {B} encrypted queries through the interactive protocol "
+                               "(coarse distances, client cluster choice, candidate distances, client top-10)",
+                   "db": n, "dim": 128, "n_list": 32, "M": 8, "K": 256, "N": Nq, "L": Lq, "K_special": Kq,
+                   "dnum": dq, "eval_limbs": lt, "n_probe": n_probe, "batch": B, "blocks": n_blocks},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                       "p99": round(float(np.percentile(lat_ms, 99)), 2),
+                       "what": "end-to-end batch (client encrypt/decrypt + server), per query = per batch"},
+        "recall_at_10": {"encrypted": round(float(np.mean(rec_f)), 4), "plaintext_adc": round(float(np.mean(rec_p)), 4),
+                         "same_top10_sets": same, "queries": len(rec_f)},
+        "setup_s": {"dataset_index": round(t_index, 1), "encode_upload": round(t_encode, 1)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -206,7 +279,7 @@ def main():
               "(data-parallel sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "hmult":
-        res = bench_ntt256(args) if args.workload == "ntt256" else bench_ks64(args)
+        res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq}[args.workload](args)
         print(json.dumps(res), flush=True)
         return
     from anns_fhe_amd.driver import FheContext, FHEHostDriver

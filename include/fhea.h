@@ -174,6 +174,10 @@ int fhea_ntt_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t npolys, u
 /* Wait for all work issued on the context's stream. */
 int fhea_sync(fhea_ctx *ctx);
 
+/* Rescale (OP 8 semantics) of n ciphertexts stored back to back: ciphertext z at
+ * src + z*2*limbs*N (c0 then c1) -> dst + z*2*(limbs-1)*N.  Ranges must not overlap. */
+int fhea_rescale_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs);
+
 /* Sentinel for an absent optional offset. */
 #define FHEA_NONE (~(uint64_t)0)
 

@@ -1,0 +1,65 @@
+"""A CPU stand-in for anns_fhe_amd.driver.FheContext backed by the oracle.
+
+Same method names and word-offset semantics (poly arena, key arena, submit work
+items, linear_sum, rescale_batch), so host-side protocol code written against
+FheContext runs unchanged on the oracle: the GPU run and this run must produce
+the same words.  Test infrastructure only.
+"""
+import numpy as np
+
+NONE = (1 << 64) - 1
+
+
+class OracleContext:
+    def __init__(self, orc, N, L, K, dnum, q_primes, p_primes, poly_words, key_words):
+        self.o = orc.OracleCtx(N, q_primes, p_primes, dnum)
+        self.N, self.L, self.K, self.dnum = N, L, K, dnum
+        self.q, self.p = list(q_primes), list(p_primes)
+        self.primes = self.q + self.p
+        self.pm = np.zeros(poly_words, np.uint64)
+        self.km = np.zeros(key_words, np.uint64)
+        self.poly_words, self.key_words = poly_words, key_words
+        self.relin = None
+        self.lib = self  # FheIvfPqServer calls ctx.lib.fhea_set_relin_key(ctx.h, off, limbs)
+        self.h = None
+
+    def fhea_set_relin_key(self, _h, off, limbs):
+        self.relin = (off, limbs)
+        return 0
+
+    def write_poly(self, off, data):
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        self.pm[off:off + a.size] = a
+
+    def read_poly(self, off, count):
+        return self.pm[off:off + count].copy()
+
+    def write_key(self, off, data):
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        self.km[off:off + a.size] = a
+
+    def submit(self, works):
+        for w in works:
+            k, l = w["kind"], w["limbs"]
+            if k == 2:
+                st = self.o.eval_automorphism_ks(self.pm, self.km, w["galois"], w["evk"], w.get("key_limbs", l),
+                                                 w["a0"], w["a1"], w["d0"], w["d1"], l)
+            elif k == 5:
+                off, kl = self.relin
+                st = self.o.eval_mult_relin(self.pm, self.km, off, kl, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"],
+                                            w["d1"], l)
+            elif k == 6:
+                st = self.o.eval_add(self.pm, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"], w["d1"], l)
+            else:
+                raise ValueError(k)
+            assert st == 0, (k, st)
+
+    def linear_sum(self, ct0, ct1, pt, out0, out1, limbs, ptb=None, bias=None):
+        assert self.o.linear_sum(self.pm, ct0, ct1, pt, out0, out1, limbs, ptb=ptb, bias=bias) == 0
+
+    def rescale_batch(self, src, dst, n, limbs):
+        W, W1 = limbs * self.N, (limbs - 1) * self.N
+        for z in range(n):
+            for k in range(2):
+                a = self.pm[src + z * 2 * W + k * W:src + z * 2 * W + (k + 1) * W]
+                self.pm[dst + z * 2 * W1 + k * W1:dst + z * 2 * W1 + (k + 1) * W1] = self.o.rescale(a).reshape(-1)

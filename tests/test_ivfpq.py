@@ -1,0 +1,76 @@
+"""IVF-PQ: the plaintext ADC searcher and the encrypted interactive protocol on the CPU.
+
+* ADC distances equal the reconstruct-then-square distances ||q - (c + r_hat)||^2
+  (the identity that lets the encrypted path evaluate ADC as a plain squared
+  distance, fhe_searcher.cpp:334-368 vs plaintext_searcher.cpp);
+* the encrypted protocol (anns_fhe_amd.ivfpq) run on the oracle-backed context
+  returns the plaintext searcher's top-k with distances within the CKKS error.
+"""
+import numpy as np
+import pytest
+
+from anns_fhe_amd import ivfpq as IV
+
+
+@pytest.fixture(scope="module")
+def small_index():
+    x, _ = IV.make_gaussian_mixture(4000, dim=128, n_centres=4, sigma=0.1, seed=3)
+    ix = IV.IvfPqIndex(n_list=4, M=8, K=256).train(x, train_size=4000, iters=8, seed=1).add(x)
+    return x, ix
+
+
+def test_dataset_is_seeded_mixture():
+    a, ca = IV.make_gaussian_mixture(1000, seed=5)
+    b, cb = IV.make_gaussian_mixture(1000, seed=5)
+    assert a.dtype == np.float32 and a.shape == (1000, 128)
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    assert np.all(np.abs(ca) <= 1.0)
+    # every point is within a few sigma of its nearest centre
+    d = np.sqrt(((a[:, None, :] - ca[None]) ** 2).sum(-1)).min(1)
+    assert np.median(d) == pytest.approx(0.1 * np.sqrt(128), rel=0.1)
+
+
+def test_adc_equals_reconstruct_then_square(small_index):
+    x, ix = small_index
+    ps = IV.PlaintextSearcher(ix)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        q = x[rng.integers(len(x))] + rng.normal(0, 0.05, 128).astype(np.float32)
+        ids, dist, probes = ps.search(q, n_probe=2, top_k=50)
+        rec = ix.reconstruct(ids).astype(np.float64)
+        direct = ((q.astype(np.float64) - rec) ** 2).sum(1)
+        assert np.allclose(dist, direct, rtol=1e-4, atol=1e-4)
+        assert set(ix.assign[ids]) <= set(probes)
+
+
+def test_codes_and_lists_are_consistent(small_index):
+    x, ix = small_index
+    assert ix.codes.shape == (len(x), 8) and ix.codes.dtype == np.uint8
+    assert sum(len(l) for l in ix.lists) == len(x)
+    for c, l in enumerate(ix.lists):
+        assert np.all(ix.assign[l] == c)
+
+
+def test_encrypted_protocol_on_oracle_matches_plaintext(orc, small_index):
+    from anns_fhe_amd.client import CkksClient
+    from tests.oracle_ctx import OracleContext
+    x, ix = small_index
+    N, L, K, dnum, lt = 4096, 4, 2, 2, 3
+    ck = CkksClient(N, L, K, dnum)
+    n_blocks = 1 + sum(-(-len(l) // (N // 2)) for l in ix.lists)
+    words = IV.FheIvfPqServer.arena_words(N, L, lt, 128, n_blocks, 2, 4)
+    ksz = ck.key_size_words(lt)
+    ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 128 * ksz + 8)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=7)
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=2, max_blocks_per_query=4)
+    rel, rots = client.evaluation_keys()
+    server.load_keys(rel, rots)
+    rng = np.random.default_rng(4)
+    queries = [x[rng.integers(len(x))] + rng.normal(0, 0.05, 128).astype(np.float32) for _ in range(2)]
+    res, _ = IV.execute_interactive_queries(server, client, queries, n_probe=2, top_k=10)
+    ps = IV.PlaintextSearcher(ix)
+    for q, (ids, dist, probes) in zip(queries, res):
+        pids, pdist, pprobes = ps.search(q, 2, 10)
+        assert list(probes) == list(pprobes)
+        assert np.allclose(dist, pdist, atol=1e-3)
+        assert set(ids) == set(pids)

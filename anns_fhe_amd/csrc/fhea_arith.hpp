@@ -26,18 +26,28 @@ typedef unsigned __int128 u128;
 
 FHEA_HD u64 mulhi64(u64 a, u64 b) { return (u64)(((u128)a * b) >> 64); }
 
-FHEA_HD u64 add_mod(u64 a, u64 b, u64 q) {
-    u64 s = a + b;
-    return s >= q ? s - q : s;
+// Conditional corrections are branch-free (q < 2^61, so the sign bit of a
+// wrapped difference marks "went negative"): data-dependent branches cost more
+// than the arithmetic in the host-side transforms.
+FHEA_HD u64 neg_mask(u64 x) { return (u64)((int64_t)x >> 63); }
+
+// x in [0, 2q) -> x mod q
+FHEA_HD u64 csub_q(u64 x, u64 q) {
+    const u64 t = x - q;
+    return t + (q & neg_mask(t));
 }
 
-FHEA_HD u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
+FHEA_HD u64 add_mod(u64 a, u64 b, u64 q) { return csub_q(a + b, q); }
+
+FHEA_HD u64 sub_mod(u64 a, u64 b, u64 q) {
+    const u64 d = a - b;
+    return d + (q & neg_mask(d));
+}
 
 // a * w mod q, wsh = floor(w * 2^64 / q), w < q, any a < 2^64 (Harvey/Shoup).
 FHEA_HD u64 mul_shoup(u64 a, u64 w, u64 wsh, u64 q) {
     u64 qh = mulhi64(a, wsh);
-    u64 r = a * w - qh * q;
-    return r >= q ? r - q : r;
+    return csub_q(a * w - qh * q, q);
 }
 
 // Same, result left in [0, 2q) (lazy form inside transforms).

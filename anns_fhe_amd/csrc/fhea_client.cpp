@@ -40,6 +40,10 @@ using namespace fhea;
 
 struct fhea_client {
     HostTables h;
+    std::vector<std::complex<double>> roots;  // exp(2 pi i k / N), k < N
+    std::vector<std::complex<double>> zeta;   // exp(i pi k / N), k < N
+    std::vector<u32> slot_t, slot_tc;          // DFT index of slot j and of its conjugate, j < N/2
+    void init_embedding();
 };
 
 namespace {
@@ -54,11 +58,11 @@ struct HostNtt {
     void forward(u64 *a, u32 pidx) const {
         const u32 N = h.N;
         const u64 q = h.primes[pidx];
-        const u64 *tw = h.tw.data() + (size_t)pidx * N;
+        const u64 *tw = h.tw.data() + (size_t)pidx * N, *tsh = h.tw_sh.data() + (size_t)pidx * N;
         u32 t = N >> 1;
         for (u32 m = 1; m < N; m <<= 1, t >>= 1) {
             for (u32 i = 0; i < m; ++i) {
-                const u64 W = tw[m + i], Wsh = shoup_companion(W, q);
+                const u64 W = tw[m + i], Wsh = tsh[m + i];
                 u64 *x = a + 2 * i * t;
                 for (u32 j = 0; j < t; ++j) {
                     u64 u = x[j], v = mul_shoup(x[j + t], W, Wsh, q);
@@ -72,11 +76,11 @@ struct HostNtt {
     void inverse(u64 *a, u32 pidx) const {
         const u32 N = h.N;
         const u64 q = h.primes[pidx];
-        const u64 *itw = h.itw.data() + (size_t)pidx * N;
+        const u64 *itw = h.itw.data() + (size_t)pidx * N, *ish = h.itw_sh.data() + (size_t)pidx * N;
         u32 t = 1;
         for (u32 m = N >> 1; m >= 1; m >>= 1, t <<= 1) {
             for (u32 i = 0; i < m; ++i) {
-                const u64 W = itw[m + i], Wsh = shoup_companion(W, q);
+                const u64 W = itw[m + i], Wsh = ish[m + i];
                 u64 *x = a + 2 * i * t;
                 for (u32 j = 0; j < t; ++j) {
                     u64 u = x[j], v = x[j + t];
@@ -161,6 +165,7 @@ std::vector<u64> make_switch_key(const HostTables &h, const std::vector<u64> &s_
         std::vector<u64> e_e = to_eval(h, e.data(), kp);
         for (u32 r = 0; r < R; ++r) {
             const u64 q = h.primes[kp[r]];
+            const ModParams mq = make_mod_params(q);
             std::uniform_int_distribution<u64> uni(0, q - 1);
             const bool in_digit = kp[r] < key_limbs && kp[r] >= p * A && kp[r] < (p + 1) * A;
             u64 *ar = ka + ((size_t)p * R + r) * N, *br = kb + ((size_t)p * R + r) * N;
@@ -169,9 +174,9 @@ std::vector<u64> make_switch_key(const HostTables &h, const std::vector<u64> &s_
             for (u32 j = 0; j < N; ++j) {
                 u64 a = uni(rng);
                 ar[j] = a;
-                u64 as = (u64)((u128)a * sr[j] % q);
+                u64 as = mulmod(a, sr[j], mq);
                 u64 b = add_mod(sub_mod(0, as, q), er[j], q);
-                if (in_digit) b = add_mod(b, (u64)((u128)pmod[r] * snr[j] % q), q);
+                if (in_digit) b = add_mod(b, mulmod(pmod[r], snr[j], mq), q);
                 br[j] = b;
             }
         }
@@ -192,8 +197,8 @@ int key_for(const HostTables &h, int kind, const int8_t *s, const int64_t *s_new
     } else if (kind == 1) {  // relinearisation: s^2
         sn.resize(s_e.size());
         for (size_t r = 0; r < kp.size(); ++r) {
-            const u64 q = h.primes[kp[r]];
-            for (u32 j = 0; j < N; ++j) sn[r * N + j] = (u64)((u128)s_e[r * N + j] * s_e[r * N + j] % q);
+            const ModParams mq = make_mod_params(h.primes[kp[r]]);
+            for (u32 j = 0; j < N; ++j) sn[r * N + j] = mulmod(s_e[r * N + j], s_e[r * N + j], mq);
         }
     } else {  // rotation: sigma_g(s), out[i] = in[map_g(i)] in the evaluation domain
         if ((galois & 1) == 0 || galois >= 2ull * N) return FHEA_ERR_ARG;
@@ -223,8 +228,9 @@ int secret(u32 N, uint64_t seed, int8_t *s_out) {
 
 // ---- canonical embedding
 
-// In-place iterative radix-2 DFT, X_t = sum_k x_k exp(sign 2 pi i t k / n).
-void dft(std::vector<cplx> &a, int sign) {
+// In-place iterative radix-2 DFT of size n = roots.size(),
+// X_t = sum_k x_k exp(sign 2 pi i t k / n).
+void dft(std::vector<cplx> &a, int sign, const std::vector<cplx> &roots) {
     const size_t n = a.size();
     for (size_t i = 1, j = 0; i < n; ++i) {
         size_t bit = n >> 1;
@@ -233,15 +239,11 @@ void dft(std::vector<cplx> &a, int sign) {
         if (i < j) std::swap(a[i], a[j]);
     }
     for (size_t len = 2; len <= n; len <<= 1) {
-        const size_t half = len >> 1;
-        std::vector<cplx> w(half);
-        for (size_t k = 0; k < half; ++k) {
-            const double ang = sign * 2.0 * kPi * (double)k / (double)len;
-            w[k] = cplx(std::cos(ang), std::sin(ang));
-        }
+        const size_t half = len >> 1, step = n / len;
         for (size_t i = 0; i < n; i += len)
             for (size_t k = 0; k < half; ++k) {
-                cplx u = a[i + k], v = a[i + k + half] * w[k];
+                const cplx w = sign > 0 ? roots[k * step] : std::conj(roots[k * step]);
+                cplx u = a[i + k], v = a[i + k + half] * w;
                 a[i + k] = u + v;
                 a[i + k + half] = u - v;
             }
@@ -274,21 +276,20 @@ u64 double_to_residue(double v, u64 q) {
 }
 
 // Encodes n values (replicated to N/2 slots) into `limbs` evaluation-domain limbs.
-int encode_one(const HostTables &h, const double *re, const double *im, u32 n, u32 limbs, double scale, u64 *pt) {
+int encode_one(const fhea_client *cl, const double *re, const double *im, u32 n, u32 limbs, double scale,
+               u64 *pt) {
+    const HostTables &h = cl->h;
     const u32 N = h.N;
     std::vector<cplx> Y(N);
     for (u32 j = 0; j < N / 2; ++j) {
         const cplx z(re[j % n] * scale, im ? im[j % n] * scale : 0.0);
-        u32 t, tc;
-        slot_index(N, j, t, tc);
-        Y[t] = z;
-        Y[tc] = std::conj(z);
+        Y[cl->slot_t[j]] = z;
+        Y[cl->slot_tc[j]] = std::conj(z);
     }
-    dft(Y, -1);
+    dft(Y, -1, cl->roots);
     std::vector<double> m(N);
     for (u32 k = 0; k < N; ++k) {
-        const double ang = -kPi * (double)k / (double)N;  // zeta^-k
-        const double v = std::nearbyint((Y[k] * cplx(std::cos(ang), std::sin(ang))).real() / (double)N);
+        const double v = std::nearbyint((Y[k] * std::conj(cl->zeta[k])).real() / (double)N);
         if (!std::isfinite(v) || std::fabs(v) > 0x1p1000) return FHEA_ERR_ARG;
         m[k] = v;
     }
@@ -305,10 +306,12 @@ int encode_one(const HostTables &h, const double *re, const double *im, u32 n, u
 // centred integer value of residues x[0..l) (Garner), as a double
 struct Crt {
     std::vector<u64> q;
+    std::vector<ModParams> mp;
     std::vector<u64> inv;  // inv[i*l + k] = q_k^-1 mod q_i
     u32 l;
     Crt(const HostTables &h, u32 limbs) : l(limbs) {
         q.assign(h.primes.begin(), h.primes.begin() + limbs);
+        for (u64 x : q) mp.push_back(make_mod_params(x));
         inv.assign((size_t)l * l, 0);
         for (u32 i = 0; i < l; ++i)
             for (u32 k = 0; k < i; ++k) inv[(size_t)i * l + k] = inv_mod(q[k] % q[i], q[i]);
@@ -319,7 +322,7 @@ struct Crt {
             u64 t = x[i * stride];
             for (u32 k = 0; k < i; ++k) {
                 const u64 d = sub_mod(t, v[k] % q[i], q[i]);
-                t = (u64)((u128)d * inv[(size_t)i * l + k] % q[i]);
+                t = mulmod(d, inv[(size_t)i * l + k], mp[i]);
             }
             v[i] = t;
         }
@@ -334,6 +337,20 @@ struct Crt {
 };
 
 }  // namespace
+
+void fhea_client::init_embedding() {
+    const u32 N = h.N;
+    roots.resize(N);
+    zeta.resize(N);
+    for (u32 k = 0; k < N; ++k) {
+        const double a = 2.0 * kPi * (double)k / (double)N, b = kPi * (double)k / (double)N;
+        roots[k] = cplx(std::cos(a), std::sin(a));
+        zeta[k] = cplx(std::cos(b), std::sin(b));
+    }
+    slot_t.resize(N / 2);
+    slot_tc.resize(N / 2);
+    for (u32 j = 0; j < N / 2; ++j) slot_index(N, j, slot_t[j], slot_tc[j]);
+}
 
 extern "C" {
 
@@ -390,6 +407,7 @@ int fhea_client_create(const fhea_params *p, fhea_client **out) {
         delete cl;
         return st;
     }
+    cl->init_embedding();
     *out = cl;
     return FHEA_OK;
 }
@@ -398,6 +416,7 @@ int fhea_client_from_ctx(const fhea_ctx *c, fhea_client **out) {
     if (!c || !out) return FHEA_ERR_STATE;
     fhea_client *cl = new fhea_client();
     cl->h = c->h;
+    cl->init_embedding();
     *out = cl;
     return FHEA_OK;
 }
@@ -465,7 +484,7 @@ int fhea_encode(const fhea_client *cl, const double *re, const double *im, uint3
     if (!cl) return FHEA_ERR_STATE;
     const HostTables &h = cl->h;
     if (!re || !pt || !valid_slots(h.N, n) || limbs == 0 || limbs > h.L || !(scale > 0.0)) return FHEA_ERR_ARG;
-    return encode_one(h, re, im, n, limbs, scale, pt);
+    return encode_one(cl, re, im, n, limbs, scale, pt);
 }
 
 int fhea_encode_batch(const fhea_client *cl, const double *re, const double *im, uint32_t count, uint32_t n,
@@ -482,7 +501,7 @@ int fhea_encode_batch(const fhea_client *cl, const double *re, const double *im,
     for (u32 t = 0; t < T; ++t)
         pool.emplace_back([&, t]() {
             for (u32 k = t; k < count; k += T) {
-                const int r = encode_one(h, re + (size_t)k * n, im ? im + (size_t)k * n : nullptr, n, limbs, scale,
+                const int r = encode_one(cl, re + (size_t)k * n, im ? im + (size_t)k * n : nullptr, n, limbs, scale,
                                          pt + k * W);
                 if (r != FHEA_OK) {
                     st[t] = r;
@@ -507,15 +526,10 @@ int fhea_decode(const fhea_client *cl, const uint64_t *pt, uint32_t limbs, doubl
     for (u32 i = 0; i < limbs; ++i) ntt.inverse(c.data() + (size_t)i * N, i);
     Crt crt(h, limbs);
     std::vector<cplx> Y(N);
-    for (u32 k = 0; k < N; ++k) {
-        const double v = crt.value(c.data() + k, N);
-        const double ang = kPi * (double)k / (double)N;  // zeta^k
-        Y[k] = v * cplx(std::cos(ang), std::sin(ang));
-    }
-    dft(Y, +1);
+    for (u32 k = 0; k < N; ++k) Y[k] = crt.value(c.data() + k, N) * cl->zeta[k];
+    dft(Y, +1, cl->roots);
     for (u32 j = 0; j < n; ++j) {
-        u32 t, tc;
-        slot_index(N, j, t, tc);
+        const u32 t = cl->slot_t[j];
         re[j] = Y[t].real() / scale;
         if (im) im[j] = Y[t].imag() / scale;
     }
@@ -536,11 +550,12 @@ int fhea_keygen_public(const fhea_client *cl, const int8_t *s, uint64_t seed, ui
     u64 *b = pk, *a = pk + (size_t)L * N;
     for (u32 i = 0; i < L; ++i) {
         const u64 q = h.primes[i];
+        const ModParams mq = make_mod_params(q);
         std::uniform_int_distribution<u64> uni(0, q - 1);
         for (u32 j = 0; j < N; ++j) {
             const size_t o = (size_t)i * N + j;
             a[o] = uni(rng);
-            b[o] = add_mod(sub_mod(0, (u64)((u128)a[o] * s_e[o] % q), q), e_e[o], q);
+            b[o] = add_mod(sub_mod(0, mulmod(a[o], s_e[o], mq), q), e_e[o], q);
         }
     }
     return FHEA_OK;
@@ -565,10 +580,11 @@ int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, 
     u64 *c0 = ct, *c1 = ct + (size_t)limbs * N;
     for (u32 i = 0; i < limbs; ++i) {
         const u64 q = h.primes[i];
+        const ModParams mq = make_mod_params(q);
         for (u32 j = 0; j < N; ++j) {
             const size_t o = (size_t)i * N + j;
-            c0[o] = add_mod(add_mod((u64)((u128)v_e[o] * b[o] % q), e0_e[o], q), pt[o], q);
-            c1[o] = add_mod((u64)((u128)v_e[o] * a[o] % q), e1_e[o], q);
+            c0[o] = add_mod(add_mod(mulmod(v_e[o], b[o], mq), e0_e[o], q), pt[o], q);
+            c1[o] = add_mod(mulmod(v_e[o], a[o], mq), e1_e[o], q);
         }
     }
     return FHEA_OK;
@@ -584,9 +600,10 @@ int fhea_decrypt(const fhea_client *cl, const int8_t *s, const uint64_t *ct, uin
     const u64 *c0 = ct, *c1 = ct + (size_t)limbs * N;
     for (u32 i = 0; i < limbs; ++i) {
         const u64 q = h.primes[i];
+        const ModParams mq = make_mod_params(q);
         for (u32 j = 0; j < N; ++j) {
             const size_t o = (size_t)i * N + j;
-            pt[o] = add_mod(c0[o], (u64)((u128)c1[o] * s_e[o] % q), q);
+            pt[o] = add_mod(c0[o], mulmod(c1[o], s_e[o], mq), q);
         }
     }
     return FHEA_OK;

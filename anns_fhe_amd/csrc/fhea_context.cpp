@@ -98,6 +98,8 @@ void build_host_tables(HostTables &h) {
     h.ninv.resize(T);
     h.tw.assign((size_t)T * N, 0);
     h.itw.assign((size_t)T * N, 0);
+    h.tw_sh.assign((size_t)T * N, 0);
+    h.itw_sh.assign((size_t)T * N, 0);
     for (u32 i = 0; i < T; ++i) {
         u64 q = h.primes[i];
         u64 psi = min_primitive_root(q, 2 * N);
@@ -117,6 +119,8 @@ void build_host_tables(HostTables &h) {
             u32 br = bit_reverse(j, h.logN);
             h.tw[(size_t)i * N + j] = p1[br];
             h.itw[(size_t)i * N + j] = p2[br];
+            h.tw_sh[(size_t)i * N + j] = shoup_companion(p1[br], q);
+            h.itw_sh[(size_t)i * N + j] = shoup_companion(p2[br], q);
         }
     }
 }

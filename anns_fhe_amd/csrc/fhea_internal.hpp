@@ -58,6 +58,7 @@ struct HostTables {
     u32 N = 0, logN = 0, L = 0, K = 0, dnum = 0, alpha = 0;
     std::vector<u64> primes, psi, ninv;
     std::vector<u64> tw, itw;  // [T*N] plain (for export / CPU client)
+    std::vector<u64> tw_sh, itw_sh;  // [T*N] Shoup companions of tw / itw (CPU transforms)
 };
 
 struct RotKey {

@@ -161,11 +161,18 @@ class FheIvfPqClient:
         self.pk = ckks.keygen_public(self.s, seed + 1)
         self.seed = seed + 100
 
-    def evaluation_keys(self):
-        """Relinearisation key and rotation keys for r = 1..D-1, all at lt limbs."""
+    def evaluation_keys(self, threads=16):
+        """Relinearisation key and rotation keys for r = 1..D-1, all at lt limbs
+        (generated on host threads; each key has its own seed, so the result does
+        not depend on the thread count)."""
+        import concurrent.futures as cf
         rel = self.ck.relin_key(self.s, self.lt, self.seed)
-        rots = {r: self.ck.rotation_key(self.s, self.ck.galois_for_step(r), self.lt, self.seed + r)
-                for r in range(1, self.D)}
+
+        def one(r):
+            return r, self.ck.rotation_key(self.s, self.ck.galois_for_step(r), self.lt, self.seed + r)
+
+        with cf.ThreadPoolExecutor(max(1, threads)) as ex:
+            rots = dict(ex.map(one, range(1, self.D)))
         return rel, rots
 
     def encrypt_query(self, q, limbs=None):

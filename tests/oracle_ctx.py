@@ -20,11 +20,16 @@ class OracleContext:
         self.km = np.zeros(key_words, np.uint64)
         self.poly_words, self.key_words = poly_words, key_words
         self.relin = None
-        self.lib = self  # FheIvfPqServer calls ctx.lib.fhea_set_relin_key(ctx.h, off, limbs)
+        self.rot = {}
+        self.lib = self  # host code calls ctx.lib.fhea_set_relin_key / fhea_copy_poly (ctx.h, ...)
         self.h = None
 
     def fhea_set_relin_key(self, _h, off, limbs):
         self.relin = (off, limbs)
+        return 0
+
+    def fhea_copy_poly(self, _h, dst, src, count):
+        self.pm[dst:dst + count] = self.pm[src:src + count].copy()
         return 0
 
     def write_poly(self, off, data):
@@ -63,3 +68,34 @@ class OracleContext:
             for k in range(2):
                 a = self.pm[src + z * 2 * W + k * W:src + z * 2 * W + (k + 1) * W]
                 self.pm[dst + z * 2 * W1 + k * W1:dst + z * 2 * W1 + (k + 1) * W1] = self.o.rescale(a).reshape(-1)
+
+
+class OracleDriver:
+    """FHEHostDriver's composites (fhe_host_driver.cpp:248-404) on an OracleContext."""
+
+    def __init__(self, octx):
+        self.c = octx
+        self.o = octx.o
+
+    def set_relin_key(self, evk, key_limbs=0):
+        self.c.relin = (evk, key_limbs)
+
+    def add_rotation_key(self, step, galois, evk, key_limbs=0):
+        self.c.rot[step] = (galois, evk, key_limbs)
+
+    def eval_add(self, a0, a1, b0, b1, d0, d1, limbs):
+        assert self.o.eval_add(self.c.pm, a0, a1, b0, b1, d0, d1, limbs) == 0
+
+    def eval_sub_plain(self, a0, a1, pt, d0, d1, limbs):
+        assert self.o.eval_sub_plain(self.c.pm, a0, a1, pt, d0, d1, limbs) == 0
+
+    def eval_mult_relin_rescale(self, a0, a1, b0, b1, d0, d1, in_limbs):
+        off, kl = self.c.relin
+        assert self.o.eval_mult_relin_rescale(self.c.pm, self.c.km, off, kl, a0, a1, b0, b1, d0, d1, in_limbs) == 0
+
+    def eval_mult_plain_rescale(self, a0, a1, pt, d0, d1, in_limbs):
+        assert self.o.eval_mult_plain_rescale(self.c.pm, a0, a1, pt, d0, d1, in_limbs) == 0
+
+    def eval_rotate(self, in0, in1, d0, d1, total_rot, limbs):
+        keys = [(st, g, off, kl) for st, (g, off, kl) in sorted(self.c.rot.items())]
+        assert self.o.eval_rotate(self.c.pm, self.c.km, keys, in0, in1, d0, d1, total_rot, limbs) == 0

@@ -74,3 +74,31 @@ def test_encrypted_protocol_on_oracle_matches_plaintext(orc, small_index):
         assert list(probes) == list(pprobes)
         assert np.allclose(dist, pdist, atol=1e-3)
         assert set(ids) == set(pids)
+
+
+def test_reference_query_sequence_on_oracle(orc):
+    """The reference's op-by-op interactive sequence (query_manager.py) on the oracle
+    returns the plaintext ADC top-k."""
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.query_manager import InteractiveQueryManager, run_reference_query
+    from tests.oracle_ctx import OracleContext, OracleDriver
+    x, _ = IV.make_gaussian_mixture(1000, dim=128, n_centres=4, sigma=0.1, seed=6)
+    ix = IV.IvfPqIndex(n_list=4, M=8, K=256).train(x, train_size=1000, iters=6, seed=2).add(x)
+    N, L, K, dnum = 4096, 4, 2, 2
+    ck = CkksClient(N, L, K, dnum)
+    B = N // 2 // 128
+    n_batches = sum(-(-len(l) // B) for l in ix.lists)
+    words = InteractiveQueryManager.arena_words(N, L, 4, n_batches, 4)
+    ksz = ck.key_size_words(L)
+    ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 40 * ksz)
+    qm = InteractiveQueryManager(ctx, OracleDriver(ctx), ck, ix, log_scale=40)
+    s = ck.keygen_secret(3)
+    pk = ck.keygen_public(s, 4)
+    qm.load_keys(s)
+    rng = np.random.default_rng(1)
+    q = x[rng.integers(len(x))] + rng.normal(0, 0.05, 128).astype(np.float32)
+    ids, dist, probes, _ = run_reference_query(qm, ck, s, pk, q, n_probe=2, top_k=10, seed=9)
+    pids, pdist, pprobes = IV.PlaintextSearcher(ix).search(q, 2, 10)
+    assert list(probes) == list(pprobes)
+    assert np.allclose(dist, pdist, atol=1e-3)
+    assert set(ids) == set(pids)

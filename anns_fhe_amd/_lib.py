@@ -43,7 +43,7 @@ class FheError(RuntimeError):
 class fhea_params(C.Structure):
     _fields_ = [("N", C.c_uint32), ("L", C.c_uint32), ("K", C.c_uint32), ("dnum", C.c_uint32),
                 ("q_bits", C.c_uint32), ("device", C.c_uint32), ("q_primes", u64p), ("p_primes", u64p),
-                ("poly_words", C.c_uint64), ("key_words", C.c_uint64)]
+                ("poly_words", C.c_uint64), ("key_words", C.c_uint64), ("psi", u64p)]
 
 
 class fhea_work(C.Structure):
@@ -101,6 +101,7 @@ SIGNATURES = {
     "fhea_client_destroy": (C.c_int, [vp]),
     "fhea_client_info": (C.c_int, [vp, u32p]),
     "fhea_client_primes": (C.c_int, [vp, u64p]),
+    "fhea_client_table": (C.c_int, [vp, C.c_int, u64p, C.c_uint64]),
     "fhea_client_keygen_secret": (C.c_int, [vp, C.c_uint64, i8p]),
     "fhea_client_switch_key": (C.c_int, [vp, i8p, i64p, C.c_uint32, C.c_uint64, u64p]),
     "fhea_client_relin_key": (C.c_int, [vp, i8p, C.c_uint32, C.c_uint64, u64p]),

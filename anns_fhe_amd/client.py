@@ -28,7 +28,7 @@ def _pf(a):
 
 
 class CkksClient:
-    def __init__(self, N, L, K, dnum, q_bits=50, q_primes=None, p_primes=None, ctx=None):
+    def __init__(self, N, L, K, dnum, q_bits=50, q_primes=None, p_primes=None, ctx=None, psi=None):
         """Build from a parameter block, or from a device context (`ctx`, same tables)."""
         self.lib = _lib.load()
         h = C.c_void_p()
@@ -41,6 +41,8 @@ class CkksClient:
             self._pa = np.array(p_primes, np.uint64) if p_primes is not None else None
             prm.q_primes = _p64(self._qa) if self._qa is not None else C.cast(None, _lib.u64p)
             prm.p_primes = _p64(self._pa) if self._pa is not None else C.cast(None, _lib.u64p)
+            self._psi = np.array(psi, np.uint64) if psi is not None else None
+            prm.psi = _p64(self._psi) if self._psi is not None else C.cast(None, _lib.u64p)
             check(self.lib.fhea_client_create(C.byref(prm), C.byref(h)), "fhea_client_create")
         self.h = h
         info = (C.c_uint32 * 5)()
@@ -62,6 +64,14 @@ class CkksClient:
             self.close()
         except Exception:
             pass
+
+    def table(self, which):
+        """0 psi, 1 N^-1 (per prime); 2 twiddles, 3 inverse twiddles ((L+K) x N)."""
+        T = self.L + self.K
+        n = T if which in (0, 1) else T * self.N
+        out = np.zeros(n, np.uint64)
+        check(self.lib.fhea_client_table(self.h, which, _p64(out), n), "fhea_client_table")
+        return out if which in (0, 1) else out.reshape(T, self.N)
 
     # ---- keys
     def keygen_secret(self, seed):

@@ -444,6 +444,17 @@ int fhea_client_primes(const fhea_client *cl, uint64_t *out) {
     return FHEA_OK;
 }
 
+int fhea_client_table(const fhea_client *cl, int which, uint64_t *out, uint64_t count) {
+    if (!cl) return FHEA_ERR_STATE;
+    if (!out) return FHEA_ERR_ARG;
+    const std::vector<u64> *v = which == 0 ? &cl->h.psi : which == 1 ? &cl->h.ninv
+                              : which == 2 ? &cl->h.tw : which == 3 ? &cl->h.itw : nullptr;
+    if (!v) return FHEA_ERR_ARG;
+    if (count != v->size()) return FHEA_ERR_RANGE;
+    std::memcpy(out, v->data(), count * 8);
+    return FHEA_OK;
+}
+
 int fhea_client_keygen_secret(const fhea_client *cl, uint64_t seed, int8_t *s_out) {
     if (!cl) return FHEA_ERR_STATE;
     return secret(cl->h.N, seed, s_out);

@@ -5,7 +5,7 @@
 // integration_tools/generate_interactive_test_vectors.cpp:246-282).  OpenFHE
 // is not available here, so the engine generates them from documented rules:
 //   * primes q == 1 (mod 2N), q < 2^bits, taken in descending order;
-//   * psi = the smallest primitive 2N-th root of unity mod q;
+//   * psi = the smallest primitive 2N-th root of unity mod q (or a given one);
 //   * tw[j] = psi^br(j), itw[j] = psi^-br(j), br over log2(N) bits
 //     (generate_interactive_test_vectors.cpp:263-275);
 //   * n_inv = N^-1 mod q.
@@ -102,7 +102,7 @@ void build_host_tables(HostTables &h) {
     h.itw_sh.assign((size_t)T * N, 0);
     for (u32 i = 0; i < T; ++i) {
         u64 q = h.primes[i];
-        u64 psi = min_primitive_root(q, 2 * N);
+        u64 psi = h.psi_given.empty() ? min_primitive_root(q, 2 * N) : h.psi_given[i];
         h.psi[i] = psi;
         h.ninv[i] = inv_mod(N, q);
         u64 psi_inv = inv_mod(psi, q);
@@ -152,6 +152,14 @@ int host_tables_from_params(const fhea_params *p, u32 min_n, HostTables &h) {
     for (size_t i = 0; i < h.primes.size(); ++i)
         for (size_t j = i + 1; j < h.primes.size(); ++j)
             if (h.primes[i] == h.primes[j]) return FHEA_ERR_ARG;
+    if (p->psi) {
+        // a primitive 2N-th root: psi^N == -1 (then its order is exactly 2N)
+        for (u32 i = 0; i < p->L + p->K; ++i) {
+            const u64 q = h.primes[i], w = p->psi[i];
+            if (w == 0 || w >= q || pow_mod(w, N, q) != q - 1) return FHEA_ERR_ARG;
+            h.psi_given.push_back(w);
+        }
+    }
     build_host_tables(h);
     return FHEA_OK;
 }

@@ -57,6 +57,7 @@ struct LevelTables {
 struct HostTables {
     u32 N = 0, logN = 0, L = 0, K = 0, dnum = 0, alpha = 0;
     std::vector<u64> primes, psi, ninv;
+    std::vector<u64> psi_given;  // explicit roots from the parameter block, if any
     std::vector<u64> tw, itw;  // [T*N] plain (for export / CPU client)
     std::vector<u64> tw_sh, itw_sh;  // [T*N] Shoup companions of tw / itw (CPU transforms)
 };

@@ -23,7 +23,7 @@ class FheContext:
     """Parameters + device-resident tables + arenas (replaces OP 99 and SimMemoryBus)."""
 
     def __init__(self, N, L, K, dnum, q_bits=50, q_primes=None, p_primes=None,
-                 poly_words=1 << 23, key_words=1 << 23, device=0):
+                 poly_words=1 << 23, key_words=1 << 23, device=0, psi=None):
         self.lib = _lib.load()
         prm = _lib.fhea_params()
         prm.N, prm.L, prm.K, prm.dnum, prm.q_bits, prm.device = N, L, K, dnum, q_bits, device
@@ -32,6 +32,8 @@ class FheContext:
         prm.q_primes = _p64(self._qa) if self._qa is not None else C.cast(None, _lib.u64p)
         prm.p_primes = _p64(self._pa) if self._pa is not None else C.cast(None, _lib.u64p)
         prm.poly_words, prm.key_words = poly_words, key_words
+        self._psi = np.array(psi, np.uint64) if psi is not None else None
+        prm.psi = _p64(self._psi) if self._psi is not None else C.cast(None, _lib.u64p)
         h = C.c_void_p()
         check(self.lib.fhea_ctx_create(C.byref(prm), C.byref(h)), "fhea_ctx_create")
         self.h = h

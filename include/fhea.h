@@ -78,6 +78,10 @@ typedef struct {
     const uint64_t *p_primes;  /* optional explicit P primes (K), NULL = generate */
     uint64_t poly_words;       /* poly arena size in 64-bit words (poly_gmem)      */
     uint64_t key_words;        /* key arena size in 64-bit words (key_gmem)        */
+    const uint64_t *psi;       /* optional primitive 2N-th root of unity per prime
+                                  (L+K, Q then P); NULL = the smallest one.  Lets
+                                  tables dumped by another library (whose twiddle
+                                  file holds psi at index N/2) be reproduced.    */
 } fhea_params;
 
 /* ---- context (replaces OP 99 + initialize_context_from_directory,
@@ -231,6 +235,8 @@ int fhea_client_from_ctx(const fhea_ctx *ctx, fhea_client **out);
 int fhea_client_destroy(fhea_client *cl);
 int fhea_client_info(const fhea_client *cl, uint32_t *info5);  /* N, L, K, dnum, alpha */
 int fhea_client_primes(const fhea_client *cl, uint64_t *out);   /* L+K primes        */
+/* which: 0 psi (L+K), 1 N^-1 (L+K), 2 twiddles, 3 inverse twiddles ((L+K)*N); as fhea_ctx_table */
+int fhea_client_table(const fhea_client *cl, int which, uint64_t *out, uint64_t count);
 int fhea_client_keygen_secret(const fhea_client *cl, uint64_t seed, int8_t *s_out);
 /* evaluation keys into a host buffer of fhea_key_size_words words, same layout as the arena */
 int fhea_client_switch_key(const fhea_client *cl, const int8_t *s, const int64_t *s_new,

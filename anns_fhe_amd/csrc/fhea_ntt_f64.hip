@@ -183,8 +183,23 @@ __device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_li
     return id;
 }
 
+// Build-time schedule knobs (defaults are the measured best on MI355X):
+//   FHEA_NTT_PREFETCH  1: a tile's global loads are issued one tile ahead (registers)
+//   FHEA_NTT_WAVES     >0: register budget for that many waves per SIMD
+#ifndef FHEA_NTT_PREFETCH
+#define FHEA_NTT_PREFETCH 1
+#endif
+#ifndef FHEA_NTT_WAVES
+#define FHEA_NTT_WAVES 0
+#endif
+#if FHEA_NTT_WAVES > 0
+#define FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu(FHEA_NTT_WAVES)))
+#else
+#define FHEA_NTT_ATTR
+#endif
+
 template <bool INV, int LOGR, bool COLS>
-__global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u64 src_stride,
+__global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
                                                   const double *__restrict__ twf,
@@ -237,9 +252,10 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
 
     TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
     u64 raw[16];
-    if (cur.tile < n_tiles) issue(cur, raw);
+    if (FHEA_NTT_PREFETCH && cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
         const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
+        if (!FHEA_NTT_PREFETCH) issue(cur, raw);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
         g.qinv = mp[cur.pidx].qinv;
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(F_TPB) void k_ntt64f(const void *src, void *dst, u6
             }
         }
         // prefetch the next tile
-        if (nxt.tile < n_tiles) issue(nxt, raw);
+        if (FHEA_NTT_PREFETCH && nxt.tile < n_tiles) issue(nxt, raw);
         if (!DIRECT_IN) {
             __syncthreads();
             if (active) {

@@ -152,7 +152,7 @@ class FheIvfPqClient:
     distances, cluster selection and top-k (fhe_context_manager.cpp:457-512,
     interactive_query_manager.cpp:166-174,276-290)."""
 
-    def __init__(self, ckks, dim=128, log_scale=40, lt=3, seed=1):
+    def __init__(self, ckks, dim=128, log_scale=45, lt=3, seed=1):
         self.ck = ckks
         self.D = dim
         self.scale = 2.0 ** log_scale
@@ -194,7 +194,7 @@ class FheIvfPqServer:
     block b: D plaintexts pt_r (lt limbs each) then ptb (||x_s||^2);
     query slot: ct (L limbs x 2), D-1 rotations, norm, tmp, outputs, rescaled outputs."""
 
-    def __init__(self, ctx, ckks, index, lt=3, log_scale=40, max_queries=64, max_blocks_per_query=None,
+    def __init__(self, ctx, ckks, index, lt=3, log_scale=45, max_queries=64, max_blocks_per_query=None,
                  threads=0, log=None):
         self.ctx, self.ck, self.ix = ctx, ckks, index
         self.N, self.L, self.lt = ctx.N, ctx.L, lt

@@ -34,7 +34,7 @@ def _pow2_steps(n):
 
 
 class InteractiveQueryManager:
-    def __init__(self, ctx, drv, ckks, index, log_scale=40, threads=0):
+    def __init__(self, ctx, drv, ckks, index, log_scale=50, threads=0):
         self.ctx, self.drv, self.ck, self.ix = ctx, drv, ckks, index
         self.N, self.L = ctx.N, ctx.L
         self.S = ctx.N // 2

@@ -224,9 +224,9 @@ def bench_ivfpq(args):
     words = IV.FheIvfPqServer.arena_words(Nq, Lq, lt, 128, n_blocks, B, max_g)
     ctx = FheContext(Nq, Lq, Kq, dq, poly_words=words, key_words=128 * 2 * dq * (lt + Kq) * Nq + 8)
     ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=11)
     t1 = time.perf_counter()
-    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=max_g,
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=B, max_blocks_per_query=max_g,
                                log=log)
     t_encode = time.perf_counter() - t1
     rel, rots = client.evaluation_keys()
@@ -324,7 +324,7 @@ def bench_serve(args):
     words = InteractiveQueryManager.arena_words(Nq, Lq, 32, n_batches, 8 * sum(per[:8]) + 8)
     ctx = FheContext(Nq, Lq, Kq, dq, poly_words=words, key_words=40 * 2 * dq * (Lq + Kq) * Nq)
     ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-    qm = InteractiveQueryManager(ctx, FHEHostDriver(ctx), ck, ix, log_scale=40)
+    qm = InteractiveQueryManager(ctx, FHEHostDriver(ctx), ck, ix)
     s = ck.keygen_secret(3)
     pk = ck.keygen_public(s, 4)
     qm.load_keys(s)

@@ -43,8 +43,8 @@ def test_protocol_words_equal_oracle(orc):
         queries = _queries(x, 2, 4)
         results = []
         for c in (ctx, octx):
-            client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=7)
-            server = IV.FheIvfPqServer(c, ck, ix, lt=lt, log_scale=40, max_queries=2, max_blocks_per_query=4)
+            client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=7)
+            server = IV.FheIvfPqServer(c, ck, ix, lt=lt, log_scale=45, max_queries=2, max_blocks_per_query=4)
             rel, rots = client.evaluation_keys()
             server.load_keys(rel, rots)
             say("protocol on", type(c).__name__)
@@ -74,9 +74,9 @@ def test_encrypted_search_recall_c4_shape():
     ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=128 * 2 * dnum * (lt + K) * N + 8)
     try:
         ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-        client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
+        client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=11)
         say("encode database")
-        server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=8)
+        server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=B, max_blocks_per_query=8)
         say("keys")
         rel, rots = client.evaluation_keys()
         server.load_keys(rel, rots)

@@ -61,8 +61,8 @@ def test_encrypted_protocol_on_oracle_matches_plaintext(orc, small_index):
     words = IV.FheIvfPqServer.arena_words(N, L, lt, 128, n_blocks, 2, 4)
     ksz = ck.key_size_words(lt)
     ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 128 * ksz + 8)
-    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=7)
-    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=2, max_blocks_per_query=4)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=7)
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=2, max_blocks_per_query=4)
     rel, rots = client.evaluation_keys()
     server.load_keys(rel, rots)
     rng = np.random.default_rng(4)
@@ -91,7 +91,7 @@ def test_reference_query_sequence_on_oracle(orc):
     words = InteractiveQueryManager.arena_words(N, L, 4, n_batches, 4)
     ksz = ck.key_size_words(L)
     ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 40 * ksz)
-    qm = InteractiveQueryManager(ctx, OracleDriver(ctx), ck, ix, log_scale=40)
+    qm = InteractiveQueryManager(ctx, OracleDriver(ctx), ck, ix)
     s = ck.keygen_secret(3)
     pk = ck.keygen_public(s, 4)
     qm.load_keys(s)
@@ -100,5 +100,31 @@ def test_reference_query_sequence_on_oracle(orc):
     ids, dist, probes, _ = run_reference_query(qm, ck, s, pk, q, n_probe=2, top_k=10, seed=9)
     pids, pdist, pprobes = IV.PlaintextSearcher(ix).search(q, 2, 10)
     assert list(probes) == list(pprobes)
+    assert np.allclose(dist, pdist, atol=1e-3)
+    assert set(ids) == set(pids)
+
+
+def test_reference_query_sequence_many_candidates(orc):
+    """The reference sequence with more candidates than one output ciphertext holds
+    (so the position shift wraps into a second output) and n_probe = n_list."""
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.query_manager import InteractiveQueryManager, run_reference_query
+    from tests.oracle_ctx import OracleContext, OracleDriver
+    x, _ = IV.make_gaussian_mixture(2600, dim=128, n_centres=4, sigma=0.1, seed=9)
+    ix = IV.IvfPqIndex(n_list=4, M=8, K=256).train(x, train_size=2600, iters=5, seed=2).add(x)
+    N, L, K, dnum = 4096, 4, 2, 2
+    ck = CkksClient(N, L, K, dnum)
+    B = N // 2 // 128
+    n_batches = sum(-(-len(l) // B) for l in ix.lists)
+    words = InteractiveQueryManager.arena_words(N, L, 4, n_batches, 4)
+    ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 40 * ck.key_size_words(L))
+    qm = InteractiveQueryManager(ctx, OracleDriver(ctx), ck, ix)
+    s = ck.keygen_secret(5)
+    pk = ck.keygen_public(s, 6)
+    qm.load_keys(s)
+    q = x[17] + 0.02
+    ids, dist, probes, _ = run_reference_query(qm, ck, s, pk, q, n_probe=4, top_k=10, seed=21)
+    pids, pdist, _ = IV.PlaintextSearcher(ix).search(q, 4, 10)
+    assert sum(len(l) for l in ix.lists) > N // 2
     assert np.allclose(dist, pdist, atol=1e-3)
     assert set(ids) == set(pids)

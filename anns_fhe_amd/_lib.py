@@ -85,6 +85,10 @@ SIGNATURES = {
     "fhea_submit": (C.c_int, [vp, C.POINTER(fhea_work), C.c_uint32]),
     "fhea_ntt_batch": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
     "fhea_sync": (C.c_int, [vp]),
+    "fhea_graph_begin": (C.c_int, [vp]),
+    "fhea_graph_end": (C.c_int, [vp, u32p]),
+    "fhea_graph_launch": (C.c_int, [vp, C.c_uint32]),
+    "fhea_graph_destroy": (C.c_int, [vp, C.c_uint32]),
     "fhea_rescale_batch": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
     "fhea_eval_linear_sum": (C.c_int, [vp, u64p, u64p, C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64,
                                        C.c_uint64, u64p, u64p, C.c_uint32]),

@@ -28,6 +28,7 @@ int check_hip(hipError_t e) { return e == hipSuccess ? FHEA_OK : FHEA_ERR_DEVICE
 
 int ensure_ws(fhea_ctx *c, u64 words) {
     if (words <= c->ws_words) return FHEA_OK;
+    if (c->capturing) return FHEA_ERR_NO_MEM;  // captured kernels may already point at it
     if (c->ws) {
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(c->ws));
@@ -43,6 +44,18 @@ int ensure_ws(fhea_ctx *c, u64 words) {
 }
 
 void *stage_args(fhea_ctx *c, const void *host, size_t bytes) {
+    if (c->capturing) {
+        // the graph owns its argument area (pinned host + device, allocated before the
+        // capture began and never reused): the copy is recorded as a graph node
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (c->cap_used + need > fhea_ctx::CAP_BYTES) return nullptr;
+        char *h = (char *)c->cap_host + c->cap_used;
+        char *d = (char *)c->cap_dev + c->cap_used;
+        memcpy(h, host, bytes);
+        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+        c->cap_used += need;
+        return d;
+    }
     size_t need = (bytes + 255) & ~(size_t)255;
     if (c->args_used + need > c->args_bytes) {
         // every earlier copy must have landed before the staging area is reused
@@ -679,10 +692,26 @@ int fhea_ctx_create(const fhea_params *p, fhea_ctx **out) {
 int fhea_ctx_destroy(fhea_ctx *c) {
     if (!c) return FHEA_ERR_STATE;
     hipSetDevice(c->device);
+    if (c->capturing) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c->stream, &g);
+        if (g) hipGraphDestroy(g);
+        c->capturing = false;
+    }
+    if (c->cap_dev) hipFree(c->cap_dev);
+    if (c->cap_host) hipHostFree(c->cap_host);
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &kv : c->graphs) {
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+        if (kv.second.dev) hipFree(kv.second.dev);
+        if (kv.second.host) hipHostFree(kv.second.host);
+    }
+    c->graphs.clear();
     if (c->poly) hipFree(c->poly);
     if (c->key) hipFree(c->key);
     if (c->ws) hipFree(c->ws);
+    if (c->pp) hipFree(c->pp);
     if (c->dblob) hipFree(c->dblob);
     if (c->args) hipFree(c->args);
     if (c->args_host) hipHostFree(c->args_host);
@@ -721,6 +750,7 @@ int fhea_ctx_table(const fhea_ctx *c, int which, uint64_t *out, uint64_t count) 
 
 int fhea_ctx_set_stream(fhea_ctx *c, void *stream) {
     CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
     if (stream) {
         if (c->own_stream) hipStreamDestroy(c->stream);
@@ -742,6 +772,7 @@ uint64_t fhea_key_words(const fhea_ctx *c) { return c ? c->key_words : 0; }
 
 static int arena_copy(fhea_ctx *c, u64 *base, u64 total, u64 off, void *host, u64 count, bool write) {
     CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;  // host copies synchronise the stream
     if (!host && count) return FHEA_ERR_ARG;
     if (!range_ok(off, count, total)) return FHEA_ERR_RANGE;
     if (!count) return FHEA_OK;
@@ -972,8 +1003,20 @@ int fhea_eval_rotate(fhea_ctx *c, uint64_t in0, uint64_t in1, uint64_t d0, uint6
     // ping-pong ciphertext buffers live in a separate allocation from the
     // pipeline workspace so that a workspace resize cannot invalidate them
     u64 *pp = nullptr;
-    if (steps.size() > 1 && hipMallocAsync((void **)&pp, 4 * W * 8, c->stream) != hipSuccess)
-        return FHEA_ERR_NO_MEM;
+    if (steps.size() > 1) {
+        if (c->pp_words < 4 * W) {
+            if (c->capturing) return FHEA_ERR_NO_MEM;
+            if (c->pp) {
+                if (hipStreamSynchronize(c->stream) != hipSuccess || hipFree(c->pp) != hipSuccess)
+                    return FHEA_ERR_DEVICE;
+                c->pp = nullptr;
+                c->pp_words = 0;
+            }
+            if (hipMalloc((void **)&c->pp, 4 * W * 8) != hipSuccess) return FHEA_ERR_NO_MEM;
+            c->pp_words = 4 * W;
+        }
+        pp = c->pp;
+    }
     const u64 *cur0 = P + in0, *cur1 = P + in1;
     int st = FHEA_OK;
     for (size_t k = 0; k < steps.size() && !st; ++k) {
@@ -985,7 +1028,6 @@ int fhea_eval_rotate(fhea_ctx *c, uint64_t in0, uint64_t in1, uint64_t d0, uint6
         cur0 = n0;
         cur1 = n1;
     }
-    if (pp) hipFreeAsync(pp, c->stream);
     return st;
 }
 
@@ -1179,6 +1221,7 @@ int fhea_ntt_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t npolys, uin
 
 int fhea_sync(fhea_ctx *c) {
     CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
     int st = check_hip(hipStreamSynchronize(c->stream));
     if (!st) c->args_used = 0;
     return st;
@@ -1186,16 +1229,77 @@ int fhea_sync(fhea_ctx *c) {
 
 int fhea_timer_start(fhea_ctx *c) {
     CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
     return check_hip(hipEventRecord(c->ev0, c->stream));
 }
 
 int fhea_timer_stop(fhea_ctx *c, float *ms) {
     CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
     if (hipEventRecord(c->ev1, c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
     if (hipEventSynchronize(c->ev1) != hipSuccess) return FHEA_ERR_DEVICE;
     float v = 0;
     if (hipEventElapsedTime(&v, c->ev0, c->ev1) != hipSuccess) return FHEA_ERR_DEVICE;
     if (ms) *ms = v;
+    return FHEA_OK;
+}
+
+// ---- HIP graph capture of engine work
+int fhea_graph_begin(fhea_ctx *c) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    c->args_used = 0;
+    if (!c->cap_dev && hipMalloc(&c->cap_dev, fhea_ctx::CAP_BYTES) != hipSuccess) return FHEA_ERR_NO_MEM;
+    if (!c->cap_host && hipHostMalloc(&c->cap_host, fhea_ctx::CAP_BYTES, hipHostMallocDefault) != hipSuccess)
+        return FHEA_ERR_NO_MEM;
+    c->cap_used = 0;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FHEA_ERR_DEVICE;
+    c->capturing = true;
+    return FHEA_OK;
+}
+
+int fhea_graph_end(fhea_ctx *c, uint32_t *graph_id) {
+    CTX_CHECK(c);
+    if (!c->capturing) return FHEA_ERR_STATE;
+    c->capturing = false;
+    fhea_ctx::Graph g;
+    int st = FHEA_OK;
+    if (hipStreamEndCapture(c->stream, &g.graph) != hipSuccess || !g.graph) st = FHEA_ERR_DEVICE;
+    if (!st && hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) st = FHEA_ERR_DEVICE;
+    if (st) {
+        if (g.graph) hipGraphDestroy(g.graph);
+        return st;  // the argument area stays with the context for the next capture
+    }
+    // the argument area now belongs to the graph
+    g.dev = c->cap_dev;
+    g.host = c->cap_host;
+    c->cap_dev = c->cap_host = nullptr;
+    const uint32_t id = c->next_graph++;
+    c->graphs[id] = std::move(g);
+    if (graph_id) *graph_id = id;
+    return FHEA_OK;
+}
+
+int fhea_graph_launch(fhea_ctx *c, uint32_t graph_id) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    auto it = c->graphs.find(graph_id);
+    if (it == c->graphs.end()) return FHEA_ERR_ARG;
+    return check_hip(hipGraphLaunch(it->second.exec, c->stream));
+}
+
+int fhea_graph_destroy(fhea_ctx *c, uint32_t graph_id) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    auto it = c->graphs.find(graph_id);
+    if (it == c->graphs.end()) return FHEA_ERR_ARG;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    hipGraphExecDestroy(it->second.exec);
+    hipGraphDestroy(it->second.graph);
+    hipFree(it->second.dev);
+    hipHostFree(it->second.host);
+    c->graphs.erase(it);
     return FHEA_OK;
 }
 

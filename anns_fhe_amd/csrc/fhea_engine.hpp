@@ -44,6 +44,24 @@ struct fhea_ctx {
     u32 relin_limbs = 0;
     std::map<int32_t, fhea::RotKey> rot;
 
+    // second workspace: ping-pong ciphertexts of multi-step rotations
+    u64 *pp = nullptr;
+    u64 pp_words = 0;
+
+    // HIP graph capture (fhea_graph_*): while capturing, staged arguments go to
+    // device buffers owned by the graph and workspaces cannot grow
+    bool capturing = false;
+    void *cap_dev = nullptr, *cap_host = nullptr;  // argument area of the graph being captured
+    size_t cap_used = 0;
+    static constexpr size_t CAP_BYTES = 4u << 20;
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        void *dev = nullptr, *host = nullptr;
+    };
+    std::map<uint32_t, Graph> graphs;
+    uint32_t next_graph = 1;
+
     u32 max_chunk = 16;
     bool force_int = false;  // FHEA_FORCE_INT=1: integer kernels even for < 2^50 primes
     u32 f64_off = 0;         // FHEA_F64_OFF: per-component switch (DevTables::f64_off)

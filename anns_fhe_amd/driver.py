@@ -117,6 +117,21 @@ class FheContext:
     def rescale_batch(self, src, dst, n, limbs):
         check(self.lib.fhea_rescale_batch(self.h, src, dst, n, limbs), "fhea_rescale_batch")
 
+    # ---- HIP graphs: record a fixed sequence once, replay it on new data at the same offsets
+    def graph_begin(self):
+        check(self.lib.fhea_graph_begin(self.h), "fhea_graph_begin")
+
+    def graph_end(self):
+        gid = C.c_uint32()
+        check(self.lib.fhea_graph_end(self.h, C.byref(gid)), "fhea_graph_end")
+        return gid.value
+
+    def graph_launch(self, gid):
+        check(self.lib.fhea_graph_launch(self.h, gid), "fhea_graph_launch")
+
+    def graph_destroy(self, gid):
+        check(self.lib.fhea_graph_destroy(self.h, gid), "fhea_graph_destroy")
+
     def sync(self):
         check(self.lib.fhea_sync(self.h), "fhea_sync")
 

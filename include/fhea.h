@@ -182,6 +182,19 @@ int fhea_sync(fhea_ctx *ctx);
  * src + z*2*limbs*N (c0 then c1) -> dst + z*2*(limbs-1)*N.  Ranges must not overlap. */
 int fhea_rescale_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs);
 
+/* HIP graph capture.  Engine work issued on the context between fhea_graph_begin
+ * and fhea_graph_end (submits, ops, composites, NTT batches, linear sums, device
+ * copies) is recorded instead of run; fhea_graph_launch replays it on the context's
+ * stream with the same word offsets, so new data written to those offsets is
+ * processed.  While capturing, calls that synchronise with the host (reads, writes,
+ * fhea_sync, timers, set_stream) return FHEA_ERR_STATE, and work that needs a larger
+ * workspace than is allocated returns FHEA_ERR_NO_MEM: run the same work once
+ * before capturing it.  No reference counterpart (the FPGA kernel is called op by op). */
+int fhea_graph_begin(fhea_ctx *ctx);
+int fhea_graph_end(fhea_ctx *ctx, uint32_t *graph_id);
+int fhea_graph_launch(fhea_ctx *ctx, uint32_t graph_id);
+int fhea_graph_destroy(fhea_ctx *ctx, uint32_t graph_id);
+
 /* Sentinel for an absent optional offset. */
 #define FHEA_NONE (~(uint64_t)0)
 

@@ -226,6 +226,9 @@ class FheIvfPqServer:
         self.slot_words = (self.ct_words + (self.D - 1) * 2 * Wl + 4 * Wl + self.max_g * 2 * Wl
                            + self.max_g * 2 * (lt - 1) * self.N)
         self.max_queries = max_queries
+        self.use_graphs = False
+        self.max_graphs = 16
+        self.graphs = {}
         need = self.pt_words + max_queries * self.slot_words
         assert need <= ctx.poly_words, (need, ctx.poly_words)
         self._encode_blocks(threads)
@@ -338,9 +341,26 @@ class FheIvfPqServer:
 
     # ---- stages (interactive_query_manager.cpp:85-274)
     def stage1_coarse_centroid_distance(self, slots):
+        """Rotations, norm and coarse distances of the queries in `slots`.  With
+        use_graphs, the launch sequence of a slot set is recorded once as a HIP graph
+        (after one eager run) and replayed for later queries in the same slots."""
+        key = tuple(slots)
+        gid = self.graphs.get(key) if self.use_graphs else None
+        if gid is not None:
+            self.ctx.graph_launch(gid)
+            return
         self.prepare(slots)
         for z in slots:
             self.distances(z, [0])
+        if self.use_graphs and len(self.graphs) < self.max_graphs:
+            self.ctx.sync()
+            self.ctx.graph_begin()
+            try:
+                self.prepare(slots)
+                for z in slots:
+                    self.distances(z, [0])
+            finally:
+                self.graphs[key] = self.ctx.graph_end()
 
     def stage3_fine_candidate_batch_distance(self, z, clusters):
         blocks = [b for c in clusters for b in self.cluster_blocks[c]]

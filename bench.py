@@ -44,6 +44,7 @@ def parse():
                          "search (1M x 128, n_list=32, M=8, K=256, B=64 queries, N=2^14, L=8, n_probe=4)")
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
+    ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="single-ciphertext op-by-op HMult runs timed for plain_path (0 = skip)")
     return ap.parse_args()
@@ -239,6 +240,7 @@ def bench_ivfpq(args):
     t1 = time.perf_counter()
     server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=B, max_blocks_per_query=max_g,
                                log=log)
+    server.use_graphs = not args.no_graphs
     t_encode = time.perf_counter() - t1
     rel, rots = client.evaluation_keys()
     server.load_keys(rel, rots)
@@ -275,7 +277,8 @@ def bench_ivfpq(args):
         "config": {"workload": f"batches of {B} encrypted queries through the interactive protocol "
                                "(coarse distances, client cluster choice, candidate distances, client top-10)",
                    "db": n, "dim": 128, "n_list": 32, "M": 8, "K": 256, "N": Nq, "L": Lq, "K_special": Kq,
-                   "dnum": dq, "eval_limbs": lt, "n_probe": n_probe, "batch": B, "blocks": n_blocks},
+                   "dnum": dq, "eval_limbs": lt, "n_probe": n_probe, "batch": B, "blocks": n_blocks,
+                   "stage1_graph": server.use_graphs},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2),
                        "what": "end-to-end batch (client encrypt/decrypt + server), per query = per batch"},

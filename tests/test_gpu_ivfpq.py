@@ -98,3 +98,40 @@ def test_encrypted_search_recall_c4_shape():
         assert rec_fhe == rec_plain
     finally:
         ctx.close()
+
+
+def test_stage1_graph_replay_equals_eager():
+    """Stage 1 recorded as a HIP graph and replayed for new queries in the same slots
+    gives the same coarse-distance ciphertexts as the eager launches."""
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.driver import FheContext
+    x, _ = IV.make_gaussian_mixture(4000, dim=128, n_centres=4, sigma=0.1, seed=3)
+    ix = IV.IvfPqIndex(n_list=4, M=8, K=256).train(x, train_size=4000, iters=6, seed=1).add(x)
+    N, L, K, dnum, lt = 8192, 5, 2, 3, 3
+    n_blocks = 1 + sum(-(-len(l) // (N // 2)) for l in ix.lists)
+    words = IV.FheIvfPqServer.arena_words(N, L, lt, 128, n_blocks, 3, 4)
+    ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=128 * 2 * dnum * (lt + K) * N + 8)
+    try:
+        ck = CkksClient(0, 0, 0, 0, ctx=ctx)
+        client = IV.FheIvfPqClient(ck, dim=128, lt=lt, seed=5)
+        server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, max_queries=3, max_blocks_per_query=4)
+        rel, rots = client.evaluation_keys()
+        server.load_keys(rel, rots)
+        slots = [0, 1, 2]
+        server.use_graphs = True
+        for rep in range(3):
+            qs = _queries(x, 3, 20 + rep)
+            for z, q in zip(slots, qs):
+                server.upload(z, client.encrypt_query(q))
+            server.stage1_coarse_centroid_distance(slots)   # rep 0: eager + capture; later: replay
+            got = [server.read_distances(z, 1) for z in slots]
+            server.use_graphs = False
+            server.stage1_coarse_centroid_distance(slots)   # eager on the same inputs
+            exp = [server.read_distances(z, 1) for z in slots]
+            server.use_graphs = True
+            say("stage-1 replay", rep)
+            for a, b in zip(got, exp):
+                assert np.array_equal(a, b)
+        assert len(server.graphs) == 1
+    finally:
+        ctx.close()

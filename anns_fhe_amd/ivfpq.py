@@ -369,19 +369,27 @@ class FheIvfPqServer:
         return blocks
 
 
-def execute_interactive_queries(server, client, queries, n_probe, top_k):
+def execute_interactive_queries(server, client, queries, n_probe, top_k, client_threads=16):
     """Runs the whole protocol for a batch of queries; returns per query
-    (ids, distances, probes) and a timing record."""
+    (ids, distances, probes) and a timing record.  Client-side work (encryption,
+    decryption, selection) runs on `client_threads` host threads, as independent
+    clients would."""
+    import concurrent.futures as cf
     ck, N = client.ck, server.N
     B = len(queries)
     assert B <= server.max_queries
     lt1 = server.lt - 1
     q_last = ck.q[server.lt - 1]
     scale = client.result_scale(q_last)
+    pool = cf.ThreadPoolExecutor(max(1, client_threads))
     t = {}
     t0 = time.perf_counter()
-    for z, q in enumerate(queries):
-        server.upload(z, client.encrypt_query(q))
+    seeds = [client.seed + 1 + z for z in range(B)]
+    client.seed += B
+    cts = list(pool.map(lambda zq: ck.encrypt_values(client.pk, np.asarray(zq[1], np.float64), ck.L, client.scale,
+                                                      seeds[zq[0]]), enumerate(queries)))
+    for z, ct in enumerate(cts):
+        server.upload(z, ct)
     t["client_encrypt_upload"] = time.perf_counter() - t0
     slots = list(range(B))
     t1 = time.perf_counter()
@@ -390,18 +398,20 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k):
     t["server_stage1"] = time.perf_counter() - t1
     # stage 2 (client): decrypt the coarse distances, pick n_probe clusters
     t2 = time.perf_counter()
-    probes = []
-    for z in slots:
+
+    def choose(z):
         d = client.decrypt_distances(coarse[z], lt1, scale, server.S)[:server.ix.n_list]
-        probes.append(np.argsort(d, kind="stable")[:n_probe])
+        return np.argsort(d, kind="stable")[:n_probe]
+
+    probes = list(pool.map(choose, slots))
     t["client_stage2"] = time.perf_counter() - t2
     t3 = time.perf_counter()
     blocks = [server.stage3_fine_candidate_batch_distance(z, probes[z]) for z in slots]
     fine = [server.read_distances(z, len(blocks[z])) for z in slots]
     t["server_stage3"] = time.perf_counter() - t3
     t4 = time.perf_counter()
-    out = []
-    for z in slots:
+
+    def top(z):
         ids, dist = [], []
         for g, b in enumerate(blocks[z]):
             cand = server.blocks[b][1]
@@ -411,6 +421,9 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k):
         ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
         dist = np.concatenate(dist) if dist else np.zeros(0)
         order = np.argsort(dist, kind="stable")[:top_k]
-        out.append((ids[order], dist[order], probes[z]))
+        return ids[order], dist[order], probes[z]
+
+    out = list(pool.map(top, slots))
     t["client_stage4"] = time.perf_counter() - t4
+    pool.shutdown()
     return out, t

@@ -26,7 +26,7 @@ STATUS = {
 OP_NTT, OP_INTT, OP_POLY_ADD, OP_POLY_SUB, OP_POLY_MUL_NTT = 1, 2, 3, 4, 5
 OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
 WORK_MULT_RELIN_RESCALE, WORK_AUTOMORPHISM_KS, WORK_NTT, WORK_INTT = 1, 2, 3, 4
-WORK_MULT_RELIN, WORK_ADD = 5, 6
+WORK_MULT_RELIN, WORK_ADD, WORK_ROTATE_HOISTED = 5, 6, 7
 NONE = (1 << 64) - 1
 
 

@@ -404,6 +404,9 @@ static u64 ks_words(const fhea_ctx *c, u32 l) {
 }
 static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2 * (l - 1)); }
 
+static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
+                      const std::vector<CombineArgs> &comb);
+
 // Key switch of nb inputs c_z = c_base + z*c_stride (l limbs, eval domain):
 // out0_z = add0_z + ks0, out1_z = add1_z + ks1 (key_switch.cpp:167-439 followed by
 // the composite's poly_add).  keys[z] = device pointer of item z's key.
@@ -442,7 +445,17 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     for (u32 z = 1; z < nb; ++z) shared &= keys[z] == keys[0];
     launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
                     c->h.dnum, shared, s);
-    // ModDown of both accumulators
+    return ks_moddown(c, l, nb, acc, pb, tt, comb);
+}
+
+// ModDown of nb accumulator pairs acc_z = [2][l+K][N] (key_switch.cpp:385-427) and the
+// composite's add: out_z = add_z + ModDown(acc_z).  pb: 2*nb*K*N words, tt: 2*nb*l*N.
+static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
+                      const std::vector<CombineArgs> &comb) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const LevelTables &lt = c->lv[l];
+    hipStream_t s = c->stream;
     launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
     launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
@@ -568,6 +581,68 @@ static int mult_relin(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs, const s
     return check_hip(hipGetLastError());
 }
 
+struct RotItem {
+    u64 *d0, *d1;
+    u64 galois;
+    const u64 *key;
+};
+
+// Rotations of one ciphertext (in0, in1) by several Galois elements with one shared
+// ModUp: digits of c1 computed once, then per rotation the permuted inner product
+// with its key, ModDown, and sigma_g(c0) added (rotation = sigma_g(c0) + KS(sigma_g(c1))
+// with sigma_g applied to the digits instead of c1).
+static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const u64 *in0, const u64 *in1,
+                             const std::vector<RotItem> &items) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const LevelTables &lt = c->lv[l];
+    const u32 np = lt.nparts;
+    hipStream_t s = c->stream;
+    const u64 cz = (u64)np * (l + K) * N;
+    const u32 chunk = 32;
+    const u64 per_item = 2 * l * N + 2 * (l + K) * N + 2 * K * N + 2 * l * N;
+    const u32 nmax = (u32)std::min<size_t>(chunk, items.size());
+    int st = ensure_ws(c, l * N + cz + nmax * per_item);
+    if (st) return st;
+    u64 *coef = c->ws, *compact = coef + l * N, *rest = compact + cz;
+    // ModUp of c1, once
+    launch_ntt(t, true, in1, coef, l * N, l * N, 1, l, t.ident, s);
+    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K, coef,
+               l * N, compact, (l + K) * N, cz, 1, s);
+    if (t.use_f64 && !(t.f64_off & 1)) {
+        NttFuse fz;
+        fz.slots = lt.modup_slots;
+        fz.nslots = lt.n_modup_slots;
+        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, 1, np * (l + K), lt.modup_ntt_prime, fz, s);
+    } else {
+        launch_ntt(t, false, compact, compact, cz, cz, 1, np * (l + K), lt.modup_ntt_prime, s);
+    }
+    for (size_t off = 0; off < items.size(); off += chunk) {
+        const u32 nb = (u32)std::min<size_t>(chunk, items.size() - off);
+        u64 *S = rest, *acc = S + (u64)nb * 2 * l * N, *pb = acc + (u64)nb * 2 * (l + K) * N;
+        u64 *tt = pb + (u64)nb * 2 * K * N;
+        std::vector<AutoArgs> aa(nb);
+        std::vector<KsRotArgs> ra(nb);
+        std::vector<CombineArgs> cb(nb);
+        for (u32 z = 0; z < nb; ++z) {
+            const RotItem &it = items[off + z];
+            u64 *s0 = S + (u64)z * 2 * l * N;
+            aa[z] = {in0, in0, s0, s0 + l * N, it.galois};  // sigma_g(c0) (second output unused)
+            ra[z] = {in1, it.key, it.galois};
+            cb[z] = {it.d0, it.d1, s0, nullptr};
+        }
+        auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
+        auto *ra_d = (const KsRotArgs *)stage_args(c, ra.data(), nb * sizeof(KsRotArgs));
+        if (!aa_d || !ra_d) return FHEA_ERR_DEVICE;
+        launch_automorph(t, aa_d, nb, l, s);
+        launch_ks_inner_rot(t, ra_d, compact, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum,
+                            s);
+        st = ks_moddown(c, l, nb, acc, pb, tt, cb);
+        if (st) return st;
+    }
+    return check_hip(hipGetLastError());
+}
+
 struct AutoItem {
     const u64 *in0, *in1;
     u64 *d0, *d1;
@@ -621,6 +696,8 @@ static bool range_ok(u64 off, u64 words, u64 total) { return off <= total && wor
         if ((off) & 7) return FHEA_ERR_ALIGN;                       \
         if (!range_ok((off), (words), (c)->poly_words)) return FHEA_ERR_RANGE; \
     } while (0)
+
+static bool overlaps(u64 a, u64 b, u64 W) { return a < b + W && b < a + W; }
 
 static int key_check(const fhea_ctx *c, u64 evk, u32 key_limbs) {
     if (evk & 7) return FHEA_ERR_ALIGN;
@@ -1091,6 +1168,34 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                 v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
             }
             st = mult_relin(c, l, c->key + c->relin_off, kl, v);
+        } else if (w0.kind == FHEA_WORK_ROTATE_HOISTED) {
+            // runs of items with the same input share one ModUp
+            if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
+            u32 kl = w0.key_limbs ? w0.key_limbs : l;
+            if (kl < l || kl > c->h.L) return FHEA_ERR_ARG;
+            const u64 W = (u64)l * c->h.N;
+            uint32_t a = i;
+            while (a < j) {
+                uint32_t b = a;
+                while (b < j && items[b].a0 == items[a].a0 && items[b].a1 == items[a].a1) ++b;
+                POLY_CHECK(c, items[a].a0, W);
+                POLY_CHECK(c, items[a].a1, W);
+                std::vector<RotItem> v;
+                for (uint32_t k = a; k < b; ++k) {
+                    const fhea_work &w = items[k];
+                    POLY_CHECK(c, w.d0, W);
+                    POLY_CHECK(c, w.d1, W);
+                    if ((w.galois & 1) == 0 || w.galois >= 2ull * c->h.N) return FHEA_ERR_ARG;
+                    if ((st = key_check(c, w.evk, kl))) return st;
+                    // outputs are written while later rotations still read the input
+                    if (overlaps(w.d0, items[a].a0, W) || overlaps(w.d0, items[a].a1, W) ||
+                        overlaps(w.d1, items[a].a0, W) || overlaps(w.d1, items[a].a1, W))
+                        return FHEA_ERR_ARG;
+                    v.push_back({P + w.d0, P + w.d1, w.galois, c->key + w.evk});
+                }
+                if ((st = rotations_hoisted(c, l, kl, P + items[a].a0, P + items[a].a1, v))) return st;
+                a = b;
+            }
         } else if (w0.kind == FHEA_WORK_ADD) {
             if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
             const u64 W = (u64)l * c->h.N;
@@ -1124,8 +1229,6 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
     }
     return FHEA_OK;
 }
-
-static bool overlaps(u64 a, u64 b, u64 W) { return a < b + W && b < a + W; }
 
 int fhea_eval_linear_sum(fhea_ctx *c, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,
                          const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,

@@ -485,6 +485,91 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
 }
 
 // ---------------------------------------------------------------------------
+// Hoisted rotations (no reference counterpart: the reference permutes, then key
+// switches each rotation, fhe_host_driver.cpp:341-358).  The ModUp digits of one
+// ciphertext are computed once; rotation z reads them through its Galois
+// permutation (sigma_g acts on every limb of the extended basis by the same
+// evaluation-domain permutation) and multiplies with its own key.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_ks_inner_rot(const KsRotArgs *__restrict__ args,
+                                                      const u64 *__restrict__ compact, u64 *__restrict__ acc,
+                                                      u64 az, const ModParams *__restrict__ mp, u32 logN, u32 l,
+                                                      u32 K, u32 L, u32 alpha, u32 nparts, u32 key_limbs,
+                                                      u32 dnum) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const KsRotArgs A = args[z];
+    const u32 src = galois_src(r, A.galois, logN);
+    const u32 pidx = (i < l) ? i : (L + i - l);
+    const u32 kl = (i < l) ? i : (key_limbs + i - l);
+    const u64 kstride = (u64)(key_limbs + K) << logN;
+    const u64 *ka = A.key + ((u64)kl << logN) + r;
+    const u64 *kb = ka + (u64)dnum * kstride;
+    u128 s0 = 0, s1 = 0;
+    for (u32 p = 0; p < nparts; ++p) {
+        const u32 start = p * alpha;
+        const u32 sz = (p == nparts - 1) ? (l - start) : alpha;
+        u64 d;
+        if (i >= start && i < start + sz)
+            d = A.c[((size_t)i << logN) + src];
+        else
+            d = compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
+        s0 += (u128)d * kb[(size_t)p * kstride];
+        s1 += (u128)d * ka[(size_t)p * kstride];
+    }
+    const ModParams m = mp[pidx];
+    u64 *o = acc + (size_t)z * az + ((size_t)i << logN) + r;
+    o[0] = reduce128(s0, m);
+    o[(size_t)(l + K) << logN] = reduce128(s1, m);
+}
+
+__global__ __launch_bounds__(TPB) void k_ks_inner_rot_f64(const KsRotArgs *__restrict__ args,
+                                                          const u64 *__restrict__ compact,
+                                                          u64 *__restrict__ acc, u64 az,
+                                                          const ModParams *__restrict__ mp, u32 logN, u32 l,
+                                                          u32 K, u32 L, u32 alpha, u32 nparts, u32 key_limbs,
+                                                          u32 dnum) {
+    const u32 i = blockIdx.y, z = blockIdx.z;
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const KsRotArgs A = args[z];
+    const u32 src = galois_src(r, A.galois, logN);
+    const u32 pidx = (i < l) ? i : (L + i - l);
+    const u32 kl = (i < l) ? i : (key_limbs + i - l);
+    const double q = mp[pidx].qd, qinv = mp[pidx].qinv;
+    const u64 kstride = (u64)(key_limbs + K) << logN;
+    const u64 *ka = A.key + ((u64)kl << logN) + r;
+    const u64 *kb = ka + (u64)dnum * kstride;
+    double s0 = 0.0, s1 = 0.0;  // <= KS_F64_MAX_PARTS terms of <= 0.875q
+    for (u32 p = 0; p < nparts; ++p) {
+        const u32 start = p * alpha;
+        const u32 sz = (p == nparts - 1) ? (l - start) : alpha;
+        u64 dw;
+        if (i >= start && i < start + sz)
+            dw = A.c[((size_t)i << logN) + src];
+        else
+            dw = compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
+        const double d = u64_to_f64(dw);
+        s0 += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)p * kstride]), q, qinv);
+        s1 += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)p * kstride]), q, qinv);
+    }
+    u64 *o = acc + (size_t)z * az + ((size_t)i << logN) + r;
+    o[0] = f64_to_canon(reduce_f64(s0, q, qinv), q);
+    o[(size_t)(l + K) << logN] = f64_to_canon(reduce_f64(s1, q, qinv), q);
+}
+
+void launch_ks_inner_rot(const DevTables &t, const KsRotArgs *args_dev, const u64 *compact, u64 *acc,
+                         u64 acc_zstride, u32 nbatch, u32 l, u32 alpha, u32 nparts, u32 key_limbs, u32 dnum,
+                         hipStream_t s) {
+    dim3 grid(t.N / TPB, l + t.K, nbatch);
+    if (t.use_f64 && !(t.f64_off & 4) && nparts <= KS_F64_MAX_PARTS)
+        hipLaunchKernelGGL(k_ks_inner_rot_f64, grid, dim3(TPB), 0, s, args_dev, compact, acc, acc_zstride, t.mp,
+                           t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+    else
+        hipLaunchKernelGGL(k_ks_inner_rot, grid, dim3(TPB), 0, s, args_dev, compact, acc, acc_zstride, t.mp,
+                           t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+}
+
+// ---------------------------------------------------------------------------
 // Batched ciphertext addition (OP 3 on both polynomials, fhe_host_driver.cpp:248-256).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_add_batch(const AddArgs *__restrict__ args,

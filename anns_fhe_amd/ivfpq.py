@@ -291,7 +291,8 @@ class FheIvfPqServer:
         self.ctx.write_poly(self.slot_base(z), ct)
 
     def prepare(self, slots):
-        """Rotations of every query by 1..D-1 and the encrypted ||q||^2 (all at lt limbs)."""
+        """Rotations of every query by 1..D-1 (hoisted: one ModUp per query) and the
+        encrypted ||q||^2 (all at lt limbs)."""
         N, lt, D = self.N, self.lt, self.D
         Wl = lt * N
         works = []
@@ -299,7 +300,7 @@ class FheIvfPqServer:
             o = self._offs(z)
             for r in range(1, D):
                 d = o["rot"] + (r - 1) * 2 * Wl
-                works.append(dict(kind=2, limbs=lt, key_limbs=lt, a0=o["q0"], a1=o["q1"], d0=d, d1=d + Wl,
+                works.append(dict(kind=7, limbs=lt, key_limbs=lt, a0=o["q0"], a1=o["q1"], d0=d, d1=d + Wl,
                                   galois=self.ck.galois_for_step(r), evk=self.rot_off[r]))
         self.ctx.submit(works)
         self.ctx.submit([dict(kind=5, limbs=lt, a0=self._offs(z)["q0"], a1=self._offs(z)["q1"],

@@ -159,6 +159,10 @@ int fhea_eval_automorphism_ks(fhea_ctx *ctx, uint64_t galois, uint64_t evk, uint
 #define FHEA_WORK_INTT 4               /* a0 -> d0 (limbs), inverse               */
 #define FHEA_WORK_MULT_RELIN 5         /* a0,a1,b0,b1 -> d0,d1, no rescale        */
 #define FHEA_WORK_ADD 6                /* (a0,a1) + (b0,b1) -> (d0,d1)            */
+#define FHEA_WORK_ROTATE_HOISTED 7     /* like 2; consecutive items with the same
+                                          input share one ModUp (sigma_g applied to
+                                          the digits; outputs must not overlap the
+                                          input).  No reference counterpart.       */
 typedef struct {
     uint32_t kind;
     uint32_t limbs;

@@ -535,20 +535,16 @@ void orc_approx_mod_down(const orc_ctx *c, const uint64_t *in_poly, uint64_t *ou
 }
 
 /* key_switch.cpp:167-439 (EvalFastKeySwitchCore; the caller adds c0). */
-void orc_key_switch(const orc_ctx *c, const uint64_t *c_in, const uint64_t *key_mem,
-                    uint64_t evk_off, uint32_t key_limbs, uint64_t *c_out0, uint64_t *c_out1,
-                    uint32_t sizeQl) {
+/* ModUp of every digit (key_switch.cpp:229-349): digits[p] holds the QlP limbs of
+ * digit p in the evaluation domain (own limbs copied, the others converted). */
+static void modup_digits(const orc_ctx *c, const uint64_t *c_in, uint32_t sizeQl, uint64_t *digits) {
     uint32_t N = c->N, K = c->K, L = c->L;
     uint32_t QlP = sizeQl + K;
-    uint64_t stride = (uint64_t)(key_limbs + K) * N;
-    uint64_t evk_a = evk_off, evk_b = evk_off + (uint64_t)c->dnum * stride;
-    uint64_t *ext0 = (uint64_t *)calloc((size_t)QlP * N, 8);
-    uint64_t *ext1 = (uint64_t *)calloc((size_t)QlP * N, 8);
     uint64_t *parts = (uint64_t *)malloc((size_t)c->alpha * N * 8);
     uint64_t *compl_ = (uint64_t *)malloc((size_t)QlP * N * 8);
-    uint64_t *digit = (uint64_t *)malloc((size_t)QlP * N * 8);
     uint32_t np = num_parts(c, sizeQl);
     for (uint32_t part = 0; part < np; part++) {
+        uint64_t *digit = digits + (size_t)part * QlP * N;
         uint32_t sz = part_size(c, sizeQl, part), start = c->alpha * part;
         memcpy(parts, c_in + (size_t)start * N, (size_t)sz * N * 8);
         for (uint32_t i = 0; i < sz; i++) {
@@ -587,7 +583,24 @@ void orc_key_switch(const orc_ctx *c, const uint64_t *c_in, const uint64_t *key_
         }
         for (uint32_t j = 0; j < K; j++)
             memcpy(digit + (size_t)(sizeQl + j) * N, compl_ + (size_t)(o++) * N, (size_t)N * 8);
-        /* evk inner product (key_switch.cpp:352-381) */
+    }
+    free(parts); free(compl_);
+}
+
+/* evk inner product (key_switch.cpp:352-381) of the digits, each read through the
+ * evaluation-domain permutation `map` (NULL = identity), then ModDown of both sums. */
+static void inner_moddown(const orc_ctx *c, const uint64_t *digits, const uint32_t *map,
+                          const uint64_t *key_mem, uint64_t evk_off, uint32_t key_limbs,
+                          uint64_t *c_out0, uint64_t *c_out1, uint32_t sizeQl) {
+    uint32_t N = c->N, K = c->K, L = c->L;
+    uint32_t QlP = sizeQl + K;
+    uint64_t stride = (uint64_t)(key_limbs + K) * N;
+    uint64_t evk_a = evk_off, evk_b = evk_off + (uint64_t)c->dnum * stride;
+    uint64_t *ext0 = (uint64_t *)calloc((size_t)QlP * N, 8);
+    uint64_t *ext1 = (uint64_t *)calloc((size_t)QlP * N, 8);
+    uint32_t np = num_parts(c, sizeQl);
+    for (uint32_t part = 0; part < np; part++) {
+        const uint64_t *digit = digits + (size_t)part * QlP * N;
         for (uint32_t i = 0; i < QlP; i++) {
             uint32_t pi = (i < sizeQl) ? i : (L + i - sizeQl);
             uint32_t kl = (i < sizeQl) ? i : (key_limbs + i - sizeQl);
@@ -595,7 +608,7 @@ void orc_key_switch(const orc_ctx *c, const uint64_t *c_in, const uint64_t *key_
             const uint64_t *ka = key_mem + evk_a + part * stride + (uint64_t)kl * N;
             const uint64_t *kb = key_mem + evk_b + part * stride + (uint64_t)kl * N;
             for (uint32_t r = 0; r < N; r++) {
-                uint64_t d = digit[(size_t)i * N + r];
+                uint64_t d = digit[(size_t)i * N + (map ? map[r] : r)];
                 uint64_t cb = orc_mod_mul(d, kb[r], q, c->m_lo[pi], c->m_hi[pi], c->k[pi]);
                 uint64_t ca = orc_mod_mul(d, ka[r], q, c->m_lo[pi], c->m_hi[pi], c->k[pi]);
                 ext0[(size_t)i * N + r] = orc_mod_add(ext0[(size_t)i * N + r], cb, q);
@@ -605,7 +618,17 @@ void orc_key_switch(const orc_ctx *c, const uint64_t *c_in, const uint64_t *key_
     }
     orc_approx_mod_down(c, ext0, c_out0, sizeQl);
     orc_approx_mod_down(c, ext1, c_out1, sizeQl);
-    free(ext0); free(ext1); free(parts); free(compl_); free(digit);
+    free(ext0); free(ext1);
+}
+
+void orc_key_switch(const orc_ctx *c, const uint64_t *c_in, const uint64_t *key_mem,
+                    uint64_t evk_off, uint32_t key_limbs, uint64_t *c_out0, uint64_t *c_out1,
+                    uint32_t sizeQl) {
+    uint32_t QlP = sizeQl + c->K;
+    uint64_t *digits = (uint64_t *)malloc((size_t)num_parts(c, sizeQl) * QlP * c->N * 8);
+    modup_digits(c, c_in, sizeQl, digits);
+    inner_moddown(c, digits, NULL, key_mem, evk_off, key_limbs, c_out0, c_out1, sizeQl);
+    free(digits);
 }
 
 /* ===================================================================== */
@@ -932,5 +955,44 @@ int orc_linear_sum(const orc_ctx *c, uint64_t *pm, uint64_t pw, const uint64_t *
         memcpy(pm + out1[g], A1, W * 8);
     }
     free(A0); free(A1); free(T);
+    return ST_OK;
+}
+
+int orc_eval_rotations_hoisted(const orc_ctx *c, uint64_t *pm, uint64_t pw, const uint64_t *km,
+                               uint64_t kw, uint64_t in0, uint64_t in1, uint32_t n,
+                               const uint64_t *galois, const uint64_t *evk, uint32_t key_limbs,
+                               const uint64_t *d0, const uint64_t *d1, uint32_t L) {
+    uint32_t N = c->N;
+    if (L < 1 || L > c->L || n == 0) return ST_ARG;
+    uint32_t kl = key_limbs ? key_limbs : L;
+    if (kl < L || kl > c->L) return ST_ARG;
+    uint64_t W = (uint64_t)L * N;
+    CHECK_RANGE(in0, W); CHECK_RANGE(in1, W);
+    for (uint32_t z = 0; z < n; z++) {
+        CHECK_RANGE(d0[z], W); CHECK_RANGE(d1[z], W);
+        if ((galois[z] & 1) == 0 || galois[z] >= 2ull * N) return ST_ARG;
+        if ((evk[z] & 7) || !in_range(evk[z], 2ull * c->dnum * (kl + c->K) * N, kw)) return ST_RANGE;
+    }
+    uint32_t QlP = L + c->K;
+    uint64_t *c0 = (uint64_t *)malloc(W * 8), *c1 = (uint64_t *)malloc(W * 8);
+    memcpy(c0, pm + in0, W * 8);
+    memcpy(c1, pm + in1, W * 8);
+    uint64_t *digits = (uint64_t *)malloc((size_t)num_parts(c, L) * QlP * N * 8);
+    modup_digits(c, c1, L, digits);
+    uint32_t *map = (uint32_t *)malloc((size_t)N * 4);
+    uint64_t *ks0 = (uint64_t *)malloc(W * 8), *ks1 = (uint64_t *)malloc(W * 8);
+    for (uint32_t z = 0; z < n; z++) {
+        orc_galois_map(N, galois[z], map);
+        inner_moddown(c, digits, map, km, evk[z], kl, ks0, ks1, L);
+        for (uint32_t i = 0; i < L; i++) {
+            uint64_t q = c->primes[i];
+            for (uint32_t r = 0; r < N; r++) {
+                uint64_t a = c0[(size_t)i * N + map[r]];
+                pm[d0[z] + (size_t)i * N + r] = orc_mod_add(a, ks0[(size_t)i * N + r], q);
+                pm[d1[z] + (size_t)i * N + r] = ks1[(size_t)i * N + r];
+            }
+        }
+    }
+    free(c0); free(c1); free(digits); free(map); free(ks0); free(ks1);
     return ST_OK;
 }

@@ -139,6 +139,17 @@ int orc_linear_sum(const orc_ctx *c, uint64_t *poly_mem, uint64_t poly_words, co
                    uint32_t G, uint64_t bias0, uint64_t bias1, const uint64_t *out0,
                    const uint64_t *out1, uint32_t limbs);
 
+/* Rotations of one ciphertext (in0, in1) by n Galois elements sharing one ModUp:
+ * the digits of c1 are computed once and read through each sigma_g's
+ * evaluation-domain map; out_z = (sigma_g(c0) + ks0, ks1).  The engine's hoisted
+ * rotation (no reference counterpart), composed of pinned parts: modup_digits and
+ * inner_moddown are the two halves of orc_key_switch. */
+int orc_eval_rotations_hoisted(const orc_ctx *c, uint64_t *poly_mem, uint64_t poly_words,
+                               const uint64_t *key_mem, uint64_t key_words, uint64_t in0,
+                               uint64_t in1, uint32_t n, const uint64_t *galois,
+                               const uint64_t *evk, uint32_t key_limbs, const uint64_t *d0,
+                               const uint64_t *d1, uint32_t limbs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,9 @@ def lib():
                                                    C.c_uint64, C.c_uint32] + [C.c_uint64] * 4 + [C.c_uint32]),
             "orc_eval_mult_relin": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
                                               C.c_uint32] + [C.c_uint64] * 6 + [C.c_uint32]),
+            "orc_eval_rotations_hoisted": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
+                                                     C.c_uint64, C.c_uint32, u64p, u64p, C.c_uint32, u64p, u64p,
+                                                     C.c_uint32]),
             "orc_linear_sum": (C.c_int, [C.c_void_p, u64p, C.c_uint64, u64p, u64p, C.c_uint32, u64p, u64p,
                                          C.c_uint32, C.c_uint64, C.c_uint64, u64p, u64p, C.c_uint32]),
         }
@@ -276,3 +279,11 @@ class OracleCtx:
         b0, b1 = bias if bias is not None else (none, none)
         return lib().orc_linear_sum(self.ptr, P64(pm), pm.size, P64(c0), P64(c1), R, P64(pt), P64(pb), G, b0, b1,
                                     P64(o0), P64(o1), limbs)
+
+    def eval_rotations_hoisted(self, pm, km, in0, in1, galois, evk, key_limbs, d0, d1, limbs):
+        g = np.ascontiguousarray(galois, dtype=np.uint64)
+        e = np.ascontiguousarray(evk, dtype=np.uint64)
+        o0 = np.ascontiguousarray(d0, dtype=np.uint64)
+        o1 = np.ascontiguousarray(d1, dtype=np.uint64)
+        return lib().orc_eval_rotations_hoisted(self.ptr, P64(pm), pm.size, P64(km), km.size, in0, in1, g.size,
+                                                P64(g), P64(e), key_limbs, P64(o0), P64(o1), limbs)

@@ -44,20 +44,38 @@ class OracleContext:
         self.km[off:off + a.size] = a
 
     def submit(self, works):
-        for w in works:
-            k, l = w["kind"], w["limbs"]
-            if k == 2:
-                st = self.o.eval_automorphism_ks(self.pm, self.km, w["galois"], w["evk"], w.get("key_limbs", l),
-                                                 w["a0"], w["a1"], w["d0"], w["d1"], l)
-            elif k == 5:
-                off, kl = self.relin
-                st = self.o.eval_mult_relin(self.pm, self.km, off, kl, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"],
-                                            w["d1"], l)
-            elif k == 6:
-                st = self.o.eval_add(self.pm, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"], w["d1"], l)
-            else:
-                raise ValueError(k)
-            assert st == 0, (k, st)
+        i = 0
+        while i < len(works):
+            w = works[i]
+            if w["kind"] == 7:  # runs of the same input share one ModUp
+                j = i
+                while j < len(works) and works[j]["kind"] == 7 and works[j]["a0"] == w["a0"] and \
+                        works[j]["a1"] == w["a1"]:
+                    j += 1
+                run = works[i:j]
+                st = self.o.eval_rotations_hoisted(self.pm, self.km, w["a0"], w["a1"], [r["galois"] for r in run],
+                                                   [r["evk"] for r in run], w.get("key_limbs", w["limbs"]),
+                                                   [r["d0"] for r in run], [r["d1"] for r in run], w["limbs"])
+                assert st == 0, st
+                i = j
+                continue
+            self._one(w)
+            i += 1
+
+    def _one(self, w):
+        k, l = w["kind"], w["limbs"]
+        if k == 2:
+            st = self.o.eval_automorphism_ks(self.pm, self.km, w["galois"], w["evk"], w.get("key_limbs", l),
+                                             w["a0"], w["a1"], w["d0"], w["d1"], l)
+        elif k == 5:
+            off, kl = self.relin
+            st = self.o.eval_mult_relin(self.pm, self.km, off, kl, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"],
+                                        w["d1"], l)
+        elif k == 6:
+            st = self.o.eval_add(self.pm, w["a0"], w["a1"], w["b0"], w["b1"], w["d0"], w["d1"], l)
+        else:
+            raise ValueError(k)
+        assert st == 0, (k, st)
 
     def linear_sum(self, ct0, ct1, pt, out0, out1, limbs, ptb=None, bias=None):
         assert self.o.linear_sum(self.pm, ct0, ct1, pt, out0, out1, limbs, ptb=ptb, bias=bias) == 0

@@ -461,3 +461,40 @@ def test_submit_mult_relin_and_add_parity(orc):
         assert o.eval_add(pm, 8 * W, 9 * W, 10 * W, 11 * W, 12 * W, 13 * W, limbs) == 0
         for off in (8, 9, 10, 11, 12, 13):
             assert np.array_equal(ctx.read_poly(off * W, limbs * N), pm[off * W:off * W + limbs * N]), (limbs, off)
+
+
+@pytest.mark.parametrize("q_bits", [50, 60])
+def test_hoisted_rotations_parity(orc, q_bits):
+    """Work kind 7 (runs of rotations of the same input share one ModUp) against the
+    oracle's hoisted rotations, two inputs in one submit, both arithmetic paths."""
+    N, L = 4096, 4
+    ctx = get_ctx(N, L, 2, 2, q_bits=q_bits)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(q_bits + 1)
+    kidx = list(range(L)) + [L + j for j in range(ctx.K)]
+    ksz = ctx.key_size_words(L)
+    n_keys = 5
+    km = np.concatenate([rand_poly(rng, [ctx.primes[i] for i in kidx], N).reshape(-1)
+                         for _ in range(2 * ctx.dnum * n_keys)])
+    ctx.write_key(0, km)
+    W = L * N
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+    ctx.write_poly(0, data)
+    gal = [ctx.galois_for_step(s) for s in (1, 2, 7, -3, 11)]
+    works = []
+    for inp in range(2):
+        for z in range(5 - inp * 2):
+            k = len(works)
+            works.append(dict(kind=7, limbs=L, key_limbs=L, a0=2 * inp * W, a1=(2 * inp + 1) * W,
+                              d0=(8 + 2 * k) * W, d1=(9 + 2 * k) * W, galois=gal[z], evk=z * ksz))
+    say("hoisted", q_bits, len(works))
+    ctx.submit(works)
+    got = ctx.read_poly(8 * W, 2 * len(works) * W)
+    pm = np.zeros((8 + 2 * len(works)) * W, np.uint64)
+    pm[:4 * W] = data
+    for inp in range(2):
+        ws = [w for w in works if w["a0"] == 2 * inp * W]
+        assert o.eval_rotations_hoisted(pm, km, ws[0]["a0"], ws[0]["a1"], [w["galois"] for w in ws],
+                                        [w["evk"] for w in ws], L, [w["d0"] for w in ws], [w["d1"] for w in ws],
+                                        L) == 0
+    assert np.array_equal(got, pm[8 * W:])

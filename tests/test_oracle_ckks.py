@@ -270,3 +270,38 @@ def test_mult_relin_then_rescale_is_mult_relin_rescale(orc):
     W1 = (L - 1) * N
     assert np.array_equal(r0, pm[6 * W:6 * W + W1])
     assert np.array_equal(r1, pm[7 * W:7 * W + W1])
+
+
+def test_hoisted_rotations_pinned(orc):
+    """Hoisted rotations (one ModUp, permuted digits): with g = 1 they equal the
+    pinned automorphism + key switch word for word; with real keys every output
+    decrypts to the rotated slots."""
+    from anns_fhe_amd.client import CkksClient
+    N, L, K, dnum = 1024, 3, 2, 2
+    cl = CkksClient(N, L, K, dnum)
+    o = orc.OracleCtx(N, cl.q, cl.p, dnum)
+    s = cl.keygen_secret(3)
+    pk = cl.keygen_public(s, 4)
+    ksz = cl.key_size_words(L)
+    steps = [1, 3, -2, 100]
+    km = np.zeros((len(steps) + 1) * ksz, np.uint64)
+    gal = [cl.galois_for_step(st) for st in steps] + [1]
+    for z, g in enumerate(gal):
+        km[z * ksz:(z + 1) * ksz] = cl.rotation_key(s, g, L, 10 + z)
+    W = L * N
+    n = N // 2
+    x = np.linspace(-1, 1, n)
+    pm = np.zeros(16 * W, np.uint64)
+    pm[:2 * W] = cl.encrypt_values(pk, x, L, 2.0 ** 40, 5)
+    outs0 = [(2 + 2 * z) * W for z in range(len(gal))]
+    outs1 = [(3 + 2 * z) * W for z in range(len(gal))]
+    assert o.eval_rotations_hoisted(pm, km, 0, W, gal, [z * ksz for z in range(len(gal))], L, outs0, outs1, L) == 0
+    # identity element: hoisting changes nothing
+    ref = pm.copy()
+    assert o.eval_automorphism_ks(ref, km, 1, len(steps) * ksz, L, 0, W, 14 * W, 15 * W, L) == 0
+    assert np.array_equal(pm[outs0[-1]:outs0[-1] + W], ref[14 * W:15 * W])
+    assert np.array_equal(pm[outs1[-1]:outs1[-1] + W], ref[15 * W:16 * W])
+    for z, st in enumerate(steps):
+        ct = np.concatenate([pm[outs0[z]:outs0[z] + W], pm[outs1[z]:outs1[z] + W]])
+        got = cl.decrypt_values(s, ct, L, 2.0 ** 40, n)
+        assert np.max(np.abs(got - np.roll(x, -st))) < 1e-6, st

@@ -68,6 +68,8 @@ SIGNATURES = {
     "fhea_write_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_copy_poly": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
+    "fhea_host_alloc": (vp, [C.c_uint64]),
+    "fhea_host_free": (C.c_int, [vp]),
     "fhea_poly_arena": (vp, [vp]),
     "fhea_key_arena": (vp, [vp]),
     "fhea_poly_words": (C.c_uint64, [vp]),

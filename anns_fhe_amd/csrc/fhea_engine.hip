@@ -875,6 +875,17 @@ int fhea_read_key(fhea_ctx *c, uint64_t off, uint64_t *h, uint64_t n) {
     return c ? arena_copy(c, c->key, c->key_words, off, h, n, false) : FHEA_ERR_STATE;
 }
 
+void *fhea_host_alloc(uint64_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+int fhea_host_free(void *p) {
+    if (!p) return FHEA_OK;
+    return hipHostFree(p) == hipSuccess ? FHEA_OK : FHEA_ERR_DEVICE;
+}
+
 int fhea_copy_poly(fhea_ctx *c, uint64_t dst, uint64_t src, uint64_t n) {
     CTX_CHECK(c);
     if (!range_ok(dst, n, c->poly_words) || !range_ok(src, n, c->poly_words)) return FHEA_ERR_RANGE;

@@ -152,7 +152,7 @@ class FheIvfPqClient:
     distances, cluster selection and top-k (fhe_context_manager.cpp:457-512,
     interactive_query_manager.cpp:166-174,276-290)."""
 
-    def __init__(self, ckks, dim=128, log_scale=45, lt=3, seed=1):
+    def __init__(self, ckks, dim=128, log_scale=40, lt=2, seed=1):
         self.ck = ckks
         self.D = dim
         self.scale = 2.0 ** log_scale
@@ -194,7 +194,7 @@ class FheIvfPqServer:
     block b: D plaintexts pt_r (lt limbs each) then ptb (||x_s||^2);
     query slot: ct (L limbs x 2), D-1 rotations, norm, tmp, outputs, rescaled outputs."""
 
-    def __init__(self, ctx, ckks, index, lt=3, log_scale=45, max_queries=64, max_blocks_per_query=None,
+    def __init__(self, ctx, ckks, index, lt=2, log_scale=40, max_queries=64, max_blocks_per_query=None,
                  threads=0, log=None):
         self.ctx, self.ck, self.ix = ctx, ckks, index
         self.N, self.L, self.lt = ctx.N, ctx.L, lt
@@ -226,6 +226,7 @@ class FheIvfPqServer:
         self.slot_words = (self.ct_words + (self.D - 1) * 2 * Wl + 4 * Wl + self.max_g * 2 * Wl
                            + self.max_g * 2 * (lt - 1) * self.N)
         self.max_queries = max_queries
+        self._pinned = None  # page-locked read buffer (engine contexts)
         self.use_graphs = False
         self.max_graphs = 16
         self.graphs = {}
@@ -338,6 +339,12 @@ class FheIvfPqServer:
 
     def read_distances(self, z, G):
         n = 2 * (self.lt - 1) * self.N
+        if self._pinned is None and hasattr(self.ctx, "graph_begin"):
+            from .driver import PinnedArray
+            self._pinned = PinnedArray(self.max_g * n)
+        if self._pinned is not None:
+            buf = self.ctx.read_poly(self._offs(z)["res"], G * n, out=self._pinned.array)
+            return buf.reshape(G, n).copy()
         return self.ctx.read_poly(self._offs(z)["res"], G * n).reshape(G, n)
 
     # ---- stages (interactive_query_manager.cpp:85-274)

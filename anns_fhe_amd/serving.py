@@ -52,7 +52,7 @@ class Request:
 class ParamSet:
     """One parameter set: context, client keys, encoded database, slots."""
 
-    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=3, log_scale=45, seed=1, threads=0, log=None):
+    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=2, log_scale=40, seed=1, threads=0, log=None):
         self.name = name
         self.ctx, self.ck = ctx, ck
         self.max_g = max_g
@@ -67,7 +67,7 @@ class ParamSet:
         self.slot_bytes = 8 * self.server.slot_words
 
     @staticmethod
-    def sizes(N, L, K, dnum, index, max_slots, lt=3, max_probe=8):
+    def sizes(N, L, K, dnum, index, max_slots, lt=2, max_probe=8):
         """(poly_words, key_words, max_g) for a set serving `index` with up to max_probe probes."""
         nb = N // 2
         n_blocks = 1 + sum(-(-len(l) // nb) for l in index.lists)
@@ -77,7 +77,7 @@ class ParamSet:
         return words, index.dim * 2 * dnum * (lt + K) * N + 8, max_g
 
     @classmethod
-    def create(cls, name, N, L, K, dnum, index, max_slots, lt=3, log_scale=45, max_probe=8, seed=1, threads=0,
+    def create(cls, name, N, L, K, dnum, index, max_slots, lt=2, log_scale=40, max_probe=8, seed=1, threads=0,
                log=None):
         words, kw, max_g = cls.sizes(N, L, K, dnum, index, max_slots, lt, max_probe)
         ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=kw)

@@ -224,7 +224,7 @@ def bench_ivfpq(args):
     from anns_fhe_amd.client import CkksClient
     from anns_fhe_amd.driver import FheContext
     n, B, n_probe = args.db_size, args.queries, 4
-    Nq, Lq, Kq, dq, lt = 1 << 14, 8, 4, 2, 3
+    Nq, Lq, Kq, dq, lt = 1 << 14, 8, 4, 2, 2
     t0 = time.perf_counter()
     x, _ = IV.make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
@@ -236,9 +236,9 @@ def bench_ivfpq(args):
     words = IV.FheIvfPqServer.arena_words(Nq, Lq, lt, 128, n_blocks, B, max_g)
     ctx = FheContext(Nq, Lq, Kq, dq, poly_words=words, key_words=128 * 2 * dq * (lt + Kq) * Nq + 8)
     ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-    client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=11)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
     t1 = time.perf_counter()
-    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=B, max_blocks_per_query=max_g,
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=max_g,
                                log=log)
     server.use_graphs = not args.no_graphs
     t_encode = time.perf_counter() - t1

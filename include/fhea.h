@@ -108,6 +108,10 @@ int fhea_write_key(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint6
 int fhea_read_key(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
 int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t count);
 /* Device base pointers of the arenas (poly_gmem_ptr / key_gmem_ptr, sim_memory_bus.h:16-17). */
+/* Page-locked host memory (hipHostMalloc) for arena reads and writes at full DMA
+ * speed; NULL on failure.  Free with fhea_host_free. */
+void *fhea_host_alloc(uint64_t bytes);
+int fhea_host_free(void *p);
 uint64_t *fhea_poly_arena(fhea_ctx *ctx);
 uint64_t *fhea_key_arena(fhea_ctx *ctx);
 uint64_t fhea_poly_words(const fhea_ctx *ctx);

@@ -64,7 +64,7 @@ def test_protocol_words_equal_oracle(orc):
 def test_encrypted_search_recall_c4_shape():
     from anns_fhe_amd.client import CkksClient
     from anns_fhe_amd.driver import FheContext
-    N, L, K, dnum, lt = 1 << 14, 8, 4, 2, 3
+    N, L, K, dnum, lt = 1 << 14, 8, 4, 2, 2
     say("dataset + index")
     x, _ = IV.make_gaussian_mixture(40000, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=20000, iters=10, seed=0).add(x)
@@ -74,9 +74,9 @@ def test_encrypted_search_recall_c4_shape():
     ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=128 * 2 * dnum * (lt + K) * N + 8)
     try:
         ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-        client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=11)
+        client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
         say("encode database")
-        server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=B, max_blocks_per_query=8)
+        server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=8)
         say("keys")
         rel, rots = client.evaluation_keys()
         server.load_keys(rel, rots)
@@ -107,7 +107,7 @@ def test_stage1_graph_replay_equals_eager():
     from anns_fhe_amd.driver import FheContext
     x, _ = IV.make_gaussian_mixture(4000, dim=128, n_centres=4, sigma=0.1, seed=3)
     ix = IV.IvfPqIndex(n_list=4, M=8, K=256).train(x, train_size=4000, iters=6, seed=1).add(x)
-    N, L, K, dnum, lt = 8192, 5, 2, 3, 3
+    N, L, K, dnum, lt = 8192, 5, 2, 3, 2
     n_blocks = 1 + sum(-(-len(l) // (N // 2)) for l in ix.lists)
     words = IV.FheIvfPqServer.arena_words(N, L, lt, 128, n_blocks, 3, 4)
     ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=128 * 2 * dnum * (lt + K) * N + 8)

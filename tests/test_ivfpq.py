@@ -55,14 +55,14 @@ def test_encrypted_protocol_on_oracle_matches_plaintext(orc, small_index):
     from anns_fhe_amd.client import CkksClient
     from tests.oracle_ctx import OracleContext
     x, ix = small_index
-    N, L, K, dnum, lt = 4096, 4, 2, 2, 3
+    N, L, K, dnum, lt = 4096, 4, 2, 2, 2
     ck = CkksClient(N, L, K, dnum)
     n_blocks = 1 + sum(-(-len(l) // (N // 2)) for l in ix.lists)
     words = IV.FheIvfPqServer.arena_words(N, L, lt, 128, n_blocks, 2, 4)
     ksz = ck.key_size_words(lt)
     ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, 128 * ksz + 8)
-    client = IV.FheIvfPqClient(ck, dim=128, log_scale=45, lt=lt, seed=7)
-    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=45, max_queries=2, max_blocks_per_query=4)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=7)
+    server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=2, max_blocks_per_query=4)
     rel, rots = client.evaluation_keys()
     server.load_keys(rel, rots)
     rng = np.random.default_rng(4)

@@ -28,6 +28,7 @@ OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
 WORK_MULT_RELIN_RESCALE, WORK_AUTOMORPHISM_KS, WORK_NTT, WORK_INTT = 1, 2, 3, 4
 WORK_MULT_RELIN, WORK_ADD, WORK_ROTATE_HOISTED = 5, 6, 7
 NONE = (1 << 64) - 1
+TUNE_NTT_SPLIT_FWD, TUNE_NTT_SPLIT_INV, TUNE_NTT_WG_PER_CU, TUNE_CHUNK = 1, 2, 3, 4
 
 
 class FheLibraryError(RuntimeError):
@@ -68,6 +69,8 @@ SIGNATURES = {
     "fhea_write_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_copy_poly": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
+    "fhea_ctx_set_tuning": (C.c_int, [vp, C.c_int, C.c_uint32]),
+    "fhea_ctx_get_tuning": (C.c_int, [vp, C.c_int, u32p]),
     "fhea_host_alloc": (vp, [C.c_uint64]),
     "fhea_host_free": (C.c_int, [vp]),
     "fhea_poly_arena": (vp, [vp]),

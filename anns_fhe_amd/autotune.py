@@ -1,0 +1,130 @@
+"""Schedule autotuner for the engine on the device it runs on.
+
+Picks, by measurement, the knobs of fhea_ctx_set_tuning that change speed but
+not results:
+  * the split of a two-pass FP64 NTT (stages in the first pass) for forward and
+    inverse transforms, among every split whose passes have 6..12 stages;
+  * the number of resident NTT workgroups per CU;
+  * the number of batch items per key-switch pipeline chunk, on a caller-given
+    list of work items (for example the HMult batch being served).
+Every NTT candidate is checked against the default schedule's output word for
+word before it can be chosen.  Choices can be cached in a JSON file keyed by the
+parameter set.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+KNOBS = {"ntt_split_fwd": _lib.TUNE_NTT_SPLIT_FWD, "ntt_split_inv": _lib.TUNE_NTT_SPLIT_INV,
+         "ntt_wg_per_cu": _lib.TUNE_NTT_WG_PER_CU, "chunk": _lib.TUNE_CHUNK}
+
+
+def _time(ctx, fn, reps):
+    fn()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        fn()
+    return ctx.timer_stop() / reps
+
+
+def ntt_splits(logN):
+    return [k for k in range(6, 13) if 6 <= logN - k <= 12] if logN > 12 else []
+
+
+def tune_ntt(ctx, scratch_off, polys, limbs, reps=5, log=None):
+    """Chooses the forward and inverse splits and the workgroups per CU on
+    `polys` x `limbs` limbs of uniform data written at scratch_off (2 copies:
+    the arena must hold 2 * polys * limbs * N words there).  Returns the timings."""
+    N = ctx.N
+    logN = N.bit_length() - 1
+    W = polys * limbs * N
+    rng = np.random.default_rng(12345)
+    data = np.concatenate([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in ctx.q[:limbs]]).reshape(-1)
+                           for _ in range(polys)])
+    src, dst = scratch_off, scratch_off + W
+    ctx.write_poly(src, data)
+    timings = {}
+    for inverse, knob in ((False, "ntt_split_fwd"), (True, "ntt_split_inv")):
+        ctx.set_tuning(KNOBS[knob], 0)
+        ctx.ntt_batch(src, dst, polys, limbs, inverse)
+        ref = ctx.read_poly(dst, W)
+        best = (None, 0)
+        for k in [0] + ntt_splits(logN):
+            ctx.set_tuning(KNOBS[knob], k)
+            ctx.ntt_batch(src, dst, polys, limbs, inverse)
+            if not np.array_equal(ctx.read_poly(dst, W), ref):
+                raise RuntimeError(f"NTT split {k} ({knob}) changed the result")
+            ms = _time(ctx, lambda: ctx.ntt_batch(src, dst, polys, limbs, inverse), reps)
+            timings[f"{knob}={k}"] = round(ms, 4)
+            if best[0] is None or ms < best[0]:
+                best = (ms, k)
+        ctx.set_tuning(KNOBS[knob], best[1])
+        if log:
+            log(f"autotune {knob}: {best[1]} ({best[0]:.3f} ms)")
+    best = (None, 0)
+    for w in (0, 4, 3, 2):
+        ctx.set_tuning(KNOBS["ntt_wg_per_cu"], w)
+        ms = _time(ctx, lambda: (ctx.ntt_batch(src, dst, polys, limbs, False),
+                                 ctx.ntt_batch(dst, dst, polys, limbs, True)), reps)
+        timings[f"ntt_wg_per_cu={w}"] = round(ms, 4)
+        if best[0] is None or ms < best[0]:
+            best = (ms, w)
+    ctx.set_tuning(KNOBS["ntt_wg_per_cu"], best[1])
+    return timings
+
+
+def tune_chunk(ctx, works, candidates=(8, 16, 32), reps=3, log=None):
+    """Chooses the key-switch chunk size on a representative list of work items."""
+    timings = {}
+    best = (None, ctx.get_tuning(KNOBS["chunk"]))
+    for c in candidates:
+        ctx.set_tuning(KNOBS["chunk"], c)
+        ms = _time(ctx, lambda: ctx.submit(works), reps)
+        timings[f"chunk={c}"] = round(ms, 4)
+        if best[0] is None or ms < best[0]:
+            best = (ms, c)
+    ctx.set_tuning(KNOBS["chunk"], best[1])
+    if log:
+        log(f"autotune chunk: {best[1]} ({best[0]:.3f} ms)")
+    return timings
+
+
+def current(ctx):
+    return {k: ctx.get_tuning(v) for k, v in KNOBS.items()}
+
+
+def apply(ctx, choices):
+    for k, v in choices.items():
+        ctx.set_tuning(KNOBS[k], int(v))
+
+
+def cache_key(ctx):
+    return f"N{ctx.N}_L{ctx.L}_K{ctx.K}_dnum{ctx.dnum}_q{ctx.q[0].bit_length()}"
+
+
+def load_cached(ctx, path):
+    """Applies cached choices for this parameter set; returns them or None."""
+    if path and os.path.exists(path):
+        with open(path) as f:
+            entry = json.load(f).get(cache_key(ctx))
+        if entry:
+            apply(ctx, entry)
+            return entry
+    return None
+
+
+def save_cached(ctx, path):
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[cache_key(ctx)] = current(ctx)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)

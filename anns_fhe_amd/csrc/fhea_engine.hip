@@ -875,6 +875,41 @@ int fhea_read_key(fhea_ctx *c, uint64_t off, uint64_t *h, uint64_t n) {
     return c ? arena_copy(c, c->key, c->key_words, off, h, n, false) : FHEA_ERR_STATE;
 }
 
+int fhea_ctx_set_tuning(fhea_ctx *c, int knob, uint32_t v) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    const u32 lg = c->dt.logN;
+    switch (knob) {
+    case FHEA_TUNE_NTT_SPLIT_FWD:
+    case FHEA_TUNE_NTT_SPLIT_INV:
+        if (v != 0 && (lg <= 12 || v < 6 || v > 12 || lg - v < 6 || lg - v > 12)) return FHEA_ERR_ARG;
+        (knob == FHEA_TUNE_NTT_SPLIT_FWD ? c->dt.k1_fwd : c->dt.k1_inv) = v;
+        return FHEA_OK;
+    case FHEA_TUNE_NTT_WG_PER_CU:
+        if (v > 4) return FHEA_ERR_ARG;
+        c->dt.wg_per_cu = v;
+        return FHEA_OK;
+    case FHEA_TUNE_CHUNK:
+        if (v < 1 || v > 64) return FHEA_ERR_ARG;
+        c->max_chunk = v;
+        return FHEA_OK;
+    default:
+        return FHEA_ERR_ARG;
+    }
+}
+
+int fhea_ctx_get_tuning(const fhea_ctx *c, int knob, uint32_t *v) {
+    if (!c || !c->poly) return FHEA_ERR_STATE;
+    if (!v) return FHEA_ERR_ARG;
+    switch (knob) {
+    case FHEA_TUNE_NTT_SPLIT_FWD: *v = c->dt.k1_fwd; return FHEA_OK;
+    case FHEA_TUNE_NTT_SPLIT_INV: *v = c->dt.k1_inv; return FHEA_OK;
+    case FHEA_TUNE_NTT_WG_PER_CU: *v = c->dt.wg_per_cu; return FHEA_OK;
+    case FHEA_TUNE_CHUNK: *v = c->max_chunk; return FHEA_OK;
+    default: return FHEA_ERR_ARG;
+    }
+}
+
 void *fhea_host_alloc(uint64_t bytes) {
     void *p = nullptr;
     if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) return nullptr;

@@ -21,6 +21,9 @@ struct DevTables {
     bool use_f64 = false;
     u32 f64_off = 0;
     u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
+    // tuning (fhea_ctx_set_tuning): stages of the first pass of a two-pass FP64
+    // forward / inverse NTT (0 = logN/2), persistent workgroups per CU (0 = occupancy)
+    u32 k1_fwd = 0, k1_inv = 0, wg_per_cu = 0;
     const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
     const double *itwf = nullptr;   // [T][N][2]
     const double *ninvf = nullptr;  // [T][2]

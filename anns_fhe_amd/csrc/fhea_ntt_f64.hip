@@ -42,6 +42,25 @@
 namespace fhea {
 
 static constexpr u32 F_TPB = 256;
+
+// Build-time schedule knobs (defaults are the measured best on MI355X):
+//   FHEA_NTT_PREFETCH  1: a tile's global loads are issued one tile ahead (registers)
+//   FHEA_NTT_WAVES     >0: register budget for that many waves per SIMD
+//   FHEA_NTT_STAGE_FENCE 1: scheduling barrier at every butterfly stage
+#ifndef FHEA_NTT_PREFETCH
+#define FHEA_NTT_PREFETCH 1
+#endif
+#ifndef FHEA_NTT_WAVES
+#define FHEA_NTT_WAVES 0
+#endif
+#ifndef FHEA_NTT_STAGE_FENCE
+#define FHEA_NTT_STAGE_FENCE 0
+#endif
+#if FHEA_NTT_WAVES > 0
+#define FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu(FHEA_NTT_WAVES)))
+#else
+#define FHEA_NTT_ATTR
+#endif
 static constexpr u32 F_LOG_TILE = 12;
 static constexpr u32 F_TILE = 1u << F_LOG_TILE;
 
@@ -74,6 +93,10 @@ __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, 
     // stage st of the round uses 2^(3-wb) distinct (w, w/q) pairs, wb = its window bit
 #pragma unroll
     for (int st = 0; st < R::NS; ++st) {
+#if FHEA_NTT_STAGE_FENCE
+        // keep each stage's twiddle loads inside the stage (fewer live registers)
+        __builtin_amdgcn_sched_barrier(0);
+#endif
         const int b = INV ? R::BIT0 + st : R::BIT0 - st;
         const int wb = b - R::P;
         const u32 k = (u32)b + g.logS + 1;
@@ -183,20 +206,6 @@ __device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_li
     return id;
 }
 
-// Build-time schedule knobs (defaults are the measured best on MI355X):
-//   FHEA_NTT_PREFETCH  1: a tile's global loads are issued one tile ahead (registers)
-//   FHEA_NTT_WAVES     >0: register budget for that many waves per SIMD
-#ifndef FHEA_NTT_PREFETCH
-#define FHEA_NTT_PREFETCH 1
-#endif
-#ifndef FHEA_NTT_WAVES
-#define FHEA_NTT_WAVES 0
-#endif
-#if FHEA_NTT_WAVES > 0
-#define FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu(FHEA_NTT_WAVES)))
-#else
-#define FHEA_NTT_ATTR
-#endif
 
 template <bool INV, int LOGR, bool COLS>
 __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
 }
 
 template <bool INV, bool COLS>
-static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
     const u32 tpl = grid.x, ntiles = grid.x * grid.y;
@@ -371,7 +380,8 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, hipStream_t
                 n = 4;                                                                               \
             per_cu = n < 4 ? n : 4;                                                                  \
         }                                                                                            \
-        const u32 cap = max_wg / 4 * (u32)per_cu;                                                    \
+        const u32 use_cu = (wg_cap && wg_cap < (u32)per_cu) ? wg_cap : (u32)per_cu;                   \
+        const u32 cap = max_wg / 4 * use_cu;                                                          \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
         hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
                            dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles,   \
@@ -403,27 +413,28 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
     if (lg <= F_LOG_TILE) {
         dim3 grid(1, npolys * ylimbs);
         if (inverse)
-            launch_pass<true, false>(lg, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
+            launch_pass<true, false>(lg, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg, lg,
                                      1, 1, 1, none);
         else
-            launch_pass<false, false>(lg, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+            launch_pass<false, false>(lg, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                       lg, 1, 1, 0, fz);
         return;
     }
-    const u32 k1 = lg / 2, k2 = lg - k1;
+    const u32 k1s = inverse ? t.k1_inv : t.k1_fwd;
+    const u32 k1 = k1s ? k1s : lg / 2, k2 = lg - k1;
     dim3 grid(1u << (lg - F_LOG_TILE), npolys * ylimbs);
     if (!inverse) {
         NttFuse pro = fz, epi = fz;
         pro.epi = FUSE_NONE;
         epi.pro = FUSE_NONE;
-        launch_pass<false, true>(k1, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, pro);
-        launch_pass<false, false>(k2, grid, npolys, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<false, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                   F_LOG_TILE, 0, 1, 0, epi);
     } else {
-        launch_pass<true, false>(k2, grid, npolys, t.max_wg, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                  F_LOG_TILE, 1, 0, 0, none);
-        launch_pass<true, true>(k1, grid, npolys, t.max_wg, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
+        launch_pass<true, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
                                 F_LOG_TILE, 0, 1, 1, none);
     }
 }

@@ -160,6 +160,14 @@ class FheContext:
     def graph_destroy(self, gid):
         check(self.lib.fhea_graph_destroy(self.h, gid), "fhea_graph_destroy")
 
+    def set_tuning(self, knob, value):
+        check(self.lib.fhea_ctx_set_tuning(self.h, knob, value), "fhea_ctx_set_tuning")
+
+    def get_tuning(self, knob):
+        v = C.c_uint32()
+        check(self.lib.fhea_ctx_get_tuning(self.h, knob, C.byref(v)), "fhea_ctx_get_tuning")
+        return v.value
+
     def sync(self):
         check(self.lib.fhea_sync(self.h), "fhea_sync")
 

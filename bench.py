@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="single-ciphertext op-by-op HMult runs timed for plain_path (0 = skip)")
     return ap.parse_args()
@@ -404,6 +405,14 @@ def main():
     works = [dict(kind=1, limbs=L, a0=z * per_item, a1=z * per_item + W, b0=z * per_item + 2 * W,
                   b1=z * per_item + 3 * W, d0=z * per_item + 4 * W, d1=z * per_item + 4 * W + W1)
              for z in range(B)]
+    schedule = None
+    if not args.no_tune:
+        from anns_fhe_amd import autotune
+        t0 = time.perf_counter()
+        autotune.tune_ntt(ctx, B * per_item, 4, L, reps=3, log=log)
+        autotune.tune_chunk(ctx, works, candidates=(4, 8, 16), reps=2, log=log)
+        schedule = autotune.current(ctx)
+        log(f"autotune {schedule} in {time.perf_counter() - t0:.1f} s")
     log("warmup")
     for _ in range(args.warmup):
         ctx.submit(works)
@@ -453,7 +462,8 @@ def main():
         "data": "synthetic (uniform residues, seeded keys)",
         "config": {"workload": "batched eval_mult_relin_rescale, N=2^16, L=24->23, K=8, dnum=3, "
                                "shared relin key, inputs resident in HBM",
-                   "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B, "q_bits": 50},
+                   "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B, "q_bits": 50,
+                   "schedule": schedule},
         "roofline": {"bound": "hbm",
                      "kernel": "batched forward NTT, 4 polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

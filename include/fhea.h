@@ -203,6 +203,16 @@ int fhea_graph_end(fhea_ctx *ctx, uint32_t *graph_id);
 int fhea_graph_launch(fhea_ctx *ctx, uint32_t graph_id);
 int fhea_graph_destroy(fhea_ctx *ctx, uint32_t graph_id);
 
+/* Schedule tuning (results are identical for every setting; only speed changes).
+ * Used by the autotuner (anns_fhe_amd/autotune.py). */
+#define FHEA_TUNE_NTT_SPLIT_FWD 1 /* stages in the first pass of a two-pass forward NTT
+                                     (both passes must have 6..12 stages), 0 = logN/2 */
+#define FHEA_TUNE_NTT_SPLIT_INV 2 /* the same for the inverse (its second pass)      */
+#define FHEA_TUNE_NTT_WG_PER_CU 3 /* resident NTT workgroups per CU, 1..4, 0 = by occupancy */
+#define FHEA_TUNE_CHUNK 4         /* batch items per key-switch pipeline chunk, 1..64 */
+int fhea_ctx_set_tuning(fhea_ctx *ctx, int knob, uint32_t value);
+int fhea_ctx_get_tuning(const fhea_ctx *ctx, int knob, uint32_t *value);
+
 /* Sentinel for an absent optional offset. */
 #define FHEA_NONE (~(uint64_t)0)
 

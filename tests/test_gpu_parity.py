@@ -498,3 +498,41 @@ def test_hoisted_rotations_parity(orc, q_bits):
                                         [w["evk"] for w in ws], L, [w["d0"] for w in ws], [w["d1"] for w in ws],
                                         L) == 0
     assert np.array_equal(got, pm[8 * W:])
+
+
+def test_ntt_splits_and_tuning_knobs(orc):
+    """Every NTT split the tuner may choose (N=2^16: first pass of 6..10 stages) gives
+    the oracle's transform word for word; invalid knob values are refused; chunk
+    size does not change a batched result."""
+    from anns_fhe_amd import _lib
+    from anns_fhe_amd._lib import FheError
+    from anns_fhe_amd import autotune as AT
+    N, L = 65536, 3
+    ctx = get_ctx(N, L, 1, 1)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(77)
+    a = rand_poly(rng, ctx.q, N)
+    W = L * N
+    ctx.write_poly(0, a)
+    fwd, inv = o.ntt_poly(a), o.intt_poly(a)
+    for k in AT.ntt_splits(16):
+        say("split", k)
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_FWD, k)
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, k)
+        for w in (0, 2):
+            ctx.set_tuning(_lib.TUNE_NTT_WG_PER_CU, w)
+            ctx.ntt_batch(0, W, 1, L, inverse=False)
+            assert np.array_equal(ctx.read_poly(W, W).reshape(L, N), fwd), (k, w)
+            ctx.ntt_batch(0, 2 * W, 1, L, inverse=True)
+            assert np.array_equal(ctx.read_poly(2 * W, W).reshape(L, N), inv), (k, w)
+    for bad in ((_lib.TUNE_NTT_SPLIT_FWD, 5), (_lib.TUNE_NTT_SPLIT_INV, 11), (_lib.TUNE_NTT_WG_PER_CU, 5),
+                (_lib.TUNE_CHUNK, 0), (99, 1)):
+        with pytest.raises(FheError):
+            ctx.set_tuning(*bad)
+    for k in (_lib.TUNE_NTT_SPLIT_FWD, _lib.TUNE_NTT_SPLIT_INV, _lib.TUNE_NTT_WG_PER_CU):
+        ctx.set_tuning(k, 0)
+    # the tuner itself on a small batch: it checks every candidate against the default
+    t = AT.tune_ntt(ctx, 3 * W, 1, L, reps=1)
+    assert any(key.startswith("ntt_split_fwd") for key in t)
+    for k in (_lib.TUNE_NTT_SPLIT_FWD, _lib.TUNE_NTT_SPLIT_INV, _lib.TUNE_NTT_WG_PER_CU):
+        ctx.set_tuning(k, 0)

@@ -114,7 +114,7 @@ static u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
 
 struct LevelOffsets {
     size_t modup_desc, modup_first, modup_ntt_prime, modup_slots, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
-        rs_last, pinvf, rs_qlinvf, rs_qlqlf;
+        rs_last, pinvf, rs_qlinvf, rs_qlqlf, rs_pqf;
 };
 
 // (value, value/modulus) pairs for the FP64 kernels, from (value, shoup) pairs
@@ -311,7 +311,7 @@ static int build_device_tables(fhea_ctx *c) {
         // Rescale l -> l-1 (rescale.cpp:3-72; tables test_vector_generator.cpp:304-325)
         {
             u32 n = l >= 2 ? l - 1 : 0;
-            std::vector<u64> qlinv(2 * n + 2), qlql(2 * n + 2), sub(n + 1);
+            std::vector<u64> qlinv(2 * n + 2), qlql(2 * n + 2), sub(n + 1), pq(2 * n + 2);
             u64 ql = h.primes[l - 1];
             for (u32 i = 0; i < n; ++i) {
                 u64 qi = h.primes[i];
@@ -322,6 +322,10 @@ static int build_device_tables(fhea_ctx *c) {
                 qlql[2 * i] = t;
                 qlql[2 * i + 1] = shoup_companion(t, qi);
                 sub[i] = mulm(ql % qi, t, qi);
+                // (P q_l)^-1 mod q_i for the ModDown-and-rescale transform
+                u64 pr = 1 % qi;
+                for (u32 k = 0; k < K; ++k) pr = mulm(pr, h.primes[L + k] % qi, qi);
+                pq[2 * i] = mulm(inv_mod(pr, qi), inv, qi);
             }
             std::vector<u32> last = {l - 1, l - 1};
             if (use_f64) {
@@ -329,6 +333,7 @@ static int build_device_tables(fhea_ctx *c) {
                 for (u32 i = 0; i < n; ++i) qm[i] = h.primes[i];
                 lo[l].rs_qlinvf = b.addv(f64_pairs(qlinv, qm));
                 lo[l].rs_qlqlf = b.addv(f64_pairs(qlql, qm));
+                lo[l].rs_pqf = b.addv(f64_pairs(pq, qm));
             }
             lo[l].rs_qlinv = b.addv(qlinv);
             lo[l].rs_qlql = b.addv(qlql);
@@ -389,6 +394,7 @@ static int build_device_tables(fhea_ctx *c) {
             lt.pinvf = (const double *)P(lo[l].pinvf);
             lt.rs_qlinvf = (const double *)P(lo[l].rs_qlinvf);
             lt.rs_qlqlf = (const double *)P(lo[l].rs_qlqlf);
+            lt.rs_pqf = (const double *)P(lo[l].rs_pqf);
         }
     }
     return FHEA_OK;
@@ -407,12 +413,23 @@ static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
                       const std::vector<CombineArgs> &comb);
 
+// The rescale that follows a key switch in eval_mult_relin_rescale: the switch's
+// add operand d_z = (d_base + z*dz, + ds) is also its output (l limbs, limb l-1
+// of both polynomials is overwritten), outs[2z+s] receive the rescaled l-1 limbs.
+struct RescaleAfter {
+    u64 *d_base;
+    u64 dz, ds;
+    const std::vector<u64 *> *outs;
+};
+static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra);
+
 // Key switch of nb inputs c_z = c_base + z*c_stride (l limbs, eval domain):
 // out0_z = add0_z + ks0, out1_z = add1_z + ks1 (key_switch.cpp:167-439 followed by
 // the composite's poly_add).  keys[z] = device pointer of item z's key.
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
-                   u64 *ws) {
+                   u64 *ws, const RescaleAfter *ra = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -445,6 +462,8 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     for (u32 z = 1; z < nb; ++z) shared &= keys[z] == keys[0];
     launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
                     c->h.dnum, shared, s);
+    // the ModUp coefficients are dead now: their region holds the rescale's lift
+    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra);
     return ks_moddown(c, l, nb, acc, pb, tt, comb);
 }
 
@@ -513,6 +532,61 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
     return FHEA_OK;
 }
 
+// ModDown + add (comb[z] = {d0, d1, d0, d1}) + rescale by q_{l-1}
+// (fhe_host_driver.cpp:272-306: key_switch.cpp:385-427, poly_add, rescale.cpp:3-72).
+// v: 2*nb*N words.  With the FP64 transforms only limb l-1 of the ModDown result
+// is formed (the rescale lifts it); the other limbs go straight to the output as
+//   out_i = acc_i*(P q_l)^-1 + d_i*q_l^-1 + NTT(c*T_i - conv_i*(P q_l)^-1)  (mod q_i),
+// which is ((acc_i - NTT(conv_i))*P^-1 + d_i)*q_l^-1 + NTT(c*T_i) rearranged by the
+// linearity of the NTT mod q_i: the same words, one transform of l-1 limbs fewer.
+static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const LevelTables &lt = c->lv[l];
+    hipStream_t s = c->stream;
+    if (!t.use_f64 || (t.f64_off & 17) || l < 2) {
+        int st = ks_moddown(c, l, nb, acc, pb, tt, comb);
+        if (st) return st;
+        return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
+    }
+    launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
+    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
+    if (!cb_d) return FHEA_ERR_DEVICE;
+    // limb l-1 of the ModDown result, (acc - NTT(conv)) * P^-1 + d, in place in d
+    NttFuse f1;
+    f1.epi = FUSE_MODDOWN_OUT;
+    f1.cst_out = lt.pinvf;
+    f1.cb = cb_d;
+    f1.acc = acc;
+    f1.acc_stride = (l + K) * N;
+    f1.slots = t.ident + (l - 1);
+    f1.nslots = 1;
+    launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, f1, s);
+    // its coefficients
+    launch_ntt(t, true, ra.d_base + (l - 1) * N, v, ra.dz, 2 * N, nb, 2, c->rs_last_map[l], s, ra.ds, N);
+    std::vector<RescaleOut> ro(2 * nb);
+    for (u32 z = 0; z < nb; ++z)
+        for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {ra.d_base + z * ra.dz + k * ra.ds, (*ra.outs)[2 * z + k]};
+    auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
+    if (!ro_d) return FHEA_ERR_DEVICE;
+    NttFuse f2;
+    f2.pro = FUSE_MDRS_IN;
+    f2.v = v;
+    f2.v_stride = N;
+    f2.ql = c->h.primes[l - 1];
+    f2.cst_in = lt.rs_qlqlf;
+    f2.cst_pq = lt.rs_pqf;
+    f2.epi = FUSE_MDRS_OUT;
+    f2.cst_out = lt.rs_qlinvf;
+    f2.ro = ro_d;
+    f2.acc = acc;
+    f2.acc_stride = (l + K) * N;
+    launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l - 1, t.ident, f2, s);
+    return FHEA_OK;
+}
+
 struct MultItem {
     const u64 *a0, *a1, *b0, *b1;
     u64 *d0, *d1;
@@ -545,9 +619,8 @@ static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
         auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
         if (!ta_d) return FHEA_ERR_DEVICE;
         launch_tensor(c->dt, ta_d, nb, l, c->stream);
-        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest);
-        if (st) return st;
-        st = rescale_core(c, l, nb, D, 3 * l * N, l * N, outs, rest);
+        const RescaleAfter ra{D, 3 * l * N, l * N, &outs};
+        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest, &ra);
         if (st) return st;
     }
     return check_hip(hipGetLastError());

@@ -52,6 +52,7 @@ struct LevelTables {
     const u64 *rs_qlql = nullptr;          // [2*(l-1)] (-q_l^-1 mod q_i, shoup)
     // FP64 forms (value, value / q_i)
     const double *pinvf = nullptr, *rs_qlinvf = nullptr, *rs_qlqlf = nullptr;
+    const double *rs_pqf = nullptr;        // [2*(l-1)] ((P q_l)^-1 mod q_i, / q_i)
 };
 
 struct HostTables {

@@ -31,19 +31,27 @@ struct DevTables {
 
 struct RescaleOut;
 struct CombineArgs;
-enum : int { FUSE_NONE = 0, FUSE_RESCALE_IN = 1, FUSE_RESCALE_OUT = 2, FUSE_MODDOWN_OUT = 3 };
+enum : int {
+    FUSE_NONE = 0,
+    FUSE_RESCALE_IN = 1,
+    FUSE_RESCALE_OUT = 2,
+    FUSE_MODDOWN_OUT = 3,
+    FUSE_MDRS_IN = 4,   // ModDown + rescale in one transform: prologue
+    FUSE_MDRS_OUT = 5,  // ... and epilogue
+};
 // Fused prologue (first forward pass) / epilogue (last forward pass) of the
 // FP64 transform; poly = batch index of the NTT launch.
 struct NttFuse {
     int pro = FUSE_NONE, epi = FUSE_NONE;
-    const u64 *v = nullptr;            // RESCALE_IN: INTT'd last limb, poly at v + poly*v_stride
+    const u64 *v = nullptr;            // RESCALE_IN, MDRS_IN: INTT'd last limb, poly at v + poly*v_stride
     u64 v_stride = 0;
-    u64 ql = 0;                        // RESCALE_IN: the dropped prime
-    const double *cst_in = nullptr;    // RESCALE_IN: (T_i, T_i/q_i) per limb slot
-    const double *cst_out = nullptr;   // RESCALE_OUT: q_l^-1 / MODDOWN_OUT: P^-1, per slot
-    const RescaleOut *ro = nullptr;    // RESCALE_OUT: per poly (a, out)
+    u64 ql = 0;                        // RESCALE_IN, MDRS_IN: the dropped prime
+    const double *cst_in = nullptr;    // RESCALE_IN, MDRS_IN: (T_i, T_i/q_i) per limb slot
+    const double *cst_out = nullptr;   // RESCALE_OUT, MDRS_OUT: q_l^-1 / MODDOWN_OUT: P^-1, per slot
+    const double *cst_pq = nullptr;    // MDRS_IN, MDRS_OUT: (P q_l)^-1 per slot
+    const RescaleOut *ro = nullptr;    // RESCALE_OUT, MDRS_OUT: per poly (a, out)
     const CombineArgs *cb = nullptr;   // MODDOWN_OUT: per item (poly>>1), half = poly&1
-    const u64 *acc = nullptr;          // MODDOWN_OUT: x operand, poly at acc + poly*acc_stride
+    const u64 *acc = nullptr;          // MODDOWN_OUT, MDRS_OUT: poly at acc + poly*acc_stride
     u64 acc_stride = 0;
     // optional list of the limb slots to transform (instead of all `limbs` slots)
     const u32 *slots = nullptr;

@@ -36,6 +36,11 @@
 //   FUSE_RESCALE_OUT  forward ROWS epilogue: out = a*q_l^-1 + x    (rescale.cpp:62-69)
 //   FUSE_MODDOWN_OUT  forward ROWS epilogue: out = (acc - x)*P^-1 [+ add]
 //                     (key_switch.cpp:253-273 with the composite's poly_add)
+//   FUSE_MDRS_IN      forward COLS prologue: x = centred(v[j])*T_i - conv[j]*(P q_l)^-1
+//   FUSE_MDRS_OUT     forward ROWS epilogue: out = acc*(P q_l)^-1 + d*q_l^-1 + x
+//                     (the ModDown with its add and the rescale as one transform;
+//                     every step is exact mod q_i, so the words are those of the
+//                     two steps run one after the other)
 #include "fhea_f64.hpp"
 #include "fhea_kernels.hpp"
 
@@ -137,11 +142,19 @@ __device__ __forceinline__ u64 load_raw(const char *sbase, u32 j, const NttFuse 
     return ((const u64 *)sbase)[j];
 }
 
-// raw word -> signed residue (double)
-__device__ __forceinline__ double raw_to_f64(u64 w, int in_canon, const NttFuse &fz, u32 slot, double q) {
+// raw word j of the pass -> signed residue (double)
+__device__ __forceinline__ double raw_to_f64(u64 w, int in_canon, const NttFuse &fz, u32 slot, double q,
+                                             u32 j, u32 poly) {
     if (fz.pro == FUSE_RESCALE_IN) {
         const double c = (w > (fz.ql >> 1)) ? u64_to_f64(w) - (double)fz.ql : u64_to_f64(w);
         return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q);
+    }
+    if (fz.pro == FUSE_MDRS_IN) {
+        // c*T_i - conv_i*(P q_l)^-1, c = centred lift of the last limb; |result| <= q + 1
+        const u64 vw = fz.v[(size_t)poly * fz.v_stride + j];
+        const double c = (vw > (fz.ql >> 1)) ? u64_to_f64(vw) - (double)fz.ql : u64_to_f64(vw);
+        return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q) -
+               mulmod_f64(u64_to_f64(w), fz.cst_pq[2 * slot], fz.cst_pq[2 * slot + 1], q);
     }
     if (in_canon) return u64_to_f64(w);
     return __longlong_as_double((long long)w);
@@ -157,6 +170,15 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
         const RescaleOut A = fz.ro[poly];
         const double a = mulmod_f64(u64_to_f64(A.a[off]), fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
         A.out[off] = f64_to_canon(reduce_f64(a + v, q, qinv), q);
+        return;
+    }
+    if (fz.epi == FUSE_MDRS_OUT) {
+        // acc*(P q_l)^-1 + d*q_l^-1 + x: |sum| <= 1.5q + 3
+        const RescaleOut A = fz.ro[poly];
+        const double x = mulmod_f64(u64_to_f64(fz.acc[(size_t)poly * fz.acc_stride + off]), fz.cst_pq[2 * slot],
+                                    fz.cst_pq[2 * slot + 1], q);
+        const double d = mulmod_f64(u64_to_f64(A.a[off]), fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
+        A.out[off] = f64_to_canon(reduce_f64(x + d + v, q, qinv), q);
         return;
     }
     if (fz.epi == FUSE_MODDOWN_OUT) {
@@ -274,14 +296,17 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
         // ---- round 0 input (this tile's words were loaded one iteration ago)
         if (DIRECT_IN) {
             if (active) {
+                const u32 jb0 = g.gidx(eb0, LOGR);
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = raw_to_f64(raw[u], in_canon, fz, cur.slot, g.q);
+                for (int u = 0; u < 16; ++u)
+                    x[u] = raw_to_f64(raw[u], in_canon, fz, cur.slot, g.q, jb0 + ((u32)u << (R0::P + g.logS)),
+                                      cur.poly);
             }
         } else {
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const u32 e = tid + (u32)k * F_TPB;
-                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q);
+                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.gidx(e, LOGR), cur.poly);
             }
         }
         // prefetch the next tile

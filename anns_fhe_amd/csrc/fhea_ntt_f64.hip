@@ -23,11 +23,12 @@
 // at bit >= 3); otherwise they are staged through LDS.
 //
 // Value bounds (q < 2^50, see fhea_f64.hpp):
-//   forward: a round starts at |x| <= 1.5q (canonical input, a centred value, or
-//            the RESCALE_IN product of |c| <= 2q, which is <= q/2 + |c|/4);
-//            stage s adds a product of magnitude <= q/2 + B_s/4, so the four
-//            stages reach at most 6.6q < 8q; then every value is centred
-//            (|x| <= q/2 + 2);
+//   forward: a round starts at |x| <= q + 1 (canonical input, a centred value,
+//            the RESCALE_IN product <= q/2 + |c|/4 of |c| <= q/2, or the MDRS_IN
+//            difference of two products); stage s adds a product of magnitude
+//            <= q/2 + 0.375 B_s (twiddles formed as products, see stage_twiddles),
+//            so the four stages reach at most 7.01q < 8q; then every value is
+//            centred (|x| <= q/2 + 2);
 //   inverse: the sum branch is centred every stage, the difference (<= 3q) is
 //            multiplied back to <= 1.25q.
 //
@@ -92,9 +93,48 @@ struct TileGeom {
     }
 };
 
+// Twiddles of a stage.  The thread needs the 2^(3-wb) pairs tw[wbase | t]; wbase
+// has those low bits clear.  Since tw[i] = psi^bitrev(i) and bit reversal adds over
+// disjoint bit sets, tw[wbase | t] = tw[wbase] * tw[t] (mod q).  FACT: load tw[wbase]
+// only and form the others as products with the constants tw[1..7] (same for every
+// thread of a prime): one load per stage instead of up to eight.  A product is a
+// signed residue, |w| < 0.75q, with w/q rounded twice (w * fl(1/q)), so a butterfly
+// product obeys |w v mod q| <= q/2 + 0.375|v|; a forward round from |x| <= q + 1 then
+// grows to 1.875q, 3.08q, 4.73q, 7.01q < 8q.
+template <bool FACT>
+__device__ __forceinline__ void stage_twiddles(double (&w)[8], double (&wq)[8], const TileGeom &g, u32 wbase,
+                                               int npairs) {
+    const double2 *tw2 = reinterpret_cast<const double2 *>(g.tw);
+    if constexpr (FACT) {
+        const double2 p0 = tw2[wbase];
+        w[0] = p0.x;
+        wq[0] = p0.y;
+#pragma unroll
+        for (int t = 1; t < 8; ++t) {
+            if (t < npairs) {
+                const double2 c = tw2[t];
+                w[t] = mulmod_f64(p0.x, c.x, c.y, g.q);
+                wq[t] = w[t] * g.qinv;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (t < npairs) {
+                const double2 pr = tw2[wbase | (u32)t];
+                w[t] = pr.x;
+                wq[t] = pr.y;
+            }
+        }
+    }
+}
+
 template <bool INV, int LOGR, int RD>
 __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, u32 j0) {
     using R = Round<INV, LOGR, RD>;
+    // forward rounds form their twiddles as products (measured faster); inverse
+    // rounds load them
+    constexpr bool FACT = !INV;
     // stage st of the round uses 2^(3-wb) distinct (w, w/q) pairs, wb = its window bit
 #pragma unroll
     for (int st = 0; st < R::NS; ++st) {
@@ -107,12 +147,7 @@ __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, 
         const u32 k = (u32)b + g.logS + 1;
         const u32 wbase = ((1u << g.logN) >> k) | (j0 >> k);
         double w[8], wq[8];
-#pragma unroll
-        for (int t = 0; t < (8 >> wb); ++t) {
-            const double2 pr = reinterpret_cast<const double2 *>(g.tw)[wbase | (u32)t];
-            w[t] = pr.x;
-            wq[t] = pr.y;
-        }
+        stage_twiddles<FACT>(w, wq, g, wbase, 8 >> wb);
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             if (u & (1 << wb)) continue;
@@ -144,17 +179,18 @@ __device__ __forceinline__ u64 load_raw(const char *sbase, u32 j, const NttFuse 
 
 // raw word j of the pass -> signed residue (double)
 __device__ __forceinline__ double raw_to_f64(u64 w, int in_canon, const NttFuse &fz, u32 slot, double q,
-                                             u32 j, u32 poly) {
+                                             double qinv, u32 j, u32 poly) {
     if (fz.pro == FUSE_RESCALE_IN) {
         const double c = (w > (fz.ql >> 1)) ? u64_to_f64(w) - (double)fz.ql : u64_to_f64(w);
         return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q);
     }
     if (fz.pro == FUSE_MDRS_IN) {
-        // c*T_i - conv_i*(P q_l)^-1, c = centred lift of the last limb; |result| <= q + 1
+        // c*T_i - conv_i*(P q_l)^-1, c = centred lift of the last limb, centred
         const u64 vw = fz.v[(size_t)poly * fz.v_stride + j];
         const double c = (vw > (fz.ql >> 1)) ? u64_to_f64(vw) - (double)fz.ql : u64_to_f64(vw);
-        return mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q) -
-               mulmod_f64(u64_to_f64(w), fz.cst_pq[2 * slot], fz.cst_pq[2 * slot + 1], q);
+        const double d = mulmod_f64(c, fz.cst_in[2 * slot], fz.cst_in[2 * slot + 1], q) -
+                         mulmod_f64(u64_to_f64(w), fz.cst_pq[2 * slot], fz.cst_pq[2 * slot + 1], q);
+        return reduce_f64(d, q, qinv);
     }
     if (in_canon) return u64_to_f64(w);
     return __longlong_as_double((long long)w);
@@ -173,7 +209,7 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
         return;
     }
     if (fz.epi == FUSE_MDRS_OUT) {
-        // acc*(P q_l)^-1 + d*q_l^-1 + x: |sum| <= 1.5q + 3
+        // acc*(P q_l)^-1 + d*q_l^-1 + x: |sum| <= 2q + 2
         const RescaleOut A = fz.ro[poly];
         const double x = mulmod_f64(u64_to_f64(fz.acc[(size_t)poly * fz.acc_stride + off]), fz.cst_pq[2 * slot],
                                     fz.cst_pq[2 * slot + 1], q);
@@ -299,14 +335,14 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                 const u32 jb0 = g.gidx(eb0, LOGR);
 #pragma unroll
                 for (int u = 0; u < 16; ++u)
-                    x[u] = raw_to_f64(raw[u], in_canon, fz, cur.slot, g.q, jb0 + ((u32)u << (R0::P + g.logS)),
+                    x[u] = raw_to_f64(raw[u], in_canon, fz, cur.slot, g.q, g.qinv, jb0 + ((u32)u << (R0::P + g.logS)),
                                       cur.poly);
             }
         } else {
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const u32 e = tid + (u32)k * F_TPB;
-                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.gidx(e, LOGR), cur.poly);
+                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.qinv, g.gidx(e, LOGR), cur.poly);
             }
         }
         // prefetch the next tile

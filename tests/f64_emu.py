@@ -103,6 +103,61 @@ def ntt_forward_f64(a, tw, q, round_stages=4, tr=NULL):
     return [f64_to_canon(v, qd) for v in x]
 
 
+def kernel_rounds(logn, k1, log_tile=12):
+    """The forward FP64 kernel's rounds: a list of rounds, each a list of
+    (global bit b, window low bit P_global) in stage order.  logn <= log_tile: one
+    pass over all bits; otherwise a first pass over the top k1 bits (columns) and a
+    second over the low logn - k1 bits (rows).  Round geometry as in the kernel:
+    4 stages per round, window [P, P+4) with P = top - 3 (or 0)."""
+    passes = [(logn, 0)] if logn <= log_tile else [(k1, logn - k1), (logn - k1, 0)]
+    rounds = []
+    for logr, shift in passes:
+        nr = (logr + 3) // 4
+        for rd in range(nr):
+            ns = 4 if rd + 1 < nr else logr - 4 * rd
+            top = logr - 1 - 4 * rd
+            p = top - 3 if top >= 3 else 0
+            rounds.append([(top - st + shift, p + shift) for st in range(ns)])
+    return rounds
+
+
+def ntt_forward_f64_kernel(a, tw, q, k1=None, tr=NULL):
+    """Forward NTT exactly as the FP64 kernel computes it: the kernel's passes and
+    radix-16 rounds (kernel_rounds), a reduction at every round end, and each
+    stage's twiddles tw[base | t] (t < 2^(P+3-b), base with those bits clear)
+    formed as mulmod(tw[base], tw[t]) with w/q taken as w * fl(1/q)."""
+    n = len(a)
+    logn = n.bit_length() - 1
+    k1 = logn // 2 if k1 is None else k1
+    qd, qinv = float(q), 1.0 / float(q)
+    pairs = [(float(w), float(w) / qd) for w in tw]
+    x = [float(v) for v in a]
+    for rnd in kernel_rounds(logn, k1):
+        for b, p in rnd:
+            tb = p + 3 - b          # bits of the group index taken from the window
+            m = n >> (b + 1)
+            for j in range(n):
+                if j >> b & 1:
+                    continue
+                i = j >> (b + 1)
+                t = i & ((1 << tb) - 1)
+                w0, wq0 = pairs[m + i - t]
+                if t:
+                    c, cq = pairs[t]
+                    w = mulmod_f64(w0, c, cq, qd, tr)
+                    tr.see("twiddle", w, qd)
+                    wq = w * qinv
+                else:
+                    w, wq = w0, wq0
+                rr = mulmod_f64(x[j + (1 << b)], w, wq, qd, tr)
+                u = x[j]
+                x[j], x[j + (1 << b)] = u + rr, u - rr
+                tr.see("butterfly_out", x[j], qd)
+                tr.see("butterfly_out", x[j + (1 << b)], qd)
+        x = [reduce_f64(v, qd, qinv, tr) for v in x]
+    return [f64_to_canon(v, qd) for v in x]
+
+
 def ntt_inverse_f64(a, itw, n_inv, q, tr=NULL):
     """Inverse as the FP64 kernel computes it: Gentleman-Sande stages with the
     sum reduced every stage and the difference multiplied by W = itw[m + i],

@@ -103,6 +103,27 @@ def test_ntt_f64_matches_oracle(n, round_stages):
     assert tr.max_ratio["reduce_in"] < 8.0
 
 
+@pytest.mark.parametrize("n,k1", [(256, 8), (1024, 10), (8192, 6), (8192, 7)])
+def test_ntt_f64_kernel_rounds_factorized_twiddles(n, k1):
+    """The forward kernel's own pass/round structure with its twiddles formed as
+    products tw[base] * tw[t] (one load per stage): equal to the oracle's NTT word
+    for word, every intermediate below 8q, every formed twiddle below 0.75q."""
+    rng = random.Random(n + k1)
+    q = _primes(50, n, 2)
+    o = O.OracleCtx(n, q[:1], q[1:], 1)
+    tr = E.Trace()
+    for idx in range(2):
+        p = o.primes[idx]
+        inputs = _adversarial_inputs(rng, n, p) if n <= 1024 else [
+            [p - 1] * n, [rng.randrange(p) for _ in range(n)]]
+        for a in inputs:
+            got = E.ntt_forward_f64_kernel(a, o.tw[idx], p, k1, tr)
+            assert got == [int(v) for v in o.ntt(np.array(a, np.uint64), idx)]
+    assert tr.max_ratio["butterfly_out"] < 8.0
+    assert tr.max_ratio["reduce_in"] < 8.0
+    assert tr.max_ratio["twiddle"] < 0.75
+
+
 def test_ntt_f64_largest_prime_below_2_50():
     """N = 1024 at the largest NTT prime below 2^50 (the tightest case for 2^53)."""
     n = 1024

@@ -100,7 +100,8 @@ struct TileGeom {
 // thread of a prime): one load per stage instead of up to eight.  A product is a
 // signed residue, |w| < 0.75q, with w/q rounded twice (w * fl(1/q)), so a butterfly
 // product obeys |w v mod q| <= q/2 + 0.375|v|; a forward round from |x| <= q + 1 then
-// grows to 1.875q, 3.08q, 4.73q, 7.01q < 8q.
+// grows to 1.875q, 3.08q, 4.73q, 7.01q < 8q; in the inverse the products of the
+// differences stay below 2q (B <= q/2 + 0.375 * 2B) and the differences below 4q.
 template <bool FACT>
 __device__ __forceinline__ void stage_twiddles(double (&w)[8], double (&wq)[8], const TileGeom &g, u32 wbase,
                                                int npairs) {
@@ -129,12 +130,13 @@ __device__ __forceinline__ void stage_twiddles(double (&w)[8], double (&wq)[8], 
     }
 }
 
-template <bool INV, int LOGR, int RD>
+template <bool INV, int LOGR, int RD, bool COLS>
 __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, u32 j0) {
     using R = Round<INV, LOGR, RD>;
-    // forward rounds form their twiddles as products (measured faster); inverse
-    // rounds load them
-    constexpr bool FACT = !INV;
+    // twiddles formed as products in every forward round and in the first inverse
+    // round along rows (the rounds where each thread's pairs are its own); the
+    // other inverse rounds load them (measured faster that way)
+    constexpr bool FACT = !INV || (!COLS && RD == 0);
     // stage st of the round uses 2^(3-wb) distinct (w, w/q) pairs, wb = its window bit
 #pragma unroll
     for (int st = 0; st < R::NS; ++st) {
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                 for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << P0))];
             }
         }
-        if (active) round_arith<INV, LOGR, 0>(x, g, g.gidx(eb0, LOGR));
+        if (active) round_arith<INV, LOGR, 0, COLS>(x, g, g.gidx(eb0, LOGR));
         // ---- later rounds (exchange through LDS)
         if constexpr (NR > 1) {
             {
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                 if (active) {
 #pragma unroll
                     for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1 | ((u32)u << P1))];
-                    round_arith<INV, LOGR, 1>(x, g, g.gidx(eb1, LOGR));
+                    round_arith<INV, LOGR, 1, COLS>(x, g, g.gidx(eb1, LOGR));
                 }
             }
             if constexpr (NR > 2) {
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                 if (active) {
 #pragma unroll
                     for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2 | ((u32)u << P2))];
-                    round_arith<INV, LOGR, 2>(x, g, g.gidx(eb2, LOGR));
+                    round_arith<INV, LOGR, 2, COLS>(x, g, g.gidx(eb2, LOGR));
                 }
             }
         }

@@ -388,29 +388,30 @@ def main():
 
     B = args.batch
     W, W1 = L * N, (L - 1) * N
-    per_item = 4 * W + 2 * W1
+    # layout: the 4B input polynomials (a0, a1, b0, b1 of each item) contiguous, then
+    # the 2B outputs, then scratch for the op-by-op path
+    out_base = 4 * B * W
+    plain_base = out_base + 2 * B * W1
     key_words = 2 * DNUM * (L + K) * N
     log(f"context N={N} L={L} K={K} dnum={DNUM} batch={B}")
     plain_words = 8 * W  # scratch of the op-by-op path
-    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=B * per_item + plain_words + 8,
+    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=plain_base + plain_words + 8,
                      key_words=key_words + 8)
     drv = FHEHostDriver(ctx)
     s = ctx.keygen_secret(42)
     ctx.keygen_relin_key(s, L, 43, 0)
     drv.set_relin_key(0, L)
     rng = np.random.default_rng(42)
-    for z in range(B):
-        for k in range(4):
-            ctx.write_poly(z * per_item + k * W, rand_limbs(rng, ctx.q, N))
-    works = [dict(kind=1, limbs=L, a0=z * per_item, a1=z * per_item + W, b0=z * per_item + 2 * W,
-                  b1=z * per_item + 3 * W, d0=z * per_item + 4 * W, d1=z * per_item + 4 * W + W1)
-             for z in range(B)]
+    for k in range(4 * B):
+        ctx.write_poly(k * W, rand_limbs(rng, ctx.q, N))
+    works = [dict(kind=1, limbs=L, a0=4 * z * W, a1=(4 * z + 1) * W, b0=(4 * z + 2) * W, b1=(4 * z + 3) * W,
+                  d0=out_base + 2 * z * W1, d1=out_base + (2 * z + 1) * W1) for z in range(B)]
     schedule = None
     if not args.no_tune:
         from anns_fhe_amd import autotune
         t0 = time.perf_counter()
-        autotune.tune_ntt(ctx, B * per_item, 4, L, reps=3, log=log)
-        autotune.tune_chunk(ctx, works, candidates=(4, 8, 16), reps=2, log=log)
+        autotune.tune(ctx, lambda: ctx.submit(works), check=lambda: ctx.read_poly(out_base, 2 * W1),
+                      reps=3, log=log)
         schedule = autotune.current(ctx)
         log(f"autotune {schedule} in {time.perf_counter() - t0:.1f} s")
     log("warmup")
@@ -428,24 +429,23 @@ def main():
     value = ops / (ms / 1e3)
     log(f"{ops} HMult in {ms:.2f} ms (wall {wall * 1e3:.2f} ms) -> {value:.1f} ops/s")
 
-    # dominant kernel: batched NTT over the 4B input polynomials (in place, then back)
-    ctx.ntt_batch(0, 0, 1, L)  # make sure the path is warm
+    # dominant kernel: one batched forward NTT over the 4B input polynomials (the
+    # pipeline's transforms are launched over whole batches the same way)
+    ctx.ntt_batch(0, 0, 4 * B, L)
+    ctx.ntt_batch(0, 0, 4 * B, L, inverse=True)
     ctx.sync()
     reps = args.ntt_reps
     ntt_ms = 0.0
     for r in range(reps):
         ctx.timer_start()
-        for z in range(B):
-            ctx.ntt_batch(z * per_item, z * per_item, 4, L, inverse=False)
+        ctx.ntt_batch(0, 0, 4 * B, L, inverse=False)
         ntt_ms += ctx.timer_stop()
-        for z in range(B):
-            ctx.ntt_batch(z * per_item, z * per_item, 4, L, inverse=True)
+        ctx.ntt_batch(0, 0, 4 * B, L, inverse=True)
         ctx.sync()
-    launches = reps * B
-    ntt_launch_ms = ntt_ms / launches
-    alg_bytes = 4 * L * 2 * N * 8  # 4 polys x L limbs, each limb read once and written once
+    ntt_launch_ms = ntt_ms / reps
+    alg_bytes = 4 * B * L * 2 * N * 8  # 4B polys x L limbs, each limb read once and written once
     achieved = alg_bytes / (ntt_launch_ms / 1e3) / 1e9
-    log(f"NTT launch (4 polys x {L} limbs): {ntt_launch_ms:.3f} ms -> {achieved:.0f} GB/s algorithmic")
+    log(f"NTT launch ({4 * B} polys x {L} limbs): {ntt_launch_ms:.3f} ms -> {achieved:.0f} GB/s algorithmic")
 
     result = {
         "metric": "HMult+relin+rescale ops/s at N=2^16,L=24",
@@ -465,7 +465,7 @@ def main():
                    "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B, "q_bits": 50,
                    "schedule": schedule},
         "roofline": {"bound": "hbm",
-                     "kernel": "batched forward NTT, 4 polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
+                     "kernel": f"batched forward NTT, {4 * B} polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},
@@ -473,14 +473,14 @@ def main():
     # calibration copy of a known size (64 MiB, D2D blit) for rocprof FETCH_SIZE / WRITE_SIZE
     # scaling; it runs after every measurement and overwrites the first input only
     calib_words = 8 << 20
-    if B * per_item >= 2 * calib_words:
+    if 4 * B * W >= 2 * calib_words:
         lib = ctx.lib
         assert lib.fhea_copy_poly(ctx.h, calib_words, 0, calib_words) == 0
         ctx.sync()
         result["calibration_copy_bytes"] = calib_words * 8
     if args.plain_steps > 0:
         log("plain per-op path")
-        result["plain_path"] = plain_hmult(ctx, drv, B * per_item, args.plain_steps)
+        result["plain_path"] = plain_hmult(ctx, drv, plain_base, args.plain_steps)
     if args.cpu_sample > 0:
         log("cpu baseline")
         key = ctx.read_key(0, key_words)

@@ -158,6 +158,48 @@ def ntt_forward_f64_kernel(a, tw, q, k1=None, tr=NULL):
     return [f64_to_canon(v, qd) for v in x]
 
 
+def ntt_inverse_f64_kernel(a, itw, n_inv, q, k1=None, log_tile=12, tr=NULL):
+    """Inverse NTT as the FP64 kernel computes it: Gentleman-Sande stages (sum
+    reduced, difference times W = itw[m + i]), bits in ascending order; in the
+    first round along rows (bits 0..3, window [0, 4)) each stage's twiddles
+    itw[base | t] (t < 2^(3-b)) are formed as mulmod(itw[base], itw[t]) with w/q
+    taken as w * fl(1/q); then the N^-1 scaling and the final reduction."""
+    n = len(a)
+    logn = n.bit_length() - 1
+    k1 = logn // 2 if k1 is None else k1
+    rows_bits = logn if logn <= log_tile else logn - k1
+    qd, qinv = float(q), 1.0 / float(q)
+    pairs = [(float(w), float(w) / qd) for w in itw]
+    x = [float(v) for v in a]
+    for b in range(logn):
+        m = n >> (b + 1)
+        fact_bits = 3 - b if (b < 4 and rows_bits >= 4) else 0
+        for j in range(n):
+            if j >> b & 1:
+                continue
+            i = j >> (b + 1)
+            t = i & ((1 << fact_bits) - 1)
+            w0, wq0 = pairs[m + i - t]
+            if t:
+                c, cq = pairs[t]
+                w = mulmod_f64(w0, c, cq, qd, tr)
+                tr.see("twiddle", w, qd)
+                wq = w * qinv
+            else:
+                w, wq = w0, wq0
+            u, v = x[j], x[j + (1 << b)]
+            tr.see("inv_diff", u - v, qd)
+            x[j] = reduce_f64(u + v, qd, qinv, tr)
+            x[j + (1 << b)] = mulmod_f64(u - v, w, wq, qd, tr)
+    ni = float(n_inv)
+    niq = ni / qd
+    out = []
+    for v in x:
+        v = mulmod_f64(v, ni, niq, qd, tr)
+        out.append(f64_to_canon(reduce_f64(v, qd, qinv, tr), qd))
+    return out
+
+
 def ntt_inverse_f64(a, itw, n_inv, q, tr=NULL):
     """Inverse as the FP64 kernel computes it: Gentleman-Sande stages with the
     sum reduced every stage and the difference multiplied by W = itw[m + i],

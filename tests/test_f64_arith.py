@@ -124,6 +124,26 @@ def test_ntt_f64_kernel_rounds_factorized_twiddles(n, k1):
     assert tr.max_ratio["twiddle"] < 0.75
 
 
+@pytest.mark.parametrize("n,k1", [(256, 8), (1024, 10), (8192, 6), (8192, 7)])
+def test_intt_f64_kernel_factorized_first_round(n, k1):
+    """The inverse with the first row round's twiddles formed as products equals
+    the oracle's INTT word for word; differences and products stay far below 8q."""
+    rng = random.Random(3 * n + k1)
+    q = _primes(50, n, 2)
+    o = O.OracleCtx(n, q[:1], q[1:], 1)
+    tr = E.Trace()
+    for idx in range(2):
+        p = o.primes[idx]
+        inputs = _adversarial_inputs(rng, n, p) if n <= 1024 else [
+            [p - 1] * n, [rng.randrange(p) for _ in range(n)]]
+        for a in inputs:
+            got = E.ntt_inverse_f64_kernel(a, o.itw[idx], int(o.n_inv[idx]), p, k1, tr=tr)
+            assert got == [int(v) for v in o.intt(np.array(a, np.uint64), idx)]
+    assert tr.max_ratio["inv_diff"] < 4.0
+    assert tr.max_ratio["mulmod_out"] < 2.0
+    assert tr.max_ratio["twiddle"] < 0.75
+
+
 def test_ntt_f64_largest_prime_below_2_50():
     """N = 1024 at the largest NTT prime below 2^50 (the tightest case for 2^53)."""
     n = 1024

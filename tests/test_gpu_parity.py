@@ -536,3 +536,38 @@ def test_ntt_splits_and_tuning_knobs(orc):
     assert any(key.startswith("ntt_split_fwd") for key in t)
     for k in (_lib.TUNE_NTT_SPLIT_FWD, _lib.TUNE_NTT_SPLIT_INV, _lib.TUNE_NTT_WG_PER_CU):
         ctx.set_tuning(k, 0)
+
+
+def test_workload_tuner_keeps_results(orc):
+    """The workload tuner times a batched eval_mult_relin_rescale under every
+    schedule candidate (NTT splits, workgroups per CU, chunk sizes), checks the
+    output words each time, and the chosen schedule still gives the oracle's words."""
+    from anns_fhe_amd import autotune as AT
+    from anns_fhe_amd.driver import FHEHostDriver
+    N, L = 16384, 4
+    ctx = get_ctx(N, L, 2, 2)
+    drv = FHEHostDriver(ctx)
+    s = ctx.keygen_secret(21)
+    ctx.keygen_relin_key(s, L, 22, 0)
+    drv.set_relin_key(0, L)
+    rng = np.random.default_rng(23)
+    W, n = L * N, 6
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4 * n)])
+    ctx.write_poly(0, data)
+    works = [dict(kind=1, limbs=L, a0=4 * i * W, a1=(4 * i + 1) * W, b0=(4 * i + 2) * W, b1=(4 * i + 3) * W,
+                  d0=(4 * n + 2 * i) * W, d1=(4 * n + 2 * i + 1) * W) for i in range(n)]
+    say("tune")
+    t = AT.tune(ctx, lambda: ctx.submit(works), check=lambda: ctx.read_poly(4 * n * W, 2 * n * W), reps=1)
+    assert len(t) >= 4 + 2 * len(AT.ntt_splits(14))
+    ctx.submit(works)
+    got = ctx.read_poly(4 * n * W, 2 * W)
+    o = get_orc(orc, ctx)
+    pm = np.zeros((4 * n + 2) * W, np.uint64)
+    pm[:4 * n * W] = data
+    km = ctx.read_key(0, ctx.key_size_words(L))
+    w = works[0]
+    assert o.eval_mult_relin_rescale(pm, km, 0, L, w["a0"], w["a1"], w["b0"], w["b1"], 4 * n * W,
+                                     (4 * n + 1) * W, L) == 0
+    assert np.array_equal(got[:(L - 1) * N], pm[4 * n * W:4 * n * W + (L - 1) * N])
+    assert np.array_equal(got[W:W + (L - 1) * N], pm[(4 * n + 1) * W:(4 * n + 1) * W + (L - 1) * N])
+    AT.apply(ctx, {"ntt_split_fwd": 0, "ntt_split_inv": 0, "ntt_wg_per_cu": 0, "chunk": 16})

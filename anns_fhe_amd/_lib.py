@@ -66,6 +66,7 @@ SIGNATURES = {
     "fhea_status_string": (C.c_char_p, [C.c_int]),
     "fhea_write_poly": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_poly": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
+    "fhea_read_polys": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p]),
     "fhea_write_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_copy_poly": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -95,8 +96,11 @@ SIGNATURES = {
     "fhea_graph_launch": (C.c_int, [vp, C.c_uint32]),
     "fhea_graph_destroy": (C.c_int, [vp, C.c_uint32]),
     "fhea_rescale_batch": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "fhea_rescale_scatter": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint32, C.c_uint32]),
     "fhea_eval_linear_sum": (C.c_int, [vp, u64p, u64p, C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64,
                                        C.c_uint64, u64p, u64p, C.c_uint32]),
+    "fhea_eval_linear_sum_batch": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, u64p, u32p, u64p, u64p, u64p, u64p,
+                                             u64p, u64p, C.c_uint32]),
     "fhea_timer_start": (C.c_int, [vp]),
     "fhea_timer_stop": (C.c_int, [vp, C.POINTER(C.c_float)]),
     "fhea_keygen_secret": (C.c_int, [vp, C.c_uint64, i8p]),

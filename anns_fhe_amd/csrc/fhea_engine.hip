@@ -43,31 +43,42 @@ int ensure_ws(fhea_ctx *c, u64 words) {
     return FHEA_OK;
 }
 
-void *stage_args(fhea_ctx *c, const void *host, size_t bytes) {
+// Reserves `bytes` of argument staging: the caller writes them at *h (and may
+// embed *d-based device addresses), then stage_commit copies them to *d on the
+// stream.  While capturing, the graph owns its argument area (pinned host +
+// device, allocated before the capture began and never reused) and the copy is
+// recorded as a graph node.
+static int stage_reserve(fhea_ctx *c, size_t bytes, char **h, char **d) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
     if (c->capturing) {
-        // the graph owns its argument area (pinned host + device, allocated before the
-        // capture began and never reused): the copy is recorded as a graph node
-        const size_t need = (bytes + 255) & ~(size_t)255;
-        if (c->cap_used + need > fhea_ctx::CAP_BYTES) return nullptr;
-        char *h = (char *)c->cap_host + c->cap_used;
-        char *d = (char *)c->cap_dev + c->cap_used;
-        memcpy(h, host, bytes);
-        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+        if (c->cap_used + need > fhea_ctx::CAP_BYTES) return FHEA_ERR_NO_MEM;
+        *h = (char *)c->cap_host + c->cap_used;
+        *d = (char *)c->cap_dev + c->cap_used;
         c->cap_used += need;
-        return d;
+        return FHEA_OK;
     }
-    size_t need = (bytes + 255) & ~(size_t)255;
     if (c->args_used + need > c->args_bytes) {
         // every earlier copy must have landed before the staging area is reused
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
         c->args_used = 0;
-        if (need > c->args_bytes) return nullptr;
+        if (need > c->args_bytes) return FHEA_ERR_NO_MEM;
     }
-    char *h = (char *)c->args_host + c->args_used;
-    char *d = (char *)c->args + c->args_used;
-    memcpy(h, host, bytes);
-    if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+    *h = (char *)c->args_host + c->args_used;
+    *d = (char *)c->args + c->args_used;
     c->args_used += need;
+    return FHEA_OK;
+}
+
+static int stage_commit(fhea_ctx *c, const char *h, char *d, size_t bytes) {
+    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess ? FHEA_OK
+                                                                                       : FHEA_ERR_DEVICE;
+}
+
+void *stage_args(fhea_ctx *c, const void *host, size_t bytes) {
+    char *h = nullptr, *d = nullptr;
+    if (stage_reserve(c, bytes, &h, &d)) return nullptr;
+    memcpy(h, host, bytes);
+    if (stage_commit(c, h, d, bytes)) return nullptr;
     return d;
 }
 
@@ -772,6 +783,12 @@ static bool range_ok(u64 off, u64 words, u64 total) { return off <= total && wor
 
 static bool overlaps(u64 a, u64 b, u64 W) { return a < b + W && b < a + W; }
 
+// POLY_CHECK as a value: FHEA_OK, FHEA_ERR_ALIGN or FHEA_ERR_RANGE
+static int poly_code(const fhea_ctx *c, u64 off, u64 words) {
+    if (off & 7) return FHEA_ERR_ALIGN;
+    return range_ok(off, words, c->poly_words) ? FHEA_OK : FHEA_ERR_RANGE;
+}
+
 static int key_check(const fhea_ctx *c, u64 evk, u32 key_limbs) {
     if (evk & 7) return FHEA_ERR_ALIGN;
     u64 words = 2ull * c->h.dnum * (key_limbs + c->h.K) * c->h.N;
@@ -940,6 +957,21 @@ int fhea_write_poly(fhea_ctx *c, uint64_t off, const uint64_t *h, uint64_t n) {
 }
 int fhea_read_poly(fhea_ctx *c, uint64_t off, uint64_t *h, uint64_t n) {
     return c ? arena_copy(c, c->poly, c->poly_words, off, h, n, false) : FHEA_ERR_STATE;
+}
+int fhea_read_polys(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    if (!off || !count || (!h && n)) return FHEA_ERR_ARG;
+    for (u32 k = 0; k < n; ++k)
+        if (!range_ok(off[k], count[k], c->poly_words)) return FHEA_ERR_RANGE;
+    u64 at = 0;
+    for (u32 k = 0; k < n; ++k) {
+        if (count[k] &&
+            hipMemcpyAsync(h + at, c->poly + off[k], count[k] * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        at += count[k];
+    }
+    return check_hip(hipStreamSynchronize(c->stream));
 }
 int fhea_write_key(fhea_ctx *c, uint64_t off, const uint64_t *h, uint64_t n) {
     return c ? arena_copy(c, c->key, c->key_words, off, (void *)h, n, true) : FHEA_ERR_STATE;
@@ -1349,84 +1381,134 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
     return FHEA_OK;
 }
 
+// Diagonal-method inner products for a batch of jobs (see fhea.h).  All units of
+// all jobs run in one launch, so every output is checked against every input and
+// every other output (intervals of W words, sorted).
+int fhea_eval_linear_sum_batch(fhea_ctx *c, uint32_t n_jobs, uint32_t R, const uint64_t *ct0,
+                               const uint64_t *ct1, const uint32_t *G, const uint64_t *pt,
+                               const uint64_t *ptb, const uint64_t *bias0, const uint64_t *bias1,
+                               const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (!ct0 || !ct1 || !G || !pt || !out0 || !out1 || n_jobs == 0 || R == 0 || limbs == 0 || limbs > c->h.L)
+        return FHEA_ERR_ARG;
+    if ((bias0 == nullptr) != (bias1 == nullptr)) return FHEA_ERR_ARG;
+    const u64 W = (u64)limbs * c->h.N;
+    size_t sumG = 0, n_units = 0;
+    for (u32 j = 0; j < n_jobs; ++j) {
+        if (G[j] == 0) return FHEA_ERR_ARG;
+        sumG += G[j];
+        n_units += (G[j] + LIN_G - 1) / LIN_G;
+    }
+    if (n_units > 65535) return FHEA_ERR_ARG;
+    std::vector<u64> ins, outs;
+    ins.reserve((size_t)n_jobs * 2 * R + sumG * (R + 1) + 2 * n_jobs);
+    outs.reserve(2 * sumG);
+    int st = FHEA_OK;
+    auto in = [&](u64 o) {
+        if (!st) st = poly_code(c, o, W);
+        ins.push_back(o);
+    };
+    for (size_t k = 0; k < (size_t)n_jobs * R; ++k) in(ct0[k]);
+    for (size_t k = 0; k < (size_t)n_jobs * R; ++k) in(ct1[k]);
+    for (size_t k = 0; k < sumG * R; ++k) in(pt[k]);
+    for (size_t k = 0; ptb && k < sumG; ++k)
+        if (ptb[k] != FHEA_NONE) in(ptb[k]);
+    for (u32 j = 0; bias0 && j < n_jobs; ++j) {
+        if ((bias0[j] == FHEA_NONE) != (bias1[j] == FHEA_NONE)) return FHEA_ERR_ARG;
+        if (bias0[j] != FHEA_NONE) {
+            in(bias0[j]);
+            in(bias1[j]);
+        }
+    }
+    for (size_t k = 0; k < sumG; ++k) {
+        if (!st) st = poly_code(c, out0[k], W);
+        if (!st) st = poly_code(c, out1[k], W);
+        outs.push_back(out0[k]);
+        outs.push_back(out1[k]);
+    }
+    if (st) return st;
+    // outputs must not overlap any input (blocks of other units read inputs late)
+    // nor each other: outputs sorted, every input looked up among them
+    std::sort(outs.begin(), outs.end());
+    for (size_t k = 0; k + 1 < outs.size(); ++k)
+        if (outs[k + 1] - outs[k] < W) return FHEA_ERR_ARG;
+    for (u64 x : ins) {
+        auto it = std::lower_bound(outs.begin(), outs.end(), x >= W - 1 ? x - (W - 1) : 0);
+        if (it != outs.end() && overlaps(x, *it, W)) return FHEA_ERR_ARG;
+    }
+    // staged: per job [ct0 R][ct1 R][pt G*R][ptb G][out0 G][out1 G], then the unit table
+    const size_t off_words = (size_t)n_jobs * 2 * R + sumG * (R + 3);
+    const size_t bytes = off_words * 8 + n_units * sizeof(LinUnit);
+    char *h = nullptr, *d = nullptr;
+    if ((st = stage_reserve(c, bytes, &h, &d))) return st;
+    u64 *hw = (u64 *)h;
+    LinUnit *hu = (LinUnit *)(h + off_words * 8);
+    size_t w = 0, u = 0, gk = 0;
+    for (u32 j = 0; j < n_jobs; ++j) {
+        const u32 g = G[j];
+        const u64 *job_dev = (const u64 *)(d + w * 8);
+        memcpy(hw + w, ct0 + (size_t)j * R, R * 8);
+        memcpy(hw + w + R, ct1 + (size_t)j * R, R * 8);
+        memcpy(hw + w + 2 * R, pt + gk * R, (size_t)g * R * 8);
+        u64 *tail = hw + w + 2 * R + (size_t)g * R;
+        for (u32 k = 0; k < g; ++k) {
+            tail[k] = ptb ? ptb[gk + k] : FHEA_NONE;
+            tail[g + k] = out0[gk + k];
+            tail[2 * g + k] = out1[gk + k];
+        }
+        const u64 b0 = bias0 ? bias0[j] : FHEA_NONE, b1 = bias1 ? bias1[j] : FHEA_NONE;
+        for (u32 g0 = 0; g0 < g; g0 += LIN_G) hu[u++] = LinUnit{job_dev, b0, b1, g, g0};
+        w += 2 * (size_t)R + (size_t)g * (R + 3);
+        gk += g;
+    }
+    if ((st = stage_commit(c, h, d, bytes))) return st;
+    launch_linear_sum(c->dt, c->poly, (const LinUnit *)(d + off_words * 8), (u32)n_units, R, limbs, c->stream);
+    return check_hip(hipGetLastError());
+}
+
 int fhea_eval_linear_sum(fhea_ctx *c, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,
                          const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,
                          uint64_t bias1, const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
     CTX_CHECK(c);
-    if (!ct0 || !ct1 || !pt || !out0 || !out1 || R == 0 || G == 0 || limbs == 0 || limbs > c->h.L)
-        return FHEA_ERR_ARG;
     if ((bias0 == FHEA_NONE) != (bias1 == FHEA_NONE)) return FHEA_ERR_ARG;
-    if (G > 65535u * 8u) return FHEA_ERR_ARG;
-    const u64 W = (u64)limbs * c->h.N;
-    std::vector<u64> offs;
-    offs.reserve(2 * R + (size_t)G * R + 3 * G);
-    for (u32 r = 0; r < R; ++r) {
-        POLY_CHECK(c, ct0[r], W);
-        offs.push_back(ct0[r]);
-    }
-    for (u32 r = 0; r < R; ++r) {
-        POLY_CHECK(c, ct1[r], W);
-        offs.push_back(ct1[r]);
-    }
-    for (size_t k = 0; k < (size_t)G * R; ++k) {
-        POLY_CHECK(c, pt[k], W);
-        offs.push_back(pt[k]);
-    }
-    for (u32 g = 0; g < G; ++g) {
-        const u64 o = ptb ? ptb[g] : FHEA_NONE;
-        if (o != FHEA_NONE) POLY_CHECK(c, o, W);
-        offs.push_back(o);
-    }
-    if (bias0 != FHEA_NONE) {
-        POLY_CHECK(c, bias0, W);
-        POLY_CHECK(c, bias1, W);
-    }
-    const size_t n_in = offs.size();
-    for (u32 g = 0; g < G; ++g) {
-        POLY_CHECK(c, out0[g], W);
-        POLY_CHECK(c, out1[g], W);
-    }
-    // outputs must not overlap any input or each other (blocks of other outputs read inputs late)
-    for (u32 g = 0; g < G; ++g) {
-        for (u64 o : {out0[g], out1[g]}) {
-            for (size_t k = 0; k < n_in; ++k)
-                if (offs[k] != FHEA_NONE && overlaps(o, offs[k], W)) return FHEA_ERR_ARG;
-            if (bias0 != FHEA_NONE && (overlaps(o, bias0, W) || overlaps(o, bias1, W))) return FHEA_ERR_ARG;
-        }
-        if (overlaps(out0[g], out1[g], W)) return FHEA_ERR_ARG;
-        for (u32 h2 = g + 1; h2 < G; ++h2)
-            if (overlaps(out0[g], out0[h2], W) || overlaps(out0[g], out1[h2], W) ||
-                overlaps(out1[g], out0[h2], W) || overlaps(out1[g], out1[h2], W))
-                return FHEA_ERR_ARG;
-    }
-    for (u32 g = 0; g < G; ++g) offs.push_back(out0[g]);
-    for (u32 g = 0; g < G; ++g) offs.push_back(out1[g]);
-    auto *od = (const u64 *)stage_args(c, offs.data(), offs.size() * 8);
-    if (!od) return FHEA_ERR_NO_MEM;
-    launch_linear_sum(c->dt, c->poly, od, R, G, bias0, bias1, limbs, c->stream);
-    return check_hip(hipGetLastError());
+    return fhea_eval_linear_sum_batch(c, 1, R, ct0, ct1, &G, pt, ptb, &bias0, &bias1, out0, out1, limbs);
 }
 
-int fhea_rescale_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs) {
+int fhea_rescale_scatter(fhea_ctx *c, uint64_t src, const uint64_t *dst, uint32_t n, uint32_t limbs) {
     CTX_CHECK(c);
-    if (limbs < 2 || limbs > c->h.L || n == 0) return FHEA_ERR_ARG;
+    if (!dst || limbs < 2 || limbs > c->h.L || n == 0) return FHEA_ERR_ARG;
     const u64 N = c->h.N, W = (u64)limbs * N, W1 = (u64)(limbs - 1) * N;
     POLY_CHECK(c, src, 2 * W * n);
-    POLY_CHECK(c, dst, 2 * W1 * n);
-    if (src < dst + 2 * W1 * n && dst < src + 2 * W * n) return FHEA_ERR_ARG;
+    std::vector<u64> d(dst, dst + n);
+    for (u64 o : d) POLY_CHECK(c, o, 2 * W1);
+    std::sort(d.begin(), d.end());
+    for (u32 z = 0; z < n; ++z) {
+        if (z + 1 < n && d[z + 1] - d[z] < 2 * W1) return FHEA_ERR_ARG;
+        if (src < d[z] + 2 * W1 && d[z] < src + 2 * W * n) return FHEA_ERR_ARG;
+    }
     for (u32 off = 0; off < n; off += 64) {
         const u32 nb = std::min<u32>(64, n - off);
         int st = ensure_ws(c, nb * rescale_words(c, limbs));
         if (st) return st;
         std::vector<u64 *> outs(2 * nb);
         for (u32 z = 0; z < nb; ++z) {
-            outs[2 * z] = c->poly + dst + (u64)(off + z) * 2 * W1;
+            outs[2 * z] = c->poly + dst[off + z];
             outs[2 * z + 1] = outs[2 * z] + W1;
         }
         st = rescale_core(c, limbs, nb, c->poly + src + (u64)off * 2 * W, 2 * W, W, outs, c->ws);
         if (st) return st;
     }
     return check_hip(hipGetLastError());
+}
+
+int fhea_rescale_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (limbs < 2 || limbs > c->h.L || n == 0) return FHEA_ERR_ARG;
+    const u64 W1 = (u64)(limbs - 1) * c->h.N;
+    POLY_CHECK(c, dst, 2 * W1 * n);
+    std::vector<u64> d(n);
+    for (u32 z = 0; z < n; ++z) d[z] = dst + (u64)z * 2 * W1;
+    return fhea_rescale_scatter(c, src, d.data(), n, limbs);
 }
 
 int fhea_ntt_batch(fhea_ctx *c, uint64_t src, uint64_t dst, uint32_t npolys, uint32_t limbs,

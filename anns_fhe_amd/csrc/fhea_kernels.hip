@@ -593,19 +593,22 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 // (interactive_query_manager.cpp:190-274); with the query's rotations computed
 // once, every candidate block is one plaintext multiply-accumulate over r, which
 // is this kernel: thread (j, limb) holds LIN_G outputs, streams the R rotated
-// ciphertext words once and the G*R plaintext words once.
+// ciphertext words once and the G*R plaintext words once.  Every unit of every
+// job of a batch is one slice of the grid, so one launch fills the GPU.
 // ---------------------------------------------------------------------------
-static constexpr u32 LIN_G = 8;
 
 __device__ __forceinline__ u64 lin_word(const u64 *P, u64 off, size_t idx) {
     return off == LIN_NONE ? 0 : P[off + idx];
 }
 
 // Integer form: 128-bit lazy sums, reduced every 32 products (< 2^127).
-__global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const u64 *__restrict__ offs,
-                                                    u32 R, u32 G, u64 bias0, u64 bias1,
+__global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const LinUnit *__restrict__ units, u32 R,
                                                     const ModParams *__restrict__ mp, u32 logN) {
-    const u32 limb = blockIdx.y, g0 = blockIdx.z * LIN_G;
+    const LinUnit U = units[blockIdx.z];
+    const u64 *offs = U.offs;
+    const u32 G = U.G, g0 = U.g0;
+    const u64 bias0 = U.bias0, bias1 = U.bias1;
+    const u32 limb = blockIdx.y;
     const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
     const ModParams m = mp[limb];
     const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
@@ -646,10 +649,13 @@ __global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const u
 
 // FP64 form (primes < 2^50): mulmod_vv_f64 terms (<= 0.875q), sums centred every
 // 8 products (<= q/2 + 2 + 7q).
-__global__ __launch_bounds__(TPB) void k_linear_sum_f64(u64 *__restrict__ P, const u64 *__restrict__ offs,
-                                                        u32 R, u32 G, u64 bias0, u64 bias1,
-                                                        const ModParams *__restrict__ mp, u32 logN) {
-    const u32 limb = blockIdx.y, g0 = blockIdx.z * LIN_G;
+__global__ __launch_bounds__(TPB) void k_linear_sum_f64(u64 *__restrict__ P, const LinUnit *__restrict__ units,
+                                                        u32 R, const ModParams *__restrict__ mp, u32 logN) {
+    const LinUnit U = units[blockIdx.z];
+    const u64 *offs = U.offs;
+    const u32 G = U.G, g0 = U.g0;
+    const u64 bias0 = U.bias0, bias1 = U.bias1;
+    const u32 limb = blockIdx.y;
     const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
     const double q = mp[limb].qd, qinv = mp[limb].qinv;
     const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
@@ -689,13 +695,13 @@ __global__ __launch_bounds__(TPB) void k_linear_sum_f64(u64 *__restrict__ P, con
     }
 }
 
-void launch_linear_sum(const DevTables &t, u64 *P, const u64 *offs_dev, u32 R, u32 G, u64 bias0, u64 bias1,
-                       u32 limbs, hipStream_t s) {
-    dim3 grid(t.N / TPB, limbs, (G + LIN_G - 1) / LIN_G);
+void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,
+                       hipStream_t s) {
+    dim3 grid(t.N / TPB, limbs, n_units);
     if (t.use_f64 && !(t.f64_off & 8))
-        hipLaunchKernelGGL(k_linear_sum_f64, grid, dim3(TPB), 0, s, P, offs_dev, R, G, bias0, bias1, t.mp, t.logN);
+        hipLaunchKernelGGL(k_linear_sum_f64, grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);
     else
-        hipLaunchKernelGGL(k_linear_sum, grid, dim3(TPB), 0, s, P, offs_dev, R, G, bias0, bias1, t.mp, t.logN);
+        hipLaunchKernelGGL(k_linear_sum, grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);
 }
 
 }  // namespace fhea

@@ -153,12 +153,21 @@ struct AddArgs {
 };
 void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u32 limbs, hipStream_t s);
 
-// Plaintext-matrix x encrypted-vector inner products (diagonal method):
+// Plaintext-matrix x encrypted-vector inner products (diagonal method), for a
+// batch of jobs, each one ciphertext set and G plaintext blocks:
 //   out_g = bias + ptb_g + sum_{r<R} ct_r (.) pt_{g,r}
-// offs (device, u64 word offsets into P): [ct0: R][ct1: R][pt: G*R][ptb: G][out0: G][out1: G];
-// ptb entries and bias0/bias1 may be LIN_NONE.
+// A job's offs (device, u64 word offsets into P):
+//   [ct0: R][ct1: R][pt: G*R][ptb: G][out0: G][out1: G];
+// ptb entries and bias0/bias1 may be LIN_NONE.  A unit is LIN_G consecutive
+// blocks g0.. of one job (one grid slice).
 static constexpr u64 LIN_NONE = ~0ull;
-void launch_linear_sum(const DevTables &t, u64 *P, const u64 *offs_dev, u32 R, u32 G, u64 bias0, u64 bias1,
-                       u32 limbs, hipStream_t s);
+static constexpr u32 LIN_G = 8;
+struct LinUnit {
+    const u64 *offs;
+    u64 bias0, bias1;
+    u32 G, g0;
+};
+void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,
+                       hipStream_t s);
 
 }  // namespace fhea

@@ -104,6 +104,17 @@ class FheContext:
         check(self.lib.fhea_read_poly(self.h, off, _p64(a), count), "read_poly")
         return a if out is None else a[:count]
 
+    def read_polys(self, ranges, out=None):
+        """Several arena ranges [(off, count), ...] back to back into one array (or
+        into `out`, e.g. a PinnedArray's array), with one synchronisation."""
+        offs = np.array([r[0] for r in ranges], np.uint64)
+        cnts = np.array([r[1] for r in ranges], np.uint64)
+        total = int(cnts.sum())
+        a = np.zeros(total, np.uint64) if out is None else out
+        assert a.dtype == np.uint64 and a.size >= total and a.flags["C_CONTIGUOUS"]
+        check(self.lib.fhea_read_polys(self.h, len(ranges), _p64(offs), _p64(cnts), _p64(a)), "read_polys")
+        return a[:total]
+
     def write_key(self, off, data):
         a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
         check(self.lib.fhea_write_key(self.h, off, _p64(a), a.size), "write_key")
@@ -142,8 +153,40 @@ class FheContext:
                                             _p64(pb) if pb is not None else C.cast(None, _lib.u64p), G, b0, b1,
                                             _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum")
 
+    def linear_sum_batch(self, jobs, limbs):
+        """One launch for several linear sums.  jobs: list of dicts with ct0, ct1
+        (R offsets each), pt (G x R offsets), out0, out1 (G each) and optional ptb (G)
+        and bias (pair of offsets)."""
+        R = len(jobs[0]["ct0"])
+        c0 = np.concatenate([np.asarray(j["ct0"], np.uint64) for j in jobs])
+        c1 = np.concatenate([np.asarray(j["ct1"], np.uint64) for j in jobs])
+        pts = [np.asarray(j["pt"], np.uint64).reshape(-1, R) for j in jobs]
+        G = np.array([p.shape[0] for p in pts], np.uint32)
+        pt = np.ascontiguousarray(np.concatenate(pts).reshape(-1))
+        o0 = np.concatenate([np.asarray(j["out0"], np.uint64) for j in jobs])
+        o1 = np.concatenate([np.asarray(j["out1"], np.uint64) for j in jobs])
+        assert c0.size == R * len(jobs) and c1.size == c0.size and o0.size == G.sum() and o1.size == G.sum()
+        nil = C.cast(None, _lib.u64p)
+        pb = None
+        if any(j.get("ptb") is not None for j in jobs):
+            pb = np.concatenate([np.asarray(j["ptb"], np.uint64) if j.get("ptb") is not None
+                                 else np.full(g, _lib.NONE, np.uint64) for j, g in zip(jobs, G)])
+        b0 = b1 = None
+        if any(j.get("bias") is not None for j in jobs):
+            b0 = np.array([j["bias"][0] if j.get("bias") is not None else _lib.NONE for j in jobs], np.uint64)
+            b1 = np.array([j["bias"][1] if j.get("bias") is not None else _lib.NONE for j in jobs], np.uint64)
+        check(self.lib.fhea_eval_linear_sum_batch(
+            self.h, len(jobs), R, _p64(c0), _p64(c1), G.ctypes.data_as(_lib.u32p), _p64(pt),
+            _p64(pb) if pb is not None else nil, _p64(b0) if b0 is not None else nil,
+            _p64(b1) if b1 is not None else nil, _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum_batch")
+
     def rescale_batch(self, src, dst, n, limbs):
         check(self.lib.fhea_rescale_batch(self.h, src, dst, n, limbs), "fhea_rescale_batch")
+
+    def rescale_scatter(self, src, dst, limbs):
+        """Rescale len(dst) ciphertexts stored back to back at src; ciphertext z to dst[z]."""
+        d = np.ascontiguousarray(dst, dtype=np.uint64)
+        check(self.lib.fhea_rescale_scatter(self.h, src, _p64(d), d.size, limbs), "fhea_rescale_scatter")
 
     # ---- HIP graphs: record a fixed sequence once, replay it on new data at the same offsets
     def graph_begin(self):

@@ -223,22 +223,26 @@ class FheIvfPqServer:
         self.max_g = max_blocks_per_query
         self.pt_words = len(self.blocks) * self.block_words
         self.ct_words = 2 * self.L * self.N
-        self.slot_words = (self.ct_words + (self.D - 1) * 2 * Wl + 4 * Wl + self.max_g * 2 * Wl
-                           + self.max_g * 2 * (lt - 1) * self.N)
+        # per query slot: the ciphertext, its rotations, norm, scratch and its results;
+        # then one output area shared by the slots of a call (packed in job order)
+        self.slot_words = (self.ct_words + (self.D - 1) * 2 * Wl + 4 * Wl + self.max_g * 2 * (lt - 1) * self.N)
+        self.out_base = self.pt_words + max_queries * self.slot_words
         self.max_queries = max_queries
-        self._pinned = None  # page-locked read buffer (engine contexts)
+        self._pinned = None  # page-locked read buffers (engine contexts)
+        self._pinned_all = None
+        self._copy_pool = None
         self.use_graphs = False
         self.max_graphs = 16
         self.graphs = {}
-        need = self.pt_words + max_queries * self.slot_words
+        need = self.out_base + max_queries * self.max_g * 2 * Wl
         assert need <= ctx.poly_words, (need, ctx.poly_words)
         self._encode_blocks(threads)
 
     @staticmethod
     def arena_words(N, L, lt, D, n_blocks, max_queries, max_g):
         Wl = lt * N
-        slot = 2 * L * N + (D - 1) * 2 * Wl + 4 * Wl + max_g * 2 * Wl + max_g * 2 * (lt - 1) * N
-        return n_blocks * (D + 1) * Wl + max_queries * slot + 64
+        slot = 2 * L * N + (D - 1) * 2 * Wl + 4 * Wl + max_g * 2 * (lt - 1) * N
+        return n_blocks * (D + 1) * Wl + max_queries * (slot + max_g * 2 * Wl) + 64
 
     def _encode_blocks(self, threads):
         D, S, lt = self.D, self.S, self.lt
@@ -284,8 +288,7 @@ class FheIvfPqServer:
         o["rot"] = b + self.ct_words  # rotation r (1..D-1) at rot + (r-1)*2*Wl
         o["norm"] = o["rot"] + (D - 1) * 2 * Wl
         o["tmp"] = o["norm"] + 2 * Wl
-        o["out"] = o["tmp"] + 2 * Wl
-        o["res"] = o["out"] + self.max_g * 2 * Wl
+        o["res"] = o["tmp"] + 2 * Wl  # result g at res + g*2*(lt-1)*N
         return o
 
     def upload(self, z, ct):
@@ -321,21 +324,55 @@ class FheIvfPqServer:
             stride *= 2
 
     def distances(self, z, block_ids):
-        """Encrypted squared distances of the candidates of `block_ids` for query slot z;
-        leaves the rescaled ciphertexts (lt-1 limbs) at o['res'] + g*2*(lt-1)*N."""
+        self.distances_batch([(z, block_ids)])
+
+    def distances_batch(self, jobs):
+        """Encrypted squared distances for several query slots in one launch: jobs is a
+        list of (z, block_ids).  Slot z's rescaled ciphertexts (lt-1 limbs) are left
+        at o['res'] + g*2*(lt-1)*N."""
         N, lt, D = self.N, self.lt, self.D
         Wl = lt * N
-        G = len(block_ids)
-        assert 0 < G <= self.max_g
-        o = self._offs(z)
-        ct0 = [o["q0"]] + [o["rot"] + (r - 1) * 2 * Wl for r in range(1, D)]
-        ct1 = [o["q1"]] + [o["rot"] + (r - 1) * 2 * Wl + Wl for r in range(1, D)]
-        pt = np.array([[b * self.block_words + r * Wl for r in range(D)] for b in block_ids], np.uint64)
-        ptb = [b * self.block_words + D * Wl for b in block_ids]
-        out0 = [o["out"] + g * 2 * Wl for g in range(G)]
-        out1 = [x + Wl for x in out0]
-        self.ctx.linear_sum(ct0, ct1, pt, out0, out1, lt, ptb=ptb, bias=(o["norm"], o["norm"] + Wl))
-        self.ctx.rescale_batch(o["out"], o["res"], G, lt)
+        lin, res, k = [], [], 0
+        rot = np.arange(1, D, dtype=np.uint64) - 1
+        for z, block_ids in jobs:
+            G = len(block_ids)
+            assert 0 < G <= self.max_g
+            o = self._offs(z)
+            b = np.asarray(block_ids, np.uint64) * self.block_words
+            out0 = self.out_base + (k + np.arange(G, dtype=np.uint64)) * (2 * Wl)
+            ct0 = np.concatenate([[o["q0"]], o["rot"] + rot * (2 * Wl)]).astype(np.uint64)
+            lin.append(dict(ct0=ct0, ct1=np.concatenate([[o["q1"]], o["rot"] + rot * (2 * Wl) + Wl]).astype(np.uint64),
+                            pt=b[:, None] + np.arange(D, dtype=np.uint64)[None, :] * Wl, ptb=b + D * Wl,
+                            out0=out0, out1=out0 + Wl, bias=(o["norm"], o["norm"] + Wl)))
+            res.append(o["res"] + np.arange(G, dtype=np.uint64) * (2 * (lt - 1) * N))
+            k += G
+        res = np.concatenate(res)
+        assert k <= self.max_queries * self.max_g
+        self.ctx.linear_sum_batch(lin, lt)
+        self.ctx.rescale_scatter(self.out_base, res, lt)
+
+    def read_distances_batch(self, items):
+        """{z: (G, 2*(lt-1)*N) array} for items [(z, G), ...]: one gathered read into
+        page-locked memory, then copies out on host threads."""
+        n = 2 * (self.lt - 1) * self.N
+        if not hasattr(self.ctx, "read_polys"):
+            return {z: self.read_distances(z, G) for z, G in items}
+        import concurrent.futures as cf
+        from .driver import PinnedArray
+        ranges = [(self._offs(z)["res"], G * n) for z, G in items]
+        total = sum(c for _, c in ranges)
+        if self._pinned_all is None or self._pinned_all.words < total:
+            if self._pinned_all is not None:
+                self._pinned_all.close()
+            self._pinned_all = PinnedArray(max(total, self.max_queries * n))
+        buf = self.ctx.read_polys(ranges, out=self._pinned_all.array)
+        at, parts = 0, []
+        for z, G in items:
+            parts.append((z, at, G))
+            at += G * n
+        if self._copy_pool is None:
+            self._copy_pool = cf.ThreadPoolExecutor(8)
+        return dict(self._copy_pool.map(lambda p: (p[0], buf[p[1]:p[1] + p[2] * n].reshape(p[2], n).copy()), parts))
 
     def read_distances(self, z, G):
         n = 2 * (self.lt - 1) * self.N
@@ -358,22 +395,26 @@ class FheIvfPqServer:
             self.ctx.graph_launch(gid)
             return
         self.prepare(slots)
-        for z in slots:
-            self.distances(z, [0])
+        self.distances_batch([(z, [0]) for z in slots])
         if self.use_graphs and len(self.graphs) < self.max_graphs:
             self.ctx.sync()
             self.ctx.graph_begin()
             try:
                 self.prepare(slots)
-                for z in slots:
-                    self.distances(z, [0])
+                self.distances_batch([(z, [0]) for z in slots])
             finally:
                 self.graphs[key] = self.ctx.graph_end()
 
     def stage3_fine_candidate_batch_distance(self, z, clusters):
-        blocks = [b for c in clusters for b in self.cluster_blocks[c]]
-        if blocks:
-            self.distances(z, blocks)
+        return self.stage3_fine_candidate_distances([(z, clusters)])[z]
+
+    def stage3_fine_candidate_distances(self, probes):
+        """Candidate distances of several queries in one launch: probes is a list of
+        (z, clusters); returns {z: candidate block ids}."""
+        blocks = {z: [b for c in clusters for b in self.cluster_blocks[c]] for z, clusters in probes}
+        jobs = [(z, b) for z, b in blocks.items() if b]
+        if jobs:
+            self.distances_batch(jobs)
         return blocks
 
 
@@ -402,7 +443,8 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     slots = list(range(B))
     t1 = time.perf_counter()
     server.stage1_coarse_centroid_distance(slots)
-    coarse = [server.read_distances(z, 1)[0] for z in slots]
+    cmap = server.read_distances_batch([(z, 1) for z in slots])
+    coarse = [cmap[z][0] for z in slots]
     t["server_stage1"] = time.perf_counter() - t1
     # stage 2 (client): decrypt the coarse distances, pick n_probe clusters
     t2 = time.perf_counter()
@@ -414,8 +456,10 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     probes = list(pool.map(choose, slots))
     t["client_stage2"] = time.perf_counter() - t2
     t3 = time.perf_counter()
-    blocks = [server.stage3_fine_candidate_batch_distance(z, probes[z]) for z in slots]
-    fine = [server.read_distances(z, len(blocks[z])) for z in slots]
+    bmap = server.stage3_fine_candidate_distances([(z, probes[z]) for z in slots])
+    blocks = [bmap[z] for z in slots]
+    fmap = server.read_distances_batch([(z, len(blocks[z])) for z in slots if blocks[z]])
+    fine = [fmap.get(z) for z in slots]
     t["server_stage3"] = time.perf_counter() - t3
     t4 = time.perf_counter()
 

@@ -64,7 +64,8 @@ class ParamSet:
         self.max_slots = max_slots
         ksz = rel.size
         self.static_bytes = 8 * (self.server.pt_words + index.dim * ksz)
-        self.slot_bytes = 8 * self.server.slot_words
+        # a slot's own words and its share of the output area
+        self.slot_bytes = 8 * (self.server.slot_words + max_g * 2 * lt * ctx.N)
 
     @staticmethod
     def sizes(N, L, K, dnum, index, max_slots, lt=2, max_probe=8):
@@ -195,17 +196,17 @@ class ServingLoop:
                         for r in b1:
                             srv.upload(r.slot, r.ct)
                         srv.stage1_coarse_centroid_distance([r.slot for r in b1])
+                        cmap = srv.read_distances_batch([(r.slot, 1) for r in b1])
                         for r in b1:
-                            coarse = srv.read_distances(r.slot, 1)[0]
+                            coarse = cmap[r.slot][0]
                             r.events["stage1"] = now()
                             pool.submit(client_stage2, r, coarse)
                     if b3:
-                        outs = []
-                        for r in b3:
-                            blocks = srv.stage3_fine_candidate_batch_distance(r.slot, r.probes)
-                            outs.append(blocks)
+                        bmap = srv.stage3_fine_candidate_distances([(r.slot, r.probes) for r in b3])
+                        outs = [bmap[r.slot] for r in b3]
+                        fmap = srv.read_distances_batch([(r.slot, len(b)) for r, b in zip(b3, outs) if b])
                         for r, blocks in zip(b3, outs):
-                            fine = srv.read_distances(r.slot, len(blocks))
+                            fine = fmap.get(r.slot)
                             r.events["stage3"] = now()
                             with lock:
                                 release(r)

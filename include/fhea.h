@@ -104,6 +104,9 @@ const char *fhea_status_string(int status);
  *      sim_memory_bus.cpp:12-64) ------------------------------------------- */
 int fhea_write_poly(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint64_t count);
 int fhea_read_poly(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
+/* n arena ranges [off[k], off[k] + count[k]) copied back to back into h (one
+ * synchronisation; page-locked h from fhea_host_alloc makes the copies DMA). */
+int fhea_read_polys(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h);
 int fhea_write_key(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint64_t count);
 int fhea_read_key(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
 int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t count);
@@ -189,6 +192,9 @@ int fhea_sync(fhea_ctx *ctx);
 /* Rescale (OP 8 semantics) of n ciphertexts stored back to back: ciphertext z at
  * src + z*2*limbs*N (c0 then c1) -> dst + z*2*(limbs-1)*N.  Ranges must not overlap. */
 int fhea_rescale_batch(fhea_ctx *ctx, uint64_t src, uint64_t dst, uint32_t n, uint32_t limbs);
+/* The same with ciphertext z written to dst[z] (2*(limbs-1)*N words each); the
+ * outputs must not overlap each other or the inputs. */
+int fhea_rescale_scatter(fhea_ctx *ctx, uint64_t src, const uint64_t *dst, uint32_t n, uint32_t limbs);
 
 /* HIP graph capture.  Engine work issued on the context between fhea_graph_begin
  * and fhea_graph_end (submits, ops, composites, NTT batches, linear sums, device
@@ -228,6 +234,17 @@ int fhea_ctx_get_tuning(const fhea_ctx *ctx, int knob, uint32_t *value);
 int fhea_eval_linear_sum(fhea_ctx *ctx, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,
                          const uint64_t *pt, const uint64_t *ptb, uint32_t G, uint64_t bias0,
                          uint64_t bias1, const uint64_t *out0, const uint64_t *out1, uint32_t limbs);
+
+/* The same for n_jobs jobs in one launch (for example every query of a batch
+ * against its candidate blocks).  Job j: ct0/ct1[j*R + r], G[j] blocks whose
+ * plaintexts, ptb and outputs follow those of job j-1 in pt (G[j]*R each), ptb,
+ * out0, out1; bias0/bias1[j] (both arrays NULL, or per job a ciphertext or
+ * FHEA_NONE twice).  No output may overlap any input of any job or another
+ * output (FHEA_ERR_ARG).  No reference counterpart (see fhea_eval_linear_sum). */
+int fhea_eval_linear_sum_batch(fhea_ctx *ctx, uint32_t n_jobs, uint32_t R, const uint64_t *ct0,
+                               const uint64_t *ct1, const uint32_t *G, const uint64_t *pt,
+                               const uint64_t *ptb, const uint64_t *bias0, const uint64_t *bias1,
+                               const uint64_t *out0, const uint64_t *out1, uint32_t limbs);
 
 /* ---- timing on the context's stream (HIP events) ----------------------- */
 int fhea_timer_start(fhea_ctx *ctx);

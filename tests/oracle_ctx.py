@@ -80,12 +80,23 @@ class OracleContext:
     def linear_sum(self, ct0, ct1, pt, out0, out1, limbs, ptb=None, bias=None):
         assert self.o.linear_sum(self.pm, ct0, ct1, pt, out0, out1, limbs, ptb=ptb, bias=bias) == 0
 
+    def linear_sum_batch(self, jobs, limbs):
+        # no output overlaps an input, so jobs run one after the other give the words
+        # of one launch
+        for j in jobs:
+            self.linear_sum(j["ct0"], j["ct1"], j["pt"], j["out0"], j["out1"], limbs, ptb=j.get("ptb"),
+                            bias=j.get("bias"))
+
     def rescale_batch(self, src, dst, n, limbs):
+        W1 = (limbs - 1) * self.N
+        self.rescale_scatter(src, [dst + z * 2 * W1 for z in range(n)], limbs)
+
+    def rescale_scatter(self, src, dst, limbs):
         W, W1 = limbs * self.N, (limbs - 1) * self.N
-        for z in range(n):
+        for z, d in enumerate(dst):
             for k in range(2):
                 a = self.pm[src + z * 2 * W + k * W:src + z * 2 * W + (k + 1) * W]
-                self.pm[dst + z * 2 * W1 + k * W1:dst + z * 2 * W1 + (k + 1) * W1] = self.o.rescale(a).reshape(-1)
+                self.pm[d + k * W1:d + (k + 1) * W1] = self.o.rescale(a).reshape(-1)
 
 
 class OracleDriver:

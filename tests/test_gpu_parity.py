@@ -571,3 +571,60 @@ def test_workload_tuner_keeps_results(orc):
     assert np.array_equal(got[:(L - 1) * N], pm[4 * n * W:4 * n * W + (L - 1) * N])
     assert np.array_equal(got[W:W + (L - 1) * N], pm[(4 * n + 1) * W:(4 * n + 1) * W + (L - 1) * N])
     AT.apply(ctx, {"ntt_split_fwd": 0, "ntt_split_inv": 0, "ntt_wg_per_cu": 0, "chunk": 16})
+
+
+@pytest.mark.parametrize("q_bits", [50, 60])
+def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
+    """fhea_eval_linear_sum_batch (jobs of different G sharing plaintext blocks, with
+    and without bias) equals the oracle's job-by-job sums; an output of one job
+    overlapping an input of another is refused; fhea_rescale_scatter equals the
+    oracle's rescale of every ciphertext."""
+    from anns_fhe_amd._lib import NONE, FheError
+    N, L = 4096, 3
+    ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(7 + q_bits)
+    W = L * N
+    R, n_blocks = 9, 6
+    n_in = 3 * 2 * R + n_blocks * R + n_blocks + 2
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(n_in)])
+    ctx.write_poly(0, data)
+    blk = lambda b: [(3 * 2 * R + b * R + r) * W for r in range(R)]
+    ptb_of = lambda b: (3 * 2 * R + n_blocks * R + b) * W if b % 2 else NONE
+    bias = ((n_in - 2) * W, (n_in - 1) * W)
+    out = n_in
+    jobs = []
+    for j, (blocks, bs) in enumerate((([0], bias), ([1, 2, 3, 4, 5], None), ([5, 4, 3, 2, 1, 0, 1, 2, 3, 4, 5], bias))):
+        G = len(blocks)
+        jobs.append(dict(ct0=[(2 * j * R + 2 * r) * W for r in range(R)],
+                         ct1=[(2 * j * R + 2 * r + 1) * W for r in range(R)],
+                         pt=np.array([blk(b) for b in blocks], np.uint64), ptb=[ptb_of(b) for b in blocks],
+                         out0=[(out + 2 * g) * W for g in range(G)], out1=[(out + 2 * g + 1) * W for g in range(G)],
+                         bias=bs))
+        out += 2 * G
+    say("linear sum batch", q_bits)
+    ctx.linear_sum_batch(jobs, L)
+    got = ctx.read_poly(n_in * W, (out - n_in) * W)
+    pm = np.zeros((out + 40) * W, np.uint64)
+    pm[:data.size] = data
+    for jb in jobs:
+        assert o.linear_sum(pm, jb["ct0"], jb["ct1"], jb["pt"], jb["out0"], jb["out1"], L, ptb=jb["ptb"],
+                            bias=jb["bias"]) == 0
+    assert np.array_equal(got, pm[n_in * W:out * W])
+    bad = [dict(jobs[0]), dict(jobs[1])]
+    bad[0]["out0"] = [jobs[1]["ct0"][4]]
+    with pytest.raises(FheError) as ei:
+        ctx.linear_sum_batch(bad, L)
+    assert ei.value.status == -4
+    # rescale of the 17 outputs (stored back to back) to scattered places
+    n = (out - n_in) // 2
+    W1 = (L - 1) * N
+    dst = [(out + 1) * W + ((7 * z) % n) * 2 * W1 for z in range(n)]
+    ctx.rescale_scatter(n_in * W, dst, L)
+    for z in range(n):
+        for k in range(2):
+            a = pm[(n_in + 2 * z + k) * W:(n_in + 2 * z + k + 1) * W]
+            assert np.array_equal(ctx.read_poly(dst[z] + k * W1, W1), o.rescale(a).reshape(-1)), (z, k)
+    with pytest.raises(FheError) as ei:
+        ctx.rescale_scatter(n_in * W, [dst[0], dst[0] + 8], L)
+    assert ei.value.status == -4

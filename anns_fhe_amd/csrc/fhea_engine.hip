@@ -669,40 +669,58 @@ struct RotItem {
     u64 *d0, *d1;
     u64 galois;
     const u64 *key;
+    u32 input;  // index into the call's inputs
+};
+struct RotInput {
+    const u64 *in0, *in1;
 };
 
-// Rotations of one ciphertext (in0, in1) by several Galois elements with one shared
-// ModUp: digits of c1 computed once, then per rotation the permuted inner product
-// with its key, ModDown, and sigma_g(c0) added (rotation = sigma_g(c0) + KS(sigma_g(c1))
-// with sigma_g applied to the digits instead of c1).
-static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const u64 *in0, const u64 *in1,
-                             const std::vector<RotItem> &items) {
+// Rotations of ciphertexts by several Galois elements each, every input ModUp'd
+// once: digits of c1 computed once per input, then per rotation the permuted
+// inner product with its key, ModDown, and sigma_g(c0) added (rotation =
+// sigma_g(c0) + KS(sigma_g(c1)) with sigma_g applied to the digits instead of c1).
+// All ModUps run first (batched); the rotations then run in chunks ordered by key,
+// so that a key is read from HBM once for all the inputs that use it.  Outputs
+// must not overlap any input (the caller checks).
+static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<RotInput> &inputs,
+                             std::vector<RotItem> items) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
     const u32 np = lt.nparts;
     hipStream_t s = c->stream;
+    const u32 ni = (u32)inputs.size();
     const u64 cz = (u64)np * (l + K) * N;
-    const u32 chunk = 32;
     const u64 per_item = 2 * l * N + 2 * (l + K) * N + 2 * K * N + 2 * l * N;
-    const u32 nmax = (u32)std::min<size_t>(chunk, items.size());
-    int st = ensure_ws(c, l * N + cz + nmax * per_item);
+    // chunk: up to 256 rotations, within 2 GiB of pipeline workspace
+    const u64 cap = std::max<u64>(1, std::min<u64>(256, (2ull << 30) / (8 * per_item)));
+    const u32 nmax = (u32)std::min<u64>(cap, items.size());
+    int st = ensure_ws(c, ni * (l * N + cz) + nmax * per_item);
     if (st) return st;
-    u64 *coef = c->ws, *compact = coef + l * N, *rest = compact + cz;
-    // ModUp of c1, once
-    launch_ntt(t, true, in1, coef, l * N, l * N, 1, l, t.ident, s);
+    u64 *coef = c->ws, *compact = coef + (u64)ni * l * N, *rest = compact + (u64)ni * cz;
+    // ModUp of every input's c1: INTT (one launch when the inputs are evenly spaced)
+    bool strided = ni > 1;
+    const long long step = ni > 1 ? (long long)(inputs[1].in1 - inputs[0].in1) : 0;
+    for (u32 k = 1; k < ni && strided; ++k) strided = (long long)(inputs[k].in1 - inputs[k - 1].in1) == step;
+    if (strided && step > 0) {
+        launch_ntt(t, true, inputs[0].in1, coef, (u64)step, l * N, ni, l, t.ident, s);
+    } else {
+        for (u32 k = 0; k < ni; ++k) launch_ntt(t, true, inputs[k].in1, coef + (u64)k * l * N, l * N, l * N, 1, l,
+                                                t.ident, s);
+    }
     launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K, coef,
-               l * N, compact, (l + K) * N, cz, 1, s);
+               l * N, compact, (l + K) * N, cz, ni, s);
     if (t.use_f64 && !(t.f64_off & 1)) {
         NttFuse fz;
         fz.slots = lt.modup_slots;
         fz.nslots = lt.n_modup_slots;
-        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, 1, np * (l + K), lt.modup_ntt_prime, fz, s);
+        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, ni, np * (l + K), lt.modup_ntt_prime, fz, s);
     } else {
-        launch_ntt(t, false, compact, compact, cz, cz, 1, np * (l + K), lt.modup_ntt_prime, s);
+        launch_ntt(t, false, compact, compact, cz, cz, ni, np * (l + K), lt.modup_ntt_prime, s);
     }
-    for (size_t off = 0; off < items.size(); off += chunk) {
-        const u32 nb = (u32)std::min<size_t>(chunk, items.size() - off);
+    std::stable_sort(items.begin(), items.end(), [](const RotItem &a, const RotItem &b) { return a.key < b.key; });
+    for (size_t off = 0; off < items.size(); off += nmax) {
+        const u32 nb = (u32)std::min<size_t>(nmax, items.size() - off);
         u64 *S = rest, *acc = S + (u64)nb * 2 * l * N, *pb = acc + (u64)nb * 2 * (l + K) * N;
         u64 *tt = pb + (u64)nb * 2 * K * N;
         std::vector<AutoArgs> aa(nb);
@@ -710,17 +728,17 @@ static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const u64 *in0, 
         std::vector<CombineArgs> cb(nb);
         for (u32 z = 0; z < nb; ++z) {
             const RotItem &it = items[off + z];
+            const RotInput &in = inputs[it.input];
             u64 *s0 = S + (u64)z * 2 * l * N;
-            aa[z] = {in0, in0, s0, s0 + l * N, it.galois};  // sigma_g(c0) (second output unused)
-            ra[z] = {in1, it.key, it.galois};
+            aa[z] = {in.in0, in.in0, s0, s0 + l * N, it.galois};  // sigma_g(c0) (second output unused)
+            ra[z] = {in.in1, it.key, it.galois, compact + (u64)it.input * cz};
             cb[z] = {it.d0, it.d1, s0, nullptr};
         }
         auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
         auto *ra_d = (const KsRotArgs *)stage_args(c, ra.data(), nb * sizeof(KsRotArgs));
         if (!aa_d || !ra_d) return FHEA_ERR_DEVICE;
         launch_automorph(t, aa_d, nb, l, s);
-        launch_ks_inner_rot(t, ra_d, compact, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum,
-                            s);
+        launch_ks_inner_rot(t, ra_d, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum, s);
         st = ks_moddown(c, l, nb, acc, pb, tt, cb);
         if (st) return st;
     }
@@ -1320,11 +1338,16 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
             }
             st = mult_relin(c, l, c->key + c->relin_off, kl, v);
         } else if (w0.kind == FHEA_WORK_ROTATE_HOISTED) {
-            // runs of items with the same input share one ModUp
+            // runs of items with the same input share one ModUp; when no output of
+            // the whole run overlaps an input of the run, all its inputs go through
+            // one pass (ModUps first, rotations key by key)
             if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
             u32 kl = w0.key_limbs ? w0.key_limbs : l;
             if (kl < l || kl > c->h.L) return FHEA_ERR_ARG;
             const u64 W = (u64)l * c->h.N;
+            std::vector<std::vector<RotItem>> groups;
+            std::vector<RotInput> ins;
+            std::vector<u64> in_offs, out_offs;
             uint32_t a = i;
             while (a < j) {
                 uint32_t b = a;
@@ -1342,10 +1365,31 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                     if (overlaps(w.d0, items[a].a0, W) || overlaps(w.d0, items[a].a1, W) ||
                         overlaps(w.d1, items[a].a0, W) || overlaps(w.d1, items[a].a1, W))
                         return FHEA_ERR_ARG;
-                    v.push_back({P + w.d0, P + w.d1, w.galois, c->key + w.evk});
+                    v.push_back({P + w.d0, P + w.d1, w.galois, c->key + w.evk, (u32)groups.size()});
+                    out_offs.push_back(w.d0);
+                    out_offs.push_back(w.d1);
                 }
-                if ((st = rotations_hoisted(c, l, kl, P + items[a].a0, P + items[a].a1, v))) return st;
+                ins.push_back({P + items[a].a0, P + items[a].a1});
+                in_offs.push_back(items[a].a0);
+                in_offs.push_back(items[a].a1);
+                groups.push_back(std::move(v));
                 a = b;
+            }
+            std::sort(out_offs.begin(), out_offs.end());
+            bool merge = groups.size() > 1;
+            for (u64 x : in_offs) {
+                auto it = std::lower_bound(out_offs.begin(), out_offs.end(), x >= W - 1 ? x - (W - 1) : 0);
+                if (it != out_offs.end() && overlaps(x, *it, W)) merge = false;
+            }
+            if (merge) {
+                std::vector<RotItem> all;
+                for (auto &g : groups) all.insert(all.end(), g.begin(), g.end());
+                if ((st = rotations_hoisted(c, l, kl, ins, std::move(all)))) return st;
+            } else {
+                for (size_t g = 0; g < groups.size(); ++g) {
+                    for (RotItem &it : groups[g]) it.input = 0;
+                    if ((st = rotations_hoisted(c, l, kl, {ins[g]}, std::move(groups[g])))) return st;
+                }
             }
         } else if (w0.kind == FHEA_WORK_ADD) {
             if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;

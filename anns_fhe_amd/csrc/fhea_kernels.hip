@@ -492,7 +492,7 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
 // evaluation-domain permutation) and multiplies with its own key.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_ks_inner_rot(const KsRotArgs *__restrict__ args,
-                                                      const u64 *__restrict__ compact, u64 *__restrict__ acc,
+                                                      u64 *__restrict__ acc,
                                                       u64 az, const ModParams *__restrict__ mp, u32 logN, u32 l,
                                                       u32 K, u32 L, u32 alpha, u32 nparts, u32 key_limbs,
                                                       u32 dnum) {
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_rot(const KsRotArgs *__restric
         if (i >= start && i < start + sz)
             d = A.c[((size_t)i << logN) + src];
         else
-            d = compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
+            d = A.compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
         s0 += (u128)d * kb[(size_t)p * kstride];
         s1 += (u128)d * ka[(size_t)p * kstride];
     }
@@ -524,7 +524,6 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_rot(const KsRotArgs *__restric
 }
 
 __global__ __launch_bounds__(TPB) void k_ks_inner_rot_f64(const KsRotArgs *__restrict__ args,
-                                                          const u64 *__restrict__ compact,
                                                           u64 *__restrict__ acc, u64 az,
                                                           const ModParams *__restrict__ mp, u32 logN, u32 l,
                                                           u32 K, u32 L, u32 alpha, u32 nparts, u32 key_limbs,
@@ -547,7 +546,7 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_rot_f64(const KsRotArgs *__res
         if (i >= start && i < start + sz)
             dw = A.c[((size_t)i << logN) + src];
         else
-            dw = compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
+            dw = A.compact[(((size_t)p * (l + K) + ((i < start) ? i : (i - sz))) << logN) + src];
         const double d = u64_to_f64(dw);
         s0 += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)p * kstride]), q, qinv);
         s1 += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)p * kstride]), q, qinv);
@@ -557,16 +556,15 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_rot_f64(const KsRotArgs *__res
     o[(size_t)(l + K) << logN] = f64_to_canon(reduce_f64(s1, q, qinv), q);
 }
 
-void launch_ks_inner_rot(const DevTables &t, const KsRotArgs *args_dev, const u64 *compact, u64 *acc,
-                         u64 acc_zstride, u32 nbatch, u32 l, u32 alpha, u32 nparts, u32 key_limbs, u32 dnum,
-                         hipStream_t s) {
+void launch_ks_inner_rot(const DevTables &t, const KsRotArgs *args_dev, u64 *acc, u64 acc_zstride, u32 nbatch,
+                         u32 l, u32 alpha, u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s) {
     dim3 grid(t.N / TPB, l + t.K, nbatch);
     if (t.use_f64 && !(t.f64_off & 4) && nparts <= KS_F64_MAX_PARTS)
-        hipLaunchKernelGGL(k_ks_inner_rot_f64, grid, dim3(TPB), 0, s, args_dev, compact, acc, acc_zstride, t.mp,
-                           t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+        hipLaunchKernelGGL(k_ks_inner_rot_f64, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride, t.mp, t.logN, l,
+                           t.K, t.L, alpha, nparts, key_limbs, dnum);
     else
-        hipLaunchKernelGGL(k_ks_inner_rot, grid, dim3(TPB), 0, s, args_dev, compact, acc, acc_zstride, t.mp,
-                           t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+        hipLaunchKernelGGL(k_ks_inner_rot, grid, dim3(TPB), 0, s, args_dev, acc, acc_zstride, t.mp, t.logN, l, t.K,
+                           t.L, alpha, nparts, key_limbs, dnum);
 }
 
 // ---------------------------------------------------------------------------

@@ -138,13 +138,13 @@ void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, cons
 // Hoisted rotations: the digits of one input (compact, eval domain, ModUp done once)
 // read through item z's Galois permutation, times item z's key.
 struct KsRotArgs {
-    const u64 *c;      // the input polynomial c1 (eval), l limbs: the digits' own limbs
-    const u64 *key;    // item's evk base (a then b)
+    const u64 *c;        // the input polynomial c1 (eval), l limbs: the digits' own limbs
+    const u64 *key;      // item's evk base (a then b)
     u64 galois;
+    const u64 *compact;  // the converted digit limbs of that input's ModUp
 };
-void launch_ks_inner_rot(const DevTables &t, const KsRotArgs *args_dev, const u64 *compact, u64 *acc,
-                         u64 acc_zstride, u32 nbatch, u32 l, u32 alpha, u32 nparts, u32 key_limbs, u32 dnum,
-                         hipStream_t s);
+void launch_ks_inner_rot(const DevTables &t, const KsRotArgs *args_dev, u64 *acc, u64 acc_zstride, u32 nbatch,
+                         u32 l, u32 alpha, u32 nparts, u32 key_limbs, u32 dnum, hipStream_t s);
 
 // Batched ciphertext addition: d = a + b (both polynomials), items of `limbs` limbs.
 struct AddArgs {

@@ -498,6 +498,21 @@ def test_hoisted_rotations_parity(orc, q_bits):
                                         [w["evk"] for w in ws], L, [w["d0"] for w in ws], [w["d1"] for w in ws],
                                         L) == 0
     assert np.array_equal(got, pm[8 * W:])
+    # a run where a later input is an earlier output (rotation of a rotation): the
+    # groups then run one after the other
+    base = 8 + 2 * len(works)
+    chain = [dict(kind=7, limbs=L, key_limbs=L, a0=0, a1=W, d0=(base + 2 * k) * W, d1=(base + 2 * k + 1) * W,
+                  galois=gal[k], evk=k * ksz) for k in range(2)]
+    chain += [dict(kind=7, limbs=L, key_limbs=L, a0=base * W, a1=(base + 1) * W, d0=(base + 4 + 2 * k) * W,
+                   d1=(base + 5 + 2 * k) * W, galois=gal[2 + k], evk=(2 + k) * ksz) for k in range(2)]
+    ctx.submit(chain)
+    got = ctx.read_poly(base * W, 8 * W)
+    pm = np.concatenate([pm, np.zeros(8 * W, np.uint64)])
+    for grp in (chain[:2], chain[2:]):
+        assert o.eval_rotations_hoisted(pm, km, grp[0]["a0"], grp[0]["a1"], [w["galois"] for w in grp],
+                                        [w["evk"] for w in grp], L, [w["d0"] for w in grp],
+                                        [w["d1"] for w in grp], L) == 0
+    assert np.array_equal(got, pm[base * W:(base + 8) * W])
 
 
 def test_ntt_splits_and_tuning_knobs(orc):

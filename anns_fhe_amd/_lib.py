@@ -101,6 +101,8 @@ SIGNATURES = {
                                        C.c_uint64, u64p, u64p, C.c_uint32]),
     "fhea_eval_linear_sum_batch": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, u64p, u32p, u64p, u64p, u64p, u64p,
                                              u64p, u64p, C.c_uint32]),
+    "fhea_eval_linear_sum_groups": (C.c_int, [vp, C.c_uint32, C.c_uint32, u32p, u32p, u64p, u64p, u64p, u64p,
+                                              u64p, u64p, u64p, u64p, C.c_uint32]),
     "fhea_timer_start": (C.c_int, [vp]),
     "fhea_timer_stop": (C.c_int, [vp, C.POINTER(C.c_float)]),
     "fhea_keygen_secret": (C.c_int, [vp, C.c_uint64, i8p]),

@@ -1425,54 +1425,57 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
     return FHEA_OK;
 }
 
-// Diagonal-method inner products for a batch of jobs (see fhea.h).  All units of
-// all jobs run in one launch, so every output is checked against every input and
-// every other output (intervals of W words, sorted).
-int fhea_eval_linear_sum_batch(fhea_ctx *c, uint32_t n_jobs, uint32_t R, const uint64_t *ct0,
-                               const uint64_t *ct1, const uint32_t *G, const uint64_t *pt,
-                               const uint64_t *ptb, const uint64_t *bias0, const uint64_t *bias1,
-                               const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
-    CTX_CHECK(c);
-    if (!ct0 || !ct1 || !G || !pt || !out0 || !out1 || n_jobs == 0 || R == 0 || limbs == 0 || limbs > c->h.L)
-        return FHEA_ERR_ARG;
-    if ((bias0 == nullptr) != (bias1 == nullptr)) return FHEA_ERR_ARG;
+// Diagonal-method inner products over groups of queries sharing plaintext blocks
+// (see fhea.h).  All units of all groups run in one launch per unit shape, so
+// every output is checked against every input and every other output (intervals
+// of W words: outputs sorted, every input looked up among them).
+struct LinGroupHost {
+    u32 nq, ng;
+    const u64 *ct0, *ct1, *bias0, *bias1;  // nq*R, nq*R, nq (or null), nq (or null)
+    const u64 *pt, *ptb;                   // ng*R, ng (or null)
+    const u64 *out0, *out1;                // nq*ng each, row-major by query
+};
+
+static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<LinGroupHost> &groups) {
+    if (R == 0 || limbs == 0 || limbs > c->h.L || groups.empty()) return FHEA_ERR_ARG;
     const u64 W = (u64)limbs * c->h.N;
-    size_t sumG = 0, n_units = 0;
-    for (u32 j = 0; j < n_jobs; ++j) {
-        if (G[j] == 0) return FHEA_ERR_ARG;
-        sumG += G[j];
-        n_units += (G[j] + LIN_G - 1) / LIN_G;
-    }
-    if (n_units > 65535) return FHEA_ERR_ARG;
+    const bool f64 = linear_sum_f64(c->dt);
     std::vector<u64> ins, outs;
-    ins.reserve((size_t)n_jobs * 2 * R + sumG * (R + 1) + 2 * n_jobs);
-    outs.reserve(2 * sumG);
+    size_t words = 0, n4 = 0, n8 = 0;
     int st = FHEA_OK;
     auto in = [&](u64 o) {
         if (!st) st = poly_code(c, o, W);
         ins.push_back(o);
     };
-    for (size_t k = 0; k < (size_t)n_jobs * R; ++k) in(ct0[k]);
-    for (size_t k = 0; k < (size_t)n_jobs * R; ++k) in(ct1[k]);
-    for (size_t k = 0; k < sumG * R; ++k) in(pt[k]);
-    for (size_t k = 0; ptb && k < sumG; ++k)
-        if (ptb[k] != FHEA_NONE) in(ptb[k]);
-    for (u32 j = 0; bias0 && j < n_jobs; ++j) {
-        if ((bias0[j] == FHEA_NONE) != (bias1[j] == FHEA_NONE)) return FHEA_ERR_ARG;
-        if (bias0[j] != FHEA_NONE) {
-            in(bias0[j]);
-            in(bias1[j]);
+    for (const LinGroupHost &g : groups) {
+        if (g.nq == 0 || g.ng == 0) return FHEA_ERR_ARG;
+        if ((g.bias0 == nullptr) != (g.bias1 == nullptr)) return FHEA_ERR_ARG;
+        for (size_t k = 0; k < (size_t)g.nq * R; ++k) in(g.ct0[k]);
+        for (size_t k = 0; k < (size_t)g.nq * R; ++k) in(g.ct1[k]);
+        for (u32 k = 0; g.bias0 && k < g.nq; ++k) {
+            if ((g.bias0[k] == FHEA_NONE) != (g.bias1[k] == FHEA_NONE)) return FHEA_ERR_ARG;
+            if (g.bias0[k] != FHEA_NONE) {
+                in(g.bias0[k]);
+                in(g.bias1[k]);
+            }
         }
-    }
-    for (size_t k = 0; k < sumG; ++k) {
-        if (!st) st = poly_code(c, out0[k], W);
-        if (!st) st = poly_code(c, out1[k], W);
-        outs.push_back(out0[k]);
-        outs.push_back(out1[k]);
+        for (size_t k = 0; k < (size_t)g.ng * R; ++k) in(g.pt[k]);
+        for (u32 k = 0; g.ptb && k < g.ng; ++k)
+            if (g.ptb[k] != FHEA_NONE) in(g.ptb[k]);
+        for (size_t k = 0; k < (size_t)g.nq * g.ng; ++k) {
+            if (!st) st = poly_code(c, g.out0[k], W);
+            if (!st) st = poly_code(c, g.out1[k], W);
+            outs.push_back(g.out0[k]);
+            outs.push_back(g.out1[k]);
+        }
+        words += 2 * (size_t)g.nq * (R + 1) + (size_t)g.ng * (R + 1) + 2 * (size_t)g.nq * g.ng;
+        if (f64)
+            n4 += (size_t)((g.nq + LIN_QC - 1) / LIN_QC) * ((g.ng + LIN_GC - 1) / LIN_GC);
+        else
+            n8 += (size_t)g.nq * ((g.ng + 7) / 8);
     }
     if (st) return st;
-    // outputs must not overlap any input (blocks of other units read inputs late)
-    // nor each other: outputs sorted, every input looked up among them
+    if (n4 > 65535 || n8 > 65535) return FHEA_ERR_ARG;
     std::sort(outs.begin(), outs.end());
     for (size_t k = 0; k + 1 < outs.size(); ++k)
         if (outs[k + 1] - outs[k] < W) return FHEA_ERR_ARG;
@@ -1480,34 +1483,81 @@ int fhea_eval_linear_sum_batch(fhea_ctx *c, uint32_t n_jobs, uint32_t R, const u
         auto it = std::lower_bound(outs.begin(), outs.end(), x >= W - 1 ? x - (W - 1) : 0);
         if (it != outs.end() && overlaps(x, *it, W)) return FHEA_ERR_ARG;
     }
-    // staged: per job [ct0 R][ct1 R][pt G*R][ptb G][out0 G][out1 G], then the unit table
-    const size_t off_words = (size_t)n_jobs * 2 * R + sumG * (R + 3);
-    const size_t bytes = off_words * 8 + n_units * sizeof(LinUnit);
+    // staged: every group's offsets, then the LDS-shape units, then the 1x8 units
+    const size_t bytes = words * 8 + (n4 + n8) * sizeof(LinUnit);
     char *h = nullptr, *d = nullptr;
     if ((st = stage_reserve(c, bytes, &h, &d))) return st;
     u64 *hw = (u64 *)h;
-    LinUnit *hu = (LinUnit *)(h + off_words * 8);
-    size_t w = 0, u = 0, gk = 0;
-    for (u32 j = 0; j < n_jobs; ++j) {
-        const u32 g = G[j];
-        const u64 *job_dev = (const u64 *)(d + w * 8);
-        memcpy(hw + w, ct0 + (size_t)j * R, R * 8);
-        memcpy(hw + w + R, ct1 + (size_t)j * R, R * 8);
-        memcpy(hw + w + 2 * R, pt + gk * R, (size_t)g * R * 8);
-        u64 *tail = hw + w + 2 * R + (size_t)g * R;
-        for (u32 k = 0; k < g; ++k) {
-            tail[k] = ptb ? ptb[gk + k] : FHEA_NONE;
-            tail[g + k] = out0[gk + k];
-            tail[2 * g + k] = out1[gk + k];
+    LinUnit *u4 = (LinUnit *)(h + words * 8), *u8 = u4 + n4;
+    size_t w = 0;
+    for (const LinGroupHost &g : groups) {
+        const u64 *dev = (const u64 *)(d + w * 8);
+        const size_t nqR = (size_t)g.nq * R, ngR = (size_t)g.ng * R, nqg = (size_t)g.nq * g.ng;
+        u64 *x = hw + w;
+        memcpy(x, g.ct0, nqR * 8);
+        memcpy(x + nqR, g.ct1, nqR * 8);
+        x += 2 * nqR;
+        for (u32 k = 0; k < g.nq; ++k) {
+            x[k] = g.bias0 ? g.bias0[k] : FHEA_NONE;
+            x[g.nq + k] = g.bias1 ? g.bias1[k] : FHEA_NONE;
         }
-        const u64 b0 = bias0 ? bias0[j] : FHEA_NONE, b1 = bias1 ? bias1[j] : FHEA_NONE;
-        for (u32 g0 = 0; g0 < g; g0 += LIN_G) hu[u++] = LinUnit{job_dev, b0, b1, g, g0};
-        w += 2 * (size_t)R + (size_t)g * (R + 3);
-        gk += g;
+        x += 2 * (size_t)g.nq;
+        memcpy(x, g.pt, ngR * 8);
+        x += ngR;
+        for (u32 k = 0; k < g.ng; ++k) x[k] = g.ptb ? g.ptb[k] : FHEA_NONE;
+        x += g.ng;
+        memcpy(x, g.out0, nqg * 8);
+        memcpy(x + nqg, g.out1, nqg * 8);
+        w += 2 * nqR + 2 * (size_t)g.nq + ngR + g.ng + 2 * nqg;
+        if (f64) {
+            for (u32 q0 = 0; q0 < g.nq; q0 += LIN_QC)
+                for (u32 g0 = 0; g0 < g.ng; g0 += LIN_GC) *u4++ = LinUnit{dev, g.nq, g.ng, q0, g0};
+        } else {
+            for (u32 q0 = 0; q0 < g.nq; ++q0)
+                for (u32 g0 = 0; g0 < g.ng; g0 += 8) *u8++ = LinUnit{dev, g.nq, g.ng, q0, g0};
+        }
     }
     if ((st = stage_commit(c, h, d, bytes))) return st;
-    launch_linear_sum(c->dt, c->poly, (const LinUnit *)(d + off_words * 8), (u32)n_units, R, limbs, c->stream);
+    const LinUnit *d4 = (const LinUnit *)(d + words * 8);
+    launch_linear_sum(c->dt, c->poly, d4, (u32)n4, R, limbs, LIN_SHAPE_LDS, c->stream);
+    launch_linear_sum(c->dt, c->poly, d4 + n4, (u32)n8, R, limbs, LIN_SHAPE_1x8, c->stream);
     return check_hip(hipGetLastError());
+}
+
+int fhea_eval_linear_sum_groups(fhea_ctx *c, uint32_t n_groups, uint32_t R, const uint32_t *nq,
+                                const uint32_t *ng, const uint64_t *ct0, const uint64_t *ct1,
+                                const uint64_t *bias0, const uint64_t *bias1, const uint64_t *pt,
+                                const uint64_t *ptb, const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (!nq || !ng || !ct0 || !ct1 || !pt || !out0 || !out1 || n_groups == 0) return FHEA_ERR_ARG;
+    if ((bias0 == nullptr) != (bias1 == nullptr)) return FHEA_ERR_ARG;
+    std::vector<LinGroupHost> gs(n_groups);
+    size_t aq = 0, ag = 0, ao = 0;
+    for (u32 k = 0; k < n_groups; ++k) {
+        gs[k] = {nq[k], ng[k], ct0 + aq * R, ct1 + aq * R, bias0 ? bias0 + aq : nullptr,
+                 bias1 ? bias1 + aq : nullptr, pt + ag * R, ptb ? ptb + ag : nullptr, out0 + ao, out1 + ao};
+        aq += nq[k];
+        ag += ng[k];
+        ao += (size_t)nq[k] * ng[k];
+    }
+    return linear_sum_groups(c, R, limbs, gs);
+}
+
+int fhea_eval_linear_sum_batch(fhea_ctx *c, uint32_t n_jobs, uint32_t R, const uint64_t *ct0,
+                               const uint64_t *ct1, const uint32_t *G, const uint64_t *pt,
+                               const uint64_t *ptb, const uint64_t *bias0, const uint64_t *bias1,
+                               const uint64_t *out0, const uint64_t *out1, uint32_t limbs) {
+    CTX_CHECK(c);
+    if (!ct0 || !ct1 || !G || !pt || !out0 || !out1 || n_jobs == 0) return FHEA_ERR_ARG;
+    if ((bias0 == nullptr) != (bias1 == nullptr)) return FHEA_ERR_ARG;
+    std::vector<LinGroupHost> gs(n_jobs);
+    size_t ag = 0;
+    for (u32 j = 0; j < n_jobs; ++j) {
+        gs[j] = {1, G[j], ct0 + (size_t)j * R, ct1 + (size_t)j * R, bias0 ? bias0 + j : nullptr,
+                 bias1 ? bias1 + j : nullptr, pt + ag * R, ptb ? ptb + ag : nullptr, out0 + ag, out1 + ag};
+        ag += G[j];
+    }
+    return linear_sum_groups(c, R, limbs, gs);
 }
 
 int fhea_eval_linear_sum(fhea_ctx *c, const uint64_t *ct0, const uint64_t *ct1, uint32_t R,

@@ -586,120 +586,223 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-method inner products: out_g = bias + ptb_g + sum_r ct_r (.) pt_{g,r}.
+// Diagonal-method inner products: out_{q,g} = bias_q + ptb_g + sum_r ct_{q,r} (.) pt_{g,r}.
 // The reference evaluates each candidate batch with its own rotate-and-sum
-// (interactive_query_manager.cpp:190-274); with the query's rotations computed
-// once, every candidate block is one plaintext multiply-accumulate over r, which
-// is this kernel: thread (j, limb) holds LIN_G outputs, streams the R rotated
-// ciphertext words once and the G*R plaintext words once.  Every unit of every
-// job of a batch is one slice of the grid, so one launch fills the GPU.
+// (interactive_query_manager.cpp:190-274); with each query's rotations computed
+// once, every (query, candidate block) pair is one plaintext multiply-accumulate
+// over r.  A group is a set of queries that all need the same blocks (the queries
+// probing one cluster, say); a unit is up to LIN_QC queries x LIN_GC blocks of a
+// group.  FP64 path: a 512-thread workgroup covers 64 coefficient positions of a
+// unit; per r it loads each of the unit's 2*QC + GC words once per position into
+// LDS (double-buffered, the next r's loads in flight during this r's products),
+// and wave w accumulates queries 2w, 2w+1 x all GC blocks.  A word read from HBM
+// thus feeds up to 2*QC*GC/(2*QC+GC) = 6.4 products.  Integer path (primes of 50
+// bits and more): one query x 8 blocks per thread.  Every unit of a call is one
+// slice of the grid, so one launch fills the GPU.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ u64 lin_word(const u64 *P, u64 off, size_t idx) {
     return off == LIN_NONE ? 0 : P[off + idx];
 }
 
+struct LinView {
+    const u64 *ct0, *ct1, *b0, *b1, *pt, *ptb, *o0, *o1;
+    __device__ LinView(const LinUnit &U, u32 R) {
+        ct0 = U.offs;
+        ct1 = ct0 + (size_t)U.nq * R;
+        b0 = ct1 + (size_t)U.nq * R;
+        b1 = b0 + U.nq;
+        pt = b1 + U.nq;
+        ptb = pt + (size_t)U.ng * R;
+        o0 = ptb + U.ng;
+        o1 = o0 + (size_t)U.nq * U.ng;
+    }
+};
+
 // Integer form: 128-bit lazy sums, reduced every 32 products (< 2^127).
+template <int QG, int GG>
 __global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const LinUnit *__restrict__ units, u32 R,
                                                     const ModParams *__restrict__ mp, u32 logN) {
     const LinUnit U = units[blockIdx.z];
-    const u64 *offs = U.offs;
-    const u32 G = U.G, g0 = U.g0;
-    const u64 bias0 = U.bias0, bias1 = U.bias1;
+    const LinView V(U, R);
     const u32 limb = blockIdx.y;
     const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
     const ModParams m = mp[limb];
-    const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
-    const u64 *out0 = ptb + G, *out1 = out0 + G;
-    const u64 b0 = lin_word(P, bias0, idx), b1 = lin_word(P, bias1, idx);
-    u128 a0[LIN_G], a1[LIN_G];
+    u128 a0[QG][GG], a1[QG][GG];
 #pragma unroll
-    for (u32 g = 0; g < LIN_G; ++g) {
-        a0[g] = (u128)b0 + (g0 + g < G ? lin_word(P, ptb[g0 + g], idx) : 0);
-        a1[g] = b1;
+    for (int i = 0; i < QG; ++i) {
+        const bool qv = U.q0 + i < U.nq;
+        const u64 b0 = qv ? lin_word(P, V.b0[U.q0 + i], idx) : 0, b1 = qv ? lin_word(P, V.b1[U.q0 + i], idx) : 0;
+#pragma unroll
+        for (int k = 0; k < GG; ++k) {
+            a0[i][k] = (u128)b0 + (U.g0 + k < U.ng ? lin_word(P, V.ptb[U.g0 + k], idx) : 0);
+            a1[i][k] = b1;
+        }
     }
     for (u32 r = 0; r < R; ++r) {
-        const u64 x0 = P[ct0[r] + idx], x1 = P[ct1[r] + idx];
+        u64 x0[QG], x1[QG], p[GG];
 #pragma unroll
-        for (u32 g = 0; g < LIN_G; ++g) {
-            if (g0 + g < G) {
-                const u64 p = P[pt[(size_t)(g0 + g) * R + r] + idx];
-                a0[g] += (u128)x0 * p;
-                a1[g] += (u128)x1 * p;
-            }
+        for (int i = 0; i < QG; ++i) {
+            const bool qv = U.q0 + i < U.nq;
+            x0[i] = qv ? P[V.ct0[(size_t)(U.q0 + i) * R + r] + idx] : 0;
+            x1[i] = qv ? P[V.ct1[(size_t)(U.q0 + i) * R + r] + idx] : 0;
         }
+#pragma unroll
+        for (int k = 0; k < GG; ++k) p[k] = U.g0 + k < U.ng ? P[V.pt[(size_t)(U.g0 + k) * R + r] + idx] : 0;
+#pragma unroll
+        for (int i = 0; i < QG; ++i)
+#pragma unroll
+            for (int k = 0; k < GG; ++k) {
+                a0[i][k] += (u128)x0[i] * p[k];
+                a1[i][k] += (u128)x1[i] * p[k];
+            }
         if ((r & 31) == 31) {
 #pragma unroll
-            for (u32 g = 0; g < LIN_G; ++g) {
-                a0[g] = reduce128(a0[g], m);
-                a1[g] = reduce128(a1[g], m);
-            }
+            for (int i = 0; i < QG; ++i)
+#pragma unroll
+                for (int k = 0; k < GG; ++k) {
+                    a0[i][k] = reduce128(a0[i][k], m);
+                    a1[i][k] = reduce128(a1[i][k], m);
+                }
         }
     }
 #pragma unroll
-    for (u32 g = 0; g < LIN_G; ++g) {
-        if (g0 + g < G) {
-            P[out0[g0 + g] + idx] = reduce128(a0[g], m);
-            P[out1[g0 + g] + idx] = reduce128(a1[g], m);
-        }
-    }
+    for (int i = 0; i < QG; ++i)
+#pragma unroll
+        for (int k = 0; k < GG; ++k)
+            if (U.q0 + i < U.nq && U.g0 + k < U.ng) {
+                const size_t o = (size_t)(U.q0 + i) * U.ng + U.g0 + k;
+                P[V.o0[o] + idx] = reduce128(a0[i][k], m);
+                P[V.o1[o] + idx] = reduce128(a1[i][k], m);
+            }
 }
 
 // FP64 form (primes < 2^50): mulmod_vv_f64 terms (<= 0.875q), sums centred every
 // 8 products (<= q/2 + 2 + 7q).
-__global__ __launch_bounds__(TPB) void k_linear_sum_f64(u64 *__restrict__ P, const LinUnit *__restrict__ units,
-                                                        u32 R, const ModParams *__restrict__ mp, u32 logN) {
+static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NS = 2 * LIN_QC + LIN_GC;
+
+__global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict__ P,
+                                                               const LinUnit *__restrict__ units, u32 R,
+                                                               const ModParams *__restrict__ mp, u32 logN) {
+    __shared__ double buf[2][LIN_NS][LIN_POS];
     const LinUnit U = units[blockIdx.z];
-    const u64 *offs = U.offs;
-    const u32 G = U.G, g0 = U.g0;
-    const u64 bias0 = U.bias0, bias1 = U.bias1;
-    const u32 limb = blockIdx.y;
-    const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
+    const LinView V(U, R);
+    const u32 limb = blockIdx.y, lane = threadIdx.x % LIN_POS, w = threadIdx.x / LIN_POS;
+    const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * LIN_POS + lane;
     const double q = mp[limb].qd, qinv = mp[limb].qinv;
-    const u64 *ct0 = offs, *ct1 = offs + R, *pt = offs + 2 * R, *ptb = pt + (size_t)G * R;
-    const u64 *out0 = ptb + G, *out1 = out0 + G;
-    const double b0 = u64_to_f64(lin_word(P, bias0, idx)), b1 = u64_to_f64(lin_word(P, bias1, idx));
-    double a0[LIN_G], a1[LIN_G];
+    const u32 nq = min(LIN_QC, U.nq - U.q0), ng = min(LIN_GC, U.ng - U.g0);
+    // streams: 2i (ct0 of query i), 2i+1 (ct1 of query i), 2*QC + k (block k);
+    // thread (w, lane) moves streams w, w + 8, ... of its position
+    constexpr u32 NW = LIN_TPB / LIN_POS, PER = (LIN_NS + NW - 1) / NW;
+    const u64 *sbase[PER];
+    u32 sstride[PER];
+    bool sok[PER];
 #pragma unroll
-    for (u32 g = 0; g < LIN_G; ++g) {
-        const double pb = g0 + g < G ? u64_to_f64(lin_word(P, ptb[g0 + g], idx)) : 0.0;
-        a0[g] = reduce_f64(b0 + pb, q, qinv);
-        a1[g] = reduce_f64(b1, q, qinv);
+    for (u32 k = 0; k < PER; ++k) {
+        const u32 j = w + k * NW;
+        if (j < 2 * LIN_QC) {
+            sok[k] = j / 2 < nq;
+            sbase[k] = ((j & 1) ? V.ct1 : V.ct0) + (size_t)(U.q0 + j / 2) * R;
+        } else {
+            sok[k] = j < LIN_NS && j - 2 * LIN_QC < ng;
+            sbase[k] = V.pt + (size_t)(U.g0 + (j < LIN_NS ? j : 2 * LIN_QC) - 2 * LIN_QC) * R;
+        }
+        sstride[k] = j;
     }
-    for (u32 r = 0; r < R; ++r) {
-        const double x0 = u64_to_f64(P[ct0[r] + idx]), x1 = u64_to_f64(P[ct1[r] + idx]);
+    u64 ld[PER];
+    // this wave's pairs: queries 2w, 2w+1 x blocks 0..GC-1
+    const u32 qa = 2 * w, qb = 2 * w + 1;
+    const bool va = qa < nq, vb = qb < nq;
+    double a0[2][LIN_GC], a1[2][LIN_GC];
 #pragma unroll
-        for (u32 g = 0; g < LIN_G; ++g) {
-            if (g0 + g < G) {
-                const double p = u64_to_f64(P[pt[(size_t)(g0 + g) * R + r] + idx]);
-                a0[g] += mulmod_vv_f64(x0, p, q, qinv);
-                a1[g] += mulmod_vv_f64(x1, p, q, qinv);
+    for (u32 i = 0; i < 2; ++i) {
+        const u32 qq = U.q0 + 2 * w + i;
+        const bool v = (2 * w + i) < nq;
+        const double b0 = v ? u64_to_f64(lin_word(P, V.b0[qq], idx)) : 0.0;
+        const double b1 = v ? u64_to_f64(lin_word(P, V.b1[qq], idx)) : 0.0;
+#pragma unroll
+        for (u32 k = 0; k < LIN_GC; ++k) {
+            const double pb = (v && k < ng) ? u64_to_f64(lin_word(P, V.ptb[U.g0 + k], idx)) : 0.0;
+            a0[i][k] = reduce_f64(b0 + pb, q, qinv);
+            a1[i][k] = b1;
+        }
+    }
+#define LIN_ISSUE(rr)                                                          \
+    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) ld[k] = P[sbase[k][rr] + idx];
+#define LIN_DEPOSIT(b)                                                         \
+    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][sstride[k]][lane] = u64_to_f64(ld[k]);
+    LIN_ISSUE(0)
+    LIN_DEPOSIT(0)
+    __syncthreads();
+    for (u32 r = 0; r < R; ++r) {
+        const u32 cur = r & 1;
+        if (r + 1 < R) {
+            LIN_ISSUE(r + 1)
+        }
+        if (va) {
+            const double x0 = buf[cur][2 * qa][lane], x1 = buf[cur][2 * qa + 1][lane];
+#pragma unroll
+            for (u32 k = 0; k < LIN_GC; ++k) {
+                if (k < ng) {
+                    const double p = buf[cur][2 * LIN_QC + k][lane];
+                    a0[0][k] += mulmod_vv_f64(x0, p, q, qinv);
+                    a1[0][k] += mulmod_vv_f64(x1, p, q, qinv);
+                }
+            }
+        }
+        if (vb) {
+            const double x0 = buf[cur][2 * qb][lane], x1 = buf[cur][2 * qb + 1][lane];
+#pragma unroll
+            for (u32 k = 0; k < LIN_GC; ++k) {
+                if (k < ng) {
+                    const double p = buf[cur][2 * LIN_QC + k][lane];
+                    a0[1][k] += mulmod_vv_f64(x0, p, q, qinv);
+                    a1[1][k] += mulmod_vv_f64(x1, p, q, qinv);
+                }
             }
         }
         if ((r & 7) == 7) {
 #pragma unroll
-            for (u32 g = 0; g < LIN_G; ++g) {
-                a0[g] = reduce_f64(a0[g], q, qinv);
-                a1[g] = reduce_f64(a1[g], q, qinv);
-            }
-        }
-    }
+            for (u32 i = 0; i < 2; ++i)
 #pragma unroll
-    for (u32 g = 0; g < LIN_G; ++g) {
-        if (g0 + g < G) {
-            P[out0[g0 + g] + idx] = f64_to_canon(reduce_f64(a0[g], q, qinv), q);
-            P[out1[g0 + g] + idx] = f64_to_canon(reduce_f64(a1[g], q, qinv), q);
+                for (u32 k = 0; k < LIN_GC; ++k) {
+                    a0[i][k] = reduce_f64(a0[i][k], q, qinv);
+                    a1[i][k] = reduce_f64(a1[i][k], q, qinv);
+                }
         }
+        // the other buffer was last read in iteration r-1, before its closing barrier
+        if (r + 1 < R) {
+            LIN_DEPOSIT(cur ^ 1)
+        }
+        __syncthreads();
+    }
+#undef LIN_ISSUE
+#undef LIN_DEPOSIT
+#pragma unroll
+    for (u32 i = 0; i < 2; ++i) {
+        if ((2 * w + i) >= nq) continue;
+#pragma unroll
+        for (u32 k = 0; k < LIN_GC; ++k)
+            if (k < ng) {
+                const size_t o = (size_t)(U.q0 + 2 * w + i) * U.ng + U.g0 + k;
+                P[V.o0[o] + idx] = f64_to_canon(reduce_f64(a0[i][k], q, qinv), q);
+                P[V.o1[o] + idx] = f64_to_canon(reduce_f64(a1[i][k], q, qinv), q);
+            }
     }
 }
 
+bool linear_sum_f64(const DevTables &t) { return t.use_f64 && !(t.f64_off & 8); }
+
 void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,
-                       hipStream_t s) {
-    dim3 grid(t.N / TPB, limbs, n_units);
-    if (t.use_f64 && !(t.f64_off & 8))
-        hipLaunchKernelGGL(k_linear_sum_f64, grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);
-    else
-        hipLaunchKernelGGL(k_linear_sum, grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);
+                       int shape, hipStream_t s) {
+    if (!n_units) return;
+    if (shape == LIN_SHAPE_LDS) {
+        dim3 grid(t.N / LIN_POS, limbs, n_units);
+        hipLaunchKernelGGL(k_linear_sum_lds_f64, grid, dim3(LIN_TPB), 0, s, P, units_dev, R, t.mp, t.logN);
+    } else {
+        dim3 grid(t.N / TPB, limbs, n_units);
+        hipLaunchKernelGGL((k_linear_sum<1, 8>), grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);
+    }
 }
 
 }  // namespace fhea

@@ -153,21 +153,24 @@ struct AddArgs {
 };
 void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u32 limbs, hipStream_t s);
 
-// Plaintext-matrix x encrypted-vector inner products (diagonal method), for a
-// batch of jobs, each one ciphertext set and G plaintext blocks:
-//   out_g = bias + ptb_g + sum_{r<R} ct_r (.) pt_{g,r}
-// A job's offs (device, u64 word offsets into P):
-//   [ct0: R][ct1: R][pt: G*R][ptb: G][out0: G][out1: G];
-// ptb entries and bias0/bias1 may be LIN_NONE.  A unit is LIN_G consecutive
-// blocks g0.. of one job (one grid slice).
+// Plaintext-matrix x encrypted-vector inner products (diagonal method) for groups
+// of queries sharing their plaintext blocks:
+//   out_{q,g} = bias_q + ptb_g + sum_{r<R} ct_{q,r} (.) pt_{g,r}
+// A group's offs (device, u64 word offsets into P), nq queries and ng blocks:
+//   [ct0: nq*R][ct1: nq*R][bias0: nq][bias1: nq][pt: ng*R][ptb: ng][out0: nq*ng][out1: nq*ng]
+// (outputs row-major by query); ptb and bias entries may be LIN_NONE.  A unit is
+// the queries q0.. and blocks g0.. of one group that one grid slice evaluates:
+// up to LIN_QC x LIN_GC (FP64, LIN_SHAPE_LDS) or 1 x 8 (integer, LIN_SHAPE_1x8).
 static constexpr u64 LIN_NONE = ~0ull;
-static constexpr u32 LIN_G = 8;
+static constexpr u32 LIN_QC = 16, LIN_GC = 8;
 struct LinUnit {
     const u64 *offs;
-    u64 bias0, bias1;
-    u32 G, g0;
+    u32 nq, ng, q0, g0;
 };
+enum : int { LIN_SHAPE_1x8 = 0, LIN_SHAPE_LDS = 1 };
+// true when the FP64 kernels run (the LDS shape exists only there)
+bool linear_sum_f64(const DevTables &t);
 void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,
-                       hipStream_t s);
+                       int shape, hipStream_t s);
 
 }  // namespace fhea

@@ -180,6 +180,31 @@ class FheContext:
             _p64(pb) if pb is not None else nil, _p64(b0) if b0 is not None else nil,
             _p64(b1) if b1 is not None else nil, _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum_batch")
 
+    def linear_sum_groups(self, groups, limbs):
+        """One launch per unit shape for groups of queries sharing blocks.  groups: list
+        of dicts with ct0, ct1 (nq x R offsets), pt (ng x R), out0, out1 (nq x ng) and
+        optional bias (nq pairs) and ptb (ng)."""
+        R = np.asarray(groups[0]["ct0"]).shape[-1]
+        nq = np.array([np.asarray(g["ct0"]).reshape(-1, R).shape[0] for g in groups], np.uint32)
+        ng = np.array([np.asarray(g["pt"]).reshape(-1, R).shape[0] for g in groups], np.uint32)
+        cat = lambda key: np.ascontiguousarray(np.concatenate([np.asarray(g[key], np.uint64).reshape(-1)
+                                                               for g in groups]))
+        c0, c1, pt, o0, o1 = cat("ct0"), cat("ct1"), cat("pt"), cat("out0"), cat("out1")
+        assert c0.size == R * nq.sum() and pt.size == R * ng.sum() and o0.size == (nq.astype(np.int64) * ng).sum()
+        nil = C.cast(None, _lib.u64p)
+        pb = b0 = b1 = None
+        if any(g.get("ptb") is not None for g in groups):
+            pb = np.concatenate([np.asarray(g["ptb"], np.uint64) if g.get("ptb") is not None
+                                 else np.full(n, _lib.NONE, np.uint64) for g, n in zip(groups, ng)])
+        if any(g.get("bias") is not None for g in groups):
+            bb = np.concatenate([np.asarray(g["bias"], np.uint64).reshape(-1, 2) if g.get("bias") is not None
+                                 else np.full((n, 2), _lib.NONE, np.uint64) for g, n in zip(groups, nq)])
+            b0, b1 = np.ascontiguousarray(bb[:, 0]), np.ascontiguousarray(bb[:, 1])
+        check(self.lib.fhea_eval_linear_sum_groups(
+            self.h, len(groups), R, nq.ctypes.data_as(_lib.u32p), ng.ctypes.data_as(_lib.u32p), _p64(c0), _p64(c1),
+            _p64(b0) if b0 is not None else nil, _p64(b1) if b1 is not None else nil, _p64(pt),
+            _p64(pb) if pb is not None else nil, _p64(o0), _p64(o1), limbs), "fhea_eval_linear_sum_groups")
+
     def rescale_batch(self, src, dst, n, limbs):
         check(self.lib.fhea_rescale_batch(self.h, src, dst, n, limbs), "fhea_rescale_batch")
 

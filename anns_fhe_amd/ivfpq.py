@@ -327,28 +327,39 @@ class FheIvfPqServer:
         self.distances_batch([(z, block_ids)])
 
     def distances_batch(self, jobs):
-        """Encrypted squared distances for several query slots in one launch: jobs is a
-        list of (z, block_ids).  Slot z's rescaled ciphertexts (lt-1 limbs) are left
-        at o['res'] + g*2*(lt-1)*N."""
+        """Encrypted squared distances, jobs = [(z, block_ids)]: slot z's rescaled
+        ciphertexts (lt-1 limbs) are left at o['res'] + g*2*(lt-1)*N."""
+        self._distances_groups([([z], list(b), [0]) for z, b in jobs], {z: len(b) for z, b in jobs})
+
+    def _distances_groups(self, groups, sizes):
+        """groups: [(slots, block_ids, starts)] - every slot of a group needs every block
+        of it, starts[i] = index of block_ids[0] in the result list of slots[i];
+        sizes: {z: number of results}.  One launch per unit shape for all groups."""
         N, lt, D = self.N, self.lt, self.D
         Wl = lt * N
-        lin, res, k = [], [], 0
-        rot = np.arange(1, D, dtype=np.uint64) - 1
-        for z, block_ids in jobs:
-            G = len(block_ids)
+        first, k = {}, 0
+        for z, G in sizes.items():
             assert 0 < G <= self.max_g
-            o = self._offs(z)
-            b = np.asarray(block_ids, np.uint64) * self.block_words
-            out0 = self.out_base + (k + np.arange(G, dtype=np.uint64)) * (2 * Wl)
-            ct0 = np.concatenate([[o["q0"]], o["rot"] + rot * (2 * Wl)]).astype(np.uint64)
-            lin.append(dict(ct0=ct0, ct1=np.concatenate([[o["q1"]], o["rot"] + rot * (2 * Wl) + Wl]).astype(np.uint64),
-                            pt=b[:, None] + np.arange(D, dtype=np.uint64)[None, :] * Wl, ptb=b + D * Wl,
-                            out0=out0, out1=out0 + Wl, bias=(o["norm"], o["norm"] + Wl)))
-            res.append(o["res"] + np.arange(G, dtype=np.uint64) * (2 * (lt - 1) * N))
+            first[z] = k
             k += G
-        res = np.concatenate(res)
         assert k <= self.max_queries * self.max_g
-        self.ctx.linear_sum_batch(lin, lt)
+        rot = (np.arange(1, D, dtype=np.uint64) - 1) * (2 * Wl)
+        lin = []
+        for zs, block_ids, starts in groups:
+            os_ = [self._offs(z) for z in zs]
+            b = np.asarray(block_ids, np.uint64) * self.block_words
+            ng = len(block_ids)
+            out0 = np.array([[self.out_base + (first[z] + s0 + g) * 2 * Wl for g in range(ng)]
+                             for z, s0 in zip(zs, starts)], np.uint64)
+            lin.append(dict(
+                ct0=np.array([np.concatenate([[o["q0"]], o["rot"] + rot]) for o in os_], np.uint64),
+                ct1=np.array([np.concatenate([[o["q1"]], o["rot"] + rot + Wl]) for o in os_], np.uint64),
+                bias=[(o["norm"], o["norm"] + Wl) for o in os_],
+                pt=b[:, None] + np.arange(D, dtype=np.uint64)[None, :] * Wl, ptb=b + D * Wl,
+                out0=out0, out1=out0 + Wl))
+        self.ctx.linear_sum_groups(lin, lt)
+        res = np.concatenate([self._offs(z)["res"] + np.arange(G, dtype=np.uint64) * (2 * (lt - 1) * N)
+                              for z, G in sizes.items()])
         self.ctx.rescale_scatter(self.out_base, res, lt)
 
     def read_distances_batch(self, items):
@@ -362,9 +373,12 @@ class FheIvfPqServer:
         ranges = [(self._offs(z)["res"], G * n) for z, G in items]
         total = sum(c for _, c in ranges)
         if self._pinned_all is None or self._pinned_all.words < total:
+            # page-locking is slow: grow by half again each time
+            size = max(total, self.max_queries * 4 * n)
             if self._pinned_all is not None:
+                size = max(size, self._pinned_all.words * 3 // 2)
                 self._pinned_all.close()
-            self._pinned_all = PinnedArray(max(total, self.max_queries * n))
+            self._pinned_all = PinnedArray(size)
         buf = self.ctx.read_polys(ranges, out=self._pinned_all.array)
         at, parts = 0, []
         for z, G in items:
@@ -394,14 +408,15 @@ class FheIvfPqServer:
         if gid is not None:
             self.ctx.graph_launch(gid)
             return
+        coarse = ([(list(slots), [0], [0] * len(slots))], {z: 1 for z in slots})
         self.prepare(slots)
-        self.distances_batch([(z, [0]) for z in slots])
+        self._distances_groups(*coarse)
         if self.use_graphs and len(self.graphs) < self.max_graphs:
             self.ctx.sync()
             self.ctx.graph_begin()
             try:
                 self.prepare(slots)
-                self.distances_batch([(z, [0]) for z in slots])
+                self._distances_groups(*coarse)
             finally:
                 self.graphs[key] = self.ctx.graph_end()
 
@@ -412,9 +427,19 @@ class FheIvfPqServer:
         """Candidate distances of several queries in one launch: probes is a list of
         (z, clusters); returns {z: candidate block ids}."""
         blocks = {z: [b for c in clusters for b in self.cluster_blocks[c]] for z, clusters in probes}
-        jobs = [(z, b) for z, b in blocks.items() if b]
-        if jobs:
-            self.distances_batch(jobs)
+        # one group per probed cluster: the queries probing it x its blocks
+        members = {}
+        for z, clusters in probes:
+            s0 = 0
+            for c in clusters:
+                if self.cluster_blocks[c]:
+                    members.setdefault(c, ([], []))
+                    members[c][0].append(z)
+                    members[c][1].append(s0)
+                s0 += len(self.cluster_blocks[c])
+        groups = [(zs, self.cluster_blocks[c], st) for c, (zs, st) in members.items()]
+        if groups:
+            self._distances_groups(groups, {z: len(b) for z, b in blocks.items() if b})
         return blocks
 
 

@@ -245,6 +245,18 @@ int fhea_eval_linear_sum_batch(fhea_ctx *ctx, uint32_t n_jobs, uint32_t R, const
                                const uint64_t *ct1, const uint32_t *G, const uint64_t *pt,
                                const uint64_t *ptb, const uint64_t *bias0, const uint64_t *bias1,
                                const uint64_t *out0, const uint64_t *out1, uint32_t limbs);
+/* Groups of queries that share their plaintext blocks (for example the queries
+ * probing one cluster): group k has nq[k] ciphertext sets (ct0/ct1: nq[k]*R
+ * offsets, bias0/bias1: nq[k]) and ng[k] blocks (pt: ng[k]*R, ptb: ng[k]), and
+ * computes every (query, block) pair, outputs out0/out1[q*ng[k] + g]; the groups'
+ * arrays follow one another.  The FP64 path evaluates up to 16 queries x 8 blocks
+ * per workgroup, so each ciphertext and plaintext word read serves several pairs.
+ * Overlap rules as for fhea_eval_linear_sum_batch.  No reference counterpart. */
+int fhea_eval_linear_sum_groups(fhea_ctx *ctx, uint32_t n_groups, uint32_t R, const uint32_t *nq,
+                                const uint32_t *ng, const uint64_t *ct0, const uint64_t *ct1,
+                                const uint64_t *bias0, const uint64_t *bias1, const uint64_t *pt,
+                                const uint64_t *ptb, const uint64_t *out0, const uint64_t *out1,
+                                uint32_t limbs);
 
 /* ---- timing on the context's stream (HIP events) ----------------------- */
 int fhea_timer_start(fhea_ctx *ctx);

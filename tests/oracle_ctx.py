@@ -87,6 +87,19 @@ class OracleContext:
             self.linear_sum(j["ct0"], j["ct1"], j["pt"], j["out0"], j["out1"], limbs, ptb=j.get("ptb"),
                             bias=j.get("bias"))
 
+    def linear_sum_groups(self, groups, limbs):
+        for g in groups:
+            R = np.asarray(g["ct0"]).shape[-1]
+            c0 = np.asarray(g["ct0"], np.uint64).reshape(-1, R)
+            c1 = np.asarray(g["ct1"], np.uint64).reshape(-1, R)
+            pt = np.asarray(g["pt"], np.uint64).reshape(-1, R)
+            o0 = np.asarray(g["out0"], np.uint64).reshape(c0.shape[0], pt.shape[0])
+            o1 = np.asarray(g["out1"], np.uint64).reshape(c0.shape[0], pt.shape[0])
+            bias = g.get("bias")
+            for i in range(c0.shape[0]):
+                self.linear_sum(c0[i], c1[i], pt, o0[i], o1[i], limbs, ptb=g.get("ptb"),
+                                bias=tuple(np.asarray(bias, np.uint64).reshape(-1, 2)[i]) if bias is not None else None)
+
     def rescale_batch(self, src, dst, n, limbs):
         W1 = (limbs - 1) * self.N
         self.rescale_scatter(src, [dst + z * 2 * W1 for z in range(n)], limbs)

@@ -643,3 +643,42 @@ def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
     with pytest.raises(FheError) as ei:
         ctx.rescale_scatter(n_in * W, [dst[0], dst[0] + 8], L)
     assert ei.value.status == -4
+
+
+@pytest.mark.parametrize("q_bits", [50, 60])
+def test_linear_sum_groups_parity(orc, q_bits):
+    """fhea_eval_linear_sum_groups: groups of 5x6, 1x3 and 4x4 (query x block),
+    queries and blocks shared between groups, partial 4x4 units; every pair equals
+    the oracle's linear sum (FP64 4x4 units at 50-bit primes, integer 1x8 units at 60)."""
+    from anns_fhe_amd._lib import NONE
+    N, L = 4096, 3
+    ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(70 + q_bits)
+    W = L * N
+    R, n_q, n_b = 7, 6, 7
+    n_in = n_q * 2 * R + n_b * R + n_b + 2 * n_q
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(n_in)])
+    ctx.write_poly(0, data)
+    ctq = lambda z, c: [(z * 2 * R + 2 * r + c) * W for r in range(R)]
+    blk = lambda b: [(n_q * 2 * R + b * R + r) * W for r in range(R)]
+    ptb = lambda b: (n_q * 2 * R + n_b * R + b) * W if b % 3 else NONE
+    bias = lambda z: ((n_q * 2 * R + n_b * R + n_b + 2 * z) * W, (n_q * 2 * R + n_b * R + n_b + 2 * z + 1) * W)
+    out = n_in
+    groups, pairs = [], []
+    for zs, bs in (([0, 1, 2, 3, 4], [0, 1, 2, 3, 4, 5]), ([5], [6, 0, 2]), ([2, 3, 5, 1], [3, 4, 5, 6])):
+        o0 = np.array([[(out + 2 * (i * len(bs) + g)) * W for g in range(len(bs))] for i in range(len(zs))], np.uint64)
+        groups.append(dict(ct0=[ctq(z, 0) for z in zs], ct1=[ctq(z, 1) for z in zs], pt=[blk(b) for b in bs],
+                           ptb=[ptb(b) for b in bs], bias=[bias(z) if z != 4 else (NONE, NONE) for z in zs],
+                           out0=o0, out1=o0 + W))
+        pairs += [(z, bs, o0[i], o0[i] + W) for i, z in enumerate(zs)]
+        out += 2 * len(zs) * len(bs)
+    say("linear sum groups", q_bits)
+    ctx.linear_sum_groups(groups, L)
+    got = ctx.read_poly(n_in * W, (out - n_in) * W)
+    pm = np.zeros(out * W, np.uint64)
+    pm[:data.size] = data
+    for z, bs, o0, o1 in pairs:
+        assert o.linear_sum(pm, ctq(z, 0), ctq(z, 1), np.array([blk(b) for b in bs], np.uint64), o0, o1, L,
+                            ptb=[ptb(b) for b in bs], bias=bias(z) if z != 4 else None) == 0
+    assert np.array_equal(got, pm[n_in * W:out * W])

@@ -1429,6 +1429,22 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
 // (see fhea.h).  All units of all groups run in one launch per unit shape, so
 // every output is checked against every input and every other output (intervals
 // of W words: outputs sorted, every input looked up among them).
+// Unit shape for a group of nq queries x ng blocks (FP64 kernel): qa tiles of 2
+// queries stacked, 8/qa tiles of 8 blocks side by side (16 x 8, 8 x 16 or 4 x 32),
+// the one needing the fewest units (ties: the taller).
+static u32 lin_qa(u32 nq, u32 ng) {
+    u32 best = 8;
+    u64 best_units = ~0ull;
+    for (u32 qa : {8u, 4u, 2u}) {
+        const u64 units = (u64)((nq + 2 * qa - 1) / (2 * qa)) * ((ng + 64 / qa - 1) / (64 / qa));
+        if (units < best_units) {
+            best_units = units;
+            best = qa;
+        }
+    }
+    return best;
+}
+
 struct LinGroupHost {
     u32 nq, ng;
     const u64 *ct0, *ct1, *bias0, *bias1;  // nq*R, nq*R, nq (or null), nq (or null)
@@ -1469,10 +1485,12 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
             outs.push_back(g.out1[k]);
         }
         words += 2 * (size_t)g.nq * (R + 1) + (size_t)g.ng * (R + 1) + 2 * (size_t)g.nq * g.ng;
-        if (f64)
-            n4 += (size_t)((g.nq + LIN_QC - 1) / LIN_QC) * ((g.ng + LIN_GC - 1) / LIN_GC);
-        else
+        if (f64) {
+            const u32 qa = lin_qa(g.nq, g.ng), uq = 2 * qa, ug = 64 / qa;
+            n4 += (size_t)((g.nq + uq - 1) / uq) * ((g.ng + ug - 1) / ug);
+        } else {
             n8 += (size_t)g.nq * ((g.ng + 7) / 8);
+        }
     }
     if (st) return st;
     if (n4 > 65535 || n8 > 65535) return FHEA_ERR_ARG;
@@ -1510,11 +1528,15 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
         memcpy(x + nqg, g.out1, nqg * 8);
         w += 2 * nqR + 2 * (size_t)g.nq + ngR + g.ng + 2 * nqg;
         if (f64) {
-            for (u32 q0 = 0; q0 < g.nq; q0 += LIN_QC)
-                for (u32 g0 = 0; g0 < g.ng; g0 += LIN_GC) *u4++ = LinUnit{dev, g.nq, g.ng, q0, g0};
+            const u32 qa = lin_qa(g.nq, g.ng), uq = 2 * qa, ug = 64 / qa;
+            for (u32 q0 = 0; q0 < g.nq; q0 += uq)
+                for (u32 g0 = 0; g0 < g.ng; g0 += ug)
+                    *u4++ = LinUnit{dev, g.nq, g.ng, q0, g0, std::min(uq, g.nq - q0), std::min(ug, g.ng - g0), qa,
+                                    0};
         } else {
             for (u32 q0 = 0; q0 < g.nq; ++q0)
-                for (u32 g0 = 0; g0 < g.ng; g0 += 8) *u8++ = LinUnit{dev, g.nq, g.ng, q0, g0};
+                for (u32 g0 = 0; g0 < g.ng; g0 += 8)
+                    *u8++ = LinUnit{dev, g.nq, g.ng, q0, g0, 1, std::min(8u, g.ng - g0), 1, 0};
         }
     }
     if ((st = stage_commit(c, h, d, bytes))) return st;

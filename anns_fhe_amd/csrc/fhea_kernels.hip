@@ -591,13 +591,12 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 // (interactive_query_manager.cpp:190-274); with each query's rotations computed
 // once, every (query, candidate block) pair is one plaintext multiply-accumulate
 // over r.  A group is a set of queries that all need the same blocks (the queries
-// probing one cluster, say); a unit is up to LIN_QC queries x LIN_GC blocks of a
-// group.  FP64 path: a 512-thread workgroup covers 64 coefficient positions of a
-// unit; per r it loads each of the unit's 2*QC + GC words once per position into
-// LDS (double-buffered, the next r's loads in flight during this r's products),
-// and wave w accumulates queries 2w, 2w+1 x all GC blocks.  A word read from HBM
-// thus feeds up to 2*QC*GC/(2*QC+GC) = 6.4 products.  Integer path (primes of 50
-// bits and more): one query x 8 blocks per thread.  Every unit of a call is one
+// probing one cluster, say); a unit is a rectangle of a group's queries x blocks.  FP64 path: a 512-thread workgroup covers 64 coefficient positions of a
+// unit of up to 128 pairs; per r it loads each of the unit's 2*uq + ug words once
+// per position into LDS (double-buffered, the next r's loads in flight during this
+// r's products) and each wave accumulates 16 pairs, so a word read from HBM feeds
+// 2*uq*ug/(2*uq + ug) products (6.9 for 9 queries x 14 blocks).  Integer path
+// (primes of 50 bits and more): one query x 8 blocks per thread.  Every unit of a call is one
 // slice of the grid, so one launch fills the GPU.
 // ---------------------------------------------------------------------------
 
@@ -678,8 +677,10 @@ __global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const L
 }
 
 // FP64 form (primes < 2^50): mulmod_vv_f64 terms (<= 0.875q), sums centred every
-// 8 products (<= q/2 + 2 + 7q).
-static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NS = 2 * LIN_QC + LIN_GC;
+// 8 products (<= q/2 + 2 + 7q).  Each wave owns a tile of 2 queries x 8 blocks of
+// the unit; the unit is qa tiles high and 8/qa tiles wide (qa = 8, 4 or 2: 16 x 8,
+// 8 x 16 or 4 x 32 pairs), chosen per group so that the tiles are well filled.
+static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NW = LIN_TPB / LIN_POS;
 
 __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict__ P,
                                                                const LinUnit *__restrict__ units, u32 R,
@@ -687,42 +688,43 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
     __shared__ double buf[2][LIN_NS][LIN_POS];
     const LinUnit U = units[blockIdx.z];
     const LinView V(U, R);
-    const u32 limb = blockIdx.y, lane = threadIdx.x % LIN_POS, w = threadIdx.x / LIN_POS;
+    // the wave index is uniform across the wave: stream and tile bookkeeping in SGPRs
+    const u32 limb = blockIdx.y, lane = threadIdx.x % LIN_POS;
+    const u32 w = __builtin_amdgcn_readfirstlane(threadIdx.x / LIN_POS);
     const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * LIN_POS + lane;
     const double q = mp[limb].qd, qinv = mp[limb].qinv;
-    const u32 nq = min(LIN_QC, U.nq - U.q0), ng = min(LIN_GC, U.ng - U.g0);
-    // streams: 2i (ct0 of query i), 2i+1 (ct1 of query i), 2*QC + k (block k);
+    const u32 uq = U.uq, ug = U.ug, ns = 2 * uq + ug;
+    // streams: 2i (ct0 of query i), 2i+1 (ct1 of query i), 2*uq + k (block k);
     // thread (w, lane) moves streams w, w + 8, ... of its position
-    constexpr u32 NW = LIN_TPB / LIN_POS, PER = (LIN_NS + NW - 1) / NW;
+    constexpr u32 PER = (LIN_NS + LIN_NW - 1) / LIN_NW;
     const u64 *sbase[PER];
-    u32 sstride[PER];
+    u32 srow[PER];
     bool sok[PER];
 #pragma unroll
     for (u32 k = 0; k < PER; ++k) {
-        const u32 j = w + k * NW;
-        if (j < 2 * LIN_QC) {
-            sok[k] = j / 2 < nq;
+        const u32 j = w + k * LIN_NW;
+        sok[k] = j < ns;
+        srow[k] = j;
+        if (j < 2 * uq)
             sbase[k] = ((j & 1) ? V.ct1 : V.ct0) + (size_t)(U.q0 + j / 2) * R;
-        } else {
-            sok[k] = j < LIN_NS && j - 2 * LIN_QC < ng;
-            sbase[k] = V.pt + (size_t)(U.g0 + (j < LIN_NS ? j : 2 * LIN_QC) - 2 * LIN_QC) * R;
-        }
-        sstride[k] = j;
+        else
+            sbase[k] = V.pt + (size_t)(U.g0 + (j < ns ? j - 2 * uq : 0)) * R;
     }
     u64 ld[PER];
-    // this wave's pairs: queries 2w, 2w+1 x blocks 0..GC-1
-    const u32 qa = 2 * w, qb = 2 * w + 1;
-    const bool va = qa < nq, vb = qb < nq;
-    double a0[2][LIN_GC], a1[2][LIN_GC];
+    // this wave's tile: queries tq, tq+1 x blocks tg .. tg+7 of the unit
+    const u32 tq = 2 * (w % U.qa), tg = 8 * (w / U.qa);
+    const bool v0 = tq < uq, v1 = tq + 1 < uq;
+    const u32 nb = tg < ug ? min(8u, ug - tg) : 0;
+    double a0[2][8], a1[2][8];
 #pragma unroll
     for (u32 i = 0; i < 2; ++i) {
-        const u32 qq = U.q0 + 2 * w + i;
-        const bool v = (2 * w + i) < nq;
+        const bool v = i ? v1 : v0;
+        const u32 qq = U.q0 + tq + i;
         const double b0 = v ? u64_to_f64(lin_word(P, V.b0[qq], idx)) : 0.0;
         const double b1 = v ? u64_to_f64(lin_word(P, V.b1[qq], idx)) : 0.0;
 #pragma unroll
-        for (u32 k = 0; k < LIN_GC; ++k) {
-            const double pb = (v && k < ng) ? u64_to_f64(lin_word(P, V.ptb[U.g0 + k], idx)) : 0.0;
+        for (u32 k = 0; k < 8; ++k) {
+            const double pb = (v && k < nb) ? u64_to_f64(lin_word(P, V.ptb[U.g0 + tg + k], idx)) : 0.0;
             a0[i][k] = reduce_f64(b0 + pb, q, qinv);
             a1[i][k] = b1;
         }
@@ -730,7 +732,7 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
 #define LIN_ISSUE(rr)                                                          \
     _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) ld[k] = P[sbase[k][rr] + idx];
 #define LIN_DEPOSIT(b)                                                         \
-    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][sstride[k]][lane] = u64_to_f64(ld[k]);
+    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][srow[k]][lane] = u64_to_f64(ld[k]);
     LIN_ISSUE(0)
     LIN_DEPOSIT(0)
     __syncthreads();
@@ -739,25 +741,16 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
         if (r + 1 < R) {
             LIN_ISSUE(r + 1)
         }
-        if (va) {
-            const double x0 = buf[cur][2 * qa][lane], x1 = buf[cur][2 * qa + 1][lane];
 #pragma unroll
-            for (u32 k = 0; k < LIN_GC; ++k) {
-                if (k < ng) {
-                    const double p = buf[cur][2 * LIN_QC + k][lane];
-                    a0[0][k] += mulmod_vv_f64(x0, p, q, qinv);
-                    a1[0][k] += mulmod_vv_f64(x1, p, q, qinv);
-                }
-            }
-        }
-        if (vb) {
-            const double x0 = buf[cur][2 * qb][lane], x1 = buf[cur][2 * qb + 1][lane];
+        for (u32 i = 0; i < 2; ++i) {
+            if (!(i ? v1 : v0)) continue;
+            const double x0 = buf[cur][2 * (tq + i)][lane], x1 = buf[cur][2 * (tq + i) + 1][lane];
 #pragma unroll
-            for (u32 k = 0; k < LIN_GC; ++k) {
-                if (k < ng) {
-                    const double p = buf[cur][2 * LIN_QC + k][lane];
-                    a0[1][k] += mulmod_vv_f64(x0, p, q, qinv);
-                    a1[1][k] += mulmod_vv_f64(x1, p, q, qinv);
+            for (u32 k = 0; k < 8; ++k) {
+                if (k < nb) {
+                    const double p = buf[cur][2 * uq + tg + k][lane];
+                    a0[i][k] += mulmod_vv_f64(x0, p, q, qinv);
+                    a1[i][k] += mulmod_vv_f64(x1, p, q, qinv);
                 }
             }
         }
@@ -765,7 +758,7 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
 #pragma unroll
             for (u32 i = 0; i < 2; ++i)
 #pragma unroll
-                for (u32 k = 0; k < LIN_GC; ++k) {
+                for (u32 k = 0; k < 8; ++k) {
                     a0[i][k] = reduce_f64(a0[i][k], q, qinv);
                     a1[i][k] = reduce_f64(a1[i][k], q, qinv);
                 }
@@ -780,11 +773,11 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
 #undef LIN_DEPOSIT
 #pragma unroll
     for (u32 i = 0; i < 2; ++i) {
-        if ((2 * w + i) >= nq) continue;
+        if (!(i ? v1 : v0)) continue;
 #pragma unroll
-        for (u32 k = 0; k < LIN_GC; ++k)
-            if (k < ng) {
-                const size_t o = (size_t)(U.q0 + 2 * w + i) * U.ng + U.g0 + k;
+        for (u32 k = 0; k < 8; ++k)
+            if (k < nb) {
+                const size_t o = (size_t)(U.q0 + tq + i) * U.ng + U.g0 + tg + k;
                 P[V.o0[o] + idx] = f64_to_canon(reduce_f64(a0[i][k], q, qinv), q);
                 P[V.o1[o] + idx] = f64_to_canon(reduce_f64(a1[i][k], q, qinv), q);
             }

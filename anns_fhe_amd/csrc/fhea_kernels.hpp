@@ -159,13 +159,15 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 // A group's offs (device, u64 word offsets into P), nq queries and ng blocks:
 //   [ct0: nq*R][ct1: nq*R][bias0: nq][bias1: nq][pt: ng*R][ptb: ng][out0: nq*ng][out1: nq*ng]
 // (outputs row-major by query); ptb and bias entries may be LIN_NONE.  A unit is
-// the queries q0.. and blocks g0.. of one group that one grid slice evaluates:
-// up to LIN_QC x LIN_GC (FP64, LIN_SHAPE_LDS) or 1 x 8 (integer, LIN_SHAPE_1x8).
+// the queries q0..q0+uq-1 and blocks g0..g0+ug-1 of one group that one grid slice
+// evaluates: FP64 (LIN_SHAPE_LDS) uq <= 2*qa, ug <= 8*(8/qa), qa in {2, 4, 8}
+// (2 x 8 tiles, 8 per unit, qa of them stacked); integer (LIN_SHAPE_1x8) uq = 1,
+// ug <= 8.
 static constexpr u64 LIN_NONE = ~0ull;
-static constexpr u32 LIN_QC = 16, LIN_GC = 8;
+static constexpr u32 LIN_NS = 40;  // LDS streams: 2*uq + ug <= 40 for every shape
 struct LinUnit {
     const u64 *offs;
-    u32 nq, ng, q0, g0;
+    u32 nq, ng, q0, g0, uq, ug, qa, pad;
 };
 enum : int { LIN_SHAPE_1x8 = 0, LIN_SHAPE_LDS = 1 };
 // true when the FP64 kernels run (the LDS shape exists only there)

@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle ops timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ntt-reps", type=int, default=5)
-    ap.add_argument("--rates", default="20,40,80,160", help="serve: Poisson arrival rates swept (queries/s)")
+    ap.add_argument("--rates", default="20,80,320,1280", help="serve: Poisson arrival rates swept (queries/s)")
     ap.add_argument("--requests", type=int, default=160, help="serve: requests per rate")
     ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
@@ -62,7 +62,7 @@ def rand_limbs(rng, primes, n):
 def traffic_from_profile():
     """HBM bytes per roofline launch from the committed rocprof PMC summary
     (FETCH_SIZE x its copy-kernel scale + WRITE_SIZE), or None if absent."""
-    path = os.path.join(ROOT, "profiles", "ntt_N65536_L24_96limbs_traffic.json")
+    path = os.path.join(ROOT, "profiles", "ntt_N65536_L24_1536limbs_traffic.json")
     try:
         with open(path) as f:
             return json.load(f)["traffic_bytes_per_launch"]

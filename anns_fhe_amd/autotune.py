@@ -35,7 +35,10 @@ def _time(ctx, fn, reps):
 
 
 def ntt_splits(logN):
-    return [k for k in range(6, 13) if 6 <= logN - k <= 12] if logN > 12 else []
+    """First-pass lengths of the two-pass NTT; at N = 2^13, 2^14 also logN (one pass
+    over the whole limb)."""
+    two = [k for k in range(6, 13) if 6 <= logN - k <= 12] if logN > 12 else []
+    return two + ([logN] if logN in (13, 14) else [])
 
 
 def tune_ntt(ctx, scratch_off, polys, limbs, reps=5, log=None, src=None):

@@ -1005,7 +1005,10 @@ int fhea_ctx_set_tuning(fhea_ctx *c, int knob, uint32_t v) {
     switch (knob) {
     case FHEA_TUNE_NTT_SPLIT_FWD:
     case FHEA_TUNE_NTT_SPLIT_INV:
-        if (v != 0 && (lg <= 12 || v < 6 || v > 12 || lg - v < 6 || lg - v > 12)) return FHEA_ERR_ARG;
+        // v = lg: one pass over the whole limb (N = 2^13, 2^14)
+        if (v != 0 && !(v == lg && (lg == 13 || lg == 14)) &&
+            (lg <= 12 || v < 6 || v > 12 || lg - v < 6 || lg - v > 12))
+            return FHEA_ERR_ARG;
         (knob == FHEA_TUNE_NTT_SPLIT_FWD ? c->dt.k1_fwd : c->dt.k1_inv) = v;
         return FHEA_OK;
     case FHEA_TUNE_NTT_WG_PER_CU:
@@ -1431,14 +1434,16 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
 // of W words: outputs sorted, every input looked up among them).
 // Unit shape for a group of nq queries x ng blocks (FP64 kernel): qa tiles of 2
 // queries stacked, 8/qa tiles of 8 blocks side by side (16 x 8, 8 x 16 or 4 x 32),
-// the one needing the fewest units (ties: the taller).
+// the one that reads the fewest words: every unit reads its queries' two
+// ciphertext streams and its blocks' plaintext stream once (ties: the taller).
 static u32 lin_qa(u32 nq, u32 ng) {
     u32 best = 8;
-    u64 best_units = ~0ull;
+    u64 best_streams = ~0ull;
     for (u32 qa : {8u, 4u, 2u}) {
-        const u64 units = (u64)((nq + 2 * qa - 1) / (2 * qa)) * ((ng + 64 / qa - 1) / (64 / qa));
-        if (units < best_units) {
-            best_units = units;
+        const u64 uq = 2 * qa, ug = 64 / qa;
+        const u64 streams = 2ull * nq * ((ng + ug - 1) / ug) + (u64)ng * ((nq + uq - 1) / uq);
+        if (streams < best_streams) {
+            best_streams = streams;
             best = qa;
         }
     }

@@ -680,7 +680,7 @@ __global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const L
 // 8 products (<= q/2 + 2 + 7q).  Each wave owns a tile of 2 queries x 8 blocks of
 // the unit; the unit is qa tiles high and 8/qa tiles wide (qa = 8, 4 or 2: 16 x 8,
 // 8 x 16 or 4 x 32 pairs), chosen per group so that the tiles are well filled.
-static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NW = LIN_TPB / LIN_POS;
+static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NW = LIN_TPB / LIN_POS, LIN_DEPTH = 4;
 
 __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict__ P,
                                                                const LinUnit *__restrict__ units, u32 R,
@@ -710,7 +710,6 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
         else
             sbase[k] = V.pt + (size_t)(U.g0 + (j < ns ? j - 2 * uq : 0)) * R;
     }
-    u64 ld[PER];
     // this wave's tile: queries tq, tq+1 x blocks tg .. tg+7 of the unit
     const u32 tq = 2 * (w % U.qa), tg = 8 * (w / U.qa);
     const bool v0 = tq < uq, v1 = tq + 1 < uq;
@@ -729,46 +728,59 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
             a1[i][k] = b1;
         }
     }
-#define LIN_ISSUE(rr)                                                          \
-    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) ld[k] = P[sbase[k][rr] + idx];
-#define LIN_DEPOSIT(b)                                                         \
-    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][srow[k]][lane] = u64_to_f64(ld[k]);
-    LIN_ISSUE(0)
-    LIN_DEPOSIT(0)
-    __syncthreads();
-    for (u32 r = 0; r < R; ++r) {
-        const u32 cur = r & 1;
-        if (r + 1 < R) {
-            LIN_ISSUE(r + 1)
-        }
-#pragma unroll
-        for (u32 i = 0; i < 2; ++i) {
-            if (!(i ? v1 : v0)) continue;
-            const double x0 = buf[cur][2 * (tq + i)][lane], x1 = buf[cur][2 * (tq + i) + 1][lane];
-#pragma unroll
-            for (u32 k = 0; k < 8; ++k) {
-                if (k < nb) {
-                    const double p = buf[cur][2 * uq + tg + k][lane];
-                    a0[i][k] += mulmod_vv_f64(x0, p, q, qinv);
-                    a1[i][k] += mulmod_vv_f64(x1, p, q, qinv);
-                }
-            }
-        }
-        if ((r & 7) == 7) {
-#pragma unroll
-            for (u32 i = 0; i < 2; ++i)
-#pragma unroll
-                for (u32 k = 0; k < 8; ++k) {
-                    a0[i][k] = reduce_f64(a0[i][k], q, qinv);
-                    a1[i][k] = reduce_f64(a1[i][k], q, qinv);
-                }
-        }
-        // the other buffer was last read in iteration r-1, before its closing barrier
-        if (r + 1 < R) {
-            LIN_DEPOSIT(cur ^ 1)
-        }
-        __syncthreads();
+    // loads run LIN_DEPTH steps ahead of the products (register ring, static slots:
+    // the r loop is unrolled by LIN_DEPTH); LDS holds the current and the next step
+    u64 ld[LIN_DEPTH][PER];
+#define LIN_ISSUE(slot, rr)                                                    \
+    if ((rr) < R) {                                                            \
+        _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) ld[slot][k] = P[sbase[k][rr] + idx]; \
     }
+#define LIN_DEPOSIT(slot, b, rr)                                               \
+    if ((rr) < R) {                                                            \
+        _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][srow[k]][lane] = u64_to_f64(ld[slot][k]); \
+    }
+#define LIN_PRODUCTS(b, rr)                                                    \
+    if ((rr) < R) {                                                            \
+        _Pragma("unroll") for (u32 i = 0; i < 2; ++i) {                        \
+            if (!(i ? v1 : v0)) continue;                                      \
+            const double x0 = buf[b][2 * (tq + i)][lane], x1 = buf[b][2 * (tq + i) + 1][lane]; \
+            _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                    \
+                if (k < nb) {                                                  \
+                    const double p = buf[b][2 * uq + tg + k][lane];            \
+                    a0[i][k] += mulmod_vv_f64(x0, p, q, qinv);                 \
+                    a1[i][k] += mulmod_vv_f64(x1, p, q, qinv);                 \
+                }                                                              \
+            }                                                                  \
+        }                                                                      \
+        if (((rr) & 7) == 7) {                                                 \
+            _Pragma("unroll") for (u32 i = 0; i < 2; ++i)                      \
+            _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                    \
+                a0[i][k] = reduce_f64(a0[i][k], q, qinv);                      \
+                a1[i][k] = reduce_f64(a1[i][k], q, qinv);                      \
+            }                                                                  \
+        }                                                                      \
+    }
+    // step rr: products on LDS buffer rr&1, then step rr+1's words (loaded
+    // LIN_DEPTH-1 steps ago) into the other buffer, last read before the previous
+    // barrier; step rr+LIN_DEPTH's loads into the freed slot
+#define LIN_STEP(j, rr)                                                        \
+    LIN_PRODUCTS((j) & 1, rr)                                                  \
+    LIN_DEPOSIT(((j) + 1) % LIN_DEPTH, ((j) + 1) & 1, (rr) + 1)                \
+    LIN_ISSUE((j) % LIN_DEPTH, (rr) + LIN_DEPTH)                               \
+    __syncthreads();
+    static_assert(LIN_DEPTH % 2 == 0, "buffer parity must follow the unrolled step");
+#pragma unroll
+    for (u32 j = 0; j < LIN_DEPTH; ++j) LIN_ISSUE(j, j)
+    LIN_DEPOSIT(0, 0, 0u)
+    __syncthreads();
+    for (u32 r = 0; r < R; r += LIN_DEPTH) {
+        LIN_STEP(0, r)
+        LIN_STEP(1, r + 1)
+        LIN_STEP(2, r + 2)
+        LIN_STEP(3, r + 3)
+    }
+#undef LIN_STEP
+#undef LIN_PRODUCTS
 #undef LIN_ISSUE
 #undef LIN_DEPOSIT
 #pragma unroll

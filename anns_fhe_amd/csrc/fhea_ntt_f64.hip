@@ -267,8 +267,8 @@ __device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_li
 }
 
 
-template <bool INV, int LOGR, bool COLS>
-__global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
+template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH>
+__global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
                                                   const double *__restrict__ twf,
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                                                   u32 logT, int in_canon, int out_canon, int scale,
                                                   NttFuse fz, u32 tiles_per_limb, u32 n_tiles,
                                                   u32 npolys) {
-    __shared__ double lds[F_TILE + F_TILE / 16];
+    __shared__ double lds[TPB * 16 + TPB];  // one tile of 16 words per thread, padded
     constexpr int NR = (LOGR + 3) / 4;
     using R0 = Round<INV, LOGR, 0>;
     using RL = Round<INV, LOGR, NR - 1>;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
         } else {
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
-                const u32 e = tid + (u32)k * F_TPB;
+                const u32 e = tid + (u32)k * TPB;
                 if (e < T) r[k] = load_raw(sbase, gg.gidx(e, LOGR), fz, id.poly);
             }
         }
@@ -321,10 +321,10 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
 
     TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
     u64 raw[16];
-    if (FHEA_NTT_PREFETCH && cur.tile < n_tiles) issue(cur, raw);
+    if (PF && cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
         const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
-        if (!FHEA_NTT_PREFETCH) issue(cur, raw);
+        if (!PF) issue(cur, raw);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
         g.qinv = mp[cur.pidx].qinv;
@@ -343,12 +343,12 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
         } else {
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
-                const u32 e = tid + (u32)k * F_TPB;
+                const u32 e = tid + (u32)k * TPB;
                 if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.qinv, g.gidx(e, LOGR), cur.poly);
             }
         }
         // prefetch the next tile
-        if (FHEA_NTT_PREFETCH && nxt.tile < n_tiles) issue(nxt, raw);
+        if (PF && nxt.tile < n_tiles) issue(nxt, raw);
         if (!DIRECT_IN) {
             __syncthreads();
             if (active) {
@@ -395,6 +395,24 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                     round_arith<INV, LOGR, 2, COLS>(x, g, g.gidx(eb2, LOGR));
                 }
             }
+            if constexpr (NR > 3) {
+                using Rp = Round<INV, LOGR, 2>;
+                using R3 = Round<INV, LOGR, 3>;
+                const u32 Pp = g.logC + Rp::P, P3 = g.logC + R3::P;
+                const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+                const u32 eb3 = (tid & ((1u << P3) - 1)) | ((tid >> P3) << (P3 + 4));
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                }
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb3 | ((u32)u << P3))];
+                    round_arith<INV, LOGR, 3, COLS>(x, g, g.gidx(eb3, LOGR));
+                }
+            }
         }
         // ---- output of the last round
         {
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src,
                     for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
                 }
                 __syncthreads();
-                for (u32 e = tid; e < T; e += F_TPB)
+                for (u32 e = tid; e < T; e += TPB)
                     store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, cur.poly,
                               cur.slot, logN, g.q, g.qinv, INV);
             }
@@ -465,6 +483,30 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
 #undef FHEA_NTT_CASE
 }
 
+// One pass over a whole limb of N = 2^13 or 2^14: N/16 threads, the limb in LDS,
+// each limb read and written once (no register prefetch: it would not fit).
+template <bool INV, int LG>
+static void launch_limb_pass(u32 ntiles, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src,
+                             void *dst, u64 ss, u64 ds, u64 sl, u64 dl, u32 limbs, const u32 *pm,
+                             const double *twf, const double *ninvf, const ModParams *mp, int ic, int oc, int sc,
+                             const NttFuse &fz) {
+    constexpr u32 TPB = (1u << LG) / 16;
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LG, false, TPB, false>, TPB, 0) !=
+                hipSuccess ||
+            n <= 0)
+            n = 1;
+        per_cu = n < 4 ? n : 4;
+    }
+    const u32 use_cu = (wg_cap && wg_cap < (u32)per_cu) ? wg_cap : (u32)per_cu;
+    const u32 cap = max_wg / 4 * use_cu;
+    const u32 nwg = ntiles < cap ? ntiles : cap;
+    hipLaunchKernelGGL((k_ntt64f<INV, LG, false, TPB, false>), dim3(nwg), dim3(TPB), 0, s, src, dst, ss, ds, sl, dl,
+                       limbs, pm, twf, ninvf, mp, (u32)LG, (u32)LG, ic, oc, sc, fz, 1u, ntiles, npolys);
+}
+
 void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 ss, u64 ds, u64 sl,
                     u64 dl, u32 npolys, u32 limbs, const u32 *pm, const NttFuse &fz, hipStream_t s) {
     const u32 lg = t.logN;
@@ -484,6 +526,23 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
         return;
     }
     const u32 k1s = inverse ? t.k1_inv : t.k1_fwd;
+    if ((lg == 13 || lg == 14) && k1s == lg) {
+        const u32 nt = npolys * ylimbs;
+#define FHEA_LIMB_PASS(LG)                                                                                   \
+    if (inverse)                                                                                              \
+        launch_limb_pass<true, LG>(nt, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, \
+                                   t.ninvf, t.mp, 1, 1, 1, none);                                             \
+    else                                                                                                      \
+        launch_limb_pass<false, LG>(nt, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, \
+                                    t.ninvf, t.mp, 1, 1, 0, fz);
+        if (lg == 13) {
+            FHEA_LIMB_PASS(13)
+        } else {
+            FHEA_LIMB_PASS(14)
+        }
+#undef FHEA_LIMB_PASS
+        return;
+    }
     const u32 k1 = k1s ? k1s : lg / 2, k2 = lg - k1;
     dim3 grid(1u << (lg - F_LOG_TILE), npolys * ylimbs);
     if (!inverse) {

@@ -212,7 +212,8 @@ int fhea_graph_destroy(fhea_ctx *ctx, uint32_t graph_id);
 /* Schedule tuning (results are identical for every setting; only speed changes).
  * Used by the autotuner (anns_fhe_amd/autotune.py). */
 #define FHEA_TUNE_NTT_SPLIT_FWD 1 /* stages in the first pass of a two-pass forward NTT
-                                     (both passes must have 6..12 stages), 0 = logN/2 */
+                                     (both passes must have 6..12 stages), 0 = logN/2;
+                                     logN itself (N = 2^13, 2^14): one pass per limb */
 #define FHEA_TUNE_NTT_SPLIT_INV 2 /* the same for the inverse (its second pass)      */
 #define FHEA_TUNE_NTT_WG_PER_CU 3 /* resident NTT workgroups per CU, 1..4, 0 = by occupancy */
 #define FHEA_TUNE_CHUNK 4         /* batch items per key-switch pipeline chunk, 1..64 */

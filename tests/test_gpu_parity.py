@@ -682,3 +682,53 @@ def test_linear_sum_groups_parity(orc, q_bits):
         assert o.linear_sum(pm, ctq(z, 0), ctq(z, 1), np.array([blk(b) for b in bs], np.uint64), o0, o1, L,
                             ptb=[ptb(b) for b in bs], bias=bias(z) if z != 4 else None) == 0
     assert np.array_equal(got, pm[n_in * W:out * W])
+
+
+@pytest.mark.parametrize("N", [8192, 16384])
+def test_single_pass_limb_ntt(orc, N):
+    """N = 2^13, 2^14 with the split knob at logN (one pass over the whole limb, the
+    limb in LDS): NTT / INTT, the fused rescale (OP 8) and eval_mult_relin_rescale
+    (fused ModDown + rescale ends) give the oracle's words."""
+    from anns_fhe_amd import _lib
+    from anns_fhe_amd.driver import FHEHostDriver
+    L = 4
+    lg = N.bit_length() - 1
+    ctx = get_ctx(N, L, 2, 2)
+    o = get_orc(orc, ctx)
+    rng = np.random.default_rng(N + 5)
+    W = L * N
+    try:
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_FWD, lg)
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, lg)
+        a = rand_poly(rng, ctx.q, N)
+        ctx.write_poly(0, a)
+        say("single-pass ntt", N)
+        assert ctx.op(1, 0, 0, W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(W, W).reshape(L, N), o.ntt_poly(a))
+        assert ctx.op(2, 0, 0, 2 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(2 * W, W).reshape(L, N), o.intt_poly(a))
+        # OP 8 rescale of a ciphertext (c0 at 0, c1 at 3W) -> 5W
+        c1 = rand_poly(rng, ctx.q, N)
+        ctx.write_poly(3 * W, c1)
+        assert ctx.op(8, 0, 3 * W, 5 * W, 0, L) == 0
+        W1 = (L - 1) * N
+        assert np.array_equal(ctx.read_poly(5 * W, W1).reshape(L - 1, N), o.rescale(a.reshape(-1)))
+        assert np.array_equal(ctx.read_poly(5 * W + W1, W1).reshape(L - 1, N), o.rescale(c1.reshape(-1)))
+        # HMult with the fused ModDown + rescale
+        drv = FHEHostDriver(ctx)
+        s = ctx.keygen_secret(5)
+        ctx.keygen_relin_key(s, L, 6, 0)
+        drv.set_relin_key(0, L)
+        ins = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+        ctx.write_poly(8 * W, ins)
+        drv.eval_mult_relin_rescale(8 * W, 9 * W, 10 * W, 11 * W, 12 * W, 13 * W, L)
+        got = ctx.read_poly(12 * W, 2 * W)
+        pm = np.zeros(14 * W, np.uint64)
+        pm[8 * W:12 * W] = ins
+        km = ctx.read_key(0, ctx.key_size_words(L))
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, 8 * W, 9 * W, 10 * W, 11 * W, 12 * W, 13 * W, L) == 0
+        assert np.array_equal(got[:W1], pm[12 * W:12 * W + W1])
+        assert np.array_equal(got[W:W + W1], pm[13 * W:13 * W + W1])
+    finally:
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_FWD, 0)
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, 0)

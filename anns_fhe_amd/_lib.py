@@ -28,7 +28,7 @@ OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
 WORK_MULT_RELIN_RESCALE, WORK_AUTOMORPHISM_KS, WORK_NTT, WORK_INTT = 1, 2, 3, 4
 WORK_MULT_RELIN, WORK_ADD, WORK_ROTATE_HOISTED = 5, 6, 7
 NONE = (1 << 64) - 1
-TUNE_NTT_SPLIT_FWD, TUNE_NTT_SPLIT_INV, TUNE_NTT_WG_PER_CU, TUNE_CHUNK = 1, 2, 3, 4
+TUNE_NTT_SPLIT_FWD, TUNE_NTT_SPLIT_INV, TUNE_NTT_WG_PER_CU, TUNE_CHUNK, TUNE_STREAMS = 1, 2, 3, 4, 5
 
 
 class FheLibraryError(RuntimeError):

@@ -9,7 +9,8 @@ The knobs:
     inverse transforms, among every split whose passes have 6..12 stages;
   * the number of resident NTT workgroups per CU;
   * the number of batch items per key-switch pipeline chunk, on a caller-given
-    list of work items (for example the HMult batch being served).
+    list of work items (for example the HMult batch being served);
+  * one or two streams for a batched HMult (its halves overlapping on the device).
 Every NTT candidate's output (first and last polynomial of the batch) is checked
 against the default schedule's word for word before it can be chosen.  Choices can be cached in a JSON file keyed by the
 parameter set.
@@ -22,7 +23,7 @@ import numpy as np
 from . import _lib
 
 KNOBS = {"ntt_split_fwd": _lib.TUNE_NTT_SPLIT_FWD, "ntt_split_inv": _lib.TUNE_NTT_SPLIT_INV,
-         "ntt_wg_per_cu": _lib.TUNE_NTT_WG_PER_CU, "chunk": _lib.TUNE_CHUNK}
+         "ntt_wg_per_cu": _lib.TUNE_NTT_WG_PER_CU, "chunk": _lib.TUNE_CHUNK, "streams": _lib.TUNE_STREAMS}
 
 
 def _time(ctx, fn, reps):
@@ -97,11 +98,13 @@ def candidates(ctx, knob):
         return [0] + ntt_splits(ctx.N.bit_length() - 1)
     if knob == "ntt_wg_per_cu":
         return [0, 4, 3, 2]
+    if knob == "streams":
+        return [1, 2]
     return [4, 8, 16, 32]
 
 
-def tune(ctx, run, check=None, knobs=("ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"), reps=3,
-         log=None):
+def tune(ctx, run, check=None, knobs=("streams", "ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"),
+         reps=3, log=None):
     """Coordinate search over `knobs`, timing `run()` (the workload itself, launched
     on the context's stream) for every candidate of one knob with the others fixed
     at their best so far.  check(): a result digest of the workload (for example its

@@ -58,8 +58,10 @@ static int stage_reserve(fhea_ctx *c, size_t bytes, char **h, char **d) {
         return FHEA_OK;
     }
     if (c->args_used + need > c->args_bytes) {
-        // every earlier copy must have landed before the staging area is reused
+        // every earlier copy (on either lane) must have landed before the staging
+        // area is reused
         if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+        if (c->stream2 && hipStreamSynchronize(c->stream2) != hipSuccess) return FHEA_ERR_DEVICE;
         c->args_used = 0;
         if (need > c->args_bytes) return FHEA_ERR_NO_MEM;
     }
@@ -605,8 +607,39 @@ struct MultItem {
 
 // eval_mult_relin_rescale (fhe_host_driver.cpp:272-306) for a batch sharing one
 // relinearisation key: tensor -> KS(d2) (+d0, +d1 fused) -> rescale.
+static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
+                                   const std::vector<MultItem> &items);
+
+// Swaps the context's stream and workspace with the second lane's.
+static void swap_lane(fhea_ctx *c) {
+    std::swap(c->stream, c->stream2);
+    std::swap(c->ws, c->ws2);
+    std::swap(c->ws_words, c->ws2_words);
+}
+
 static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
                               const std::vector<MultItem> &items) {
+    if (c->lanes < 2 || c->capturing || items.size() < 2 || !c->stream2)
+        return mult_relin_rescale_lane(c, l, key, key_limbs, items);
+    // second half on the second lane: it starts after everything issued so far on
+    // the stream, and the stream continues only after it
+    const size_t half = items.size() / 2;
+    std::vector<MultItem> a(items.begin(), items.begin() + half), b(items.begin() + half, items.end());
+    if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->stream2, c->ev_fork, 0) != hipSuccess)
+        return FHEA_ERR_DEVICE;
+    swap_lane(c);
+    int st = mult_relin_rescale_lane(c, l, key, key_limbs, b);
+    swap_lane(c);
+    const int st2 = mult_relin_rescale_lane(c, l, key, key_limbs, a);
+    if (hipEventRecord(c->ev_join, c->stream2) != hipSuccess ||
+        hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess)
+        return FHEA_ERR_DEVICE;
+    return st ? st : st2;
+}
+
+static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
+                                   const std::vector<MultItem> &items) {
     const u64 N = c->h.N;
     for (size_t off = 0; off < items.size(); off += c->max_chunk) {
         u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
@@ -902,6 +935,13 @@ int fhea_ctx_destroy(fhea_ctx *c) {
     if (c->args_host) hipHostFree(c->args_host);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream2) {
+        hipStreamSynchronize(c->stream2);
+        hipStreamDestroy(c->stream2);
+    }
+    if (c->ws2) hipFree(c->ws2);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return FHEA_OK;
@@ -1019,6 +1059,16 @@ int fhea_ctx_set_tuning(fhea_ctx *c, int knob, uint32_t v) {
         if (v < 1 || v > 64) return FHEA_ERR_ARG;
         c->max_chunk = v;
         return FHEA_OK;
+    case FHEA_TUNE_STREAMS:
+        if (v < 1 || v > 2) return FHEA_ERR_ARG;
+        if (v == 2 && !c->stream2) {
+            if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return FHEA_ERR_DEVICE;
+            if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+                return FHEA_ERR_DEVICE;
+        }
+        c->lanes = v;
+        return FHEA_OK;
     default:
         return FHEA_ERR_ARG;
     }
@@ -1032,6 +1082,7 @@ int fhea_ctx_get_tuning(const fhea_ctx *c, int knob, uint32_t *v) {
     case FHEA_TUNE_NTT_SPLIT_INV: *v = c->dt.k1_inv; return FHEA_OK;
     case FHEA_TUNE_NTT_WG_PER_CU: *v = c->dt.wg_per_cu; return FHEA_OK;
     case FHEA_TUNE_CHUNK: *v = c->max_chunk; return FHEA_OK;
+    case FHEA_TUNE_STREAMS: *v = c->lanes; return FHEA_OK;
     default: return FHEA_ERR_ARG;
     }
 }

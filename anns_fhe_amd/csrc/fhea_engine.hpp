@@ -62,6 +62,15 @@ struct fhea_ctx {
     std::map<uint32_t, Graph> graphs;
     uint32_t next_graph = 1;
 
+    // second lane (FHEA_TUNE_STREAMS = 2): a batched HMult runs half of its items on
+    // a second stream with its own workspace, forked from and joined back into
+    // `stream` with events, so that the two halves' kernels overlap on the device
+    u32 lanes = 1;
+    hipStream_t stream2 = nullptr;
+    u64 *ws2 = nullptr;
+    u64 ws2_words = 0;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     u32 max_chunk = 16;
     bool force_int = false;  // FHEA_FORCE_INT=1: integer kernels even for < 2^50 primes
     u32 f64_off = 0;         // FHEA_F64_OFF: per-component switch (DevTables::f64_off)

@@ -217,6 +217,8 @@ int fhea_graph_destroy(fhea_ctx *ctx, uint32_t graph_id);
 #define FHEA_TUNE_NTT_SPLIT_INV 2 /* the same for the inverse (its second pass)      */
 #define FHEA_TUNE_NTT_WG_PER_CU 3 /* resident NTT workgroups per CU, 1..4, 0 = by occupancy */
 #define FHEA_TUNE_CHUNK 4         /* batch items per key-switch pipeline chunk, 1..64 */
+#define FHEA_TUNE_STREAMS 5       /* 1 or 2: a batched HMult runs its two halves on two
+                                     streams that overlap on the device              */
 int fhea_ctx_set_tuning(fhea_ctx *ctx, int knob, uint32_t value);
 int fhea_ctx_get_tuning(const fhea_ctx *ctx, int knob, uint32_t *value);
 

@@ -585,7 +585,8 @@ def test_workload_tuner_keeps_results(orc):
                                      (4 * n + 1) * W, L) == 0
     assert np.array_equal(got[:(L - 1) * N], pm[4 * n * W:4 * n * W + (L - 1) * N])
     assert np.array_equal(got[W:W + (L - 1) * N], pm[(4 * n + 1) * W:(4 * n + 1) * W + (L - 1) * N])
-    AT.apply(ctx, {"ntt_split_fwd": 0, "ntt_split_inv": 0, "ntt_wg_per_cu": 0, "chunk": 16})
+    assert any(k.startswith("streams=2") for k in t)  # the two-lane schedule was run and checked
+    AT.apply(ctx, {"ntt_split_fwd": 0, "ntt_split_inv": 0, "ntt_wg_per_cu": 0, "chunk": 16, "streams": 1})
 
 
 @pytest.mark.parametrize("q_bits", [50, 60])

@@ -144,6 +144,31 @@ class CkksClient:
         check(self.lib.fhea_decrypt(self.h, s.ctypes.data_as(_lib.i8p), _p64(ct), limbs, _p64(pt)), "fhea_decrypt")
         return pt
 
+    def secret_eval(self, s, limbs=None):
+        """The secret in the evaluation domain on the first `limbs` Q primes (for
+        decrypt_values_batch; computed once per secret)."""
+        limbs = limbs or self.L
+        out = np.zeros(limbs * self.N, np.uint64)
+        check(self.lib.fhea_secret_eval(self.h, s.ctypes.data_as(_lib.i8p), limbs, _p64(out)), "fhea_secret_eval")
+        return out
+
+    def decrypt_values_batch(self, s_eval, cts, limbs, scale, n, threads=1, complex_out=False):
+        """Decrypts and decodes `cts` (count x 2*limbs*N words, or one flat array of
+        them) with s_eval = secret_eval(s): (count, n) slot values, equal to
+        decrypt_values on each ciphertext."""
+        cts = np.ascontiguousarray(cts, dtype=np.uint64).reshape(-1)
+        W = 2 * limbs * self.N
+        if cts.size % W or s_eval.size < limbs * self.N:
+            raise ValueError("ciphertext or secret size does not match limbs")
+        count = cts.size // W
+        re = np.zeros((count, n), np.float64)
+        im = np.zeros((count, n), np.float64) if complex_out else None
+        check(self.lib.fhea_decrypt_decode_batch(self.h, _p64(np.ascontiguousarray(s_eval, np.uint64)), _p64(cts),
+                                                 count, limbs, scale, n, _pf(re),
+                                                 _pf(im) if im is not None else C.cast(None, _lib.f64p), threads),
+              "fhea_decrypt_decode_batch")
+        return re + 1j * im if complex_out else re
+
     def encrypt_values(self, pk, values, limbs, scale, seed):
         return self.encrypt(pk, self.encode(values, limbs, scale), limbs, seed)
 

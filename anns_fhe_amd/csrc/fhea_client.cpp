@@ -15,19 +15,21 @@
 //
 // Encoding (canonical embedding).  zeta = exp(i pi / N).  Slot j holds
 // m(zeta^(5^j mod 2N)) / scale, j < N/2; the conjugate slots are implied, so m
-// has real (integer) coefficients.  Since zeta^(2t+1) runs over the odd powers,
-//     m(zeta^(2t+1)) = sum_k (m_k zeta^k) omega^(t k),   omega = zeta^2,
-// is a length-N DFT of the twisted coefficients; slot j is DFT index
-// t_j = (5^j mod 2N - 1) / 2 and its conjugate t'_j = (2N - 5^j mod 2N - 1) / 2.
-// Encoding runs the inverse DFT from those N values and rounds.  A vector of n
-// values (n a power of two dividing N/2) is replicated with period n, so a
-// rotation by r (the Galois element 5^r) rotates it cyclically.
+// has real (integer) coefficients.  Every 5^j is 1 mod 4, and the powers of 5
+// run over all N/2 such residues mod 2N, so the slots are m at zeta^(4u+1),
+// u < M = N/2.  With zeta^((4u+1)M) = i,
+//     m(zeta^(4u+1)) = sum_{k<M} w_k omega^(u k),
+//     w_k = (m_k + i m_{k+M}) zeta^k,   omega = zeta^4 = exp(2 pi i / M),
+// is one length-M DFT; slot j is DFT index u_j = (5^j mod 2N - 1) / 4.  Encoding
+// runs the inverse DFT from the M slot values, untwists w_k and rounds its real
+// and imaginary parts to m_k and m_{k+M}.  A vector of n values (n a power of
+// two dividing N/2) is replicated with period n, so a rotation by r (the Galois
+// element 5^r) rotates it cyclically.
 //
 // Decryption m = c0 + c1 s (evaluation domain); decoding lifts each coefficient
 // from its residues to the centred integer by Garner's mixed-radix CRT, then
 // runs the forward DFT.
 #include <cmath>
-#include <complex>
 #include <cstring>
 #include <algorithm>
 #include <random>
@@ -40,24 +42,25 @@ using namespace fhea;
 
 struct fhea_client {
     HostTables h;
-    std::vector<std::complex<double>> roots;  // exp(2 pi i k / N), k < N
-    std::vector<std::complex<double>> zeta;   // exp(i pi k / N), k < N
-    std::vector<u32> slot_t, slot_tc;          // DFT index of slot j and of its conjugate, j < N/2
+    // canonical embedding as one DFT of length M = N/2 (split real / imaginary parts)
+    std::vector<double> tw_re, tw_im;      // exp(2 pi i k / M), k < M/2
+    std::vector<double> zeta_re, zeta_im;  // exp(i pi k / N), k < M
+    std::vector<u32> slot_u;               // DFT index of slot j, j < M
     void init_embedding();
 };
 
 namespace {
 
-typedef std::complex<double> cplx;
 const double kPi = 3.14159265358979323846;
 
 struct HostNtt {
     const HostTables &h;
     explicit HostNtt(const HostTables &ht) : h(ht) {}
-    // forward negacyclic NTT of one limb (ntt.cpp:4-43), Shoup products
+    // forward negacyclic NTT of one limb (ntt.cpp:4-43), Shoup products, lazy
+    // butterflies: values stay in [0, 4q) (q < 2^62), canonical at the end
     void forward(u64 *a, u32 pidx) const {
         const u32 N = h.N;
-        const u64 q = h.primes[pidx];
+        const u64 q = h.primes[pidx], q2 = 2 * q;
         const u64 *tw = h.tw.data() + (size_t)pidx * N, *tsh = h.tw_sh.data() + (size_t)pidx * N;
         u32 t = N >> 1;
         for (u32 m = 1; m < N; m <<= 1, t >>= 1) {
@@ -65,17 +68,26 @@ struct HostNtt {
                 const u64 W = tw[m + i], Wsh = tsh[m + i];
                 u64 *x = a + 2 * i * t;
                 for (u32 j = 0; j < t; ++j) {
-                    u64 u = x[j], v = mul_shoup(x[j + t], W, Wsh, q);
-                    x[j] = add_mod(u, v, q);
-                    x[j + t] = sub_mod(u, v, q);
+                    u64 u = x[j];
+                    if (u >= q2) u -= q2;
+                    const u64 v = mul_shoup_lazy(x[j + t], W, Wsh, q);  // [0, 2q)
+                    x[j] = u + v;
+                    x[j + t] = u - v + q2;
                 }
             }
         }
+        for (u32 j = 0; j < N; ++j) {
+            u64 v = a[j];
+            if (v >= q2) v -= q2;
+            if (v >= q) v -= q;
+            a[j] = v;
+        }
     }
-    // inverse (ntt.cpp:46-80): Gentleman-Sande with itw, then N^-1
+    // inverse (ntt.cpp:46-80): Gentleman-Sande with itw, then N^-1; values stay
+    // in [0, 2q) between the stages
     void inverse(u64 *a, u32 pidx) const {
         const u32 N = h.N;
-        const u64 q = h.primes[pidx];
+        const u64 q = h.primes[pidx], q2 = 2 * q;
         const u64 *itw = h.itw.data() + (size_t)pidx * N, *ish = h.itw_sh.data() + (size_t)pidx * N;
         u32 t = 1;
         for (u32 m = N >> 1; m >= 1; m >>= 1, t <<= 1) {
@@ -83,9 +95,11 @@ struct HostNtt {
                 const u64 W = itw[m + i], Wsh = ish[m + i];
                 u64 *x = a + 2 * i * t;
                 for (u32 j = 0; j < t; ++j) {
-                    u64 u = x[j], v = x[j + t];
-                    x[j] = add_mod(u, v, q);
-                    x[j + t] = mul_shoup(sub_mod(u, v, q), W, Wsh, q);
+                    const u64 u = x[j], v = x[j + t];
+                    u64 w = u + v;
+                    if (w >= q2) w -= q2;
+                    x[j] = w;
+                    x[j + t] = mul_shoup_lazy(u - v + q2, W, Wsh, q);
                 }
             }
         }
@@ -94,9 +108,21 @@ struct HostNtt {
     }
 };
 
-u64 to_residue(int64_t v, u64 q) {
-    u64 r = v >= 0 ? (u64)v % q : q - ((u64)(-(v + 1)) % q) - 1;
-    return r;
+// x mod q for any 64-bit x, q < 2^62: with r = floor((2^64 - 1) / q) the quotient
+// estimate floor(x r / 2^64) is at most 2 below floor(x / q).
+struct Reducer {
+    u64 q, r;
+    explicit Reducer(u64 qq) : q(qq), r(~0ull / qq) {}
+    u64 operator()(u64 x) const {
+        u64 t = x - mulhi64(x, r) * q;
+        if (t >= q) t -= q;
+        if (t >= q) t -= q;
+        return t;
+    }
+};
+
+u64 to_residue(int64_t v, const Reducer &red) {
+    return v >= 0 ? red((u64)v) : red.q - red((u64)(-(v + 1))) - 1;
 }
 
 // Signed coefficients -> evaluation-domain limbs for the given prime indices.
@@ -105,9 +131,9 @@ std::vector<u64> to_eval(const HostTables &h, const int64_t *coef, const std::ve
     std::vector<u64> out((size_t)pidx.size() * N);
     HostNtt ntt(h);
     for (size_t r = 0; r < pidx.size(); ++r) {
-        const u64 q = h.primes[pidx[r]];
+        const Reducer red(h.primes[pidx[r]]);
         u64 *row = out.data() + r * N;
-        for (u32 j = 0; j < N; ++j) row[j] = to_residue(coef[j], q);
+        for (u32 j = 0; j < N; ++j) row[j] = to_residue(coef[j], red);
         ntt.forward(row, pidx[r]);
     }
     return out;
@@ -228,44 +254,45 @@ int secret(u32 N, uint64_t seed, int8_t *s_out) {
 
 // ---- canonical embedding
 
-// In-place iterative radix-2 DFT of size n = roots.size(),
-// X_t = sum_k x_k exp(sign 2 pi i t k / n).
-void dft(std::vector<cplx> &a, int sign, const std::vector<cplx> &roots) {
-    const size_t n = a.size();
-    for (size_t i = 1, j = 0; i < n; ++i) {
-        size_t bit = n >> 1;
+// In-place iterative radix-2 DFT of length M on split parts (re, im):
+// X_t = sum_k x_k exp(sign 2 pi i t k / M); tw_re/tw_im = exp(2 pi i k / M), k < M/2.
+void dft_split(double *re, double *im, u32 M, int sign, const double *tw_re, const double *tw_im) {
+    for (u32 i = 1, j = 0; i < M; ++i) {
+        u32 bit = M >> 1;
         for (; j & bit; bit >>= 1) j ^= bit;
         j ^= bit;
-        if (i < j) std::swap(a[i], a[j]);
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
     }
-    for (size_t len = 2; len <= n; len <<= 1) {
-        const size_t half = len >> 1, step = n / len;
-        for (size_t i = 0; i < n; i += len)
-            for (size_t k = 0; k < half; ++k) {
-                const cplx w = sign > 0 ? roots[k * step] : std::conj(roots[k * step]);
-                cplx u = a[i + k], v = a[i + k + half] * w;
-                a[i + k] = u + v;
-                a[i + k + half] = u - v;
+    const double sg = sign > 0 ? 1.0 : -1.0;
+    for (u32 len = 2; len <= M; len <<= 1) {
+        const u32 half = len >> 1, step = M / len;
+        for (u32 i = 0; i < M; i += len) {
+            double *ar = re + i, *ai = im + i, *br = ar + half, *bi = ai + half;
+            for (u32 k = 0; k < half; ++k) {
+                const double wr = tw_re[k * step], wi = sg * tw_im[k * step];
+                const double vr = br[k] * wr - bi[k] * wi, vi = br[k] * wi + bi[k] * wr;
+                br[k] = ar[k] - vr;
+                bi[k] = ai[k] - vi;
+                ar[k] += vr;
+                ai[k] += vi;
             }
+        }
     }
-}
-
-// DFT index of slot j and of its conjugate
-void slot_index(u32 N, u32 j, u32 &t, u32 &tc) {
-    const u64 e = pow_mod(5, j, 2ull * N);
-    t = (u32)((e - 1) / 2);
-    tc = (u32)((2ull * N - e - 1) / 2);
 }
 
 bool valid_slots(u32 N, u32 n) { return n >= 1 && n <= N / 2 && (n & (n - 1)) == 0; }
 
 // Integer-valued double -> residue mod q, exact for any magnitude: |v| = mant * 2^e
 // with a 53-bit integer mantissa.
-u64 double_to_residue(double v, u64 q) {
+u64 double_to_residue(double v, const Reducer &red) {
+    const u64 q = red.q;
     const double a = std::fabs(v);
     u64 r;
     if (a < 4.0e18) {
-        r = (u64)(int64_t)a % q;
+        r = red((u64)a);
     } else {
         int e = 0;
         const double m = std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
@@ -279,25 +306,31 @@ u64 double_to_residue(double v, u64 q) {
 int encode_one(const fhea_client *cl, const double *re, const double *im, u32 n, u32 limbs, double scale,
                u64 *pt) {
     const HostTables &h = cl->h;
-    const u32 N = h.N;
-    std::vector<cplx> Y(N);
-    for (u32 j = 0; j < N / 2; ++j) {
-        const cplx z(re[j % n] * scale, im ? im[j % n] * scale : 0.0);
-        Y[cl->slot_t[j]] = z;
-        Y[cl->slot_tc[j]] = std::conj(z);
+    const u32 N = h.N, M = N / 2;
+    std::vector<double> xr(M), xi(M);
+    for (u32 j = 0; j < M; ++j) {
+        const u32 u = cl->slot_u[j];
+        xr[u] = re[j % n] * scale;
+        xi[u] = im ? im[j % n] * scale : 0.0;
     }
-    dft(Y, -1, cl->roots);
+    dft_split(xr.data(), xi.data(), M, -1, cl->tw_re.data(), cl->tw_im.data());
+    // w_k zeta^-k / M = m_k + i m_{k+M}
     std::vector<double> m(N);
-    for (u32 k = 0; k < N; ++k) {
-        const double v = std::nearbyint((Y[k] * std::conj(cl->zeta[k])).real() / (double)N);
-        if (!std::isfinite(v) || std::fabs(v) > 0x1p1000) return FHEA_ERR_ARG;
-        m[k] = v;
+    const double inv = 1.0 / (double)M;
+    for (u32 k = 0; k < M; ++k) {
+        const double zr = cl->zeta_re[k], zi = cl->zeta_im[k];
+        const double a = std::nearbyint((xr[k] * zr + xi[k] * zi) * inv);
+        const double b = std::nearbyint((xi[k] * zr - xr[k] * zi) * inv);
+        if (!std::isfinite(a) || !std::isfinite(b) || std::fabs(a) > 0x1p1000 || std::fabs(b) > 0x1p1000)
+            return FHEA_ERR_ARG;
+        m[k] = a;
+        m[k + M] = b;
     }
     HostNtt ntt(h);
     for (u32 i = 0; i < limbs; ++i) {
-        const u64 q = h.primes[i];
+        const Reducer red(h.primes[i]);
         u64 *row = pt + (size_t)i * N;
-        for (u32 k = 0; k < N; ++k) row[k] = double_to_residue(m[k], q);
+        for (u32 k = 0; k < N; ++k) row[k] = double_to_residue(m[k], red);
         ntt.forward(row, i);
     }
     return FHEA_OK;
@@ -336,20 +369,46 @@ struct Crt {
     }
 };
 
+// First n slots of a plaintext whose limbs c (limbs*N words) are in the
+// coefficient domain: lift (Garner), twist, one length-N/2 DFT, slot order.
+void decode_coeffs(const fhea_client *cl, const Crt &crt, const u64 *c, double scale, u32 n, double *re,
+                   double *im) {
+    const u32 N = cl->h.N, M = N / 2;
+    std::vector<double> xr(M), xi(M);
+    for (u32 k = 0; k < M; ++k) {
+        const double a = crt.value(c + k, N), b = crt.value(c + k + M, N);
+        const double zr = cl->zeta_re[k], zi = cl->zeta_im[k];
+        xr[k] = a * zr - b * zi;
+        xi[k] = a * zi + b * zr;
+    }
+    dft_split(xr.data(), xi.data(), M, +1, cl->tw_re.data(), cl->tw_im.data());
+    for (u32 j = 0; j < n; ++j) {
+        const u32 u = cl->slot_u[j];
+        re[j] = xr[u] / scale;
+        if (im) im[j] = xi[u] / scale;
+    }
+}
+
 }  // namespace
 
 void fhea_client::init_embedding() {
-    const u32 N = h.N;
-    roots.resize(N);
-    zeta.resize(N);
-    for (u32 k = 0; k < N; ++k) {
-        const double a = 2.0 * kPi * (double)k / (double)N, b = kPi * (double)k / (double)N;
-        roots[k] = cplx(std::cos(a), std::sin(a));
-        zeta[k] = cplx(std::cos(b), std::sin(b));
+    const u32 N = h.N, M = N / 2;
+    tw_re.resize(M / 2);
+    tw_im.resize(M / 2);
+    for (u32 k = 0; k < M / 2; ++k) {
+        const double a = 2.0 * kPi * (double)k / (double)M;
+        tw_re[k] = std::cos(a);
+        tw_im[k] = std::sin(a);
     }
-    slot_t.resize(N / 2);
-    slot_tc.resize(N / 2);
-    for (u32 j = 0; j < N / 2; ++j) slot_index(N, j, slot_t[j], slot_tc[j]);
+    zeta_re.resize(M);
+    zeta_im.resize(M);
+    for (u32 k = 0; k < M; ++k) {
+        const double b = kPi * (double)k / (double)N;
+        zeta_re[k] = std::cos(b);
+        zeta_im[k] = std::sin(b);
+    }
+    slot_u.resize(M);
+    for (u32 j = 0; j < M; ++j) slot_u[j] = (u32)((pow_mod(5, j, 2ull * N) - 1) / 4);
 }
 
 extern "C" {
@@ -535,15 +594,7 @@ int fhea_decode(const fhea_client *cl, const uint64_t *pt, uint32_t limbs, doubl
     std::vector<u64> c(pt, pt + (size_t)limbs * N);
     HostNtt ntt(h);
     for (u32 i = 0; i < limbs; ++i) ntt.inverse(c.data() + (size_t)i * N, i);
-    Crt crt(h, limbs);
-    std::vector<cplx> Y(N);
-    for (u32 k = 0; k < N; ++k) Y[k] = crt.value(c.data() + k, N) * cl->zeta[k];
-    dft(Y, +1, cl->roots);
-    for (u32 j = 0; j < n; ++j) {
-        const u32 t = cl->slot_t[j];
-        re[j] = Y[t].real() / scale;
-        if (im) im[j] = Y[t].imag() / scale;
-    }
+    decode_coeffs(cl, Crt(h, limbs), c.data(), scale, n, re, im);
     return FHEA_OK;
 }
 
@@ -598,6 +649,52 @@ int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, 
             c1[o] = add_mod(mulmod(v_e[o], a[o], mq), e1_e[o], q);
         }
     }
+    return FHEA_OK;
+}
+
+int fhea_secret_eval(const fhea_client *cl, const int8_t *s, uint32_t limbs, uint64_t *s_eval) {
+    if (!cl) return FHEA_ERR_STATE;
+    if (!s || !s_eval || limbs == 0 || limbs > cl->h.L) return FHEA_ERR_ARG;
+    const std::vector<u64> e = to_eval(cl->h, s_to_i64(s, cl->h.N).data(), q_primes(limbs));
+    std::memcpy(s_eval, e.data(), e.size() * 8);
+    return FHEA_OK;
+}
+
+int fhea_decrypt_decode_batch(const fhea_client *cl, const uint64_t *s_eval, const uint64_t *cts, uint32_t count,
+                              uint32_t limbs, double scale, uint32_t n, double *re, double *im, uint32_t threads) {
+    if (!cl) return FHEA_ERR_STATE;
+    const HostTables &h = cl->h;
+    const u32 N = h.N;
+    if (!s_eval || !cts || !re || !valid_slots(N, n) || limbs == 0 || limbs > h.L || !(scale > 0.0))
+        return FHEA_ERR_ARG;
+    if (count == 0) return FHEA_OK;
+    u32 T = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    T = std::min<u32>(T, count);
+    const Crt crt(h, limbs);
+    std::vector<ModParams> mq(limbs);
+    for (u32 i = 0; i < limbs; ++i) mq[i] = make_mod_params(h.primes[i]);
+    const size_t W = (size_t)limbs * N;
+    auto work = [&](u32 t) {
+        std::vector<u64> c(W);
+        HostNtt ntt(h);
+        for (u32 k = t; k < count; k += T) {
+            const u64 *c0 = cts + 2 * W * k, *c1 = c0 + W;
+            for (u32 i = 0; i < limbs; ++i) {
+                const u64 q = h.primes[i];
+                const size_t o = (size_t)i * N;
+                for (u32 j = 0; j < N; ++j) c[o + j] = add_mod(c0[o + j], mulmod(c1[o + j], s_eval[o + j], mq[i]), q);
+                ntt.inverse(c.data() + o, i);
+            }
+            decode_coeffs(cl, crt, c.data(), scale, n, re + (size_t)k * n, im ? im + (size_t)k * n : nullptr);
+        }
+    };
+    if (T == 1) {
+        work(0);
+        return FHEA_OK;
+    }
+    std::vector<std::thread> pool;
+    for (u32 t = 0; t < T; ++t) pool.emplace_back(work, t);
+    for (auto &th : pool) th.join();
     return FHEA_OK;
 }
 

@@ -136,8 +136,16 @@ class PlaintextSearcher:
 
 
 def exact_knn(x, q, k):
-    d = ((x.astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
-    return np.argsort(d, kind="stable")[:k]
+    """The first k of a stable argsort of the squared distances (float64) from q to
+    the rows of x: the k smallest by partition, ties broken by index.  Pass x as
+    float64 to avoid a conversion per call."""
+    d = ((np.asarray(x, np.float64) - np.asarray(q, np.float64)) ** 2).sum(1)
+    if k >= len(d):
+        return np.argsort(d, kind="stable")[:k]
+    t = np.partition(d, k - 1)[k - 1]
+    below = np.flatnonzero(d < t)
+    cand = np.concatenate([below, np.flatnonzero(d == t)[:k - len(below)]])
+    return cand[np.argsort(d[cand], kind="stable")]
 
 
 def recall_at(found, truth):
@@ -158,6 +166,7 @@ class FheIvfPqClient:
         self.scale = 2.0 ** log_scale
         self.lt = lt
         self.s = ckks.keygen_secret(seed)
+        self.s_eval = ckks.secret_eval(self.s)  # decryption reuses the transformed secret
         self.pk = ckks.keygen_public(self.s, seed + 1)
         self.seed = seed + 100
 
@@ -184,7 +193,11 @@ class FheIvfPqClient:
         return self.scale * self.scale / q_last
 
     def decrypt_distances(self, ct, limbs, scale, n):
-        return self.ck.decrypt_values(self.s, ct, limbs, scale, n)
+        return self.decrypt_distances_batch(ct, limbs, scale, n)[0]
+
+    def decrypt_distances_batch(self, cts, limbs, scale, n, threads=1):
+        """Slot values of several result ciphertexts (count x 2*limbs*N words): (count, n)."""
+        return self.ck.decrypt_values_batch(self.s_eval, cts, limbs, scale, n, threads)
 
 
 class FheIvfPqServer:
@@ -490,11 +503,11 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
 
     def top(z):
         ids, dist = [], []
+        dec = client.decrypt_distances_batch(fine[z], lt1, scale, server.S) if blocks[z] else None
         for g, b in enumerate(blocks[z]):
             cand = server.blocks[b][1]
-            d = client.decrypt_distances(fine[z][g], lt1, scale, server.S)[:len(cand)]
             ids.append(cand)
-            dist.append(d)
+            dist.append(dec[g][:len(cand)])
         ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
         dist = np.concatenate(dist) if dist else np.zeros(0)
         order = np.argsort(dist, kind="stable")[:top_k]

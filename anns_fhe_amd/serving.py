@@ -165,10 +165,11 @@ class ServingLoop:
             srv = p.server
             scale = p.client.result_scale(p.ck.q[srv.lt - 1])
             ids, dist = [], []
+            dec = p.client.decrypt_distances_batch(fine, srv.lt - 1, scale, srv.S) if blocks else None
             for g, b in enumerate(blocks):
                 cand = srv.blocks[b][1]
                 ids.append(cand)
-                dist.append(p.client.decrypt_distances(fine[g], srv.lt - 1, scale, srv.S)[:len(cand)])
+                dist.append(dec[g][:len(cand)])
             ids = np.concatenate(ids)
             dist = np.concatenate(dist)
             order = np.argsort(dist, kind="stable")[:10]

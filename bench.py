@@ -243,8 +243,10 @@ def bench_ivfpq(args):
                                log=log)
     server.use_graphs = not args.no_graphs
     t_encode = time.perf_counter() - t1
+    t2 = time.perf_counter()
     rel, rots = client.evaluation_keys()
     server.load_keys(rel, rots)
+    log(f"ivfpq: {len(rots) + 1} evaluation keys in {time.perf_counter() - t2:.1f} s")
     rng = np.random.default_rng(5)
 
     def batch():
@@ -252,20 +254,27 @@ def bench_ivfpq(args):
 
     for _ in range(args.warmup):
         IV.execute_interactive_queries(server, client, batch(), n_probe, 10)
+    log("ivfpq: warmup done")
     ps = IV.PlaintextSearcher(ix)
+    import concurrent.futures as cf
+    x64 = x.astype(np.float64)
+    check_pool = cf.ThreadPoolExecutor(16)
     server_s, lat, rec_f, rec_p, same = [], [], [], [], 0
-    for _ in range(args.steps):
+    for step in range(args.steps):
         qs = batch()
         w0 = time.perf_counter()
         res, t = IV.execute_interactive_queries(server, client, qs, n_probe, 10)
         lat.append(time.perf_counter() - w0)
         server_s.append(t["server_stage1"] + t["server_stage3"])
-        for q, (ids, dist, probes) in zip(qs, res):
+        log(f"ivfpq: step {step}: batch {lat[-1] * 1e3:.1f} ms, " +
+            ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in t.items()))
+        truths = list(check_pool.map(lambda q: IV.exact_knn(x64, q, 10), qs))
+        for q, truth, (ids, dist, probes) in zip(qs, truths, res):
             pids, _, _ = ps.search(q, n_probe, 10)
-            truth = IV.exact_knn(x, q, 10)
             rec_f.append(IV.recall_at(ids, truth))
             rec_p.append(IV.recall_at(pids, truth))
             same += set(ids) == set(pids)
+    check_pool.shutdown()
     srv = float(np.sum(server_s))
     qps = B * args.steps / srv
     lat_ms = np.array(lat) * 1e3

@@ -327,6 +327,17 @@ int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, 
 /* pt = c0 + c1 s (limbs*N words) */
 int fhea_decrypt(const fhea_client *cl, const int8_t *s, const uint64_t *ct, uint32_t limbs,
                  uint64_t *pt);
+/* the secret in the evaluation domain on the first `limbs` Q primes (limbs*N words),
+ * for fhea_decrypt_decode_batch (OpenFHE Decrypt keeps the same precomputed form) */
+int fhea_secret_eval(const fhea_client *cl, const int8_t *s, uint32_t limbs, uint64_t *s_eval);
+/* Decrypt + decode of `count` ciphertexts of `limbs` limbs (ct k = c0 then c1 at
+ * cts + 2*k*limbs*N) with s_eval from fhea_secret_eval (>= limbs limbs):
+ * re[k*n + j] (im likewise, or NULL) = slot j / scale, the values fhea_decrypt then
+ * fhea_decode give, on `threads` host threads (0 = all cores)
+ * (OpenFHE Decrypt + GetRealPackedValue, fhe_context_manager.cpp:457-512). */
+int fhea_decrypt_decode_batch(const fhea_client *cl, const uint64_t *s_eval, const uint64_t *cts,
+                              uint32_t count, uint32_t limbs, double scale, uint32_t n, double *re,
+                              double *im, uint32_t threads);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,31 @@ def test_encrypt_decrypt_round_trip():
     assert np.max(np.abs(bad - z)) > 1.0
 
 
+def test_batch_decrypt_decode_equals_single_calls():
+    """decrypt_values_batch (secret transformed once, several host threads) gives the
+    words of decrypt_values on each ciphertext; wrong sizes are refused."""
+    N, L = 1024, 3
+    cl = _client(N, L, 1, 3)
+    s = cl.keygen_secret(11)
+    pk = cl.keygen_public(s, 12)
+    rng = np.random.default_rng(13)
+    se = cl.secret_eval(s)
+    for limbs in (1, 3):
+        zs = [rng.uniform(-2, 2, N // 2) for _ in range(5)]
+        cts = np.stack([cl.encrypt_values(pk, z, limbs, 2.0 ** 35, seed=20 + k) for k, z in enumerate(zs)])
+        single = np.stack([cl.decrypt_values(s, c, limbs, 2.0 ** 35, N // 2) for c in cts])
+        for threads in (1, 3, 0):
+            got = cl.decrypt_values_batch(se, cts, limbs, 2.0 ** 35, N // 2, threads=threads)
+            assert np.array_equal(got, single), (limbs, threads)
+        assert np.max(np.abs(single - np.stack(zs))) < 2.0 ** -15
+        z = cl.decrypt_values_batch(se, cts[1], limbs, 2.0 ** 35, N // 2, complex_out=True)
+        assert z.shape == (1, N // 2) and np.max(np.abs(z[0].real - zs[1])) < 2.0 ** -15
+    with pytest.raises(ValueError):
+        cl.decrypt_values_batch(se, cts[:, :-1], 3, 2.0 ** 35, N // 2)
+    with pytest.raises(ValueError):
+        cl.decrypt_values_batch(cl.secret_eval(s, 1), cts, 3, 2.0 ** 35, N // 2)
+
+
 def test_hmult_and_rotation_semantics_through_oracle(orc):
     """Client keys + oracle evaluation: products and rotations decrypt to the
     expected slot vectors (the GPU path equals the oracle word for word)."""

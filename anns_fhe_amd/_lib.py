@@ -128,6 +128,7 @@ SIGNATURES = {
     "fhea_keygen_public": (C.c_int, [vp, i8p, C.c_uint64, u64p]),
     "fhea_encrypt": (C.c_int, [vp, u64p, u64p, C.c_uint32, C.c_uint64, u64p]),
     "fhea_decrypt": (C.c_int, [vp, i8p, u64p, C.c_uint32, u64p]),
+    "fhea_client_set_host_ntt": (C.c_int, [vp, C.c_int]),
     "fhea_secret_eval": (C.c_int, [vp, i8p, C.c_uint32, u64p]),
     "fhea_decrypt_decode_batch": (C.c_int, [vp, u64p, u64p, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, f64p,
                                             f64p, C.c_uint32]),

@@ -144,6 +144,11 @@ class CkksClient:
         check(self.lib.fhea_decrypt(self.h, s.ctypes.data_as(_lib.i8p), _p64(ct), limbs, _p64(pt)), "fhea_decrypt")
         return pt
 
+    def set_host_ntt(self, mode):
+        """0: FP64 CPU transforms where q < 2^50 and the CPU has AVX2+FMA (default);
+        1: 64-bit integer transforms.  The words are the same."""
+        check(self.lib.fhea_client_set_host_ntt(self.h, mode), "fhea_client_set_host_ntt")
+
     def secret_eval(self, s, limbs=None):
         """The secret in the evaluation domain on the first `limbs` Q primes (for
         decrypt_values_batch; computed once per secret)."""

@@ -43,9 +43,10 @@ using namespace fhea;
 struct fhea_client {
     HostTables h;
     // canonical embedding as one DFT of length M = N/2 (split real / imaginary parts)
-    std::vector<double> tw_re, tw_im;      // exp(2 pi i k / M), k < M/2
+    std::vector<double> tw_re, tw_im;      // stage of length 2h: exp(2 pi i k / 2h) at [h + k], k < h
     std::vector<double> zeta_re, zeta_im;  // exp(i pi k / N), k < M
     std::vector<u32> slot_u;               // DFT index of slot j, j < M
+    std::vector<u32> brev;                 // bit reversal on log2(M) bits
     void init_embedding();
 };
 
@@ -53,15 +54,165 @@ namespace {
 
 const double kPi = 3.14159265358979323846;
 
+// FP64 form of the CPU transforms for q < 2^50 (the device's exact products:
+// for |v| <= 8q, v w - rint(v w/q) q is computed exactly as fma(-k, q, v w) plus the
+// low part fma(v, w, -v w), and is at most q/2 + |v|/4 in magnitude).  Values are
+// centred every stage (|x| <= q/2 + 2, and |x| <= q on the inverse's products).
+// Four lanes per AVX2 instruction, the stages with butterfly distance 1 and 2
+// looping over the groups so that they vectorize too; the words equal the integer
+// transform's.  Every function here keeps products and sums as written (no
+// contraction into other FMAs).
+__attribute__((target("avx2,fma"))) inline double hmulmod(double v, double w, double wq, double q) {
+#pragma clang fp contract(off)
+    const double k = std::rint(v * wq);
+    const double hi = v * w;
+    const double lo = std::fma(v, w, -hi);
+    return std::fma(-k, q, hi) + lo;
+}
+__attribute__((target("avx2,fma"))) inline double hreduce(double x, double q, double qinv) {
+#pragma clang fp contract(off)
+    return std::fma(-std::rint(x * qinv), q, x);
+}
+// [0, 2^52) integers <-> doubles, exact (exponent-bias trick)
+inline double u52_to_f64(u64 x) {
+    double d;
+    const u64 b = x | 0x4330000000000000ull;
+    std::memcpy(&d, &b, 8);
+    return d - 4503599627370496.0;
+}
+inline u64 f64_to_u52(double y) {
+    const double z = y + 4503599627370496.0;
+    u64 b;
+    std::memcpy(&b, &z, 8);
+    return b & 0x000FFFFFFFFFFFFFull;
+}
+__attribute__((target("avx2,fma"))) inline void fwd_bfly(double &u, double &v, double w, double wq, double q,
+                                                        double qinv) {
+#pragma clang fp contract(off)
+    const double r = hmulmod(v, w, wq, q), a = u;
+    u = hreduce(a + r, q, qinv);
+    v = hreduce(a - r, q, qinv);
+}
+__attribute__((target("avx2,fma"))) inline void inv_bfly(double &u, double &v, double w, double wq, double q,
+                                                        double qinv) {
+#pragma clang fp contract(off)
+    const double a = u, b = v;
+    u = hreduce(a + b, q, qinv);
+    v = hmulmod(a - b, w, wq, q);
+}
+__attribute__((target("avx2,fma"))) void ntt_forward_f64_host(u64 *a, u32 N, u64 q, const double *W,
+                                                             const double *WQ) {
+#pragma clang fp contract(off)
+    std::vector<double> xv(N);
+    double *x = xv.data();
+    const double qd = (double)q, qinv = 1.0 / qd;
+    for (u32 j = 0; j < N; ++j) x[j] = u52_to_f64(a[j]);
+    u32 t = N >> 1;
+    for (u32 m = 1; m < N; m <<= 1, t >>= 1) {
+        if (t >= 4) {
+            for (u32 i = 0; i < m; ++i) {
+                const double w = W[m + i], wq = WQ[m + i];
+                double *xa = x + 2 * i * t, *xb = xa + t;
+                for (u32 j = 0; j < t; ++j) fwd_bfly(xa[j], xb[j], w, wq, qd, qinv);
+            }
+        } else if (t == 2) {
+            for (u32 i = 0; i < m; ++i) {
+                double *xa = x + 4 * i;
+                fwd_bfly(xa[0], xa[2], W[m + i], WQ[m + i], qd, qinv);
+                fwd_bfly(xa[1], xa[3], W[m + i], WQ[m + i], qd, qinv);
+            }
+        } else {
+            for (u32 i = 0; i < m; ++i) fwd_bfly(x[2 * i], x[2 * i + 1], W[m + i], WQ[m + i], qd, qinv);
+        }
+    }
+    for (u32 j = 0; j < N; ++j) a[j] = f64_to_u52(x[j] < 0.0 ? x[j] + qd : x[j]);
+}
+__attribute__((target("avx2,fma"))) void ntt_inverse_f64_host(u64 *a, u32 N, u64 q, const double *W,
+                                                             const double *WQ, double ni, double niq) {
+#pragma clang fp contract(off)
+    std::vector<double> xv(N);
+    double *x = xv.data();
+    const double qd = (double)q, qinv = 1.0 / qd;
+    for (u32 j = 0; j < N; ++j) x[j] = u52_to_f64(a[j]);
+    u32 t = 1;
+    for (u32 m = N >> 1; m >= 1; m >>= 1, t <<= 1) {
+        if (t == 1) {
+            for (u32 i = 0; i < m; ++i) inv_bfly(x[2 * i], x[2 * i + 1], W[m + i], WQ[m + i], qd, qinv);
+        } else if (t == 2) {
+            for (u32 i = 0; i < m; ++i) {
+                double *xa = x + 4 * i;
+                inv_bfly(xa[0], xa[2], W[m + i], WQ[m + i], qd, qinv);
+                inv_bfly(xa[1], xa[3], W[m + i], WQ[m + i], qd, qinv);
+            }
+        } else {
+            for (u32 i = 0; i < m; ++i) {
+                const double w = W[m + i], wq = WQ[m + i];
+                double *xa = x + 2 * i * t, *xb = xa + t;
+                for (u32 j = 0; j < t; ++j) inv_bfly(xa[j], xb[j], w, wq, qd, qinv);
+            }
+        }
+    }
+    for (u32 j = 0; j < N; ++j) {
+        const double y = hreduce(hmulmod(x[j], ni, niq, qd), qd, qinv);
+        a[j] = f64_to_u52(y < 0.0 ? y + qd : y);
+    }
+}
+
+// c = c0 + c1 s mod q, canonical inputs and output (|c1 s - k q| <= 0.875q exactly,
+// then the sum is centred and made canonical)
+__attribute__((target("avx2,fma"))) void dec_mul_f64(u64 *c, const u64 *c0, const u64 *c1, const u64 *s, u32 N,
+                                                    u64 q) {
+#pragma clang fp contract(off)
+    const double qd = (double)q, qinv = 1.0 / qd;
+    for (u32 j = 0; j < N; ++j) {
+        const double a = u52_to_f64(c1[j]), b = u52_to_f64(s[j]);
+        const double hi = a * b;
+        const double lo = std::fma(a, b, -hi);
+        const double k = std::rint(hi * qinv);
+        const double y = hreduce(std::fma(-k, qd, hi) + lo + u52_to_f64(c0[j]), qd, qinv);
+        c[j] = f64_to_u52(y < 0.0 ? y + qd : y);
+    }
+}
+
+// the FP64 twiddles of prime pidx (built on first use)
+const HostF64Twiddles::Prime &f64_twiddles(const HostTables &h, u32 pidx) {
+    HostF64Twiddles::Prime &P = h.f64tw->p[pidx];
+    std::call_once(P.once, [&] {
+        const u32 N = h.N;
+        const double q = (double)h.primes[pidx];
+        P.w.resize(N);
+        P.wq.resize(N);
+        P.iw.resize(N);
+        P.iwq.resize(N);
+        for (u32 k = 0; k < N; ++k) {
+            P.w[k] = (double)h.tw[(size_t)pidx * N + k];
+            P.wq[k] = P.w[k] / q;
+            P.iw[k] = (double)h.itw[(size_t)pidx * N + k];
+            P.iwq[k] = P.iw[k] / q;
+        }
+    });
+    return P;
+}
+
+bool host_f64_ok() {
+    static const bool ok = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    return ok;
+}
+
 struct HostNtt {
     const HostTables &h;
     explicit HostNtt(const HostTables &ht) : h(ht) {}
     // forward negacyclic NTT of one limb (ntt.cpp:4-43), Shoup products, lazy
     // butterflies: values stay in [0, 4q) (q < 2^62), canonical at the end
+    bool f64(u64 q) const { return h.host_ntt_mode == 0 && h.f64tw && q < (1ull << 50) && host_f64_ok(); }
     void forward(u64 *a, u32 pidx) const {
         const u32 N = h.N;
         const u64 q = h.primes[pidx], q2 = 2 * q;
         const u64 *tw = h.tw.data() + (size_t)pidx * N, *tsh = h.tw_sh.data() + (size_t)pidx * N;
+        if (f64(q)) {
+            const HostF64Twiddles::Prime &P = f64_twiddles(h, pidx);
+            return ntt_forward_f64_host(a, N, q, P.w.data(), P.wq.data());
+        }
         u32 t = N >> 1;
         for (u32 m = 1; m < N; m <<= 1, t >>= 1) {
             for (u32 i = 0; i < m; ++i) {
@@ -89,6 +240,11 @@ struct HostNtt {
         const u32 N = h.N;
         const u64 q = h.primes[pidx], q2 = 2 * q;
         const u64 *itw = h.itw.data() + (size_t)pidx * N, *ish = h.itw_sh.data() + (size_t)pidx * N;
+        if (f64(q)) {
+            const HostF64Twiddles::Prime &P = f64_twiddles(h, pidx);
+            const double ni = (double)h.ninv[pidx];
+            return ntt_inverse_f64_host(a, N, q, P.iw.data(), P.iwq.data(), ni, ni / (double)q);
+        }
         u32 t = 1;
         for (u32 m = N >> 1; m >= 1; m >>= 1, t <<= 1) {
             for (u32 i = 0; i < m; ++i) {
@@ -255,25 +411,51 @@ int secret(u32 N, uint64_t seed, int8_t *s_out) {
 // ---- canonical embedding
 
 // In-place iterative radix-2 DFT of length M on split parts (re, im):
-// X_t = sum_k x_k exp(sign 2 pi i t k / M); tw_re/tw_im = exp(2 pi i k / M), k < M/2.
-void dft_split(double *re, double *im, u32 M, int sign, const double *tw_re, const double *tw_im) {
-    for (u32 i = 1, j = 0; i < M; ++i) {
-        u32 bit = M >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
+// X_t = sum_k x_k exp(sign 2 pi i t k / M); the twiddles of the stage of length 2h
+// are contiguous at tw[h + k] (k < h), and the stages of length 2 and 4 loop over
+// the groups, so that every stage's inner loop runs over consecutive words.
+// Compiled twice (AVX2+FMA and the baseline target), picked at run time.
+__attribute__((always_inline)) inline void dft_split_body(double *re, double *im, u32 M, int sign,
+                                                          const double *tw_re, const double *tw_im,
+                                                          const u32 *brev) {
+    for (u32 i = 1; i < M; ++i) {
+        const u32 j = brev[i];
         if (i < j) {
             std::swap(re[i], re[j]);
             std::swap(im[i], im[j]);
         }
     }
     const double sg = sign > 0 ? 1.0 : -1.0;
-    for (u32 len = 2; len <= M; len <<= 1) {
-        const u32 half = len >> 1, step = M / len;
-        for (u32 i = 0; i < M; i += len) {
+    if (M >= 2)  // length 2: w = 1
+        for (u32 i = 0; i < M; i += 2) {
+            const double ar = re[i], ai = im[i], br = re[i + 1], bi = im[i + 1];
+            re[i] = ar + br;
+            im[i] = ai + bi;
+            re[i + 1] = ar - br;
+            im[i + 1] = ai - bi;
+        }
+    if (M >= 4)  // length 4: w = 1, sign i
+        for (u32 i = 0; i < M; i += 4) {
+            double ar = re[i], ai = im[i], br = re[i + 2], bi = im[i + 2];
+            re[i] = ar + br;
+            im[i] = ai + bi;
+            re[i + 2] = ar - br;
+            im[i + 2] = ai - bi;
+            ar = re[i + 1];
+            ai = im[i + 1];
+            const double vr = -sg * im[i + 3], vi = sg * re[i + 3];  // (br + i bi) * (sign i)
+            re[i + 1] = ar + vr;
+            im[i + 1] = ai + vi;
+            re[i + 3] = ar - vr;
+            im[i + 3] = ai - vi;
+        }
+    for (u32 half = 4; half < M; half <<= 1) {
+        const double *wr = tw_re + half, *wi = tw_im + half;
+        for (u32 i = 0; i < M; i += 2 * half) {
             double *ar = re + i, *ai = im + i, *br = ar + half, *bi = ai + half;
             for (u32 k = 0; k < half; ++k) {
-                const double wr = tw_re[k * step], wi = sg * tw_im[k * step];
-                const double vr = br[k] * wr - bi[k] * wi, vi = br[k] * wi + bi[k] * wr;
+                const double cr = wr[k], ci = sg * wi[k];
+                const double vr = br[k] * cr - bi[k] * ci, vi = br[k] * ci + bi[k] * cr;
                 br[k] = ar[k] - vr;
                 bi[k] = ai[k] - vi;
                 ar[k] += vr;
@@ -281,6 +463,22 @@ void dft_split(double *re, double *im, u32 M, int sign, const double *tw_re, con
             }
         }
     }
+}
+
+void dft_split_base(double *re, double *im, u32 M, int sign, const double *tw_re, const double *tw_im,
+                    const u32 *brev) {
+    dft_split_body(re, im, M, sign, tw_re, tw_im, brev);
+}
+__attribute__((target("avx2,fma"))) void dft_split_avx2(double *re, double *im, u32 M, int sign,
+                                                       const double *tw_re, const double *tw_im, const u32 *brev) {
+    dft_split_body(re, im, M, sign, tw_re, tw_im, brev);
+}
+void dft_split(const fhea_client *cl, double *re, double *im, int sign) {
+    const u32 M = cl->h.N / 2;
+    if (host_f64_ok())
+        dft_split_avx2(re, im, M, sign, cl->tw_re.data(), cl->tw_im.data(), cl->brev.data());
+    else
+        dft_split_base(re, im, M, sign, cl->tw_re.data(), cl->tw_im.data(), cl->brev.data());
 }
 
 bool valid_slots(u32 N, u32 n) { return n >= 1 && n <= N / 2 && (n & (n - 1)) == 0; }
@@ -313,7 +511,7 @@ int encode_one(const fhea_client *cl, const double *re, const double *im, u32 n,
         xr[u] = re[j % n] * scale;
         xi[u] = im ? im[j % n] * scale : 0.0;
     }
-    dft_split(xr.data(), xi.data(), M, -1, cl->tw_re.data(), cl->tw_im.data());
+    dft_split(cl, xr.data(), xi.data(), -1);
     // w_k zeta^-k / M = m_k + i m_{k+M}
     std::vector<double> m(N);
     const double inv = 1.0 / (double)M;
@@ -375,13 +573,25 @@ void decode_coeffs(const fhea_client *cl, const Crt &crt, const u64 *c, double s
                    double *im) {
     const u32 N = cl->h.N, M = N / 2;
     std::vector<double> xr(M), xi(M);
-    for (u32 k = 0; k < M; ++k) {
-        const double a = crt.value(c + k, N), b = crt.value(c + k + M, N);
-        const double zr = cl->zeta_re[k], zi = cl->zeta_im[k];
-        xr[k] = a * zr - b * zi;
-        xi[k] = a * zi + b * zr;
+    if (crt.l == 1) {  // one limb: the centred residue itself
+        const u64 q = crt.q[0], hq = (q - 1) / 2;
+        const double qd = (double)q;
+        for (u32 k = 0; k < M; ++k) {
+            const double a = (double)c[k] - (c[k] > hq ? qd : 0.0);
+            const double b = (double)c[k + M] - (c[k + M] > hq ? qd : 0.0);
+            const double zr = cl->zeta_re[k], zi = cl->zeta_im[k];
+            xr[k] = a * zr - b * zi;
+            xi[k] = a * zi + b * zr;
+        }
+    } else {
+        for (u32 k = 0; k < M; ++k) {
+            const double a = crt.value(c + k, N), b = crt.value(c + k + M, N);
+            const double zr = cl->zeta_re[k], zi = cl->zeta_im[k];
+            xr[k] = a * zr - b * zi;
+            xi[k] = a * zi + b * zr;
+        }
     }
-    dft_split(xr.data(), xi.data(), M, +1, cl->tw_re.data(), cl->tw_im.data());
+    dft_split(cl, xr.data(), xi.data(), +1);
     for (u32 j = 0; j < n; ++j) {
         const u32 u = cl->slot_u[j];
         re[j] = xr[u] / scale;
@@ -393,13 +603,14 @@ void decode_coeffs(const fhea_client *cl, const Crt &crt, const u64 *c, double s
 
 void fhea_client::init_embedding() {
     const u32 N = h.N, M = N / 2;
-    tw_re.resize(M / 2);
-    tw_im.resize(M / 2);
-    for (u32 k = 0; k < M / 2; ++k) {
-        const double a = 2.0 * kPi * (double)k / (double)M;
-        tw_re[k] = std::cos(a);
-        tw_im[k] = std::sin(a);
-    }
+    tw_re.assign(M, 0.0);
+    tw_im.assign(M, 0.0);
+    for (u32 half = 1; half < M; half <<= 1)
+        for (u32 k = 0; k < half; ++k) {
+            const double a = kPi * (double)k / (double)half;  // 2 pi k / (2 half)
+            tw_re[half + k] = std::cos(a);
+            tw_im[half + k] = std::sin(a);
+        }
     zeta_re.resize(M);
     zeta_im.resize(M);
     for (u32 k = 0; k < M; ++k) {
@@ -407,6 +618,10 @@ void fhea_client::init_embedding() {
         zeta_re[k] = std::cos(b);
         zeta_im[k] = std::sin(b);
     }
+    brev.resize(M);
+    u32 lm = 0;
+    while ((1u << lm) < M) ++lm;
+    for (u32 i = 0; i < M; ++i) brev[i] = bit_reverse(i, lm);
     slot_u.resize(M);
     for (u32 j = 0; j < M; ++j) slot_u[j] = (u32)((pow_mod(5, j, 2ull * N) - 1) / 4);
 }
@@ -511,6 +726,13 @@ int fhea_client_table(const fhea_client *cl, int which, uint64_t *out, uint64_t 
     if (!v) return FHEA_ERR_ARG;
     if (count != v->size()) return FHEA_ERR_RANGE;
     std::memcpy(out, v->data(), count * 8);
+    return FHEA_OK;
+}
+
+int fhea_client_set_host_ntt(fhea_client *cl, int mode) {
+    if (!cl) return FHEA_ERR_STATE;
+    if (mode != 0 && mode != 1) return FHEA_ERR_ARG;
+    cl->h.host_ntt_mode = (u32)mode;
     return FHEA_OK;
 }
 
@@ -682,7 +904,11 @@ int fhea_decrypt_decode_batch(const fhea_client *cl, const uint64_t *s_eval, con
             for (u32 i = 0; i < limbs; ++i) {
                 const u64 q = h.primes[i];
                 const size_t o = (size_t)i * N;
-                for (u32 j = 0; j < N; ++j) c[o + j] = add_mod(c0[o + j], mulmod(c1[o + j], s_eval[o + j], mq[i]), q);
+                if (ntt.f64(q))
+                    dec_mul_f64(c.data() + o, c0 + o, c1 + o, s_eval + o, N, q);
+                else
+                    for (u32 j = 0; j < N; ++j)
+                        c[o + j] = add_mod(c0[o + j], mulmod(c1[o + j], s_eval[o + j], mq[i]), q);
                 ntt.inverse(c.data() + o, i);
             }
             decode_coeffs(cl, crt, c.data(), scale, n, re + (size_t)k * n, im ? im + (size_t)k * n : nullptr);

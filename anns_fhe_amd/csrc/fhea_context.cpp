@@ -100,6 +100,8 @@ void build_host_tables(HostTables &h) {
     h.itw.assign((size_t)T * N, 0);
     h.tw_sh.assign((size_t)T * N, 0);
     h.itw_sh.assign((size_t)T * N, 0);
+    h.f64tw = std::make_shared<HostF64Twiddles>();
+    h.f64tw->p.reset(new HostF64Twiddles::Prime[T]);
     for (u32 i = 0; i < T; ++i) {
         u64 q = h.primes[i];
         u64 psi = h.psi_given.empty() ? min_primitive_root(q, 2 * N) : h.psi_given[i];

@@ -3,6 +3,8 @@
 #pragma once
 #include <stdint.h>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,12 +57,26 @@ struct LevelTables {
     const double *rs_pqf = nullptr;        // [2*(l-1)] ((P q_l)^-1 mod q_i, / q_i)
 };
 
+// Twiddles of the CPU FP64 transforms as doubles (w and w/q, forward and inverse),
+// built per prime on first use.
+struct HostF64Twiddles {
+    struct Prime {
+        std::once_flag once;
+        std::vector<double> w, wq, iw, iwq;  // [N] each, the order of HostTables::tw / itw
+    };
+    std::unique_ptr<Prime[]> p;
+};
+
 struct HostTables {
     u32 N = 0, logN = 0, L = 0, K = 0, dnum = 0, alpha = 0;
     std::vector<u64> primes, psi, ninv;
     std::vector<u64> psi_given;  // explicit roots from the parameter block, if any
     std::vector<u64> tw, itw;  // [T*N] plain (for export / CPU client)
     std::vector<u64> tw_sh, itw_sh;  // [T*N] Shoup companions of tw / itw (CPU transforms)
+    // CPU transforms: 0 = FP64 (exact, AVX2+FMA) where q < 2^50 and the CPU has it,
+    // else 64-bit integer; 1 = always 64-bit integer (same words either way)
+    u32 host_ntt_mode = 0;
+    std::shared_ptr<HostF64Twiddles> f64tw;  // set with the tables; copies share it
 };
 
 struct RotKey {

@@ -131,21 +131,28 @@ class PlaintextSearcher:
             dist.append(d)
         cand = np.concatenate(cand) if cand else np.zeros(0, np.int64)
         dist = np.concatenate(dist) if dist else np.zeros(0)
-        order = np.argsort(dist, kind="stable")[:top_k]
+        order = topk_stable(dist, top_k)
         return cand[order], dist[order], probes
 
 
-def exact_knn(x, q, k):
-    """The first k of a stable argsort of the squared distances (float64) from q to
-    the rows of x: the k smallest by partition, ties broken by index.  Pass x as
-    float64 to avoid a conversion per call."""
-    d = ((np.asarray(x, np.float64) - np.asarray(q, np.float64)) ** 2).sum(1)
+def topk_stable(d, k):
+    """np.argsort(d, kind="stable")[:k] without sorting all of d: the k smallest by
+    partition, ties broken by index."""
+    d = np.asarray(d)
     if k >= len(d):
         return np.argsort(d, kind="stable")[:k]
+    if k <= 0:
+        return np.zeros(0, np.int64)
     t = np.partition(d, k - 1)[k - 1]
     below = np.flatnonzero(d < t)
     cand = np.concatenate([below, np.flatnonzero(d == t)[:k - len(below)]])
     return cand[np.argsort(d[cand], kind="stable")]
+
+
+def exact_knn(x, q, k):
+    """The first k of a stable argsort of the squared distances (float64) from q to
+    the rows of x.  Pass x as float64 to avoid a conversion per call."""
+    return topk_stable(((np.asarray(x, np.float64) - np.asarray(q, np.float64)) ** 2).sum(1), k)
 
 
 def recall_at(found, truth):
@@ -510,7 +517,7 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
             dist.append(dec[g][:len(cand)])
         ids = np.concatenate(ids) if ids else np.zeros(0, np.int64)
         dist = np.concatenate(dist) if dist else np.zeros(0)
-        order = np.argsort(dist, kind="stable")[:top_k]
+        order = topk_stable(dist, top_k)
         return ids[order], dist[order], probes[z]
 
     out = list(pool.map(top, slots))

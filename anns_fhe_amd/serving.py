@@ -172,7 +172,7 @@ class ServingLoop:
                 dist.append(dec[g][:len(cand)])
             ids = np.concatenate(ids)
             dist = np.concatenate(dist)
-            order = np.argsort(dist, kind="stable")[:10]
+            order = IV.topk_stable(dist, 10)
             r.ids, r.dist = ids[order], dist[order]
             r.t_done = now()
             with lock:

@@ -300,6 +300,10 @@ int fhea_client_info(const fhea_client *cl, uint32_t *info5);  /* N, L, K, dnum,
 int fhea_client_primes(const fhea_client *cl, uint64_t *out);   /* L+K primes        */
 /* which: 0 psi (L+K), 1 N^-1 (L+K), 2 twiddles, 3 inverse twiddles ((L+K)*N); as fhea_ctx_table */
 int fhea_client_table(const fhea_client *cl, int which, uint64_t *out, uint64_t count);
+/* CPU transforms of this client: 0 = FP64 (exact products, AVX2+FMA) for primes below
+ * 2^50 when the CPU has it, else 64-bit integer (default); 1 = always 64-bit integer.
+ * Both give the same words (no reference counterpart: OpenFHE picks its own). */
+int fhea_client_set_host_ntt(fhea_client *cl, int mode);
 int fhea_client_keygen_secret(const fhea_client *cl, uint64_t seed, int8_t *s_out);
 /* evaluation keys into a host buffer of fhea_key_size_words words, same layout as the arena */
 int fhea_client_switch_key(const fhea_client *cl, const int8_t *s, const int64_t *s_new,

@@ -126,6 +126,35 @@ def test_batch_decrypt_decode_equals_single_calls():
         cl.decrypt_values_batch(cl.secret_eval(s, 1), cts, 3, 2.0 ** 35, N // 2)
 
 
+@pytest.mark.parametrize("N,q_bits", [(4096, 50), (16384, 49), (1024, 60)])
+def test_fp64_host_transforms_equal_integer_ones(orc, N, q_bits):
+    """The CPU transforms on exact FP64 products (AVX2+FMA, primes below 2^50) give
+    the words of the 64-bit integer ones: the secret's evaluation form (forward),
+    the plaintexts of encode (forward) and decrypt+decode (inverse) agree exactly
+    between the two modes, and the forward form equals the oracle's NTT."""
+    L = 3
+    cl = _client(N, L, 1, 3, q_bits=q_bits)
+    o = orc.OracleCtx(N, cl.q, cl.p, 3)
+    s = cl.keygen_secret(31)
+    pk = cl.keygen_public(s, 32)
+    rng = np.random.default_rng(33)
+    z = rng.uniform(-2, 2, N // 2)
+    out = {}
+    for mode in (1, 0):
+        cl.set_host_ntt(mode)
+        se = cl.secret_eval(s)
+        pt = cl.encode(z, L, 2.0 ** 35)
+        ct = cl.encrypt(pk, pt, L, 34)
+        out[mode] = (se, pt, ct, cl.decrypt_values_batch(se, ct, L, 2.0 ** 35, N // 2))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    sc = np.stack([np.array([int(v) % q for v in s], np.uint64) for q in cl.q])
+    assert np.array_equal(out[0][0].reshape(L, N), o.ntt_poly(sc))
+    assert np.max(np.abs(out[0][3][0] - z)) < 2.0 ** -15
+    with pytest.raises(Exception):
+        cl.set_host_ntt(2)
+
+
 def test_hmult_and_rotation_semantics_through_oracle(orc):
     """Client keys + oracle evaluation: products and rotations decrypt to the
     expected slot vectors (the GPU path equals the oracle word for word)."""

@@ -128,3 +128,14 @@ def test_reference_query_sequence_many_candidates(orc):
     assert sum(len(l) for l in ix.lists) > N // 2
     assert np.allclose(dist, pdist, atol=1e-3)
     assert set(ids) == set(pids)
+
+
+def test_topk_stable_equals_stable_argsort():
+    """The partition-based top-k (client top-10, exact kNN) returns the first k of a
+    stable argsort, ties at the boundary included."""
+    from anns_fhe_amd import ivfpq as IV
+    rng = np.random.default_rng(8)
+    for n in (1, 5, 37, 5000):
+        for d in (rng.normal(size=n), rng.integers(0, 4, n).astype(np.float64), np.zeros(n)):
+            for k in (0, 1, 3, 10, n, n + 2):
+                assert np.array_equal(IV.topk_stable(d, k), np.argsort(d, kind="stable")[:k]), (n, k)

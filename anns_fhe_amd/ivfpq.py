@@ -382,9 +382,10 @@ class FheIvfPqServer:
                               for z, G in sizes.items()])
         self.ctx.rescale_scatter(self.out_base, res, lt)
 
-    def read_distances_batch(self, items):
+    def read_distances_batch(self, items, copy=True):
         """{z: (G, 2*(lt-1)*N) array} for items [(z, G), ...]: one gathered read into
-        page-locked memory, then copies out on host threads."""
+        page-locked memory, then copies out on host threads.  copy=False returns views
+        of the page-locked buffer instead, valid until the next read."""
         n = 2 * (self.lt - 1) * self.N
         if not hasattr(self.ctx, "read_polys"):
             return {z: self.read_distances(z, G) for z, G in items}
@@ -404,6 +405,8 @@ class FheIvfPqServer:
         for z, G in items:
             parts.append((z, at, G))
             at += G * n
+        if not copy:
+            return {z: buf[a:a + G * n].reshape(G, n) for z, a, G in parts}
         if self._copy_pool is None:
             self._copy_pool = cf.ThreadPoolExecutor(8)
         return dict(self._copy_pool.map(lambda p: (p[0], buf[p[1]:p[1] + p[2] * n].reshape(p[2], n).copy()), parts))
@@ -488,7 +491,7 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     slots = list(range(B))
     t1 = time.perf_counter()
     server.stage1_coarse_centroid_distance(slots)
-    cmap = server.read_distances_batch([(z, 1) for z in slots])
+    cmap = server.read_distances_batch([(z, 1) for z in slots], copy=False)
     coarse = [cmap[z][0] for z in slots]
     t["server_stage1"] = time.perf_counter() - t1
     # stage 2 (client): decrypt the coarse distances, pick n_probe clusters
@@ -503,7 +506,8 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     t3 = time.perf_counter()
     bmap = server.stage3_fine_candidate_distances([(z, probes[z]) for z in slots])
     blocks = [bmap[z] for z in slots]
-    fmap = server.read_distances_batch([(z, len(blocks[z])) for z in slots if blocks[z]])
+    # views of the read buffer: every client is done with them before the next read
+    fmap = server.read_distances_batch([(z, len(blocks[z])) for z in slots if blocks[z]], copy=False)
     fine = [fmap.get(z) for z in slots]
     t["server_stage3"] = time.perf_counter() - t3
     t4 = time.perf_counter()

@@ -919,6 +919,13 @@ int fhea_ctx_destroy(fhea_ctx *c) {
     if (c->cap_dev) hipFree(c->cap_dev);
     if (c->cap_host) hipHostFree(c->cap_host);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->copy_stream) {
+        hipStreamSynchronize(c->copy_stream);
+        hipStreamDestroy(c->copy_stream);
+        for (hipEvent_t &e : c->read_done)
+            if (e) hipEventDestroy(e);
+        if (c->read_dep) hipEventDestroy(c->read_dep);
+    }
     for (auto &kv : c->graphs) {
         if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) hipGraphDestroy(kv.second.graph);
@@ -1031,6 +1038,46 @@ int fhea_read_polys(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t
     }
     return check_hip(hipStreamSynchronize(c->stream));
 }
+int fhea_read_polys_async(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h,
+                          uint32_t *ticket) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    if (!off || !count || (!h && n) || !ticket) return FHEA_ERR_ARG;
+    for (u32 k = 0; k < n; ++k)
+        if (!range_ok(off[k], count[k], c->poly_words)) return FHEA_ERR_RANGE;
+    if (!c->copy_stream) {
+        if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->read_dep, hipEventDisableTiming) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        for (hipEvent_t &e : c->read_done)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return FHEA_ERR_DEVICE;
+    }
+    const u32 seq = c->read_seq;
+    hipEvent_t done = c->read_done[seq % fhea_ctx::READ_RING];
+    // a ticket's event is reused READ_RING reads later: that read must be complete
+    if (seq >= fhea_ctx::READ_RING && check_hip(hipEventSynchronize(done)) != FHEA_OK) return FHEA_ERR_DEVICE;
+    if (hipEventRecord(c->read_dep, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->copy_stream, c->read_dep, 0) != hipSuccess)
+        return FHEA_ERR_DEVICE;
+    u64 at = 0;
+    for (u32 k = 0; k < n; ++k) {
+        if (count[k] && hipMemcpyAsync(h + at, c->poly + off[k], count[k] * 8, hipMemcpyDeviceToHost,
+                                       c->copy_stream) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        at += count[k];
+    }
+    if (hipEventRecord(done, c->copy_stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    *ticket = seq;
+    c->read_seq = seq + 1;
+    return FHEA_OK;
+}
+
+int fhea_read_wait(fhea_ctx *c, uint32_t ticket) {
+    CTX_CHECK(c);
+    if (ticket >= c->read_seq || c->read_seq - ticket > fhea_ctx::READ_RING) return FHEA_ERR_ARG;
+    return check_hip(hipEventSynchronize(c->read_done[ticket % fhea_ctx::READ_RING]));
+}
+
 int fhea_write_key(fhea_ctx *c, uint64_t off, const uint64_t *h, uint64_t n) {
     return c ? arena_copy(c, c->key, c->key_words, off, (void *)h, n, true) : FHEA_ERR_STATE;
 }

@@ -71,6 +71,14 @@ struct fhea_ctx {
     u64 ws2_words = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
+    // reads in flight (fhea_read_polys_async): copies on their own stream, after an
+    // event of `stream`, each ticket's completion event in a ring
+    static constexpr u32 READ_RING = 16;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t read_dep = nullptr;
+    hipEvent_t read_done[READ_RING] = {};
+    u32 read_seq = 0;
+
     u32 max_chunk = 16;
     bool force_int = false;  // FHEA_FORCE_INT=1: integer kernels even for < 2^50 primes
     u32 f64_off = 0;         // FHEA_F64_OFF: per-component switch (DevTables::f64_off)

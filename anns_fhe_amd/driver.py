@@ -115,6 +115,22 @@ class FheContext:
         check(self.lib.fhea_read_polys(self.h, len(ranges), _p64(offs), _p64(cnts), _p64(a)), "read_polys")
         return a[:total]
 
+    def read_polys_async(self, ranges, out):
+        """read_polys into `out` (page-locked, e.g. a PinnedArray's array or a slice
+        of it), returned at once: the copies run on the context's copy stream after
+        the work submitted so far.  Returns a ticket for read_wait; `out` must stay
+        alive and the ranges unwritten until then."""
+        offs = np.array([r[0] for r in ranges], np.uint64)
+        cnts = np.array([r[1] for r in ranges], np.uint64)
+        assert out.dtype == np.uint64 and out.size >= int(cnts.sum()) and out.flags["C_CONTIGUOUS"]
+        t = C.c_uint32()
+        check(self.lib.fhea_read_polys_async(self.h, len(ranges), _p64(offs), _p64(cnts), _p64(out), C.byref(t)),
+              "read_polys_async")
+        return t.value
+
+    def read_wait(self, ticket):
+        check(self.lib.fhea_read_wait(self.h, ticket), "read_wait")
+
     def write_key(self, off, data):
         a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
         check(self.lib.fhea_write_key(self.h, off, _p64(a), a.size), "write_key")

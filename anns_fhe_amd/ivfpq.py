@@ -207,6 +207,17 @@ class FheIvfPqClient:
         return self.ck.decrypt_values_batch(self.s_eval, cts, limbs, scale, n, threads)
 
 
+class _HostArray:
+    """Pageable stand-in for PinnedArray (contexts without a device)."""
+
+    def __init__(self, words):
+        self.words = words
+        self.array = np.zeros(words, np.uint64)
+
+    def close(self):
+        self.array = None
+
+
 class FheIvfPqServer:
     """Server side: the encoded database resident in HBM, keys, and the batched
     homomorphic stages.  Word layout of the poly arena:
@@ -382,6 +393,21 @@ class FheIvfPqServer:
                               for z, G in sizes.items()])
         self.ctx.rescale_scatter(self.out_base, res, lt)
 
+    def _result_buffer(self, words):
+        """The page-locked read buffer, at least `words` long (page-locking is slow:
+        it grows by half again each time)."""
+        if self._pinned_all is None or self._pinned_all.words < words:
+            size = max(words, self.max_queries * 4 * 2 * (self.lt - 1) * self.N)
+            if self._pinned_all is not None:
+                size = max(size, self._pinned_all.words * 3 // 2)
+                self._pinned_all.close()
+            if hasattr(self.ctx, "graph_begin"):  # a device context: page-locked
+                from .driver import PinnedArray
+                self._pinned_all = PinnedArray(size)
+            else:
+                self._pinned_all = _HostArray(size)
+        return self._pinned_all.array
+
     def read_distances_batch(self, items, copy=True):
         """{z: (G, 2*(lt-1)*N) array} for items [(z, G), ...]: one gathered read into
         page-locked memory, then copies out on host threads.  copy=False returns views
@@ -389,25 +415,23 @@ class FheIvfPqServer:
         n = 2 * (self.lt - 1) * self.N
         if not hasattr(self.ctx, "read_polys"):
             return {z: self.read_distances(z, G) for z, G in items}
-        import concurrent.futures as cf
-        from .driver import PinnedArray
         ranges = [(self._offs(z)["res"], G * n) for z, G in items]
         total = sum(c for _, c in ranges)
-        if self._pinned_all is None or self._pinned_all.words < total:
-            # page-locking is slow: grow by half again each time
-            size = max(total, self.max_queries * 4 * n)
-            if self._pinned_all is not None:
-                size = max(size, self._pinned_all.words * 3 // 2)
-                self._pinned_all.close()
-            self._pinned_all = PinnedArray(size)
-        buf = self.ctx.read_polys(ranges, out=self._pinned_all.array)
+        buf = self.ctx.read_polys(ranges, out=self._result_buffer(total))
         at, parts = 0, []
         for z, G in items:
             parts.append((z, at, G))
             at += G * n
+        return self._hand_out(buf, parts, copy)
+
+    def _hand_out(self, buf, parts, copy):
+        """{z: (G, n) results} from [(z, word offset in buf, G)]: views of buf, or copies
+        made on host threads."""
+        n = 2 * (self.lt - 1) * self.N
         if not copy:
             return {z: buf[a:a + G * n].reshape(G, n) for z, a, G in parts}
         if self._copy_pool is None:
+            import concurrent.futures as cf
             self._copy_pool = cf.ThreadPoolExecutor(8)
         return dict(self._copy_pool.map(lambda p: (p[0], buf[p[1]:p[1] + p[2] * n].reshape(p[2], n).copy()), parts))
 
@@ -450,6 +474,48 @@ class FheIvfPqServer:
         """Candidate distances of several queries in one launch: probes is a list of
         (z, clusters); returns {z: candidate block ids}."""
         blocks = {z: [b for c in clusters for b in self.cluster_blocks[c]] for z, clusters in probes}
+        self._stage3_launch(probes, blocks)
+        return blocks
+
+    def stage3_and_read(self, probes, chunks=2, copy=True):
+        """Stage 3 and the read of its results: ({z: candidate block ids}, {z: (G, n)
+        results}).  The queries go in `chunks` parts of about equal result counts, each
+        launched and then read on the copy stream, so that a part's read-back overlaps
+        the next part's sums (a cluster's plaintext words are then read once per part).
+        copy=False returns views of the page-locked buffer, valid until the next read."""
+        n = 2 * (self.lt - 1) * self.N
+        if chunks <= 1 or not hasattr(self.ctx, "read_polys_async"):
+            blocks = self.stage3_fine_candidate_distances(probes)
+            return blocks, self.read_distances_batch([(z, len(b)) for z, b in blocks.items() if b], copy=copy)
+        blocks = {z: [b for c in clusters for b in self.cluster_blocks[c]] for z, clusters in probes}
+        total = sum(len(b) for b in blocks.values())
+        buf = self._result_buffer(total * n)
+        parts, cur, acc = [], [], 0
+        for z, clusters in probes:
+            cur.append((z, clusters))
+            acc += len(blocks[z])
+            if acc * chunks >= total * (len(parts) + 1) and len(parts) < chunks - 1:
+                parts.append(cur)
+                cur = []
+        if cur:
+            parts.append(cur)
+        where, tickets, at = [], [], 0
+        for part in parts:
+            self._stage3_launch(part, {z: blocks[z] for z, _ in part})
+            items = [(z, len(blocks[z])) for z, _ in part if blocks[z]]
+            if not items:
+                continue
+            ranges = [(self._offs(z)["res"], G * n) for z, G in items]
+            cnt = sum(c for _, c in ranges)
+            tickets.append(self.ctx.read_polys_async(ranges, buf[at:at + cnt]))
+            for z, G in items:
+                where.append((z, at, G))
+                at += G * n
+        for tk in tickets:
+            self.ctx.read_wait(tk)
+        return blocks, self._hand_out(buf, where, copy)
+
+    def _stage3_launch(self, probes, blocks):
         # one group per probed cluster: the queries probing it x its blocks
         members = {}
         for z, clusters in probes:
@@ -463,14 +529,14 @@ class FheIvfPqServer:
         groups = [(zs, self.cluster_blocks[c], st) for c, (zs, st) in members.items()]
         if groups:
             self._distances_groups(groups, {z: len(b) for z, b in blocks.items() if b})
-        return blocks
 
 
-def execute_interactive_queries(server, client, queries, n_probe, top_k, client_threads=16):
+def execute_interactive_queries(server, client, queries, n_probe, top_k, client_threads=16, stage3_chunks=2):
     """Runs the whole protocol for a batch of queries; returns per query
     (ids, distances, probes) and a timing record.  Client-side work (encryption,
     decryption, selection) runs on `client_threads` host threads, as independent
-    clients would."""
+    clients would; stage 3 goes in `stage3_chunks` parts whose result reads overlap
+    the next part's evaluation."""
     import concurrent.futures as cf
     ck, N = client.ck, server.N
     B = len(queries)
@@ -504,10 +570,9 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     probes = list(pool.map(choose, slots))
     t["client_stage2"] = time.perf_counter() - t2
     t3 = time.perf_counter()
-    bmap = server.stage3_fine_candidate_distances([(z, probes[z]) for z in slots])
-    blocks = [bmap[z] for z in slots]
     # views of the read buffer: every client is done with them before the next read
-    fmap = server.read_distances_batch([(z, len(blocks[z])) for z in slots if blocks[z]], copy=False)
+    bmap, fmap = server.stage3_and_read([(z, probes[z]) for z in slots], chunks=stage3_chunks, copy=False)
+    blocks = [bmap[z] for z in slots]
     fine = [fmap.get(z) for z in slots]
     t["server_stage3"] = time.perf_counter() - t3
     t4 = time.perf_counter()

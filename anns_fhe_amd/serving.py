@@ -203,9 +203,8 @@ class ServingLoop:
                             r.events["stage1"] = now()
                             pool.submit(client_stage2, r, coarse)
                     if b3:
-                        bmap = srv.stage3_fine_candidate_distances([(r.slot, r.probes) for r in b3])
+                        bmap, fmap = srv.stage3_and_read([(r.slot, r.probes) for r in b3], chunks=2)
                         outs = [bmap[r.slot] for r in b3]
-                        fmap = srv.read_distances_batch([(r.slot, len(b)) for r, b in zip(b3, outs) if b])
                         for r, blocks in zip(b3, outs):
                             fine = fmap.get(r.slot)
                             r.events["stage3"] = now()

@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
+    ap.add_argument("--stage3-chunks", type=int, default=2,
+                    help="ivfpq: stage-3 parts whose result reads overlap the next part's evaluation")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="plain_path runs (hmult: single-ciphertext op-by-op HMults; serve: reference "
@@ -254,7 +256,7 @@ def bench_ivfpq(args):
         return [x[rng.integers(n)] + rng.normal(0, 0.05, 128).astype(np.float32) for _ in range(B)]
 
     for _ in range(args.warmup):
-        IV.execute_interactive_queries(server, client, batch(), n_probe, 10)
+        IV.execute_interactive_queries(server, client, batch(), n_probe, 10, stage3_chunks=args.stage3_chunks)
     log("ivfpq: warmup done")
     ps = IV.PlaintextSearcher(ix)
     import concurrent.futures as cf
@@ -264,7 +266,7 @@ def bench_ivfpq(args):
     for step in range(args.steps):
         qs = batch()
         w0 = time.perf_counter()
-        res, t = IV.execute_interactive_queries(server, client, qs, n_probe, 10)
+        res, t = IV.execute_interactive_queries(server, client, qs, n_probe, 10, stage3_chunks=args.stage3_chunks)
         lat.append(time.perf_counter() - w0)
         server_s.append(t["server_stage1"] + t["server_stage3"])
         log(f"ivfpq: step {step}: batch {lat[-1] * 1e3:.1f} ms, " +

@@ -107,6 +107,15 @@ int fhea_read_poly(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t co
 /* n arena ranges [off[k], off[k] + count[k]) copied back to back into h (one
  * synchronisation; page-locked h from fhea_host_alloc makes the copies DMA). */
 int fhea_read_polys(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h);
+/* The same gathered read, returned at once: the copies run on a copy stream after
+ * the work submitted so far, beside the kernels submitted next; *ticket names the
+ * read for fhea_read_wait, which must come within 16 reads.  Work submitted before
+ * the wait must not write the ranges; h must stay valid until then (page-locked h
+ * lets the copies overlap kernels).  No reference counterpart (reads overlapped with
+ * later evaluation, for the batched query stages). */
+int fhea_read_polys_async(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint64_t *count,
+                          uint64_t *h, uint32_t *ticket);
+int fhea_read_wait(fhea_ctx *ctx, uint32_t ticket);
 int fhea_write_key(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint64_t count);
 int fhea_read_key(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
 int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t count);

@@ -39,6 +39,24 @@ class OracleContext:
     def read_poly(self, off, count):
         return self.pm[off:off + count].copy()
 
+    def read_polys(self, ranges, out=None):
+        total = sum(c for _, c in ranges)
+        a = np.zeros(total, np.uint64) if out is None else out
+        at = 0
+        for off, c in ranges:
+            a[at:at + c] = self.pm[off:off + c]
+            at += c
+        return a[:total]
+
+    def read_polys_async(self, ranges, out):
+        # the oracle runs every call to completion: the read is done at once
+        self.read_polys(ranges, out)
+        self._tickets = getattr(self, "_tickets", 0) + 1
+        return self._tickets - 1
+
+    def read_wait(self, ticket):
+        assert 0 <= ticket < getattr(self, "_tickets", 0)
+
     def write_key(self, off, data):
         a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
         self.km[off:off + a.size] = a

@@ -589,6 +589,39 @@ def test_workload_tuner_keeps_results(orc):
     AT.apply(ctx, {"ntt_split_fwd": 0, "ntt_split_inv": 0, "ntt_wg_per_cu": 0, "chunk": 16, "streams": 1})
 
 
+def test_async_reads_overlap_later_work():
+    """fhea_read_polys_async: gathered reads on the copy stream equal the synchronous
+    read, also while later kernels (NTTs on other words) run; tickets may be waited
+    out of order; a ticket more than 16 reads old, or not yet issued, is refused."""
+    from anns_fhe_amd._lib import FheError
+    from anns_fhe_amd.driver import PinnedArray
+    N, L = 8192, 3
+    ctx = get_ctx(N, L, 1, 1)
+    rng = np.random.default_rng(41)
+    W = L * N
+    data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(8)])
+    ctx.write_poly(0, data)
+    ranges = [(0, 100), (3 * W + 17, W), (5 * W, 2 * W), (W + 8, 1)]
+    ref = ctx.read_polys(ranges)
+    pin = PinnedArray(4 * ref.size)
+    tickets = []
+    for k in range(3):
+        say("async read", k)
+        tickets.append(ctx.read_polys_async(ranges, pin.array[k * ref.size:(k + 1) * ref.size]))
+        ctx.ntt_batch(8 * W, 8 * W, 2, L)  # later work on words no read touches
+    for k in (2, 0, 1):
+        ctx.read_wait(tickets[k])
+        assert np.array_equal(pin.array[k * ref.size:(k + 1) * ref.size], ref), k
+    with pytest.raises(FheError):
+        ctx.read_wait(tickets[-1] + 1)
+    for _ in range(17):
+        t = ctx.read_polys_async(ranges[:1], pin.array[3 * ref.size:])
+    ctx.read_wait(t)
+    with pytest.raises(FheError):
+        ctx.read_wait(tickets[0])
+    pin.close()
+
+
 @pytest.mark.parametrize("q_bits", [50, 60])
 def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
     """fhea_eval_linear_sum_batch (jobs of different G sharing plaintext blocks, with

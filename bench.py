@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
-    ap.add_argument("--stage3-chunks", type=int, default=2,
+    ap.add_argument("--stage3-chunks", type=int, default=4,
                     help="ivfpq: stage-3 parts whose result reads overlap the next part's evaluation")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
     ap.add_argument("--plain-steps", type=int, default=3,
@@ -291,7 +291,7 @@ def bench_ivfpq(args):
                                "(coarse distances, client cluster choice, candidate distances, client top-10)",
                    "db": n, "dim": 128, "n_list": 32, "M": 8, "K": 256, "N": Nq, "L": Lq, "K_special": Kq,
                    "dnum": dq, "eval_limbs": lt, "n_probe": n_probe, "batch": B, "blocks": n_blocks,
-                   "stage1_graph": server.use_graphs},
+                   "stage1_graph": server.use_graphs, "stage3_chunks": args.stage3_chunks},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2),
                        "what": "end-to-end batch (client encrypt/decrypt + server), per query = per batch"},

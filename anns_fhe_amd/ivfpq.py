@@ -260,7 +260,7 @@ class FheIvfPqServer:
         self.out_base = self.pt_words + max_queries * self.slot_words
         self.max_queries = max_queries
         self._pinned = None  # page-locked read buffers (engine contexts)
-        self._pinned_all = None
+        self._bufs = {}
         self._copy_pool = None
         self.use_graphs = False
         self.max_graphs = 16
@@ -393,20 +393,23 @@ class FheIvfPqServer:
                               for z, G in sizes.items()])
         self.ctx.rescale_scatter(self.out_base, res, lt)
 
-    def _result_buffer(self, words):
-        """The page-locked read buffer, at least `words` long (page-locking is slow:
-        it grows by half again each time)."""
-        if self._pinned_all is None or self._pinned_all.words < words:
+    def _result_buffer(self, words, which="main"):
+        """Page-locked read buffer `which` ("main": stage 3 and gathered reads; "read":
+        reads queued beside them), at least `words` long (page-locking is slow: a
+        buffer grows by half again each time)."""
+        cur = self._bufs.get(which)
+        if cur is None or cur.words < words:
             size = max(words, self.max_queries * 4 * 2 * (self.lt - 1) * self.N)
-            if self._pinned_all is not None:
-                size = max(size, self._pinned_all.words * 3 // 2)
-                self._pinned_all.close()
+            if cur is not None:
+                size = max(size, cur.words * 3 // 2)
+                cur.close()
             if hasattr(self.ctx, "graph_begin"):  # a device context: page-locked
                 from .driver import PinnedArray
-                self._pinned_all = PinnedArray(size)
+                cur = PinnedArray(size)
             else:
-                self._pinned_all = _HostArray(size)
-        return self._pinned_all.array
+                cur = _HostArray(size)
+            self._bufs[which] = cur
+        return cur.array
 
     def read_distances_batch(self, items, copy=True):
         """{z: (G, 2*(lt-1)*N) array} for items [(z, G), ...]: one gathered read into
@@ -483,10 +486,16 @@ class FheIvfPqServer:
         launched and then read on the copy stream, so that a part's read-back overlaps
         the next part's sums (a cluster's plaintext words are then read once per part).
         copy=False returns views of the page-locked buffer, valid until the next read."""
-        n = 2 * (self.lt - 1) * self.N
         if chunks <= 1 or not hasattr(self.ctx, "read_polys_async"):
             blocks = self.stage3_fine_candidate_distances(probes)
             return blocks, self.read_distances_batch([(z, len(b)) for z, b in blocks.items() if b], copy=copy)
+        pending = self.stage3_submit(probes, chunks)
+        return pending["blocks"], self.collect(pending, copy)
+
+    def stage3_submit(self, probes, chunks=2):
+        """stage3_and_read without the wait: launches the parts, queues their reads and
+        returns a pending record for collect()."""
+        n = 2 * (self.lt - 1) * self.N
         blocks = {z: [b for c in clusters for b in self.cluster_blocks[c]] for z, clusters in probes}
         total = sum(len(b) for b in blocks.values())
         buf = self._result_buffer(total * n)
@@ -511,9 +520,27 @@ class FheIvfPqServer:
             for z, G in items:
                 where.append((z, at, G))
                 at += G * n
-        for tk in tickets:
+        return {"blocks": blocks, "buf": buf, "where": where, "tickets": tickets}
+
+    def read_distances_submit(self, items):
+        """Queues the read of results [(z, G), ...] (e.g. stage 1's) into a buffer of its
+        own on the copy stream; returns a pending record for collect()."""
+        n = 2 * (self.lt - 1) * self.N
+        ranges = [(self._offs(z)["res"], G * n) for z, G in items]
+        buf = self._result_buffer(sum(c for _, c in ranges), which="read")
+        where, at = [], 0
+        for z, G in items:
+            where.append((z, at, G))
+            at += G * n
+        tickets = [self.ctx.read_polys_async(ranges, buf)] if ranges else []
+        return {"buf": buf, "where": where, "tickets": tickets}
+
+    def collect(self, pending, copy=True):
+        """Waits for a pending record's reads: {z: (G, n) results} (views of the buffer,
+        valid until its next use, or copies)."""
+        for tk in pending["tickets"]:
             self.ctx.read_wait(tk)
-        return blocks, self._hand_out(buf, where, copy)
+        return self._hand_out(pending["buf"], pending["where"], copy)
 
     def _stage3_launch(self, probes, blocks):
         # one group per probed cluster: the queries probing it x its blocks

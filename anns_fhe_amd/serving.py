@@ -14,7 +14,10 @@ FPGA path is strictly sequential, cpp_host_model/src/interactive_query_manager.c
   since its last launch goes through stage 1 together (the rotations of all of
   their queries are one batched key-switch submission), every request whose
   client round trip has returned goes through stage 3 together.  Requests that
-  arrive while a batch is on the GPU join the next launch.  The two sets' work
+  arrive while a batch is on the GPU join the next launch.  Both stages are
+  launched before either's results are read back; the reads run on the context's
+  copy stream, so stage 1's results reach their clients while stage 3 still runs,
+  and stage 3's parts are read while the next part's sums run.  The two sets' work
   overlaps on the device (separate streams), and the client side (decrypt,
   cluster choice, top-k) runs on a thread pool, overlapping the GPU.
 * Latency of a request = completion of its top-k minus its arrival time.
@@ -193,18 +196,25 @@ class ServingLoop:
                         if not b1 and not b3:
                             lock.wait(0.002)
                             continue
+                    # launch both stages, their reads queued on the copy stream; stage 1's
+                    # results come back (and go to the clients) while stage 3 runs
+                    pend1 = pend3 = None
                     if b1:
                         for r in b1:
                             srv.upload(r.slot, r.ct)
                         srv.stage1_coarse_centroid_distance([r.slot for r in b1])
-                        cmap = srv.read_distances_batch([(r.slot, 1) for r in b1])
+                        pend1 = srv.read_distances_submit([(r.slot, 1) for r in b1])
+                    if b3:
+                        pend3 = srv.stage3_submit([(r.slot, r.probes) for r in b3], chunks=2)
+                    if pend1 is not None:
+                        cmap = srv.collect(pend1)
                         for r in b1:
                             coarse = cmap[r.slot][0]
                             r.events["stage1"] = now()
                             pool.submit(client_stage2, r, coarse)
-                    if b3:
-                        bmap, fmap = srv.stage3_and_read([(r.slot, r.probes) for r in b3], chunks=2)
-                        outs = [bmap[r.slot] for r in b3]
+                    if pend3 is not None:
+                        fmap = srv.collect(pend3)
+                        outs = [pend3["blocks"][r.slot] for r in b3]
                         for r, blocks in zip(b3, outs):
                             fine = fmap.get(r.slot)
                             r.events["stage3"] = now()

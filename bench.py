@@ -36,8 +36,12 @@ def parse():
     ap.add_argument("--batch", type=int, default=64, help="hmult: ciphertext pairs per step")
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle ops timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ntt-reps", type=int, default=5)
-    ap.add_argument("--rates", default="20,80,320,1280", help="serve: Poisson arrival rates swept (queries/s)")
+    ap.add_argument("--rates", default="20,80,320,640,1280", help="serve: Poisson arrival rates swept (queries/s)")
     ap.add_argument("--requests", type=int, default=160, help="serve: requests per rate")
+    ap.add_argument("--serve-small-slots", type=int, default=48,
+                    help="serve: N=2^14 query slots the HBM budget admits at once (at most 48)")
+    ap.add_argument("--serve-large-slots", type=int, default=16,
+                    help="serve: N=2^16 query slots the HBM budget admits at once (at most 16)")
     ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
                          "ks64: 64 rotations with distinct resident keys; ivfpq: encrypted IVF-PQ "
@@ -317,7 +321,8 @@ def bench_serve(args):
     log("serve: building parameter sets")
     small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=48, seed=1, log=log)
     large = SV.ParamSet.create("n16", 1 << 16, 24, 8, 3, ix, max_slots=16, seed=2, log=log)
-    budget = small.static_bytes + large.static_bytes + 32 * small.slot_bytes + 8 * large.slot_bytes
+    budget = small.static_bytes + large.static_bytes + args.serve_small_slots * small.slot_bytes + \
+        args.serve_large_slots * large.slot_bytes
     rng = np.random.default_rng(7)
     sweep = []
     for rate in [float(r) for r in args.rates.split(",")]:

@@ -364,9 +364,12 @@ def bench_serve(args):
         qm.load_keys(s)
         plain_lat, plain_ok = [], 0
         ps = IV.PlaintextSearcher(ix)
+        # its own generator (independent of the rates swept); n_probe spread evenly over
+        # 1..8 (mean 4.5 at 8 runs, like the served requests' uniform n_probe)
+        prng = np.random.default_rng(11)
         for k in range(args.plain_steps):
-            q = x[rng.integers(n)] + rng.normal(0, 0.05, 128).astype(np.float32)
-            npb = int(rng.integers(1, 9))
+            q = x[prng.integers(n)] + prng.normal(0, 0.05, 128).astype(np.float32)
+            npb = 1 + round(7 * k / max(1, args.plain_steps - 1))
             ids, dist, probes, sec = run_reference_query(qm, ck, s, pk, q, npb, 10, 100 + 2 * k)
             plain_lat.append(sec)
             plain_ok += set(ids) == set(ps.search(q, npb, 10)[0])

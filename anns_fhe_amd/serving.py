@@ -6,7 +6,10 @@ FPGA path is strictly sequential, cpp_host_model/src/interactive_query_manager.c
 
 * Parameter sets.  Each set (for example N=2^14/L=8 and N=2^16/L=24) owns one
   engine context (its own HIP stream and HBM arenas), its encoded database, its
-  evaluation keys and a fixed number of query slots.
+  evaluation keys and a fixed number of query slots.  Sets of one `group` are
+  replicas (same parameters and seed, so the same keys and database words): a
+  request names the group and is admitted to the replica with the most free slots,
+  which gives the group one more worker and stream on the device.
 * Admission.  A request is admitted when its set has a free slot and the HBM
   budget (static state of every set + the slots of every admitted request)
   allows it; otherwise it waits, first come first served.
@@ -38,7 +41,7 @@ from .driver import FheContext
 @dataclass
 class Request:
     rid: int
-    pset: str
+    pset: str          # parameter set, or group of replica sets
     query: np.ndarray
     n_probe: int
     arrival: float
@@ -50,13 +53,16 @@ class Request:
     t_admit: float = 0.0
     t_done: float = 0.0
     events: dict = field(default_factory=dict)
+    replica: str = ""  # the set it was admitted to
 
 
 class ParamSet:
     """One parameter set: context, client keys, encoded database, slots."""
 
-    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=2, log_scale=40, seed=1, threads=0, log=None):
+    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=2, log_scale=40, seed=1, threads=0, log=None,
+                 group=None):
         self.name = name
+        self.group = group or name
         self.ctx, self.ck = ctx, ck
         self.max_g = max_g
         self.client = IV.FheIvfPqClient(ck, dim=index.dim, log_scale=log_scale, lt=lt, seed=seed)
@@ -82,11 +88,11 @@ class ParamSet:
 
     @classmethod
     def create(cls, name, N, L, K, dnum, index, max_slots, lt=2, log_scale=40, max_probe=8, seed=1, threads=0,
-               log=None):
+               log=None, group=None):
         words, kw, max_g = cls.sizes(N, L, K, dnum, index, max_slots, lt, max_probe)
         ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=kw)
         ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-        return cls(name, ctx, ck, index, max_slots, max_g, lt, log_scale, seed, threads, log)
+        return cls(name, ctx, ck, index, max_slots, max_g, lt, log_scale, seed, threads, log, group)
 
     def encrypt(self, q, seed):
         return self.ck.encrypt_values(self.client.pk, np.asarray(q, np.float64), self.ck.L, self.client.scale, seed)
@@ -105,6 +111,13 @@ def poisson_arrivals(rate, n, seed):
 class ServingLoop:
     def __init__(self, psets, hbm_budget_bytes, max_batch=32, client_workers=8):
         self.ps = {p.name: p for p in psets}
+        self.groups = {}
+        for p in psets:
+            self.groups.setdefault(p.group, []).append(p.name)
+        for names in self.groups.values():  # replicas answer each other's clients
+            pk0 = self.ps[names[0]].client.pk
+            if any(not np.array_equal(self.ps[m].client.pk, pk0) for m in names[1:]):
+                raise ValueError("replica sets of a group must share their keys (same parameters and seed)")
         self.budget = hbm_budget_bytes
         self.static = sum(p.static_bytes for p in psets)
         if self.static > self.budget:
@@ -117,7 +130,7 @@ class ServingLoop:
         if encrypt:
             for r in requests:
                 if r.ct is None:
-                    r.ct = self.ps[r.pset].encrypt(r.query, 7919 * (r.rid + 1))
+                    r.ct = self.ps[self.groups[r.pset][0]].encrypt(r.query, 7919 * (r.rid + 1))
         lock = threading.Condition()
         waiting = collections.deque()
         free = {n: list(range(p.max_slots)) for n, p in self.ps.items()}
@@ -138,33 +151,35 @@ class ServingLoop:
             # FCFS: the head of the queue blocks the ones behind it
             while waiting:
                 r = waiting[0]
-                p = self.ps[r.pset]
-                if not free[r.pset] or used[0] + p.slot_bytes > self.budget:
+                name = max(self.groups[r.pset], key=lambda m: len(free[m]))  # first of the freest
+                p = self.ps[name]
+                if not free[name] or used[0] + p.slot_bytes > self.budget:
                     break
                 waiting.popleft()
-                r.slot = free[r.pset].pop()
+                r.replica = name
+                r.slot = free[name].pop()
                 used[0] += p.slot_bytes
                 r.t_admit = now()
-                st1[r.pset].append(r)
+                st1[name].append(r)
 
         def release(r):
-            p = self.ps[r.pset]
-            free[r.pset].append(r.slot)
+            p = self.ps[r.replica]
+            free[r.replica].append(r.slot)
             used[0] -= p.slot_bytes
 
         def client_stage2(r, coarse):
-            p = self.ps[r.pset]
+            p = self.ps[r.replica]
             srv = p.server
             scale = p.client.result_scale(p.ck.q[srv.lt - 1])
             d = p.client.decrypt_distances(coarse, srv.lt - 1, scale, srv.S)[:srv.ix.n_list]
             r.probes = np.argsort(d, kind="stable")[:r.n_probe]
             r.events["stage2"] = now()
             with lock:
-                st3[r.pset].append(r)
+                st3[r.replica].append(r)
                 lock.notify_all()
 
         def client_stage4(r, blocks, fine):
-            p = self.ps[r.pset]
+            p = self.ps[r.replica]
             srv = p.server
             scale = p.client.result_scale(p.ck.q[srv.lt - 1])
             ids, dist = [], []

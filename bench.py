@@ -41,7 +41,9 @@ def parse():
     ap.add_argument("--serve-small-slots", type=int, default=48,
                     help="serve: N=2^14 query slots the HBM budget admits at once (at most 48)")
     ap.add_argument("--serve-large-slots", type=int, default=16,
-                    help="serve: N=2^16 query slots the HBM budget admits at once (at most 16)")
+                    help="serve: N=2^16 query slots the HBM budget admits at once (at most 16 per replica)")
+    ap.add_argument("--serve-large-replicas", type=int, default=1,
+                    help="serve: contexts serving the N=2^16 set (replicas: own stream and worker each)")
     ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
                          "ks64: 64 rotations with distinct resident keys; ivfpq: encrypted IVF-PQ "
@@ -320,8 +322,11 @@ def bench_serve(args):
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
     log("serve: building parameter sets")
     small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=48, seed=1, log=log)
-    large = SV.ParamSet.create("n16", 1 << 16, 24, 8, 3, ix, max_slots=16, seed=2, log=log)
-    budget = small.static_bytes + large.static_bytes + args.serve_small_slots * small.slot_bytes + \
+    nrep = max(1, args.serve_large_replicas)
+    larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, 8, 3, ix, max_slots=16, seed=2,
+                                 log=log, group="n16") for i in range(nrep)]
+    large = larges[0]
+    budget = small.static_bytes + nrep * large.static_bytes + args.serve_small_slots * small.slot_bytes + \
         args.serve_large_slots * large.slot_bytes
     rng = np.random.default_rng(7)
     sweep = []
@@ -332,7 +337,7 @@ def bench_serve(args):
             reqs.append(SV.Request(rid=i, pset="n14" if rng.random() < 0.5 else "n16",
                                    query=x[rng.integers(n)] + rng.normal(0, 0.05, 128).astype(np.float32),
                                    n_probe=int(rng.integers(1, 9)), arrival=float(arr[i])))
-        loop = SV.ServingLoop([small, large], budget, max_batch=16, client_workers=16)
+        loop = SV.ServingLoop([small] + larges, budget, max_batch=16, client_workers=16)
         out = loop.run(reqs)
         stt = SV.latency_stats(out)
         per = {}
@@ -346,7 +351,8 @@ def bench_serve(args):
                       "p50_ms": round(stt["p50_ms"], 1), "p99_ms": round(stt["p99_ms"], 1), "by_params": per})
         log(f"serve rate {rate}: {sweep[-1]}")
     small.close()
-    large.close()
+    for p in larges:
+        p.close()
     # plain per-op path: the reference's op-by-op query sequence at N=2^14/L=8, one query at a time
     plain = None
     if args.plain_steps > 0:
@@ -387,7 +393,8 @@ def bench_serve(args):
         "data": "synthetic (seeded 32-Gaussian mixture, Poisson arrivals)",
         "config": {"workload": "Poisson stream of interactive encrypted IVF-PQ queries, 50/50 over two parameter "
                                "sets, n_probe uniform 1..8, HBM-budget admission, continuous batching",
-                   "db": n, "requests_per_rate": args.requests, "budget_bytes": budget},
+                   "db": n, "requests_per_rate": args.requests, "budget_bytes": budget,
+                   "large_replicas": nrep},
         "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "at_rate": best["rate"]},
         "sweep": sweep,
         "plain_path": plain,

@@ -15,13 +15,13 @@ def tiny():
     return x, ix
 
 
-def _oracle_set(orc, name, N, L, K, dnum, ix, slots, seed):
+def _oracle_set(orc, name, N, L, K, dnum, ix, slots, seed, group=None):
     from anns_fhe_amd.client import CkksClient
     from tests.oracle_ctx import OracleContext
     ck = CkksClient(N, L, K, dnum)
     words, kw, max_g = SV.ParamSet.sizes(N, L, K, dnum, ix, slots, max_probe=2)
     ctx = OracleContext(orc, N, L, K, dnum, ck.q, ck.p, words, kw)
-    return SV.ParamSet(name, ctx, ck, ix, slots, max_g, seed=seed)
+    return SV.ParamSet(name, ctx, ck, ix, slots, max_g, seed=seed, group=group)
 
 
 def test_serving_loop_mixed_sets_match_alone(orc, tiny):
@@ -67,3 +67,29 @@ def test_admission_respects_budget_and_order(orc, tiny):
     assert admits == sorted(admits)
     for i in range(1, 4):
         assert out[i].t_admit >= out[i - 1].events["stage3"]
+
+
+def test_replica_sets_share_a_group(orc, tiny):
+    """Two replicas of one parameter set (same seed) serve the requests of their group:
+    both take requests, and every result equals the plaintext search and the same
+    request served by one replica alone; replicas with different keys are refused."""
+    x, ix = tiny
+    a1 = _oracle_set(orc, "small_a", 4096, 4, 2, 2, ix, 2, seed=1, group="small")
+    a2 = _oracle_set(orc, "small_b", 4096, 4, 2, 2, ix, 2, seed=1, group="small")
+    rng = np.random.default_rng(5)
+    qs = [x[rng.integers(len(x))] + rng.normal(0, 0.05, 128).astype(np.float32) for _ in range(4)]
+    reqs = [SV.Request(rid=i, pset="small", query=qs[i], n_probe=1 + i % 2, arrival=0.0) for i in range(4)]
+    budget = a1.static_bytes + a2.static_bytes + 4 * a1.slot_bytes
+    out = SV.ServingLoop([a1, a2], budget, max_batch=2, client_workers=2).run(reqs)
+    assert {r.replica for r in out} == {"small_a", "small_b"}
+    ps = IV.PlaintextSearcher(ix)
+    for r in out:
+        pids, pdist, pprobes = ps.search(r.query, r.n_probe, 10)
+        assert list(r.probes) == list(pprobes)
+        assert set(r.ids) == set(pids)
+        solo = SV.Request(rid=r.rid, pset="small", query=r.query, n_probe=r.n_probe, arrival=0.0)
+        SV.ServingLoop([a1], a1.static_bytes + a1.slot_bytes, max_batch=1, client_workers=1).run([solo])
+        assert np.array_equal(solo.ids, r.ids) and np.array_equal(solo.dist, r.dist)
+    other = _oracle_set(orc, "small_c", 4096, 4, 2, 2, ix, 2, seed=9, group="small")
+    with pytest.raises(ValueError):
+        SV.ServingLoop([a1, other], budget)

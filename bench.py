@@ -406,7 +406,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(data-parallel sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the path shards by independent ciphertexts and queries, one engine per GPU; that "
+              "sharding is not built)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "hmult":
         res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq,

@@ -67,6 +67,7 @@ SIGNATURES = {
     "fhea_write_poly": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_poly": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),
     "fhea_read_polys": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p]),
+    "fhea_write_polys": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p]),
     "fhea_read_polys_async": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, C.POINTER(C.c_uint32)]),
     "fhea_read_wait": (C.c_int, [vp, C.c_uint32]),
     "fhea_write_key": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64]),

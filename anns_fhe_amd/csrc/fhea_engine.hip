@@ -1038,6 +1038,22 @@ int fhea_read_polys(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t
     }
     return check_hip(hipStreamSynchronize(c->stream));
 }
+int fhea_write_polys(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t *count, const uint64_t *h) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    if (!off || !count || (!h && n)) return FHEA_ERR_ARG;
+    for (u32 k = 0; k < n; ++k)
+        if (!range_ok(off[k], count[k], c->poly_words)) return FHEA_ERR_RANGE;
+    u64 at = 0;
+    for (u32 k = 0; k < n; ++k) {
+        if (count[k] &&
+            hipMemcpyAsync(c->poly + off[k], h + at, count[k] * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        at += count[k];
+    }
+    return check_hip(hipStreamSynchronize(c->stream));
+}
+
 int fhea_read_polys_async(fhea_ctx *c, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h,
                           uint32_t *ticket) {
     CTX_CHECK(c);

@@ -115,6 +115,15 @@ class FheContext:
         check(self.lib.fhea_read_polys(self.h, len(ranges), _p64(offs), _p64(cnts), _p64(a)), "read_polys")
         return a[:total]
 
+    def write_polys(self, ranges, data):
+        """`data` (the ranges' words back to back, ideally page-locked) into the arena
+        ranges [(off, count), ...], with one synchronisation."""
+        offs = np.array([r[0] for r in ranges], np.uint64)
+        cnts = np.array([r[1] for r in ranges], np.uint64)
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        assert a.size >= int(cnts.sum())
+        check(self.lib.fhea_write_polys(self.h, len(ranges), _p64(offs), _p64(cnts), _p64(a)), "write_polys")
+
     def read_polys_async(self, ranges, out):
         """read_polys into `out` (page-locked, e.g. a PinnedArray's array or a slice
         of it), returned at once: the copies run on the context's copy stream after

@@ -325,6 +325,21 @@ class FheIvfPqServer:
     def upload(self, z, ct):
         self.ctx.write_poly(self.slot_base(z), ct)
 
+    def upload_batch(self, items):
+        """Query ciphertexts [(z, ct), ...] into their slots: staged back to back in a
+        page-locked buffer, then one gathered write."""
+        if not items or not hasattr(self.ctx, "write_polys"):
+            for z, ct in items:
+                self.upload(z, ct)
+            return
+        W = self.ct_words
+        buf = self._result_buffer(len(items) * W, which="up")
+        for k, (z, ct) in enumerate(items):
+            a = np.asarray(ct, np.uint64).reshape(-1)
+            assert a.size == W, (a.size, W)
+            buf[k * W:(k + 1) * W] = a
+        self.ctx.write_polys([(self.slot_base(z), W) for z, _ in items], buf[:len(items) * W])
+
     def prepare(self, slots):
         """Rotations of every query by 1..D-1 (hoisted: one ModUp per query) and the
         encrypted ||q||^2 (all at lt limbs)."""
@@ -578,8 +593,7 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     client.seed += B
     cts = list(pool.map(lambda zq: ck.encrypt_values(client.pk, np.asarray(zq[1], np.float64), ck.L, client.scale,
                                                       seeds[zq[0]]), enumerate(queries)))
-    for z, ct in enumerate(cts):
-        server.upload(z, ct)
+    server.upload_batch(list(enumerate(cts)))
     t["client_encrypt_upload"] = time.perf_counter() - t0
     slots = list(range(B))
     t1 = time.perf_counter()

@@ -107,6 +107,10 @@ int fhea_read_poly(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t co
 /* n arena ranges [off[k], off[k] + count[k]) copied back to back into h (one
  * synchronisation; page-locked h from fhea_host_alloc makes the copies DMA). */
 int fhea_read_polys(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint64_t *count, uint64_t *h);
+/* The reverse: h (the ranges back to back) into n arena ranges, one synchronisation
+ * (SimMemoryBus::write_poly per range, sim_memory_bus.cpp:12-30). */
+int fhea_write_polys(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint64_t *count,
+                     const uint64_t *h);
 /* The same gathered read, returned at once: the copies run on a copy stream after
  * the work submitted so far, beside the kernels submitted next; *ticket names the
  * read for fhea_read_wait, which must come within 16 reads.  Work submitted before

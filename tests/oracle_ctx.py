@@ -48,6 +48,13 @@ class OracleContext:
             at += c
         return a[:total]
 
+    def write_polys(self, ranges, data):
+        a = np.ascontiguousarray(data, dtype=np.uint64).reshape(-1)
+        at = 0
+        for off, c in ranges:
+            self.pm[off:off + c] = a[at:at + c]
+            at += c
+
     def read_polys_async(self, ranges, out):
         # the oracle runs every call to completion: the read is done at once
         self.read_polys(ranges, out)

@@ -590,7 +590,8 @@ def test_workload_tuner_keeps_results(orc):
 
 
 def test_async_reads_overlap_later_work():
-    """fhea_read_polys_async: gathered reads on the copy stream equal the synchronous
+    """fhea_write_polys puts back-to-back words into scattered ranges;
+    fhea_read_polys_async: gathered reads on the copy stream equal the synchronous
     read, also while later kernels (NTTs on other words) run; tickets may be waited
     out of order; a ticket more than 16 reads old, or not yet issued, is refused."""
     from anns_fhe_amd._lib import FheError
@@ -603,6 +604,10 @@ def test_async_reads_overlap_later_work():
     ctx.write_poly(0, data)
     ranges = [(0, 100), (3 * W + 17, W), (5 * W, 2 * W), (W + 8, 1)]
     ref = ctx.read_polys(ranges)
+    words = rng.integers(0, 1 << 40, ref.size, dtype=np.uint64)
+    ctx.write_polys(ranges, words)
+    assert np.array_equal(ctx.read_polys(ranges), words)
+    ctx.write_polys(ranges, ref)
     pin = PinnedArray(4 * ref.size)
     tickets = []
     for k in range(3):

@@ -93,6 +93,12 @@ __attribute__((target("avx2,fma"))) inline void fwd_bfly(double &u, double &v, d
     u = hreduce(a + r, q, qinv);
     v = hreduce(a - r, q, qinv);
 }
+__attribute__((target("avx2,fma"))) inline void fwd_bfly_lazy(double &u, double &v, double w, double wq, double q) {
+#pragma clang fp contract(off)
+    const double r = hmulmod(v, w, wq, q), a = u;
+    u = a + r;
+    v = a - r;
+}
 __attribute__((target("avx2,fma"))) inline void inv_bfly(double &u, double &v, double w, double wq, double q,
                                                         double qinv) {
 #pragma clang fp contract(off)
@@ -107,23 +113,37 @@ __attribute__((target("avx2,fma"))) void ntt_forward_f64_host(u64 *a, u32 N, u64
     double *x = xv.data();
     const double qd = (double)q, qinv = 1.0 / qd;
     for (u32 j = 0; j < N; ++j) x[j] = u52_to_f64(a[j]);
-    u32 t = N >> 1;
-    for (u32 m = 1; m < N; m <<= 1, t >>= 1) {
+    // centred every fourth stage and at the last: from |x| < q three unreduced stages
+    // reach 3.86q (< 8q, the products' bound), from |x| <= q/2 + 2 they reach 2.9q
+    u32 t = N >> 1, st = 0;
+    for (u32 m = 1; m < N; m <<= 1, t >>= 1, ++st) {
+        const bool red = (st & 3) == 3 || 2 * m == N;
+#define FHEA_FWD_BFLY(U, V, W_, WQ_)                      \
+    if (red)                                              \
+        fwd_bfly(U, V, W_, WQ_, qd, qinv);                \
+    else                                                  \
+        fwd_bfly_lazy(U, V, W_, WQ_, qd);
         if (t >= 4) {
             for (u32 i = 0; i < m; ++i) {
                 const double w = W[m + i], wq = WQ[m + i];
                 double *xa = x + 2 * i * t, *xb = xa + t;
-                for (u32 j = 0; j < t; ++j) fwd_bfly(xa[j], xb[j], w, wq, qd, qinv);
+                if (red)
+                    for (u32 j = 0; j < t; ++j) fwd_bfly(xa[j], xb[j], w, wq, qd, qinv);
+                else
+                    for (u32 j = 0; j < t; ++j) fwd_bfly_lazy(xa[j], xb[j], w, wq, qd);
             }
         } else if (t == 2) {
             for (u32 i = 0; i < m; ++i) {
                 double *xa = x + 4 * i;
-                fwd_bfly(xa[0], xa[2], W[m + i], WQ[m + i], qd, qinv);
-                fwd_bfly(xa[1], xa[3], W[m + i], WQ[m + i], qd, qinv);
+                FHEA_FWD_BFLY(xa[0], xa[2], W[m + i], WQ[m + i])
+                FHEA_FWD_BFLY(xa[1], xa[3], W[m + i], WQ[m + i])
             }
         } else {
-            for (u32 i = 0; i < m; ++i) fwd_bfly(x[2 * i], x[2 * i + 1], W[m + i], WQ[m + i], qd, qinv);
+            for (u32 i = 0; i < m; ++i) {
+                FHEA_FWD_BFLY(x[2 * i], x[2 * i + 1], W[m + i], WQ[m + i])
+            }
         }
+#undef FHEA_FWD_BFLY
     }
     for (u32 j = 0; j < N; ++j) a[j] = f64_to_u52(x[j] < 0.0 ? x[j] + qd : x[j]);
 }
@@ -286,10 +306,18 @@ std::vector<u64> to_eval(const HostTables &h, const int64_t *coef, const std::ve
     const u32 N = h.N;
     std::vector<u64> out((size_t)pidx.size() * N);
     HostNtt ntt(h);
+    // keys and noise are small: |c| < q gives the residue by one conditional add
+    u64 mag = 0;
+    for (u32 j = 0; j < N; ++j) mag = std::max<u64>(mag, coef[j] < 0 ? (u64)(-(coef[j] + 1)) + 1 : (u64)coef[j]);
     for (size_t r = 0; r < pidx.size(); ++r) {
-        const Reducer red(h.primes[pidx[r]]);
+        const u64 q = h.primes[pidx[r]];
         u64 *row = out.data() + r * N;
-        for (u32 j = 0; j < N; ++j) row[j] = to_residue(coef[j], red);
+        if (mag < q) {
+            for (u32 j = 0; j < N; ++j) row[j] = (u64)(coef[j] + (coef[j] < 0 ? (int64_t)q : 0));
+        } else {
+            const Reducer red(q);
+            for (u32 j = 0; j < N; ++j) row[j] = to_residue(coef[j], red);
+        }
         ntt.forward(row, pidx[r]);
     }
     return out;

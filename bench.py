@@ -93,8 +93,15 @@ def cpu_baseline(ctx, key, sample):
         st = o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L)
         assert st == 0
     dt = time.perf_counter() - t0
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": sample / dt, "unit": "HMult+relin+rescale ops/s", "cores": 1, "kind": "port",
-            "sample": f"{sample} x eval_mult_relin_rescale N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread)"}
+            "sample": f"{sample} x eval_mult_relin_rescale N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread of "
+                      f"{cpu}, {os.cpu_count()} logical CPUs)"}
 
 
 def plain_hmult(ctx, drv, base, steps):
@@ -502,6 +509,18 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},
     }
+    # achievable HBM rate on this box: a 1 GiB device-to-device copy (read + write), best of 3;
+    # it runs after every HMult/NTT measurement and overwrites inputs the rest does not check
+    big = 128 << 20
+    if 4 * B * W >= 2 * big:
+        best = None
+        for _ in range(3):
+            ctx.timer_start()
+            assert ctx.lib.fhea_copy_poly(ctx.h, big, 0, big) == 0
+            t = ctx.timer_stop()
+            best = t if best is None else min(best, t)
+        result["roofline"]["stream_copy_GBs"] = round(2 * big * 8 / (best / 1e3) / 1e9, 1)
+        log(f"1 GiB device copy: {best:.3f} ms -> {result['roofline']['stream_copy_GBs']} GB/s (read + write)")
     # calibration copy of a known size (64 MiB, D2D blit) for rocprof FETCH_SIZE / WRITE_SIZE
     # scaling; it runs after every measurement and overwrites the first input only
     calib_words = 8 << 20

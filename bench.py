@@ -99,9 +99,25 @@ def cpu_baseline(ctx, key, sample):
             cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
         pass
+    # all cores: independent ciphertexts on one thread each (the oracle's calls release the GIL)
+    import concurrent.futures as cf
+    T = max(1, min(16, os.cpu_count() or 1))
+    per = 2
+    bufs = [pm.copy() for _ in range(T)]
+
+    def work(k):
+        for _ in range(per):
+            assert o.eval_mult_relin_rescale(bufs[k], key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+
+    t1 = time.perf_counter()
+    with cf.ThreadPoolExecutor(T) as ex:
+        list(ex.map(work, range(T)))
+    dt_all = time.perf_counter() - t1
     return {"value": sample / dt, "unit": "HMult+relin+rescale ops/s", "cores": 1, "kind": "port",
             "sample": f"{sample} x eval_mult_relin_rescale N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread of "
-                      f"{cpu}, {os.cpu_count()} logical CPUs)"}
+                      f"{cpu}, {os.cpu_count()} logical CPUs)",
+            "all_cores": {"value": round(T * per / dt_all, 3), "threads": T,
+                          "sample": f"{T * per} ops on {T} threads, one ciphertext pair each ({dt_all:.1f} s)"}}
 
 
 def plain_hmult(ctx, drv, base, steps):

@@ -305,3 +305,33 @@ def test_hoisted_rotations_pinned(orc):
         ct = np.concatenate([pm[outs0[z]:outs0[z] + W], pm[outs1[z]:outs1[z] + W]])
         got = cl.decrypt_values(s, ct, L, 2.0 ** 40, n)
         assert np.max(np.abs(got - np.roll(x, -st))) < 1e-6, st
+
+
+def test_oracle_calls_on_threads_equal_sequential_calls():
+    """The CPU baseline times oracle HMults on several host threads at once (one
+    ciphertext pair per thread, one shared context and key): the words equal those
+    of the same calls made one after another."""
+    import concurrent.futures as cf
+    from anns_fhe_amd.client import CkksClient
+    from oracle import oracle as O
+    O.build()
+    N, L, K, D = 1 << 11, 3, 2, 2
+    ck = CkksClient(N, L, K, D)
+    o = O.OracleCtx(N, ck.q, ck.p, D)
+    key = ck.relin_key(ck.keygen_secret(3), L, 5)
+    W = L * N
+    rng = np.random.default_rng(1)
+    bufs = []
+    for _ in range(6):
+        pm = np.zeros(6 * W, np.uint64)
+        pm[:4 * W] = np.concatenate([rng.integers(0, q, N, dtype=np.uint64) for _ in range(4) for q in ck.q])
+        bufs.append(pm)
+    seq = [b.copy() for b in bufs]
+    for b in seq:
+        assert o.eval_mult_relin_rescale(b, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+    par = [b.copy() for b in bufs]
+    with cf.ThreadPoolExecutor(6) as ex:
+        st = list(ex.map(lambda b: o.eval_mult_relin_rescale(b, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L),
+                         par))
+    assert st == [0] * 6
+    assert all(np.array_equal(a, b) for a, b in zip(seq, par))

@@ -93,12 +93,7 @@ def cpu_baseline(ctx, key, sample):
         st = o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L)
         assert st == 0
     dt = time.perf_counter() - t0
-    cpu = "unknown CPU"
-    try:
-        with open("/proc/cpuinfo") as f:
-            cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
-    except (OSError, StopIteration):
-        pass
+    cpu = _cpu_name()
     # all cores: independent ciphertexts on one thread each (the oracle's calls release the GIL)
     import concurrent.futures as cf
     T = max(1, min(16, os.cpu_count() or 1))
@@ -155,6 +150,49 @@ def plain_hmult(ctx, drv, base, steps):
                     "synchronised per op"}
 
 
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown CPU"
+
+
+def cpu_ntt(ctx, sample_polys):
+    """Oracle forward + inverse NTT of `sample_polys` x L limbs, single thread."""
+    from oracle import oracle as O
+    O.build()
+    o = O.OracleCtx(N, ctx.q, ctx.p, 1)
+    rng = np.random.default_rng(3)
+    polys = [rand_limbs(rng, ctx.q, N).reshape(L, N) for _ in range(sample_polys)]
+    t0 = time.perf_counter()
+    for a in polys:
+        o.intt_poly(o.ntt_poly(a))
+    dt = time.perf_counter() - t0
+    n = 2 * sample_polys * L
+    return {"value": n / dt, "unit": "limb-transforms/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_polys} polys x {L} limbs forward + inverse, N=2^16 ({dt:.1f} s, 1 thread of "
+                      f"{_cpu_name()})"}
+
+
+def cpu_rotations(ctx, key_blob, sample):
+    """Oracle eval_rotate by one step with one resident key, single thread."""
+    from oracle import oracle as O
+    O.build()
+    o = O.OracleCtx(N, ctx.q, ctx.p, DNUM)
+    W = L * N
+    rng = np.random.default_rng(4)
+    pm = np.zeros(4 * W, np.uint64)
+    pm[:2 * W] = np.concatenate([rand_limbs(rng, ctx.q, N).reshape(-1) for _ in range(2)])
+    t0 = time.perf_counter()
+    for _ in range(sample):
+        assert o.eval_rotate(pm, key_blob, [(1, 5, 0, L)], 0, W, 2 * W, 3 * W, 1, L) == 0
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "rotations/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} x eval_rotate(+1) N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread of "
+                      f"{_cpu_name()})"}
+
+
 def bench_ntt256(args):
     """256 polynomials x 24 limbs, forward then inverse (in place), 50-bit primes."""
     from anns_fhe_amd.driver import FheContext
@@ -181,6 +219,7 @@ def bench_ntt256(args):
     alg = limbs * 2 * N * 8
     gbs = alg / (ms / 1e3) / 1e9
     log(f"ntt256: {ms:.2f} ms per fwd+inv -> {gbs:.0f} GB/s algorithmic")
+    cpu = cpu_ntt(ctx, 4) if args.cpu_sample > 0 else None
     return {
         "metric": "NTT+INTT limb-transforms/s at N=2^16 (256 polys x 24 limbs)",
         "value": round(limbs / (ms / 1e3), 1), "unit": "limb-transforms/s", "n_gpus": 1,
@@ -193,6 +232,7 @@ def bench_ntt256(args):
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "fwd_ms": round(fwd_ms / args.steps, 3), "inv_ms": round(inv_ms / args.steps, 3),
                      "alg_bytes_per_step": alg},
+        "cpu_baseline": cpu,
     }
 
 
@@ -234,6 +274,7 @@ def bench_ks64(args):
     alg = n_ct * (2 * W + ksz + 2 * W) * 8  # ct in, own key, ct out
     gbs = alg / (ms / 1e3) / 1e9
     log(f"ks64: {ms:.2f} ms per 64 rotations -> {gbs:.0f} GB/s algorithmic")
+    cpu = cpu_rotations(ctx, blob, 3) if args.cpu_sample > 0 else None
     return {
         "metric": "rotations (automorphism + hybrid key switch) /s at N=2^16,L=24,K=8,dnum=3, distinct keys",
         "value": round(n_ct / (ms / 1e3), 1), "unit": "rotations/s", "n_gpus": 1, "steps": args.steps,
@@ -246,6 +287,7 @@ def bench_ks64(args):
         "roofline": {"bound": "hbm", "kernel": "whole rotation (ct in + key + ct out per rotation)",
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg},
+        "cpu_baseline": cpu,
     }
 
 

@@ -1564,6 +1564,11 @@ static u32 lin_qa(u32 nq, u32 ng) {
     return best;
 }
 
+// Groups of at most LIN_TILED_NQ queries go to the tile kernel (one wave per 2 x 8
+// tile, eight tiles per workgroup): as LDS units most of their waves would idle.
+static constexpr u32 LIN_TILED_NQ = 4;
+static bool lin_tiled(u32 nq) { return nq <= LIN_TILED_NQ; }
+
 struct LinGroupHost {
     u32 nq, ng;
     const u64 *ct0, *ct1, *bias0, *bias1;  // nq*R, nq*R, nq (or null), nq (or null)
@@ -1576,7 +1581,7 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
     const u64 W = (u64)limbs * c->h.N;
     const bool f64 = linear_sum_f64(c->dt);
     std::vector<u64> ins, outs;
-    size_t words = 0, n4 = 0, n8 = 0;
+    size_t words = 0, n4 = 0, nt = 0, n8 = 0;
     int st = FHEA_OK;
     auto in = [&](u64 o) {
         if (!st) st = poly_code(c, o, W);
@@ -1604,7 +1609,9 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
             outs.push_back(g.out1[k]);
         }
         words += 2 * (size_t)g.nq * (R + 1) + (size_t)g.ng * (R + 1) + 2 * (size_t)g.nq * g.ng;
-        if (f64) {
+        if (f64 && lin_tiled(g.nq)) {
+            nt += (size_t)((g.nq + 1) / 2) * ((g.ng + 7) / 8);
+        } else if (f64) {
             const u32 qa = lin_qa(g.nq, g.ng), uq = 2 * qa, ug = 64 / qa;
             n4 += (size_t)((g.nq + uq - 1) / uq) * ((g.ng + ug - 1) / ug);
         } else {
@@ -1612,7 +1619,7 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
         }
     }
     if (st) return st;
-    if (n4 > 65535 || n8 > 65535) return FHEA_ERR_ARG;
+    if (n4 > 65535 || n8 > 65535 || nt > 65535ull * 8) return FHEA_ERR_ARG;
     std::sort(outs.begin(), outs.end());
     for (size_t k = 0; k + 1 < outs.size(); ++k)
         if (outs[k + 1] - outs[k] < W) return FHEA_ERR_ARG;
@@ -1620,12 +1627,12 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
         auto it = std::lower_bound(outs.begin(), outs.end(), x >= W - 1 ? x - (W - 1) : 0);
         if (it != outs.end() && overlaps(x, *it, W)) return FHEA_ERR_ARG;
     }
-    // staged: every group's offsets, then the LDS-shape units, then the 1x8 units
-    const size_t bytes = words * 8 + (n4 + n8) * sizeof(LinUnit);
+    // staged: every group's offsets, then the LDS-shape units, the tiles, the 1x8 units
+    const size_t bytes = words * 8 + (n4 + nt + n8) * sizeof(LinUnit);
     char *h = nullptr, *d = nullptr;
     if ((st = stage_reserve(c, bytes, &h, &d))) return st;
     u64 *hw = (u64 *)h;
-    LinUnit *u4 = (LinUnit *)(h + words * 8), *u8 = u4 + n4;
+    LinUnit *u4 = (LinUnit *)(h + words * 8), *ut = u4 + n4, *u8 = ut + nt;
     size_t w = 0;
     for (const LinGroupHost &g : groups) {
         const u64 *dev = (const u64 *)(d + w * 8);
@@ -1646,7 +1653,11 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
         memcpy(x, g.out0, nqg * 8);
         memcpy(x + nqg, g.out1, nqg * 8);
         w += 2 * nqR + 2 * (size_t)g.nq + ngR + g.ng + 2 * nqg;
-        if (f64) {
+        if (f64 && lin_tiled(g.nq)) {
+            for (u32 q0 = 0; q0 < g.nq; q0 += 2)
+                for (u32 g0 = 0; g0 < g.ng; g0 += 8)
+                    *ut++ = LinUnit{dev, g.nq, g.ng, q0, g0, std::min(2u, g.nq - q0), std::min(8u, g.ng - g0), 0, 0};
+        } else if (f64) {
             const u32 qa = lin_qa(g.nq, g.ng), uq = 2 * qa, ug = 64 / qa;
             for (u32 q0 = 0; q0 < g.nq; q0 += uq)
                 for (u32 g0 = 0; g0 < g.ng; g0 += ug)
@@ -1661,7 +1672,8 @@ static int linear_sum_groups(fhea_ctx *c, u32 R, u32 limbs, const std::vector<Li
     if ((st = stage_commit(c, h, d, bytes))) return st;
     const LinUnit *d4 = (const LinUnit *)(d + words * 8);
     launch_linear_sum(c->dt, c->poly, d4, (u32)n4, R, limbs, LIN_SHAPE_LDS, c->stream);
-    launch_linear_sum(c->dt, c->poly, d4 + n4, (u32)n8, R, limbs, LIN_SHAPE_1x8, c->stream);
+    launch_linear_sum(c->dt, c->poly, d4 + n4, (u32)nt, R, limbs, LIN_SHAPE_TILES, c->stream);
+    launch_linear_sum(c->dt, c->poly, d4 + n4 + nt, (u32)n8, R, limbs, LIN_SHAPE_1x8, c->stream);
     return check_hip(hipGetLastError());
 }
 

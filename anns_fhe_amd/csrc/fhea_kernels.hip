@@ -600,14 +600,22 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 // slice of the grid, so one launch fills the GPU.
 // ---------------------------------------------------------------------------
 
+// an absent operand (LIN_NONE) reads word idx of the arena and yields 0, so that
+// the loads of a prologue issue back to back without branches
 __device__ __forceinline__ u64 lin_word(const u64 *P, u64 off, size_t idx) {
-    return off == LIN_NONE ? 0 : P[off + idx];
+    const u64 v = P[(off == LIN_NONE ? 0 : off) + idx];
+    return off == LIN_NONE ? 0 : v;
 }
 
+// A unit's offset table is uniform and fixed for the launch: read through the
+// constant address space it comes in by scalar loads, which neither occupy vector
+// registers nor make the vector loads behind them wait.
+typedef __attribute__((address_space(4))) const u64 CU64;
+
 struct LinView {
-    const u64 *ct0, *ct1, *b0, *b1, *pt, *ptb, *o0, *o1;
+    const CU64 *ct0, *ct1, *b0, *b1, *pt, *ptb, *o0, *o1;
     __device__ LinView(const LinUnit &U, u32 R) {
-        ct0 = U.offs;
+        ct0 = (const CU64 *)U.offs;
         ct1 = ct0 + (size_t)U.nq * R;
         b0 = ct1 + (size_t)U.nq * R;
         b1 = b0 + U.nq;
@@ -680,12 +688,13 @@ __global__ __launch_bounds__(TPB) void k_linear_sum(u64 *__restrict__ P, const L
 // 8 products (<= q/2 + 2 + 7q).  Each wave owns a tile of 2 queries x 8 blocks of
 // the unit; the unit is qa tiles high and 8/qa tiles wide (qa = 8, 4 or 2: 16 x 8,
 // 8 x 16 or 4 x 32 pairs), chosen per group so that the tiles are well filled.
-static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NW = LIN_TPB / LIN_POS, LIN_DEPTH = 4;
+static constexpr u32 LIN_TPB = 512, LIN_POS = 64, LIN_NW = LIN_TPB / LIN_POS;
 
 __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict__ P,
                                                                const LinUnit *__restrict__ units, u32 R,
                                                                const ModParams *__restrict__ mp, u32 logN) {
-    __shared__ double buf[2][LIN_NS][LIN_POS];
+    // row LIN_NS takes the words of the streams a unit does not have
+    __shared__ double buf[2][LIN_NS + 1][LIN_POS];
     const LinUnit U = units[blockIdx.z];
     const LinView V(U, R);
     // the wave index is uniform across the wave: stream and tile bookkeeping in SGPRs
@@ -697,14 +706,15 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
     // streams: 2i (ct0 of query i), 2i+1 (ct1 of query i), 2*uq + k (block k);
     // thread (w, lane) moves streams w, w + 8, ... of its position
     constexpr u32 PER = (LIN_NS + LIN_NW - 1) / LIN_NW;
-    const u64 *sbase[PER];
+    // Every thread loads and deposits all PER slots whether or not the unit has that
+    // stream (a missing one reads block 0's words into the spare row): the loads of
+    // a step are straight-line code.
+    const CU64 *sbase[PER];
     u32 srow[PER];
-    bool sok[PER];
 #pragma unroll
     for (u32 k = 0; k < PER; ++k) {
         const u32 j = w + k * LIN_NW;
-        sok[k] = j < ns;
-        srow[k] = j;
+        srow[k] = j < ns ? j : LIN_NS;
         if (j < 2 * uq)
             sbase[k] = ((j & 1) ? V.ct1 : V.ct0) + (size_t)(U.q0 + j / 2) * R;
         else
@@ -714,70 +724,77 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
     const u32 tq = 2 * (w % U.qa), tg = 8 * (w / U.qa);
     const bool v0 = tq < uq, v1 = tq + 1 < uq;
     const u32 nb = tg < ug ? min(8u, ug - tg) : 0;
+    // biases: indices clamped into the unit's tables so that all loads issue back to
+    // back; the accumulators of pairs outside the unit are never stored
     double a0[2][8], a1[2][8];
+    {
+        double pb[8], b0[2], b1[2];
 #pragma unroll
-    for (u32 i = 0; i < 2; ++i) {
-        const bool v = i ? v1 : v0;
-        const u32 qq = U.q0 + tq + i;
-        const double b0 = v ? u64_to_f64(lin_word(P, V.b0[qq], idx)) : 0.0;
-        const double b1 = v ? u64_to_f64(lin_word(P, V.b1[qq], idx)) : 0.0;
+        for (u32 k = 0; k < 8; ++k) pb[k] = u64_to_f64(lin_word(P, V.ptb[min(U.g0 + tg + k, U.ng - 1)], idx));
 #pragma unroll
-        for (u32 k = 0; k < 8; ++k) {
-            const double pb = (v && k < nb) ? u64_to_f64(lin_word(P, V.ptb[U.g0 + tg + k], idx)) : 0.0;
-            a0[i][k] = reduce_f64(b0 + pb, q, qinv);
-            a1[i][k] = b1;
+        for (u32 i = 0; i < 2; ++i) {
+            const u32 qq = min(U.q0 + tq + i, U.nq - 1);
+            b0[i] = u64_to_f64(lin_word(P, V.b0[qq], idx));
+            b1[i] = u64_to_f64(lin_word(P, V.b1[qq], idx));
         }
-    }
-    // loads run LIN_DEPTH steps ahead of the products (register ring, static slots:
-    // the r loop is unrolled by LIN_DEPTH); LDS holds the current and the next step
-    u64 ld[LIN_DEPTH][PER];
-#define LIN_ISSUE(slot, rr)                                                    \
-    if ((rr) < R) {                                                            \
-        _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) ld[slot][k] = P[sbase[k][rr] + idx]; \
-    }
-#define LIN_DEPOSIT(slot, b, rr)                                               \
-    if ((rr) < R) {                                                            \
-        _Pragma("unroll") for (u32 k = 0; k < PER; ++k) if (sok[k]) buf[b][srow[k]][lane] = u64_to_f64(ld[slot][k]); \
-    }
-#define LIN_PRODUCTS(b, rr)                                                    \
-    if ((rr) < R) {                                                            \
-        _Pragma("unroll") for (u32 i = 0; i < 2; ++i) {                        \
-            if (!(i ? v1 : v0)) continue;                                      \
-            const double x0 = buf[b][2 * (tq + i)][lane], x1 = buf[b][2 * (tq + i) + 1][lane]; \
-            _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                    \
-                if (k < nb) {                                                  \
-                    const double p = buf[b][2 * uq + tg + k][lane];            \
-                    a0[i][k] += mulmod_vv_f64(x0, p, q, qinv);                 \
-                    a1[i][k] += mulmod_vv_f64(x1, p, q, qinv);                 \
-                }                                                              \
-            }                                                                  \
-        }                                                                      \
-        if (((rr) & 7) == 7) {                                                 \
-            _Pragma("unroll") for (u32 i = 0; i < 2; ++i)                      \
-            _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                    \
-                a0[i][k] = reduce_f64(a0[i][k], q, qinv);                      \
-                a1[i][k] = reduce_f64(a1[i][k], q, qinv);                      \
-            }                                                                  \
-        }                                                                      \
-    }
-    // step rr: products on LDS buffer rr&1, then step rr+1's words (loaded
-    // LIN_DEPTH-1 steps ago) into the other buffer, last read before the previous
-    // barrier; step rr+LIN_DEPTH's loads into the freed slot
-#define LIN_STEP(j, rr)                                                        \
-    LIN_PRODUCTS((j) & 1, rr)                                                  \
-    LIN_DEPOSIT(((j) + 1) % LIN_DEPTH, ((j) + 1) & 1, (rr) + 1)                \
-    LIN_ISSUE((j) % LIN_DEPTH, (rr) + LIN_DEPTH)                               \
-    __syncthreads();
-    static_assert(LIN_DEPTH % 2 == 0, "buffer parity must follow the unrolled step");
 #pragma unroll
-    for (u32 j = 0; j < LIN_DEPTH; ++j) LIN_ISSUE(j, j)
-    LIN_DEPOSIT(0, 0, 0u)
+        for (u32 i = 0; i < 2; ++i)
+#pragma unroll
+            for (u32 k = 0; k < 8; ++k) {
+                a0[i][k] = reduce_f64(b0[i] + pb[k], q, qinv);
+                a1[i][k] = b1[i];
+            }
+    }
+    // two register slots: the loads of step r+2 are issued at step r, the words of
+    // step r+1 (issued a step earlier) go into LDS after step r's products
+    u64 la[PER], lb[PER];
+#define LIN_ISSUE(ld, rr)                                                      \
+    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) ld[k] = P[sbase[k][rr] + idx];
+#define LIN_DEPOSIT(ld, b)                                                     \
+    _Pragma("unroll") for (u32 k = 0; k < PER; ++k) buf[b][srow[k]][lane] = u64_to_f64(ld[k]);
+#define LIN_PRODUCTS(b, rr)                                                    \
+    _Pragma("unroll") for (u32 i = 0; i < 2; ++i) {                            \
+        if (!(i ? v1 : v0)) continue;                                          \
+        const double x0 = buf[b][2 * (tq + i)][lane], x1 = buf[b][2 * (tq + i) + 1][lane]; \
+        _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                        \
+            if (k < nb) {                                                      \
+                const double p = buf[b][2 * uq + tg + k][lane];                \
+                a0[i][k] += mulmod_vv_f64(x0, p, q, qinv);                     \
+                a1[i][k] += mulmod_vv_f64(x1, p, q, qinv);                     \
+            }                                                                  \
+        }                                                                      \
+    }                                                                          \
+    if (((rr) & 7) == 7) {                                                     \
+        _Pragma("unroll") for (u32 i = 0; i < 2; ++i)                          \
+        _Pragma("unroll") for (u32 k = 0; k < 8; ++k) {                        \
+            a0[i][k] = reduce_f64(a0[i][k], q, qinv);                          \
+            a1[i][k] = reduce_f64(a1[i][k], q, qinv);                          \
+        }                                                                      \
+    }
+// G: the step may lie past R (the tail loop); in the main loop it never does
+#define LIN_STEP(G, X, Y, rr, b)                                               \
+    if (!(G) || (rr) + 2 < R) {                                                \
+        LIN_ISSUE(X, (rr) + 2)                                                 \
+    }                                                                          \
+    if (!(G) || (rr) < R) {                                                    \
+        LIN_PRODUCTS(b, rr)                                                    \
+    }                                                                          \
+    if (!(G) || (rr) + 1 < R) {                                                \
+        LIN_DEPOSIT(Y, (b) ^ 1)                                                \
+    }                                                                          \
     __syncthreads();
-    for (u32 r = 0; r < R; r += LIN_DEPTH) {
-        LIN_STEP(0, r)
-        LIN_STEP(1, r + 1)
-        LIN_STEP(2, r + 2)
-        LIN_STEP(3, r + 3)
+    LIN_ISSUE(la, 0u)
+    LIN_ISSUE(lb, R > 1 ? 1u : 0u)
+    LIN_DEPOSIT(la, 0)
+    __syncthreads();
+    u32 r = 0;
+    for (; r + 4 <= R; r += 2) {
+        LIN_STEP(false, la, lb, r, 0)
+        LIN_STEP(false, lb, la, r + 1, 1)
+    }
+    for (; r < R; r += 2) {
+        LIN_STEP(true, la, lb, r, 0)
+        LIN_STEP(true, lb, la, r + 1, 1)
     }
 #undef LIN_STEP
 #undef LIN_PRODUCTS
@@ -796,6 +813,102 @@ __global__ __launch_bounds__(LIN_TPB) void k_linear_sum_lds_f64(u64 *__restrict_
     }
 }
 
+// FP64 form for small groups: each wave owns one tile (uq <= 2 queries x ug <= 8
+// blocks of a group, 64 positions) and reads its words itself; a workgroup holds
+// LIN_TW independent tiles (no LDS, no barriers), so groups probed by one or two
+// queries do not each hold a whole LDS unit.  Loads of step r+1 are issued before
+// the products of step r; tile rows and blocks past uq / ug repeat the last valid
+// stream (cache hits) so that every step's loads are straight-line code.
+static constexpr u32 LIN_TW = 8;
+
+__global__ __launch_bounds__(LIN_TW * LIN_POS) void k_linear_sum_tiles_f64(u64 *__restrict__ P,
+                                                                         const LinUnit *__restrict__ tiles,
+                                                                         u32 n_tiles, u32 R,
+                                                                         const ModParams *__restrict__ mp,
+                                                                         u32 logN) {
+    const u32 t = blockIdx.z * LIN_TW + __builtin_amdgcn_readfirstlane(threadIdx.x / LIN_POS);
+    if (t >= n_tiles) return;
+    const LinUnit U = tiles[t];
+    const LinView V(U, R);
+    const u32 limb = blockIdx.y, lane = threadIdx.x % LIN_POS;
+    const size_t idx = ((size_t)limb << logN) + (size_t)blockIdx.x * LIN_POS + lane;
+    const double q = mp[limb].qd, qinv = mp[limb].qinv;
+    const u32 uq = U.uq, ug = U.ug;
+    const CU64 *c0[2], *c1[2], *pr[8];
+#pragma unroll
+    for (u32 i = 0; i < 2; ++i) {
+        const size_t row = (size_t)(U.q0 + min(i, uq - 1)) * R;
+        c0[i] = V.ct0 + row;
+        c1[i] = V.ct1 + row;
+    }
+#pragma unroll
+    for (u32 k = 0; k < 8; ++k) pr[k] = V.pt + (size_t)(U.g0 + min(k, ug - 1)) * R;
+    double a0[2][8], a1[2][8];
+    {
+        double pb[8], b0[2], b1[2];
+#pragma unroll
+        for (u32 k = 0; k < 8; ++k) pb[k] = u64_to_f64(lin_word(P, V.ptb[U.g0 + min(k, ug - 1)], idx));
+#pragma unroll
+        for (u32 i = 0; i < 2; ++i) {
+            const u32 qq = U.q0 + min(i, uq - 1);
+            b0[i] = u64_to_f64(lin_word(P, V.b0[qq], idx));
+            b1[i] = u64_to_f64(lin_word(P, V.b1[qq], idx));
+        }
+#pragma unroll
+        for (u32 i = 0; i < 2; ++i)
+#pragma unroll
+            for (u32 k = 0; k < 8; ++k) {
+                a0[i][k] = reduce_f64(b0[i] + pb[k], q, qinv);
+                a1[i][k] = b1[i];
+            }
+    }
+    u64 nx0[2], nx1[2], np[8];
+#define LIN_TLOAD(rr)                                                          \
+    _Pragma("unroll") for (u32 i = 0; i < 2; ++i) {                            \
+        nx0[i] = P[c0[i][rr] + idx];                                           \
+        nx1[i] = P[c1[i][rr] + idx];                                           \
+    }                                                                          \
+    _Pragma("unroll") for (u32 k = 0; k < 8; ++k) np[k] = P[pr[k][rr] + idx];
+    LIN_TLOAD(0u)
+    for (u32 r = 0; r < R; ++r) {
+        double x0[2], x1[2], p[8];
+#pragma unroll
+        for (u32 i = 0; i < 2; ++i) {
+            x0[i] = u64_to_f64(nx0[i]);
+            x1[i] = u64_to_f64(nx1[i]);
+        }
+#pragma unroll
+        for (u32 k = 0; k < 8; ++k) p[k] = u64_to_f64(np[k]);
+        LIN_TLOAD(min(r + 1, R - 1))
+#pragma unroll
+        for (u32 i = 0; i < 2; ++i)
+#pragma unroll
+            for (u32 k = 0; k < 8; ++k) {
+                a0[i][k] += mulmod_vv_f64(x0[i], p[k], q, qinv);
+                a1[i][k] += mulmod_vv_f64(x1[i], p[k], q, qinv);
+            }
+        if ((r & 7) == 7) {
+#pragma unroll
+            for (u32 i = 0; i < 2; ++i)
+#pragma unroll
+                for (u32 k = 0; k < 8; ++k) {
+                    a0[i][k] = reduce_f64(a0[i][k], q, qinv);
+                    a1[i][k] = reduce_f64(a1[i][k], q, qinv);
+                }
+        }
+    }
+#undef LIN_TLOAD
+#pragma unroll
+    for (u32 i = 0; i < 2; ++i)
+#pragma unroll
+        for (u32 k = 0; k < 8; ++k)
+            if (i < uq && k < ug) {
+                const size_t o = (size_t)(U.q0 + i) * U.ng + U.g0 + k;
+                P[V.o0[o] + idx] = f64_to_canon(reduce_f64(a0[i][k], q, qinv), q);
+                P[V.o1[o] + idx] = f64_to_canon(reduce_f64(a1[i][k], q, qinv), q);
+            }
+}
+
 bool linear_sum_f64(const DevTables &t) { return t.use_f64 && !(t.f64_off & 8); }
 
 void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,
@@ -804,6 +917,10 @@ void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32
     if (shape == LIN_SHAPE_LDS) {
         dim3 grid(t.N / LIN_POS, limbs, n_units);
         hipLaunchKernelGGL(k_linear_sum_lds_f64, grid, dim3(LIN_TPB), 0, s, P, units_dev, R, t.mp, t.logN);
+    } else if (shape == LIN_SHAPE_TILES) {
+        dim3 grid(t.N / LIN_POS, limbs, (n_units + LIN_TW - 1) / LIN_TW);
+        hipLaunchKernelGGL(k_linear_sum_tiles_f64, grid, dim3(LIN_TW * LIN_POS), 0, s, P, units_dev, n_units, R,
+                           t.mp, t.logN);
     } else {
         dim3 grid(t.N / TPB, limbs, n_units);
         hipLaunchKernelGGL((k_linear_sum<1, 8>), grid, dim3(TPB), 0, s, P, units_dev, R, t.mp, t.logN);

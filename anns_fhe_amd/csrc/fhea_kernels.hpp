@@ -161,15 +161,16 @@ void launch_add_batch(const DevTables &t, const AddArgs *args_dev, u32 nbatch, u
 // (outputs row-major by query); ptb and bias entries may be LIN_NONE.  A unit is
 // the queries q0..q0+uq-1 and blocks g0..g0+ug-1 of one group that one grid slice
 // evaluates: FP64 (LIN_SHAPE_LDS) uq <= 2*qa, ug <= 8*(8/qa), qa in {2, 4, 8}
-// (2 x 8 tiles, 8 per unit, qa of them stacked); integer (LIN_SHAPE_1x8) uq = 1,
-// ug <= 8.
+// (2 x 8 tiles, 8 per unit, qa of them stacked); FP64 small groups
+// (LIN_SHAPE_TILES) uq <= 2, ug <= 8, one wave each; integer (LIN_SHAPE_1x8)
+// uq = 1, ug <= 8.
 static constexpr u64 LIN_NONE = ~0ull;
 static constexpr u32 LIN_NS = 40;  // LDS streams: 2*uq + ug <= 40 for every shape
 struct LinUnit {
     const u64 *offs;
     u32 nq, ng, q0, g0, uq, ug, qa, pad;
 };
-enum : int { LIN_SHAPE_1x8 = 0, LIN_SHAPE_LDS = 1 };
+enum : int { LIN_SHAPE_1x8 = 0, LIN_SHAPE_LDS = 1, LIN_SHAPE_TILES = 2 };
 // true when the FP64 kernels run (the LDS shape exists only there)
 bool linear_sum_f64(const DevTables &t);
 void launch_linear_sum(const DevTables &t, u64 *P, const LinUnit *units_dev, u32 n_units, u32 R, u32 limbs,

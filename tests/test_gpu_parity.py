@@ -686,16 +686,18 @@ def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
 
 @pytest.mark.parametrize("q_bits", [50, 60])
 def test_linear_sum_groups_parity(orc, q_bits):
-    """fhea_eval_linear_sum_groups: groups of 5x6, 1x3 and 4x4 (query x block),
-    queries and blocks shared between groups, partial 4x4 units; every pair equals
-    the oracle's linear sum (FP64 4x4 units at 50-bit primes, integer 1x8 units at 60)."""
+    """fhea_eval_linear_sum_groups: groups of 5x6, 1x3, 4x4 and 3x9 (query x block),
+    queries and blocks shared between groups, absent biases and block biases; every
+    pair equals the oracle's linear sum (at 50-bit primes FP64: the 5x6 group as one
+    LDS unit, the others as 2x8 tiles, partial in both directions; integer 1x8 units
+    at 60)."""
     from anns_fhe_amd._lib import NONE
     N, L = 4096, 3
     ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
     o = get_orc(orc, ctx)
     rng = np.random.default_rng(70 + q_bits)
     W = L * N
-    R, n_q, n_b = 7, 6, 7
+    R, n_q, n_b = 7, 6, 9
     n_in = n_q * 2 * R + n_b * R + n_b + 2 * n_q
     data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(n_in)])
     ctx.write_poly(0, data)
@@ -705,7 +707,8 @@ def test_linear_sum_groups_parity(orc, q_bits):
     bias = lambda z: ((n_q * 2 * R + n_b * R + n_b + 2 * z) * W, (n_q * 2 * R + n_b * R + n_b + 2 * z + 1) * W)
     out = n_in
     groups, pairs = [], []
-    for zs, bs in (([0, 1, 2, 3, 4], [0, 1, 2, 3, 4, 5]), ([5], [6, 0, 2]), ([2, 3, 5, 1], [3, 4, 5, 6])):
+    for zs, bs in (([0, 1, 2, 3, 4], [0, 1, 2, 3, 4, 5]), ([5], [6, 0, 2]), ([2, 3, 5, 1], [3, 4, 5, 6]),
+                   ([4, 0, 3], [8, 7, 6, 5, 4, 3, 2, 1, 0])):
         o0 = np.array([[(out + 2 * (i * len(bs) + g)) * W for g in range(len(bs))] for i in range(len(zs))], np.uint64)
         groups.append(dict(ct0=[ctq(z, 0) for z in zs], ct1=[ctq(z, 1) for z in zs], pt=[blk(b) for b in bs],
                            ptb=[ptb(b) for b in bs], bias=[bias(z) if z != 4 else (NONE, NONE) for z in zs],

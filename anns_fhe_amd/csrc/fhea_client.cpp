@@ -194,6 +194,23 @@ __attribute__((target("avx2,fma"))) void dec_mul_f64(u64 *c, const u64 *c0, cons
     }
 }
 
+// c0 = v b + e0 + m, c1 = v a + e1 (mod q < 2^50), every operand canonical: the
+// product as in dec_mul_f64 (|r| <= 0.875q), the sums below 3q, one reduction.
+__attribute__((target("avx2,fma"))) void enc_mul_f64(u64 *c0, u64 *c1, const u64 *v, const u64 *b, const u64 *a,
+                                                    const u64 *e0, const u64 *e1, const u64 *m, u32 N, u64 q) {
+#pragma clang fp contract(off)
+    const double qd = (double)q, qinv = 1.0 / qd;
+    for (u32 j = 0; j < N; ++j) {
+        const double x = u52_to_f64(v[j]), pb = u52_to_f64(b[j]), pa = u52_to_f64(a[j]);
+        const double hb = x * pb, lb = std::fma(x, pb, -hb), kb = std::rint(hb * qinv);
+        const double ha = x * pa, la = std::fma(x, pa, -ha), ka = std::rint(ha * qinv);
+        const double y0 = hreduce(std::fma(-kb, qd, hb) + lb + u52_to_f64(e0[j]) + u52_to_f64(m[j]), qd, qinv);
+        const double y1 = hreduce(std::fma(-ka, qd, ha) + la + u52_to_f64(e1[j]), qd, qinv);
+        c0[j] = f64_to_u52(y0 < 0.0 ? y0 + qd : y0);
+        c1[j] = f64_to_u52(y1 < 0.0 ? y1 + qd : y1);
+    }
+}
+
 // the FP64 twiddles of prime pidx (built on first use)
 const HostF64Twiddles::Prime &f64_twiddles(const HostTables &h, u32 pidx) {
     HostF64Twiddles::Prime &P = h.f64tw->p[pidx];
@@ -890,8 +907,15 @@ int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, 
                      e1_e = to_eval(h, e1.data(), qp);
     const u64 *b = pk, *a = pk + (size_t)L * N;
     u64 *c0 = ct, *c1 = ct + (size_t)limbs * N;
+    const HostNtt ntt(h);
     for (u32 i = 0; i < limbs; ++i) {
         const u64 q = h.primes[i];
+        const size_t at = (size_t)i * N;
+        if (ntt.f64(q)) {
+            enc_mul_f64(c0 + at, c1 + at, v_e.data() + at, b + at, a + at, e0_e.data() + at, e1_e.data() + at, pt + at,
+                        N, q);
+            continue;
+        }
         const ModParams mq = make_mod_params(q);
         for (u32 j = 0; j < N; ++j) {
             const size_t o = (size_t)i * N + j;

@@ -130,8 +130,9 @@ def test_batch_decrypt_decode_equals_single_calls():
 def test_fp64_host_transforms_equal_integer_ones(orc, N, q_bits):
     """The CPU transforms on exact FP64 products (AVX2+FMA, primes below 2^50) give
     the words of the 64-bit integer ones: the secret's evaluation form (forward),
-    the plaintexts of encode (forward) and decrypt+decode (inverse) agree exactly
-    between the two modes, and the forward form equals the oracle's NTT."""
+    the plaintexts of encode (forward), the ciphertexts of encrypt (forward and the
+    FP64 products) and decrypt+decode (inverse) agree exactly between the two
+    modes, and the forward form equals the oracle's NTT."""
     L = 3
     cl = _client(N, L, 1, 3, q_bits=q_bits)
     o = orc.OracleCtx(N, cl.q, cl.p, 3)

@@ -573,7 +573,7 @@ class FheIvfPqServer:
             self._distances_groups(groups, {z: len(b) for z, b in blocks.items() if b})
 
 
-def execute_interactive_queries(server, client, queries, n_probe, top_k, client_threads=16, stage3_chunks=4):
+def execute_interactive_queries(server, client, queries, n_probe, top_k, client_threads=16, stage3_chunks=8):
     """Runs the whole protocol for a batch of queries; returns per query
     (ids, distances, probes) and a timing record.  Client-side work (encryption,
     decryption, selection) runs on `client_threads` host threads, as independent

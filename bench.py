@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
-    ap.add_argument("--stage3-chunks", type=int, default=4,
+    ap.add_argument("--stage3-chunks", type=int, default=8,
                     help="ivfpq: stage-3 parts whose result reads overlap the next part's evaluation")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
     ap.add_argument("--plain-steps", type=int, default=3,

@@ -11,7 +11,7 @@
 // Keys.  Digit p of a key switching s' -> s is, over the Q limbs of the key level
 // and all P limbs (evaluation domain):
 //     a_p uniform,   b_p = -a_p s + e_p + [limb i in digit p] (P mod q_i) s'
-// with e_p Gaussian (sigma 3.2).  Public key: (b, a) = (-a s + e, a) over the Q limbs.
+// with e_p rounded Gaussian (sigma 3.2).  Public key: (b, a) = (-a s + e, a) over the Q limbs.
 //
 // Encoding (canonical embedding).  zeta = exp(i pi / N).  Slot j holds
 // m(zeta^(5^j mod 2N)) / scale, j < N/2; the conjugate slots are implied, so m
@@ -364,9 +364,38 @@ std::vector<int64_t> s_to_i64(const int8_t *s, u32 N) {
     return v;
 }
 
+// Rounded Gaussian noise: x = round(y), y ~ N(0, 3.2^2), sampled by inversion of
+// the distribution of |x| (64-bit thresholds t[k] = 2^64 P(|x| <= k), P(|x| <= k) =
+// 1 - erfc((k + 1/2) / (3.2 sqrt 2))) and a sign bit; |x| > 40 has probability
+// below 2^-64 and is not produced.
+struct GaussTable {
+    static constexpr u32 K = 40;
+    u64 t[K + 1];
+    GaussTable() {
+        const long double two64 = 18446744073709551616.0L, d = 3.2L * std::sqrt(2.0L);
+        for (u32 k = 0; k <= K; ++k) {
+            const long double tail = std::erfc(((long double)k + 0.5L) / d) * two64;  // 2^64 P(|x| > k)
+            t[k] = tail < 1.0L ? ~0ull : (u64)(two64 - tail);
+        }
+    }
+};
+
 void gaussian(std::mt19937_64 &rng, std::vector<int64_t> &e) {
-    std::normal_distribution<double> g(0.0, 3.2);
-    for (auto &x : e) x = (int64_t)std::llround(g(rng));
+    static const GaussTable g;
+    u64 signs = 0;
+    u32 left = 0;
+    for (auto &x : e) {
+        const u64 u = rng();
+        u32 k = 0;
+        while (k < GaussTable::K && u >= g.t[k]) ++k;
+        if (left == 0) {
+            signs = rng();
+            left = 64;
+        }
+        x = (signs & 1) ? -(int64_t)k : (int64_t)k;
+        signs >>= 1;
+        --left;
+    }
 }
 
 // Key for s_new -> s (both evaluation domain on key_primes), host layout
@@ -379,7 +408,6 @@ std::vector<u64> make_switch_key(const HostTables &h, const std::vector<u64> &s_
     std::vector<u64> key(2ull * dnum * R * N);
     u64 *ka = key.data(), *kb = key.data() + (size_t)dnum * R * N;
     std::mt19937_64 rng(seed);
-    std::normal_distribution<double> gauss(0.0, 3.2);
     std::vector<int64_t> e(N);
     std::vector<u64> pmod(R);  // P mod q_i
     for (u32 r = 0; r < R; ++r) {
@@ -388,7 +416,7 @@ std::vector<u64> make_switch_key(const HostTables &h, const std::vector<u64> &s_
         pmod[r] = pm;
     }
     for (u32 p = 0; p < dnum; ++p) {
-        for (u32 j = 0; j < N; ++j) e[j] = (int64_t)std::llround(gauss(rng));
+        gaussian(rng, e);
         std::vector<u64> e_e = to_eval(h, e.data(), kp);
         for (u32 r = 0; r < R; ++r) {
             const u64 q = h.primes[kp[r]];

@@ -218,3 +218,38 @@ def test_large_scale_encoding_and_batch():
     batch = cl.encode_batch(vals, 2, 2.0 ** 45, threads=3)
     for k in range(7):
         assert np.array_equal(batch[k], cl.encode(vals[k], 2, 2.0 ** 45))
+
+
+def test_key_noise_is_rounded_gaussian(orc):
+    """The public key's noise e = b + a s (taken to coefficients by the oracle's
+    inverse NTT; every limb gives the same small integers) follows round(N(0, 3.2^2)):
+    mean 0, variance 3.2^2 + 1/12, |e| <= 40, and the frequency of each |e| = k,
+    k <= 10, within 5 binomial standard deviations of 2^[k>0] (Phi((k+1/2)/3.2) -
+    Phi((k-1/2)/3.2))."""
+    import math
+    N, L = 4096, 2
+    cl = _client(N, L, 1, 2, q_bits=50)
+    o = orc.OracleCtx(N, cl.q, cl.p, 2)
+    es = []
+    for seed in range(8):
+        s = cl.keygen_secret(100 + seed)
+        pk = cl.keygen_public(s, 200 + seed).reshape(2, L, N)
+        se = cl.secret_eval(s, L).reshape(L, N)
+        limbs = []
+        for i in range(L):
+            q = int(cl.q[i])
+            ee = np.array([(int(b) + int(a) * int(x)) % q for b, a, x in zip(pk[0][i], pk[1][i], se[i])], np.uint64)
+            c = o.intt(ee, i).astype(object)
+            limbs.append(np.array([int(v) - q if int(v) > q // 2 else int(v) for v in c], np.int64))
+        assert np.array_equal(limbs[0], limbs[1])
+        es.append(limbs[0])
+    e = np.concatenate(es)
+    n = e.size
+    assert abs(e.mean()) < 0.1
+    assert abs(e.var() - (3.2 ** 2 + 1.0 / 12.0)) < 0.4
+    assert np.abs(e).max() <= 40
+    Phi = lambda x: 0.5 * math.erfc(-x / math.sqrt(2.0))
+    for k in range(11):
+        p = (Phi((k + 0.5) / 3.2) - Phi((k - 0.5) / 3.2)) * (2 if k else 1)
+        got = int((np.abs(e) == k).sum())
+        assert abs(got - n * p) <= 5 * math.sqrt(n * p * (1 - p)) + 2, (k, got, n * p)

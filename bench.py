@@ -328,6 +328,25 @@ def bench_ivfpq(args):
 
     for _ in range(args.warmup):
         IV.execute_interactive_queries(server, client, batch(), n_probe, 10, stage3_chunks=args.stage3_chunks)
+    schedule = None
+    if not args.no_tune:
+        # the schedule tuner on stage 1 of the warm-up's queries (run eagerly; its HIP
+        # graph is recorded again afterwards under the chosen schedule)
+        from anns_fhe_amd import autotune
+        slots = list(range(B))
+        use_graphs, server.use_graphs = server.use_graphs, False
+        t0 = time.perf_counter()
+        autotune.tune(ctx, lambda: server.stage1_coarse_centroid_distance(slots),
+                      check=lambda: np.concatenate([ctx.read_poly(server._offs(z)["res"], 2 * (lt - 1) * Nq)
+                                                    for z in (0, B - 1)]),
+                      knobs=("ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"), reps=3, log=log)
+        server.use_graphs = use_graphs
+        for gid in server.graphs.values():
+            ctx.graph_destroy(gid)
+        server.graphs.clear()
+        schedule = autotune.current(ctx)
+        log(f"ivfpq: autotune {schedule} in {time.perf_counter() - t0:.1f} s")
+        IV.execute_interactive_queries(server, client, batch(), n_probe, 10, stage3_chunks=args.stage3_chunks)
     log("ivfpq: warmup done")
     ps = IV.PlaintextSearcher(ix)
     import concurrent.futures as cf
@@ -362,7 +381,7 @@ def bench_ivfpq(args):
                                "(coarse distances, client cluster choice, candidate distances, client top-10)",
                    "db": n, "dim": 128, "n_list": 32, "M": 8, "K": 256, "N": Nq, "L": Lq, "K_special": Kq,
                    "dnum": dq, "eval_limbs": lt, "n_probe": n_probe, "batch": B, "blocks": n_blocks,
-                   "stage1_graph": server.use_graphs, "stage3_chunks": args.stage3_chunks},
+                   "stage1_graph": server.use_graphs, "stage3_chunks": args.stage3_chunks, "schedule": schedule},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2),
                        "what": "end-to-end batch (client encrypt/decrypt + server), per query = per batch"},

@@ -100,7 +100,7 @@ def candidates(ctx, knob):
         return [0, 4, 3, 2]
     if knob == "streams":
         return [1, 2]
-    return [4, 8, 16, 32]
+    return [4, 8, 16, 32, 64]
 
 
 def tune(ctx, run, check=None, knobs=("streams", "ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"),

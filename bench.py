@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
+    ap.add_argument("--serve-chunk", type=int, default=0,
+                    help="serve: key-switch items per pipeline chunk in every parameter set (0 = engine default)")
     ap.add_argument("--stage3-chunks", type=int, default=8,
                     help="ivfpq: stage-3 parts whose result reads overlap the next part's evaluation")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
@@ -410,6 +412,10 @@ def bench_serve(args):
     larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, 8, 3, ix, max_slots=16, seed=2,
                                  log=log, group="n16") for i in range(nrep)]
     large = larges[0]
+    if args.serve_chunk:
+        from anns_fhe_amd import _lib
+        for p in [small] + larges:
+            p.ctx.set_tuning(_lib.TUNE_CHUNK, args.serve_chunk)
     budget = small.static_bytes + nrep * large.static_bytes + args.serve_small_slots * small.slot_bytes + \
         args.serve_large_slots * large.slot_bytes
     rng = np.random.default_rng(7)
@@ -478,7 +484,7 @@ def bench_serve(args):
         "config": {"workload": "Poisson stream of interactive encrypted IVF-PQ queries, 50/50 over two parameter "
                                "sets, n_probe uniform 1..8, HBM-budget admission, continuous batching",
                    "db": n, "requests_per_rate": args.requests, "budget_bytes": budget,
-                   "large_replicas": nrep},
+                   "large_replicas": nrep, "ks_chunk": args.serve_chunk or None},
         "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "at_rate": best["rate"]},
         "sweep": sweep,
         "plain_path": plain,

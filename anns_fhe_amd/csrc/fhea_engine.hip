@@ -26,12 +26,27 @@ int check_hip(hipError_t e) { return e == hipSuccess ? FHEA_OK : FHEA_ERR_DEVICE
         if ((x) != hipSuccess) return FHEA_ERR_DEVICE; \
     } while (0)
 
+// Drops a device buffer that is being replaced by a larger one.  A recorded graph
+// may address it (its kernel nodes hold raw pointers), so while any graph exists the
+// buffer is parked in c->retired; otherwise it is freed once the stream is idle.
+int retire_buffer(fhea_ctx *c, void *p) {
+    if (!p) return FHEA_OK;
+    if (!c->graphs.empty()) {
+        c->retired.push_back(p);
+        return FHEA_OK;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipFree(p));
+    return FHEA_OK;
+}
+
 int ensure_ws(fhea_ctx *c, u64 words) {
     if (words <= c->ws_words) return FHEA_OK;
     if (c->capturing) return FHEA_ERR_NO_MEM;  // captured kernels may already point at it
     if (c->ws) {
-        HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipFree(c->ws));
+        const int st = retire_buffer(c, c->ws);
+        if (st) return st;
         c->ws = nullptr;
         c->ws_words = 0;
     }
@@ -605,10 +620,20 @@ struct MultItem {
     u64 *d0, *d1;
 };
 
+// A caller-provided workspace: the pipeline uses it as is and never grows it.
+struct FixedWs {
+    u64 *p;
+    u64 words;
+};
+
 // eval_mult_relin_rescale (fhe_host_driver.cpp:272-306) for a batch sharing one
-// relinearisation key: tensor -> KS(d2) (+d0, +d1 fused) -> rescale.
+// relinearisation key: tensor -> KS(d2) (+d0, +d1 fused) -> rescale.  With `fixed`
+// the chunks run in that workspace (one item per chunk if need be) instead of
+// the context's, and nothing is allocated.
 static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
-                                   const std::vector<MultItem> &items);
+                                   const std::vector<MultItem> &items, const FixedWs *fixed = nullptr);
+// workspace words of one chunk of nb items
+static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb);
 
 // Swaps the context's stream and workspace with the second lane's.
 static void swap_lane(fhea_ctx *c) {
@@ -638,16 +663,31 @@ static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
     return st ? st : st2;
 }
 
+static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb) {
+    return (u64)nb * 3 * l * c->h.N + std::max<u64>(nb * ks_words(c, l), nb * rescale_words(c, l));
+}
+
 static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
-                                   const std::vector<MultItem> &items) {
+                                   const std::vector<MultItem> &items, const FixedWs *fixed) {
     const u64 N = c->h.N;
-    for (size_t off = 0; off < items.size(); off += c->max_chunk) {
-        u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+    u32 chunk = c->max_chunk;
+    if (fixed) {
+        while (chunk > 1 && mult_relin_rescale_words(c, l, chunk) > fixed->words) --chunk;
+        if (mult_relin_rescale_words(c, l, chunk) > fixed->words) return FHEA_ERR_NO_MEM;
+    }
+    for (size_t off = 0; off < items.size(); off += chunk) {
+        u32 nb = (u32)std::min<size_t>(chunk, items.size() - off);
         u64 dwords = (u64)nb * 3 * l * N;
-        u64 need = dwords + std::max<u64>(nb * ks_words(c, l), nb * rescale_words(c, l));
-        int st = ensure_ws(c, need);
-        if (st) return st;
-        u64 *D = c->ws, *rest = D + dwords;
+        u64 *D = nullptr;
+        if (fixed) {
+            D = fixed->p;
+        } else {
+            int st = ensure_ws(c, mult_relin_rescale_words(c, l, nb));
+            if (st) return st;
+            D = c->ws;
+        }
+        u64 *rest = D + dwords;
+        int st = FHEA_OK;
         std::vector<TensorArgs> ta(nb);
         std::vector<CombineArgs> cb(nb);
         std::vector<const u64 *> keys(nb, key);
@@ -846,6 +886,26 @@ static int key_check(const fhea_ctx *c, u64 evk, u32 key_limbs) {
     return range_ok(evk, words, c->key_words) ? FHEA_OK : FHEA_ERR_RANGE;
 }
 
+// Items of one fhea_submit group run as one pipeline (several chunks, possibly two
+// streams), so an item may read its own inputs in place but no other item's outputs:
+// every output range must be disjoint from every other output range and from the
+// input ranges of the other items.  ins/outs: (word offset, item index); inputs span
+// in_w words, outputs out_w words.
+static bool batch_aliasing_ok(std::vector<std::pair<u64, u32>> outs, const std::vector<std::pair<u64, u32>> &ins,
+                              u64 in_w, u64 out_w) {
+    std::sort(outs.begin(), outs.end());
+    for (size_t k = 0; k + 1 < outs.size(); ++k)
+        if (outs[k + 1].first - outs[k].first < out_w) return false;
+    for (const auto &x : ins) {
+        // first output whose range ends after the input starts
+        auto it = std::lower_bound(outs.begin(), outs.end(), x.first >= out_w - 1 ? x.first - (out_w - 1) : 0,
+                                   [](const std::pair<u64, u32> &a, u64 v) { return a.first < v; });
+        for (; it != outs.end() && it->first < x.first + in_w; ++it)
+            if (it->second != x.second) return false;
+    }
+    return true;
+}
+
 extern "C" {
 
 const char *fhea_status_string(int st) {
@@ -933,6 +993,8 @@ int fhea_ctx_destroy(fhea_ctx *c) {
         if (kv.second.host) hipHostFree(kv.second.host);
     }
     c->graphs.clear();
+    for (void *p : c->retired) hipFree(p);
+    c->retired.clear();
     if (c->poly) hipFree(c->poly);
     if (c->key) hipFree(c->key);
     if (c->ws) hipFree(c->ws);
@@ -1090,7 +1152,10 @@ int fhea_read_polys_async(fhea_ctx *c, uint32_t n, const uint64_t *off, const ui
 
 int fhea_read_wait(fhea_ctx *c, uint32_t ticket) {
     CTX_CHECK(c);
-    if (ticket >= c->read_seq || c->read_seq - ticket > fhea_ctx::READ_RING) return FHEA_ERR_ARG;
+    if (ticket >= c->read_seq) return FHEA_ERR_ARG;
+    // a ticket more than READ_RING reads old has completed: fhea_read_polys_async
+    // waited for its event before handing the event to a later read
+    if (c->read_seq - ticket > fhea_ctx::READ_RING) return FHEA_OK;
     return check_hip(hipEventSynchronize(c->read_done[ticket % fhea_ctx::READ_RING]));
 }
 
@@ -1370,8 +1435,7 @@ int fhea_eval_rotate(fhea_ctx *c, uint64_t in0, uint64_t in1, uint64_t d0, uint6
         if (c->pp_words < 4 * W) {
             if (c->capturing) return FHEA_ERR_NO_MEM;
             if (c->pp) {
-                if (hipStreamSynchronize(c->stream) != hipSuccess || hipFree(c->pp) != hipSuccess)
-                    return FHEA_ERR_DEVICE;
+                if (retire_buffer(c, c->pp)) return FHEA_ERR_DEVICE;
                 c->pp = nullptr;
                 c->pp_words = 0;
             }
@@ -1413,11 +1477,17 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
             u32 kl = c->relin_limbs ? c->relin_limbs : l;
             if (kl < l) return FHEA_ERR_ARG;
             std::vector<MultItem> v;
+            std::vector<std::pair<u64, u32>> ins, outs;
             for (uint32_t k = i; k < j; ++k) {
                 const fhea_work &w = items[k];
                 if ((st = check_mult_args(c, w.a0, w.a1, w.b0, w.b1, w.d0, w.d1, l))) return st;
                 v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+                for (u64 o : {w.a0, w.a1, w.b0, w.b1}) ins.push_back({o, k});
+                outs.push_back({w.d0, k});
+                outs.push_back({w.d1, k});
             }
+            if (!batch_aliasing_ok(std::move(outs), ins, (u64)l * c->h.N, (u64)(l - 1) * c->h.N))
+                return FHEA_ERR_ARG;
             st = mult_relin_rescale(c, l, c->key + c->relin_off, kl, v);
         } else if (w0.kind == FHEA_WORK_AUTOMORPHISM_KS) {
             if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
@@ -1425,6 +1495,7 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
             if (kl < l || kl > c->h.L) return FHEA_ERR_ARG;
             const u64 W = (u64)l * c->h.N;
             std::vector<AutoItem> v;
+            std::vector<std::pair<u64, u32>> ins, outs;
             for (uint32_t k = i; k < j; ++k) {
                 const fhea_work &w = items[k];
                 POLY_CHECK(c, w.a0, W);
@@ -1434,7 +1505,12 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                 if ((w.galois & 1) == 0 || w.galois >= 2ull * c->h.N) return FHEA_ERR_ARG;
                 if ((st = key_check(c, w.evk, kl))) return st;
                 v.push_back({P + w.a0, P + w.a1, P + w.d0, P + w.d1, w.galois, c->key + w.evk});
+                ins.push_back({w.a0, k});
+                ins.push_back({w.a1, k});
+                outs.push_back({w.d0, k});
+                outs.push_back({w.d1, k});
             }
+            if (!batch_aliasing_ok(std::move(outs), ins, W, W)) return FHEA_ERR_ARG;
             st = automorphism_ks(c, l, kl, v);
         } else if (w0.kind == FHEA_WORK_MULT_RELIN) {
             if (!c->has_relin) return FHEA_ERR_NO_KEY;
@@ -1442,6 +1518,7 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
             if (kl < l) return FHEA_ERR_ARG;
             const u64 W = (u64)l * c->h.N;
             std::vector<MultItem> v;
+            std::vector<std::pair<u64, u32>> ins, outs;
             for (uint32_t k = i; k < j; ++k) {
                 const fhea_work &w = items[k];
                 if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
@@ -1452,7 +1529,11 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                 POLY_CHECK(c, w.d0, W);
                 POLY_CHECK(c, w.d1, W);
                 v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+                for (u64 o : {w.a0, w.a1, w.b0, w.b1}) ins.push_back({o, k});
+                outs.push_back({w.d0, k});
+                outs.push_back({w.d1, k});
             }
+            if (!batch_aliasing_ok(std::move(outs), ins, W, W)) return FHEA_ERR_ARG;
             st = mult_relin(c, l, c->key + c->relin_off, kl, v);
         } else if (w0.kind == FHEA_WORK_ROTATE_HOISTED) {
             // runs of items with the same input share one ModUp; when no output of
@@ -1512,6 +1593,7 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
             if (l == 0 || l > c->h.L) return FHEA_ERR_ARG;
             const u64 W = (u64)l * c->h.N;
             std::vector<AddArgs> v;
+            std::vector<std::pair<u64, u32>> ins, outs;
             for (uint32_t k = i; k < j; ++k) {
                 const fhea_work &w = items[k];
                 POLY_CHECK(c, w.a0, W);
@@ -1521,7 +1603,11 @@ int fhea_submit(fhea_ctx *c, const fhea_work *items, uint32_t n) {
                 POLY_CHECK(c, w.d0, W);
                 POLY_CHECK(c, w.d1, W);
                 v.push_back({P + w.a0, P + w.a1, P + w.b0, P + w.b1, P + w.d0, P + w.d1});
+                for (u64 o : {w.a0, w.a1, w.b0, w.b1}) ins.push_back({o, k});
+                outs.push_back({w.d0, k});
+                outs.push_back({w.d1, k});
             }
+            if (!batch_aliasing_ok(std::move(outs), ins, W, W)) return FHEA_ERR_ARG;
             for (size_t o = 0; o < v.size(); o += 65535) {
                 const u32 nb = (u32)std::min<size_t>(65535, v.size() - o);
                 auto *ad = (const AddArgs *)stage_args(c, v.data() + o, nb * sizeof(AddArgs));
@@ -1851,6 +1937,11 @@ int fhea_graph_destroy(fhea_ctx *c, uint32_t graph_id) {
     hipFree(it->second.dev);
     hipHostFree(it->second.host);
     c->graphs.erase(it);
+    if (c->graphs.empty()) {
+        // no recorded kernel addresses the replaced workspaces any more
+        for (void *p : c->retired) hipFree(p);
+        c->retired.clear();
+    }
     return FHEA_OK;
 }
 

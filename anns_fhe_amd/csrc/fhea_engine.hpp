@@ -61,6 +61,10 @@ struct fhea_ctx {
     };
     std::map<uint32_t, Graph> graphs;
     uint32_t next_graph = 1;
+    // device buffers (workspace, rotation ping-pong) replaced while graphs existed:
+    // recorded kernel nodes may still address them, so they are freed only when the
+    // last graph is destroyed (or with the context)
+    std::vector<void *> retired;
 
     // second lane (FHEA_TUNE_STREAMS = 2): a batched HMult runs half of its items on
     // a second stream with its own workspace, forked from and joined back into

@@ -113,7 +113,8 @@ int fhea_write_polys(fhea_ctx *ctx, uint32_t n, const uint64_t *off, const uint6
                      const uint64_t *h);
 /* The same gathered read, returned at once: the copies run on a copy stream after
  * the work submitted so far, beside the kernels submitted next; *ticket names the
- * read for fhea_read_wait, which must come within 16 reads.  Work submitted before
+ * read for fhea_read_wait (a ticket 16 or more reads old is complete already: the
+ * call returns FHEA_OK at once; a ticket not yet issued is FHEA_ERR_ARG).  Work submitted before
  * the wait must not write the ranges; h must stay valid until then (page-locked h
  * lets the copies overlap kernels).  No reference counterpart (reads overlapped with
  * later evaluation, for the batched query stages). */
@@ -193,6 +194,11 @@ typedef struct {
     uint32_t key_limbs;
     uint32_t reserved;
 } fhea_work;
+/* Consecutive items of the same kind, level and key level form one group, run as
+ * one pipeline; groups run in order.  Within a group of kinds 1, 2, 5 and 6 an item
+ * may write over its own inputs, but no output may overlap another output or an
+ * input of another item of the group (FHEA_ERR_ARG): the group's items run in
+ * chunks and, with FHEA_TUNE_STREAMS = 2, on two streams at once. */
 int fhea_submit(fhea_ctx *ctx, const fhea_work *items, uint32_t n_items);
 /* Batched forward (inverse != 0: inverse) NTT of npolys contiguous polynomials
  * of `limbs` limbs each (poly z at src + z*limbs*N), limb i mod prime i

@@ -1,0 +1,181 @@
+"""GPU parity at the headline sizes and on the integer path.
+
+* The benchmark's own configuration: batched eval_mult_relin_rescale (work kind 1)
+  at N=2^16, L=24, K=8, dnum=3, B=64 on two streams with key-switch chunks of 16
+  and of 64; items at the chunk and lane boundaries against the oracle.
+* eval_mult_plain_rescale (fhe_host_driver.cpp:308-329) at the C1 shape and at
+  N=2^16, L=24.
+* The integer kernels, which run when a prime is >= 2^50 or the chain spans a
+  ratio >= 4: 60-bit primes, and the reference's own chain shape (a 60-bit q0,
+  45-bit scaling primes, 60-bit special primes;
+  integration_tools/generate_interactive_test_vectors.cpp:161-166), at N=2^13,
+  the reference's N=2^14/L=11/K=4/dnum=3 and N=2^16/L=24/K=8/dnum=3: OP 1/2,
+  OP 7, OP 8, eval_mult_relin_rescale and eval_rotate against the oracle.
+All comparisons are bit-exact on every residue."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-fullsize]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_poly(rng, primes, N):
+    return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+
+
+def rand_key(rng, ctx, key_limbs):
+    L, K, N = ctx.L, ctx.K, ctx.N
+    kidx = list(range(key_limbs)) + [L + j for j in range(K)]
+    return np.concatenate([rand_poly(rng, [ctx.primes[i] for i in kidx], N).reshape(-1)
+                           for _ in range(2 * ctx.dnum)])
+
+
+def test_bench_config_batched_hmult_vs_oracle(orc):
+    """B=64 HMults sharing one relin key at N=2^16/L=24/K=8/dnum=3, two lanes
+    (items 32..63 on the second stream), chunks of 16 and of 64: items 0, 15, 16,
+    31, 32, 63 equal the oracle's eval_mult_relin_rescale."""
+    from anns_fhe_amd import _lib
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    N, L, K, dnum, B = 65536, 24, 8, 3, 64
+    W, W1 = L * N, (L - 1) * N
+    out_base = 4 * B * W
+    ctx = FheContext(N, L, K, dnum, poly_words=out_base + 2 * B * W1 + 8,
+                     key_words=2 * dnum * (L + K) * N + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        rng = np.random.default_rng(64)
+        key = rand_key(rng, ctx, L)
+        ctx.write_key(0, key)
+        drv.set_relin_key(0, L)
+        sample = (0, 15, 16, 31, 32, 63)
+        inputs = {}
+        say("writing", B, "ciphertext pairs")
+        for z in range(B):
+            d = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+            ctx.write_poly(4 * z * W, d)
+            if z in sample:
+                inputs[z] = d
+        works = [dict(kind=1, limbs=L, a0=4 * z * W, a1=(4 * z + 1) * W, b0=(4 * z + 2) * W,
+                      b1=(4 * z + 3) * W, d0=out_base + 2 * z * W1, d1=out_base + (2 * z + 1) * W1)
+                 for z in range(B)]
+        got = {}
+        for chunk in (16, 64):
+            ctx.set_tuning(_lib.TUNE_STREAMS, 2)
+            ctx.set_tuning(_lib.TUNE_CHUNK, chunk)
+            say("submit chunk", chunk)
+            ctx.write_poly(out_base, np.zeros(2 * B * W1, np.uint64))
+            ctx.submit(works)
+            ctx.sync()
+            got[chunk] = {z: ctx.read_poly(out_base + 2 * z * W1, 2 * W1) for z in sample}
+        ctx.set_tuning(_lib.TUNE_STREAMS, 1)
+        ctx.set_tuning(_lib.TUNE_CHUNK, 16)
+        o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
+        for z in sample:
+            say("oracle item", z)
+            pm = np.zeros(6 * W, np.uint64)
+            pm[:4 * W] = inputs[z]
+            assert o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+            for chunk in (16, 64):
+                g = got[chunk][z]
+                assert np.array_equal(g[:W1], pm[4 * W:4 * W + W1]), (chunk, z, "c0")
+                assert np.array_equal(g[W1:], pm[5 * W:5 * W + W1]), (chunk, z, "c1")
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("N,L,K,dnum", [(8192, 4, 1, 2), (65536, 24, 8, 3)])
+def test_mult_plain_rescale_vs_oracle(orc, N, L, K, dnum):
+    """fhea_eval_mult_plain_rescale: (c0 pt, c1 pt) then OP 8, also with the output
+    over the input (d0 == a0)."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    W, W1 = L * N, (L - 1) * N
+    ctx = FheContext(N, L, K, dnum, poly_words=8 * W, key_words=64)
+    try:
+        drv = FHEHostDriver(ctx)
+        rng = np.random.default_rng(N + L)
+        data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(3)])
+        data[:8] = ctx.q[0] - 1
+        ctx.write_poly(0, data)
+        say("mult_plain_rescale", N, L)
+        drv.eval_mult_plain_rescale(0, W, 2 * W, 3 * W, 4 * W, L)
+        got = ctx.read_poly(3 * W, 2 * W)
+        o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
+        pm = np.zeros(5 * W, np.uint64)
+        pm[:3 * W] = data
+        assert o.eval_mult_plain_rescale(pm, 0, W, 2 * W, 3 * W, 4 * W, L) == 0
+        assert np.array_equal(got[:W1], pm[3 * W:3 * W + W1])
+        assert np.array_equal(got[W:W + W1], pm[4 * W:4 * W + W1])
+        drv.eval_mult_plain_rescale(0, W, 2 * W, 0, W, L)  # in place
+        got = ctx.read_poly(0, 2 * W)
+        assert np.array_equal(got[:W1], pm[3 * W:3 * W + W1])
+        assert np.array_equal(got[W:W + W1], pm[4 * W:4 * W + W1])
+    finally:
+        ctx.close()
+
+
+def _chain(orc, N, L, K, kind):
+    """(q primes, p primes) for an integer-path parameter set."""
+    if kind == "q60":
+        ps = orc.gen_primes(2 * N, 60, L + K)
+        return ps[:L], ps[L:]
+    # the reference's chain shape: q0 of 60 bits, 45-bit scaling primes, 60-bit P
+    big = orc.gen_primes(2 * N, 60, K + 1)
+    return [big[0]] + orc.gen_primes(2 * N, 45, L - 1), big[1:]
+
+
+@pytest.mark.parametrize("N,L,K,dnum,kind", [(8192, 4, 1, 2, "q60"), (8192, 4, 1, 2, "ref_chain"),
+                                             (16384, 11, 4, 3, "ref_chain"), (65536, 24, 8, 3, "ref_chain")])
+def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    q, p = _chain(orc, N, L, K, kind)
+    W, W1 = L * N, (L - 1) * N
+    ksz = 2 * dnum * (L + K) * N
+    ctx = FheContext(N, L, K, dnum, q_primes=q, p_primes=p, poly_words=16 * W, key_words=2 * ksz + 8)
+    try:
+        assert ctx.q == q and ctx.p == p
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, q, p, dnum)
+        rng = np.random.default_rng(N + L + len(kind))
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g1 = ctx.galois_for_step(1)
+        drv.add_rotation_key(1, g1, ksz, L)
+        data = np.concatenate([rand_poly(rng, q, N).reshape(-1) for _ in range(4)])
+        for i in range(L):
+            data[i * N] = q[i] - 1
+        ctx.write_poly(0, data)
+        pm = np.zeros(16 * W, np.uint64)
+        pm[:4 * W] = data
+        say("int path", kind, N, L, "ntt/intt")
+        for op in (1, 2):
+            assert ctx.op(op, 0, 0, 4 * W, 0, L) == 0
+            assert o.accel_op(pm, km, op, 0, 0, 4 * W, 0, L) == 0
+            assert np.array_equal(ctx.read_poly(4 * W, W), pm[4 * W:5 * W]), op
+        say("int path", kind, "op7")
+        assert ctx.op(7, W, 0, 5 * W, 0, L, L) == 0
+        assert o.accel_op(pm, km, 7, W, 0, 5 * W, 0, L, L) == 0
+        assert np.array_equal(ctx.read_poly(5 * W, 2 * W), pm[5 * W:7 * W])
+        say("int path", kind, "op8")
+        assert ctx.op(8, 0, W, 7 * W, 0, L) == 0
+        assert o.accel_op(pm, km, 8, 0, W, 7 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(7 * W, 2 * W1), pm[7 * W:7 * W + 2 * W1])
+        say("int path", kind, "hmult")
+        drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 9 * W, 10 * W, L)
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 9 * W, 10 * W, L) == 0
+        assert np.array_equal(ctx.read_poly(9 * W, W1), pm[9 * W:9 * W + W1])
+        assert np.array_equal(ctx.read_poly(10 * W, W1), pm[10 * W:10 * W + W1])
+        say("int path", kind, "rotate")
+        drv.eval_rotate(9 * W, 10 * W, 11 * W, 12 * W, 1, L - 1)
+        assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 9 * W, 10 * W, 11 * W, 12 * W, 1, L - 1) == 0
+        assert np.array_equal(ctx.read_poly(11 * W, W1), pm[11 * W:11 * W + W1])
+        assert np.array_equal(ctx.read_poly(12 * W, W1), pm[12 * W:12 * W + W1])
+    finally:
+        ctx.close()

@@ -593,7 +593,8 @@ def test_async_reads_overlap_later_work():
     """fhea_write_polys puts back-to-back words into scattered ranges;
     fhea_read_polys_async: gathered reads on the copy stream equal the synchronous
     read, also while later kernels (NTTs on other words) run; tickets may be waited
-    out of order; a ticket more than 16 reads old, or not yet issued, is refused."""
+    out of order; a ticket more than 16 reads old returns at once (it is complete),
+    one not yet issued is refused."""
     from anns_fhe_amd._lib import FheError
     from anns_fhe_amd.driver import PinnedArray
     N, L = 8192, 3
@@ -622,8 +623,7 @@ def test_async_reads_overlap_later_work():
     for _ in range(17):
         t = ctx.read_polys_async(ranges[:1], pin.array[3 * ref.size:])
     ctx.read_wait(t)
-    with pytest.raises(FheError):
-        ctx.read_wait(tickets[0])
+    ctx.read_wait(tickets[0])  # more than 16 reads old: complete already
     pin.close()
 
 

@@ -46,17 +46,20 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # C++ host program over the C-ABI (the reference's main_sim flow)
-    exe = os.path.join(HERE, "c1_sequence")
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
-           os.path.join(CSRC, "host", "c1_sequence_main.cpp"), "-o", exe + ".tmp", "-L", HERE, "-lfhea",
-           "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"host program build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(exe + ".tmp", exe)
+    # C++ host programs over the C-ABI (the reference's main_sim flow: seeded keys,
+    # and the context and keys read from a test-vector directory)
+    exes = []
+    for name, src in (("c1_sequence", "c1_sequence_main.cpp"), ("dir_sequence", "dir_sequence_main.cpp")):
+        exe = os.path.join(HERE, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+               os.path.join(CSRC, "host", src), "-o", exe + ".tmp", "-L", HERE, "-lfhea", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host program build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        exes.append(exe)
     if verbose:
-        print(f"built {LIB} and {exe}")
+        print(f"built {LIB} and {', '.join(exes)}")
     return LIB
 
 

@@ -335,6 +335,21 @@ orc_ctx *orc_ctx_create(uint32_t N, uint32_t L, uint32_t K, uint32_t dnum,
     return c;
 }
 
+/* Tables from given primitive 2N-th roots (Q then P) instead of the minimal ones:
+ * the reference loads its twiddles from files (fhe_host_driver.cpp:56-66), whose
+ * roots are OpenFHE's choice; psi[i] must satisfy psi^N == -1 mod q_i (returns -1
+ * otherwise, tables unchanged). */
+int orc_ctx_set_psi(orc_ctx *c, const uint64_t *psi) {
+    uint32_t T = c->L + c->K;
+    for (uint32_t i = 0; i < T; i++)
+        if (orc_pow_mod(psi[i] % c->primes[i], c->N, c->primes[i]) != c->primes[i] - 1) return -1;
+    for (uint32_t i = 0; i < T; i++) {
+        c->psi[i] = psi[i] % c->primes[i];
+        orc_twiddles(c->primes[i], c->psi[i], c->N, c->tw + (size_t)i * c->N, c->itw + (size_t)i * c->N);
+    }
+    return 0;
+}
+
 void orc_ctx_destroy(orc_ctx *c) {
     if (!c) return;
     free(c->primes); free(c->psi); free(c->n_inv); free(c->m_lo); free(c->m_hi);

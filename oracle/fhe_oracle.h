@@ -57,6 +57,7 @@ typedef struct orc_ctx orc_ctx;
 /* q_primes: L, p_primes: K.  Twiddles use the minimal primitive 2N-th root. */
 orc_ctx *orc_ctx_create(uint32_t N, uint32_t L, uint32_t K, uint32_t dnum,
                         const uint64_t *q_primes, const uint64_t *p_primes);
+int orc_ctx_set_psi(orc_ctx *c, const uint64_t *psi);
 void orc_ctx_destroy(orc_ctx *c);
 uint32_t orc_ctx_alpha(const orc_ctx *c);
 /* Copies of the per-prime tables, index 0..L-1 = Q, L..L+K-1 = P. */

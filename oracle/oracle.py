@@ -48,6 +48,7 @@ def lib():
             "orc_galois_map": (None, [C.c_uint32, C.c_uint64, u32p]),
             "orc_ctx_create": (C.c_void_p, [C.c_uint32] * 4 + [u64p, u64p]),
             "orc_ctx_destroy": (None, [C.c_void_p]),
+            "orc_ctx_set_psi": (C.c_int, [C.c_void_p, u64p]),
             "orc_ctx_alpha": (C.c_uint32, [C.c_void_p]),
             "orc_ctx_tables": (None, [C.c_void_p, u64p, u64p, u64p, u64p, u32p, u64p, u64p]),
             "orc_modup_tables": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_uint32, u64p, u64p]),
@@ -132,7 +133,8 @@ def pack_map(map_u32):
 class OracleCtx:
     """Parameter block (N, L Q-primes, K P-primes, dnum) plus its tables."""
 
-    def __init__(self, N, q_primes, p_primes, dnum):
+    def __init__(self, N, q_primes, p_primes, dnum, psi=None):
+        """psi: optional primitive 2N-th roots (Q then P) instead of the minimal ones."""
         self.N, self.L, self.K, self.dnum = N, len(q_primes), len(p_primes), dnum
         self.q = [int(x) for x in q_primes]
         self.p = [int(x) for x in p_primes]
@@ -140,6 +142,9 @@ class OracleCtx:
         pa = np.array(self.p or [0], dtype=np.uint64)
         self.ptr = lib().orc_ctx_create(N, self.L, self.K, dnum, P64(qa), P64(pa))
         assert self.ptr, "orc_ctx_create failed"
+        if psi is not None:
+            ps = np.array([int(x) for x in psi], dtype=np.uint64)
+            assert lib().orc_ctx_set_psi(self.ptr, P64(ps)) == 0, "not primitive 2N-th roots"
         self.alpha = lib().orc_ctx_alpha(self.ptr)
         T = self.L + self.K
         self.psi = np.zeros(T, np.uint64)

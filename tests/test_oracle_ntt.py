@@ -25,6 +25,26 @@ def test_ntt_closed_form(orc, N, bits):
         assert [int(x) for x in ctx.intt(a, idx)] == intt_closed_form(a, int(ctx.psi[idx]), q)
 
 
+def test_foreign_roots_closed_form(orc):
+    """orc_ctx_set_psi: tables from another primitive root (psi^3, as a directory
+    dumped by another library may hold) still give the closed-form transform with
+    that root; a root that is not a primitive 2N-th one is refused."""
+    N = 64
+    q = orc.gen_primes(2 * N, 50, 2)
+    base = orc.OracleCtx(N, q[:1], q[1:], 1)
+    psi3 = [pow(int(w), 3, p) for w, p in zip(base.psi, q)]
+    ctx = orc.OracleCtx(N, q[:1], q[1:], 1, psi=psi3)
+    assert [int(x) for x in ctx.psi] == psi3
+    rng = np.random.default_rng(3)
+    for idx in range(2):
+        a = rng.integers(0, q[idx], N, dtype=np.uint64)
+        assert [int(x) for x in ctx.ntt(a, idx)] == ntt_closed_form(a, psi3[idx], q[idx])
+        assert [int(x) for x in ctx.intt(a, idx)] == intt_closed_form(a, psi3[idx], q[idx])
+    bad = [pow(psi3[0], 2, q[0]), psi3[1]]
+    with pytest.raises(AssertionError):
+        orc.OracleCtx(N, q[:1], q[1:], 1, psi=bad)
+
+
 def test_ntt_convolution_theorem(orc):
     N = 1024
     ctx = mkctx(orc, N, 1, 55)

@@ -21,7 +21,10 @@ STATUS = {
     0: "ok", -1: "unknown op code", -2: "offset or size outside the arena",
     -3: "offset not a multiple of 8 words", -4: "invalid argument", -5: "missing evaluation key",
     -6: "device allocation failed", -7: "HIP runtime error", -8: "invalid context",
+    -9: "work exceeds the static scratch area of the no-allocation entry point",
 }
+ERR_SCRATCH = -9
+NOALLOC_SCRATCH_WORDS = 8 << 20
 
 OP_NTT, OP_INTT, OP_POLY_ADD, OP_POLY_SUB, OP_POLY_MUL_NTT = 1, 2, 3, 4, 5
 OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
@@ -86,6 +89,7 @@ SIGNATURES = {
     "fhea_eval_sub_plain": (C.c_int, [vp] + [C.c_uint64] * 5 + [C.c_uint32]),
     "fhea_set_relin_key": (C.c_int, [vp, C.c_uint64, C.c_uint32]),
     "fhea_eval_mult_relin_rescale": (C.c_int, [vp] + [C.c_uint64] * 6 + [C.c_uint32]),
+    "fhea_eval_mult_relin_rescale_noalloc": (C.c_int, [vp] + [C.c_uint64] * 6 + [C.c_uint32]),
     "fhea_eval_mult_plain_rescale": (C.c_int, [vp] + [C.c_uint64] * 5 + [C.c_uint32]),
     "fhea_add_rotation_key": (C.c_int, [vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint32]),
     "fhea_eval_rotate": (C.c_int, [vp] + [C.c_uint64] * 4 + [C.c_int32, C.c_uint32]),

@@ -15,7 +15,41 @@
 
 #include "fhea_engine.hpp"
 
+#include <mutex>
+
 using namespace fhea;
+
+// Scratch of fhea_eval_mult_relin_rescale_noalloc: one static array per GPU the
+// library runs on (the loader places it when the code object is loaded, and it
+// stays until the process ends).  Host code never passes `scratch` itself to a
+// kernel: in host code the name is a stand-in, not the device address.  The
+// address is looked up once per device with hipGetSymbolAddress (with that
+// device current) at context creation and kept in g_scratch_dev.
+static __device__ __align__(256) u64 scratch[FHEA_NOALLOC_SCRATCH_WORDS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+u64 *g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+// Looks up (once per device) the device address of `scratch` on the current
+// device `dev`; FHEA_ERR_DEVICE if the runtime cannot resolve it.
+static int scratch_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return FHEA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return FHEA_OK;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FHEA_ERR_DEVICE;
+    g_scratch_dev[dev] = (u64 *)ptr;
+    return FHEA_OK;
+}
+
+static u64 *scratch_ptr(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[dev];
+}
 
 namespace fhea {
 
@@ -673,7 +707,7 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
     u32 chunk = c->max_chunk;
     if (fixed) {
         while (chunk > 1 && mult_relin_rescale_words(c, l, chunk) > fixed->words) --chunk;
-        if (mult_relin_rescale_words(c, l, chunk) > fixed->words) return FHEA_ERR_NO_MEM;
+        if (mult_relin_rescale_words(c, l, chunk) > fixed->words) return FHEA_ERR_SCRATCH;
     }
     for (size_t off = 0; off < items.size(); off += chunk) {
         u32 nb = (u32)std::min<size_t>(chunk, items.size() - off);
@@ -919,6 +953,7 @@ const char *fhea_status_string(int st) {
     case FHEA_ERR_NO_MEM: return "device allocation failed";
     case FHEA_ERR_DEVICE: return "HIP runtime error";
     case FHEA_ERR_STATE: return "invalid context";
+    case FHEA_ERR_SCRATCH: return "work exceeds the static scratch area of the no-allocation entry point";
     }
     return "unknown status";
 }
@@ -959,6 +994,7 @@ int fhea_ctx_create(const fhea_params *p, fhea_ctx **out) {
     if (!st && hipHostMalloc(&c->args_host, c->args_bytes, hipHostMallocDefault) != hipSuccess)
         st = FHEA_ERR_NO_MEM;
     if (!st) st = build_device_tables(c);
+    if (!st) st = scratch_init(c->device);  // device c->device is current (hipSetDevice above)
     if (st) {
         fhea_ctx_destroy(c);
         return st;
@@ -1354,6 +1390,24 @@ int fhea_eval_mult_relin_rescale(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t
     u64 *P = c->poly;
     std::vector<MultItem> it = {{P + a0, P + a1, P + b0, P + b1, P + d0, P + d1}};
     return mult_relin_rescale(c, l, c->key + c->relin_off, kl, it);
+}
+
+int fhea_eval_mult_relin_rescale_noalloc(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1,
+                                         uint64_t d0, uint64_t d1, uint32_t l) {
+    CTX_CHECK(c);
+    if (c->capturing) return FHEA_ERR_STATE;
+    int st = check_mult_args(c, a0, a1, b0, b1, d0, d1, l);
+    if (st) return st;
+    if (!c->has_relin) return FHEA_ERR_NO_KEY;
+    u32 kl = c->relin_limbs ? c->relin_limbs : l;
+    if (kl < l) return FHEA_ERR_ARG;
+    if (mult_relin_rescale_words(c, l, 1) > FHEA_NOALLOC_SCRATCH_WORDS) return FHEA_ERR_SCRATCH;
+    u64 *ptr = scratch_ptr(c->device);
+    if (!ptr) return FHEA_ERR_STATE;
+    const FixedWs ws{ptr, FHEA_NOALLOC_SCRATCH_WORDS};
+    u64 *P = c->poly;
+    std::vector<MultItem> it = {{P + a0, P + a1, P + b0, P + b1, P + d0, P + d1}};
+    return mult_relin_rescale_lane(c, l, c->key + c->relin_off, kl, it, &ws);
 }
 
 int fhea_eval_mult_plain_rescale(fhea_ctx *c, uint64_t a0, uint64_t a1, uint64_t pt, uint64_t d0,

@@ -52,6 +52,7 @@ extern "C" {
 #define FHEA_ERR_NO_MEM (-6)     /* device allocation failed / arena exhausted   */
 #define FHEA_ERR_DEVICE (-7)     /* HIP runtime error                            */
 #define FHEA_ERR_STATE (-8)      /* null context / not initialised               */
+#define FHEA_ERR_SCRATCH (-9)    /* work exceeds the no-allocation scratch area  */
 
 /* ---- op codes (hls_params.h:18-26) -------------------------------------- */
 #define FHEA_OP_NTT 1
@@ -154,6 +155,22 @@ int fhea_set_relin_key(fhea_ctx *ctx, uint64_t evk, uint32_t key_limbs);
 /* d = Rescale(Relin(a * b)), in_limbs -> in_limbs-1 (fhe_host_driver.cpp:272-306). */
 int fhea_eval_mult_relin_rescale(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t b0,
                                  uint64_t b1, uint64_t d0, uint64_t d1, uint32_t in_limbs);
+/* No reference counterpart.  The same words as fhea_eval_mult_relin_rescale, for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): no hipMalloc/hipFree or other
+ * allocating call is made; the pipeline's scratch is a static device array of
+ * FHEA_NOALLOC_SCRATCH_WORDS words (64 MiB) that every GPU the library runs on holds
+ * until the process ends.  Work needing more than that, i.e.
+ * N * (3*l + max(l + nparts*(l+K) + 2*(l+K) + 2*K + 2*l, 2*l)) words for one
+ * ciphertext pair at level l (nparts = ceil(l/alpha)), returns FHEA_ERR_SCRATCH
+ * (N=2^14 fits up to l=24 with K<=8; N=2^16, L=24, K=8 does not).  Every call on one
+ * GPU shares that GPU's array, which keeps the last call's contents: calls that
+ * overlap on one GPU (contexts with different streams, or several host threads)
+ * race and must be serialised by the caller.  Not allowed while capturing with
+ * fhea_graph_begin (FHEA_ERR_STATE). */
+#define FHEA_NOALLOC_SCRATCH_WORDS (8u << 20)
+int fhea_eval_mult_relin_rescale_noalloc(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t b0,
+                                         uint64_t b1, uint64_t d0, uint64_t d1, uint32_t in_limbs);
 /* d = Rescale(a * pt) (fhe_host_driver.cpp:308-329). */
 int fhea_eval_mult_plain_rescale(fhea_ctx *ctx, uint64_t a0, uint64_t a1, uint64_t pt,
                                  uint64_t d0, uint64_t d1, uint32_t in_limbs);

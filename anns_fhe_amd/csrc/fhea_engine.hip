@@ -473,7 +473,7 @@ static u64 ks_words(const fhea_ctx *c, u32 l) {
 static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2 * (l - 1)); }
 
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
-                      const std::vector<CombineArgs> &comb);
+                      const std::vector<CombineArgs> &comb, bool pb_mid = false);
 
 // The rescale that follows a key switch in eval_mult_relin_rescale: the switch's
 // add operand d_z = (d_base + z*dz, + ds) is also its output (l limbs, limb l-1
@@ -484,7 +484,22 @@ struct RescaleAfter {
     const std::vector<u64 *> *outs;
 };
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
-                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra);
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid);
+
+// The ModDown's INTT of the P part of nb accumulator pairs into pb: the whole
+// transform from acc, or (pb_mid) only its second pass over the intermediates the
+// fused inner product left in pb.
+static void moddown_intt(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb, bool pb_mid) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    if (pb_mid) {
+        NttFuse fz;
+        fz.only_pass = 2;
+        launch_ntt_f64(t, true, pb, pb, K * N, K * N, N, N, 2 * nb, K, t.ident + t.L, fz, c->stream);
+        return;
+    }
+    launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, c->stream);
+}
 
 // Key switch of nb inputs c_z = c_base + z*c_stride (l limbs, eval domain):
 // out0_z = add0_z + ks0, out1_z = add1_z + ks1 (key_switch.cpp:167-439 followed by
@@ -507,37 +522,52 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
     launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K,
                coef, l * N, compact, (l + K) * N, cz, nb, s);
-    if (t.use_f64 && !(t.f64_off & 1)) {
-        NttFuse fz;  // only the converted slots of each digit
-        fz.slots = lt.modup_slots;
-        fz.nslots = lt.n_modup_slots;
-        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
-    } else {
-        launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
-    }
-    // inner product with the evaluation key
     std::vector<KsIpArgs> ip(nb);
     for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
     auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
     if (!ip_d) return FHEA_ERR_DEVICE;
-    bool shared = true;
-    for (u32 z = 1; z < nb; ++z) shared &= keys[z] == keys[0];
-    launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
-                    c->h.dnum, shared, s);
+    // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
+    const bool fused = t.use_f64 && !(t.f64_off & (1 | 4 | 64)) && np <= 8 && ks_rows_fusable(t);
+    if (fused) {
+        // first forward pass of the converted digit limbs; the second runs inside the
+        // inner product, which for the P slots also runs the ModDown INTT's first pass
+        NttFuse fz;
+        fz.slots = lt.modup_slots;
+        fz.nslots = lt.n_modup_slots;
+        fz.only_pass = 1;
+        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+        const KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
+                            key_limbs, c->h.dnum, nb};
+        launch_ks_rows_f64(t, ka, s);
+    } else {
+        if (t.use_f64 && !(t.f64_off & 1)) {
+            NttFuse fz;  // only the converted slots of each digit
+            fz.slots = lt.modup_slots;
+            fz.nslots = lt.n_modup_slots;
+            launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+        } else {
+            launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
+        }
+        // inner product with the evaluation key
+        bool shared = true;
+        for (u32 z = 1; z < nb; ++z) shared &= keys[z] == keys[0];
+        launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
+                        c->h.dnum, shared, s);
+    }
     // the ModUp coefficients are dead now: their region holds the rescale's lift
-    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra);
-    return ks_moddown(c, l, nb, acc, pb, tt, comb);
+    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, fused);
+    return ks_moddown(c, l, nb, acc, pb, tt, comb, fused);
 }
 
 // ModDown of nb accumulator pairs acc_z = [2][l+K][N] (key_switch.cpp:385-427) and the
 // composite's add: out_z = add_z + ModDown(acc_z).  pb: 2*nb*K*N words, tt: 2*nb*l*N.
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
-                      const std::vector<CombineArgs> &comb) {
+                      const std::vector<CombineArgs> &comb, bool pb_mid) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
     hipStream_t s = c->stream;
-    launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
+    moddown_intt(c, l, nb, acc, pb, pb_mid);
     launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
@@ -602,17 +632,17 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
 // which is ((acc_i - NTT(conv_i))*P^-1 + d_i)*q_l^-1 + NTT(c*T_i) rearranged by the
 // linearity of the NTT mod q_i: the same words, one transform of l-1 limbs fewer.
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
-                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra) {
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
     hipStream_t s = c->stream;
     if (!t.use_f64 || (t.f64_off & 17) || l < 2) {
-        int st = ks_moddown(c, l, nb, acc, pb, tt, comb);
+        int st = ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
         if (st) return st;
         return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
     }
-    launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, s);
+    moddown_intt(c, l, nb, acc, pb, pb_mid);
     launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;

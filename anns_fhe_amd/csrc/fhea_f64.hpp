@@ -47,6 +47,18 @@ __device__ __forceinline__ double reduce_f64(double x, double q, double qinv) {
     return __builtin_fma(-__builtin_rint(x * qinv), q, x);
 }
 
+// A pointer read from device memory (an argument array) is a generic pointer, and
+// loads through it become flat loads with a 64-bit address per element; these
+// casts state that it addresses global memory (loads with a scalar base + offsets).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr_w(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // x in [0, 2^52) integer <-> double, exact (exponent-bias trick).
 __device__ __forceinline__ double u64_to_f64(uint64_t x) {
     return __longlong_as_double((long long)(x | 0x4330000000000000ull)) - 4503599627370496.0;

@@ -17,7 +17,8 @@ struct DevTables {
     // FP64 path, used when every prime is below 2^50; f64_off (FHEA_F64_OFF
     // bitmask) turns single components back to the integer kernels:
     // 1 transforms, 2 basis conversion, 4 inner product, 8 tensor,
-    // 16 fused transform ends, 32 rescale / ModDown combines
+    // 16 fused transform ends, 32 rescale / ModDown combines, 64 the inner
+    // product fused into the transforms around it
     bool use_f64 = false;
     u32 f64_off = 0;
     u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
@@ -31,6 +32,7 @@ struct DevTables {
 
 struct RescaleOut;
 struct CombineArgs;
+struct KsIpArgs;
 enum : int {
     FUSE_NONE = 0,
     FUSE_RESCALE_IN = 1,
@@ -56,7 +58,32 @@ struct NttFuse {
     // optional list of the limb slots to transform (instead of all `limbs` slots)
     const u32 *slots = nullptr;
     u32 nslots = 0;
+    // two-pass transforms (N > 4096): 0 both passes; 1 only the first (its output,
+    // FP64 intermediates, left at dst); 2 only the second (reading such intermediates
+    // at src)
+    int only_pass = 0;
 };
+
+// Key switching's inner product fused into the transforms around it (FP64 path,
+// two-pass N): after the ModUp's first forward pass (only_pass = 1 on the digits),
+// work item (tile, item z, slot i) runs the second forward pass of every converted
+// digit limb of slot i in registers, adds the digit's own limb where it has one,
+// multiplies by the key and sums over the digits (key_switch.cpp:309-381).  Q slots
+// (i < l) store acc0/acc1 as canonical words; P slots run the first inverse pass of
+// the ModDown INTT (key_switch.cpp:385-400) on the sums and store its FP64
+// intermediates in pb, so only the second inverse pass (only_pass = 2) remains.
+struct KsRowsArgs {
+    const u64 *compact;  // ModUp digits after the first forward pass: item z at + z*cz,
+    u64 cz;              // digit p, converted position pos at + (p*(l+K) + pos)*N
+    const KsIpArgs *args;  // per item: c (own limbs, eval domain), key base
+    u64 *acc;            // Q slots: acc0 at acc + z*az + i*N, acc1 at + (l+K)*N
+    u64 az;
+    u64 *pb;             // P slots: intermediates of poly 2z+s, limb j = i-l at pb + (2z+s)*K*N + j*N
+    u32 l, K, L, alpha, nparts, key_limbs, dnum, nb;
+};
+// true when launch_ks_rows_f64 has kernels for the context's current splits
+bool ks_rows_fusable(const DevTables &t);
+void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
 
 // Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
 // has limb slot s at src + z*src_stride + s*src_slot (src_slot 0 = N); slot s

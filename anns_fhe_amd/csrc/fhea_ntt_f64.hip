@@ -545,20 +545,279 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
     }
     const u32 k1 = k1s ? k1s : lg / 2, k2 = lg - k1;
     dim3 grid(1u << (lg - F_LOG_TILE), npolys * ylimbs);
+    const bool first = fz.only_pass != 2, second = fz.only_pass != 1;
+    // the second pass reads the first's output at dst, or at src when it runs alone
+    const void *s2 = first ? (const void *)dst : (const void *)src;
+    const u64 ss2 = first ? ds : ss, sl2 = first ? dl : sl;
     if (!inverse) {
         NttFuse pro = fz, epi = fz;
         pro.epi = FUSE_NONE;
         epi.pro = FUSE_NONE;
-        launch_pass<false, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
-                                 F_LOG_TILE, 1, 0, 0, pro);
-        launch_pass<false, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
-                                  F_LOG_TILE, 0, 1, 0, epi);
+        if (first)
+            launch_pass<false, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw,
+                                     t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, pro);
+        if (second)
+            launch_pass<false, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, s2, dst, ss2, ds, sl2, dl, limbs, pm, tw,
+                                      t.ninvf, t.mp, lg, F_LOG_TILE, 0, 1, 0, epi);
     } else {
-        launch_pass<true, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
-                                 F_LOG_TILE, 1, 0, 0, none);
-        launch_pass<true, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, dst, dst, ds, ds, dl, dl, limbs, pm, tw, t.ninvf, t.mp, lg,
-                                F_LOG_TILE, 0, 1, 1, none);
+        if (first)
+            launch_pass<true, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw,
+                                     t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, none);
+        if (second)
+            launch_pass<true, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, s2, dst, ss2, ds, sl2, dl, limbs, pm, tw,
+                                    t.ninvf, t.mp, lg, F_LOG_TILE, 0, 1, 1, none);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Key-switch inner product fused into the transforms around it (see KsRowsArgs).
+// A work item is one 4096-word tile (positions tI*4096 ..) of one QlP slot i of one
+// batch item z; numbering: tile fastest (so that on a persistent grid of a
+// multiple of 8 workgroups every XCD keeps the same tiles, and the key tile of
+// (i, tI) is reused from its L2 by the batch items that follow), then item, then slot.
+// Arithmetic as k_ks_inner_f64: canonical digit words times canonical key words
+// by mulmod_vv_f64 (<= 0.875q each), at most KS_FUSE_MAX_PARTS terms, centred once.
+// ---------------------------------------------------------------------------
+static constexpr u32 KS_FUSE_MAX_PARTS = 8;
+#ifndef FHEA_KS_ROWS_WAVES
+#define FHEA_KS_ROWS_WAVES 0
+#endif
+#if FHEA_KS_ROWS_WAVES > 0
+#define FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(FHEA_KS_ROWS_WAVES)))
+#else
+#define FHEA_KS_ROWS_ATTR
+#endif
+
+// Rounds 1.. of a tile transform after round 0 (x: round-0 layout in, last-round
+// layout out), exchanging through lds.
+template <bool INV, int LOGR, bool COLS, int RD>
+__device__ __forceinline__ void next_round(double (&x)[16], double *lds, const TileGeom &g, u32 tid) {
+    using Rp = Round<INV, LOGR, RD - 1>;
+    using Rn = Round<INV, LOGR, RD>;
+    const u32 Pp = g.logC + Rp::P, Pn = g.logC + Rn::P;
+    const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
+    const u32 ebn = (tid & ((1u << Pn) - 1)) | ((tid >> Pn) << (Pn + 4));
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(ebn | ((u32)u << Pn))];
+    round_arith<INV, LOGR, RD, COLS>(x, g, g.gidx(ebn, LOGR));
+}
+
+template <bool INV, int LOGR, bool COLS>
+__device__ __forceinline__ void later_rounds(double (&x)[16], double *lds, const TileGeom &g, u32 tid) {
+    constexpr int NR = (LOGR + 3) / 4;
+    if constexpr (NR > 1) next_round<INV, LOGR, COLS, 1>(x, lds, g, tid);
+    if constexpr (NR > 2) next_round<INV, LOGR, COLS, 2>(x, lds, g, tid);
+    if constexpr (NR > 3) next_round<INV, LOGR, COLS, 3>(x, lds, g, tid);
+}
+
+// x (last-round layout of a row tile) into lds in position order: lds[lds_pad(e)] =
+// element e of the tile (position tI*4096 + e).
+template <bool INV, int LOGR>
+__device__ __forceinline__ void rows_to_lds(const double (&x)[16], double *lds, u32 tid) {
+    using RL = Round<INV, LOGR, (LOGR + 3) / 4 - 1>;
+    constexpr u32 P = RL::P;
+    const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
+    __syncthreads();
+}
+
+template <int LOGRF, int LOGRI, bool PSLOT>
+__global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
+                                                     const double *__restrict__ itwf,
+                                                     const ModParams *__restrict__ mp, u32 logN, u32 slot0,
+                                                     u32 n_work) {
+    __shared__ double lds[F_TPB * 16 + F_TPB];
+    const u32 tid = threadIdx.x;
+    const u32 T = 1u << (logN - F_LOG_TILE);
+    const u32 N = 1u << logN;
+    const u32 lk = a.l + a.K;
+    const u64 kstride = (u64)(a.key_limbs + a.K) << logN;
+    using RF0 = Round<false, LOGRF, 0>;
+    constexpr bool DIRECT_F = RF0::P >= 3;
+    for (u32 w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const u32 tI = w % T;
+        const u32 zi = w / T;
+        const u32 z = zi % a.nb;
+        const u32 i = slot0 + zi / a.nb;
+        const u32 pidx = (i < a.l) ? i : (a.L + i - a.l);
+        const u32 kl = (i < a.l) ? i : (a.key_limbs + i - a.l);
+        const double q = mp[pidx].qd, qinv = mp[pidx].qinv;
+        const u32 base = tI << F_LOG_TILE;
+        const KsIpArgs A = a.args[z];
+        double s0[16], s1[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s0[k] = s1[k] = 0.0;
+        for (u32 p = 0; p < a.nparts; ++p) {
+            const u32 start = p * a.alpha;
+            const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
+            const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);
+            const auto *kb = ka + (size_t)a.dnum * kstride;
+            if (i >= start && i < start + sz) {
+                // the digit's own limb: the input itself (eval domain, canonical)
+                const auto *c = gptr(A.c + ((size_t)i << logN) + base + tid);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    // groups of four elements: the loads of one group are in flight together
+                    // (fewer live registers than all 48 at once)
+                    if ((k & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                    const double d = u64_to_f64(c[(size_t)k * F_TPB]);
+                    s0[k] += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)k * F_TPB]), q, qinv);
+                    s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
+                }
+                continue;
+            }
+            // second forward pass of the converted limb, in registers
+            const u32 pos = (i < start) ? i : (i - sz);
+            const auto *src = gptr((const double *)(a.compact + (size_t)z * a.cz + ((size_t)(p * lk + pos) << logN)));
+            TileGeom g;
+            g.logN = logN;
+            g.logT = F_LOG_TILE;
+            g.logC = 0;
+            g.logS = 0;
+            g.c0 = 0;
+            g.b0 = tI << (F_LOG_TILE - LOGRF);
+            g.q = q;
+            g.qinv = qinv;
+            g.tw = twf + ((size_t)pidx << (logN + 1));
+            {
+                double x[16];
+                const u32 eb0 = (tid & ((1u << RF0::P) - 1)) | ((tid >> RF0::P) << (RF0::P + 4));
+                if (DIRECT_F) {
+                    const u32 jb = g.gidx(eb0, LOGRF);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = src[jb + ((u32)u << RF0::P)];
+                } else {
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) lds[lds_pad(tid + (u32)k * F_TPB)] = src[base + tid + (u32)k * F_TPB];
+                    __syncthreads();
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << RF0::P))];
+                }
+                round_arith<false, LOGRF, 0, false>(x, g, g.gidx(eb0, LOGRF));
+                later_rounds<false, LOGRF, false>(x, lds, g, tid);
+                rows_to_lds<false, LOGRF>(x, lds, tid);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if ((k & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                const double v = lds[lds_pad(tid + (u32)k * F_TPB)];
+                const double d = v < 0.0 ? v + q : v;  // |v| <= q/2 + 2: the canonical word
+                s0[k] += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)k * F_TPB]), q, qinv);
+                s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
+            }
+        }
+        if (!PSLOT) {
+            auto *o0 = gptr_w(a.acc + (size_t)z * a.az + ((size_t)i << logN) + base + tid);
+            auto *o1 = o0 + ((size_t)lk << logN);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                o0[(size_t)k * F_TPB] = f64_to_canon(reduce_f64(s0[k], q, qinv), q);
+                o1[(size_t)k * F_TPB] = f64_to_canon(reduce_f64(s1[k], q, qinv), q);
+            }
+        } else {
+            // first pass of the ModDown INTT on each sum (input: the canonical words)
+            using RI0 = Round<true, LOGRI, 0>;
+            using RIL = Round<true, LOGRI, (LOGRI + 3) / 4 - 1>;
+            TileGeom g;
+            g.logN = logN;
+            g.logT = F_LOG_TILE;
+            g.logC = 0;
+            g.logS = 0;
+            g.c0 = 0;
+            g.b0 = tI << (F_LOG_TILE - LOGRI);
+            g.q = q;
+            g.qinv = qinv;
+            g.tw = itwf + ((size_t)pidx << (logN + 1));
+            const u32 j = i - a.l;
+            for (u32 sp = 0; sp < 2; ++sp) {
+                auto *dst = gptr_w((double *)(a.pb + ((size_t)(2 * z + sp) * a.K + j) * N));
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const double v = reduce_f64(sp ? s1[k] : s0[k], q, qinv);
+                    lds[lds_pad(tid + (u32)k * F_TPB)] = v < 0.0 ? v + q : v;
+                }
+                __syncthreads();
+                double x[16];
+                const u32 eb0 = (tid & ((1u << RI0::P) - 1)) | ((tid >> RI0::P) << (RI0::P + 4));
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << RI0::P))];
+                round_arith<true, LOGRI, 0, false>(x, g, g.gidx(eb0, LOGRI));
+                later_rounds<true, LOGRI, false>(x, lds, g, tid);
+                if (RIL::P >= 3) {
+                    const u32 eb = (tid & ((1u << RIL::P) - 1)) | ((tid >> RIL::P) << (RIL::P + 4));
+                    const u32 jb = g.gidx(eb, LOGRI);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) dst[jb + ((u32)u << RIL::P)] = reduce_f64(x[u], q, qinv);
+                } else {
+                    rows_to_lds<true, LOGRI>(x, lds, tid);
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                        dst[base + tid + (u32)k * F_TPB] = reduce_f64(lds[lds_pad(tid + (u32)k * F_TPB)], q, qinv);
+                }
+            }
+        }
+        __syncthreads();  // this item's LDS reads are done before the next item writes
+    }
+}
+
+template <int LOGRF, int LOGRI, bool PSLOT>
+static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ks_rows_f64<LOGRF, LOGRI, PSLOT>, F_TPB, 0) !=
+                hipSuccess ||
+            n <= 0)
+            n = 2;
+        per_cu = n < 4 ? n : 4;
+    }
+    const u32 n_work = nslots * a.nb * (1u << (t.logN - F_LOG_TILE));
+    const u32 cap = t.max_wg / 4 * (u32)per_cu;
+    const u32 nwg = n_work < cap ? n_work : cap;
+    hipLaunchKernelGGL((k_ks_rows_f64<LOGRF, LOGRI, PSLOT>), dim3(nwg), dim3(F_TPB), 0, s, a, t.twf, t.itwf, t.mp,
+                       t.logN, slot0, n_work);
+}
+
+// second-pass lengths (stages along rows) of the current forward / inverse splits
+static void rows_logs(const DevTables &t, u32 &lf, u32 &li) {
+    const u32 lg = t.logN;
+    lf = lg - (t.k1_fwd ? t.k1_fwd : lg / 2);
+    li = lg - (t.k1_inv ? t.k1_inv : lg / 2);
+}
+
+bool ks_rows_fusable(const DevTables &t) {
+    if (!t.use_f64 || (t.f64_off & 1) || t.logN <= F_LOG_TILE) return false;
+    if ((t.logN == 13 || t.logN == 14) && (t.k1_fwd == t.logN || t.k1_inv == t.logN)) return false;
+    u32 lf, li;
+    rows_logs(t, lf, li);
+    return lf >= 6 && lf <= 9 && li >= 6 && li <= 9;
+}
+
+void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s) {
+    u32 lf, li;
+    rows_logs(t, lf, li);
+    // the Q-slot kernel does not use LOGRI: one instance per forward split
+#define FHEA_KS_ROWS(LF, LI)                                                  \
+    if (lf == LF && li == LI) {                                               \
+        launch_ks_rows_inst<LF, 8, false>(t, a, 0, a.l, s);                   \
+        launch_ks_rows_inst<LF, LI, true>(t, a, a.l, a.K, s);                 \
+        return;                                                               \
+    }
+#define FHEA_KS_ROWS_F(LF) FHEA_KS_ROWS(LF, 6) FHEA_KS_ROWS(LF, 7) FHEA_KS_ROWS(LF, 8) FHEA_KS_ROWS(LF, 9)
+    FHEA_KS_ROWS_F(6)
+    FHEA_KS_ROWS_F(7)
+    FHEA_KS_ROWS_F(8)
+    FHEA_KS_ROWS_F(9)
+#undef FHEA_KS_ROWS_F
+#undef FHEA_KS_ROWS
 }
 
 }  // namespace fhea

@@ -11,7 +11,13 @@ Prints ONE JSON line.  Besides the contract fields it carries
   roofline      the dominant kernel (batched negacyclic NTT) measured live with
                 HIP events on the engine's stream, against MI355X HBM peak;
   cpu_baseline  the CPU oracle (single-thread restatement of the reference
-                C-model) timed on a bounded sample of the same op.
+                C-model) timed on a bounded sample of the same op;
+  rotate, ivfpq, serving
+                the metric's other halves: 64 rotations with distinct keys,
+                encrypted IVF-PQ queries/s with p50/p99 and recall@10 (1M x 128,
+                B=64, n_probe=4), and the serving loop's queries/s at p50/p99
+                against its unbatched and op-by-op plain paths (--no-extras
+                leaves them out).
 """
 import argparse
 import json
@@ -56,6 +62,12 @@ def parse():
     ap.add_argument("--stage3-chunks", type=int, default=8,
                     help="ivfpq: stage-3 parts whose result reads overlap the next part's evaluation")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT split / chunk schedule")
+    ap.add_argument("--chain", default="q50", choices=["q50", "ref"],
+                    help="hmult primes: q50 = 24 + 8 primes of 50 bits (FP64 kernels); ref = the reference's "
+                         "chain shape, a 60-bit q0, 23 x 45-bit scaling primes and 8 x 60-bit special primes "
+                         "(integer kernels)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="hmult: leave out the rotate / ivfpq / serving objects of the headline line")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="plain_path runs (hmult: single-ciphertext op-by-op HMults; serve: reference "
                          "op-by-op queries); 0 = skip")
@@ -79,6 +91,47 @@ def traffic_from_profile():
             return json.load(f)["traffic_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _primes_below(bits, count, skip=()):
+    """The largest `count` primes q = 1 mod 2N below 2^bits (descending)."""
+    out, q = [], ((1 << bits) - 1) // (2 * N) * (2 * N) + 1
+    while len(out) < count:
+        if q not in skip and _is_prime(q):
+            out.append(q)
+        q -= 2 * N
+    return out
+
+
+def ref_chain():
+    """The reference's chain shape (generate_interactive_test_vectors.cpp:161-166:
+    SetFirstModSize(60), SetScalingModSize(45)) at L=24, K=8: q0 of 60 bits, 23
+    scaling primes of 45 bits, 8 special primes of 60 bits."""
+    big = _primes_below(60, K + 1)
+    return [big[0]] + _primes_below(45, L - 1), big[1:]
 
 
 def cpu_baseline(ctx, key, sample):
@@ -440,6 +493,25 @@ def bench_serve(args):
         sweep.append({"rate": rate, "queries_per_s": round(stt["queries_per_s"], 2),
                       "p50_ms": round(stt["p50_ms"], 1), "p99_ms": round(stt["p99_ms"], 1), "by_params": per})
         log(f"serve rate {rate}: {sweep[-1]}")
+    # the same protocol without cross-request batching: one slot per parameter set and
+    # batches of one request, offered at the highest swept rate (saturated)
+    unbatched = None
+    if args.plain_steps > 0:
+        rate = max(float(r) for r in args.rates.split(","))
+        arr = SV.poisson_arrivals(rate, args.requests, seed=int(rate))
+        urng = np.random.default_rng(7)
+        reqs = [SV.Request(rid=i, pset="n14" if urng.random() < 0.5 else "n16",
+                           query=x[urng.integers(n)] + urng.normal(0, 0.05, 128).astype(np.float32),
+                           n_probe=int(urng.integers(1, 9)), arrival=float(arr[i])) for i in range(args.requests)]
+        one = small.static_bytes + sum(p.static_bytes for p in larges) + small.slot_bytes + \
+            sum(p.slot_bytes for p in larges)
+        loop = SV.ServingLoop([small] + larges, one, max_batch=1, client_workers=16)
+        stt = SV.latency_stats(loop.run(reqs))
+        unbatched = {"queries_per_s": round(stt["queries_per_s"], 2), "p50_ms": round(stt["p50_ms"], 1),
+                     "p99_ms": round(stt["p99_ms"], 1), "rate": rate,
+                     "what": "same protocol and parameter sets, one query slot per set and batches of one "
+                             "request (no cross-request batching), saturated"}
+        log(f"serve unbatched: {unbatched}")
     small.close()
     for p in larges:
         p.close()
@@ -484,12 +556,55 @@ def bench_serve(args):
         "config": {"workload": "Poisson stream of interactive encrypted IVF-PQ queries, 50/50 over two parameter "
                                "sets, n_probe uniform 1..8, HBM-budget admission, continuous batching",
                    "db": n, "requests_per_rate": args.requests, "budget_bytes": budget,
-                   "large_replicas": nrep, "ks_chunk": args.serve_chunk or None},
+                   "large_replicas": nrep, "ks_chunk": args.serve_chunk or None,
+                   "eval_limbs": {"n14": small.server.lt, "n16": large.server.lt},
+                   "query_limbs": {"n14": 8, "n16": 24}},
         "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "at_rate": best["rate"]},
         "sweep": sweep,
+        "unbatched_same_protocol": unbatched,
+        "speedup_vs_unbatched": round(best["queries_per_s"] / unbatched["queries_per_s"], 1) if unbatched else None,
         "plain_path": plain,
         "speedup_vs_plain": round(best["queries_per_s"] / plain["queries_per_s"], 1) if plain else None,
     }
+
+
+def extras(args):
+    """The rest of the metric beside the headline: C3 rotations, C4 encrypted IVF-PQ
+    (queries/s, p50/p99, recall@10) and C5 serving (queries/s at one sub-saturation
+    and one saturating rate, against the unbatched and the op-by-op plain paths).
+    Each runs in its own contexts after the headline's are freed; a failure is
+    recorded instead of the object."""
+    import argparse as _ap
+    out = {}
+
+    def sub(**kw):
+        ns = _ap.Namespace(**vars(args))
+        for k, v in kw.items():
+            setattr(ns, k, v)
+        return ns
+
+    def run(name, fn, ns, pick):
+        t0 = time.perf_counter()
+        try:
+            r = fn(ns)
+            out[name] = pick(r)
+            out[name]["bench_s"] = round(time.perf_counter() - t0, 1)
+        except Exception as e:  # the headline line is printed whatever happens here
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"{name}: {json.dumps(out[name])[:300]}")
+
+    run("rotate", bench_ks64, sub(steps=5, warmup=1, cpu_sample=1),
+        lambda r: {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
+                   "eager_ms_per_step": r["config"]["eager_ms_per_step"], "roofline": r["roofline"],
+                   "cpu_baseline": r["cpu_baseline"], "config": {k: r["config"][k] for k in
+                                                                  ("N", "L", "K", "dnum", "cts", "key_bytes")}})
+    run("ivfpq", bench_ivfpq, sub(steps=3, warmup=1),
+        lambda r: {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
+                   "latency_ms": r["latency_ms"], "recall_at_10": r["recall_at_10"], "config": r["config"]})
+    run("serving", bench_serve, sub(rates="80,1280", requests=160, plain_steps=3),
+        lambda r: {k: r[k] for k in ("metric", "value", "unit", "latency_ms", "sweep", "unbatched_same_protocol",
+                                     "speedup_vs_unbatched", "plain_path", "speedup_vs_plain", "config")})
+    return out
 
 
 def main():
@@ -515,8 +630,11 @@ def main():
     key_words = 2 * DNUM * (L + K) * N
     log(f"context N={N} L={L} K={K} dnum={DNUM} batch={B}")
     plain_words = 8 * W  # scratch of the op-by-op path
-    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=plain_base + plain_words + 8,
-                     key_words=key_words + 8)
+    q_primes = p_primes = None
+    if args.chain == "ref":
+        q_primes, p_primes = ref_chain()
+    ctx = FheContext(N, L, K, DNUM, q_bits=50, q_primes=q_primes, p_primes=p_primes,
+                     poly_words=plain_base + plain_words + 8, key_words=key_words + 8)
     drv = FHEHostDriver(ctx)
     s = ctx.keygen_secret(42)
     ctx.keygen_relin_key(s, L, 43, 0)
@@ -529,9 +647,19 @@ def main():
     schedule = None
     if not args.no_tune:
         from anns_fhe_amd import autotune
+        from anns_fhe_amd.driver import PinnedArray
         t0 = time.perf_counter()
-        autotune.tune(ctx, lambda: ctx.submit(works), check=lambda: ctx.read_poly(out_base, 2 * W1),
-                      reps=3, log=log)
+        pin = PinnedArray(2 * B * W1)
+        wts = np.random.default_rng(99).integers(1, 1 << 63, 2 * W1, dtype=np.uint64) | np.uint64(1)
+
+        def digest():
+            # every output word of every item (both lanes, every chunk): per item a sum of
+            # the words and a sum weighted by odd pseudo-random factors (mod 2^64)
+            out = ctx.read_poly(out_base, 2 * B * W1, out=pin.array).reshape(B, 2 * W1)
+            return np.array([[o.sum(dtype=np.uint64), (o * wts).sum(dtype=np.uint64)] for o in out])
+
+        autotune.tune(ctx, lambda: ctx.submit(works), check=digest, reps=3, log=log)
+        pin.close()
         schedule = autotune.current(ctx)
         log(f"autotune {schedule} in {time.perf_counter() - t0:.1f} s")
     log("warmup")
@@ -584,7 +712,9 @@ def main():
         "data": "synthetic (uniform residues, seeded keys)",
         "config": {"workload": "batched eval_mult_relin_rescale, N=2^16, L=24->23, K=8, dnum=3, "
                                "shared relin key, inputs resident in HBM",
-                   "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B, "q_bits": 50,
+                   "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B,
+                   "q_bits": 50 if args.chain == "q50" else "60 (q0), 45 (q1..q23), 60 (P)",
+                   "kernels": "FP64" if args.chain == "q50" else "64-bit integer",
                    "schedule": schedule},
         "roofline": {"bound": "hbm",
                      "kernel": f"batched forward NTT, {npr} polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
@@ -619,6 +749,9 @@ def main():
         log("cpu baseline")
         key = ctx.read_key(0, key_words)
         result["cpu_baseline"] = cpu_baseline(ctx, key, args.cpu_sample)
+    if not args.no_extras:
+        ctx.close()
+        result.update(extras(args))
     print(json.dumps(result), flush=True)
 
 

@@ -31,6 +31,7 @@ OP_AUTOMORPHISM, OP_KEY_SWITCH, OP_RESCALE, OP_LOAD_CONTEXT = 6, 7, 8, 99
 WORK_MULT_RELIN_RESCALE, WORK_AUTOMORPHISM_KS, WORK_NTT, WORK_INTT = 1, 2, 3, 4
 WORK_MULT_RELIN, WORK_ADD, WORK_ROTATE_HOISTED = 5, 6, 7
 NONE = (1 << 64) - 1
+SEED_RANDOM = (1 << 64) - 1  # FHEA_SEED_RANDOM: the secure (ChaCha20) source
 TUNE_NTT_SPLIT_FWD, TUNE_NTT_SPLIT_INV, TUNE_NTT_WG_PER_CU, TUNE_CHUNK, TUNE_STREAMS = 1, 2, 3, 4, 5
 
 
@@ -78,6 +79,7 @@ SIGNATURES = {
     "fhea_copy_poly": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
     "fhea_ctx_set_tuning": (C.c_int, [vp, C.c_int, C.c_uint32]),
     "fhea_ctx_get_tuning": (C.c_int, [vp, C.c_int, u32p]),
+    "fhea_mem_info": (C.c_int, [C.c_uint32, u64p, u64p]),
     "fhea_host_alloc": (vp, [C.c_uint64]),
     "fhea_host_free": (C.c_int, [vp]),
     "fhea_poly_arena": (vp, [vp]),
@@ -132,6 +134,7 @@ SIGNATURES = {
     "fhea_encode_batch": (C.c_int, [vp, f64p, f64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, u64p,
                                     C.c_uint32]),
     "fhea_decode": (C.c_int, [vp, u64p, C.c_uint32, C.c_double, C.c_uint32, f64p, f64p]),
+    "fhea_chacha20_block": (C.c_int, [u32p, C.c_uint32, u32p, u32p]),
     "fhea_keygen_public": (C.c_int, [vp, i8p, C.c_uint64, u64p]),
     "fhea_encrypt": (C.c_int, [vp, u64p, u64p, C.c_uint32, C.c_uint64, u64p]),
     "fhea_decrypt": (C.c_int, [vp, i8p, u64p, C.c_uint32, u64p]),
@@ -161,6 +164,14 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def mem_info(device=0):
+    """(free, total) bytes of device memory (fhea_mem_info)."""
+    lib = load()
+    f, t = C.c_uint64(), C.c_uint64()
+    check(lib.fhea_mem_info(device, C.byref(f), C.byref(t)), "fhea_mem_info")
+    return f.value, t.value
 
 
 def check(status, what):

@@ -23,6 +23,15 @@ def _p64(a):
     return a.ctypes.data_as(_lib.u64p)
 
 
+def _seed(seed):
+    """None -> FHEA_SEED_RANDOM (secure generator); an integer -> reproducible stream."""
+    if seed is None:
+        return _lib.SEED_RANDOM
+    if not 0 <= int(seed) < _lib.SEED_RANDOM:
+        raise ValueError("seed must be None or an integer in [0, 2^64 - 1)")
+    return int(seed)
+
+
 def _pf(a):
     return a.ctypes.data_as(_lib.f64p)
 
@@ -73,29 +82,31 @@ class CkksClient:
         check(self.lib.fhea_client_table(self.h, which, _p64(out), n), "fhea_client_table")
         return out if which in (0, 1) else out.reshape(T, self.N)
 
-    # ---- keys
-    def keygen_secret(self, seed):
+    # ---- keys.  seed None (default): the secure generator (ChaCha20 keyed by the OS);
+    # an integer: a reproducible std::mt19937_64 stream for tests, not secure.
+    def keygen_secret(self, seed=None):
         s = np.zeros(self.N, np.int8)
-        check(self.lib.fhea_client_keygen_secret(self.h, seed, s.ctypes.data_as(_lib.i8p)), "keygen_secret")
+        check(self.lib.fhea_client_keygen_secret(self.h, _seed(seed), s.ctypes.data_as(_lib.i8p)), "keygen_secret")
         return s
 
-    def keygen_public(self, s, seed):
+    def keygen_public(self, s, seed=None):
         pk = np.zeros(2 * self.L * self.N, np.uint64)
-        check(self.lib.fhea_keygen_public(self.h, s.ctypes.data_as(_lib.i8p), seed, _p64(pk)), "keygen_public")
+        check(self.lib.fhea_keygen_public(self.h, s.ctypes.data_as(_lib.i8p), _seed(seed), _p64(pk)),
+              "keygen_public")
         return pk
 
     def key_size_words(self, key_limbs):
         return 2 * self.dnum * (key_limbs + self.K) * self.N
 
-    def relin_key(self, s, key_limbs, seed):
+    def relin_key(self, s, key_limbs, seed=None):
         k = np.zeros(self.key_size_words(key_limbs), np.uint64)
-        check(self.lib.fhea_client_relin_key(self.h, s.ctypes.data_as(_lib.i8p), key_limbs, seed, _p64(k)),
+        check(self.lib.fhea_client_relin_key(self.h, s.ctypes.data_as(_lib.i8p), key_limbs, _seed(seed), _p64(k)),
               "relin_key")
         return k
 
-    def rotation_key(self, s, galois, key_limbs, seed):
+    def rotation_key(self, s, galois, key_limbs, seed=None):
         k = np.zeros(self.key_size_words(key_limbs), np.uint64)
-        check(self.lib.fhea_client_rotation_key(self.h, s.ctypes.data_as(_lib.i8p), galois, key_limbs, seed,
+        check(self.lib.fhea_client_rotation_key(self.h, s.ctypes.data_as(_lib.i8p), galois, key_limbs, _seed(seed),
                                                 _p64(k)), "rotation_key")
         return k
 
@@ -132,10 +143,10 @@ class CkksClient:
         return re + 1j * im if complex_out else re
 
     # ---- encryption
-    def encrypt(self, pk, pt, limbs, seed):
+    def encrypt(self, pk, pt, limbs, seed=None):
         pt = np.ascontiguousarray(pt, dtype=np.uint64)
         ct = np.zeros(2 * limbs * self.N, np.uint64)
-        check(self.lib.fhea_encrypt(self.h, _p64(pk), _p64(pt), limbs, seed, _p64(ct)), "fhea_encrypt")
+        check(self.lib.fhea_encrypt(self.h, _p64(pk), _p64(pt), limbs, _seed(seed), _p64(ct)), "fhea_encrypt")
         return ct
 
     def decrypt(self, s, ct, limbs):
@@ -174,7 +185,7 @@ class CkksClient:
               "fhea_decrypt_decode_batch")
         return re + 1j * im if complex_out else re
 
-    def encrypt_values(self, pk, values, limbs, scale, seed):
+    def encrypt_values(self, pk, values, limbs, scale, seed=None):
         return self.encrypt(pk, self.encode(values, limbs, scale), limbs, seed)
 
     def decrypt_values(self, s, ct, limbs, scale, n, complex_out=False):

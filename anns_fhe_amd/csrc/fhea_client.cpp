@@ -36,6 +36,11 @@
 #include <thread>
 #include <vector>
 
+#include <cerrno>
+#include <fcntl.h>
+#include <sys/random.h>
+#include <unistd.h>
+
 #include "fhea_engine.hpp"
 
 using namespace fhea;
@@ -368,6 +373,114 @@ std::vector<int64_t> s_to_i64(const int8_t *s, u32 N) {
 // the distribution of |x| (64-bit thresholds t[k] = 2^64 P(|x| <= k), P(|x| <= k) =
 // 1 - erfc((k + 1/2) / (3.2 sqrt 2))) and a sign bit; |x| > 40 has probability
 // below 2^-64 and is not produced.
+// ---------------------------------------------------------------------------
+// Randomness.  A seed of FHEA_SEED_RANDOM (the default of the Python client) draws
+// from a ChaCha20 keystream (RFC 8439: 20 rounds, 256-bit key from getrandom(2),
+// zero nonce, a fresh key per call), the cryptographically secure source that key
+// generation and encryption need.  Any other seed selects std::mt19937_64 seeded
+// with it: reproducible, for tests and for checking against the oracle, and NOT
+// secure (its state follows from its outputs, which the public key exposes).
+// ---------------------------------------------------------------------------
+inline u32 rotl32(u32 x, int r) { return (x << r) | (x >> (32 - r)); }
+
+inline void chacha_quarter(u32 &a, u32 &b, u32 &c, u32 &d) {
+    a += b; d ^= a; d = rotl32(d, 16);
+    c += d; b ^= c; b = rotl32(b, 12);
+    a += b; d ^= a; d = rotl32(d, 8);
+    c += d; b ^= c; b = rotl32(b, 7);
+}
+
+// RFC 8439 section 2.3: the 64-byte block for (key, counter, nonce).
+void chacha20_block(const u32 key[8], u32 counter, const u32 nonce[3], u32 out[16]) {
+    u32 st[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2], key[3],
+                  key[4],      key[5],      key[6],      key[7],      counter, nonce[0], nonce[1], nonce[2]};
+    u32 x[16];
+    std::memcpy(x, st, sizeof x);
+    for (int i = 0; i < 10; ++i) {
+        chacha_quarter(x[0], x[4], x[8], x[12]);
+        chacha_quarter(x[1], x[5], x[9], x[13]);
+        chacha_quarter(x[2], x[6], x[10], x[14]);
+        chacha_quarter(x[3], x[7], x[11], x[15]);
+        chacha_quarter(x[0], x[5], x[10], x[15]);
+        chacha_quarter(x[1], x[6], x[11], x[12]);
+        chacha_quarter(x[2], x[7], x[8], x[13]);
+        chacha_quarter(x[3], x[4], x[9], x[14]);
+    }
+    for (int i = 0; i < 16; ++i) out[i] = x[i] + st[i];
+}
+
+bool os_random(void *buf, size_t n) {
+    uint8_t *p = (uint8_t *)buf;
+    while (n) {
+        const ssize_t r = getrandom(p, n, 0);
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            // no getrandom: the kernel's pool through /dev/urandom
+            const int fd = open("/dev/urandom", O_RDONLY | O_CLOEXEC);
+            if (fd < 0) return false;
+            while (n) {
+                const ssize_t k = read(fd, p, n);
+                if (k <= 0) {
+                    close(fd);
+                    return false;
+                }
+                p += k;
+                n -= (size_t)k;
+            }
+            close(fd);
+            return true;
+        }
+        p += r;
+        n -= (size_t)r;
+    }
+    return true;
+}
+
+// UniformRandomBitGenerator over the ChaCha20 keystream, 64 bits per draw.
+class ChaChaRng {
+public:
+    using result_type = uint64_t;
+    static constexpr result_type min() { return 0; }
+    static constexpr result_type max() { return ~0ull; }
+    // the key is drawn in the body, after every member has been initialised
+    ChaChaRng() { ok_ = os_random(key_, sizeof key_); }
+    bool ok() const { return ok_; }
+    result_type operator()() {
+        if (pos_ == 16) {
+            const u32 nonce[3] = {0, 0, 0};
+            chacha20_block(key_, counter_++, nonce, buf_);
+            pos_ = 0;
+        }
+        const u64 v = (u64)buf_[pos_] | ((u64)buf_[pos_ + 1] << 32);
+        pos_ += 2;
+        return v;
+    }
+
+private:
+    bool ok_ = false;
+    u32 key_[8] = {};
+    u32 counter_ = 0;
+    u32 buf_[16];
+    int pos_ = 16;
+};
+
+// The generator a seed selects (see above); both satisfy UniformRandomBitGenerator.
+class Rng {
+public:
+    using result_type = uint64_t;
+    static constexpr result_type min() { return 0; }
+    static constexpr result_type max() { return ~0ull; }
+    explicit Rng(uint64_t seed) : secure_(seed == FHEA_SEED_RANDOM), mt_(seed) {}
+    // false when the secure source could not be keyed (no OS randomness)
+    bool ok() const { return !secure_ || cc_.ok(); }
+    result_type operator()() { return secure_ ? cc_() : mt_(); }
+
+private:
+    bool secure_;
+    std::mt19937_64 mt_;
+    ChaChaRng cc_;
+};
+
 struct GaussTable {
     static constexpr u32 K = 40;
     u64 t[K + 1];
@@ -380,7 +493,7 @@ struct GaussTable {
     }
 };
 
-void gaussian(std::mt19937_64 &rng, std::vector<int64_t> &e) {
+void gaussian(Rng &rng, std::vector<int64_t> &e) {
     static const GaussTable g;
     u64 signs = 0;
     u32 left = 0;
@@ -407,7 +520,8 @@ std::vector<u64> make_switch_key(const HostTables &h, const std::vector<u64> &s_
     const u32 R = (u32)kp.size();
     std::vector<u64> key(2ull * dnum * R * N);
     u64 *ka = key.data(), *kb = key.data() + (size_t)dnum * R * N;
-    std::mt19937_64 rng(seed);
+    Rng rng(seed);
+    if (!rng.ok()) return {};
     std::vector<int64_t> e(N);
     std::vector<u64> pmod(R);  // P mod q_i
     for (u32 r = 0; r < R; ++r) {
@@ -464,7 +578,7 @@ int key_for(const HostTables &h, int kind, const int8_t *s, const int64_t *s_new
             for (u32 i = 0; i < N; ++i) sn[r * N + i] = s_e[r * N + map[i]];
     }
     out = make_switch_key(h, s_e, sn, key_limbs, seed);
-    return FHEA_OK;
+    return out.empty() ? FHEA_ERR_STATE : FHEA_OK;
 }
 
 int write_key_to_arena(fhea_ctx *c, const std::vector<u64> &key, uint64_t evk) {
@@ -475,7 +589,8 @@ int write_key_to_arena(fhea_ctx *c, const std::vector<u64> &key, uint64_t evk) {
 
 int secret(u32 N, uint64_t seed, int8_t *s_out) {
     if (!s_out) return FHEA_ERR_ARG;
-    std::mt19937_64 rng(seed);
+    Rng rng(seed);
+    if (!rng.ok()) return FHEA_ERR_STATE;
     std::uniform_int_distribution<int> d(-1, 1);
     for (u32 j = 0; j < N; ++j) s_out[j] = (int8_t)d(rng);
     return FHEA_OK;
@@ -893,6 +1008,12 @@ int fhea_decode(const fhea_client *cl, const uint64_t *pt, uint32_t limbs, doubl
     return FHEA_OK;
 }
 
+int fhea_chacha20_block(const uint32_t *key, uint32_t counter, const uint32_t *nonce, uint32_t *out) {
+    if (!key || !nonce || !out) return FHEA_ERR_ARG;
+    chacha20_block(key, counter, nonce, out);
+    return FHEA_OK;
+}
+
 int fhea_keygen_public(const fhea_client *cl, const int8_t *s, uint64_t seed, uint64_t *pk) {
     if (!cl) return FHEA_ERR_STATE;
     const HostTables &h = cl->h;
@@ -901,7 +1022,8 @@ int fhea_keygen_public(const fhea_client *cl, const int8_t *s, uint64_t seed, ui
     std::vector<int64_t> si = s_to_i64(s, N), e(N);
     const std::vector<u32> qp = q_primes(L);
     std::vector<u64> s_e = to_eval(h, si.data(), qp);
-    std::mt19937_64 rng(seed);
+    Rng rng(seed);
+    if (!rng.ok()) return FHEA_ERR_STATE;
     gaussian(rng, e);
     std::vector<u64> e_e = to_eval(h, e.data(), qp);
     u64 *b = pk, *a = pk + (size_t)L * N;
@@ -924,7 +1046,8 @@ int fhea_encrypt(const fhea_client *cl, const uint64_t *pk, const uint64_t *pt, 
     const HostTables &h = cl->h;
     const u32 N = h.N, L = h.L;
     if (!pk || !pt || !ct || limbs == 0 || limbs > L) return FHEA_ERR_ARG;
-    std::mt19937_64 rng(seed);
+    Rng rng(seed);
+    if (!rng.ok()) return FHEA_ERR_STATE;
     std::uniform_int_distribution<int> tern(-1, 1);
     std::vector<int64_t> v(N), e0(N), e1(N);
     for (auto &x : v) x = tern(rng);

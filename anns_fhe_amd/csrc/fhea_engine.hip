@@ -1281,6 +1281,20 @@ int fhea_ctx_get_tuning(const fhea_ctx *c, int knob, uint32_t *v) {
     }
 }
 
+int fhea_mem_info(uint32_t device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return FHEA_ERR_ARG;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return FHEA_ERR_DEVICE;
+    if ((int)device != cur && hipSetDevice((int)device) != hipSuccess) return FHEA_ERR_DEVICE;
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    if ((int)device != cur) hipSetDevice(cur);
+    if (e != hipSuccess) return FHEA_ERR_DEVICE;
+    *free_bytes = f;
+    *total_bytes = t;
+    return FHEA_OK;
+}
+
 void *fhea_host_alloc(uint64_t bytes) {
     void *p = nullptr;
     if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) return nullptr;

@@ -19,6 +19,15 @@ def _p64(a):
     return a.ctypes.data_as(_lib.u64p)
 
 
+def _seed(seed):
+    """None -> FHEA_SEED_RANDOM (secure generator); an integer -> reproducible stream."""
+    if seed is None:
+        return _lib.SEED_RANDOM
+    if not 0 <= int(seed) < _lib.SEED_RANDOM:
+        raise ValueError("seed must be None or an integer in [0, 2^64 - 1)")
+    return int(seed)
+
+
 class PinnedArray:
     """A uint64 numpy array in page-locked host memory (fhea_host_alloc): arena reads
     and writes through it run at full DMA speed instead of through a bounce buffer."""
@@ -273,9 +282,10 @@ class FheContext:
         return ms.value
 
     # ---- client key generation (CPU)
-    def keygen_secret(self, seed):
+    def keygen_secret(self, seed=None):
+        """seed None: the secure generator; an integer: a reproducible stream (tests)."""
         s = np.zeros(self.N, np.int8)
-        check(self.lib.fhea_keygen_secret(self.h, seed, s.ctypes.data_as(_lib.i8p)), "keygen_secret")
+        check(self.lib.fhea_keygen_secret(self.h, _seed(seed), s.ctypes.data_as(_lib.i8p)), "keygen_secret")
         return s
 
     def key_size_words(self, key_limbs):
@@ -285,17 +295,17 @@ class FheContext:
         return int(self.lib.fhea_galois_for_step(self.h, step))
 
     def keygen_relin_key(self, s, key_limbs, seed, evk):
-        check(self.lib.fhea_keygen_relin_key(self.h, s.ctypes.data_as(_lib.i8p), key_limbs, seed, evk),
+        check(self.lib.fhea_keygen_relin_key(self.h, s.ctypes.data_as(_lib.i8p), key_limbs, _seed(seed), evk),
               "keygen_relin_key")
 
     def keygen_rotation_key(self, s, galois, key_limbs, seed, evk):
-        check(self.lib.fhea_keygen_rotation_key(self.h, s.ctypes.data_as(_lib.i8p), galois, key_limbs, seed,
+        check(self.lib.fhea_keygen_rotation_key(self.h, s.ctypes.data_as(_lib.i8p), galois, key_limbs, _seed(seed),
                                                 evk), "keygen_rotation_key")
 
     def keygen_switch_key(self, s, s_new, key_limbs, seed, evk):
         sn = np.ascontiguousarray(s_new, dtype=np.int64)
         check(self.lib.fhea_keygen_switch_key(self.h, s.ctypes.data_as(_lib.i8p), sn.ctypes.data_as(_lib.i64p),
-                                              key_limbs, seed, evk), "keygen_switch_key")
+                                              key_limbs, _seed(seed), evk), "keygen_switch_key")
 
 
 class FHEHostDriver:

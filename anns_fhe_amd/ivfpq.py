@@ -167,25 +167,32 @@ class FheIvfPqClient:
     distances, cluster selection and top-k (fhe_context_manager.cpp:457-512,
     interactive_query_manager.cpp:166-174,276-290)."""
 
-    def __init__(self, ckks, dim=128, log_scale=40, lt=2, seed=1):
+    def __init__(self, ckks, dim=128, log_scale=40, lt=2, seed=None):
+        """seed None (default): keys and encryptions draw from the secure generator;
+        an integer makes them reproducible (tests, replicas sharing keys), not secure."""
         self.ck = ckks
         self.D = dim
         self.scale = 2.0 ** log_scale
         self.lt = lt
-        self.s = ckks.keygen_secret(seed)
+        self.seed0 = seed
+        self.s = ckks.keygen_secret(self.next_seed(0))
         self.s_eval = ckks.secret_eval(self.s)  # decryption reuses the transformed secret
-        self.pk = ckks.keygen_public(self.s, seed + 1)
-        self.seed = seed + 100
+        self.pk = ckks.keygen_public(self.s, self.next_seed(1))
+        self.seed = 100
+
+    def next_seed(self, k):
+        """The seed of draw k: None (secure) without a base seed, else base + k."""
+        return None if self.seed0 is None else self.seed0 + k
 
     def evaluation_keys(self, threads=16):
         """Relinearisation key and rotation keys for r = 1..D-1, all at lt limbs
         (generated on host threads; each key has its own seed, so the result does
         not depend on the thread count)."""
         import concurrent.futures as cf
-        rel = self.ck.relin_key(self.s, self.lt, self.seed)
+        rel = self.ck.relin_key(self.s, self.lt, self.next_seed(self.seed))
 
         def one(r):
-            return r, self.ck.rotation_key(self.s, self.ck.galois_for_step(r), self.lt, self.seed + r)
+            return r, self.ck.rotation_key(self.s, self.ck.galois_for_step(r), self.lt, self.next_seed(self.seed + r))
 
         with cf.ThreadPoolExecutor(max(1, threads)) as ex:
             rots = dict(ex.map(one, range(1, self.D)))
@@ -194,7 +201,8 @@ class FheIvfPqClient:
     def encrypt_query(self, q, limbs=None):
         limbs = limbs or self.ck.L
         self.seed += 1
-        return self.ck.encrypt_values(self.pk, np.asarray(q, np.float64), limbs, self.scale, self.seed)
+        return self.ck.encrypt_values(self.pk, np.asarray(q, np.float64), limbs, self.scale,
+                                      self.next_seed(self.seed))
 
     def result_scale(self, q_last):
         return self.scale * self.scale / q_last
@@ -589,7 +597,7 @@ def execute_interactive_queries(server, client, queries, n_probe, top_k, client_
     pool = cf.ThreadPoolExecutor(max(1, client_threads))
     t = {}
     t0 = time.perf_counter()
-    seeds = [client.seed + 1 + z for z in range(B)]
+    seeds = [client.next_seed(client.seed + 1 + z) for z in range(B)]
     client.seed += B
     cts = list(pool.map(lambda zq: ck.encrypt_values(client.pk, np.asarray(zq[1], np.float64), ck.L, client.scale,
                                                       seeds[zq[0]]), enumerate(queries)))

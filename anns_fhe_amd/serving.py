@@ -6,8 +6,10 @@ FPGA path is strictly sequential, cpp_host_model/src/interactive_query_manager.c
 
 * Parameter sets.  Each set (for example N=2^14/L=8 and N=2^16/L=24) owns one
   engine context (its own HIP stream and HBM arenas), its encoded database, its
-  evaluation keys and a fixed number of query slots.  Sets of one `group` are
-  replicas (same parameters and seed, so the same keys and database words): a
+  evaluation keys (from the secure generator unless a seed is given) and a number
+  of query slots (fixed, or as many as a share of the free HBM holds).  Sets of one
+  `group` are replicas (same parameters and an explicit seed, so the same keys and
+  database words): a
   request names the group and is admitted to the replica with the most free slots,
   which gives the group one more worker and stream on the device.
 * Admission.  A request is admitted when its set has a free slot and the HBM
@@ -59,7 +61,7 @@ class Request:
 class ParamSet:
     """One parameter set: context, client keys, encoded database, slots."""
 
-    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=2, log_scale=40, seed=1, threads=0, log=None,
+    def __init__(self, name, ctx, ck, index, max_slots, max_g, lt=2, log_scale=40, seed=None, threads=0, log=None,
                  group=None):
         self.name = name
         self.group = group or name
@@ -87,20 +89,49 @@ class ParamSet:
         return words, index.dim * 2 * dnum * (lt + K) * N + 8, max_g
 
     @classmethod
-    def create(cls, name, N, L, K, dnum, index, max_slots, lt=2, log_scale=40, max_probe=8, seed=1, threads=0,
-               log=None, group=None):
+    def create(cls, name, N, L, K, dnum, index, max_slots=None, lt=2, log_scale=40, max_probe=8, seed=None, threads=0,
+               log=None, group=None, hbm_share=0.5, device=0, slot_cap=256):
+        """max_slots None: as many query slots as `hbm_share` of the device's free HBM holds
+        beside the set's database, keys and pipeline workspace (at most slot_cap)."""
+        if max_slots is None:
+            max_slots = cls.slots_for_memory(N, L, K, dnum, index, lt, max_probe, hbm_share, device, slot_cap)
         words, kw, max_g = cls.sizes(N, L, K, dnum, index, max_slots, lt, max_probe)
-        ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=kw)
+        ctx = FheContext(N, L, K, dnum, poly_words=words, key_words=kw, device=device)
         ck = CkksClient(0, 0, 0, 0, ctx=ctx)
         return cls(name, ctx, ck, index, max_slots, max_g, lt, log_scale, seed, threads, log, group)
 
-    def encrypt(self, q, seed):
+    @classmethod
+    def slots_for_memory(cls, N, L, K, dnum, index, lt=2, max_probe=8, hbm_share=0.5, device=0, slot_cap=256):
+        """Query slots that fit in hbm_share of the free device memory (hipMemGetInfo): the
+        arena grows by a fixed number of words per slot; one slot's worth of key-switch
+        workspace per slot is reserved on top (the stage-1 pipeline's chunks)."""
+        from . import _lib
+        free, total = _lib.mem_info(device)
+        w1, kw, _ = cls.sizes(N, L, K, dnum, index, 1, lt, max_probe)
+        w2, _, _ = cls.sizes(N, L, K, dnum, index, 2, lt, max_probe)
+        per_slot = 8 * (w2 - w1)
+        static = 8 * (w1 - (w2 - w1) + kw)
+        avail = int(free * hbm_share) - static
+        per_slot_total = 2 * per_slot  # the slot's words and its share of the workspace
+        if avail < per_slot_total:
+            raise MemoryError(f"{name_of(N, L)}: {free} free bytes hold no query slot")
+        return max(1, min(slot_cap, avail // per_slot_total))
+
+    def encrypt(self, q, seed=None):
+        """The query under the set's key: with a set seed (reproducible keys) `seed` makes
+        the encryption reproducible too; without one it is drawn from the secure generator."""
+        if self.client.seed0 is None:
+            seed = None
         return self.ck.encrypt_values(self.client.pk, np.asarray(q, np.float64), self.ck.L, self.client.scale, seed)
 
     def close(self):
         close = getattr(self.ctx, "close", None)
         if close:
             close()
+
+
+def name_of(N, L):
+    return f"N=2^{N.bit_length() - 1}/L={L}"
 
 
 def poisson_arrivals(rate, n, seed):

@@ -366,7 +366,7 @@ def bench_ivfpq(args):
     words = IV.FheIvfPqServer.arena_words(Nq, Lq, lt, 128, n_blocks, B, max_g)
     ctx = FheContext(Nq, Lq, Kq, dq, poly_words=words, key_words=128 * 2 * dq * (lt + Kq) * Nq + 8)
     ck = CkksClient(0, 0, 0, 0, ctx=ctx)
-    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt, seed=11)
+    client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=lt)  # keys and encryptions from the secure generator
     t1 = time.perf_counter()
     server = IV.FheIvfPqServer(ctx, ck, ix, lt=lt, log_scale=40, max_queries=B, max_blocks_per_query=max_g,
                                log=log)
@@ -460,10 +460,11 @@ def bench_serve(args):
     x, _ = IV.make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
     log("serve: building parameter sets")
-    small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=48, seed=1, log=log)
+    small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=48, log=log)
     nrep = max(1, args.serve_large_replicas)
-    larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, 8, 3, ix, max_slots=16, seed=2,
-                                 log=log, group="n16") for i in range(nrep)]
+    # replicas share their keys through an explicit seed; a single set draws them securely
+    larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, 8, 3, ix, max_slots=16,
+                                 seed=None if nrep == 1 else 2, log=log, group="n16") for i in range(nrep)]
     large = larges[0]
     if args.serve_chunk:
         from anns_fhe_amd import _lib
@@ -471,6 +472,11 @@ def bench_serve(args):
             p.ctx.set_tuning(_lib.TUNE_CHUNK, args.serve_chunk)
     budget = small.static_bytes + nrep * large.static_bytes + args.serve_small_slots * small.slot_bytes + \
         args.serve_large_slots * large.slot_bytes
+    from anns_fhe_amd import _lib
+    hbm_free, hbm_total = _lib.mem_info(0)
+    # the slot counts the device's free HBM would give each set (half of it each, at most 256)
+    mem_slots = {"n14": SV.ParamSet.slots_for_memory(1 << 14, 8, 4, 2, ix, hbm_share=0.5),
+                 "n16": SV.ParamSet.slots_for_memory(1 << 16, 24, 8, 3, ix, hbm_share=0.5 / nrep)}
     rng = np.random.default_rng(7)
     sweep = []
     for rate in [float(r) for r in args.rates.split(",")]:
@@ -558,6 +564,9 @@ def bench_serve(args):
                    "db": n, "requests_per_rate": args.requests, "budget_bytes": budget,
                    "large_replicas": nrep, "ks_chunk": args.serve_chunk or None,
                    "eval_limbs": {"n14": small.server.lt, "n16": large.server.lt},
+                   "hbm": {"free_bytes_after_sets": hbm_free, "total_bytes": hbm_total,
+                           "slots_admitted": {"n14": args.serve_small_slots, "n16": args.serve_large_slots},
+                           "slots_free_hbm_would_hold": mem_slots},
                    "query_limbs": {"n14": 8, "n16": 24}},
         "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "at_rate": best["rate"]},
         "sweep": sweep,

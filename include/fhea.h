@@ -126,6 +126,10 @@ int fhea_write_key(fhea_ctx *ctx, uint64_t word_off, const uint64_t *host, uint6
 int fhea_read_key(fhea_ctx *ctx, uint64_t word_off, uint64_t *host, uint64_t count);
 int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t count);
 /* Device base pointers of the arenas (poly_gmem_ptr / key_gmem_ptr, sim_memory_bus.h:16-17). */
+/* Free and total device memory of HIP device `device` in bytes (hipMemGetInfo), for
+ * sizing arenas and admission budgets from what the GPU actually has.  No reference
+ * counterpart (its memories are fixed-size host vectors, sim_memory_bus.h:13). */
+int fhea_mem_info(uint32_t device, uint64_t *free_bytes, uint64_t *total_bytes);
 /* Page-locked host memory (hipHostMalloc) for arena reads and writes at full DMA
  * speed; NULL on failure.  Free with fhea_host_free. */
 void *fhea_host_alloc(uint64_t bytes);
@@ -301,6 +305,17 @@ int fhea_eval_linear_sum_groups(fhea_ctx *ctx, uint32_t n_groups, uint32_t R, co
 int fhea_timer_start(fhea_ctx *ctx);
 /* milliseconds since fhea_timer_start, synchronising the stream */
 int fhea_timer_stop(fhea_ctx *ctx, float *ms);
+
+/* ---- randomness of key generation and encryption -----------------------
+ * Every seed argument below selects the source: FHEA_SEED_RANDOM draws from a
+ * cryptographically secure generator (ChaCha20, RFC 8439, keyed from the OS with
+ * getrandom(2); the reference relies on OpenFHE's PRNG for this); any other value
+ * seeds std::mt19937_64, which is reproducible (tests, checks against the oracle)
+ * and NOT secure: its state follows from the outputs a public key exposes. */
+#define FHEA_SEED_RANDOM (~(uint64_t)0)
+/* The keystream block of that generator: out = ChaCha20(key[8], counter, nonce[3])
+ * (RFC 8439 section 2.3), exported so that its known-answer vectors can be checked. */
+int fhea_chacha20_block(const uint32_t *key, uint32_t counter, const uint32_t *nonce, uint32_t *out);
 
 /* ---- CPU client (key generation), the role of OpenFHE KeyGen /
  *      EvalMultKeyGen / EvalRotateKeyGen in

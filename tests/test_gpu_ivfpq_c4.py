@@ -1,0 +1,122 @@
+"""Encrypted IVF-PQ at the benchmark's own configuration (C4): 1M x 128 vectors
+from a seeded mixture of 32 Gaussians (sigma 0.1), n_list=32, M=8, K=256 (uint8
+codes), a batch of B=64 queries encrypted at N=2^14, L=8 (K=4, dnum=2), n_probe=4
+(interactive_query_manager.cpp:182-274 generalised to batches and n_probe > 1).
+
+* Every returned candidate's decrypted distance equals its plaintext ADC distance
+  (the squared distance to its PQ reconstruction) within 1e-3, the probed clusters equal the plaintext search's, the top-10 ids
+  equal the plaintext top-10 up to candidates tied with the 10th distance within
+  the CKKS error, and recall@10 against exact k-NN is unchanged except for such ties.
+* At the same parameter shape, one query's server-side result words (stage-1
+  coarse distances and stage-3 candidate distances) equal an oracle-backed run of
+  the same protocol, on a 20k-vector database with the same index structure
+  (every cluster one block, so every stage-3 sum is checked).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from anns_fhe_amd import ivfpq as IV
+
+pytestmark = pytest.mark.gpu
+
+N, L, K, DNUM, LT = 1 << 14, 8, 4, 2, 2
+
+
+def say(*a):
+    print("[gpu-ivfpq-c4]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _queries(x, B, seed):
+    rng = np.random.default_rng(seed)
+    return [x[rng.integers(len(x))] + rng.normal(0, 0.05, x.shape[1]).astype(np.float32) for _ in range(B)]
+
+
+def test_c4_full_configuration_batch():
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.driver import FheContext
+    say("dataset 1M x 128 + index")
+    x, _ = IV.make_gaussian_mixture(1 << 20, dim=128, n_centres=32, sigma=0.1, seed=42)
+    ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
+    B, n_probe = 64, 4
+    per = sorted((-(-len(l) // (N // 2)) for l in ix.lists), reverse=True)
+    max_g = sum(per[:n_probe])
+    n_blocks = 1 + sum(per)
+    words = IV.FheIvfPqServer.arena_words(N, L, LT, 128, n_blocks, B, max_g)
+    ctx = FheContext(N, L, K, DNUM, poly_words=words, key_words=128 * 2 * DNUM * (LT + K) * N + 8)
+    try:
+        ck = CkksClient(0, 0, 0, 0, ctx=ctx)
+        client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=LT, seed=11)
+        say("encode database", n_blocks, "blocks")
+        server = IV.FheIvfPqServer(ctx, ck, ix, lt=LT, log_scale=40, max_queries=B, max_blocks_per_query=max_g)
+        rel, rots = client.evaluation_keys()
+        server.load_keys(rel, rots)
+        queries = _queries(x, B, 2024)
+        say("batch of", B)
+        res, t = IV.execute_interactive_queries(server, client, queries, n_probe=n_probe, top_k=10)
+        say("timing", {k: round(v, 3) for k, v in t.items()})
+        ps = IV.PlaintextSearcher(ix)
+        x64 = x.astype(np.float64)
+        rec_fhe, rec_plain, swapped, max_err = [], [], 0, 0.0
+        for q, (ids, dist, probes) in zip(queries, res):
+            pids, pdist, pprobes = ps.search(q, n_probe, 10)
+            assert list(probes) == list(pprobes)
+            err = float(np.max(np.abs(dist - pdist)))
+            max_err = max(max_err, err)
+            assert err < 1e-3, err
+            # every candidate's decrypted distance against its plaintext ADC distance
+            cand = np.concatenate([ix.lists[c] for c in probes])
+            adc = np.sum((ix.reconstruct(cand).astype(np.float64) - q.astype(np.float64)) ** 2, axis=1)
+            got = dict(zip(ids, dist))
+            for i, d in got.items():
+                assert abs(d - adc[np.nonzero(cand == i)[0][0]]) < 1e-3
+            diff = set(ids) ^ set(pids)
+            if diff:
+                # only candidates tied with the 10th plaintext distance may trade places
+                tenth = float(pdist[-1])
+                for i in diff:
+                    assert abs(adc[np.nonzero(cand == i)[0][0]] - tenth) < 2e-3, (i, tenth)
+                swapped += len(diff) // 2
+            truth = IV.exact_knn(x64, q, 10)
+            rec_fhe.append(IV.recall_at(ids, truth))
+            rec_plain.append(IV.recall_at(pids, truth))
+        say("recall@10", np.mean(rec_fhe), np.mean(rec_plain), "tie swaps", swapped, "max err", max_err)
+        assert abs(np.mean(rec_fhe) - np.mean(rec_plain)) <= swapped / (10.0 * B) + 1e-12
+    finally:
+        ctx.close()
+
+
+def test_c4_shape_result_words_equal_oracle(orc):
+    from anns_fhe_amd.client import CkksClient
+    from anns_fhe_amd.driver import FheContext
+    from tests.oracle_ctx import OracleContext
+    x, _ = IV.make_gaussian_mixture(20000, dim=128, n_centres=32, sigma=0.1, seed=42)
+    ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=20000, iters=8, seed=0).add(x)
+    assert all(len(l) <= N // 2 for l in ix.lists)  # every cluster one block
+    n_probe = 4
+    n_blocks = 1 + len(ix.lists)
+    words = IV.FheIvfPqServer.arena_words(N, L, LT, 128, n_blocks, 1, n_probe)
+    kw = 128 * 2 * DNUM * (LT + K) * N + 8
+    ctx = FheContext(N, L, K, DNUM, poly_words=words, key_words=kw)
+    try:
+        ck = CkksClient(0, 0, 0, 0, ctx=ctx)
+        octx = OracleContext(orc, N, L, K, DNUM, ck.q, ck.p, words, kw)
+        q = _queries(x, 1, 77)
+        out = []
+        for c in (ctx, octx):
+            client = IV.FheIvfPqClient(ck, dim=128, log_scale=40, lt=LT, seed=5)
+            server = IV.FheIvfPqServer(c, ck, ix, lt=LT, log_scale=40, max_queries=1, max_blocks_per_query=n_probe)
+            rel, rots = client.evaluation_keys()
+            server.load_keys(rel, rots)
+            say("protocol on", type(c).__name__)
+            res, _ = IV.execute_interactive_queries(server, client, q, n_probe=n_probe, top_k=10)
+            o = server._offs(0)
+            out.append((res, c.read_poly(o["norm"], 2 * LT * N), c.read_poly(o["res"], 2 * (LT - 1) * N * n_probe)))
+        (rg, ng, wg), (ro, no, wo) = out
+        assert np.array_equal(ng, no)
+        assert np.array_equal(wg, wo)
+        assert list(rg[0][2]) == list(ro[0][2]) and np.array_equal(rg[0][0], ro[0][0])
+    finally:
+        ctx.close()

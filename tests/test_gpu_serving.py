@@ -52,3 +52,20 @@ def test_serving_mixed_params_poisson():
     finally:
         small.close()
         large.close()
+
+
+def test_slots_sized_from_device_memory():
+    """ParamSet.slots_for_memory: the query slots a share of the device's free HBM
+    (hipMemGetInfo through fhea_mem_info) holds; more share, more slots, capped."""
+    from anns_fhe_amd import _lib
+    free, total = _lib.mem_info(0)
+    say("device memory", free, total)
+    assert 0 < free <= total
+    x, _ = IV.make_gaussian_mixture(20000, dim=128, n_centres=8, sigma=0.1, seed=1)
+    ix = IV.IvfPqIndex(n_list=8, M=8, K=256).train(x, train_size=20000, iters=4, seed=0).add(x)
+    s1 = SV.ParamSet.slots_for_memory(1 << 14, 8, 4, 2, ix, hbm_share=0.01, slot_cap=10 ** 6)
+    s2 = SV.ParamSet.slots_for_memory(1 << 14, 8, 4, 2, ix, hbm_share=0.02, slot_cap=10 ** 6)
+    assert 1 <= s1 < s2
+    w1, kw, _ = SV.ParamSet.sizes(1 << 14, 8, 4, 2, ix, s1)
+    assert 8 * (w1 + kw) <= 0.01 * free + 8 * kw
+    assert SV.ParamSet.slots_for_memory(1 << 14, 8, 4, 2, ix, hbm_share=0.5, slot_cap=48) == 48

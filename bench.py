@@ -50,7 +50,7 @@ def parse():
                     help="serve: N=2^16 query slots the HBM budget admits at once (at most 16 per replica)")
     ap.add_argument("--serve-large-replicas", type=int, default=1,
                     help="serve: contexts serving the N=2^16 set (replicas: own stream and worker each)")
-    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve"],
+    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve", "c1"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
                          "ks64: 64 rotations with distinct resident keys; ivfpq: encrypted IVF-PQ "
                          "search (1M x 128, n_list=32, M=8, K=256, B=64 queries, N=2^14, L=8, n_probe=4)")
@@ -246,6 +246,69 @@ def cpu_rotations(ctx, key_blob, sample):
     return {"value": sample / dt, "unit": "rotations/s", "cores": 1, "kind": "port",
             "sample": f"{sample} x eval_rotate(+1) N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread of "
                       f"{_cpu_name()})"}
+
+
+def bench_c1(args):
+    """C1: the reference's plain single-ciphertext sequence at N=2^13, L=4 (K=1, dnum=2):
+    eval_sub_plain -> eval_mult_relin_rescale -> eval_rotate(+1) (main_sim.cpp order),
+    one ciphertext, synchronised after the sequence; the oracle runs the same sequence."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    Nc, Lc, Kc, dc = 8192, 4, 1, 2
+    W = Lc * Nc
+    ctx = FheContext(Nc, Lc, Kc, dc, poly_words=12 * W, key_words=2 * 2 * dc * (Lc + Kc) * Nc + 8)
+    drv = FHEHostDriver(ctx)
+    s = ctx.keygen_secret(42)
+    ksz = ctx.key_size_words(Lc)
+    ctx.keygen_relin_key(s, Lc, 43, 0)
+    g1 = ctx.galois_for_step(1)
+    ctx.keygen_rotation_key(s, g1, Lc, 44, ksz)
+    drv.set_relin_key(0, Lc)
+    drv.add_rotation_key(1, g1, ksz, Lc)
+    rng = np.random.default_rng(42)
+    data = np.concatenate([rand_limbs(rng, ctx.q, Nc).reshape(-1) for _ in range(3)])
+    ctx.write_poly(0, data)
+
+    def seq():
+        drv.eval_sub_plain(0, W, 2 * W, 3 * W, 4 * W, Lc)
+        drv.eval_mult_relin_rescale(3 * W, 4 * W, 3 * W, 4 * W, 5 * W, 6 * W, Lc)
+        drv.eval_rotate(5 * W, 6 * W, 7 * W, 8 * W, 1, Lc - 1)
+
+    for _ in range(max(1, args.warmup)):
+        seq()
+    ctx.sync()
+    lat = []
+    for _ in range(args.steps):
+        ctx.timer_start()
+        seq()
+        lat.append(ctx.timer_stop())
+    ms = float(np.median(lat))
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import oracle as O
+        O.build()
+        o = O.OracleCtx(Nc, ctx.q, ctx.p, dc)
+        km = ctx.read_key(0, 2 * ksz)
+        pm = np.zeros(12 * W, np.uint64)
+        pm[:3 * W] = data
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert o.eval_sub_plain(pm, 0, W, 2 * W, 3 * W, 4 * W, Lc) == 0
+            assert o.eval_mult_relin_rescale(pm, km, 0, Lc, 3 * W, 4 * W, 3 * W, 4 * W, 5 * W, 6 * W, Lc) == 0
+            assert o.eval_rotate(pm, km, [(1, g1, ksz, Lc)], 5 * W, 6 * W, 7 * W, 8 * W, 1, Lc - 1) == 0
+        dt = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(dt * 1e3, 2), "unit": "ms per sequence", "cores": 1, "kind": "port",
+               "sample": f"{reps} sequences, 1 thread of {_cpu_name()}"}
+    log(f"c1: {ms:.3f} ms per sequence (median of {args.steps})")
+    return {
+        "metric": "C1 sequence latency (eval_sub_plain, eval_mult_relin_rescale, eval_rotate(+1)), N=2^13, L=4",
+        "value": round(ms, 4), "unit": "ms per sequence", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic (uniform residues, seeded keys)",
+        "config": {"workload": "one ciphertext through the reference's C1 sequence, HIP events around it",
+                   "N": Nc, "L": Lc, "K": Kc, "dnum": dc, "p90_ms": round(float(np.percentile(lat, 90)), 4)},
+        "cpu_baseline": cpu,
+    }
 
 
 def bench_ntt256(args):
@@ -625,7 +688,7 @@ def main():
         sys.exit(2)
     if args.workload != "hmult":
         res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq,
-               "serve": bench_serve}[args.workload](args)
+               "serve": bench_serve, "c1": bench_c1}[args.workload](args)
         print(json.dumps(res), flush=True)
         return
     from anns_fhe_amd.driver import FheContext, FHEHostDriver

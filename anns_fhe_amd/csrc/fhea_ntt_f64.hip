@@ -44,6 +44,7 @@
 //                     two steps run one after the other)
 #include "fhea_f64.hpp"
 #include "fhea_kernels.hpp"
+#include "fhea_ntt_tile.hpp"
 
 namespace fhea {
 
@@ -70,27 +71,12 @@ static constexpr u32 F_TPB = 256;
 static constexpr u32 F_LOG_TILE = 12;
 static constexpr u32 F_TILE = 1u << F_LOG_TILE;
 
-__device__ __forceinline__ u32 lds_pad(u32 e) { return e + (e >> 4); }
-
-template <bool INV, int LOGR, int RD>
-struct Round {
-    static constexpr int NR = (LOGR + 3) / 4;
-    static constexpr int NS = (RD + 1 < NR) ? 4 : (LOGR - 4 * RD);  // stages in this round
-    static constexpr int TOP = LOGR - 1 - 4 * RD;
-    // window [P, P+4) of row-index bits; BIT0 = bit of the round's first stage
-    static constexpr int P = INV ? ((4 * RD + 4 <= LOGR) ? 4 * RD : LOGR - 4) : (TOP >= 3 ? TOP - 3 : 0);
-    static constexpr int BIT0 = INV ? 4 * RD : TOP;
-};
-
 struct TileGeom {
     u32 logN, logS, logC, logT, b0, c0;
     const double *tw;
     double q, qinv;
     // global word index of tile element e
-    __device__ __forceinline__ u32 gidx(u32 e, u32 logR) const {
-        const u32 c = e & ((1u << logC) - 1), r = (e >> logC) & ((1u << logR) - 1), ob = e >> (logC + logR);
-        return ((b0 + ob) << (logR + logS)) + (r << logS) + c0 + c;
-    }
+    __device__ __forceinline__ u32 gidx(u32 e, u32 logR) const { return tile_gidx(e, logR, logS, logC, b0, c0); }
 };
 
 // Twiddles of a stage.  The thread needs the 2^(3-wb) pairs tw[wbase | t]; wbase
@@ -234,39 +220,8 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
     else ((double *)dbase)[j] = v;
 }
 
-// Work item = one 4096-word tile (tiles_per_limb per limb slot, n_tiles in all).
-// The kernel is persistent: workgroup w takes tiles w, w + G, w + 2G, ... and
-// issues the next tile's global loads before the current tile's arithmetic, so
-// loads overlap FP64 work.  Tiles of skipped slots (prime map 0xFFFFFFFF) are
-// passed over.
-struct TileId {
-    u32 tile, poly, slot, pidx, tI;
-};
-
-// Tiles are numbered (limb slot, polynomial, tile-in-limb) with the polynomial
-// varying fastest after the tile: the same prime's limbs of a whole batch are
-// in flight together and share its twiddle table in L2.
-__device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 npolys,
-                                            const u32 *prime_map, const u32 *slots) {
-    TileId id{n_tiles, 0, 0, 0, 0};
-    for (; t < n_tiles; t += gridDim.x) {
-        const u32 yl = t / tiles_per_limb;
-        const u32 k = yl / npolys;
-        const u32 slot = slots ? slots[k] : k;
-        const u32 p = prime_map[slot];
-        if (p != 0xFFFFFFFFu) {
-            id.tile = t;
-            id.poly = yl % npolys;
-            id.slot = slot;
-            id.pidx = p;
-            id.tI = t % tiles_per_limb;
-            return id;
-        }
-    }
-    return id;
-}
-
-
+// The kernel is persistent (find_tile) and issues the next tile's global loads
+// before the current tile's arithmetic, so loads overlap FP64 work.
 template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH>
 __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,

@@ -82,10 +82,10 @@ def rand_limbs(rng, primes, n):
     return np.stack([rng.integers(0, q, n, dtype=np.uint64) for q in primes])
 
 
-def traffic_from_profile():
+def traffic_from_profile(name="ntt_N65536_L24_1536limbs_traffic.json"):
     """HBM bytes per roofline launch from the committed rocprof PMC summary
     (FETCH_SIZE x its copy-kernel scale + WRITE_SIZE), or None if absent."""
-    path = os.path.join(ROOT, "profiles", "ntt_N65536_L24_1536limbs_traffic.json")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return json.load(f)["traffic_bytes_per_launch"]
@@ -769,6 +769,8 @@ def main():
     achieved = alg_bytes / (ntt_launch_ms / 1e3) / 1e9
     log(f"NTT launch ({npr} polys x {L} limbs): {ntt_launch_ms:.3f} ms -> {achieved:.0f} GB/s algorithmic")
 
+    ntt_kernel, traffic_file = ("k_ntt64f", "ntt_N65536_L24_1536limbs_traffic.json") if args.chain == "q50" else \
+        ("k_ntt64i", "ntt_int_N65536_L24_1536limbs_traffic.json")
     result = {
         "metric": "HMult+relin+rescale ops/s at N=2^16,L=24",
         "value": round(value, 2),
@@ -789,9 +791,9 @@ def main():
                    "kernels": "FP64" if args.chain == "q50" else "64-bit integer",
                    "schedule": schedule},
         "roofline": {"bound": "hbm",
-                     "kernel": f"batched forward NTT, {npr} polys x 24 limbs (k_ntt64f COLS + ROWS dispatches)",
+                     "kernel": f"batched forward NTT, {npr} polys x 24 limbs ({ntt_kernel} COLS + ROWS dispatches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(traffic_file),
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},
     }
     # achievable HBM rate on this box: a 1 GiB device-to-device copy (read + write), best of 3;

@@ -48,11 +48,14 @@ def test_tables_match_oracle(orc):
     assert np.array_equal(ctx.table(3), o.itw)
 
 
+@pytest.mark.parametrize("q_bits", [50, 61])
 @pytest.mark.parametrize("N,L", [(1024, 3), (2048, 2), (4096, 2), (8192, 4), (16384, 3), (32768, 2),
                                  (65536, 3), (131072, 2)])
-def test_ntt_intt_parity(orc, N, L):
-    say("ntt", N, L)
-    ctx = get_ctx(N, L, 1, 1)
+def test_ntt_intt_parity(orc, N, L, q_bits):
+    """q_bits 50: the FP64 transform; 61: the integer transform (lazy words up to 4q,
+    the largest primes the tables accept)."""
+    say("ntt", N, L, q_bits)
+    ctx = get_ctx(N, L, 1, 1, q_bits=q_bits)
     o = get_orc(orc, ctx)
     rng = np.random.default_rng(42)
     a = rand_poly(rng, ctx.q, N)
@@ -515,15 +518,17 @@ def test_hoisted_rotations_parity(orc, q_bits):
     assert np.array_equal(got, pm[base * W:(base + 8) * W])
 
 
-def test_ntt_splits_and_tuning_knobs(orc):
+@pytest.mark.parametrize("q_bits", [50, 61])
+def test_ntt_splits_and_tuning_knobs(orc, q_bits):
     """Every NTT split the tuner may choose (N=2^16: first pass of 6..10 stages) gives
-    the oracle's transform word for word; invalid knob values are refused; chunk
-    size does not change a batched result."""
+    the oracle's transform word for word, on the FP64 (50-bit) and the integer
+    (61-bit) kernels; invalid knob values are refused; chunk size does not change a
+    batched result."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd._lib import FheError
     from anns_fhe_amd import autotune as AT
     N, L = 65536, 3
-    ctx = get_ctx(N, L, 1, 1)
+    ctx = get_ctx(N, L, 1, 1, q_bits=q_bits)
     o = get_orc(orc, ctx)
     rng = np.random.default_rng(77)
     a = rand_poly(rng, ctx.q, N)
@@ -531,7 +536,7 @@ def test_ntt_splits_and_tuning_knobs(orc):
     ctx.write_poly(0, a)
     fwd, inv = o.ntt_poly(a), o.intt_poly(a)
     for k in AT.ntt_splits(16):
-        say("split", k)
+        say("split", k, q_bits)
         ctx.set_tuning(_lib.TUNE_NTT_SPLIT_FWD, k)
         ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, k)
         for w in (0, 2):

@@ -225,8 +225,14 @@ static int build_device_tables(fhea_ctx *c) {
         qmax = std::max(qmax, h.primes[i]);
     }
     use_f64 &= qmax / qmin < 4;
+    // a chain with primes on both sides of 2^50 runs the integer kernels, except
+    // for the transforms of its small-prime limbs (FP64 tables for those)
+    std::vector<u64> mask_small(4, 0), mask_big(4, 0);
+    for (u32 i = 0; i < T; ++i) (h.primes[i] < (1ull << 50) ? mask_small : mask_big)[i >> 6] |= 1ull << (i & 63);
+    const bool mixed = !use_f64 && mask_small != std::vector<u64>(4, 0);
+    const size_t o_mask_small = b.addv(mask_small), o_mask_big = b.addv(mask_big);
     size_t o_twf = 0, o_itwf = 0, o_ninvf = 0;
-    if (use_f64) {
+    if (use_f64 || mixed) {
         std::vector<double> twf((size_t)T * N * 2), itwf((size_t)T * N * 2), ninvf(2 * T);
         for (u32 i = 0; i < T; ++i) {
             const double qd = (double)h.primes[i];
@@ -429,7 +435,10 @@ static int build_device_tables(fhea_ctx *c) {
             c->dt.max_wg = 4u * (u32)cus;
     }
     c->dt.f64_off = c->f64_off;
-    if (use_f64) {
+    c->dt.ntt_mixed = mixed && !c->force_int;
+    c->dt.pmask_small = (const u64 *)P(o_mask_small);
+    c->dt.pmask_big = (const u64 *)P(o_mask_big);
+    if (use_f64 || mixed) {
         c->dt.twf = (const double *)P(o_twf);
         c->dt.itwf = (const double *)P(o_itwf);
         c->dt.ninvf = (const double *)P(o_ninvf);

@@ -158,6 +158,54 @@ __global__ __launch_bounds__(TPB) void k_fbc(const ConvDesc *__restrict__ descs,
     }
 }
 
+// Same arithmetic with the inputs held in registers (NIN >= n_in, fully unrolled)
+// and the descriptor's constants in LDS: qhat_i mod p_j for every (i, j) and each
+// output prime's (q, floor(2^128/q)) for the 128-bit reduction.
+static constexpr u32 FBC_I_MAX_PAIRS = 2048;  // n_in * n_out staged (16 KiB)
+static constexpr u32 FBC_I_MAX_OUT = 256;
+
+template <u32 NIN>
+__global__ __launch_bounds__(TPB) void k_fbc_i(const ConvDesc *__restrict__ descs,
+                                               const u32 *__restrict__ in_first,
+                                               const ModParams *__restrict__ mp,
+                                               const u64 *__restrict__ in, u64 in_zstride,
+                                               u64 *__restrict__ out, u64 out_pstride,
+                                               u64 out_zstride, u32 logN) {
+    __shared__ u64 s_hat[FBC_I_MAX_PAIRS];  // [i][j]
+    __shared__ u64 s_q[FBC_I_MAX_OUT], s_rlo[FBC_I_MAX_OUT], s_rhi[FBC_I_MAX_OUT];
+    const u32 p = blockIdx.y, z = blockIdx.z;
+    const ConvDesc D = descs[p];
+    const u32 n_in = D.n_in, n_out = D.n_out;
+    for (u32 k = threadIdx.x; k < n_in * n_out; k += TPB) s_hat[k] = D.hat[k];
+    for (u32 j = threadIdx.x; j < n_out; j += TPB) {
+        const ModParams &m = mp[D.out_prime[j]];
+        s_q[j] = m.q;
+        s_rlo[j] = m.r_lo;
+        s_rhi[j] = m.r_hi;
+    }
+    const u32 r = blockIdx.x * TPB + threadIdx.x;
+    const u64 *x = in + (size_t)z * in_zstride + ((size_t)in_first[p] << logN) + r;
+    u64 *o = out + (size_t)z * out_zstride + (size_t)p * out_pstride + r;
+    u64 y[NIN];
+#pragma unroll
+    for (u32 i = 0; i < NIN; ++i) {
+        y[i] = 0;
+        if (i < n_in) y[i] = mul_shoup(x[(size_t)i << logN], D.in_cst[2 * i], D.in_cst[2 * i + 1], mp[D.in_prime[i]].q);
+    }
+    __syncthreads();
+    for (u32 j = 0; j < n_out; ++j) {
+        u128 acc = 0;
+#pragma unroll
+        for (u32 i = 0; i < NIN; ++i)
+            if (i < n_in) acc += (u128)y[i] * s_hat[i * n_out + j];
+        ModParams m;
+        m.q = s_q[j];
+        m.r_lo = s_rlo[j];
+        m.r_hi = s_rhi[j];
+        o[(size_t)j << logN] = reduce128(acc, m);
+    }
+}
+
 // FP64 form: y_i = [x_i * qhat_i^-1]_{q_i} in [0, q_i) (x_i, qhat_i^-1 < q_i, so
 // the centred product lies in (-q_i, q_i) and one conditional add makes it
 // canonical; y_i < q_i < 4 p_j because max(q)/min(q) < 4), each term
@@ -223,7 +271,18 @@ void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_fir
         max_nout <= FBC_F64_MAX_PAIRS / FBC_F64_MAX_IN)
         hipLaunchKernelGGL(k_fbc_f64, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in,
                            in_zstride, out, out_pstride, out_zstride, t.logN);
-    else
+    else if (max_nin * max_nout <= FBC_I_MAX_PAIRS && max_nout <= FBC_I_MAX_OUT && max_nin <= 32) {
+#define FHEA_FBC_I(NI)                                                                                      \
+    hipLaunchKernelGGL(k_fbc_i<NI>, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in, in_zstride, out, \
+                       out_pstride, out_zstride, t.logN)
+        if (max_nin <= 8)
+            FHEA_FBC_I(8);
+        else if (max_nin <= 16)
+            FHEA_FBC_I(16);
+        else
+            FHEA_FBC_I(32);
+#undef FHEA_FBC_I
+    } else
         hipLaunchKernelGGL(k_fbc, grid, dim3(TPB), 0, s, descs_dev, in_first_dev, t.mp, in, in_zstride,
                            out, out_pstride, out_zstride, t.logN);
 }

@@ -28,6 +28,12 @@ struct DevTables {
     const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
     const double *itwf = nullptr;   // [T][N][2]
     const double *ninvf = nullptr;  // [T][2]
+    // mixed chain (some primes >= 2^50, others below): the integer kernels run
+    // everything but the transforms of limbs whose prime is below 2^50, which go
+    // to the FP64 transform (its tables are built for such chains too);
+    // pmask_small / pmask_big: bit p of [p / 64] set for prime p below / not below 2^50
+    bool ntt_mixed = false;
+    const u64 *pmask_small = nullptr, *pmask_big = nullptr;
 };
 
 struct RescaleOut;
@@ -58,6 +64,9 @@ struct NttFuse {
     // optional list of the limb slots to transform (instead of all `limbs` slots)
     const u32 *slots = nullptr;
     u32 nslots = 0;
+    // optional prime bitmask: only limbs whose prime index p has bit p set
+    // (DevTables::pmask_small on a mixed chain)
+    const u64 *pmask = nullptr;
     // two-pass transforms (N > 4096): 0 both passes; 1 only the first (its output,
     // FP64 intermediates, left at dst); 2 only the second (reading such intermediates
     // at src)

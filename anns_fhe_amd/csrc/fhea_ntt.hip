@@ -120,7 +120,7 @@ __global__ __launch_bounds__(I_TPB) void k_ntt64i(const u64 *src, u64 *dst, u64 
                                                   const u64 *__restrict__ twtab, const u64 *__restrict__ ninv,
                                                   const ModParams *__restrict__ mp, u32 logN, u32 logT,
                                                   int out_canon, int scale, u32 tiles_per_limb, u32 n_tiles,
-                                                  u32 npolys) {
+                                                  u32 npolys, const u64 *__restrict__ pmask) {
     __shared__ u64 lds[I_TPB * 16 + I_TPB];  // one tile, padded
     constexpr int NR = (LOGR + 3) / 4;
     using R0 = Round<INV, LOGR, 0>;
@@ -165,11 +165,11 @@ __global__ __launch_bounds__(I_TPB) void k_ntt64i(const u64 *src, u64 *dst, u64 
         }
     };
 
-    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, nullptr);
+    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, nullptr, pmask);
     u64 raw[16];
     if (cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
-        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, nullptr);
+        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, nullptr, pmask);
         geom(cur, g);
         g.q = mp[cur.pidx].q;
         g.tw = reinterpret_cast<const ulonglong2 *>(twtab) + ((size_t)cur.pidx << logN);
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(I_TPB) void k_ntt64i(const u64 *src, u64 *dst, u64 
 template <bool INV, bool COLS>
 static void pass_i(const DevTables &t, int logR, u32 tiles_per_limb, u32 npolys, u32 limbs, hipStream_t s,
                    const u64 *src, u64 *dst, u64 ss, u64 ds, u64 sl, u64 dl, const u32 *pm, u32 logT, int oc,
-                   int sc) {
+                   int sc, const u64 *pmask) {
     const u64 *tw = INV ? t.itw : t.tw;
     const u32 ntiles = tiles_per_limb * npolys * limbs;
     // persistent grid: the workgroups that can be resident at once (at most 4 per CU)
@@ -247,7 +247,7 @@ static void pass_i(const DevTables &t, int logR, u32 tiles_per_limb, u32 npolys,
         const u32 cap = t.max_wg / 4 * use_cu;                                                              \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                        \
         hipLaunchKernelGGL((k_ntt64i<INV, LR, COLS>), dim3(nwg), dim3(I_TPB), 0, s, src, dst, ss, ds, sl, dl, \
-                           pm, tw, t.ninv, t.mp, t.logN, logT, oc, sc, tiles_per_limb, ntiles, npolys);     \
+                           pm, tw, t.ninv, t.mp, t.logN, logT, oc, sc, tiles_per_limb, ntiles, npolys, pmask); \
         break;                                                                                              \
     }
     switch (logR) {
@@ -275,14 +275,23 @@ void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 
                        NttFuse(), s);
         return;
     }
+    // mixed chain: limbs of primes below 2^50 on the FP64 transform, the rest here
+    // (disjoint limbs, so the two launches do not touch each other's words)
+    const u64 *pmask = nullptr;
+    if (t.ntt_mixed && !(t.f64_off & 1)) {
+        NttFuse fz;
+        fz.pmask = t.pmask_small;
+        launch_ntt_f64(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map, fz, s);
+        pmask = t.pmask_big;
+    }
     if (lg <= I_LOG_TILE) {
         // the whole limb is one tile
         if (inverse)
             pass_i<true, false>(t, lg, 1, npolys, limbs, s, src, dst, src_stride, dst_stride, sl, dl, prime_map, lg,
-                                1, 1);
+                                1, 1, pmask);
         else
             pass_i<false, false>(t, lg, 1, npolys, limbs, s, src, dst, src_stride, dst_stride, sl, dl, prime_map,
-                                 lg, 1, 0);
+                                 lg, 1, 0, pmask);
         return;
     }
     // column stages k1 as tuned (DevTables::k1_fwd / k1_inv) when both passes fit
@@ -293,14 +302,14 @@ void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 
     if (!inverse) {
         // stages 0..k1-1 over columns (stride 2^k2), then k1..lg-1 over rows
         pass_i<false, true>(t, k1, tpl, npolys, limbs, s, src, dst, src_stride, dst_stride, sl, dl, prime_map,
-                            I_LOG_TILE, 0, 0);
+                            I_LOG_TILE, 0, 0, pmask);
         pass_i<false, false>(t, k2, tpl, npolys, limbs, s, dst, dst, dst_stride, dst_stride, dl, dl, prime_map,
-                             I_LOG_TILE, 1, 0);
+                             I_LOG_TILE, 1, 0, pmask);
     } else {
         pass_i<true, false>(t, k2, tpl, npolys, limbs, s, src, dst, src_stride, dst_stride, sl, dl, prime_map,
-                            I_LOG_TILE, 0, 0);
+                            I_LOG_TILE, 0, 0, pmask);
         pass_i<true, true>(t, k1, tpl, npolys, limbs, s, dst, dst, dst_stride, dst_stride, dl, dl, prime_map,
-                           I_LOG_TILE, 1, 1);
+                           I_LOG_TILE, 1, 1, pmask);
     }
 }
 

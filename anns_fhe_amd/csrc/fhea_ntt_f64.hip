@@ -274,11 +274,11 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
         }
     };
 
-    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
+    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
     u64 raw[16];
     if (PF && cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
-        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots);
+        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
         if (!PF) issue(cur, raw);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
@@ -469,6 +469,7 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
     NttFuse none;
     none.slots = fz.slots;
     none.nslots = fz.nslots;
+    none.pmask = fz.pmask;
     const u32 ylimbs = fz.slots ? fz.nslots : limbs;
     if (lg <= F_LOG_TILE) {
         dim3 grid(1, npolys * ylimbs);

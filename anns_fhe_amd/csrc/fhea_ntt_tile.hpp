@@ -32,7 +32,9 @@ __device__ __forceinline__ u32 tile_gidx(u32 e, u32 logR, u32 logS, u32 logC, u3
 
 // Work item = one 4096-word tile (tiles_per_limb per limb slot, n_tiles in all).
 // The kernels are persistent: workgroup w takes tiles w, w + G, w + 2G, ...
-// Tiles of skipped slots (prime map 0xFFFFFFFF) are passed over.
+// Tiles of skipped slots (prime map 0xFFFFFFFF) are passed over, and so are the
+// tiles of primes outside pmask when one is given (bit p of pmask[p / 64]: a
+// chain whose transforms run partly on the FP64 and partly on the integer kernel).
 struct TileId {
     u32 tile, poly, slot, pidx, tI;
 };
@@ -41,14 +43,14 @@ struct TileId {
 // varying fastest after the tile: the same prime's limbs of a whole batch are
 // in flight together and share its twiddle table in L2.
 __device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 npolys,
-                                            const u32 *prime_map, const u32 *slots) {
+                                            const u32 *prime_map, const u32 *slots, const u64 *pmask) {
     TileId id{n_tiles, 0, 0, 0, 0};
     for (; t < n_tiles; t += gridDim.x) {
         const u32 yl = t / tiles_per_limb;
         const u32 k = yl / npolys;
         const u32 slot = slots ? slots[k] : k;
         const u32 p = prime_map[slot];
-        if (p != 0xFFFFFFFFu) {
+        if (p != 0xFFFFFFFFu && (!pmask || ((pmask[p >> 6] >> (p & 63)) & 1))) {
             id.tile = t;
             id.poly = yl % npolys;
             id.slot = slot;

@@ -72,6 +72,36 @@ def test_ntt_intt_parity(orc, N, L, q_bits):
     assert np.array_equal(ctx.read_poly(2 * W, W).reshape(L, N), o.intt_poly(a))
 
 
+@pytest.mark.parametrize("N", [1024, 4096, 32768])
+def test_ntt_mixed_chain_parity(orc, N):
+    """A chain with primes on both sides of 2^50 (60-bit q0 and P, 45-bit q1..):
+    the transforms of the 45-bit limbs run on the FP64 kernel and those of the
+    60-bit limbs on the integer kernel, in launches over the same polynomials.
+    Every limb matches the oracle; in-place INTT restores the input."""
+    from anns_fhe_amd.driver import FheContext
+    L = 4
+    big = orc.gen_primes(2 * N, 60, 2)
+    q, p = [big[0]] + orc.gen_primes(2 * N, 45, L - 1), big[1:]
+    W = L * N
+    ctx = FheContext(N, L, 1, 1, q_primes=q, p_primes=p, poly_words=4 * W, key_words=1 << 12)
+    try:
+        o = orc.OracleCtx(N, q, p, 1)
+        rng = np.random.default_rng(N + 5)
+        a = rand_poly(rng, q, N)
+        for i in range(L):
+            a[i, :2] = [q[i] - 1, 0]
+        say("mixed ntt", N)
+        ctx.write_poly(0, a)
+        assert ctx.op(1, 0, 0, W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(W, W).reshape(L, N), o.ntt_poly(a))
+        assert ctx.op(2, W, 0, W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(W, W).reshape(L, N), a)
+        assert ctx.op(2, 0, 0, 2 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(2 * W, W).reshape(L, N), o.intt_poly(a))
+    finally:
+        ctx.close()
+
+
 def test_elementwise_parity(orc):
     N, L = 8192, 4
     ctx = get_ctx(N, L, 1, 2)

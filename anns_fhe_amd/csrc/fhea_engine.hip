@@ -564,8 +564,12 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
                         c->h.dnum, shared, s);
     }
     // the ModUp coefficients are dead now: their region holds the rescale's lift
-    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, fused);
-    return ks_moddown(c, l, nb, acc, pb, tt, comb, fused);
+    // the P slots' sums are stored like the Q slots' and the ModDown INTT runs whole
+    // in its own launch; FHEA_F64_OFF bit 128 runs its first pass inside the inner
+    // product instead (one wave per SIMD there: measured 2.6% slower per HMult)
+    const bool pb_mid = fused && (t.f64_off & 128);
+    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid);
+    return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
 }
 
 // ModDown of nb accumulator pairs acc_z = [2][l+K][N] (key_switch.cpp:385-427) and the

@@ -18,7 +18,8 @@ struct DevTables {
     // bitmask) turns single components back to the integer kernels:
     // 1 transforms, 2 basis conversion, 4 inner product, 8 tensor,
     // 16 fused transform ends, 32 rescale / ModDown combines, 64 the inner
-    // product fused into the transforms around it
+    // product fused into the transforms around it; 128 is a switch ON: with the
+    // fused inner product, the P slots also run the ModDown INTT's first pass there
     bool use_f64 = false;
     u32 f64_off = 0;
     u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
@@ -77,17 +78,18 @@ struct NttFuse {
 // two-pass N): after the ModUp's first forward pass (only_pass = 1 on the digits),
 // work item (tile, item z, slot i) runs the second forward pass of every converted
 // digit limb of slot i in registers, adds the digit's own limb where it has one,
-// multiplies by the key and sums over the digits (key_switch.cpp:309-381).  Q slots
-// (i < l) store acc0/acc1 as canonical words; P slots run the first inverse pass of
-// the ModDown INTT (key_switch.cpp:385-400) on the sums and store its FP64
-// intermediates in pb, so only the second inverse pass (only_pass = 2) remains.
+// multiplies by the key and sums over the digits (key_switch.cpp:309-381), storing
+// acc0/acc1 as canonical words for every QlP slot.  With FHEA_F64_OFF bit 128 the P
+// slots (i >= l) instead run the first inverse pass of the ModDown INTT
+// (key_switch.cpp:385-400) on their sums and store its FP64 intermediates in pb, so
+// only the second inverse pass (only_pass = 2) remains.
 struct KsRowsArgs {
     const u64 *compact;  // ModUp digits after the first forward pass: item z at + z*cz,
     u64 cz;              // digit p, converted position pos at + (p*(l+K) + pos)*N
     const KsIpArgs *args;  // per item: c (own limbs, eval domain), key base
-    u64 *acc;            // Q slots: acc0 at acc + z*az + i*N, acc1 at + (l+K)*N
+    u64 *acc;            // acc0 of slot i at acc + z*az + i*N, acc1 at + (l+K)*N
     u64 az;
-    u64 *pb;             // P slots: intermediates of poly 2z+s, limb j = i-l at pb + (2z+s)*K*N + j*N
+    u64 *pb;             // bit 128, P slots: intermediates of poly 2z+s, limb j = i-l at pb + (2z+s)*K*N + j*N
     u32 l, K, L, alpha, nparts, key_limbs, dnum, nb;
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits

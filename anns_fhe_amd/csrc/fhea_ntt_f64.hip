@@ -763,6 +763,10 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s) 
     // the Q-slot kernel does not use LOGRI: one instance per forward split
 #define FHEA_KS_ROWS(LF, LI)                                                  \
     if (lf == LF && li == LI) {                                               \
+        if (!(t.f64_off & 128)) {                                             \
+            launch_ks_rows_inst<LF, 8, false>(t, a, 0, a.l + a.K, s);         \
+            return;                                                           \
+        }                                                                     \
         launch_ks_rows_inst<LF, 8, false>(t, a, 0, a.l, s);                   \
         launch_ks_rows_inst<LF, LI, true>(t, a, a.l, a.K, s);                 \
         return;                                                               \

@@ -179,3 +179,52 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
         assert np.array_equal(ctx.read_poly(12 * W, W1), pm[12 * W:12 * W + W1])
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("switch", [0, 128, 64, 1 | 2 | 4 | 8 | 16 | 32])
+def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
+    """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
+    P slots' ModDown INTT pass inside the fused inner product; 64: the inner product
+    as its own kernel; 1..32: each component on the integer kernels).  Every setting
+    gives the oracle's words for a batched HMult (3 items, one lane and two) and a
+    rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
+    from anns_fhe_amd import _lib
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    monkeypatch.setenv("FHEA_F64_OFF", str(switch))
+    N, L, K, dnum = 16384, 8, 2, 3
+    W, W1 = L * N, (L - 1) * N
+    ksz = 2 * dnum * (L + K) * N
+    ctx = FheContext(N, L, K, dnum, poly_words=24 * W, key_words=2 * ksz + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
+        rng = np.random.default_rng(300 + switch)
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g1 = ctx.galois_for_step(1)
+        drv.add_rotation_key(1, g1, ksz, L)
+        B = 3
+        data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4 * B)])
+        ctx.write_poly(0, data)
+        pm = np.zeros(24 * W, np.uint64)
+        pm[:4 * B * W] = data
+        works = [dict(kind=1, limbs=L, a0=4 * z * W, a1=(4 * z + 1) * W, b0=(4 * z + 2) * W,
+                      b1=(4 * z + 3) * W, d0=(12 + 2 * z) * W, d1=(13 + 2 * z) * W) for z in range(B)]
+        for streams in (1, 2):
+            ctx.set_tuning(_lib.TUNE_STREAMS, streams)
+            say("switch", switch, "streams", streams)
+            ctx.submit(works)
+            for z in range(B):
+                assert o.eval_mult_relin_rescale(pm, km, 0, L, 4 * z * W, (4 * z + 1) * W, (4 * z + 2) * W,
+                                                 (4 * z + 3) * W, (12 + 2 * z) * W, (13 + 2 * z) * W, L) == 0
+                for d in (12 + 2 * z, 13 + 2 * z):
+                    assert np.array_equal(ctx.read_poly(d * W, W1), pm[d * W:d * W + W1]), (switch, streams, z)
+        say("switch", switch, "rotate")
+        drv.eval_rotate(0, W, 20 * W, 21 * W, 1, L)
+        assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 0, W, 20 * W, 21 * W, 1, L) == 0
+        assert np.array_equal(ctx.read_poly(20 * W, W), pm[20 * W:21 * W])
+        assert np.array_equal(ctx.read_poly(21 * W, W), pm[21 * W:22 * W])
+    finally:
+        ctx.close()

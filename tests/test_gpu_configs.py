@@ -109,6 +109,28 @@ def test_batched_keyswitch_64_cts_distinct_keys(orc):
             km = blobs[z % 8]
             assert o.eval_automorphism_ks(pm, km, gal[z], 0, L, 0, W, 2 * W, 3 * W, L) == 0
             assert np.array_equal(batched[2 * z * W:2 * (z + 1) * W], pm[2 * W:4 * W])
+        # the same 64 rotations recorded as one HIP graph (as the C3 bench runs them),
+        # replayed on new ciphertexts with eager work in between: every replay equals
+        # the eager result, and one item of the last replay equals the oracle's
+        say("graph capture")
+        ctx.graph_begin()
+        ctx.submit(works)
+        gid = ctx.graph_end()
+        for rep in range(3):
+            cts = _uniform(rng, ctx.q, (n_ct, 2))
+            ctx.write_poly(0, cts)
+            ctx.graph_launch(gid)
+            ctx.sync()
+            replay = ctx.read_poly(out_b, 2 * n_ct * W)
+            ctx.submit(works)
+            say("graph replay", rep)
+            assert np.array_equal(replay, ctx.read_poly(out_b, 2 * n_ct * W)), rep
+        z = 17
+        pm = np.zeros(4 * W, np.uint64)
+        pm[:2 * W] = cts[z].reshape(-1)
+        assert o.eval_automorphism_ks(pm, blobs[z % 8], gal[z], 0, L, 0, W, 2 * W, 3 * W, L) == 0
+        assert np.array_equal(replay[2 * z * W:2 * (z + 1) * W], pm[2 * W:4 * W])
+        ctx.graph_destroy(gid)
     finally:
         ctx.close()
 

@@ -135,15 +135,22 @@ def test_batched_keyswitch_64_cts_distinct_keys(orc):
         ctx.close()
 
 
-def test_full_size_ckks_semantics_on_gpu():
+@pytest.mark.parametrize("chain", ["q50", "ref"])
+def test_full_size_ckks_semantics_on_gpu(orc, chain):
     """Client-encrypted vectors at N=2^16, L=24, K=8, dnum=3 through the GPU:
-    Dec(HMult(Enc(x), Enc(y))) = x*y and Dec(rotate(Enc(x), r)) = x shifted by r."""
+    Dec(HMult(Enc(x), Enc(y))) = x*y and Dec(rotate(Enc(x), r)) = x shifted by r.
+    q50: 50-bit primes (FP64 kernels); ref: the reference's chain shape, 60-bit q0
+    and P with 45-bit q1..q23 (integer kernels, FP64 transforms on 45-bit limbs)."""
     from anns_fhe_amd.client import CkksClient
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
     W = L * N
     ksz = 2 * DNUM * (L + K) * N
     steps = [1, 2, 4, -1]
-    ctx = FheContext(N, L, K, DNUM, q_bits=50, poly_words=8 * W + 8, key_words=(1 + len(steps)) * ksz + 8)
+    primes = dict(q_bits=50)
+    if chain == "ref":
+        big = orc.gen_primes(2 * N, 60, K + 1)
+        primes = dict(q_primes=[big[0]] + orc.gen_primes(2 * N, 45, L - 1), p_primes=big[1:])
+    ctx = FheContext(N, L, K, DNUM, poly_words=8 * W + 8, key_words=(1 + len(steps)) * ksz + 8, **primes)
     try:
         cl = CkksClient(0, 0, 0, 0, ctx=ctx)
         assert cl.primes == ctx.primes
@@ -161,7 +168,7 @@ def test_full_size_ckks_semantics_on_gpu():
         n = N // 2
         x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
         scale = float(ctx.q[L - 1])
-        say("encrypt")
+        say(chain, "encrypt")
         ctx.write_poly(0, cl.encrypt_values(pk, x, L, scale, 30))
         ctx.write_poly(2 * W, cl.encrypt_values(pk, y, L, scale, 31))
         drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 4 * W, 5 * W, L)

@@ -19,13 +19,20 @@ def rand_poly(rng, primes, N):
     return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
 
 
-def test_noalloc_equals_allocating_entry_point(orc):
+@pytest.mark.parametrize("chain", ["q50", "q60_45"])
+def test_noalloc_equals_allocating_entry_point(orc, chain):
+    """q50: every prime below 2^50 (FP64 kernels); q60_45: a 60-bit q0 and P with
+    45-bit q1.. (integer kernels, FP64 transforms for the 45-bit limbs)."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd.driver import FheContext
     lib = _lib.load()
     N, L, K, dnum = 8192, 4, 1, 2
     W = L * N
-    ctx = FheContext(N, L, K, dnum, poly_words=16 * W, key_words=2 * dnum * (L + K) * N + 8)
+    primes = {}
+    if chain == "q60_45":
+        big = orc.gen_primes(2 * N, 60, K + 1)
+        primes = dict(q_primes=[big[0]] + orc.gen_primes(2 * N, 45, L - 1), p_primes=big[1:])
+    ctx = FheContext(N, L, K, dnum, poly_words=16 * W, key_words=2 * dnum * (L + K) * N + 8, **primes)
     s = ctx.keygen_secret(5)
     ctx.keygen_relin_key(s, L, 6, 0)
     assert lib.fhea_set_relin_key(ctx.h, 0, L) == 0
@@ -33,7 +40,7 @@ def test_noalloc_equals_allocating_entry_point(orc):
     data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
     ctx.write_poly(0, data)
     for limbs in (4, 3):  # second call: a smaller input (one level down)
-        say("level", limbs)
+        say(chain, "level", limbs)
         W1 = (limbs - 1) * N
         args = (0, W, 2 * W, 3 * W)  # the first `limbs` limbs of each input polynomial
         assert lib.fhea_eval_mult_relin_rescale(ctx.h, *args, 4 * W, 5 * W, limbs) == 0

@@ -371,8 +371,10 @@ static int build_device_tables(fhea_ctx *c) {
                     }
                 md[0].in_cstf = off_ptr<const double>(b.addv(f64_pairs(cst, pm)));
                 md[0].hatf = off_ptr<const double>(b.addv(hf));
-                lo[l].pinvf = b.addv(f64_pairs(pinv, qm));
             }
+            // (value, value/q) forms for the FP64 fused transform ends (on a mixed
+            // chain used for the limbs whose prime is below 2^50 only)
+            if (use_f64 || mixed) lo[l].pinvf = b.addv(f64_pairs(pinv, qm));
             lo[l].moddown_desc = b.add_descs(md);
             lo[l].pinv = b.addv(pinv);
         }
@@ -396,7 +398,7 @@ static int build_device_tables(fhea_ctx *c) {
                 pq[2 * i] = mulm(inv_mod(pr, qi), inv, qi);
             }
             std::vector<u32> last = {l - 1, l - 1};
-            if (use_f64) {
+            if (use_f64 || mixed) {
                 std::vector<u64> qm(n + 1, 1);
                 for (u32 i = 0; i < n; ++i) qm[i] = h.primes[i];
                 lo[l].rs_qlinvf = b.addv(f64_pairs(qlinv, qm));
@@ -436,6 +438,14 @@ static int build_device_tables(fhea_ctx *c) {
     }
     c->dt.f64_off = c->f64_off;
     c->dt.ntt_mixed = mixed && !c->force_int;
+    {
+        u32 qb = 0;
+        while (qb < L && h.primes[qb] >= (1ull << 50)) ++qb;
+        bool rest_small = true;
+        for (u32 i = qb; i < L; ++i) rest_small &= h.primes[i] < (1ull << 50);
+        c->dt.q_big_prefix = qb;
+        c->dt.q_mixed_fusable = c->dt.ntt_mixed && rest_small;
+    }
     c->dt.pmask_small = (const u64 *)P(o_mask_small);
     c->dt.pmask_big = (const u64 *)P(o_mask_big);
     if (use_f64 || mixed) {
@@ -461,7 +471,7 @@ static int build_device_tables(fhea_ctx *c) {
         lt.rs_qlql = (const u64 *)P(lo[l].rs_qlql);
         c->rs_sub[l] = (const u64 *)P(lo[l].rs_sub);
         c->rs_last_map[l] = (const u32 *)P(lo[l].rs_last);
-        if (use_f64) {
+        if (use_f64 || mixed) {
             lt.pinvf = (const double *)P(lo[l].pinvf);
             lt.rs_qlinvf = (const double *)P(lo[l].rs_qlinvf);
             lt.rs_qlqlf = (const double *)P(lo[l].rs_qlqlf);
@@ -494,6 +504,10 @@ struct RescaleAfter {
 };
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
                               const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid);
+
+// Mixed chain whose Q primes past a leading run of >= 2^50 primes are all below
+// 2^50: the ModDown / rescale transforms take the FP64 fused ends on those limbs.
+static bool mixed_fused(const DevTables &t) { return t.q_mixed_fusable && !(t.f64_off & 17); }
 
 // The ModDown's INTT of the P part of nb accumulator pairs into pb: the whole
 // transform from acc, or (pb_mid) only its second pass over the intermediates the
@@ -593,6 +607,25 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
         fz.acc = acc;
         fz.acc_stride = (l + K) * N;
         launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, fz, s);
+    } else if (mixed_fused(t)) {
+        // mixed chain: the fused FP64 transform on the limbs below 2^50, the integer
+        // transform and combine on the leading limbs of primes >= 2^50
+        const u32 qb = std::min(t.q_big_prefix, l);
+        if (qb < l) {
+            NttFuse fz;
+            fz.epi = FUSE_MODDOWN_OUT;
+            fz.cst_out = lt.pinvf;
+            fz.cb = cb_d;
+            fz.acc = acc;
+            fz.acc_stride = (l + K) * N;
+            fz.pmask = t.pmask_small;
+            launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, fz, s);
+        }
+        if (qb) {
+            launch_ntt_int(t, false, tt, tt, l * N, l * N, 2 * nb, qb, t.ident, s);
+            launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
+                                   lt.pinvf, nb, qb, s);
+        }
     } else {
         launch_ntt(t, false, tt, tt, l * N, l * N, 2 * nb, l, t.ident, s);
         launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv,
@@ -631,6 +664,28 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
         launch_ntt_f64(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, N, N, 2 * nb, l - 1, t.ident, fz, s);
         return FHEA_OK;
     }
+    if (mixed_fused(t) && t.q_big_prefix < l - 1) {
+        // mixed chain, q_{l-1} below 2^50: the fused FP64 transform on the output limbs
+        // past the big-prime prefix, the integer route on the prefix
+        const u32 qb = t.q_big_prefix;
+        NttFuse fz;
+        fz.pro = FUSE_RESCALE_IN;
+        fz.v = v;
+        fz.v_stride = N;
+        fz.ql = c->h.primes[l - 1];
+        fz.cst_in = lt.rs_qlqlf;
+        fz.epi = FUSE_RESCALE_OUT;
+        fz.cst_out = lt.rs_qlinvf;
+        fz.ro = ro_d;
+        fz.pmask = t.pmask_small;
+        launch_ntt_f64(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, N, N, 2 * nb, l - 1, t.ident, fz, s);
+        if (qb) {
+            launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s, qb);
+            launch_ntt_int(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, qb, t.ident, s);
+            launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, lt.rs_qlinvf, 2 * nb, l, s, qb);
+        }
+        return FHEA_OK;
+    }
     launch_rescale_prep(t, v, N, tmp, (l - 1) * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s);
     launch_ntt(t, false, tmp, tmp, (l - 1) * N, (l - 1) * N, 2 * nb, l - 1, t.ident, s);
     launch_rescale_combine(t, ro_d, tmp, (l - 1) * N, lt.rs_qlinv, lt.rs_qlinvf, 2 * nb, l, s);
@@ -650,7 +705,11 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
     hipStream_t s = c->stream;
-    if (!t.use_f64 || (t.f64_off & 17) || l < 2) {
+    // mixed chain with q_{l-1} below 2^50: as below on the limbs past the
+    // big-prime prefix (FP64 fused ends, prime bitmask), ModDown + rescale by the
+    // integer kernels on the prefix
+    const bool mix = !t.use_f64 && mixed_fused(t) && l >= 2 && t.q_big_prefix < l - 1;
+    if ((!t.use_f64 && !mix) || (t.f64_off & 17) || l < 2) {
         int st = ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
         if (st) return st;
         return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
@@ -668,6 +727,7 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f1.acc_stride = (l + K) * N;
     f1.slots = t.ident + (l - 1);
     f1.nslots = 1;
+    if (mix) f1.pmask = t.pmask_small;
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, f1, s);
     // its coefficients
     launch_ntt(t, true, ra.d_base + (l - 1) * N, v, ra.dz, 2 * N, nb, 2, c->rs_last_map[l], s, ra.ds, N);
@@ -688,7 +748,20 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f2.ro = ro_d;
     f2.acc = acc;
     f2.acc_stride = (l + K) * N;
+    if (mix) f2.pmask = t.pmask_small;
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l - 1, t.ident, f2, s);
+    if (mix && t.q_big_prefix) {
+        // the prefix: its ModDown + add in place in d, then its rescale; its converted
+        // limbs in tt are consumed by the combine, and tt (dead once the fused
+        // transform above has read it) holds the rescale's lifted words
+        const u32 qb = t.q_big_prefix;
+        launch_ntt_int(t, false, tt, tt, l * N, l * N, 2 * nb, qb, t.ident, s);
+        launch_moddown_combine(t, cb_d, acc, 2 * (l + K) * N, (l + K) * N, tt, 2 * l * N, l * N, lt.pinv, lt.pinvf,
+                               nb, qb, s);
+        launch_rescale_prep(t, v, N, tt, l * N, lt.rs_qlql, c->rs_sub[l], lt.rs_qlqlf, 2 * nb, l, s, qb);
+        launch_ntt_int(t, false, tt, tt, l * N, l * N, 2 * nb, qb, t.ident, s);
+        launch_rescale_combine(t, ro_d, tt, l * N, lt.rs_qlinv, lt.rs_qlinvf, 2 * nb, l, s, qb);
+    }
     return FHEA_OK;
 }
 

@@ -492,8 +492,8 @@ __global__ __launch_bounds__(TPB) void k_rescale_prep_f64(const u64 *__restrict_
 
 void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp, u64 tmp_stride,
                          const u64 *qlql, const u64 *sub, const double *qlqlf, u32 npolys, u32 l,
-                         hipStream_t s) {
-    dim3 grid(t.N / TPB, l - 1, npolys);
+                         hipStream_t s, u32 nlimbs) {
+    dim3 grid(t.N / TPB, nlimbs ? nlimbs : l - 1, npolys);
     if (t.use_f64 && !(t.f64_off & 32))
         hipLaunchKernelGGL(k_rescale_prep_f64, grid, dim3(TPB), 0, s, v, v_stride, tmp, tmp_stride,
                            qlqlf, t.mp, t.logN, l);
@@ -533,8 +533,8 @@ __global__ __launch_bounds__(TPB) void k_rescale_combine_f64(const RescaleOut *_
 
 void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
                             u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys, u32 l,
-                            hipStream_t s) {
-    dim3 grid(t.N / TPB, l - 1, npolys);
+                            hipStream_t s, u32 nlimbs) {
+    dim3 grid(t.N / TPB, nlimbs ? nlimbs : l - 1, npolys);
     if (t.use_f64 && !(t.f64_off & 32))
         hipLaunchKernelGGL(k_rescale_combine_f64, grid, dim3(TPB), 0, s, args_dev, tmp, tmp_stride,
                            qlinvf, t.mp, t.logN);

@@ -35,6 +35,11 @@ struct DevTables {
     // pmask_small / pmask_big: bit p of [p / 64] set for prime p below / not below 2^50
     bool ntt_mixed = false;
     const u64 *pmask_small = nullptr, *pmask_big = nullptr;
+    // the first q_big_prefix Q primes are >= 2^50; q_mixed_fusable: every later Q
+    // prime is below 2^50, so the ModDown / rescale transforms of the limbs past
+    // the prefix take the FP64 fused ends and only the prefix the integer kernels
+    u32 q_big_prefix = 0;
+    bool q_mixed_fusable = false;
 };
 
 struct RescaleOut;
@@ -103,6 +108,10 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
 void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
                 u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
                 u64 src_slot = 0, u64 dst_slot = 0);
+// The same on the 64-bit integer kernel only, whatever the chain (on a mixed chain:
+// the limbs whose prime is >= 2^50; callers pass only such slots).
+void launch_ntt_int(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
+                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s);
 
 // FP64 transform (t.use_f64) with optional fused ends (forward only).
 void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
@@ -162,16 +171,17 @@ void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, con
                             u32 l, hipStream_t s);
 
 // Rescale (rescale.cpp:3-72) helpers.  v: INTT of the last limb, [z][2][N].
+// Level l, output limbs 0 .. nlimbs-1 (0 = all l-1).
 void launch_rescale_prep(const DevTables &t, const u64 *v, u64 v_stride, u64 *tmp,
                          u64 tmp_stride, const u64 *qlql, const u64 *sub, const double *qlqlf,
-                         u32 npolys, u32 l, hipStream_t s);
+                         u32 npolys, u32 l, hipStream_t s, u32 nlimbs = 0);
 struct RescaleOut {
     const u64 *a;  // input polynomial (l limbs)
     u64 *out;      // output polynomial (l-1 limbs)
 };
 void launch_rescale_combine(const DevTables &t, const RescaleOut *args_dev, const u64 *tmp,
                             u64 tmp_stride, const u64 *qlinv, const double *qlinvf, u32 npolys,
-                            u32 l, hipStream_t s);
+                            u32 l, hipStream_t s, u32 nlimbs = 0);
 
 // Hoisted rotations: the digits of one input (compact, eval domain, ModUp done once)
 // read through item z's Galois permutation, times item z's key.

@@ -264,26 +264,11 @@ static void pass_i(const DevTables &t, int logR, u32 tiles_per_limb, u32 npolys,
 #undef FHEA_NTTI_CASE
 }
 
-void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
-                u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
-                u64 src_slot, u64 dst_slot) {
-    if (npolys == 0 || limbs == 0) return;
+// the integer transform of `limbs` slots; pmask restricts it to a prime subset
+static void ntt_int(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride, u64 dst_stride,
+                    u64 sl, u64 dl, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
+                    const u64 *pmask) {
     const u32 lg = t.logN;
-    const u64 sl = src_slot ? src_slot : t.N, dl = dst_slot ? dst_slot : t.N;
-    if (t.use_f64 && !(t.f64_off & 1)) {
-        launch_ntt_f64(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map,
-                       NttFuse(), s);
-        return;
-    }
-    // mixed chain: limbs of primes below 2^50 on the FP64 transform, the rest here
-    // (disjoint limbs, so the two launches do not touch each other's words)
-    const u64 *pmask = nullptr;
-    if (t.ntt_mixed && !(t.f64_off & 1)) {
-        NttFuse fz;
-        fz.pmask = t.pmask_small;
-        launch_ntt_f64(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map, fz, s);
-        pmask = t.pmask_big;
-    }
     if (lg <= I_LOG_TILE) {
         // the whole limb is one tile
         if (inverse)
@@ -311,6 +296,34 @@ void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 
         pass_i<true, true>(t, k1, tpl, npolys, limbs, s, dst, dst, dst_stride, dst_stride, dl, dl, prime_map,
                            I_LOG_TILE, 1, 1, pmask);
     }
+}
+
+void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
+                u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
+                u64 src_slot, u64 dst_slot) {
+    if (npolys == 0 || limbs == 0) return;
+    const u64 sl = src_slot ? src_slot : t.N, dl = dst_slot ? dst_slot : t.N;
+    if (t.use_f64 && !(t.f64_off & 1)) {
+        launch_ntt_f64(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map,
+                       NttFuse(), s);
+        return;
+    }
+    // mixed chain: limbs of primes below 2^50 on the FP64 transform, the rest here
+    // (disjoint limbs, so the two launches do not touch each other's words)
+    const u64 *pmask = nullptr;
+    if (t.ntt_mixed && !(t.f64_off & 1)) {
+        NttFuse fz;
+        fz.pmask = t.pmask_small;
+        launch_ntt_f64(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map, fz, s);
+        pmask = t.pmask_big;
+    }
+    ntt_int(t, inverse, src, dst, src_stride, dst_stride, sl, dl, npolys, limbs, prime_map, s, pmask);
+}
+
+void launch_ntt_int(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
+                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s) {
+    if (npolys == 0 || limbs == 0) return;
+    ntt_int(t, inverse, src, dst, src_stride, dst_stride, t.N, t.N, npolys, limbs, prime_map, s, nullptr);
 }
 
 }  // namespace fhea

@@ -228,3 +228,60 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
         assert np.array_equal(ctx.read_poly(21 * W, W), pm[21 * W:22 * W])
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("shape", ["p60", "q2big"])
+def test_mixed_chain_levels_vs_oracle(orc, shape):
+    """Chains with primes on both sides of 2^50 take the FP64 fused ModDown / rescale
+    transforms on the limbs past a leading run of >= 2^50 Q primes and the integer
+    kernels on that run.  p60: every Q prime 45-bit, P 60-bit (no run); q2big: q0, q1
+    of 60 bits, then 45-bit primes (a run of two, so at levels 3 and 2 every output
+    limb of the rescale is in the run and the integer path takes over).  At every
+    level: OP 7, OP 8, eval_mult_relin_rescale and a rotation = oracle."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    N, L, K, dnum = 8192, 6, 2, 3
+    if shape == "p60":
+        big = orc.gen_primes(2 * N, 60, K)
+        q, p = orc.gen_primes(2 * N, 45, L), big
+    else:
+        big = orc.gen_primes(2 * N, 60, K + 2)
+        q, p = big[:2] + orc.gen_primes(2 * N, 45, L - 2), big[2:]
+    W = L * N
+    ksz = 2 * dnum * (L + K) * N
+    ctx = FheContext(N, L, K, dnum, q_primes=q, p_primes=p, poly_words=16 * W, key_words=2 * ksz + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, q, p, dnum)
+        rng = np.random.default_rng(len(shape) + 77)
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g1 = ctx.galois_for_step(1)
+        drv.add_rotation_key(1, g1, ksz, L)
+        data = np.concatenate([rand_poly(rng, q, N).reshape(-1) for _ in range(4)])
+        for i in range(L):
+            data[i * N] = q[i] - 1
+        ctx.write_poly(0, data)
+        pm = np.zeros(16 * W, np.uint64)
+        pm[:4 * W] = data
+        for l in (6, 4, 3, 2):
+            l1 = (l - 1) * N
+            say("mixed", shape, "level", l)
+            assert ctx.op(7, W, 0, 5 * W, 0, l, L) == 0
+            assert o.accel_op(pm, km, 7, W, 0, 5 * W, 0, l, L) == 0
+            assert np.array_equal(ctx.read_poly(5 * W, l * N), pm[5 * W:5 * W + l * N]), ("op7", l)
+            assert np.array_equal(ctx.read_poly(5 * W + l * N, l * N), pm[5 * W + l * N:5 * W + 2 * l * N]), ("op7", l)
+            assert ctx.op(8, 0, W, 7 * W, 0, l) == 0
+            assert o.accel_op(pm, km, 8, 0, W, 7 * W, 0, l) == 0
+            assert np.array_equal(ctx.read_poly(7 * W, 2 * l1), pm[7 * W:7 * W + 2 * l1]), ("op8", l)
+            drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 9 * W, 10 * W, l)
+            assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 9 * W, 10 * W, l) == 0
+            assert np.array_equal(ctx.read_poly(9 * W, l1), pm[9 * W:9 * W + l1]), ("hmult", l)
+            assert np.array_equal(ctx.read_poly(10 * W, l1), pm[10 * W:10 * W + l1]), ("hmult", l)
+            drv.eval_rotate(0, W, 11 * W, 12 * W, 1, l)
+            assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 0, W, 11 * W, 12 * W, 1, l) == 0
+            assert np.array_equal(ctx.read_poly(11 * W, l * N), pm[11 * W:11 * W + l * N]), ("rot", l)
+            assert np.array_equal(ctx.read_poly(12 * W, l * N), pm[12 * W:12 * W + l * N]), ("rot", l)
+    finally:
+        ctx.close()

@@ -562,6 +562,28 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         const KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
                             key_limbs, c->h.dnum, nb};
         launch_ks_rows_f64(t, ka, s);
+    } else if (!t.use_f64 && mixed_fused(t) && !(t.f64_off & (4 | 64)) && np <= 8 && ks_rows_fusable(t) &&
+               t.q_big_prefix < l) {
+        // mixed chain: the converted limbs of primes below 2^50 take the FP64 first
+        // pass and the fused second pass + inner product on Q slots qb .. l-1; those of
+        // primes >= 2^50 the whole integer transform and the integer inner product on
+        // slots 0 .. qb-1 and the P slots
+        const u32 qb = t.q_big_prefix;
+        NttFuse fz;
+        fz.slots = lt.modup_slots;
+        fz.nslots = lt.n_modup_slots;
+        fz.only_pass = 1;
+        fz.pmask = t.pmask_small;
+        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+        launch_ntt_int(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s, t.pmask_big);
+        const KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
+                            key_limbs, c->h.dnum, nb};
+        launch_ks_rows_f64_q(t, ka, qb, l - qb, s);
+        if (qb)
+            launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs,
+                            c->h.dnum, false, s, 0, qb);
+        launch_ks_inner(t, ip_d, compact, cz, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum,
+                        false, s, l, (u32)K);
     } else {
         if (t.use_f64 && !(t.f64_off & 1)) {
             NttFuse fz;  // only the converted slots of each digit

@@ -299,8 +299,8 @@ __global__ __launch_bounds__(TPB) void k_ks_inner(const KsIpArgs *__restrict__ a
                                                   u64 *__restrict__ acc, u64 az,
                                                   const ModParams *__restrict__ mp, u32 logN, u32 l,
                                                   u32 K, u32 L, u32 alpha, u32 nparts,
-                                                  u32 key_limbs, u32 dnum) {
-    const u32 i = blockIdx.y;  // QlP slot
+                                                  u32 key_limbs, u32 dnum, u32 slot0) {
+    const u32 i = slot0 + blockIdx.y;  // QlP slot
     const u32 z = blockIdx.z;
     const u32 r = blockIdx.x * TPB + threadIdx.x;
     const KsIpArgs A = args[z];
@@ -384,7 +384,15 @@ __global__ __launch_bounds__(TPB) void k_ks_inner_f64(const KsIpArgs *__restrict
 
 void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
                      u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
-                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s) {
+                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s, u32 slot0,
+                     u32 nslots) {
+    if (nslots) {
+        // a slot range on the integer kernel (mixed chains: the slots of primes >= 2^50)
+        dim3 grid(t.N / TPB, nslots, nbatch);
+        hipLaunchKernelGGL(k_ks_inner, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride, acc,
+                           acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum, slot0);
+        return;
+    }
     if (t.use_f64 && !(t.f64_off & 4) && nparts <= KS_F64_MAX_PARTS) {
         const u32 gs = shared_key ? nbatch : 1;
         dim3 grid(t.N / TPB, l + t.K, (nbatch + gs - 1) / gs);
@@ -395,7 +403,7 @@ void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *co
     }
     dim3 grid(t.N / TPB, l + t.K, nbatch);
     hipLaunchKernelGGL(k_ks_inner, grid, dim3(TPB), 0, s, args_dev, compact, compact_zstride, acc,
-                       acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum);
+                       acc_zstride, t.mp, t.logN, l, t.K, t.L, alpha, nparts, key_limbs, dnum, 0u);
 }
 
 // ---------------------------------------------------------------------------

@@ -100,6 +100,9 @@ struct KsRowsArgs {
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);
 void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
+// the same over Q slots slot0 .. slot0+nslots-1 only (mixed chains: the Q slots
+// whose primes are below 2^50)
+void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s);
 
 // Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
 // has limb slot s at src + z*src_stride + s*src_slot (src_slot 0 = N); slot s
@@ -111,7 +114,8 @@ void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 
 // The same on the 64-bit integer kernel only, whatever the chain (on a mixed chain:
 // the limbs whose prime is >= 2^50; callers pass only such slots).
 void launch_ntt_int(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
-                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s);
+                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
+                    const u64 *pmask = nullptr);
 
 // FP64 transform (t.use_f64) with optional fused ends (forward only).
 void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
@@ -156,9 +160,11 @@ struct KsIpArgs {
     const u64 *key;    // evk base (a then b)
 };
 // shared_key: every item uses args_dev[0].key (read once per batch).
+// nslots > 0: only QlP slots slot0 .. slot0+nslots-1, on the integer kernel.
 void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *compact,
                      u64 compact_zstride, u64 *acc, u64 acc_zstride, u32 nbatch, u32 l, u32 alpha,
-                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s);
+                     u32 nparts, u32 key_limbs, u32 dnum, bool shared_key, hipStream_t s, u32 slot0 = 0,
+                     u32 nslots = 0);
 
 // ModDown combine (key_switch.cpp:253-273): out_i = (x_i - t_i) * P^-1 [+ add_i].
 struct CombineArgs {

@@ -321,9 +321,10 @@ void launch_ntt(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 
 }
 
 void launch_ntt_int(const DevTables &t, bool inverse, const u64 *src, u64 *dst, u64 src_stride,
-                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s) {
+                    u64 dst_stride, u32 npolys, u32 limbs, const u32 *prime_map, hipStream_t s,
+                    const u64 *pmask) {
     if (npolys == 0 || limbs == 0) return;
-    ntt_int(t, inverse, src, dst, src_stride, dst_stride, t.N, t.N, npolys, limbs, prime_map, s, nullptr);
+    ntt_int(t, inverse, src, dst, src_stride, dst_stride, t.N, t.N, npolys, limbs, prime_map, s, pmask);
 }
 
 }  // namespace fhea

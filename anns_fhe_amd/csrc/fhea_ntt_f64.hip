@@ -750,7 +750,7 @@ static void rows_logs(const DevTables &t, u32 &lf, u32 &li) {
 }
 
 bool ks_rows_fusable(const DevTables &t) {
-    if (!t.use_f64 || (t.f64_off & 1) || t.logN <= F_LOG_TILE) return false;
+    if (!(t.use_f64 || t.q_mixed_fusable) || (t.f64_off & 1) || t.logN <= F_LOG_TILE) return false;
     if ((t.logN == 13 || t.logN == 14) && (t.k1_fwd == t.logN || t.k1_inv == t.logN)) return false;
     u32 lf, li;
     rows_logs(t, lf, li);
@@ -778,6 +778,18 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s) 
     FHEA_KS_ROWS_F(9)
 #undef FHEA_KS_ROWS_F
 #undef FHEA_KS_ROWS
+}
+
+void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
+    u32 lf, li;
+    rows_logs(t, lf, li);
+    switch (lf) {
+    case 6: launch_ks_rows_inst<6, 8, false>(t, a, slot0, nslots, s); break;
+    case 7: launch_ks_rows_inst<7, 8, false>(t, a, slot0, nslots, s); break;
+    case 8: launch_ks_rows_inst<8, 8, false>(t, a, slot0, nslots, s); break;
+    case 9: launch_ks_rows_inst<9, 8, false>(t, a, slot0, nslots, s); break;
+    default: break;
+    }
 }
 
 }  // namespace fhea

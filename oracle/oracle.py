@@ -19,9 +19,27 @@ u32p = C.POINTER(C.c_uint32)
 i32p = C.POINTER(C.c_int32)
 
 
+_REF_SRC = "/root/reference/hardware_fpga_model/hls/src"
+REF_AUTOMORPHISM = os.path.join(_HERE, "_ref", "libref_automorphism.so")
+
+
 def build():
-    """Compile the oracle with gcc (oracle/Makefile)."""
+    """Compile the oracle with gcc (oracle/Makefile) and, where the reference's
+    sources are present, the part of them that builds here (oracle/_ref)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.isdir(_REF_SRC):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def ref_automorphism():
+    """The reference's own automorphism() (hardware_fpga_model/hls/src/automorphism.cpp:3-12,
+    N fixed at 16384), compiled unmodified into oracle/_ref; None when not built."""
+    if not os.path.exists(REF_AUTOMORPHISM):
+        return None
+    f = getattr(C.CDLL(REF_AUTOMORPHISM), "_Z12automorphismPKmPmS0_")  # automorphism(const u64*, u64*, const u64*)
+    f.restype = None
+    f.argtypes = [u64p, u64p, u64p]
+    return f
 
 
 def lib():

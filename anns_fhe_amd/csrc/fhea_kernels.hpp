@@ -19,7 +19,8 @@ struct DevTables {
     // 1 transforms, 2 basis conversion, 4 inner product, 8 tensor,
     // 16 fused transform ends, 32 rescale / ModDown combines, 64 the inner
     // product fused into the transforms around it; 128 is a switch ON: with the
-    // fused inner product, the P slots also run the ModDown INTT's first pass there
+    // fused inner product, the P slots also run the ModDown INTT's first pass there;
+    // 256: slot-major tile walk for the first pass of rescale-lift prologues
     bool use_f64 = false;
     u32 f64_off = 0;
     u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
@@ -73,6 +74,8 @@ struct NttFuse {
     // optional prime bitmask: only limbs whose prime index p has bit p set
     // (DevTables::pmask_small on a mixed chain)
     const u64 *pmask = nullptr;
+    // first pass: walk the tiles unit-major (find_tile_um) instead of slot-major
+    int unit_major = 0;
     // two-pass transforms (N > 4096): 0 both passes; 1 only the first (its output,
     // FP64 intermediates, left at dst); 2 only the second (reading such intermediates
     // at src)

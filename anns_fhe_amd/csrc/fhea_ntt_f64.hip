@@ -274,11 +274,16 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
         }
     };
 
-    TileId cur = find_tile(blockIdx.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
+    const u32 nsl = fz.slots ? fz.nslots : limbs;
+    auto next_tile = [&](u32 t) {
+        return fz.unit_major ? find_tile_um(t, n_tiles, tiles_per_limb, npolys, nsl, prime_map, fz.slots, fz.pmask)
+                             : find_tile(t, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
+    };
+    TileId cur = next_tile(blockIdx.x);
     u64 raw[16];
     if (PF && cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
-        const TileId nxt = find_tile(cur.tile + gridDim.x, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
+        const TileId nxt = next_tile(cur.tile + gridDim.x);
         if (!PF) issue(cur, raw);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
@@ -403,7 +408,9 @@ template <bool INV, bool COLS>
 static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
-    const u32 tpl = grid.x, ntiles = grid.x * grid.y;
+    const u32 tpl = grid.x;
+    u32 ntiles = grid.x * grid.y;
+    if (fz.unit_major) ntiles = (npolys * tpl + 7) / 8 * 8 * (grid.y / npolys);  // find_tile_um numbering
     // persistent grid: as many workgroups as can be resident at once (max_wg is
     // 4 per CU; a kernel instance that fits fewer per CU gets fewer)
 #define FHEA_NTT_CASE(LR)                                                                            \
@@ -509,6 +516,12 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
         NttFuse pro = fz, epi = fz;
         pro.epi = FUSE_NONE;
         epi.pro = FUSE_NONE;
+        // the first pass of a transform whose prologue reads the rescale lift walks its
+        // tiles unit-major, so every slot reads the lift's words from one XCD's L2
+        // (ModDown + rescale first pass 336 -> 319 us per 16-item chunk); FHEA_F64_OFF
+        // bit 256 keeps the slot-major walk
+        pro.unit_major = !(t.f64_off & 256) && (fz.pro == FUSE_RESCALE_IN || fz.pro == FUSE_MDRS_IN);
+        epi.unit_major = 0;
         if (first)
             launch_pass<false, true>(k1, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw,
                                      t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, pro);

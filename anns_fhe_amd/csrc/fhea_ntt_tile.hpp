@@ -62,4 +62,34 @@ __device__ __forceinline__ TileId find_tile(u32 t, u32 n_tiles, u32 tiles_per_li
     return id;
 }
 
+// Unit-major order for prologues that read one word per position shared by every
+// slot (the rescale lift v): unit u = (polynomial, tile-in-limb), and the nslots
+// tiles of a unit are numbered 8 apart inside groups of 8 units,
+//   t = (u / 8) * 8 * nslots + slot * 8 + u % 8,
+// so on a persistent grid of a multiple of 8 workgroups (dispatched round-robin
+// over the 8 XCDs) all slots of a unit run on one XCD and read its v words from
+// that XCD's L2.  n_tiles = ceil(units / 8) * 8 * nslots; indices of units past
+// the end are skipped.
+__device__ __forceinline__ TileId find_tile_um(u32 t, u32 n_tiles, u32 tiles_per_limb, u32 npolys, u32 nslots,
+                                               const u32 *prime_map, const u32 *slots, const u64 *pmask) {
+    TileId id{n_tiles, 0, 0, 0, 0};
+    const u32 units = npolys * tiles_per_limb;
+    for (; t < n_tiles; t += gridDim.x) {
+        const u32 rem = t % (8 * nslots);
+        const u32 k = rem >> 3, u = (t / (8 * nslots)) * 8 + (rem & 7);
+        if (u >= units) continue;
+        const u32 slot = slots ? slots[k] : k;
+        const u32 p = prime_map[slot];
+        if (p != 0xFFFFFFFFu && (!pmask || ((pmask[p >> 6] >> (p & 63)) & 1))) {
+            id.tile = t;
+            id.poly = u / tiles_per_limb;
+            id.slot = slot;
+            id.pidx = p;
+            id.tI = u % tiles_per_limb;
+            return id;
+        }
+    }
+    return id;
+}
+
 }  // namespace fhea

@@ -181,11 +181,14 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
         ctx.close()
 
 
-@pytest.mark.parametrize("switch", [0, 128, 64, 1 | 2 | 4 | 8 | 16 | 32])
+@pytest.mark.parametrize("switch", [0, 128, 256, 64, 1 | 2 | 4 | 8 | 16 | 32])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
     """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
-    P slots' ModDown INTT pass inside the fused inner product; 64: the inner product
-    as its own kernel; 1..32: each component on the integer kernels).  Every setting
+    P slots' ModDown INTT pass inside the fused inner product; 256: the slot-major
+    tile order (instead of unit-major) for the passes whose prologue reads the
+    rescale lift; 64:
+    the inner product as its own kernel; 1..32: each component on the integer
+    kernels).  Every setting
     gives the oracle's words for a batched HMult (3 items, one lane and two) and a
     rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
     from anns_fhe_amd import _lib

@@ -26,12 +26,25 @@ _CTX = {}
 
 
 def get_ctx(N, L, K, dnum, q_bits=50, poly_words=1 << 24, key_words=1 << 24):
+    """q_bits: generated primes of that size, or "mixed": the reference's chain shape
+    (60-bit q0 and P primes, 45-bit q1..), which runs the FP64 kernels on its 45-bit
+    limbs and the integer kernels on its 60-bit ones."""
     from anns_fhe_amd.driver import FheContext
     key = (N, L, K, dnum, q_bits, poly_words, key_words)
     if key not in _CTX:
         say("ctx create", key)
-        _CTX[key] = FheContext(N, L, K, dnum, q_bits=q_bits, poly_words=poly_words, key_words=key_words)
+        if q_bits == "mixed":
+            from oracle import oracle as O
+            big = O.gen_primes(2 * N, 60, K + 1)
+            prm = dict(q_primes=[big[0]] + O.gen_primes(2 * N, 45, L - 1), p_primes=big[1:])
+        else:
+            prm = dict(q_bits=q_bits)
+        _CTX[key] = FheContext(N, L, K, dnum, poly_words=poly_words, key_words=key_words, **prm)
     return _CTX[key]
+
+
+def seed_of(q_bits):
+    return q_bits if isinstance(q_bits, int) else 45
 
 
 def get_orc(orc, ctx):
@@ -427,7 +440,7 @@ def test_cpp_host_driver_c1_sequence(orc, tmp_path):
     assert np.array_equal(rot[W1:], pm[8 * W:8 * W + W1])
 
 
-@pytest.mark.parametrize("q_bits", [50, 60])
+@pytest.mark.parametrize("q_bits", [50, 60, "mixed"])
 def test_linear_sum_parity(orc, q_bits):
     """fhea_eval_linear_sum (FP64 path at 50-bit primes, integer path at 60-bit)
     against the oracle's OP 5 / OP 3 composition, with bias and per-output plaintexts."""
@@ -435,7 +448,7 @@ def test_linear_sum_parity(orc, q_bits):
     N, L = 4096, 3
     ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
     o = get_orc(orc, ctx)
-    rng = np.random.default_rng(q_bits)
+    rng = np.random.default_rng(seed_of(q_bits))
     W = L * N
     R, G = 20, 11
     n_slots = 2 * R + G * R + G + 2 + 2 * G
@@ -496,14 +509,14 @@ def test_submit_mult_relin_and_add_parity(orc):
             assert np.array_equal(ctx.read_poly(off * W, limbs * N), pm[off * W:off * W + limbs * N]), (limbs, off)
 
 
-@pytest.mark.parametrize("q_bits", [50, 60])
+@pytest.mark.parametrize("q_bits", [50, 60, "mixed"])
 def test_hoisted_rotations_parity(orc, q_bits):
     """Work kind 7 (runs of rotations of the same input share one ModUp) against the
     oracle's hoisted rotations, two inputs in one submit, both arithmetic paths."""
     N, L = 4096, 4
     ctx = get_ctx(N, L, 2, 2, q_bits=q_bits)
     o = get_orc(orc, ctx)
-    rng = np.random.default_rng(q_bits + 1)
+    rng = np.random.default_rng(seed_of(q_bits) + 1)
     kidx = list(range(L)) + [L + j for j in range(ctx.K)]
     ksz = ctx.key_size_words(L)
     n_keys = 5
@@ -662,7 +675,7 @@ def test_async_reads_overlap_later_work():
     pin.close()
 
 
-@pytest.mark.parametrize("q_bits", [50, 60])
+@pytest.mark.parametrize("q_bits", [50, 60, "mixed"])
 def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
     """fhea_eval_linear_sum_batch (jobs of different G sharing plaintext blocks, with
     and without bias) equals the oracle's job-by-job sums; an output of one job
@@ -672,7 +685,7 @@ def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
     N, L = 4096, 3
     ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
     o = get_orc(orc, ctx)
-    rng = np.random.default_rng(7 + q_bits)
+    rng = np.random.default_rng(7 + seed_of(q_bits))
     W = L * N
     R, n_blocks = 9, 6
     n_in = 3 * 2 * R + n_blocks * R + n_blocks + 2
@@ -719,7 +732,7 @@ def test_linear_sum_batch_and_rescale_scatter(orc, q_bits):
     assert ei.value.status == -4
 
 
-@pytest.mark.parametrize("q_bits", [50, 60])
+@pytest.mark.parametrize("q_bits", [50, 60, "mixed"])
 def test_linear_sum_groups_parity(orc, q_bits):
     """fhea_eval_linear_sum_groups: groups of 5x6, 1x3, 4x4 and 3x9 (query x block),
     queries and blocks shared between groups, absent biases and block biases; every
@@ -730,7 +743,7 @@ def test_linear_sum_groups_parity(orc, q_bits):
     N, L = 4096, 3
     ctx = get_ctx(N, L, 1, 3, q_bits=q_bits)
     o = get_orc(orc, ctx)
-    rng = np.random.default_rng(70 + q_bits)
+    rng = np.random.default_rng(70 + seed_of(q_bits))
     W = L * N
     R, n_q, n_b = 7, 6, 9
     n_in = n_q * 2 * R + n_b * R + n_b + 2 * n_q

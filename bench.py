@@ -641,7 +641,8 @@ def bench_serve(args):
 
 
 def extras(args):
-    """The rest of the metric beside the headline: C3 rotations, C4 encrypted IVF-PQ
+    """The rest of the metric beside the headline: the HMult on the reference's own
+    60/45-bit prime chain, C3 rotations, C4 encrypted IVF-PQ
     (queries/s, p50/p99, recall@10) and C5 serving (queries/s at one sub-saturation
     and one saturating rate, against the unbatched and the op-by-op plain paths).
     Each runs in its own contexts after the headline's are freed; a failure is
@@ -665,6 +666,16 @@ def extras(args):
             out[name] = {"error": f"{type(e).__name__}: {e}"}
         log(f"{name}: {json.dumps(out[name])[:300]}")
 
+    def hmult_closed(ns):
+        r, c = bench_hmult(ns)
+        c.close()
+        return r
+
+    run("ref_chain", hmult_closed, sub(chain="ref", steps=5, warmup=1, cpu_sample=0, plain_steps=0, ntt_reps=1),
+        lambda r: {"metric": r["metric"] + ", the reference's 60/45-bit prime chain", "value": r["value"],
+                   "unit": r["unit"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+                   "config": {k: r["config"][k] for k in ("N", "L", "K", "dnum", "batch", "q_bits", "kernels",
+                                                           "schedule")}})
     run("rotate", bench_ks64, sub(steps=5, warmup=1, cpu_sample=1),
         lambda r: {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
                    "eager_ms_per_step": r["config"]["eager_ms_per_step"], "roofline": r["roofline"],
@@ -679,18 +690,10 @@ def extras(args):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by independent ciphertexts and queries, one engine per GPU; that "
-              "sharding is not built)", file=sys.stderr)
-        sys.exit(2)
-    if args.workload != "hmult":
-        res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq,
-               "serve": bench_serve, "c1": bench_c1}[args.workload](args)
-        print(json.dumps(res), flush=True)
-        return
+def bench_hmult(args):
+    """The headline: batched eval_mult_relin_rescale at N=2^16, L=24, K=8, dnum=3 on
+    the chosen prime chain, with the NTT roofline launch, the achievable copy rate,
+    the op-by-op plain path and the CPU baseline.  Returns (result, open context)."""
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
 
     B = args.batch
@@ -788,7 +791,8 @@ def main():
                                "shared relin key, inputs resident in HBM",
                    "N": N, "L": L, "K": K, "dnum": DNUM, "batch": B,
                    "q_bits": 50 if args.chain == "q50" else "60 (q0), 45 (q1..q23), 60 (P)",
-                   "kernels": "FP64" if args.chain == "q50" else "64-bit integer",
+                   "kernels": "FP64" if args.chain == "q50" else
+                   "FP64 on the 45-bit limbs, 64-bit integer on the 60-bit limbs",
                    "schedule": schedule},
         "roofline": {"bound": "hbm",
                      "kernel": f"batched forward NTT, {npr} polys x 24 limbs ({ntt_kernel} COLS + ROWS dispatches)",
@@ -823,6 +827,22 @@ def main():
         log("cpu baseline")
         key = ctx.read_key(0, key_words)
         result["cpu_baseline"] = cpu_baseline(ctx, key, args.cpu_sample)
+    return result, ctx
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by independent ciphertexts and queries, one engine per GPU; that "
+              "sharding is not built)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload != "hmult":
+        res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq,
+               "serve": bench_serve, "c1": bench_c1}[args.workload](args)
+        print(json.dumps(res), flush=True)
+        return
+    result, ctx = bench_hmult(args)
     if not args.no_extras:
         ctx.close()
         result.update(extras(args))

@@ -80,6 +80,7 @@ SIGNATURES = {
     "fhea_ctx_set_tuning": (C.c_int, [vp, C.c_int, C.c_uint32]),
     "fhea_ctx_get_tuning": (C.c_int, [vp, C.c_int, u32p]),
     "fhea_mem_info": (C.c_int, [C.c_uint32, u64p, u64p]),
+    "fhea_probe_valu_rate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     "fhea_host_alloc": (vp, [C.c_uint64]),
     "fhea_host_free": (C.c_int, [vp]),
     "fhea_poly_arena": (vp, [vp]),
@@ -172,6 +173,15 @@ def mem_info(device=0):
     f, t = C.c_uint64(), C.c_uint64()
     check(lib.fhea_mem_info(device, C.byref(f), C.byref(t)), "fhea_mem_info")
     return f.value, t.value
+
+
+def probe_valu_rate(kind, iters=4096, device=0):
+    """Lane operations per second of one VALU instruction kind (fhea_probe_valu_rate):
+    0 v_mul_lo_u32, 1 v_mul_hi_u32, 2 v_mad_u64_u32, 3 v_fma_f64."""
+    lib = load()
+    r = C.c_double()
+    check(lib.fhea_probe_valu_rate(device, kind, iters, C.byref(r)), "fhea_probe_valu_rate")
+    return r.value
 
 
 def check(status, what):

@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FHEA_ARCH", "gfx950")
 
 SOURCES = ["fhea_context.cpp", "fhea_client.cpp", "fhea_ntt.hip", "fhea_ntt_f64.hip", "fhea_kernels.hip",
-           "fhea_engine.hip"]
+           "fhea_engine.hip", "fhea_probe.hip"]
 
 FLAGS = [
     "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -5,9 +5,10 @@
 // identical to the reference's Barrett mod_mul whatever reduction is used
 // here.  Primes are limited to q < 2^61 (the reference's largest is 60 bits).
 //
-// On gfx950 the 64x64->128 products lower to v_mad_u64_u32 / v_mul_hi_u32
-// sequences; there is no 64-bit multiplier, so the reductions below are
-// chosen for the fewest 32-bit multiplies:
+// On gfx950 the 64x64->128 products lower to v_mad_u64_u32 / v_mul_hi_u32 /
+// v_mul_lo_u32 sequences (there is no 64-bit multiplier; each of those issues at
+// about the FP64 FMA rate), so the reductions below are chosen for the fewest
+// 32-bit multiplies:
 //   * mul_shoup   a*w mod q for a precomputed constant w (twiddles, CRT
 //                 constants): 1 mulhi64 + 2 mullo64.
 //   * mulmod      a*b mod q for a, b < q: pre-shifted Barrett with

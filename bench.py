@@ -50,7 +50,7 @@ def parse():
                     help="serve: N=2^16 query slots the HBM budget admits at once (at most 16 per replica)")
     ap.add_argument("--serve-large-replicas", type=int, default=1,
                     help="serve: contexts serving the N=2^16 set (replicas: own stream and worker each)")
-    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve", "c1"],
+    ap.add_argument("--workload", default="hmult", choices=["hmult", "ntt256", "ks64", "ivfpq", "serve", "c1", "valu"],
                     help="hmult: the headline line; ntt256: 256 polys x 24 limbs NTT+INTT; "
                          "ks64: 64 rotations with distinct resident keys; ivfpq: encrypted IVF-PQ "
                          "search (1M x 128, n_list=32, M=8, K=256, B=64 queries, N=2^14, L=8, n_probe=4)")
@@ -246,6 +246,31 @@ def cpu_rotations(ctx, key_blob, sample):
     return {"value": sample / dt, "unit": "rotations/s", "cores": 1, "kind": "port",
             "sample": f"{sample} x eval_rotate(+1) N=2^16 L=24 K=8 dnum=3 ({dt:.1f} s, 1 thread of "
                       f"{_cpu_name()})"}
+
+
+def bench_valu(args):
+    """Issue rates of the VALU instructions the 64-bit integer kernels are made of
+    (fhea_probe_valu_rate), against FP64 FMA, which issues at the full rate
+    (64 lanes per CU per clock: 78.6 TFLOP/s at 2 FLOP per FMA)."""
+    from anns_fhe_amd import _lib
+    names = ["v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v_fma_f64"]
+    iters = 16384
+    rates = {}
+    for k, n in enumerate(names):
+        best = max(_lib.probe_valu_rate(k, iters) for _ in range(3))
+        rates[n] = best
+        log(f"{n}: {best / 1e12:.2f} T lane-ops/s")
+    full = rates["v_fma_f64"]
+    return {
+        "metric": "VALU issue rate (lane operations/s) of the integer kernels' 32-bit multiplies",
+        "value": round(rates["v_mul_lo_u32"] / 1e12, 3), "unit": "T lane-ops/s (v_mul_lo_u32)",
+        "higher_is_better": True, "n_gpus": 1,
+        "rates_T_per_s": {n: round(r / 1e12, 3) for n, r in rates.items()},
+        "relative_to_fp64_fma": {n: round(r / full, 3) for n, r in rates.items()},
+        "fp64_fma_as_TFLOPs": round(2 * full / 1e12, 1),
+        "config": {"workload": "8 workgroups of 256 threads per CU, 8 independent dependent chains of one "
+                               f"instruction per thread, {iters} steps; best of 3"},
+    }
 
 
 def bench_c1(args):
@@ -839,7 +864,7 @@ def main():
         sys.exit(2)
     if args.workload != "hmult":
         res = {"ntt256": bench_ntt256, "ks64": bench_ks64, "ivfpq": bench_ivfpq,
-               "serve": bench_serve, "c1": bench_c1}[args.workload](args)
+               "serve": bench_serve, "c1": bench_c1, "valu": bench_valu}[args.workload](args)
         print(json.dumps(res), flush=True)
         return
     result, ctx = bench_hmult(args)

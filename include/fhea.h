@@ -130,6 +130,12 @@ int fhea_copy_poly(fhea_ctx *ctx, uint64_t dst_off, uint64_t src_off, uint64_t c
  * sizing arenas and admission budgets from what the GPU actually has.  No reference
  * counterpart (its memories are fixed-size host vectors, sim_memory_bus.h:13). */
 int fhea_mem_info(uint32_t device, uint64_t *free_bytes, uint64_t *total_bytes);
+/* VALU issue rate of one instruction kind on HIP device `device`, in lane operations
+ * per second: kind 0 v_mul_lo_u32, 1 v_mul_hi_u32, 2 v_mad_u64_u32 (the 32-bit
+ * multiplies a 64-bit modular product lowers to), 3 v_fma_f64.  Eight workgroups of
+ * 256 threads per CU, eight independent chains of `iters` dependent steps per thread.
+ * A diagnostic for the integer kernels' op-rate ceiling; no reference counterpart. */
+int fhea_probe_valu_rate(uint32_t device, uint32_t kind, uint32_t iters, double *lane_ops_per_s);
 /* Page-locked host memory (hipHostMalloc) for arena reads and writes at full DMA
  * speed; NULL on failure.  Free with fhea_host_free. */
 void *fhea_host_alloc(uint64_t bytes);

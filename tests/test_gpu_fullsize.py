@@ -288,3 +288,56 @@ def test_mixed_chain_levels_vs_oracle(orc, shape):
             assert np.array_equal(ctx.read_poly(12 * W, l * N), pm[12 * W:12 * W + l * N]), ("rot", l)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("chain", ["q50", "mixed"])
+def test_largest_n_vs_oracle(orc, chain):
+    """N = 2^17, the largest ring the tables accept: the transforms' two passes are
+    2^8 x 2^9 (the fused key-switch row pass runs its 2^9 instance).  OP 7, OP 8,
+    eval_mult_relin_rescale and a rotation = oracle, on 50-bit primes and on a mixed
+    60/45-bit chain."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    N, L, K, dnum = 1 << 17, 4, 2, 2
+    if chain == "q50":
+        ps = orc.gen_primes(2 * N, 50, L + K)
+        q, p = ps[:L], ps[L:]
+    else:
+        big = orc.gen_primes(2 * N, 60, K + 1)
+        q, p = [big[0]] + orc.gen_primes(2 * N, 45, L - 1), big[1:]
+    W, W1 = L * N, (L - 1) * N
+    ksz = 2 * dnum * (L + K) * N
+    ctx = FheContext(N, L, K, dnum, q_primes=q, p_primes=p, poly_words=16 * W, key_words=2 * ksz + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, q, p, dnum)
+        rng = np.random.default_rng(17 + len(chain))
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g1 = ctx.galois_for_step(-1)
+        drv.add_rotation_key(-1, g1, ksz, L)
+        data = np.concatenate([rand_poly(rng, q, N).reshape(-1) for _ in range(4)])
+        ctx.write_poly(0, data)
+        pm = np.zeros(16 * W, np.uint64)
+        pm[:4 * W] = data
+        say("N=2^17", chain, "op7")
+        assert ctx.op(7, W, 0, 5 * W, 0, L, L) == 0
+        assert o.accel_op(pm, km, 7, W, 0, 5 * W, 0, L, L) == 0
+        assert np.array_equal(ctx.read_poly(5 * W, 2 * W), pm[5 * W:7 * W])
+        say("N=2^17", chain, "op8")
+        assert ctx.op(8, 0, W, 7 * W, 0, L) == 0
+        assert o.accel_op(pm, km, 8, 0, W, 7 * W, 0, L) == 0
+        assert np.array_equal(ctx.read_poly(7 * W, 2 * W1), pm[7 * W:7 * W + 2 * W1])
+        say("N=2^17", chain, "hmult")
+        drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 9 * W, 10 * W, L)
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 9 * W, 10 * W, L) == 0
+        assert np.array_equal(ctx.read_poly(9 * W, W1), pm[9 * W:9 * W + W1])
+        assert np.array_equal(ctx.read_poly(10 * W, W1), pm[10 * W:10 * W + W1])
+        say("N=2^17", chain, "rotate")
+        drv.eval_rotate(0, W, 11 * W, 12 * W, -1, L)
+        assert o.eval_rotate(pm, km, [(-1, g1, ksz, L)], 0, W, 11 * W, 12 * W, -1, L) == 0
+        assert np.array_equal(ctx.read_poly(11 * W, W), pm[11 * W:12 * W])
+        assert np.array_equal(ctx.read_poly(12 * W, W), pm[12 * W:13 * W])
+    finally:
+        ctx.close()

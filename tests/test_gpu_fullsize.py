@@ -341,3 +341,46 @@ def test_largest_n_vs_oracle(orc, chain):
         assert np.array_equal(ctx.read_poly(12 * W, W), pm[12 * W:13 * W])
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("N", [1024, 2048])
+@pytest.mark.parametrize("chain", ["q50", "mixed"])
+def test_smallest_n_vs_oracle(orc, N, chain):
+    """The smallest rings (one-pass transforms of a whole limb, unfused key switch):
+    eval_mult_relin_rescale and a rotation = oracle, on 50-bit and mixed chains."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    L, K, dnum = 3, 1, 3
+    if chain == "q50":
+        ps = orc.gen_primes(2 * N, 50, L + K)
+        q, p = ps[:L], ps[L:]
+    else:
+        big = orc.gen_primes(2 * N, 60, K + 1)
+        q, p = [big[0]] + orc.gen_primes(2 * N, 45, L - 1), big[1:]
+    W, W1 = L * N, (L - 1) * N
+    ksz = 2 * dnum * (L + K) * N
+    ctx = FheContext(N, L, K, dnum, q_primes=q, p_primes=p, poly_words=16 * W, key_words=2 * ksz + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, q, p, dnum)
+        rng = np.random.default_rng(N + len(chain))
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g = ctx.galois_for_step(4)
+        drv.add_rotation_key(4, g, ksz, L)
+        data = np.concatenate([rand_poly(rng, q, N).reshape(-1) for _ in range(4)])
+        ctx.write_poly(0, data)
+        pm = np.zeros(16 * W, np.uint64)
+        pm[:4 * W] = data
+        say("small N", N, chain)
+        drv.eval_mult_relin_rescale(0, W, 2 * W, 3 * W, 9 * W, 10 * W, L)
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 9 * W, 10 * W, L) == 0
+        assert np.array_equal(ctx.read_poly(9 * W, W1), pm[9 * W:9 * W + W1])
+        assert np.array_equal(ctx.read_poly(10 * W, W1), pm[10 * W:10 * W + W1])
+        drv.eval_rotate(0, W, 11 * W, 12 * W, 4, L)
+        assert o.eval_rotate(pm, km, [(4, g, ksz, L)], 0, W, 11 * W, 12 * W, 4, L) == 0
+        assert np.array_equal(ctx.read_poly(11 * W, W), pm[11 * W:12 * W])
+        assert np.array_equal(ctx.read_poly(12 * W, W), pm[12 * W:13 * W])
+    finally:
+        ctx.close()

@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
+    ap.add_argument("--single-steps", type=int, default=20,
+                    help="ivfpq: single queries timed after the batches (0 = skip)")
     ap.add_argument("--serve-chunk", type=int, default=0,
                     help="serve: key-switch items per pipeline chunk in every parameter set (0 = engine default)")
     ap.add_argument("--stage3-chunks", type=int, default=8,
@@ -515,6 +517,24 @@ def bench_ivfpq(args):
     qps = B * args.steps / srv
     lat_ms = np.array(lat) * 1e3
     log(f"ivfpq: {qps:.1f} queries/s (server), batch latency p50 {np.percentile(lat_ms, 50):.1f} ms")
+    # one query at a time on the same server (slot 0): the latency the reference quotes per query
+    single = None
+    if args.single_steps > 0:
+        s_lat, s_srv = [], []
+        for i in range(3 + args.single_steps):
+            q = batch()[:1]
+            w0 = time.perf_counter()
+            _, t = IV.execute_interactive_queries(server, client, q, n_probe, 10, stage3_chunks=args.stage3_chunks)
+            if i >= 3:
+                s_lat.append((time.perf_counter() - w0) * 1e3)
+                s_srv.append((t["server_stage1"] + t["server_stage3"]) * 1e3)
+        single = {"latency_ms": {"p50": round(float(np.percentile(s_lat, 50)), 2),
+                                 "p99": round(float(np.percentile(s_lat, 99)), 2)},
+                  "server_ms_p50": round(float(np.percentile(s_srv, 50)), 2), "queries": len(s_lat),
+                  "note": "one encrypted query through the same protocol (client and server, end to end); the "
+                          "reference quotes 3,544 ms (CPU) and 137.5 ms (FPGA) per query at N=2^14/L=11 with "
+                          "its own query sequence (README.md:315)"}
+        log(f"ivfpq: single query p50 {single['latency_ms']['p50']} ms, server {single['server_ms_p50']} ms")
     return {
         "metric": "encrypted IVF-PQ queries/s (interactive, N=2^14, L=8, n_probe=4, 1M x 128)",
         "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -531,6 +551,7 @@ def bench_ivfpq(args):
         "recall_at_10": {"encrypted": round(float(np.mean(rec_f)), 4), "plaintext_adc": round(float(np.mean(rec_p)), 4),
                          "same_top10_sets": same, "queries": len(rec_f)},
         "setup_s": {"dataset_index": round(t_index, 1), "encode_upload": round(t_encode, 1)},
+        "single_query": single,
     }
 
 
@@ -708,7 +729,8 @@ def extras(args):
                                                                   ("N", "L", "K", "dnum", "cts", "key_bytes")}})
     run("ivfpq", bench_ivfpq, sub(steps=3, warmup=1),
         lambda r: {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
-                   "latency_ms": r["latency_ms"], "recall_at_10": r["recall_at_10"], "config": r["config"]})
+                   "latency_ms": r["latency_ms"], "recall_at_10": r["recall_at_10"], "config": r["config"],
+                   "single_query": r["single_query"]})
     run("serving", bench_serve, sub(rates="80,1280", requests=160, plain_steps=3),
         lambda r: {k: r[k] for k in ("metric", "value", "unit", "latency_ms", "sweep", "unbatched_same_protocol",
                                      "speedup_vs_unbatched", "plain_path", "speedup_vs_plain", "config")})

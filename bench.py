@@ -57,6 +57,12 @@ def parse():
     ap.add_argument("--db-size", type=int, default=1 << 20, help="ivfpq: database vectors")
     ap.add_argument("--queries", type=int, default=64, help="ivfpq: queries per batch")
     ap.add_argument("--no-graphs", action="store_true", help="ivfpq: launch stage 1 eagerly instead of a HIP graph")
+    ap.add_argument("--ivf-special", type=int, default=2,
+                    help="ivfpq: special primes K of the N=2^14 context (key switching runs at 2 limbs, "
+                         "one digit of at most 2 primes, so 2 special primes exceed it)")
+    ap.add_argument("--serve-special", type=int, default=2,
+                    help="serve: special primes K of both parameter sets (both evaluate at 2 limbs; "
+                         "0 = 4 for N=2^14 and 8 for N=2^16)")
     ap.add_argument("--single-steps", type=int, default=20,
                     help="ivfpq: single queries timed after the batches (0 = skip)")
     ap.add_argument("--serve-chunk", type=int, default=0,
@@ -444,7 +450,7 @@ def bench_ivfpq(args):
     from anns_fhe_amd.client import CkksClient
     from anns_fhe_amd.driver import FheContext
     n, B, n_probe = args.db_size, args.queries, 4
-    Nq, Lq, Kq, dq, lt = 1 << 14, 8, 4, 2, 2
+    Nq, Lq, Kq, dq, lt = 1 << 14, 8, args.ivf_special, 2, 2
     t0 = time.perf_counter()
     x, _ = IV.make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
@@ -569,10 +575,11 @@ def bench_serve(args):
     x, _ = IV.make_gaussian_mixture(n, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=65536, iters=10, seed=0).add(x)
     log("serve: building parameter sets")
-    small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=48, log=log)
+    k14, k16 = (args.serve_special, args.serve_special) if args.serve_special else (4, 8)
+    small = SV.ParamSet.create("n14", 1 << 14, 8, k14, 2, ix, max_slots=48, log=log)
     nrep = max(1, args.serve_large_replicas)
     # replicas share their keys through an explicit seed; a single set draws them securely
-    larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, 8, 3, ix, max_slots=16,
+    larges = [SV.ParamSet.create("n16" if nrep == 1 else f"n16_{i}", 1 << 16, 24, k16, 3, ix, max_slots=16,
                                  seed=None if nrep == 1 else 2, log=log, group="n16") for i in range(nrep)]
     large = larges[0]
     if args.serve_chunk:
@@ -584,8 +591,8 @@ def bench_serve(args):
     from anns_fhe_amd import _lib
     hbm_free, hbm_total = _lib.mem_info(0)
     # the slot counts the device's free HBM would give each set (half of it each, at most 256)
-    mem_slots = {"n14": SV.ParamSet.slots_for_memory(1 << 14, 8, 4, 2, ix, hbm_share=0.5),
-                 "n16": SV.ParamSet.slots_for_memory(1 << 16, 24, 8, 3, ix, hbm_share=0.5 / nrep)}
+    mem_slots = {"n14": SV.ParamSet.slots_for_memory(1 << 14, 8, k14, 2, ix, hbm_share=0.5),
+                 "n16": SV.ParamSet.slots_for_memory(1 << 16, 24, k16, 3, ix, hbm_share=0.5 / nrep)}
     rng = np.random.default_rng(7)
     sweep = []
     for rate in [float(r) for r in args.rates.split(",")]:
@@ -673,6 +680,7 @@ def bench_serve(args):
                    "db": n, "requests_per_rate": args.requests, "budget_bytes": budget,
                    "large_replicas": nrep, "ks_chunk": args.serve_chunk or None,
                    "eval_limbs": {"n14": small.server.lt, "n16": large.server.lt},
+                   "special_primes": {"n14": k14, "n16": k16},
                    "hbm": {"free_bytes_after_sets": hbm_free, "total_bytes": hbm_total,
                            "slots_admitted": {"n14": args.serve_small_slots, "n16": args.serve_large_slots},
                            "slots_free_hbm_would_hold": mem_slots},

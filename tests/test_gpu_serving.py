@@ -18,12 +18,15 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_serving_mixed_params_poisson():
+@pytest.mark.parametrize("k_special", [(4, 8), (2, 2)])
+def test_serving_mixed_params_poisson(k_special):
+    """k_special: special primes of the two sets; (2, 2) is the bench's serving
+    configuration (key switching at 2 limbs needs no more)."""
     x, _ = IV.make_gaussian_mixture(20000, dim=128, n_centres=32, sigma=0.1, seed=42)
     ix = IV.IvfPqIndex(n_list=32, M=8, K=256).train(x, train_size=20000, iters=8, seed=0).add(x)
-    say("param sets")
-    small = SV.ParamSet.create("n14", 1 << 14, 8, 4, 2, ix, max_slots=8, seed=1)
-    large = SV.ParamSet.create("n16", 1 << 16, 24, 8, 3, ix, max_slots=4, seed=2)
+    say("param sets", k_special)
+    small = SV.ParamSet.create("n14", 1 << 14, 8, k_special[0], 2, ix, max_slots=8, seed=1)
+    large = SV.ParamSet.create("n16", 1 << 16, 24, k_special[1], 3, ix, max_slots=4, seed=2)
     try:
         rng = np.random.default_rng(3)
         n = 24

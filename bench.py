@@ -487,10 +487,11 @@ def bench_ivfpq(args):
         slots = list(range(B))
         use_graphs, server.use_graphs = server.use_graphs, False
         t0 = time.perf_counter()
-        autotune.tune(ctx, lambda: server.stage1_coarse_centroid_distance(slots),
-                      check=lambda: np.concatenate([ctx.read_poly(server._offs(z)["res"], 2 * (lt - 1) * Nq)
-                                                    for z in (0, B - 1)]),
-                      knobs=("ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"), reps=3, log=log)
+        tim = autotune.tune(ctx, lambda: server.stage1_coarse_centroid_distance(slots),
+                            check=lambda: np.concatenate([ctx.read_poly(server._offs(z)["res"], 2 * (lt - 1) * Nq)
+                                                          for z in (0, B - 1)]),
+                            knobs=("ntt_split_fwd", "ntt_split_inv", "ntt_wg_per_cu", "chunk"), reps=3, log=log)
+        log(f"ivfpq: autotune timings (ms, stage 1 eager) {tim}")
         server.use_graphs = use_graphs
         for gid in server.graphs.values():
             ctx.graph_destroy(gid)

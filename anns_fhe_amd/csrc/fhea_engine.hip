@@ -975,14 +975,14 @@ static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const std::vecto
             const RotItem &it = items[off + z];
             const RotInput &in = inputs[it.input];
             u64 *s0 = S + (u64)z * 2 * l * N;
-            aa[z] = {in.in0, in.in0, s0, s0 + l * N, it.galois};  // sigma_g(c0) (second output unused)
+            aa[z] = {in.in0, nullptr, s0, nullptr, it.galois};  // sigma_g(c0) only
             ra[z] = {in.in1, it.key, it.galois, compact + (u64)it.input * cz};
             cb[z] = {it.d0, it.d1, s0, nullptr};
         }
         auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
         auto *ra_d = (const KsRotArgs *)stage_args(c, ra.data(), nb * sizeof(KsRotArgs));
         if (!aa_d || !ra_d) return FHEA_ERR_DEVICE;
-        launch_automorph(t, aa_d, nb, l, s);
+        launch_automorph(t, aa_d, nb, l, s, 1);
         launch_ks_inner_rot(t, ra_d, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum, s);
         st = ks_moddown(c, l, nb, acc, pb, tt, cb);
         if (st) return st;

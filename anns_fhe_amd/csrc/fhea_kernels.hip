@@ -117,8 +117,8 @@ __global__ __launch_bounds__(TPB) void k_automorph(const AutoArgs *__restrict__ 
 }
 
 void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, u32 limbs,
-                      hipStream_t s) {
-    dim3 grid(t.N / TPB, 2 * limbs, nbatch);
+                      hipStream_t s, u32 npolys) {
+    dim3 grid(t.N / TPB, npolys * limbs, nbatch);
     hipLaunchKernelGGL(k_automorph, grid, dim3(TPB), 0, s, args_dev, limbs, t.logN);
 }
 

@@ -140,14 +140,15 @@ void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u
 // out[i] = in[map[i]] with map packed two u32 per word (automorphism.cpp:3-12).
 void launch_automorph_map(const DevTables &t, const u64 *in, u64 *out, const u64 *packed_map,
                           u32 limbs, hipStream_t s);
-// out = sigma_g(in) for c0 and c1 of nbatch ciphertexts (closed-form map).
+// out = sigma_g(in) for c0 and c1 of nbatch ciphertexts (closed-form map);
+// npolys = 1: c0 only (in1 / out1 unused).
 struct AutoArgs {
     const u64 *in0, *in1;
     u64 *out0, *out1;
     u64 galois;
 };
 void launch_automorph(const DevTables &t, const AutoArgs *args_dev, u32 nbatch, u32 limbs,
-                      hipStream_t s);
+                      hipStream_t s, u32 npolys = 2);
 
 // Basis conversion (key_switch.cpp:7-60) for a batch: block (x, p, z) converts
 // descriptor descs[p] reading in + z*in_zstride + (in_first_p)*N and writing

@@ -2025,8 +2025,11 @@ int fhea_rescale_scatter(fhea_ctx *c, uint64_t src, const uint64_t *dst, uint32_
         if (z + 1 < n && d[z + 1] - d[z] < 2 * W1) return FHEA_ERR_ARG;
         if (src < d[z] + 2 * W1 && d[z] < src + 2 * W * n) return FHEA_ERR_ARG;
     }
-    for (u32 off = 0; off < n; off += 64) {
-        const u32 nb = std::min<u32>(64, n - off);
+    // as many items per launch as 1 GiB of workspace holds (at most 1024): at small N
+    // and few limbs, batches of 64 left most of the device idle in each transform
+    const u32 cap = (u32)std::max<u64>(1, std::min<u64>(1024, (1ull << 27) / rescale_words(c, limbs)));
+    for (u32 off = 0; off < n; off += cap) {
+        const u32 nb = std::min<u32>(cap, n - off);
         int st = ensure_ws(c, nb * rescale_words(c, limbs));
         if (st) return st;
         std::vector<u64 *> outs(2 * nb);

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libfhea.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FHEA_ARCH", "gfx950")
 
-SOURCES = ["fhea_context.cpp", "fhea_client.cpp", "fhea_ntt.hip", "fhea_ntt_f64.hip", "fhea_kernels.hip",
+SOURCES = ["fhea_context.cpp", "fhea_client.cpp", "fhea_ntt.hip", "fhea_ntt_f64.hip", "fhea_conv_cols.hip", "fhea_kernels.hip",
            "fhea_engine.hip", "fhea_probe.hip"]
 
 FLAGS = [

@@ -1,0 +1,346 @@
+// fhea_conv_cols.hip -- fast basis conversion fused between the column passes of
+// the transforms on either side of it (FP64 path, two-pass N).
+//
+// The key switch's ModUp is INTT -> fast_bases_conv -> NTT on every digit
+// (key_switch.cpp:229-329), its ModDown INTT(P) -> conversion P -> Q -> NTT
+// (key_switch.cpp:385-427).  With the two-pass transform of a limb viewed as
+// [2^LOGR rows][2^logS columns], the INTT's second pass runs down the columns
+// (stages on row bits 0 .. LOGR-1) and the NTT's first pass also runs down the
+// columns (row bits LOGR-1 .. 0), and the conversion is position-wise.  So one
+// workgroup that owns a tile of 2^LOGC adjacent columns (all their rows) can run
+// the column pass of every input limb of a digit, keep the converted inputs
+// [x_i qhat_i^-1]_{q_i} of its positions in registers, and produce each output
+// limb's conversion and forward column pass from them: the converted limbs never
+// go through HBM in coefficient form, and each input word is read once instead of
+// once per output.  The rows passes before (INTT) and after (NTT) stay as they are.
+//
+// Layout.  TPB = 2^(LOGC+LOGR-W) threads; a thread holds E = 2^W elements of the
+// tile: its column c = tid mod 2^LOGC and the rows whose index differs only in the
+// W-bit window of the current round ("radix-E rounds", exchanges through LDS).
+// The inverse's last window and the forward's first window are both the top W row
+// bits, so the converted inputs sit in the registers where the forward column pass
+// starts.  Every global access is a run of 2^LOGC consecutive words per row.
+//
+// Arithmetic: the FP64 exact products of fhea_f64.hpp.  Input limb i is scaled by
+// N^-1 qhat_i^-1 mod q_i (the INTT's N^-1 and the conversion's first factor in one
+// product), centred, and moved into [0, q_i): the conversion's input integer.  The
+// conversion is k_fbc_f64's sum (terms mulmod_f64(y_i, qhat_i mod p_j), centred
+// every four terms), then the forward column pass starts from the centred sum.
+// Every step is exact mod p_j, so the words equal INTT -> fast_bases_conv -> NTT.
+#include "fhea_f64.hpp"
+#include "fhea_kernels.hpp"
+#include "fhea_ntt_tile.hpp"
+
+namespace fhea {
+
+
+// Round RD of a LOGR-stage column transform in W-bit windows.  Forward stages run
+// on row bits LOGR-1 .. 0, inverse stages on 0 .. LOGR-1; P is the window's low
+// bit (a short last round keeps a full-width window, shifted).
+template <bool INV, int LOGR, int W, int RD>
+struct RoundW {
+    static constexpr int NR = (LOGR + W - 1) / W;
+    static constexpr int TOP = LOGR - 1 - W * RD;
+    static constexpr int NS = INV ? ((W * RD + W <= LOGR) ? W : LOGR - W * RD) : ((TOP + 1 >= W) ? W : TOP + 1);
+    static constexpr int P = INV ? ((W * RD + W <= LOGR) ? W * RD : LOGR - W) : ((TOP >= W - 1) ? TOP - (W - 1) : 0);
+    static constexpr int BIT0 = INV ? W * RD : TOP;
+};
+
+// tile element of the thread's element 0 in a round whose window starts at tile bit PP
+// (PP = LOGC + P: the column bits sit below the row bits)
+template <int W>
+__device__ __forceinline__ u32 elem_base(u32 tid, u32 PP) {
+    return (tid & ((1u << PP) - 1)) | ((tid >> PP) << (PP + W));
+}
+
+// Butterfly stages of one round on the thread's E elements.  j0 = global word
+// index (within the limb) of element 0; the stage on row bit b uses twiddle
+// tw[(N >> k) | (j >> k)], k = b + logS + 1 (ntt.cpp:4-80): the thread's 2^(W-1-wb)
+// distinct (w, w/q) pairs of a stage are loaded (at most 2 at W = 2).
+template <bool INV, int LOGR, int W, int RD>
+__device__ __forceinline__ void round_w(double (&x)[1 << W], const double *__restrict__ tw, double q, double qinv,
+                                        u32 logN, u32 logS, u32 j0) {
+    using R = RoundW<INV, LOGR, W, RD>;
+    constexpr int E = 1 << W;
+    const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
+#pragma unroll
+    for (int st = 0; st < R::NS; ++st) {
+        const int b = INV ? R::BIT0 + st : R::BIT0 - st;
+        const int wb = b - R::P;
+        const u32 k = (u32)b + logS + 1;
+        const u32 wbase = ((1u << logN) >> k) | (j0 >> k);
+        constexpr int MAXP = E / 2;
+        double w[MAXP], wq[MAXP];
+        const int np = E >> (wb + 1);
+#pragma unroll
+        for (int t = 0; t < MAXP; ++t) {
+            if (t < np) {
+                const double2 pr = tw2[wbase | (u32)t];
+                w[t] = pr.x;
+                wq[t] = pr.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            if (u & (1 << wb)) continue;
+            const int u2 = u | (1 << wb);
+            const double ww = w[u >> (wb + 1)], wwq = wq[u >> (wb + 1)];
+            if (!INV) {
+                const double rr = mulmod_f64(x[u2], ww, wwq, q);
+                const double a0 = x[u];
+                x[u] = a0 + rr;
+                x[u2] = a0 - rr;
+            } else {
+                const double a0 = x[u], b0 = x[u2];
+                x[u] = reduce_f64(a0 + b0, q, qinv);
+                x[u2] = mulmod_f64(a0 - b0, ww, wwq, q);
+            }
+        }
+    }
+    // forward values are centred after every second round and after the last: from
+    // |x| <= q/2 + 2 a stage grows the bound B to 1.25 B + q/2 (canonical twiddles,
+    // mulmod_f64's bound), so four stages stay below 4.1q and six below 7.53q < 8q
+    if (!INV && ((RD & 1) || RD == R::NR - 1)) {
+#pragma unroll
+        for (int u = 0; u < E; ++u) x[u] = reduce_f64(x[u], q, qinv);
+    }
+}
+
+// Exchange between round RDP and round RDN (same direction) through one of the two
+// LDS buffers; the buffers alternate, so one barrier per exchange suffices (a
+// buffer is written again only after the barrier of the exchange that followed
+// its last reads).
+template <bool INV, int LOGR, int W, int LOGC, int RDP, int RDN>
+__device__ __forceinline__ void exch(double (&x)[1 << W], double *buf, u32 tid) {
+    constexpr int E = 1 << W;
+    constexpr u32 PP = LOGC + RoundW<INV, LOGR, W, RDP>::P, PN = LOGC + RoundW<INV, LOGR, W, RDN>::P;
+    const u32 ebp = elem_base<W>(tid, PP), ebn = elem_base<W>(tid, PN);
+#pragma unroll
+    for (int u = 0; u < E; ++u) buf[lds_pad(ebp | ((u32)u << PP))] = x[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < E; ++u) x[u] = buf[lds_pad(ebn | ((u32)u << PN))];
+}
+
+// global word index (within the limb) of tile element e: row e >> LOGC, column c0 + (e mod 2^LOGC)
+template <int LOGC>
+__device__ __forceinline__ u32 col_gidx(u32 e, u32 logS, u32 c0) {
+    return ((e >> LOGC) << logS) + c0 + (e & ((1u << LOGC) - 1));
+}
+
+// All rounds of a column transform after round 0 (exchanges included); `xb` is the
+// running LDS buffer index.
+template <bool INV, int LOGR, int W, int LOGC, int RD>
+__device__ __forceinline__ void rounds_from(double (&x)[1 << W], double *lds, u32 &xb, u32 tid,
+                                            const double *__restrict__ tw, double q, double qinv, u32 logN,
+                                            u32 logS, u32 c0) {
+    constexpr int NR = RoundW<INV, LOGR, W, 0>::NR;
+    if constexpr (RD < NR) {
+        constexpr u32 TILE = 1u << (LOGC + LOGR);
+        constexpr u32 PADT = TILE + TILE / 16;
+        exch<INV, LOGR, W, LOGC, RD - 1, RD>(x, lds + xb * PADT, tid);
+        xb ^= 1u;
+        constexpr u32 PN = LOGC + RoundW<INV, LOGR, W, RD>::P;
+        round_w<INV, LOGR, W, RD>(x, tw, q, qinv, logN, logS, col_gidx<LOGC>(elem_base<W>(tid, PN), logS, c0));
+        rounds_from<INV, LOGR, W, LOGC, RD + 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+    }
+}
+
+// Work item = (polynomial, digit, tile of 2^LOGC columns).  With LOGC < 4 the tiles
+// of a 16-column block are items 8 apart, so on a persistent grid of a multiple of
+// 8 * 2^(4-LOGC) workgroups (dispatched round-robin over the 8 XCDs) they run at the
+// same time on one XCD and share its L2 lines (128-byte rows).
+// WV > 0: register budget of WV waves per SIMD (the compiler's choice otherwise)
+template <int LOGR, int W, int LOGC, bool MDRS, int WV>
+__global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu_waves_per_eu(WV > 0 ? WV : 1))) void k_conv_cols_f64(
+    ConvColsArgs a, const double *__restrict__ twf, const double *__restrict__ itwf,
+    const ModParams *__restrict__ mp, u32 logN, u32 n_work) {
+    constexpr int E = 1 << W;
+    constexpr u32 TILE = 1u << (LOGC + LOGR);
+    constexpr u32 PADT = TILE + TILE / 16;
+    constexpr int NR = RoundW<true, LOGR, W, 0>::NR;
+    constexpr u32 PI0 = LOGC + RoundW<true, LOGR, W, 0>::P;       // input layout (inverse round 0)
+    constexpr u32 PC = LOGC + RoundW<false, LOGR, W, 0>::P;       // conversion layout
+    constexpr u32 PO = LOGC + RoundW<false, LOGR, W, NR - 1>::P;  // output layout (forward last round)
+    static_assert(RoundW<true, LOGR, W, NR - 1>::P == RoundW<false, LOGR, W, 0>::P, "windows must meet");
+    static_assert(W <= 3, "forward values are centred every 2W <= 6 stages (bound 7.53q < 8q)");
+    __shared__ double lds[2 * PADT];
+    const u32 tid = threadIdx.x;
+    const u32 logS = logN - LOGR;
+    const u32 ntiles = 1u << (logS - LOGC);
+    const u32 ebi = elem_base<W>(tid, PI0), ebc = elem_base<W>(tid, PC), ebo = elem_base<W>(tid, PO);
+    constexpr u32 LG = (LOGC < 4) ? 4 - LOGC : 0;  // log2 tiles per 16-column block
+    const u32 n_round = (n_work + (8u << LG) - 1) & ~((8u << LG) - 1);
+    u32 xb = 0;
+    for (u32 w = blockIdx.x; w < n_round; w += gridDim.x) {
+        const u32 tt = (w & ~((8u << LG) - 1)) | ((w & 7u) << LG) | ((w >> 3) & ((1u << LG) - 1));
+        if (tt >= n_work) continue;
+        const u32 tI = tt % ntiles, unit = tt / ntiles;
+        const u32 d = unit % a.ndig, poly = unit / a.ndig;
+        const u32 c0 = tI << LOGC;
+        const ConvDesc D = a.descs[d];
+        const u32 n_in = D.n_in, n_out = D.n_out;
+        const auto *src = gptr(a.in + (size_t)poly * a.in_pstride + ((size_t)a.in_first[d] << logN));
+        const u32 jin = col_gidx<LOGC>(ebi, logS, c0);
+        const u32 rstep_in = 1u << (PI0 - LOGC);  // row distance of consecutive elements (in rows)
+        // ---- column INTT pass of every input limb; y[i][u]: [x_i N^-1 qhat_i^-1]_{q_i}
+        double y[CC_MAX_IN][E];
+        double raw[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) raw[u] = src[jin + (((u32)u * rstep_in) << logS)];
+#pragma unroll
+        for (int i = 0; i < (int)CC_MAX_IN; ++i) {
+            if ((u32)i < n_in) {
+                double x[E];
+#pragma unroll
+                for (int u = 0; u < E; ++u) x[u] = raw[u];
+                if ((u32)i + 1 < n_in) {
+                    const auto *s1 = src + ((size_t)(i + 1) << logN);
+#pragma unroll
+                    for (int u = 0; u < E; ++u) raw[u] = s1[jin + (((u32)u * rstep_in) << logS)];
+                }
+                const u32 pi = D.in_prime[i];
+                const double q = mp[pi].qd, qinv = mp[pi].qinv;
+                const double *tw = itwf + ((size_t)pi << (logN + 1));
+                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
+                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+                const double cs = D.in_ncstf[2 * i], csq = D.in_ncstf[2 * i + 1];
+#pragma unroll
+                for (int u = 0; u < E; ++u) {
+                    const double v = reduce_f64(mulmod_f64(x[u], cs, csq, q), q, qinv);
+                    y[i][u] = v < 0.0 ? v + q : v;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < E; ++u) y[i][u] = 0.0;
+            }
+        }
+        // ---- MDRS prologue operand: the centred lift of the rescaled limb at these positions
+        const u32 jc = col_gidx<LOGC>(ebc, logS, c0);
+        const u32 rstep_c = 1u << (PC - LOGC);
+        double cl[E];
+        if (MDRS) {
+            const auto *vp = gptr(a.v + (size_t)poly * a.v_stride);
+            const u64 half = a.ql >> 1;
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                const u64 vw = vp[jc + (((u32)u * rstep_c) << logS)];
+                cl[u] = (vw > half) ? u64_to_f64(vw) - (double)a.ql : u64_to_f64(vw);
+            }
+        }
+        // ---- every output limb: conversion, then the forward column pass
+        const u32 jo = col_gidx<LOGC>(ebo, logS, c0);
+        const u32 rstep_o = 1u << (PO - LOGC);
+        const u32 jend = (a.j0 + a.nj < n_out) ? a.j0 + a.nj : n_out;
+        double *dst_base = a.out + (size_t)poly * a.out_pstride + (size_t)d * a.out_dstride;
+        for (u32 j = a.j0; j < jend; ++j) {
+            const u32 pj = D.out_prime[j];
+            const double q = mp[pj].qd, qinv = mp[pj].qinv;
+            double x[E];
+#pragma unroll
+            for (int u = 0; u < E; ++u) x[u] = 0.0;
+#pragma unroll
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) {
+                if ((u32)i < n_in) {
+                    const double h = D.hatf[2 * ((size_t)i * n_out + j)], hq = D.hatf[2 * ((size_t)i * n_out + j) + 1];
+#pragma unroll
+                    for (int u = 0; u < E; ++u) {
+                        x[u] += mulmod_f64(y[i][u], h, hq, q);
+                        if ((i & 3) == 3) x[u] = reduce_f64(x[u], q, qinv);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < E; ++u) x[u] = reduce_f64(x[u], q, qinv);
+            if (MDRS) {
+                // c*T_j - conv_j*(P q_l)^-1 (output j is Q slot j)
+                const double T = a.cst_in[2 * j], Tq = a.cst_in[2 * j + 1];
+                const double pq = a.cst_pq[2 * j], pqq = a.cst_pq[2 * j + 1];
+#pragma unroll
+                for (int u = 0; u < E; ++u)
+                    x[u] = reduce_f64(mulmod_f64(cl[u], T, Tq, q) - mulmod_f64(x[u], pq, pqq, q), q, qinv);
+            }
+            const double *tw = twf + ((size_t)pj << (logN + 1));
+            round_w<false, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jc);
+            rounds_from<false, LOGR, W, LOGC, 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+            auto *dst = gptr_w(dst_base + ((size_t)j << logN));
+#pragma unroll
+            for (int u = 0; u < E; ++u) dst[jo + (((u32)u * rstep_o) << logS)] = x[u];
+        }
+    }
+}
+
+template <int LOGR, int W, int LOGC, bool MDRS, int WV>
+static void launch_cc_inst(const DevTables &t, const ConvColsArgs &a, hipStream_t s) {
+    constexpr u32 TPB = (1u << (LOGC + LOGR)) >> W;
+    const u32 ntiles = 1u << (t.logN - LOGR - LOGC);
+    const u32 n_work = a.npolys * a.ndig * ntiles;
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_cols_f64<LOGR, W, LOGC, MDRS, WV>, TPB, 0) !=
+                hipSuccess ||
+            n <= 0)
+            n = 1;
+        per_cu = n;
+    }
+    // persistent grid: all resident, a multiple of 8 * (tiles per 16-column block)
+    constexpr u32 GR = 8u << ((LOGC < 4) ? 4 - LOGC : 0);
+    u32 nwg = t.max_wg / 4 * (u32)per_cu;
+    const u32 need = (n_work + GR - 1) & ~(GR - 1);
+    if (nwg > need) nwg = need;
+    nwg = nwg >= GR ? (nwg & ~(GR - 1)) : nwg;
+    hipLaunchKernelGGL((k_conv_cols_f64<LOGR, W, LOGC, MDRS, WV>), dim3(nwg), dim3(TPB), 0, s, a, t.twf, t.itwf,
+                       t.mp, t.logN, n_work);
+}
+
+// column-pass length of the two-pass FP64 transforms when forward and inverse agree
+static u32 cols_log(const DevTables &t) {
+    const u32 lg = t.logN;
+    const u32 kf = t.k1_fwd ? t.k1_fwd : lg / 2, ki = t.k1_inv ? t.k1_inv : lg / 2;
+    if ((lg == 13 || lg == 14) && (kf == lg || ki == lg)) return 0;  // one-pass transforms
+    return kf == ki ? kf : 0;
+}
+
+bool conv_cols_fusable(const DevTables &t, u32 max_in) {
+    if (!t.use_f64 || (t.f64_off & (1 | 2 | 512)) || t.logN <= 12 || max_in > CC_MAX_IN) return false;
+    const u32 lr = cols_log(t);
+    return lr >= 6 && lr <= 9 && t.logN - lr >= 4;
+}
+
+// Variants (DevTables::cc_var, FHEA_CC_VARIANT): 0 radix-4 rounds on 4-column
+// tiles (256 threads at 2^8 rows, within 3 waves per SIMD: three workgroups per
+// CU); 1 8-column tiles (512 threads) at the compiler's register choice (one
+// workgroup per CU); 2 8-column tiles within 4 waves per SIMD (spills).  Measured
+// on MI355X (HMult at 2^16, B = 64, one lane): 6,100 / 5,550 / 5,260 ops/s.
+template <bool MDRS>
+static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hipStream_t s) {
+#define FHEA_CC(LR)                                                                   \
+    case LR:                                                                          \
+        if (t.cc_var == 1)                                                            \
+            launch_cc_inst<LR, 2, 3, MDRS, 0>(t, a, s);                               \
+        else if (t.cc_var == 2)                                                       \
+            launch_cc_inst<LR, 2, 3, MDRS, 4>(t, a, s);                               \
+        else                                                                          \
+            launch_cc_inst<LR, 2, 2, MDRS, 3>(t, a, s);                               \
+        break;
+    switch (lr) {
+        FHEA_CC(6)
+        FHEA_CC(7)
+        FHEA_CC(8)
+        FHEA_CC(9)
+    default:
+        break;
+    }
+#undef FHEA_CC
+}
+
+void launch_conv_cols_f64(const DevTables &t, const ConvColsArgs &a, hipStream_t s) {
+    const u32 lr = cols_log(t);
+    if (a.v)
+        launch_cc_var<true>(t, a, lr, s);
+    else
+        launch_cc_var<false>(t, a, lr, s);
+}
+
+}  // namespace fhea

@@ -162,6 +162,7 @@ struct Blob {
             fix.push_back(b + offsetof(ConvDesc, hat));
             if (v[i].in_cstf) fix.push_back(b + offsetof(ConvDesc, in_cstf));
             if (v[i].hatf) fix.push_back(b + offsetof(ConvDesc, hatf));
+            if (v[i].in_ncstf) fix.push_back(b + offsetof(ConvDesc, in_ncstf));
         }
         return off;
     }
@@ -312,6 +313,11 @@ static int build_device_tables(fhea_ctx *c) {
                     }
                 D.in_cstf = off_ptr<const double>(b.addv(f64_pairs(cst, inm)));
                 D.hatf = off_ptr<const double>(b.addv(hf));
+                // N^-1 qhat_i^-1 mod q_i: the INTT's last scaling and the conversion's
+                // first product in one (fused column passes)
+                std::vector<u64> ncst(2 * sz);
+                for (u32 i = 0; i < sz; ++i) ncst[2 * i] = mulm(h.ninv[inp[i]], cst[2 * i], inm[i]);
+                D.in_ncstf = off_ptr<const double>(b.addv(f64_pairs(ncst, inm)));
             }
         }
         lo[l].modup_desc = b.add_descs(descs);
@@ -371,6 +377,9 @@ static int build_device_tables(fhea_ctx *c) {
                     }
                 md[0].in_cstf = off_ptr<const double>(b.addv(f64_pairs(cst, pm)));
                 md[0].hatf = off_ptr<const double>(b.addv(hf));
+                std::vector<u64> ncst(2 * K);
+                for (u32 j = 0; j < K; ++j) ncst[2 * j] = mulm(h.ninv[L + j], cst[2 * j], pm[j]);
+                md[0].in_ncstf = off_ptr<const double>(b.addv(f64_pairs(ncst, pm)));
             }
             // (value, value/q) forms for the FP64 fused transform ends (on a mixed
             // chain used for the limbs whose prime is below 2^50 only)
@@ -437,6 +446,7 @@ static int build_device_tables(fhea_ctx *c) {
             c->dt.max_wg = 4u * (u32)cus;
     }
     c->dt.f64_off = c->f64_off;
+    c->dt.cc_var = c->cc_var;
     c->dt.ntt_mixed = mixed && !c->force_int;
     {
         u32 qb = 0;
@@ -524,6 +534,75 @@ static void moddown_intt(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb, bo
     launch_ntt(t, true, acc + l * N, pb, (l + K) * N, K * N, 2 * nb, K, t.ident + t.L, c->stream);
 }
 
+// The fused conversion column kernel applies to the ModUp of level l (digits of
+// at most CC_MAX_IN limbs) and to the ModDown (K limbs in); FHEA_F64_OFF bit 512
+// keeps the INTT column pass, the conversion and the NTT column pass as three launches.
+static bool modup_cc(const fhea_ctx *c, u32 l) { return conv_cols_fusable(c->dt, std::min(l, c->h.alpha)); }
+static bool moddown_cc(const fhea_ctx *c) { return conv_cols_fusable(c->dt, c->dt.K); }
+
+// ModUp of npolys inputs through the fused column kernel: coef holds the INTT's
+// first-pass intermediates of input z at coef + z*l*N (written by the caller), and
+// compact receives every digit's converted limbs after the forward transform's first
+// (column) pass, item z at compact + z*cz.
+static void modup_cols(fhea_ctx *c, u32 l, u32 npolys, const u64 *coef, u64 *compact) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const u32 np = c->lv[l].nparts;
+    ConvColsArgs a;
+    a.in = (const double *)coef;
+    a.in_pstride = l * N;
+    a.in_first = c->modup_first_dev[l];
+    a.descs = c->modup_desc_dev[l];
+    a.out = (double *)compact;
+    a.out_pstride = (u64)np * (l + K) * N;
+    a.out_dstride = (l + K) * N;
+    a.j0 = 0;
+    a.nj = l + (u32)K;
+    a.npolys = npolys;
+    a.ndig = np;
+    launch_conv_cols_f64(t, a, c->stream);
+}
+
+// The ModDown's conversion P -> Q with its INTT column pass before it and the NTT
+// column pass after it: pb holds the first-pass (rows) intermediates of the P part
+// of the 2*nb accumulators, tt receives Q slots j0 .. j0+nj-1 (poly at tt + poly*l*N);
+// with v the transform input is the ModDown + rescale prologue (FUSE_MDRS_IN).
+static void moddown_cols(fhea_ctx *c, u32 l, u32 nb, const u64 *pb, u64 *tt, u32 j0, u32 nj,
+                         const u64 *v = nullptr) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    const LevelTables &lt = c->lv[l];
+    ConvColsArgs a;
+    a.in = (const double *)pb;
+    a.in_pstride = K * N;
+    a.in_first = c->zero_u32;
+    a.descs = c->moddown_desc_dev[l];
+    a.out = (double *)tt;
+    a.out_pstride = l * N;
+    a.out_dstride = 0;
+    a.j0 = j0;
+    a.nj = nj;
+    a.npolys = 2 * nb;
+    a.ndig = 1;
+    if (v) {
+        a.v = v;
+        a.v_stride = N;
+        a.ql = c->h.primes[l - 1];
+        a.cst_in = lt.rs_qlqlf;
+        a.cst_pq = lt.rs_pqf;
+    }
+    launch_conv_cols_f64(t, a, c->stream);
+}
+
+// First (rows) pass of the ModDown INTT of the P part of nb accumulator pairs into pb.
+static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
+    const DevTables &t = c->dt;
+    const u64 N = t.N, K = t.K;
+    NttFuse fz;
+    fz.only_pass = 1;
+    launch_ntt_f64(t, true, acc + l * N, pb, (l + K) * N, K * N, N, N, 2 * nb, K, t.ident + t.L, fz, c->stream);
+}
+
 // Key switch of nb inputs c_z = c_base + z*c_stride (l limbs, eval domain):
 // out0_z = add0_z + ks0, out1_z = add1_z + ks1 (key_switch.cpp:167-439 followed by
 // the composite's poly_add).  keys[z] = device pointer of item z's key.
@@ -542,9 +621,19 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     u64 *tt = pb + nb * 2 * K * N;
     const u64 cz = (u64)np * (l + K) * N;
     // ModUp: INTT of every input limb, convert each digit, NTT the converted limbs
-    launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
-    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K,
-               coef, l * N, compact, (l + K) * N, cz, nb, s);
+    // (fused: the INTT's rows pass, then one kernel for its column pass, the
+    // conversion and the NTT's column pass)
+    const bool cc = modup_cc(c, l);
+    if (cc) {
+        NttFuse fz;
+        fz.only_pass = 1;
+        launch_ntt_f64(t, true, c_base, coef, c_stride, l * N, N, N, nb, l, t.ident, fz, s);
+        modup_cols(c, l, nb, coef, compact);
+    } else {
+        launch_ntt(t, true, c_base, coef, c_stride, l * N, nb, l, t.ident, s);
+        launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K,
+                   coef, l * N, compact, (l + K) * N, cz, nb, s);
+    }
     std::vector<KsIpArgs> ip(nb);
     for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
     auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
@@ -554,11 +643,13 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     if (fused) {
         // first forward pass of the converted digit limbs; the second runs inside the
         // inner product, which for the P slots also runs the ModDown INTT's first pass
-        NttFuse fz;
-        fz.slots = lt.modup_slots;
-        fz.nslots = lt.n_modup_slots;
-        fz.only_pass = 1;
-        launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+        if (!cc) {
+            NttFuse fz;
+            fz.slots = lt.modup_slots;
+            fz.nslots = lt.n_modup_slots;
+            fz.only_pass = 1;
+            launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
+        }
         const KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
                             key_limbs, c->h.dnum, nb};
         launch_ks_rows_f64(t, ka, s);
@@ -589,6 +680,7 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
             NttFuse fz;  // only the converted slots of each digit
             fz.slots = lt.modup_slots;
             fz.nslots = lt.n_modup_slots;
+            if (cc) fz.only_pass = 2;  // the column pass ran in the fused conversion
             launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
         } else {
             launch_ntt(t, false, compact, compact, cz, cz, nb, np * (l + K), lt.modup_ntt_prime, s);
@@ -616,8 +708,15 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
     hipStream_t s = c->stream;
-    moddown_intt(c, l, nb, acc, pb, pb_mid);
-    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    const bool cc = t.use_f64 && !(t.f64_off & 17) && moddown_cc(c);
+    if (cc) {
+        // INTT rows pass, then its column pass + conversion + the NTT column pass fused
+        if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
+        moddown_cols(c, l, nb, pb, tt, 0, l);
+    } else {
+        moddown_intt(c, l, nb, acc, pb, pb_mid);
+        launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    }
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     if (t.use_f64 && !(t.f64_off & 17)) {
@@ -628,6 +727,7 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
         fz.cb = cb_d;
         fz.acc = acc;
         fz.acc_stride = (l + K) * N;
+        if (cc) fz.only_pass = 2;
         launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, fz, s);
     } else if (mixed_fused(t)) {
         // mixed chain: the fused FP64 transform on the limbs below 2^50, the integer
@@ -736,8 +836,17 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         if (st) return st;
         return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
     }
-    moddown_intt(c, l, nb, acc, pb, pb_mid);
-    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    // with the fused column kernel: the INTT rows pass, then the conversion of the
+    // last limb alone (its ModDown result feeds the rescale's lift) and, once the lift
+    // exists, the conversion of the other limbs with the ModDown + rescale prologue
+    const bool cc = !mix && moddown_cc(c);
+    if (cc) {
+        if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
+        moddown_cols(c, l, nb, pb, tt, l - 1, 1);
+    } else {
+        moddown_intt(c, l, nb, acc, pb, pb_mid);
+        launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+    }
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     // limb l-1 of the ModDown result, (acc - NTT(conv)) * P^-1 + d, in place in d
@@ -750,6 +859,7 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f1.slots = t.ident + (l - 1);
     f1.nslots = 1;
     if (mix) f1.pmask = t.pmask_small;
+    if (cc) f1.only_pass = 2;
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, f1, s);
     // its coefficients
     launch_ntt(t, true, ra.d_base + (l - 1) * N, v, ra.dz, 2 * N, nb, 2, c->rs_last_map[l], s, ra.ds, N);
@@ -771,6 +881,10 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f2.acc = acc;
     f2.acc_stride = (l + K) * N;
     if (mix) f2.pmask = t.pmask_small;
+    if (cc) {
+        moddown_cols(c, l, nb, pb, tt, 0, l - 1, v);
+        f2.only_pass = 2;
+    }
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l - 1, t.ident, f2, s);
     if (mix && t.q_big_prefix) {
         // the prefix: its ModDown + add in place in d, then its rescale; its converted
@@ -947,18 +1061,35 @@ static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const std::vecto
     bool strided = ni > 1;
     const long long step = ni > 1 ? (long long)(inputs[1].in1 - inputs[0].in1) : 0;
     for (u32 k = 1; k < ni && strided; ++k) strided = (long long)(inputs[k].in1 - inputs[k - 1].in1) == step;
+    // (with the fused column kernel: the INTT's rows pass only, then its column pass,
+    // the conversion and the NTT's column pass in one launch over all inputs)
+    const bool cc = modup_cc(c, l);
+    NttFuse rows;
+    rows.only_pass = 1;
     if (strided && step > 0) {
-        launch_ntt(t, true, inputs[0].in1, coef, (u64)step, l * N, ni, l, t.ident, s);
+        if (cc)
+            launch_ntt_f64(t, true, inputs[0].in1, coef, (u64)step, l * N, N, N, ni, l, t.ident, rows, s);
+        else
+            launch_ntt(t, true, inputs[0].in1, coef, (u64)step, l * N, ni, l, t.ident, s);
     } else {
-        for (u32 k = 0; k < ni; ++k) launch_ntt(t, true, inputs[k].in1, coef + (u64)k * l * N, l * N, l * N, 1, l,
-                                                t.ident, s);
+        for (u32 k = 0; k < ni; ++k) {
+            if (cc)
+                launch_ntt_f64(t, true, inputs[k].in1, coef + (u64)k * l * N, l * N, l * N, N, N, 1, l, t.ident, rows,
+                               s);
+            else
+                launch_ntt(t, true, inputs[k].in1, coef + (u64)k * l * N, l * N, l * N, 1, l, t.ident, s);
+        }
     }
-    launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K, coef,
-               l * N, compact, (l + K) * N, cz, ni, s);
+    if (cc)
+        modup_cols(c, l, ni, coef, compact);
+    else
+        launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K, coef,
+                   l * N, compact, (l + K) * N, cz, ni, s);
     if (t.use_f64 && !(t.f64_off & 1)) {
         NttFuse fz;
         fz.slots = lt.modup_slots;
         fz.nslots = lt.n_modup_slots;
+        if (cc) fz.only_pass = 2;
         launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, ni, np * (l + K), lt.modup_ntt_prime, fz, s);
     } else {
         launch_ntt(t, false, compact, compact, cz, cz, ni, np * (l + K), lt.modup_ntt_prime, s);
@@ -1112,6 +1243,8 @@ int fhea_ctx_create(const fhea_params *p, fhea_ctx **out) {
         c->force_int = fi && atoi(fi) != 0;
         const char *fo = getenv("FHEA_F64_OFF");
         c->f64_off = fo ? (u32)strtoul(fo, nullptr, 0) : 0;
+        const char *cv = getenv("FHEA_CC_VARIANT");
+        c->cc_var = cv ? (u32)strtoul(cv, nullptr, 0) : 0;
     }
     c->device = (int)p->device;
     int st = FHEA_OK;

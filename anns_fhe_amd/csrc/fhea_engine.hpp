@@ -86,6 +86,7 @@ struct fhea_ctx {
     u32 max_chunk = 16;
     bool force_int = false;  // FHEA_FORCE_INT=1: integer kernels even for < 2^50 primes
     u32 f64_off = 0;         // FHEA_F64_OFF: per-component switch (DevTables::f64_off)
+    u32 cc_var = 0;          // FHEA_CC_VARIANT: fused conversion column kernel variant
 };
 
 namespace fhea {

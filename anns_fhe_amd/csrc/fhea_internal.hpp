@@ -35,6 +35,7 @@ struct ConvDesc {
     // FP64 forms (only when every prime is below 2^50): (value, value / modulus)
     const double *in_cstf = nullptr; // [2*n_in]
     const double *hatf = nullptr;    // [2*n_in*n_out]
+    const double *in_ncstf = nullptr; // [2*n_in] (N^-1 qhat_i^-1 mod q_i, / q_i): INTT scaling and conversion in one product
 };
 
 // Per-level tables (sizeQl = l).

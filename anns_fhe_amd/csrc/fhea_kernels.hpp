@@ -20,9 +20,13 @@ struct DevTables {
     // 16 fused transform ends, 32 rescale / ModDown combines, 64 the inner
     // product fused into the transforms around it; 128 is a switch ON: with the
     // fused inner product, the P slots also run the ModDown INTT's first pass there;
-    // 256: slot-major tile walk for the first pass of rescale-lift prologues
+    // 256: slot-major tile walk for the first pass of rescale-lift prologues;
+    // 512: INTT column pass, basis conversion and NTT column pass as three launches
+    // instead of the fused conversion kernel (fhea_conv_cols.hip)
     bool use_f64 = false;
     u32 f64_off = 0;
+    // kernel variant of the fused conversion column kernel (FHEA_CC_VARIANT)
+    u32 cc_var = 0;
     u32 max_wg = 1024;  // persistent-kernel grid: resident workgroups (CUs x 4)
     // tuning (fhea_ctx_set_tuning): stages of the first pass of a two-pass FP64
     // forward / inverse NTT (0 = logN/2), persistent workgroups per CU (0 = occupancy)
@@ -106,6 +110,37 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
 // the same over Q slots slot0 .. slot0+nslots-1 only (mixed chains: the Q slots
 // whose primes are below 2^50)
 void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s);
+
+// Basis conversion fused between the column passes of the transforms around it
+// (fhea_conv_cols.hip; FP64 path, two-pass N, forward and inverse column passes
+// of equal length).  Unit (polynomial, digit d): the n_in input limbs at
+// in + poly*in_pstride + (in_first[d] + i)*N hold the INTT's first-pass (rows)
+// intermediates; output j of descs[d] (for j in j0 .. j0+nj-1) receives the
+// forward transform's first-pass (columns) intermediate of the converted limb at
+// out + poly*out_pstride + d*out_dstride + j*N, ready for a second pass
+// (only_pass = 2, or the fused key-switch rows kernel).  With v set (ModDown +
+// rescale), output j is Q slot j and its transform input is
+// c*T_j - conv_j*(P q_l)^-1, c = the centred lift of v (FUSE_MDRS_IN's prologue).
+static constexpr u32 CC_MAX_IN = 8;  // inputs per digit held in registers
+struct ConvColsArgs {
+    const double *in;
+    u64 in_pstride;
+    const u32 *in_first;
+    const ConvDesc *descs;
+    double *out;
+    u64 out_pstride, out_dstride;
+    u32 j0, nj;
+    u32 npolys, ndig;
+    const u64 *v = nullptr;  // MDRS: INTT'd last limb of poly at v + poly*v_stride
+    u64 v_stride = 0;
+    u64 ql = 0;
+    const double *cst_in = nullptr;  // MDRS: (T_j, T_j/q_j)
+    const double *cst_pq = nullptr;  // MDRS: ((P q_l)^-1, /q_j)
+};
+// true when launch_conv_cols_f64 has a kernel for the context's splits and
+// conversions of up to max_in inputs
+bool conv_cols_fusable(const DevTables &t, u32 max_in);
+void launch_conv_cols_f64(const DevTables &t, const ConvColsArgs &a, hipStream_t s);
 
 // Batched negacyclic NTT / INTT (ntt.cpp:4-80).  Polynomial z of the batch
 // has limb slot s at src + z*src_stride + s*src_slot (src_slot 0 = N); slot s

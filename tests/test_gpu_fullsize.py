@@ -181,19 +181,22 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
         ctx.close()
 
 
-@pytest.mark.parametrize("switch", [0, 128, 256, 64, 1 | 2 | 4 | 8 | 16 | 32])
-def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
+@pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (128, 0), (256, 0), (64, 0), (512, 0),
+                                               (512 | 128, 0), (1 | 2 | 4 | 8 | 16 | 32, 0)])
+def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
     P slots' ModDown INTT pass inside the fused inner product; 256: the slot-major
     tile order (instead of unit-major) for the passes whose prologue reads the
-    rescale lift; 64:
-    the inner product as its own kernel; 1..32: each component on the integer
-    kernels).  Every setting
+    rescale lift; 64: the inner product as its own kernel; 512: the INTT column
+    pass, basis conversion and NTT column pass as three launches instead of the
+    fused conversion kernel; 1..32: each component on the integer kernels), and
+    FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting
     gives the oracle's words for a batched HMult (3 items, one lane and two) and a
     rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
     monkeypatch.setenv("FHEA_F64_OFF", str(switch))
+    monkeypatch.setenv("FHEA_CC_VARIANT", str(cc_variant))
     N, L, K, dnum = 16384, 8, 2, 3
     W, W1 = L * N, (L - 1) * N
     ksz = 2 * dnum * (L + K) * N
@@ -201,7 +204,7 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
     try:
         drv = FHEHostDriver(ctx)
         o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
-        rng = np.random.default_rng(300 + switch)
+        rng = np.random.default_rng(300 + switch + 7 * cc_variant)
         relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
         km = np.concatenate([relin, rot])
         ctx.write_key(0, km)
@@ -217,7 +220,7 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch):
                       b1=(4 * z + 3) * W, d0=(12 + 2 * z) * W, d1=(13 + 2 * z) * W) for z in range(B)]
         for streams in (1, 2):
             ctx.set_tuning(_lib.TUNE_STREAMS, streams)
-            say("switch", switch, "streams", streams)
+            say("switch", switch, "cc_variant", cc_variant, "streams", streams)
             ctx.submit(works)
             for z in range(B):
                 assert o.eval_mult_relin_rescale(pm, km, 0, L, 4 * z * W, (4 * z + 1) * W, (4 * z + 2) * W,

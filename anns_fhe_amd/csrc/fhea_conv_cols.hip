@@ -122,6 +122,14 @@ __device__ __forceinline__ void exch(double (&x)[1 << W], double *buf, u32 tid) 
     for (int u = 0; u < E; ++u) x[u] = buf[lds_pad(ebn | ((u32)u << PN))];
 }
 
+// The thread index as a value the compiler cannot hoist: addresses derived from it
+// inside a loop are recomputed there (a few integer instructions) instead of being
+// kept live across the loop (dozens of registers for the exchanges' LDS addresses).
+__device__ __forceinline__ u32 opaque_tid(u32 v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // global word index (within the limb) of tile element e: row e >> LOGC, column c0 + (e mod 2^LOGC)
 template <int LOGC>
 __device__ __forceinline__ u32 col_gidx(u32 e, u32 logS, u32 c0) {
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     const u32 tid = threadIdx.x;
     const u32 logS = logN - LOGR;
     const u32 ntiles = 1u << (logS - LOGC);
-    const u32 ebi = elem_base<W>(tid, PI0), ebc = elem_base<W>(tid, PC), ebo = elem_base<W>(tid, PO);
+    const u32 ebi = elem_base<W>(tid, PI0);
     constexpr u32 LG = (LOGC < 4) ? 4 - LOGC : 0;  // log2 tiles per 16-column block
     const u32 n_round = (n_work + (8u << LG) - 1) & ~((8u << LG) - 1);
     u32 xb = 0;
@@ -194,16 +202,18 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                 double x[E];
 #pragma unroll
                 for (int u = 0; u < E; ++u) x[u] = raw[u];
+                const u32 tl = opaque_tid(tid);
+                const u32 jl = col_gidx<LOGC>(elem_base<W>(tl, PI0), logS, c0);
                 if ((u32)i + 1 < n_in) {
                     const auto *s1 = src + ((size_t)(i + 1) << logN);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) raw[u] = s1[jin + (((u32)u * rstep_in) << logS)];
+                    for (int u = 0; u < E; ++u) raw[u] = s1[jl + (((u32)u * rstep_in) << logS)];
                 }
                 const u32 pi = D.in_prime[i];
                 const double q = mp[pi].qd, qinv = mp[pi].qinv;
                 const double *tw = itwf + ((size_t)pi << (logN + 1));
-                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
-                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jl);
+                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
                 const double cs = D.in_ncstf[2 * i], csq = D.in_ncstf[2 * i + 1];
 #pragma unroll
                 for (int u = 0; u < E; ++u) {
@@ -215,21 +225,60 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                 for (int u = 0; u < E; ++u) y[i][u] = 0.0;
             }
         }
-        // ---- MDRS prologue operand: the centred lift of the rescaled limb at these positions
-        const u32 jc = col_gidx<LOGC>(ebc, logS, c0);
-        const u32 rstep_c = 1u << (PC - LOGC);
+        // ---- ModDown + rescale: the rescaled limb's ModDown result in coefficient form
+        // at these positions, by the linearity of the transforms (mod q_{l-1}):
+        //   v = INTT(d_{l-1}) + P^-1 (INTT(acc_{l-1}) - conv_{l-1}),
+        // from the rows-pass intermediates of acc_{l-1} and d_{l-1}, and its centred lift
+        // (rescale.cpp:46-54): the words of the ModDown written out, transformed back and lifted
         double cl[E];
         if (MDRS) {
-            const auto *vp = gptr(a.v + (size_t)poly * a.v_stride);
-            const u64 half = a.ql >> 1;
+            const u32 pl = a.lift_prime;
+            const double q = mp[pl].qd, qinv = mp[pl].qinv;
+            const double *tw = itwf + ((size_t)pl << (logN + 1));
+            const auto *lp = gptr(a.lift_in + (size_t)poly * a.lift_stride);
+            // lift_cst: (N^-1, /q), (P^-1 N^-1, /q), (P^-1, /q) mod q_{l-1}
+            const double ni = a.lift_cst[0], niq = a.lift_cst[1], pni = a.lift_cst[2], pniq = a.lift_cst[3];
+            const double pi_ = a.lift_cst[4], piq = a.lift_cst[5];
+            // one transform at a time, each folded into the partial sum vs (|terms| <= 0.82q)
+            double vs[E], x[E];
+#pragma unroll
+            for (int u = 0; u < E; ++u) x[u] = lp[jin + (((u32)u * rstep_in) << logS)];
+            round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
+            rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
+#pragma unroll
+            for (int u = 0; u < E; ++u) vs[u] = mulmod_f64(x[u], pni, pniq, q);
+#pragma unroll
+            for (int u = 0; u < E; ++u) x[u] = lp[(1u << logN) + jin + (((u32)u * rstep_in) << logS)];
+            round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
+            rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
+#pragma unroll
+            for (int u = 0; u < E; ++u) vs[u] += mulmod_f64(x[u], ni, niq, q);
+            // conv_{l-1}
+#pragma unroll
+            for (int u = 0; u < E; ++u) x[u] = 0.0;
+            const u32 lj = a.lift_j;
+#pragma unroll
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) {
+                if ((u32)i < n_in) {
+                    const double h = D.hatf[2 * ((size_t)i * n_out + lj)], hq = D.hatf[2 * ((size_t)i * n_out + lj) + 1];
+#pragma unroll
+                    for (int u = 0; u < E; ++u) {
+                        x[u] += mulmod_f64(y[i][u], h, hq, q);
+                        if ((i & 3) == 3) x[u] = reduce_f64(x[u], q, qinv);
+                    }
+                }
+            }
+            const double hq_l = 0.5 * q;
 #pragma unroll
             for (int u = 0; u < E; ++u) {
-                const u64 vw = vp[jc + (((u32)u * rstep_c) << logS)];
-                cl[u] = (vw > half) ? u64_to_f64(vw) - (double)a.ql : u64_to_f64(vw);
+                // |vs| <= 1.64q, |conv term| <= 0.63q: centred, then canonical, then lifted
+                const double cv = reduce_f64(x[u], q, qinv);
+                double v = reduce_f64(vs[u] - mulmod_f64(cv, pi_, piq, q), q, qinv);
+                v = v < 0.0 ? v + q : v;
+                cl[u] = v > hq_l ? v - q : v;
             }
         }
         // ---- every output limb: conversion, then the forward column pass
-        const u32 jo = col_gidx<LOGC>(ebo, logS, c0);
         const u32 rstep_o = 1u << (PO - LOGC);
         const u32 jend = (a.j0 + a.nj < n_out) ? a.j0 + a.nj : n_out;
         double *dst_base = a.out + (size_t)poly * a.out_pstride + (size_t)d * a.out_dstride;
@@ -261,8 +310,10 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                     x[u] = reduce_f64(mulmod_f64(cl[u], T, Tq, q) - mulmod_f64(x[u], pq, pqq, q), q, qinv);
             }
             const double *tw = twf + ((size_t)pj << (logN + 1));
-            round_w<false, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jc);
-            rounds_from<false, LOGR, W, LOGC, 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+            const u32 tl = opaque_tid(tid);
+            round_w<false, LOGR, W, 0>(x, tw, q, qinv, logN, logS, col_gidx<LOGC>(elem_base<W>(tl, PC), logS, c0));
+            rounds_from<false, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
+            const u32 jo = col_gidx<LOGC>(elem_base<W>(tl, PO), logS, c0);
             auto *dst = gptr_w(dst_base + ((size_t)j << logN));
 #pragma unroll
             for (int u = 0; u < E; ++u) dst[jo + (((u32)u * rstep_o) << logS)] = x[u];
@@ -309,15 +360,18 @@ bool conv_cols_fusable(const DevTables &t, u32 max_in) {
 }
 
 // Variants (DevTables::cc_var, FHEA_CC_VARIANT): 0 radix-4 rounds on 4-column
-// tiles (256 threads at 2^8 rows, within 3 waves per SIMD: three workgroups per
-// CU); 1 8-column tiles (512 threads) at the compiler's register choice (one
-// workgroup per CU); 2 8-column tiles within 4 waves per SIMD (spills).  Measured
-// on MI355X (HMult at 2^16, B = 64, one lane): 6,100 / 5,550 / 5,260 ops/s.
+// tiles (256 threads at 2^8 rows; within 3 waves per SIMD, which the conversion
+// without the lift meets at 4); 1 8-column tiles (512 threads) at the compiler's
+// register choice; 2 8-column tiles within 4 waves per SIMD; 3 4-column tiles within
+// 4 waves per SIMD (the lift instance spills).  Measured on MI355X (HMult at 2^16,
+// B = 64, one lane): 6,468 / 6,394 / 6,429 / 6,315 ops/s (5,903 unfused).
 template <bool MDRS>
 static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hipStream_t s) {
 #define FHEA_CC(LR)                                                                   \
     case LR:                                                                          \
-        if (t.cc_var == 1)                                                            \
+        if (t.cc_var == 3)                                                            \
+            launch_cc_inst<LR, 2, 2, MDRS, 4>(t, a, s);                               \
+        else if (t.cc_var == 1)                                                       \
             launch_cc_inst<LR, 2, 3, MDRS, 0>(t, a, s);                               \
         else if (t.cc_var == 2)                                                       \
             launch_cc_inst<LR, 2, 3, MDRS, 4>(t, a, s);                               \
@@ -337,7 +391,7 @@ static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hip
 
 void launch_conv_cols_f64(const DevTables &t, const ConvColsArgs &a, hipStream_t s) {
     const u32 lr = cols_log(t);
-    if (a.v)
+    if (a.lift_in)
         launch_cc_var<true>(t, a, lr, s);
     else
         launch_cc_var<false>(t, a, lr, s);

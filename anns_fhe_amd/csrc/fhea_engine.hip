@@ -177,7 +177,7 @@ static u64 mulm(u64 a, u64 b, u64 q) { return (u64)((u128)a * b % q); }
 
 struct LevelOffsets {
     size_t modup_desc, modup_first, modup_ntt_prime, modup_slots, moddown_desc, pinv, rs_qlinv, rs_qlql, rs_sub,
-        rs_last, pinvf, rs_qlinvf, rs_qlqlf, rs_pqf;
+        rs_last, pinvf, rs_qlinvf, rs_qlqlf, rs_pqf, rs_liftf;
 };
 
 // (value, value/modulus) pairs for the FP64 kernels, from (value, shoup) pairs
@@ -413,6 +413,12 @@ static int build_device_tables(fhea_ctx *c) {
                 lo[l].rs_qlinvf = b.addv(f64_pairs(qlinv, qm));
                 lo[l].rs_qlqlf = b.addv(f64_pairs(qlql, qm));
                 lo[l].rs_pqf = b.addv(f64_pairs(pq, qm));
+                // the rescaled limb's ModDown result from coefficient-domain parts
+                u64 pl = 1 % ql;
+                for (u32 k = 0; k < K; ++k) pl = mulm(pl, h.primes[L + k] % ql, ql);
+                const u64 pinv_l = inv_mod(pl, ql), ninv_l = h.ninv[l - 1];
+                std::vector<u64> lc = {ninv_l, 0, mulm(pinv_l, ninv_l, ql), 0, pinv_l, 0};
+                lo[l].rs_liftf = b.addv(f64_pairs(lc, std::vector<u64>(3, ql)));
             }
             lo[l].rs_qlinv = b.addv(qlinv);
             lo[l].rs_qlql = b.addv(qlql);
@@ -486,6 +492,7 @@ static int build_device_tables(fhea_ctx *c) {
             lt.rs_qlinvf = (const double *)P(lo[l].rs_qlinvf);
             lt.rs_qlqlf = (const double *)P(lo[l].rs_qlqlf);
             lt.rs_pqf = (const double *)P(lo[l].rs_pqf);
+            lt.rs_liftf = (const double *)P(lo[l].rs_liftf);
         }
     }
     return FHEA_OK;
@@ -565,10 +572,12 @@ static void modup_cols(fhea_ctx *c, u32 l, u32 npolys, const u64 *coef, u64 *com
 
 // The ModDown's conversion P -> Q with its INTT column pass before it and the NTT
 // column pass after it: pb holds the first-pass (rows) intermediates of the P part
-// of the 2*nb accumulators, tt receives Q slots j0 .. j0+nj-1 (poly at tt + poly*l*N);
-// with v the transform input is the ModDown + rescale prologue (FUSE_MDRS_IN).
+// of the 2*nb accumulators, tt receives Q slots j0 .. j0+nj-1 (poly at tt + poly*l*N).
+// With lift (ModDown + rescale by q_{l-1}): the rows-pass intermediates of acc_{l-1}
+// and d_{l-1} of poly p at lift + 2p*N (+ N), and the transform input is the
+// ModDown + rescale prologue (FUSE_MDRS_IN) with the lift formed in the kernel.
 static void moddown_cols(fhea_ctx *c, u32 l, u32 nb, const u64 *pb, u64 *tt, u32 j0, u32 nj,
-                         const u64 *v = nullptr) {
+                         const u64 *lift = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -584,10 +593,12 @@ static void moddown_cols(fhea_ctx *c, u32 l, u32 nb, const u64 *pb, u64 *tt, u32
     a.nj = nj;
     a.npolys = 2 * nb;
     a.ndig = 1;
-    if (v) {
-        a.v = v;
-        a.v_stride = N;
-        a.ql = c->h.primes[l - 1];
+    if (lift) {
+        a.lift_in = (const double *)lift;
+        a.lift_stride = 2 * N;
+        a.lift_prime = l - 1;
+        a.lift_j = l - 1;
+        a.lift_cst = lt.rs_liftf;
         a.cst_in = lt.rs_qlqlf;
         a.cst_pq = lt.rs_pqf;
     }
@@ -836,17 +847,39 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         if (st) return st;
         return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
     }
-    // with the fused column kernel: the INTT rows pass, then the conversion of the
-    // last limb alone (its ModDown result feeds the rescale's lift) and, once the lift
-    // exists, the conversion of the other limbs with the ModDown + rescale prologue
-    const bool cc = !mix && moddown_cc(c);
-    if (cc) {
+    std::vector<RescaleOut> ro(2 * nb);
+    for (u32 z = 0; z < nb; ++z)
+        for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {ra.d_base + z * ra.dz + k * ra.ds, (*ra.outs)[2 * z + k]};
+    if (!mix && moddown_cc(c)) {
+        // With the fused column kernel: the INTT rows passes of the P part and of the
+        // last limbs acc_{l-1}, d_{l-1}, then one kernel for the INTT column passes, the
+        // conversion, the rescaled limb's ModDown result and lift (in registers, by
+        // linearity) and the ModDown + rescale prologue with the forward column pass;
+        // then the rows pass with the epilogue.  lift (4*nb*N words) reuses the dead
+        // ModUp region at v, which extends into the dead digit region for l < 4.
+        NttFuse r1;
+        r1.only_pass = 1;
         if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
-        moddown_cols(c, l, nb, pb, tt, l - 1, 1);
-    } else {
-        moddown_intt(c, l, nb, acc, pb, pb_mid);
-        launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
+        u64 *lift = v;
+        launch_ntt_f64(t, true, acc + (l - 1) * N, lift, (l + K) * N, 2 * N, N, N, 2 * nb, 1, t.ident + (l - 1), r1, s);
+        launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2, c->rs_last_map[l],
+                       r1, s);
+        moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift);
+        auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
+        if (!ro_d) return FHEA_ERR_DEVICE;
+        NttFuse f2;
+        f2.epi = FUSE_MDRS_OUT;
+        f2.cst_out = lt.rs_qlinvf;
+        f2.cst_pq = lt.rs_pqf;
+        f2.ro = ro_d;
+        f2.acc = acc;
+        f2.acc_stride = (l + K) * N;
+        f2.only_pass = 2;
+        launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l - 1, t.ident, f2, s);
+        return FHEA_OK;
     }
+    moddown_intt(c, l, nb, acc, pb, pb_mid);
+    launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     // limb l-1 of the ModDown result, (acc - NTT(conv)) * P^-1 + d, in place in d
@@ -859,13 +892,9 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f1.slots = t.ident + (l - 1);
     f1.nslots = 1;
     if (mix) f1.pmask = t.pmask_small;
-    if (cc) f1.only_pass = 2;
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, f1, s);
     // its coefficients
     launch_ntt(t, true, ra.d_base + (l - 1) * N, v, ra.dz, 2 * N, nb, 2, c->rs_last_map[l], s, ra.ds, N);
-    std::vector<RescaleOut> ro(2 * nb);
-    for (u32 z = 0; z < nb; ++z)
-        for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {ra.d_base + z * ra.dz + k * ra.ds, (*ra.outs)[2 * z + k]};
     auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
     if (!ro_d) return FHEA_ERR_DEVICE;
     NttFuse f2;
@@ -881,10 +910,6 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     f2.acc = acc;
     f2.acc_stride = (l + K) * N;
     if (mix) f2.pmask = t.pmask_small;
-    if (cc) {
-        moddown_cols(c, l, nb, pb, tt, 0, l - 1, v);
-        f2.only_pass = 2;
-    }
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l - 1, t.ident, f2, s);
     if (mix && t.q_big_prefix) {
         // the prefix: its ModDown + add in place in d, then its rescale; its converted

@@ -56,6 +56,7 @@ struct LevelTables {
     // FP64 forms (value, value / q_i)
     const double *pinvf = nullptr, *rs_qlinvf = nullptr, *rs_qlqlf = nullptr;
     const double *rs_pqf = nullptr;        // [2*(l-1)] ((P q_l)^-1 mod q_i, / q_i)
+    const double *rs_liftf = nullptr;      // [6] (N^-1, P^-1 N^-1, P^-1) mod q_{l-1}, each (value, / q_{l-1})
 };
 
 // Twiddles of the CPU FP64 transforms as doubles (w and w/q, forward and inverse),

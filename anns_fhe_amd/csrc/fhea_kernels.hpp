@@ -118,9 +118,13 @@ void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u3
 // intermediates; output j of descs[d] (for j in j0 .. j0+nj-1) receives the
 // forward transform's first-pass (columns) intermediate of the converted limb at
 // out + poly*out_pstride + d*out_dstride + j*N, ready for a second pass
-// (only_pass = 2, or the fused key-switch rows kernel).  With v set (ModDown +
-// rescale), output j is Q slot j and its transform input is
-// c*T_j - conv_j*(P q_l)^-1, c = the centred lift of v (FUSE_MDRS_IN's prologue).
+// (only_pass = 2, or the fused key-switch rows kernel).
+// With lift_in set (ModDown followed by the rescale by q_{l-1}, ndig = 1): the
+// rows-pass intermediates of acc_{l-1} and d_{l-1} of poly sit at
+// lift_in + poly*lift_stride (+ N for d); the kernel forms the rescaled limb's
+// ModDown result v = INTT(d) + P^-1 (INTT(acc) - conv_{lift_j}) (mod q_{l-1},
+// lift_prime) in registers, and output j (Q slot j) transforms
+// c*T_j - conv_j*(P q_l)^-1 with c the centred lift of v (FUSE_MDRS_IN's prologue).
 static constexpr u32 CC_MAX_IN = 8;  // inputs per digit held in registers
 struct ConvColsArgs {
     const double *in;
@@ -131,11 +135,12 @@ struct ConvColsArgs {
     u64 out_pstride, out_dstride;
     u32 j0, nj;
     u32 npolys, ndig;
-    const u64 *v = nullptr;  // MDRS: INTT'd last limb of poly at v + poly*v_stride
-    u64 v_stride = 0;
-    u64 ql = 0;
-    const double *cst_in = nullptr;  // MDRS: (T_j, T_j/q_j)
-    const double *cst_pq = nullptr;  // MDRS: ((P q_l)^-1, /q_j)
+    const double *lift_in = nullptr;
+    u64 lift_stride = 0;
+    u32 lift_prime = 0, lift_j = 0;
+    const double *lift_cst = nullptr;  // (N^-1, P^-1 N^-1, P^-1) mod q_{l-1}, each (value, value/q)
+    const double *cst_in = nullptr;    // (T_j, T_j/q_j)
+    const double *cst_pq = nullptr;    // ((P q_l)^-1, /q_j)
 };
 // true when launch_conv_cols_f64 has a kernel for the context's splits and
 // conversions of up to max_in inputs

@@ -122,14 +122,6 @@ __device__ __forceinline__ void exch(double (&x)[1 << W], double *buf, u32 tid) 
     for (int u = 0; u < E; ++u) x[u] = buf[lds_pad(ebn | ((u32)u << PN))];
 }
 
-// The thread index as a value the compiler cannot hoist: addresses derived from it
-// inside a loop are recomputed there (a few integer instructions) instead of being
-// kept live across the loop (dozens of registers for the exchanges' LDS addresses).
-__device__ __forceinline__ u32 opaque_tid(u32 v) {
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
 // global word index (within the limb) of tile element e: row e >> LOGC, column c0 + (e mod 2^LOGC)
 template <int LOGC>
 __device__ __forceinline__ u32 col_gidx(u32 e, u32 logS, u32 c0) {

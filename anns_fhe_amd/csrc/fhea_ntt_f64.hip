@@ -237,7 +237,8 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     using R0 = Round<INV, LOGR, 0>;
     using RL = Round<INV, LOGR, NR - 1>;
     constexpr bool DIRECT_IN = COLS || R0::P >= 3;
-    const u32 tid = threadIdx.x;
+    const u32 tid0 = threadIdx.x;
+    const u32 tid = tid0;
     const u32 T = 1u << logT;
     const bool active = tid < (T >> 4);
     TileGeom g;
@@ -283,6 +284,10 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     u64 raw[16];
     if (PF && cur.tile < n_tiles) issue(cur, raw);
     while (cur.tile < n_tiles) {
+        // per-tile thread-derived indices (not kept live across the loop); the forward
+        // row pass (fused epilogues) keeps them live: measured faster at 3 waves there
+        const u32 tid = (INV || COLS) ? opaque_tid(tid0) : tid0;
+        const u32 eb0 = (tid & ((1u << P0) - 1)) | ((tid >> P0) << (P0 + 4));
         const TileId nxt = next_tile(cur.tile + gridDim.x);
         if (!PF) issue(cur, raw);
         geom(cur, g);
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsA
                                                      const ModParams *__restrict__ mp, u32 logN, u32 slot0,
                                                      u32 n_work) {
     __shared__ double lds[F_TPB * 16 + F_TPB];
-    const u32 tid = threadIdx.x;
+    const u32 tid0 = threadIdx.x;
     const u32 T = 1u << (logN - F_LOG_TILE);
     const u32 N = 1u << logN;
     const u32 lk = a.l + a.K;
@@ -610,6 +615,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsA
     using RF0 = Round<false, LOGRF, 0>;
     constexpr bool DIRECT_F = RF0::P >= 3;
     for (u32 w = blockIdx.x; w < n_work; w += gridDim.x) {
+        const u32 tid = opaque_tid(tid0);
         const u32 tI = w % T;
         const u32 zi = w / T;
         const u32 z = zi % a.nb;
@@ -623,6 +629,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsA
 #pragma unroll
         for (int k = 0; k < 16; ++k) s0[k] = s1[k] = 0.0;
         for (u32 p = 0; p < a.nparts; ++p) {
+            const u32 tid = opaque_tid(tid0);
             const u32 start = p * a.alpha;
             const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
             const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);

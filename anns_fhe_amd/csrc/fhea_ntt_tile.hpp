@@ -10,6 +10,14 @@
 
 namespace fhea {
 
+// The thread index as a value the compiler cannot hoist: addresses derived from it
+// inside a loop are recomputed there (a few integer instructions) instead of being
+// kept live across the loop (dozens of registers for the exchanges' LDS addresses).
+__device__ __forceinline__ u32 opaque_tid(u32 v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // LDS index of tile element e, padded by one word per 16 (bank spread)
 __device__ __forceinline__ u32 lds_pad(u32 e) { return e + (e >> 4); }
 

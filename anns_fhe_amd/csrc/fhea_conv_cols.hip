@@ -220,7 +220,8 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
         // ---- ModDown + rescale: the rescaled limb's ModDown result in coefficient form
         // at these positions, by the linearity of the transforms (mod q_{l-1}):
         //   v = INTT(d_{l-1}) + P^-1 (INTT(acc_{l-1}) - conv_{l-1}),
-        // from the rows-pass intermediates of acc_{l-1} and d_{l-1}, and its centred lift
+        // from the rows-pass intermediates of acc_{l-1} and d_{l-1} (without the d term
+        // when d is already in acc: lift_d = 0), and its centred lift
         // (rescale.cpp:46-54): the words of the ModDown written out, transformed back and lifted
         double cl[E];
         if (MDRS) {
@@ -239,12 +240,14 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
 #pragma unroll
             for (int u = 0; u < E; ++u) vs[u] = mulmod_f64(x[u], pni, pniq, q);
+            if (a.lift_d) {
 #pragma unroll
-            for (int u = 0; u < E; ++u) x[u] = lp[(1u << logN) + jin + (((u32)u * rstep_in) << logS)];
-            round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
-            rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
+                for (int u = 0; u < E; ++u) x[u] = lp[(1u << logN) + jin + (((u32)u * rstep_in) << logS)];
+                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
+                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
 #pragma unroll
-            for (int u = 0; u < E; ++u) vs[u] += mulmod_f64(x[u], ni, niq, q);
+                for (int u = 0; u < E; ++u) vs[u] += mulmod_f64(x[u], ni, niq, q);
+            }
             // conv_{l-1}
 #pragma unroll
             for (int u = 0; u < E; ++u) x[u] = 0.0;

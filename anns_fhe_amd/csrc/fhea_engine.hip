@@ -232,7 +232,7 @@ static int build_device_tables(fhea_ctx *c) {
     for (u32 i = 0; i < T; ++i) (h.primes[i] < (1ull << 50) ? mask_small : mask_big)[i >> 6] |= 1ull << (i & 63);
     const bool mixed = !use_f64 && mask_small != std::vector<u64>(4, 0);
     const size_t o_mask_small = b.addv(mask_small), o_mask_big = b.addv(mask_big);
-    size_t o_twf = 0, o_itwf = 0, o_ninvf = 0;
+    size_t o_twf = 0, o_itwf = 0, o_ninvf = 0, o_pmodqf = 0;
     if (use_f64 || mixed) {
         std::vector<double> twf((size_t)T * N * 2), itwf((size_t)T * N * 2), ninvf(2 * T);
         for (u32 i = 0; i < T; ++i) {
@@ -250,6 +250,14 @@ static int build_device_tables(fhea_ctx *c) {
         o_twf = b.addv(twf);
         o_itwf = b.addv(itwf);
         o_ninvf = b.addv(ninvf);
+        std::vector<u64> pmq(2 * L), qv(L);
+        for (u32 i = 0; i < L; ++i) {
+            u64 pr = 1 % h.primes[i];
+            for (u32 k = 0; k < K; ++k) pr = mulm(pr, h.primes[L + k] % h.primes[i], h.primes[i]);
+            pmq[2 * i] = pr;
+            qv[i] = h.primes[i];
+        }
+        o_pmodqf = b.addv(f64_pairs(pmq, qv));
     }
     std::vector<u32> ident(T);
     for (u32 i = 0; i < T; ++i) ident[i] = i;
@@ -468,6 +476,7 @@ static int build_device_tables(fhea_ctx *c) {
         c->dt.twf = (const double *)P(o_twf);
         c->dt.itwf = (const double *)P(o_itwf);
         c->dt.ninvf = (const double *)P(o_ninvf);
+        c->dt.pmodqf = (const double *)P(o_pmodqf);
     }
     c->zero_u32 = (const u32 *)P(o_zero);
     c->modup_desc_dev.assign(L + 1, nullptr);
@@ -520,7 +529,8 @@ struct RescaleAfter {
     const std::vector<u64 *> *outs;
 };
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
-                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid);
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
+                              bool folded = false);
 
 // Mixed chain whose Q primes past a leading run of >= 2^50 primes are all below
 // 2^50: the ModDown / rescale transforms take the FP64 fused ends on those limbs.
@@ -574,10 +584,11 @@ static void modup_cols(fhea_ctx *c, u32 l, u32 npolys, const u64 *coef, u64 *com
 // column pass after it: pb holds the first-pass (rows) intermediates of the P part
 // of the 2*nb accumulators, tt receives Q slots j0 .. j0+nj-1 (poly at tt + poly*l*N).
 // With lift (ModDown + rescale by q_{l-1}): the rows-pass intermediates of acc_{l-1}
-// and d_{l-1} of poly p at lift + 2p*N (+ N), and the transform input is the
-// ModDown + rescale prologue (FUSE_MDRS_IN) with the lift formed in the kernel.
+// and d_{l-1} of poly p at lift + 2p*N (+ N) (lift_d false: acc_{l-1} alone at
+// lift + p*N, d being folded into acc), and the transform input is the ModDown +
+// rescale prologue (FUSE_MDRS_IN) with the lift formed in the kernel.
 static void moddown_cols(fhea_ctx *c, u32 l, u32 nb, const u64 *pb, u64 *tt, u32 j0, u32 nj,
-                         const u64 *lift = nullptr) {
+                         const u64 *lift = nullptr, bool lift_d = true) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -595,7 +606,8 @@ static void moddown_cols(fhea_ctx *c, u32 l, u32 nb, const u64 *pb, u64 *tt, u32
     a.ndig = 1;
     if (lift) {
         a.lift_in = (const double *)lift;
-        a.lift_stride = 2 * N;
+        a.lift_stride = lift_d ? 2 * N : N;
+        a.lift_d = lift_d ? 1 : 0;
         a.lift_prime = l - 1;
         a.lift_j = l - 1;
         a.lift_cst = lt.rs_liftf;
@@ -619,7 +631,7 @@ static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
 // the composite's poly_add).  keys[z] = device pointer of item z's key.
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
-                   u64 *ws, const RescaleAfter *ra = nullptr) {
+                   u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -661,8 +673,13 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
             fz.only_pass = 1;
             launch_ntt_f64(t, false, compact, compact, cz, cz, N, N, nb, np * (l + K), lt.modup_ntt_prime, fz, s);
         }
-        const KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
-                            key_limbs, c->h.dnum, nb};
+        KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
+                      key_limbs, c->h.dnum, nb};
+        if (fold) {
+            ka.ta = (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
+            if (!ka.ta) return FHEA_ERR_DEVICE;
+            ka.pmodq = t.pmodqf;
+        }
         launch_ks_rows_f64(t, ka, s);
     } else if (!t.use_f64 && mixed_fused(t) && !(t.f64_off & (4 | 64)) && np <= 8 && ks_rows_fusable(t) &&
                t.q_big_prefix < l) {
@@ -707,7 +724,7 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // in its own launch; FHEA_F64_OFF bit 128 runs its first pass inside the inner
     // product instead (one wave per SIMD there: measured 2.6% slower per HMult)
     const bool pb_mid = fused && (t.f64_off & 128);
-    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid);
+    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid, fold != nullptr);
     return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
 }
 
@@ -833,7 +850,8 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
 // which is ((acc_i - NTT(conv_i))*P^-1 + d_i)*q_l^-1 + NTT(c*T_i) rearranged by the
 // linearity of the NTT mod q_i: the same words, one transform of l-1 limbs fewer.
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
-                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid) {
+                              const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
+                              bool folded) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -849,7 +867,8 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     }
     std::vector<RescaleOut> ro(2 * nb);
     for (u32 z = 0; z < nb; ++z)
-        for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {ra.d_base + z * ra.dz + k * ra.ds, (*ra.outs)[2 * z + k]};
+        for (u32 k = 0; k < 2; ++k)
+            ro[2 * z + k] = {folded ? nullptr : ra.d_base + z * ra.dz + k * ra.ds, (*ra.outs)[2 * z + k]};
     if (!mix && moddown_cc(c)) {
         // With the fused column kernel: the INTT rows passes of the P part and of the
         // last limbs acc_{l-1}, d_{l-1}, then one kernel for the INTT column passes, the
@@ -860,11 +879,14 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         NttFuse r1;
         r1.only_pass = 1;
         if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
+        // (folded: the tensor's d is already in acc, the lift needs acc_{l-1} alone)
         u64 *lift = v;
-        launch_ntt_f64(t, true, acc + (l - 1) * N, lift, (l + K) * N, 2 * N, N, N, 2 * nb, 1, t.ident + (l - 1), r1, s);
-        launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2, c->rs_last_map[l],
-                       r1, s);
-        moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift);
+        const u64 ls = folded ? N : 2 * N;
+        launch_ntt_f64(t, true, acc + (l - 1) * N, lift, (l + K) * N, ls, N, N, 2 * nb, 1, t.ident + (l - 1), r1, s);
+        if (!folded)
+            launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2,
+                           c->rs_last_map[l], r1, s);
+        moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift, !folded);
         auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
         if (!ro_d) return FHEA_ERR_DEVICE;
         NttFuse f2;
@@ -946,6 +968,17 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
 // workspace words of one chunk of nb items
 static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb);
 
+// HMult with the tensor folded into the key switch: the tensor kernel writes d2
+// only, the fused rows kernel adds P*d0 and P*d1 to the Q slots' accumulators, and
+// the ModDown + rescale then yields the composite's sum (FP64 path with the fused
+// key-switch rows kernel and the fused ModDown conversion; FHEA_F64_OFF bit 1024
+// keeps the whole tensor).
+static bool tensor_folded(const fhea_ctx *c, u32 l) {
+    const DevTables &t = c->dt;
+    return t.use_f64 && !(t.f64_off & (1 | 4 | 8 | 16 | 64 | 1024)) && l >= 2 && c->lv[l].nparts <= 8 &&
+           ks_rows_fusable(t) && moddown_cc(c);
+}
+
 // Swaps the context's stream and workspace with the second lane's.
 static void swap_lane(fhea_ctx *c) {
     std::swap(c->stream, c->stream2);
@@ -1003,10 +1036,11 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         std::vector<CombineArgs> cb(nb);
         std::vector<const u64 *> keys(nb, key);
         std::vector<u64 *> outs(2 * nb);
+        const bool fold = tensor_folded(c, l);
         for (u32 z = 0; z < nb; ++z) {
             const MultItem &it = items[off + z];
             u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
-            ta[z] = {it.a0, it.a1, it.b0, it.b1, d0, d1, d2};
+            ta[z] = {it.a0, it.a1, it.b0, it.b1, fold ? nullptr : d0, fold ? nullptr : d1, d2};
             cb[z] = {d0, d1, d0, d1};
             outs[2 * z] = it.d0;
             outs[2 * z + 1] = it.d1;
@@ -1015,7 +1049,7 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         if (!ta_d) return FHEA_ERR_DEVICE;
         launch_tensor(c->dt, ta_d, nb, l, c->stream);
         const RescaleAfter ra{D, 3 * l * N, l * N, &outs};
-        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest, &ra);
+        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest, &ra, fold ? &ta : nullptr);
         if (st) return st;
     }
     return check_hip(hipGetLastError());

@@ -56,10 +56,11 @@ __global__ __launch_bounds__(TPB) void k_tensor_f64(const TensorArgs *__restrict
     const TensorArgs A = args[blockIdx.z];
     const size_t i = ((size_t)limb << logN) + (size_t)blockIdx.x * TPB + threadIdx.x;
     const double q = mp[limb].qd, qinv = mp[limb].qinv;
-    const double a0 = u64_to_f64(A.a0[i]), a1 = u64_to_f64(A.a1[i]);
-    const double b0 = u64_to_f64(A.b0[i]), b1 = u64_to_f64(A.b1[i]);
-    A.d0[i] = f64_to_canon(mulmod_vv_f64(a0, b0, q, qinv), q);
+    const double a1 = u64_to_f64(A.a1[i]), b1 = u64_to_f64(A.b1[i]);
     A.d2[i] = f64_to_canon(mulmod_vv_f64(a1, b1, q, qinv), q);
+    if (!A.d0) return;  // d2 only (the key switch forms d0, d1 itself)
+    const double a0 = u64_to_f64(A.a0[i]), b0 = u64_to_f64(A.b0[i]);
+    A.d0[i] = f64_to_canon(mulmod_vv_f64(a0, b0, q, qinv), q);
     const double x = mulmod_vv_f64(a0, b1, q, qinv) + mulmod_vv_f64(a1, b0, q, qinv);
     A.d1[i] = f64_to_canon(reduce_f64(x, q, qinv), q);
 }

@@ -22,7 +22,8 @@ struct DevTables {
     // fused inner product, the P slots also run the ModDown INTT's first pass there;
     // 256: slot-major tile walk for the first pass of rescale-lift prologues;
     // 512: INTT column pass, basis conversion and NTT column pass as three launches
-    // instead of the fused conversion kernel (fhea_conv_cols.hip)
+    // instead of the fused conversion kernel (fhea_conv_cols.hip); 1024: the HMult
+    // tensor in full instead of folded into the key switch (tensor_folded)
     bool use_f64 = false;
     u32 f64_off = 0;
     // kernel variant of the fused conversion column kernel (FHEA_CC_VARIANT)
@@ -34,6 +35,7 @@ struct DevTables {
     const double *twf = nullptr;    // [T][N][2]  (psi^br(j), psi^br(j)/q)
     const double *itwf = nullptr;   // [T][N][2]
     const double *ninvf = nullptr;  // [T][2]
+    const double *pmodqf = nullptr; // [L][2] (P mod q_i, / q_i): the HMult tensor folded into the key switch
     // mixed chain (some primes >= 2^50, others below): the integer kernels run
     // everything but the transforms of limbs whose prime is below 2^50, which go
     // to the FP64 transform (its tables are built for such chains too);
@@ -50,6 +52,12 @@ struct DevTables {
 struct RescaleOut;
 struct CombineArgs;
 struct KsIpArgs;
+// Tensor product of two ciphertexts (fhe_host_driver.cpp:280-286):
+// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 (d0 null: d2 only).
+struct TensorArgs {
+    const u64 *a0, *a1, *b0, *b1;
+    u64 *d0, *d1, *d2;
+};
 enum : int {
     FUSE_NONE = 0,
     FUSE_RESCALE_IN = 1,
@@ -103,6 +111,11 @@ struct KsRowsArgs {
     u64 az;
     u64 *pb;             // bit 128, P slots: intermediates of poly 2z+s, limb j = i-l at pb + (2z+s)*K*N + j*N
     u32 l, K, L, alpha, nparts, key_limbs, dnum, nb;
+    // HMult with the tensor folded in (ta != null): item z's own limbs are d2 = a1 b1
+    // from ta[z], and the Q slots also receive P*d0 (acc0) and P*d1 (acc1), d0 = a0 b0,
+    // d1 = a0 b1 + a1 b0: the ModDown (acc - NTT(conv)) P^-1 then yields ModDown(acc) + d
+    const TensorArgs *ta = nullptr;
+    const double *pmodq = nullptr;  // DevTables::pmodqf
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);
@@ -138,6 +151,7 @@ struct ConvColsArgs {
     const double *lift_in = nullptr;
     u64 lift_stride = 0;
     u32 lift_prime = 0, lift_j = 0;
+    u32 lift_d = 1;  // 0: lift_in holds acc'_{l-1} only (d folded into acc'), v = P^-1 (INTT(acc') - conv)
     const double *lift_cst = nullptr;  // (N^-1, P^-1 N^-1, P^-1) mod q_{l-1}, each (value, value/q)
     const double *cst_in = nullptr;    // (T_j, T_j/q_j)
     const double *cst_pq = nullptr;    // ((P q_l)^-1, /q_j)
@@ -169,12 +183,7 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
 // op: 0 add, 1 sub, 2 mul.
 void launch_binop(const DevTables &t, int op, const u64 *a, const u64 *b, u64 *o, u32 limbs,
                   hipStream_t s);
-// Tensor product of two ciphertexts (fhe_host_driver.cpp:280-286):
-// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1.  Pointer arrays of npolys.
-struct TensorArgs {
-    const u64 *a0, *a1, *b0, *b1;
-    u64 *d0, *d1, *d2;
-};
+// Tensor products of nbatch ciphertext pairs (TensorArgs above).
 void launch_tensor(const DevTables &t, const TensorArgs *args_dev, u32 nbatch, u32 limbs,
                    hipStream_t s);
 // out[i] = in[map[i]] with map packed two u32 per word (automorphism.cpp:3-12).

@@ -201,7 +201,8 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
         const RescaleOut A = fz.ro[poly];
         const double x = mulmod_f64(u64_to_f64(fz.acc[(size_t)poly * fz.acc_stride + off]), fz.cst_pq[2 * slot],
                                     fz.cst_pq[2 * slot + 1], q);
-        const double d = mulmod_f64(u64_to_f64(A.a[off]), fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
+        // a null: d already folded into acc (HMult with the tensor in the key switch)
+        const double d = A.a ? mulmod_f64(u64_to_f64(A.a[off]), fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q) : 0.0;
         A.out[off] = f64_to_canon(reduce_f64(x + d + v, q, qinv), q);
         return;
     }
@@ -602,7 +603,7 @@ __device__ __forceinline__ void rows_to_lds(const double (&x)[16], double *lds, 
 }
 
 template <int LOGRF, int LOGRI, bool PSLOT>
-__global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
+__global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
                                                      const ModParams *__restrict__ mp, u32 logN, u32 slot0,
                                                      u32 n_work) {
@@ -634,6 +635,36 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR void k_ks_rows_f64(KsRowsA
             const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
             const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);
             const auto *kb = ka + (size_t)a.dnum * kstride;
+            if (i >= start && i < start + sz && a.ta) {
+                // HMult with the tensor folded in: the own limb d2 = a1 b1, and
+                // acc0 += P d0, acc1 += P d1 (d0 = a0 b0, d1 = a0 b1 + a1 b0), every product
+                // of canonical words (mulmod_vv_f64, <= 0.875q); the sums are centred after
+                // (<= q/2 + 2) so the later digits' terms keep them below 8q
+                const TensorArgs TA = a.ta[z];
+                const size_t o = ((size_t)i << logN) + base + tid;
+                const auto *pa0 = gptr(TA.a0 + o), *pa1 = gptr(TA.a1 + o);
+                const auto *pb0 = gptr(TA.b0 + o), *pb1 = gptr(TA.b1 + o);
+                const double pm = a.pmodq[2 * i], pmq = a.pmodq[2 * i + 1];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    // two elements' six loads in flight at a time (register budget)
+                    if ((k & 1) == 0) __builtin_amdgcn_sched_barrier(0);
+                    const size_t e = (size_t)k * F_TPB;
+                    const double a0 = u64_to_f64(pa0[e]), a1 = u64_to_f64(pa1[e]);
+                    const double b0 = u64_to_f64(pb0[e]), b1 = u64_to_f64(pb1[e]);
+                    double d2 = mulmod_vv_f64(a1, b1, q, qinv);
+                    d2 = d2 < 0.0 ? d2 + q : d2;  // the canonical word
+                    const double d0 = mulmod_vv_f64(a0, b0, q, qinv);
+                    const double d1 = mulmod_vv_f64(a0, b1, q, qinv) + mulmod_vv_f64(a1, b0, q, qinv);
+                    s0[k] = reduce_f64(s0[k] + mulmod_vv_f64(d2, u64_to_f64(kb[e]), q, qinv) +
+                                           mulmod_f64(d0, pm, pmq, q),
+                                       q, qinv);
+                    s1[k] = reduce_f64(s1[k] + mulmod_vv_f64(d2, u64_to_f64(ka[e]), q, qinv) +
+                                           mulmod_f64(d1, pm, pmq, q),
+                                       q, qinv);
+                }
+                continue;
+            }
             if (i >= start && i < start + sz) {
                 // the digit's own limb: the input itself (eval domain, canonical)
                 const auto *c = gptr(A.c + ((size_t)i << logN) + base + tid);

@@ -182,14 +182,16 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
 
 
 @pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (128, 0), (256, 0), (64, 0), (512, 0),
-                                               (512 | 128, 0), (1 | 2 | 4 | 8 | 16 | 32, 0)])
+                                               (512 | 128, 0), (1024, 0), (1024 | 128, 0),
+                                               (1 | 2 | 4 | 8 | 16 | 32, 0)])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
     P slots' ModDown INTT pass inside the fused inner product; 256: the slot-major
     tile order (instead of unit-major) for the passes whose prologue reads the
     rescale lift; 64: the inner product as its own kernel; 512: the INTT column
     pass, basis conversion and NTT column pass as three launches instead of the
-    fused conversion kernel; 1..32: each component on the integer kernels), and
+    fused conversion kernel; 1024: the HMult tensor in full instead of folded into
+    the key switch; 1..32: each component on the integer kernels), and
     FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting
     gives the oracle's words for a batched HMult (3 items, one lane and two) and a
     rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""

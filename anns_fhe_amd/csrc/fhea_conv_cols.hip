@@ -165,13 +165,23 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     static_assert(RoundW<true, LOGR, W, NR - 1>::P == RoundW<false, LOGR, W, 0>::P, "windows must meet");
     static_assert(W <= 3, "forward values are centred every 2W <= 6 stages (bound 7.53q < 8q)");
     __shared__ double lds[2 * PADT];
+    // the unit's constants, staged once per digit: conversion pairs (qhat_i mod p_j,
+    // / p_j) as [i][j] with zero pairs for i >= n_in, per output j its prime index and
+    // (p_j, 1/p_j), per input i (q_i, 1/q_i) and the scaling pair, per output j the
+    // ModDown + rescale pairs.  Read with uniform addresses (LDS broadcast) instead of
+    // one dependent global load per conversion term.
+    __shared__ double2 s_hat[CC_MAX_IN * CC_MAX_OUT];
+    __shared__ double2 s_oq[CC_MAX_OUT];
+    __shared__ double2 s_md[2 * CC_MAX_OUT];
+    __shared__ double2 s_iq[2 * CC_MAX_IN];
+    __shared__ u32 s_oprime[CC_MAX_OUT], s_iprime[CC_MAX_IN];
     const u32 tid = threadIdx.x;
     const u32 logS = logN - LOGR;
     const u32 ntiles = 1u << (logS - LOGC);
     const u32 ebi = elem_base<W>(tid, PI0);
     constexpr u32 LG = (LOGC < 4) ? 4 - LOGC : 0;  // log2 tiles per 16-column block
     const u32 n_round = (n_work + (8u << LG) - 1) & ~((8u << LG) - 1);
-    u32 xb = 0;
+    u32 xb = 0, staged = ~0u;
     for (u32 w = blockIdx.x; w < n_round; w += gridDim.x) {
         const u32 tt = (w & ~((8u << LG) - 1)) | ((w & 7u) << LG) | ((w >> 3) & ((1u << LG) - 1));
         if (tt >= n_work) continue;
@@ -180,6 +190,31 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
         const u32 c0 = tI << LOGC;
         const ConvDesc D = a.descs[d];
         const u32 n_in = D.n_in, n_out = D.n_out;
+        if (d != staged) {
+            __syncthreads();  // the previous digit's constants are no longer read
+            for (u32 k = tid; k < CC_MAX_IN * n_out; k += blockDim.x) {
+                const u32 i = k / n_out, j = k % n_out;
+                s_hat[i * CC_MAX_OUT + j] =
+                    i < n_in ? make_double2(D.hatf[2 * k], D.hatf[2 * k + 1]) : make_double2(0.0, 0.0);
+            }
+            for (u32 j = tid; j < n_out; j += blockDim.x) {
+                const u32 pj = D.out_prime[j];
+                s_oprime[j] = pj;
+                s_oq[j] = make_double2(mp[pj].qd, mp[pj].qinv);
+                if (MDRS && j + 1 < n_out) {
+                    s_md[2 * j] = make_double2(a.cst_in[2 * j], a.cst_in[2 * j + 1]);
+                    s_md[2 * j + 1] = make_double2(a.cst_pq[2 * j], a.cst_pq[2 * j + 1]);
+                }
+            }
+            for (u32 i = tid; i < n_in; i += blockDim.x) {
+                const u32 pi = D.in_prime[i];
+                s_iprime[i] = pi;
+                s_iq[2 * i] = make_double2(mp[pi].qd, mp[pi].qinv);
+                s_iq[2 * i + 1] = make_double2(D.in_ncstf[2 * i], D.in_ncstf[2 * i + 1]);
+            }
+            __syncthreads();
+            staged = d;
+        }
         const auto *src = gptr(a.in + (size_t)poly * a.in_pstride + ((size_t)a.in_first[d] << logN));
         const u32 jin = col_gidx<LOGC>(ebi, logS, c0);
         const u32 rstep_in = 1u << (PI0 - LOGC);  // row distance of consecutive elements (in rows)
@@ -201,12 +236,13 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
 #pragma unroll
                     for (int u = 0; u < E; ++u) raw[u] = s1[jl + (((u32)u * rstep_in) << logS)];
                 }
-                const u32 pi = D.in_prime[i];
-                const double q = mp[pi].qd, qinv = mp[pi].qinv;
+                const u32 pi = s_iprime[i];
+                const double2 qq = s_iq[2 * i], cc = s_iq[2 * i + 1];
+                const double q = qq.x, qinv = qq.y;
                 const double *tw = itwf + ((size_t)pi << (logN + 1));
                 round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jl);
                 rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
-                const double cs = D.in_ncstf[2 * i], csq = D.in_ncstf[2 * i + 1];
+                const double cs = cc.x, csq = cc.y;
 #pragma unroll
                 for (int u = 0; u < E; ++u) {
                     const double v = reduce_f64(mulmod_f64(x[u], cs, csq, q), q, qinv);
@@ -252,10 +288,13 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
 #pragma unroll
             for (int u = 0; u < E; ++u) x[u] = 0.0;
             const u32 lj = a.lift_j;
+            double2 hv[CC_MAX_IN];
+#pragma unroll
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) hv[i] = s_hat[i * CC_MAX_OUT + lj];
 #pragma unroll
             for (int i = 0; i < (int)CC_MAX_IN; ++i) {
                 if ((u32)i < n_in) {
-                    const double h = D.hatf[2 * ((size_t)i * n_out + lj)], hq = D.hatf[2 * ((size_t)i * n_out + lj) + 1];
+                    const double h = hv[i].x, hq = hv[i].y;
 #pragma unroll
                     for (int u = 0; u < E; ++u) {
                         x[u] += mulmod_f64(y[i][u], h, hq, q);
@@ -278,15 +317,19 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
         const u32 jend = (a.j0 + a.nj < n_out) ? a.j0 + a.nj : n_out;
         double *dst_base = a.out + (size_t)poly * a.out_pstride + (size_t)d * a.out_dstride;
         for (u32 j = a.j0; j < jend; ++j) {
-            const u32 pj = D.out_prime[j];
-            const double q = mp[pj].qd, qinv = mp[pj].qinv;
+            const u32 pj = s_oprime[j];
+            const double2 qq = s_oq[j];
+            const double q = qq.x, qinv = qq.y;
+            double2 hv[CC_MAX_IN];
+#pragma unroll
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) hv[i] = s_hat[i * CC_MAX_OUT + j];
             double x[E];
 #pragma unroll
             for (int u = 0; u < E; ++u) x[u] = 0.0;
 #pragma unroll
             for (int i = 0; i < (int)CC_MAX_IN; ++i) {
                 if ((u32)i < n_in) {
-                    const double h = D.hatf[2 * ((size_t)i * n_out + j)], hq = D.hatf[2 * ((size_t)i * n_out + j) + 1];
+                    const double h = hv[i].x, hq = hv[i].y;
 #pragma unroll
                     for (int u = 0; u < E; ++u) {
                         x[u] += mulmod_f64(y[i][u], h, hq, q);
@@ -298,8 +341,8 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             for (int u = 0; u < E; ++u) x[u] = reduce_f64(x[u], q, qinv);
             if (MDRS) {
                 // c*T_j - conv_j*(P q_l)^-1 (output j is Q slot j)
-                const double T = a.cst_in[2 * j], Tq = a.cst_in[2 * j + 1];
-                const double pq = a.cst_pq[2 * j], pqq = a.cst_pq[2 * j + 1];
+                const double2 tt2 = s_md[2 * j], pq2 = s_md[2 * j + 1];
+                const double T = tt2.x, Tq = tt2.y, pq = pq2.x, pqq = pq2.y;
 #pragma unroll
                 for (int u = 0; u < E; ++u)
                     x[u] = reduce_f64(mulmod_f64(cl[u], T, Tq, q) - mulmod_f64(x[u], pq, pqq, q), q, qinv);
@@ -350,6 +393,7 @@ static u32 cols_log(const DevTables &t) {
 
 bool conv_cols_fusable(const DevTables &t, u32 max_in) {
     if (!t.use_f64 || (t.f64_off & (1 | 2 | 512)) || t.logN <= 12 || max_in > CC_MAX_IN) return false;
+    if (t.L + t.K > CC_MAX_OUT) return false;  // outputs per digit staged in LDS
     const u32 lr = cols_log(t);
     return lr >= 6 && lr <= 9 && t.logN - lr >= 4;
 }

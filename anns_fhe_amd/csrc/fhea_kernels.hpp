@@ -139,6 +139,7 @@ void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u3
 // lift_prime) in registers, and output j (Q slot j) transforms
 // c*T_j - conv_j*(P q_l)^-1 with c the centred lift of v (FUSE_MDRS_IN's prologue).
 static constexpr u32 CC_MAX_IN = 8;  // inputs per digit held in registers
+static constexpr u32 CC_MAX_OUT = 64; // outputs per digit (constants staged in LDS)
 struct ConvColsArgs {
     const double *in;
     u64 in_pstride;

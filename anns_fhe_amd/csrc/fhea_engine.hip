@@ -647,9 +647,20 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // (fused: the INTT's rows pass, then one kernel for its column pass, the
     // conversion and the NTT's column pass)
     const bool cc = modup_cc(c, l);
+    // the folded HMult tensor (fold): its factors per item, for the fused rows kernel
+    // and (with cc) for the ModUp INTT's first pass, which then forms d2 = a1 b1 itself
+    const TensorArgs *fold_d = nullptr;
+    if (fold) {
+        fold_d = (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
+        if (!fold_d) return FHEA_ERR_DEVICE;
+    }
     if (cc) {
         NttFuse fz;
         fz.only_pass = 1;
+        if (fold) {
+            fz.pro = FUSE_PROD_IN;
+            fz.ta = fold_d;
+        }
         launch_ntt_f64(t, true, c_base, coef, c_stride, l * N, N, N, nb, l, t.ident, fz, s);
         modup_cols(c, l, nb, coef, compact);
     } else {
@@ -676,8 +687,7 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         KsRowsArgs ka{compact, cz, ip_d, acc, 2 * (l + K) * N, pb, l, (u32)K, t.L, c->h.alpha, np,
                       key_limbs, c->h.dnum, nb};
         if (fold) {
-            ka.ta = (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
-            if (!ka.ta) return FHEA_ERR_DEVICE;
+            ka.ta = fold_d;
             ka.pmodq = t.pmodqf;
         }
         launch_ks_rows_f64(t, ka, s);
@@ -1037,6 +1047,8 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         std::vector<const u64 *> keys(nb, key);
         std::vector<u64 *> outs(2 * nb);
         const bool fold = tensor_folded(c, l);
+        // with the fused ModUp as well, the ModUp INTT forms d2 itself: no tensor launch
+        const bool prod = fold && modup_cc(c, l);
         for (u32 z = 0; z < nb; ++z) {
             const MultItem &it = items[off + z];
             u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
@@ -1045,9 +1057,11 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
             outs[2 * z] = it.d0;
             outs[2 * z + 1] = it.d1;
         }
-        auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
-        if (!ta_d) return FHEA_ERR_DEVICE;
-        launch_tensor(c->dt, ta_d, nb, l, c->stream);
+        if (!prod) {
+            auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
+            if (!ta_d) return FHEA_ERR_DEVICE;
+            launch_tensor(c->dt, ta_d, nb, l, c->stream);
+        }
         const RescaleAfter ra{D, 3 * l * N, l * N, &outs};
         st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest, &ra, fold ? &ta : nullptr);
         if (st) return st;

@@ -65,6 +65,7 @@ enum : int {
     FUSE_MODDOWN_OUT = 3,
     FUSE_MDRS_IN = 4,   // ModDown + rescale in one transform: prologue
     FUSE_MDRS_OUT = 5,  // ... and epilogue
+    FUSE_PROD_IN = 6,   // inverse first pass: input word = a1 * b1 (the HMult tensor's d2, from ta[poly])
 };
 // Fused prologue (first forward pass) / epilogue (last forward pass) of the
 // FP64 transform; poly = batch index of the NTT launch.
@@ -88,6 +89,8 @@ struct NttFuse {
     const u64 *pmask = nullptr;
     // first pass: walk the tiles unit-major (find_tile_um) instead of slot-major
     int unit_major = 0;
+    // FUSE_PROD_IN: per poly the factors (a1, b1 of TensorArgs; slot s at + s*src_slot)
+    const TensorArgs *ta = nullptr;
     // two-pass transforms (N > 4096): 0 both passes; 1 only the first (its output,
     // FP64 intermediates, left at dst); 2 only the second (reading such intermediates
     // at src)

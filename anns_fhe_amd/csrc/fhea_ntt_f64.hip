@@ -223,7 +223,9 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
 
 // The kernel is persistent (find_tile) and issues the next tile's global loads
 // before the current tile's arithmetic, so loads overlap FP64 work.
-template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH>
+// PROD (inverse first pass only): the input word is the product a1 * b1 of fz.ta[poly]
+// (FUSE_PROD_IN), both factors prefetched.
+template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, bool PROD = false>
 __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
@@ -257,9 +259,23 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
         gg.c0 = (id.tI & ((1u << ncgl) - 1)) << gg.logC;
         gg.b0 = (id.tI >> ncgl) << logNB;
     };
-    auto issue = [&](const TileId &id, u64 (&r)[16]) {
+    auto issue = [&](const TileId &id, u64 (&r)[16], u64 (&r2)[PROD ? 16 : 1]) {
         TileGeom gg = g;
         geom(id, gg);
+        if constexpr (PROD) {
+            const TensorArgs TA = fz.ta[id.poly];
+            const auto *pa = gptr(TA.a1 + (size_t)id.slot * src_slot), *pb = gptr(TA.b1 + (size_t)id.slot * src_slot);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const u32 e = tid + (u32)k * TPB;
+                if (e < T) {
+                    const u32 j = gg.gidx(e, LOGR);
+                    r[k] = pa[j];
+                    r2[k] = pb[j];
+                }
+            }
+            return;
+        }
         const char *sbase = (const char *)src + 8 * ((size_t)id.poly * src_stride + (size_t)id.slot * src_slot);
         if (DIRECT_IN) {
             if (active) {
@@ -282,15 +298,16 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
                              : find_tile(t, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
     };
     TileId cur = next_tile(blockIdx.x);
-    u64 raw[16];
-    if (PF && cur.tile < n_tiles) issue(cur, raw);
+    static_assert(!PROD || (INV && !COLS && !DIRECT_IN), "product prologue: inverse first (row) pass");
+    u64 raw[16], raw2[PROD ? 16 : 1];
+    if (PF && cur.tile < n_tiles) issue(cur, raw, raw2);
     while (cur.tile < n_tiles) {
         // per-tile thread-derived indices (not kept live across the loop); the forward
         // row pass (fused epilogues) keeps them live: measured faster at 3 waves there
         const u32 tid = (INV || COLS) ? opaque_tid(tid0) : tid0;
         const u32 eb0 = (tid & ((1u << P0) - 1)) | ((tid >> P0) << (P0 + 4));
         const TileId nxt = next_tile(cur.tile + gridDim.x);
-        if (!PF) issue(cur, raw);
+        if (!PF) issue(cur, raw, raw2);
         geom(cur, g);
         g.q = mp[cur.pidx].qd;
         g.qinv = mp[cur.pidx].qinv;
@@ -310,11 +327,17 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const u32 e = tid + (u32)k * TPB;
-                if (e < T) lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.qinv, g.gidx(e, LOGR), cur.poly);
+                if (e < T) {
+                    if constexpr (PROD)
+                        // d2 = a1 b1 of canonical words: |.| <= 0.875q
+                        lds[lds_pad(e)] = mulmod_vv_f64(u64_to_f64(raw[k]), u64_to_f64(raw2[k]), g.q, g.qinv);
+                    else
+                        lds[lds_pad(e)] = raw_to_f64(raw[k], in_canon, fz, cur.slot, g.q, g.qinv, g.gidx(e, LOGR), cur.poly);
+                }
             }
         }
         // prefetch the next tile
-        if (PF && nxt.tile < n_tiles) issue(nxt, raw);
+        if (PF && nxt.tile < n_tiles) issue(nxt, raw, raw2);
         if (!DIRECT_IN) {
             __syncthreads();
             if (active) {
@@ -410,7 +433,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     }
 }
 
-template <bool INV, bool COLS>
+template <bool INV, bool COLS, bool PROD = false>
 static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
@@ -424,7 +447,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         static int per_cu = 0;                                                                       \
         if (per_cu == 0) {                                                                           \
             int n = 0;                                                                               \
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS>, F_TPB, 0) != \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PROD>, F_TPB, 0) != \
                     hipSuccess || n <= 0)                                                            \
                 n = 4;                                                                               \
             per_cu = n < 4 ? n : 4;                                                                  \
@@ -432,7 +455,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         const u32 use_cu = (wg_cap && wg_cap < (u32)per_cu) ? wg_cap : (u32)per_cu;                   \
         const u32 cap = max_wg / 4 * use_cu;                                                          \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
-        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
+        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PROD>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
                            dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles,   \
                            npolys);                                                                  \
         break;                                                                                       \
@@ -535,7 +558,14 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
             launch_pass<false, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, s2, dst, ss2, ds, sl2, dl, limbs, pm, tw,
                                       t.ninvf, t.mp, lg, F_LOG_TILE, 0, 1, 0, epi);
     } else {
-        if (first)
+        // FUSE_PROD_IN: the first pass reads its input as the product a1 * b1 (fz.ta)
+        NttFuse prod = none;
+        prod.pro = fz.pro;
+        prod.ta = fz.ta;
+        if (first && fz.pro == FUSE_PROD_IN)
+            launch_pass<true, false, true>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm,
+                                           tw, t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, prod);
+        else if (first)
             launch_pass<true, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw,
                                      t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, none);
         if (second)

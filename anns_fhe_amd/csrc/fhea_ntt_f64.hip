@@ -221,6 +221,59 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
     else ((double *)dbase)[j] = v;
 }
 
+// The fused epilogues of a thread's 16 tile elements (e = tid + k*TPB, values in lds),
+// in two groups of eight whose operand words are loaded first: one wait per group
+// instead of a load, wait and store per element.  Same arithmetic as store_out.
+template <u32 TPB>
+__device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const TileGeom &g, u32 logR,
+                                              const NttFuse &fz, u32 poly, u32 slot, u32 logN, double q,
+                                              double qinv) {
+    const size_t sb = (size_t)slot << logN;
+    const int epi = fz.epi;
+    const RescaleOut R = (epi == FUSE_MODDOWN_OUT) ? RescaleOut{nullptr, nullptr} : fz.ro[poly];
+    const CombineArgs C = (epi == FUSE_MODDOWN_OUT) ? fz.cb[poly >> 1] : CombineArgs{nullptr, nullptr, nullptr, nullptr};
+    const u32 half = poly & 1;
+    // first operand (acc, or a for the rescale), optional second (d, or the add)
+    const u64 *p1 = (epi == FUSE_RESCALE_OUT) ? R.a : fz.acc + (size_t)poly * fz.acc_stride;
+    const u64 *p2 = (epi == FUSE_MDRS_OUT) ? R.a : (epi == FUSE_MODDOWN_OUT) ? (half ? C.add1 : C.add0) : nullptr;
+    u64 *po = (epi == FUSE_MODDOWN_OUT) ? (half ? C.out1 : C.out0) : R.out;
+    const auto *a1 = gptr(p1 + sb);
+    const auto *a2 = p2 ? gptr(p2 + sb) : nullptr;
+    auto *out = gptr_w(po + sb);
+    const double co = fz.cst_out[2 * slot], coq = fz.cst_out[2 * slot + 1];
+    const double cp = (epi == FUSE_MDRS_OUT) ? fz.cst_pq[2 * slot] : 0.0;
+    const double cpq = (epi == FUSE_MDRS_OUT) ? fz.cst_pq[2 * slot + 1] : 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        u64 w1[8], w2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w1[k] = a1[g.gidx(tid + (u32)(8 * h + k) * TPB, logR)];
+        if (a2) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w2[k] = a2[g.gidx(tid + (u32)(8 * h + k) * TPB, logR)];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const u32 e = tid + (u32)(8 * h + k) * TPB;
+            const double v = lds[lds_pad(e)];
+            double r;
+            if (epi == FUSE_MDRS_OUT) {
+                // acc*(P q_l)^-1 + d*q_l^-1 + x (d folded into acc: no second operand)
+                r = mulmod_f64(u64_to_f64(w1[k]), cp, cpq, q) + v;
+                if (a2) r += mulmod_f64(u64_to_f64(w2[k]), co, coq, q);
+            } else if (epi == FUSE_MODDOWN_OUT) {
+                // (acc - v)*P^-1 [+ add]; |acc - v| < 2q
+                r = mulmod_f64(u64_to_f64(w1[k]) - v, co, coq, q);
+                if (a2) r += u64_to_f64(w2[k]);
+            } else {
+                // a*q_l^-1 + x
+                r = mulmod_f64(u64_to_f64(w1[k]), co, coq, q) + v;
+            }
+            out[g.gidx(e, logR)] = f64_to_canon(reduce_f64(r, q, qinv), q);
+        }
+    }
+}
+
 // The kernel is persistent (find_tile) and issues the next tile's global loads
 // before the current tile's arithmetic, so loads overlap FP64 work.
 // PROD (inverse first pass only): the input word is the product a1 * b1 of fz.ta[poly]
@@ -304,7 +357,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     while (cur.tile < n_tiles) {
         // per-tile thread-derived indices (not kept live across the loop); the forward
         // row pass (fused epilogues) keeps them live: measured faster at 3 waves there
-        const u32 tid = (INV || COLS) ? opaque_tid(tid0) : tid0;
+        const u32 tid = opaque_tid(tid0);
         const u32 eb0 = (tid & ((1u << P0) - 1)) | ((tid >> P0) << (P0 + 4));
         const TileId nxt = next_tile(cur.tile + gridDim.x);
         if (!PF) issue(cur, raw, raw2);
@@ -423,9 +476,12 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
                     for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
                 }
                 __syncthreads();
-                for (u32 e = tid; e < T; e += TPB)
-                    store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, cur.poly,
-                              cur.slot, logN, g.q, g.qinv, INV);
+                if (!INV && !COLS && fz.epi != FUSE_NONE && T == 16 * TPB)
+                    store_fused16<TPB>(lds, tid, g, LOGR, fz, cur.poly, cur.slot, logN, g.q, g.qinv);
+                else
+                    for (u32 e = tid; e < T; e += TPB)
+                        store_out(dbase, g.gidx(e, LOGR), lds[lds_pad(e)], out_canon, scale, ni, niq, fz, cur.poly,
+                                  cur.slot, logN, g.q, g.qinv, INV);
             }
         }
         __syncthreads();  // LDS reads of this tile are done before the next tile writes

@@ -631,7 +631,8 @@ static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
 // the composite's poly_add).  keys[z] = device pointer of item z's key.
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
-                   u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr) {
+                   u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr,
+                   const std::vector<GatherArgs> *gather = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -654,12 +655,24 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         fold_d = (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
         if (!fold_d) return FHEA_ERR_DEVICE;
     }
+    // gather (rotations): input z is sigma_g of (*gather)[z].in, formed while loading
+    // (the ModUp INTT's first pass, the rows kernel's own limbs); needs cc and the
+    // fused rows kernel (ks_gather_ok)
+    const GatherArgs *gather_d = nullptr;
+    if (gather) {
+        gather_d = (const GatherArgs *)stage_args(c, gather->data(), nb * sizeof(GatherArgs));
+        if (!gather_d) return FHEA_ERR_DEVICE;
+    }
     if (cc) {
         NttFuse fz;
         fz.only_pass = 1;
         if (fold) {
             fz.pro = FUSE_PROD_IN;
             fz.ta = fold_d;
+        }
+        if (gather) {
+            fz.pro = FUSE_GATHER_IN;
+            fz.ga = gather_d;
         }
         launch_ntt_f64(t, true, c_base, coef, c_stride, l * N, N, N, nb, l, t.ident, fz, s);
         modup_cols(c, l, nb, coef, compact);
@@ -669,7 +682,9 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
                    coef, l * N, compact, (l + K) * N, cz, nb, s);
     }
     std::vector<KsIpArgs> ip(nb);
-    for (u32 z = 0; z < nb; ++z) ip[z] = {c_base + (u64)z * c_stride, keys[z]};
+    for (u32 z = 0; z < nb; ++z)
+        ip[z] = gather ? KsIpArgs{(*gather)[z].in, keys[z], (*gather)[z].galois}
+                       : KsIpArgs{c_base + (u64)z * c_stride, keys[z], 0};
     auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
     if (!ip_d) return FHEA_ERR_DEVICE;
     // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
@@ -690,6 +705,7 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
             ka.ta = fold_d;
             ka.pmodq = t.pmodqf;
         }
+        ka.gather = gather ? 1 : 0;
         launch_ks_rows_f64(t, ka, s);
     } else if (!t.use_f64 && mixed_fused(t) && !(t.f64_off & (4 | 64)) && np <= 8 && ks_rows_fusable(t) &&
                t.q_big_prefix < l) {
@@ -989,6 +1005,15 @@ static bool tensor_folded(const fhea_ctx *c, u32 l) {
            ks_rows_fusable(t) && moddown_cc(c);
 }
 
+// Rotations with sigma_g applied while loading (no automorphism launch): the FP64
+// path with the fused ModUp, the fused rows kernel and the fused ModDown row pass
+// (FHEA_F64_OFF bit 4096 keeps the automorphism kernel).
+static bool ks_gather_ok(const fhea_ctx *c, u32 l) {
+    const DevTables &t = c->dt;
+    return t.use_f64 && !(t.f64_off & (1 | 4 | 16 | 64 | 4096)) && c->lv[l].nparts <= 8 && ks_rows_fusable(t) &&
+           modup_cc(c, l) && moddown_cc(c);
+}
+
 // Swaps the context's stream and workspace with the second lane's.
 static void swap_lane(fhea_ctx *c) {
     std::swap(c->stream, c->stream2);
@@ -1168,6 +1193,9 @@ static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const std::vecto
         launch_ntt(t, false, compact, compact, cz, cz, ni, np * (l + K), lt.modup_ntt_prime, s);
     }
     std::stable_sort(items.begin(), items.end(), [](const RotItem &a, const RotItem &b) { return a.key < b.key; });
+    // sigma_g(c0) read by the fused ModDown row pass (FP64 path with the fused ModDown
+    // conversion) instead of an automorphism launch; outputs never overlap inputs here
+    const bool add_gather = t.use_f64 && !(t.f64_off & (16 | 4096)) && moddown_cc(c);
     for (size_t off = 0; off < items.size(); off += nmax) {
         const u32 nb = (u32)std::min<size_t>(nmax, items.size() - off);
         u64 *S = rest, *acc = S + (u64)nb * 2 * l * N, *pb = acc + (u64)nb * 2 * (l + K) * N;
@@ -1181,12 +1209,17 @@ static int rotations_hoisted(fhea_ctx *c, u32 l, u32 key_limbs, const std::vecto
             u64 *s0 = S + (u64)z * 2 * l * N;
             aa[z] = {in.in0, nullptr, s0, nullptr, it.galois};  // sigma_g(c0) only
             ra[z] = {in.in1, it.key, it.galois, compact + (u64)it.input * cz};
-            cb[z] = {it.d0, it.d1, s0, nullptr};
+            // the fused ModDown row pass reads c0 through sigma_g itself
+            cb[z] = add_gather ? CombineArgs{it.d0, it.d1, in.in0, nullptr, it.galois}
+                               : CombineArgs{it.d0, it.d1, s0, nullptr, 0};
         }
-        auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
         auto *ra_d = (const KsRotArgs *)stage_args(c, ra.data(), nb * sizeof(KsRotArgs));
-        if (!aa_d || !ra_d) return FHEA_ERR_DEVICE;
-        launch_automorph(t, aa_d, nb, l, s, 1);
+        if (!ra_d) return FHEA_ERR_DEVICE;
+        if (!add_gather) {
+            auto *aa_d = (const AutoArgs *)stage_args(c, aa.data(), nb * sizeof(AutoArgs));
+            if (!aa_d) return FHEA_ERR_DEVICE;
+            launch_automorph(t, aa_d, nb, l, s, 1);
+        }
         launch_ks_inner_rot(t, ra_d, acc, 2 * (l + K) * N, nb, l, c->h.alpha, np, key_limbs, c->h.dnum, s);
         st = ks_moddown(c, l, nb, acc, pb, tt, cb);
         if (st) return st;
@@ -1205,8 +1238,34 @@ struct AutoItem {
 // elements: sigma(c0), sigma(c1) -> KS(sigma(c1)) -> (sigma(c0) + ks0, ks1).
 static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<AutoItem> &items) {
     const u64 N = c->h.N;
+    const bool gok = ks_gather_ok(c, l);
     for (size_t off = 0; off < items.size(); off += c->max_chunk) {
         u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        // sigma_g while loading, unless an item writes over its own c0 (its last kernel
+        // reads c0 through sigma_g while it writes the outputs)
+        bool gather = gok;
+        for (u32 z = 0; z < nb && gather; ++z) {
+            const AutoItem &it = items[off + z];
+            const u64 W = (u64)l * N;
+            auto ov = [W](const u64 *a, const u64 *b) { return a < b + W && b < a + W; };
+            gather = !ov(it.d0, it.in0) && !ov(it.d1, it.in0);
+        }
+        if (gather) {
+            int st = ensure_ws(c, nb * ks_words(c, l));
+            if (st) return st;
+            std::vector<GatherArgs> ga(nb);
+            std::vector<CombineArgs> cb(nb);
+            std::vector<const u64 *> keys(nb);
+            for (u32 z = 0; z < nb; ++z) {
+                const AutoItem &it = items[off + z];
+                ga[z] = {it.in1, it.galois};
+                cb[z] = {it.d0, it.d1, it.in0, nullptr, it.galois};
+                keys[z] = it.key;
+            }
+            st = ks_core(c, l, key_limbs, nb, nullptr, 0, keys, cb, c->ws, nullptr, nullptr, &ga);
+            if (st) return st;
+            continue;
+        }
         u64 swords = (u64)nb * 2 * l * N;
         int st = ensure_ws(c, swords + nb * ks_words(c, l));
         if (st) return st;

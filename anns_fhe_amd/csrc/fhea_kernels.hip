@@ -7,6 +7,7 @@
 // wave-uniform and come through the scalar cache.
 #include "fhea_f64.hpp"
 #include "fhea_kernels.hpp"
+#include "fhea_ntt_tile.hpp"
 
 namespace fhea {
 
@@ -98,12 +99,6 @@ void launch_automorph_map(const DevTables &t, const u64 *in, u64 *out, const u64
 //   map[i] = br(((g (2 br(i) + 1)) mod 2N - 1) / 2)
 // (the evaluation-domain image of a(X) -> a(X^g) in bit-reversed slot order).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ u32 galois_src(u32 i, u64 g, u32 logN) {
-    const u32 br = __brev(i) >> (32 - logN);
-    const u64 e = (g * (2ull * br + 1)) & ((2ull << logN) - 1);
-    const u32 k = (u32)((e - 1) >> 1);
-    return __brev(k) >> (32 - logN);
-}
 
 __global__ __launch_bounds__(TPB) void k_automorph(const AutoArgs *__restrict__ args, u32 limbs,
                                                    u32 logN) {

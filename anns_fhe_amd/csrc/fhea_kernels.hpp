@@ -23,7 +23,8 @@ struct DevTables {
     // 256: slot-major tile walk for the first pass of rescale-lift prologues;
     // 512: INTT column pass, basis conversion and NTT column pass as three launches
     // instead of the fused conversion kernel (fhea_conv_cols.hip); 1024: the HMult
-    // tensor in full instead of folded into the key switch (tensor_folded)
+    // tensor in full instead of folded into the key switch (tensor_folded); 4096:
+    // rotations through the automorphism kernel instead of sigma_g applied while loading
     bool use_f64 = false;
     u32 f64_off = 0;
     // kernel variant of the fused conversion column kernel (FHEA_CC_VARIANT)
@@ -58,6 +59,12 @@ struct TensorArgs {
     const u64 *a0, *a1, *b0, *b1;
     u64 *d0, *d1, *d2;
 };
+// A polynomial read through sigma_g: word j of the result is in[galois_src(j)]
+// (automorphism.cpp:3-12 with the closed-form map).
+struct GatherArgs {
+    const u64 *in;
+    u64 galois;
+};
 enum : int {
     FUSE_NONE = 0,
     FUSE_RESCALE_IN = 1,
@@ -66,6 +73,7 @@ enum : int {
     FUSE_MDRS_IN = 4,   // ModDown + rescale in one transform: prologue
     FUSE_MDRS_OUT = 5,  // ... and epilogue
     FUSE_PROD_IN = 6,   // inverse first pass: input word = a1 * b1 (the HMult tensor's d2, from ta[poly])
+    FUSE_GATHER_IN = 7, // inverse first pass: input word j = in[galois_src(j)] (sigma_g applied on the fly, ga[poly])
 };
 // Fused prologue (first forward pass) / epilogue (last forward pass) of the
 // FP64 transform; poly = batch index of the NTT launch.
@@ -91,6 +99,8 @@ struct NttFuse {
     int unit_major = 0;
     // FUSE_PROD_IN: per poly the factors (a1, b1 of TensorArgs; slot s at + s*src_slot)
     const TensorArgs *ta = nullptr;
+    // FUSE_GATHER_IN: per poly the source polynomial and the Galois element
+    const GatherArgs *ga = nullptr;
     // two-pass transforms (N > 4096): 0 both passes; 1 only the first (its output,
     // FP64 intermediates, left at dst); 2 only the second (reading such intermediates
     // at src)
@@ -119,6 +129,7 @@ struct KsRowsArgs {
     // d1 = a0 b1 + a1 b0: the ModDown (acc - NTT(conv)) P^-1 then yields ModDown(acc) + d
     const TensorArgs *ta = nullptr;
     const double *pmodq = nullptr;  // DevTables::pmodqf
+    u32 gather = 0;  // the items' own limbs are read through sigma_g (KsIpArgs::galois)
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);
@@ -215,6 +226,7 @@ void launch_fbc(const DevTables &t, const ConvDesc *descs_dev, const u32 *in_fir
 struct KsIpArgs {
     const u64 *c;      // input polynomial (eval), l limbs
     const u64 *key;    // evk base (a then b)
+    u64 galois = 0;    // fused rows kernel: c read through sigma_g (0: as is)
 };
 // shared_key: every item uses args_dev[0].key (read once per batch).
 // nslots > 0: only QlP slots slot0 .. slot0+nslots-1, on the integer kernel.
@@ -227,6 +239,7 @@ void launch_ks_inner(const DevTables &t, const KsIpArgs *args_dev, const u64 *co
 struct CombineArgs {
     u64 *out0, *out1;
     const u64 *add0, *add1;  // may be null
+    u64 add0_galois = 0;     // FP64 fused epilogue: add0 read through sigma_g (0: as is)
 };
 void launch_moddown_combine(const DevTables &t, const CombineArgs *args_dev, const u64 *acc,
                             u64 acc_zstride, u64 acc_pstride, const u64 *tt, u64 tt_zstride,

@@ -213,7 +213,10 @@ __device__ __forceinline__ void store_out(char *dbase, u32 j, double v, int out_
         // x - v: |x| < q, |v| <= q/2 + 1  ->  |x - v| < 2q
         double r = mulmod_f64(x - v, fz.cst_out[2 * slot], fz.cst_out[2 * slot + 1], q);
         const u64 *add = half ? A.add1 : A.add0;
-        if (add) r += u64_to_f64(add[off]);
+        if (add) {
+            const u64 ag = half ? 0 : A.add0_galois;
+            r += u64_to_f64(add[ag ? ((size_t)slot << logN) + galois_src(j, ag, logN) : off]);
+        }
         (half ? A.out1 : A.out0)[off] = f64_to_canon(reduce_f64(r, q, qinv), q);
         return;
     }
@@ -243,6 +246,8 @@ __device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const 
     const double co = fz.cst_out[2 * slot], coq = fz.cst_out[2 * slot + 1];
     const double cp = (epi == FUSE_MDRS_OUT) ? fz.cst_pq[2 * slot] : 0.0;
     const double cpq = (epi == FUSE_MDRS_OUT) ? fz.cst_pq[2 * slot + 1] : 0.0;
+    // the ModDown's add0 read through sigma_g (a rotation's sigma_g(c0))
+    const u64 ag = (epi == FUSE_MODDOWN_OUT && !half) ? C.add0_galois : 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         u64 w1[8], w2[8];
@@ -250,7 +255,10 @@ __device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const 
         for (int k = 0; k < 8; ++k) w1[k] = a1[g.gidx(tid + (u32)(8 * h + k) * TPB, logR)];
         if (a2) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) w2[k] = a2[g.gidx(tid + (u32)(8 * h + k) * TPB, logR)];
+            for (int k = 0; k < 8; ++k) {
+                const u32 j = g.gidx(tid + (u32)(8 * h + k) * TPB, logR);
+                w2[k] = a2[ag ? galois_src(j, ag, logN) : j];
+            }
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -276,9 +284,10 @@ __device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const 
 
 // The kernel is persistent (find_tile) and issues the next tile's global loads
 // before the current tile's arithmetic, so loads overlap FP64 work.
-// PROD (inverse first pass only): the input word is the product a1 * b1 of fz.ta[poly]
-// (FUSE_PROD_IN), both factors prefetched.
-template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, bool PROD = false>
+// PRO (inverse first pass only): 1 the input word is the product a1 * b1 of fz.ta[poly]
+// (FUSE_PROD_IN), both factors prefetched; 2 it is in[galois_src(j)] of fz.ga[poly]
+// (FUSE_GATHER_IN, sigma_g applied while loading).
+template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, int PRO = 0>
 __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
@@ -312,9 +321,20 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
         gg.c0 = (id.tI & ((1u << ncgl) - 1)) << gg.logC;
         gg.b0 = (id.tI >> ncgl) << logNB;
     };
+    constexpr bool PROD = PRO == 1;
     auto issue = [&](const TileId &id, u64 (&r)[16], u64 (&r2)[PROD ? 16 : 1]) {
         TileGeom gg = g;
         geom(id, gg);
+        if constexpr (PRO == 2) {
+            const GatherArgs G = fz.ga[id.poly];
+            const auto *pin = gptr(G.in + (size_t)id.slot * src_slot);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const u32 e = tid + (u32)k * TPB;
+                if (e < T) r[k] = pin[galois_src(gg.gidx(e, LOGR), G.galois, logN)];
+            }
+            return;
+        }
         if constexpr (PROD) {
             const TensorArgs TA = fz.ta[id.poly];
             const auto *pa = gptr(TA.a1 + (size_t)id.slot * src_slot), *pb = gptr(TA.b1 + (size_t)id.slot * src_slot);
@@ -351,7 +371,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
                              : find_tile(t, n_tiles, tiles_per_limb, npolys, prime_map, fz.slots, fz.pmask);
     };
     TileId cur = next_tile(blockIdx.x);
-    static_assert(!PROD || (INV && !COLS && !DIRECT_IN), "product prologue: inverse first (row) pass");
+    static_assert(PRO == 0 || (INV && !COLS && !DIRECT_IN), "product / gather prologue: inverse first (row) pass");
     u64 raw[16], raw2[PROD ? 16 : 1];
     if (PF && cur.tile < n_tiles) issue(cur, raw, raw2);
     while (cur.tile < n_tiles) {
@@ -489,7 +509,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     }
 }
 
-template <bool INV, bool COLS, bool PROD = false>
+template <bool INV, bool COLS, int PRO = 0>
 static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
@@ -503,7 +523,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         static int per_cu = 0;                                                                       \
         if (per_cu == 0) {                                                                           \
             int n = 0;                                                                               \
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PROD>, F_TPB, 0) != \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO>, F_TPB, 0) != \
                     hipSuccess || n <= 0)                                                            \
                 n = 4;                                                                               \
             per_cu = n < 4 ? n : 4;                                                                  \
@@ -511,7 +531,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         const u32 use_cu = (wg_cap && wg_cap < (u32)per_cu) ? wg_cap : (u32)per_cu;                   \
         const u32 cap = max_wg / 4 * use_cu;                                                          \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
-        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PROD>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
+        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
                            dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles,   \
                            npolys);                                                                  \
         break;                                                                                       \
@@ -615,12 +635,17 @@ void launch_ntt_f64(const DevTables &t, bool inverse, const u64 *src, u64 *dst, 
                                       t.ninvf, t.mp, lg, F_LOG_TILE, 0, 1, 0, epi);
     } else {
         // FUSE_PROD_IN: the first pass reads its input as the product a1 * b1 (fz.ta)
+        // FUSE_GATHER_IN: ... as sigma_g of fz.ga's polynomials
         NttFuse prod = none;
         prod.pro = fz.pro;
         prod.ta = fz.ta;
+        prod.ga = fz.ga;
         if (first && fz.pro == FUSE_PROD_IN)
-            launch_pass<true, false, true>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm,
-                                           tw, t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, prod);
+            launch_pass<true, false, 1>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm,
+                                        tw, t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, prod);
+        else if (first && fz.pro == FUSE_GATHER_IN)
+            launch_pass<true, false, 2>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm,
+                                        tw, t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, prod);
         else if (first)
             launch_pass<true, false>(k2, grid, npolys, t.max_wg, t.wg_per_cu, s, src, dst, ss, ds, sl, dl, limbs, pm, tw,
                                      t.ninvf, t.mp, lg, F_LOG_TILE, 1, 0, 0, none);
@@ -688,7 +713,10 @@ __device__ __forceinline__ void rows_to_lds(const double (&x)[16], double *lds, 
     __syncthreads();
 }
 
-template <int LOGRF, int LOGRI, bool PSLOT>
+// OWN: how an item's own limbs are read: 0 as words of c, 1 formed from the HMult
+// factors (ta: d2 = a1 b1, with P d0 / P d1 added), 2 through sigma_g (c read at
+// galois_src); one instance each, so each keeps its own register budget.
+template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
 __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
                                                      const ModParams *__restrict__ mp, u32 logN, u32 slot0,
@@ -721,7 +749,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
             const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);
             const auto *kb = ka + (size_t)a.dnum * kstride;
-            if (i >= start && i < start + sz && a.ta) {
+            if (OWN == 1 && i >= start && i < start + sz) {
                 // HMult with the tensor folded in: the own limb d2 = a1 b1, and
                 // acc0 += P d0, acc1 += P d1 (d0 = a0 b0, d1 = a0 b1 + a1 b0), every product
                 // of canonical words (mulmod_vv_f64, <= 0.875q); the sums are centred after
@@ -753,13 +781,15 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             }
             if (i >= start && i < start + sz) {
                 // the digit's own limb: the input itself (eval domain, canonical)
-                const auto *c = gptr(A.c + ((size_t)i << logN) + base + tid);
+                // (through sigma_g when the item's input is the unpermuted polynomial)
+                const auto *c = gptr(A.c + ((size_t)i << logN));
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
                     // groups of four elements: the loads of one group are in flight together
                     // (fewer live registers than all 48 at once)
                     if ((k & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-                    const double d = u64_to_f64(c[(size_t)k * F_TPB]);
+                    const u32 pos = base + tid + (u32)k * F_TPB;
+                    const double d = u64_to_f64(c[OWN == 2 ? galois_src(pos, A.galois, logN) : pos]);
                     s0[k] += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)k * F_TPB]), q, qinv);
                     s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
                 }
@@ -861,12 +891,12 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
     }
 }
 
-template <int LOGRF, int LOGRI, bool PSLOT>
-static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
+template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
+static void launch_ks_rows_own(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ks_rows_f64<LOGRF, LOGRI, PSLOT>, F_TPB, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ks_rows_f64<LOGRF, LOGRI, PSLOT, OWN>, F_TPB, 0) !=
                 hipSuccess ||
             n <= 0)
             n = 2;
@@ -875,8 +905,18 @@ static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slo
     const u32 n_work = nslots * a.nb * (1u << (t.logN - F_LOG_TILE));
     const u32 cap = t.max_wg / 4 * (u32)per_cu;
     const u32 nwg = n_work < cap ? n_work : cap;
-    hipLaunchKernelGGL((k_ks_rows_f64<LOGRF, LOGRI, PSLOT>), dim3(nwg), dim3(F_TPB), 0, s, a, t.twf, t.itwf, t.mp,
+    hipLaunchKernelGGL((k_ks_rows_f64<LOGRF, LOGRI, PSLOT, OWN>), dim3(nwg), dim3(F_TPB), 0, s, a, t.twf, t.itwf, t.mp,
                        t.logN, slot0, n_work);
+}
+
+template <int LOGRF, int LOGRI, bool PSLOT>
+static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
+    if (a.ta)
+        launch_ks_rows_own<LOGRF, LOGRI, PSLOT, 1>(t, a, slot0, nslots, s);
+    else if (a.gather)
+        launch_ks_rows_own<LOGRF, LOGRI, PSLOT, 2>(t, a, slot0, nslots, s);
+    else
+        launch_ks_rows_own<LOGRF, LOGRI, PSLOT, 0>(t, a, slot0, nslots, s);
 }
 
 // second-pass lengths (stages along rows) of the current forward / inverse splits

@@ -18,6 +18,16 @@ __device__ __forceinline__ u32 opaque_tid(u32 v) {
     return v;
 }
 
+// Source index of word i of sigma_g(a) in the bit-reversed evaluation order:
+//   map[i] = br(((g (2 br(i) + 1)) mod 2N - 1) / 2)
+// (the image of a(X) -> a(X^g); automorphism.cpp:3-12 reads a[map[i]]).
+__device__ __forceinline__ u32 galois_src(u32 i, u64 g, u32 logN) {
+    const u32 br = __brev(i) >> (32 - logN);
+    const u64 e = (g * (2ull * br + 1)) & ((2ull << logN) - 1);
+    const u32 k = (u32)((e - 1) >> 1);
+    return __brev(k) >> (32 - logN);
+}
+
 // LDS index of tile element e, padded by one word per 16 (bank spread)
 __device__ __forceinline__ u32 lds_pad(u32 e) { return e + (e >> 4); }
 

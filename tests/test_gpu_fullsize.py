@@ -182,7 +182,7 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
 
 
 @pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (128, 0), (256, 0), (64, 0), (512, 0),
-                                               (512 | 128, 0), (1024, 0), (1024 | 128, 0),
+                                               (512 | 128, 0), (1024, 0), (1024 | 128, 0), (4096, 0),
                                                (1 | 2 | 4 | 8 | 16 | 32, 0)])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
@@ -191,7 +191,8 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     rescale lift; 64: the inner product as its own kernel; 512: the INTT column
     pass, basis conversion and NTT column pass as three launches instead of the
     fused conversion kernel; 1024: the HMult tensor in full instead of folded into
-    the key switch; 1..32: each component on the integer kernels), and
+    the key switch; 4096: rotations through the automorphism kernel instead of sigma_g
+    applied while loading; 1..32: each component on the integer kernels), and
     FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting
     gives the oracle's words for a batched HMult (3 items, one lane and two) and a
     rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
@@ -234,6 +235,12 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
         assert o.eval_rotate(pm, km, [(1, g1, ksz, L)], 0, W, 20 * W, 21 * W, 1, L) == 0
         assert np.array_equal(ctx.read_poly(20 * W, W), pm[20 * W:21 * W])
         assert np.array_equal(ctx.read_poly(21 * W, W), pm[21 * W:22 * W])
+        # in place (outputs over the inputs): sigma_g(c0) cannot be read while the
+        # outputs are written, so this takes the automorphism-kernel route
+        say("switch", switch, "rotate in place")
+        drv.eval_automorphism_ks(g1, ksz, L, 20 * W, 21 * W, 20 * W, 21 * W, L)
+        assert o.eval_automorphism_ks(pm, km, g1, ksz, L, 20 * W, 21 * W, 20 * W, 21 * W, L) == 0
+        assert np.array_equal(ctx.read_poly(20 * W, 2 * W), pm[20 * W:22 * W])
     finally:
         ctx.close()
 

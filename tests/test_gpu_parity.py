@@ -509,11 +509,12 @@ def test_submit_mult_relin_and_add_parity(orc):
             assert np.array_equal(ctx.read_poly(off * W, limbs * N), pm[off * W:off * W + limbs * N]), (limbs, off)
 
 
-@pytest.mark.parametrize("q_bits", [50, 60, "mixed"])
-def test_hoisted_rotations_parity(orc, q_bits):
+@pytest.mark.parametrize("q_bits,N", [(50, 4096), (60, 4096), ("mixed", 4096), (50, 16384)])
+def test_hoisted_rotations_parity(orc, q_bits, N):
     """Work kind 7 (runs of rotations of the same input share one ModUp) against the
-    oracle's hoisted rotations, two inputs in one submit, both arithmetic paths."""
-    N, L = 4096, 4
+    oracle's hoisted rotations, two inputs in one submit, both arithmetic paths (at
+    N=2^14 the fused ModDown row pass reads sigma_g(c0) itself)."""
+    L = 4
     ctx = get_ctx(N, L, 2, 2, q_bits=q_bits)
     o = get_orc(orc, ctx)
     rng = np.random.default_rng(seed_of(q_bits) + 1)

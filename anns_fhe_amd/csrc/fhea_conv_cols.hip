@@ -34,6 +34,16 @@
 namespace fhea {
 
 
+// Values every lane holds equally (read from LDS with a uniform address) moved to
+// scalar registers: the FP64 instructions take them as their SGPR operand and
+// addresses built from them stay scalar.
+__device__ __forceinline__ u32 uniform_u32(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)b), hi = __builtin_amdgcn_readfirstlane((u32)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Round RD of a LOGR-stage column transform in W-bit windows.  Forward stages run
 // on row bits LOGR-1 .. 0, inverse stages on 0 .. LOGR-1; P is the window's low
 // bit (a short last round keeps a full-width window, shifted).
@@ -72,10 +82,11 @@ __device__ __forceinline__ void round_w(double (&x)[1 << W], const double *__res
         constexpr int MAXP = E / 2;
         double w[MAXP], wq[MAXP];
         const int np = E >> (wb + 1);
+        const double2 *tb = tw2 + wbase;  // wbase has its low bits clear: + t
 #pragma unroll
         for (int t = 0; t < MAXP; ++t) {
             if (t < np) {
-                const double2 pr = tw2[wbase | (u32)t];
+                const double2 pr = tb[t];
                 w[t] = pr.x;
                 wq[t] = pr.y;
             }
@@ -110,16 +121,20 @@ __device__ __forceinline__ void round_w(double (&x)[1 << W], const double *__res
 // LDS buffers; the buffers alternate, so one barrier per exchange suffices (a
 // buffer is written again only after the barrier of the exchange that followed
 // its last reads).
+// (element u sits at lds_pad(eb) + (u << P) + ((u << P) >> 4): eb and u << P have
+// disjoint bits, so the padded index splits into a per-thread base and a constant,
+// which becomes the LDS instruction's immediate offset)
 template <bool INV, int LOGR, int W, int LOGC, int RDP, int RDN>
 __device__ __forceinline__ void exch(double (&x)[1 << W], double *buf, u32 tid) {
     constexpr int E = 1 << W;
     constexpr u32 PP = LOGC + RoundW<INV, LOGR, W, RDP>::P, PN = LOGC + RoundW<INV, LOGR, W, RDN>::P;
-    const u32 ebp = elem_base<W>(tid, PP), ebn = elem_base<W>(tid, PN);
+    double *bp = buf + lds_pad(elem_base<W>(tid, PP));
+    const double *bn = buf + lds_pad(elem_base<W>(tid, PN));
 #pragma unroll
-    for (int u = 0; u < E; ++u) buf[lds_pad(ebp | ((u32)u << PP))] = x[u];
+    for (int u = 0; u < E; ++u) bp[((u32)u << PP) + (((u32)u << PP) >> 4)] = x[u];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < E; ++u) x[u] = buf[lds_pad(ebn | ((u32)u << PN))];
+    for (int u = 0; u < E; ++u) x[u] = bn[((u32)u << PN) + (((u32)u << PN) >> 4)];
 }
 
 // global word index (within the limb) of tile element e: row e >> LOGC, column c0 + (e mod 2^LOGC)
@@ -236,13 +251,13 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
 #pragma unroll
                     for (int u = 0; u < E; ++u) raw[u] = s1[jl + (((u32)u * rstep_in) << logS)];
                 }
-                const u32 pi = s_iprime[i];
+                const u32 pi = uniform_u32(s_iprime[i]);
                 const double2 qq = s_iq[2 * i], cc = s_iq[2 * i + 1];
-                const double q = qq.x, qinv = qq.y;
+                const double q = uniform_f64(qq.x), qinv = uniform_f64(qq.y);
                 const double *tw = itwf + ((size_t)pi << (logN + 1));
                 round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jl);
                 rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
-                const double cs = cc.x, csq = cc.y;
+                const double cs = uniform_f64(cc.x), csq = uniform_f64(cc.y);
 #pragma unroll
                 for (int u = 0; u < E; ++u) {
                     const double v = reduce_f64(mulmod_f64(x[u], cs, csq, q), q, qinv);
@@ -290,7 +305,10 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             const u32 lj = a.lift_j;
             double2 hv[CC_MAX_IN];
 #pragma unroll
-            for (int i = 0; i < (int)CC_MAX_IN; ++i) hv[i] = s_hat[i * CC_MAX_OUT + lj];
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) {
+                const double2 hh = s_hat[i * CC_MAX_OUT + lj];
+                hv[i] = make_double2(uniform_f64(hh.x), uniform_f64(hh.y));
+            }
 #pragma unroll
             for (int i = 0; i < (int)CC_MAX_IN; ++i) {
                 if ((u32)i < n_in) {
@@ -317,12 +335,15 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
         const u32 jend = (a.j0 + a.nj < n_out) ? a.j0 + a.nj : n_out;
         double *dst_base = a.out + (size_t)poly * a.out_pstride + (size_t)d * a.out_dstride;
         for (u32 j = a.j0; j < jend; ++j) {
-            const u32 pj = s_oprime[j];
+            const u32 pj = uniform_u32(s_oprime[j]);
             const double2 qq = s_oq[j];
-            const double q = qq.x, qinv = qq.y;
+            const double q = uniform_f64(qq.x), qinv = uniform_f64(qq.y);
             double2 hv[CC_MAX_IN];
 #pragma unroll
-            for (int i = 0; i < (int)CC_MAX_IN; ++i) hv[i] = s_hat[i * CC_MAX_OUT + j];
+            for (int i = 0; i < (int)CC_MAX_IN; ++i) {
+                const double2 hh = s_hat[i * CC_MAX_OUT + j];
+                hv[i] = make_double2(uniform_f64(hh.x), uniform_f64(hh.y));
+            }
             double x[E];
 #pragma unroll
             for (int u = 0; u < E; ++u) x[u] = 0.0;
@@ -342,7 +363,8 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             if (MDRS) {
                 // c*T_j - conv_j*(P q_l)^-1 (output j is Q slot j)
                 const double2 tt2 = s_md[2 * j], pq2 = s_md[2 * j + 1];
-                const double T = tt2.x, Tq = tt2.y, pq = pq2.x, pqq = pq2.y;
+                const double T = uniform_f64(tt2.x), Tq = uniform_f64(tt2.y);
+                const double pq = uniform_f64(pq2.x), pqq = uniform_f64(pq2.y);
 #pragma unroll
                 for (int u = 0; u < E; ++u)
                     x[u] = reduce_f64(mulmod_f64(cl[u], T, Tq, q) - mulmod_f64(x[u], pq, pqq, q), q, qinv);

@@ -108,7 +108,7 @@ __device__ __forceinline__ void stage_twiddles(double (&w)[8], double (&wq)[8], 
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             if (t < npairs) {
-                const double2 pr = tw2[wbase | (u32)t];
+                const double2 pr = tw2[wbase + (u32)t];  // wbase has its low bits clear
                 w[t] = pr.x;
                 wq[t] = pr.y;
             }
@@ -313,7 +313,12 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     g.logS = COLS ? (logN - LOGR) : 0;
     const u32 logNB = logT - LOGR - g.logC;
     const u32 ncgl = g.logS - g.logC;  // log2 of column groups per limb
-    const u32 P0 = g.logC + R0::P;
+    // column bits below the row window: a COLS tile (only two-pass transforms, 4096-word
+    // tiles) has 2^(12 - LOGR) columns, a ROWS tile none; compile-time, so the LDS
+    // offsets of a thread's 16 elements are immediates (lds_pad(eb | u << P) =
+    // lds_pad(eb) + (u << P) + ((u << P) >> 4): disjoint bits)
+    constexpr u32 LC = COLS ? F_LOG_TILE - LOGR : 0;
+    constexpr u32 P0 = LC + R0::P;
     const u32 eb0 = (tid & ((1u << P0) - 1)) | ((tid >> P0) << (P0 + 4));
     const bool direct_out_possible = (COLS || RL::P >= 3);
 
@@ -415,7 +420,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
             __syncthreads();
             if (active) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << P0))];
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0) + ((u32)u << P0) + (((u32)u << P0) >> 4)];
             }
         }
         if (active) round_arith<INV, LOGR, 0, COLS>(x, g, g.gidx(eb0, LOGR));
@@ -424,61 +429,61 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
             {
                 using Rp = Round<INV, LOGR, 0>;
                 using R1 = Round<INV, LOGR, 1>;
-                const u32 Pp = g.logC + Rp::P, P1 = g.logC + R1::P;
+                constexpr u32 Pp = LC + Rp::P, P1 = LC + R1::P;
                 const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
                 const u32 eb1 = (tid & ((1u << P1) - 1)) | ((tid >> P1) << (P1 + 4));
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp) + ((u32)u << Pp) + (((u32)u << Pp) >> 4)] = x[u];
                 }
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1 | ((u32)u << P1))];
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb1) + ((u32)u << P1) + (((u32)u << P1) >> 4)];
                     round_arith<INV, LOGR, 1, COLS>(x, g, g.gidx(eb1, LOGR));
                 }
             }
             if constexpr (NR > 2) {
                 using Rp = Round<INV, LOGR, 1>;
                 using R2 = Round<INV, LOGR, 2>;
-                const u32 Pp = g.logC + Rp::P, P2 = g.logC + R2::P;
+                constexpr u32 Pp = LC + Rp::P, P2 = LC + R2::P;
                 const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
                 const u32 eb2 = (tid & ((1u << P2) - 1)) | ((tid >> P2) << (P2 + 4));
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp) + ((u32)u << Pp) + (((u32)u << Pp) >> 4)] = x[u];
                 }
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2 | ((u32)u << P2))];
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb2) + ((u32)u << P2) + (((u32)u << P2) >> 4)];
                     round_arith<INV, LOGR, 2, COLS>(x, g, g.gidx(eb2, LOGR));
                 }
             }
             if constexpr (NR > 3) {
                 using Rp = Round<INV, LOGR, 2>;
                 using R3 = Round<INV, LOGR, 3>;
-                const u32 Pp = g.logC + Rp::P, P3 = g.logC + R3::P;
+                constexpr u32 Pp = LC + Rp::P, P3 = LC + R3::P;
                 const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
                 const u32 eb3 = (tid & ((1u << P3) - 1)) | ((tid >> P3) << (P3 + 4));
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp) + ((u32)u << Pp) + (((u32)u << Pp) >> 4)] = x[u];
                 }
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb3 | ((u32)u << P3))];
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb3) + ((u32)u << P3) + (((u32)u << P3) >> 4)];
                     round_arith<INV, LOGR, 3, COLS>(x, g, g.gidx(eb3, LOGR));
                 }
             }
         }
         // ---- output of the last round
         {
-            const u32 P = g.logC + RL::P;
+            constexpr u32 P = LC + RL::P;
             const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
             const double ni = scale ? ninvf[2 * cur.pidx] : 0.0, niq = scale ? ninvf[2 * cur.pidx + 1] : 0.0;
             if (direct_out_possible && fz.epi == FUSE_NONE) {
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
                 __syncthreads();
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
+                    for (int u = 0; u < 16; ++u) lds[lds_pad(eb) + ((u32)u << P) + (((u32)u << P) >> 4)] = x[u];
                 }
                 __syncthreads();
                 if (!INV && !COLS && fz.epi != FUSE_NONE && T == 16 * TPB)
@@ -680,15 +685,15 @@ template <bool INV, int LOGR, bool COLS, int RD>
 __device__ __forceinline__ void next_round(double (&x)[16], double *lds, const TileGeom &g, u32 tid) {
     using Rp = Round<INV, LOGR, RD - 1>;
     using Rn = Round<INV, LOGR, RD>;
-    const u32 Pp = g.logC + Rp::P, Pn = g.logC + Rn::P;
+    constexpr u32 Pp = (COLS ? F_LOG_TILE - LOGR : 0) + Rp::P, Pn = (COLS ? F_LOG_TILE - LOGR : 0) + Rn::P;
     const u32 ebp = (tid & ((1u << Pp) - 1)) | ((tid >> Pp) << (Pp + 4));
     const u32 ebn = (tid & ((1u << Pn) - 1)) | ((tid >> Pn) << (Pn + 4));
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp | ((u32)u << Pp))] = x[u];
+    for (int u = 0; u < 16; ++u) lds[lds_pad(ebp) + ((u32)u << Pp) + (((u32)u << Pp) >> 4)] = x[u];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(ebn | ((u32)u << Pn))];
+    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(ebn) + ((u32)u << Pn) + (((u32)u << Pn) >> 4)];
     round_arith<INV, LOGR, RD, COLS>(x, g, g.gidx(ebn, LOGR));
 }
 
@@ -709,7 +714,7 @@ __device__ __forceinline__ void rows_to_lds(const double (&x)[16], double *lds, 
     const u32 eb = (tid & ((1u << P) - 1)) | ((tid >> P) << (P + 4));
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 16; ++u) lds[lds_pad(eb | ((u32)u << P))] = x[u];
+    for (int u = 0; u < 16; ++u) lds[lds_pad(eb) + ((u32)u << P) + (((u32)u << P) >> 4)] = x[u];
     __syncthreads();
 }
 
@@ -821,7 +826,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
                     for (int k = 0; k < 16; ++k) lds[lds_pad(tid + (u32)k * F_TPB)] = src[base + tid + (u32)k * F_TPB];
                     __syncthreads();
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << RF0::P))];
+                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0) + ((u32)u << RF0::P) + (((u32)u << RF0::P) >> 4)];
                 }
                 round_arith<false, LOGRF, 0, false>(x, g, g.gidx(eb0, LOGRF));
                 later_rounds<false, LOGRF, false>(x, lds, g, tid);
@@ -871,7 +876,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
                 double x[16];
                 const u32 eb0 = (tid & ((1u << RI0::P) - 1)) | ((tid >> RI0::P) << (RI0::P + 4));
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0 | ((u32)u << RI0::P))];
+                for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0) + ((u32)u << RI0::P) + (((u32)u << RI0::P) >> 4)];
                 round_arith<true, LOGRI, 0, false>(x, g, g.gidx(eb0, LOGRI));
                 later_rounds<true, LOGRI, false>(x, lds, g, tid);
                 if (RIL::P >= 3) {

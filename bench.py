@@ -812,6 +812,11 @@ def bench_hmult(args):
     # (the pipeline launches its transforms over whole batches the same way: 384 to
     # 4,608 limbs per launch at B=64)
     npr = min(4 * B, 64)
+    # the transform alone at its own default schedule: the HMult tuner may cap the
+    # transform workgroups per CU for two overlapping lanes, which a lone launch does not have
+    from anns_fhe_amd import _lib
+    wg_cap = ctx.get_tuning(_lib.TUNE_NTT_WG_PER_CU)
+    ctx.set_tuning(_lib.TUNE_NTT_WG_PER_CU, 0)
     ctx.ntt_batch(0, 0, npr, L)
     ctx.ntt_batch(0, 0, npr, L, inverse=True)
     ctx.sync()
@@ -824,6 +829,7 @@ def bench_hmult(args):
         ctx.ntt_batch(0, 0, npr, L, inverse=True)
         ctx.sync()
     ntt_launch_ms = ntt_ms / reps
+    ctx.set_tuning(_lib.TUNE_NTT_WG_PER_CU, wg_cap)
     alg_bytes = npr * L * 2 * N * 8  # npr polys x L limbs, each limb read once and written once
     achieved = alg_bytes / (ntt_launch_ms / 1e3) / 1e9
     log(f"NTT launch ({npr} polys x {L} limbs): {ntt_launch_ms:.3f} ms -> {achieved:.0f} GB/s algorithmic")
@@ -851,7 +857,8 @@ def bench_hmult(args):
                    "FP64 on the 45-bit limbs, 64-bit integer on the 60-bit limbs",
                    "schedule": schedule},
         "roofline": {"bound": "hbm",
-                     "kernel": f"batched forward NTT, {npr} polys x 24 limbs ({ntt_kernel} COLS + ROWS dispatches)",
+                     "kernel": f"batched forward NTT, {npr} polys x 24 limbs ({ntt_kernel} COLS + ROWS dispatches, "
+                               "workgroups per CU by occupancy)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(traffic_file),
                      "launch_ms": round(ntt_launch_ms, 4), "alg_bytes_per_launch": alg_bytes},

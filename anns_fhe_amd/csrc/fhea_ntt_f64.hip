@@ -287,16 +287,23 @@ __device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const 
 // PRO (inverse first pass only): 1 the input word is the product a1 * b1 of fz.ta[poly]
 // (FUSE_PROD_IN), both factors prefetched; 2 it is in[galois_src(j)] of fz.ga[poly]
 // (FUSE_GATHER_IN, sigma_g applied while loading).
-template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, int PRO = 0>
-__global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, void *dst, u64 src_stride,
+// FZ false: an instance for plain transforms (no fused prologue or epilogue compiled
+// in, so it keeps fewer registers); the tile walk fields of fz still apply.
+template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, int PRO = 0, bool FZ = true>
+__global__ __launch_bounds__(TPB) FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu((!FZ && !INV) ? 4 : 1))) void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
                                                   const double *__restrict__ twf,
                                                   const double *__restrict__ ninvf,
                                                   const ModParams *__restrict__ mp, u32 logN,
                                                   u32 logT, int in_canon, int out_canon, int scale,
-                                                  NttFuse fz, u32 tiles_per_limb, u32 n_tiles,
+                                                  NttFuse fz_in, u32 tiles_per_limb, u32 n_tiles,
                                                   u32 npolys) {
+    NttFuse fz = fz_in;
+    if constexpr (!FZ) {
+        fz.pro = FUSE_NONE;
+        fz.epi = FUSE_NONE;
+    }
     __shared__ double lds[TPB * 16 + TPB];  // one tile of 16 words per thread, padded
     constexpr int NR = (LOGR + 3) / 4;
     using R0 = Round<INV, LOGR, 0>;
@@ -514,8 +521,8 @@ __global__ __launch_bounds__(TPB) FHEA_NTT_ATTR void k_ntt64f(const void *src, v
     }
 }
 
-template <bool INV, bool COLS, int PRO = 0>
-static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+template <bool INV, bool COLS, int PRO, bool FZ>
+static void launch_pass_fz(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
                         u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
                         const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
     const u32 tpl = grid.x;
@@ -528,7 +535,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         static int per_cu = 0;                                                                       \
         if (per_cu == 0) {                                                                           \
             int n = 0;                                                                               \
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO>, F_TPB, 0) != \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO, FZ>, F_TPB, 0) != \
                     hipSuccess || n <= 0)                                                            \
                 n = 4;                                                                               \
             per_cu = n < 4 ? n : 4;                                                                  \
@@ -536,7 +543,7 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         const u32 use_cu = (wg_cap && wg_cap < (u32)per_cu) ? wg_cap : (u32)per_cu;                   \
         const u32 cap = max_wg / 4 * use_cu;                                                          \
         const u32 nwg = ntiles < cap ? ntiles : cap;                                                 \
-        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
+        hipLaunchKernelGGL((k_ntt64f<INV, LR, COLS, F_TPB, FHEA_NTT_PREFETCH, PRO, FZ>), dim3(nwg), dim3(F_TPB), 0, s, src, dst, ss, ds, sl,   \
                            dl, limbs, pm, twf, ninvf, mp, logN, logT, ic, oc, sc, fz, tpl, ntiles,   \
                            npolys);                                                                  \
         break;                                                                                       \
@@ -553,6 +560,18 @@ static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap,
         break;
     }
 #undef FHEA_NTT_CASE
+}
+
+template <bool INV, bool COLS, int PRO = 0>
+static void launch_pass(int logR, dim3 grid, u32 npolys, u32 max_wg, u32 wg_cap, hipStream_t s, const void *src, void *dst, u64 ss, u64 ds,
+                        u64 sl, u64 dl, u32 limbs, const u32 *pm, const double *twf, const double *ninvf,
+                        const ModParams *mp, u32 logN, u32 logT, int ic, int oc, int sc, const NttFuse &fz) {
+    if (PRO == 0 && fz.pro == FUSE_NONE && fz.epi == FUSE_NONE)
+        launch_pass_fz<INV, COLS, PRO, false>(logR, grid, npolys, max_wg, wg_cap, s, src, dst, ss, ds, sl, dl, limbs, pm,
+                                              twf, ninvf, mp, logN, logT, ic, oc, sc, fz);
+    else
+        launch_pass_fz<INV, COLS, PRO, true>(logR, grid, npolys, max_wg, wg_cap, s, src, dst, ss, ds, sl, dl, limbs, pm,
+                                             twf, ninvf, mp, logN, logT, ic, oc, sc, fz);
 }
 
 // One pass over a whole limb of N = 2^13 or 2^14: N/16 threads, the limb in LDS,

@@ -102,8 +102,12 @@ __device__ __forceinline__ void round_w(double (&x)[1 << W], const double *__res
                 x[u] = a0 + rr;
                 x[u2] = a0 - rr;
             } else {
+                // the sum is centred at the round's last stage only: from |x| <= q at
+                // the round's start (canonical inputs, or centred values and products
+                // <= q/2 + 1) it stays below 2^W q <= 8q, and a difference below 8q is
+                // multiplied back to <= q/2 + 1 (canonical twiddles)
                 const double a0 = x[u], b0 = x[u2];
-                x[u] = reduce_f64(a0 + b0, q, qinv);
+                x[u] = (st == R::NS - 1) ? reduce_f64(a0 + b0, q, qinv) : a0 + b0;
                 x[u2] = mulmod_f64(a0 - b0, ww, wwq, q);
             }
         }
@@ -121,20 +125,26 @@ __device__ __forceinline__ void round_w(double (&x)[1 << W], const double *__res
 // LDS buffers; the buffers alternate, so one barrier per exchange suffices (a
 // buffer is written again only after the barrier of the exchange that followed
 // its last reads).
-// (element u sits at lds_pad(eb) + (u << P) + ((u << P) >> 4): eb and u << P have
-// disjoint bits, so the padded index splits into a per-thread base and a constant,
-// which becomes the LDS instruction's immediate offset)
+// Tile element e sits at cc_pad(e) = e + 4 floor(e / 16): element bits 4 and 5 move
+// the address by 4 and 8 words mod 16, so in the exchanges whose lanes differ in
+// element bits {0, 1, 4, 5} (window at row bit 0 with 4 columns) as in those whose
+// lanes differ in bits {0, 1, 2, 3}, every 16 lanes of a 64-bit access fall on
+// distinct LDS banks.  cc_pad is additive over disjoint bits, so element u sits at
+// cc_pad(eb) + cc_pad(u << P): a per-thread base and a constant, which becomes the
+// LDS instruction's immediate offset.
+__device__ __forceinline__ constexpr u32 cc_pad(u32 e) { return e + ((e >> 4) << 2); }
+
 template <bool INV, int LOGR, int W, int LOGC, int RDP, int RDN>
 __device__ __forceinline__ void exch(double (&x)[1 << W], double *buf, u32 tid) {
     constexpr int E = 1 << W;
     constexpr u32 PP = LOGC + RoundW<INV, LOGR, W, RDP>::P, PN = LOGC + RoundW<INV, LOGR, W, RDN>::P;
-    double *bp = buf + lds_pad(elem_base<W>(tid, PP));
-    const double *bn = buf + lds_pad(elem_base<W>(tid, PN));
+    double *bp = buf + cc_pad(elem_base<W>(tid, PP));
+    const double *bn = buf + cc_pad(elem_base<W>(tid, PN));
 #pragma unroll
-    for (int u = 0; u < E; ++u) bp[((u32)u << PP) + (((u32)u << PP) >> 4)] = x[u];
+    for (int u = 0; u < E; ++u) bp[cc_pad((u32)u << PP)] = x[u];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < E; ++u) x[u] = bn[((u32)u << PN) + (((u32)u << PN) >> 4)];
+    for (int u = 0; u < E; ++u) x[u] = bn[cc_pad((u32)u << PN)];
 }
 
 // global word index (within the limb) of tile element e: row e >> LOGC, column c0 + (e mod 2^LOGC)
@@ -152,7 +162,7 @@ __device__ __forceinline__ void rounds_from(double (&x)[1 << W], double *lds, u3
     constexpr int NR = RoundW<INV, LOGR, W, 0>::NR;
     if constexpr (RD < NR) {
         constexpr u32 TILE = 1u << (LOGC + LOGR);
-        constexpr u32 PADT = TILE + TILE / 16;
+        constexpr u32 PADT = TILE + TILE / 4;  // cc_pad's span
         exch<INV, LOGR, W, LOGC, RD - 1, RD>(x, lds + xb * PADT, tid);
         xb ^= 1u;
         constexpr u32 PN = LOGC + RoundW<INV, LOGR, W, RD>::P;
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     const ModParams *__restrict__ mp, u32 logN, u32 n_work) {
     constexpr int E = 1 << W;
     constexpr u32 TILE = 1u << (LOGC + LOGR);
-    constexpr u32 PADT = TILE + TILE / 16;
+    constexpr u32 PADT = TILE + TILE / 4;  // cc_pad's span
     constexpr int NR = RoundW<true, LOGR, W, 0>::NR;
     constexpr u32 PI0 = LOGC + RoundW<true, LOGR, W, 0>::P;       // input layout (inverse round 0)
     constexpr u32 PC = LOGC + RoundW<false, LOGR, W, 0>::P;       // conversion layout
@@ -421,11 +431,15 @@ bool conv_cols_fusable(const DevTables &t, u32 max_in) {
 }
 
 // Variants (DevTables::cc_var, FHEA_CC_VARIANT): 0 radix-4 rounds on 4-column
-// tiles (256 threads at 2^8 rows; within 3 waves per SIMD, which the conversion
-// without the lift meets at 4); 1 8-column tiles (512 threads) at the compiler's
-// register choice; 2 8-column tiles within 4 waves per SIMD; 3 4-column tiles within
-// 4 waves per SIMD (the lift instance spills).  Measured on MI355X (HMult at 2^16,
-// B = 64, one lane): 6,468 / 6,394 / 6,429 / 6,315 ops/s (5,903 unfused).
+// tiles (256 threads at 2^8 rows) with a register budget of 3 waves per SIMD (131 /
+// 118 registers with and without the lift); 1 8-column tiles (512 threads) at the
+// compiler's register choice; 2 8-column tiles within 4 waves per SIMD; 3 4-column
+// tiles within 4 waves per SIMD (127 / 118 registers).  Measured on MI355X (HMult at
+// 2^16, B = 64, one lane): variant 0 8,621-8,707 ops/s, 3 8,433-8,469; budgets of 2
+// waves or the compiler's default (127 registers each) 8,537-8,595.  The budget
+// changes the instruction schedule more than the occupancy: in the build before
+// cc_pad the lift instance took 127 registers under both 3 and 4 and ran at 8,627
+// and 8,443 ops/s.
 template <bool MDRS>
 static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hipStream_t s) {
 #define FHEA_CC(LR)                                                                   \

@@ -181,7 +181,7 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
         ctx.close()
 
 
-@pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (128, 0), (256, 0), (64, 0), (512, 0),
+@pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (0, 3), (128, 0), (256, 0), (64, 0), (512, 0),
                                                (512 | 128, 0), (1024, 0), (1024 | 128, 0), (4096, 0),
                                                (1 | 2 | 4 | 8 | 16 | 32, 0)])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):

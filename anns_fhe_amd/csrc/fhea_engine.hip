@@ -530,7 +530,7 @@ struct RescaleAfter {
 };
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
                               const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
-                              bool folded = false);
+                              bool folded = false, const KsRowsArgs *late = nullptr);
 
 // Mixed chain whose Q primes past a leading run of >= 2^50 primes are all below
 // 2^50: the ModDown / rescale transforms take the FP64 fused ends on those limbs.
@@ -689,6 +689,13 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     if (!ip_d) return FHEA_ERR_DEVICE;
     // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
     const bool fused = t.use_f64 && !(t.f64_off & (1 | 4 | 64)) && np <= 8 && ks_rows_fusable(t);
+    // HMult (the tensor folded in, the rescale after, both conversions fused): the
+    // rows kernel runs twice, first for slot l-1 and the P slots, whose sums the
+    // ModDown + rescale transforms and converts, then for slots 0 .. l-2 with the
+    // last row pass of the ModDown + rescale and its epilogue fused in, so the Q
+    // slots' sums never go through HBM (FHEA_F64_OFF bit 2048 keeps one launch)
+    const bool late = fused && cc && fold && ra && l >= 2 && !(t.f64_off & (16 | 128 | 2048)) && moddown_cc(c);
+    KsRowsArgs ka_late{};
     if (fused) {
         // first forward pass of the converted digit limbs; the second runs inside the
         // inner product, which for the P slots also runs the ModDown INTT's first pass
@@ -706,7 +713,12 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
             ka.pmodq = t.pmodqf;
         }
         ka.gather = gather ? 1 : 0;
-        launch_ks_rows_f64(t, ka, s);
+        if (late) {
+            launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
+            ka_late = ka;
+        } else {
+            launch_ks_rows_f64(t, ka, s);
+        }
     } else if (!t.use_f64 && mixed_fused(t) && !(t.f64_off & (4 | 64)) && np <= 8 && ks_rows_fusable(t) &&
                t.q_big_prefix < l) {
         // mixed chain: the converted limbs of primes below 2^50 take the FP64 first
@@ -750,7 +762,9 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // in its own launch; FHEA_F64_OFF bit 128 runs its first pass inside the inner
     // product instead (one wave per SIMD there: measured 2.6% slower per HMult)
     const bool pb_mid = fused && (t.f64_off & 128);
-    if (ra) return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid, fold != nullptr);
+    if (ra)
+        return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid, fold != nullptr,
+                                  late ? &ka_late : nullptr);
     return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
 }
 
@@ -877,7 +891,7 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
 // linearity of the NTT mod q_i: the same words, one transform of l-1 limbs fewer.
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
                               const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
-                              bool folded) {
+                              bool folded, const KsRowsArgs *late) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -915,6 +929,17 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift, !folded);
         auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
         if (!ro_d) return FHEA_ERR_DEVICE;
+        if (late) {
+            // the Q slots' rows kernel with this row pass and epilogue in place of
+            // the stores of their sums
+            KsRowsArgs ka = *late;
+            ka.tt = (const double *)tt;
+            ka.tt_pstride = l * N;
+            ka.ro = ro_d;
+            ka.cst_pq = lt.rs_pqf;
+            launch_ks_rows_f64_q(t, ka, 0, l - 1, s);
+            return FHEA_OK;
+        }
         NttFuse f2;
         f2.epi = FUSE_MDRS_OUT;
         f2.cst_out = lt.rs_qlinvf;

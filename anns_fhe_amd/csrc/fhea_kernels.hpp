@@ -24,7 +24,9 @@ struct DevTables {
     // 512: INTT column pass, basis conversion and NTT column pass as three launches
     // instead of the fused conversion kernel (fhea_conv_cols.hip); 1024: the HMult
     // tensor in full instead of folded into the key switch (tensor_folded); 4096:
-    // rotations through the automorphism kernel instead of sigma_g applied while loading
+    // rotations through the automorphism kernel instead of sigma_g applied while loading;
+    // 2048: the HMult's Q-slot sums stored and the ModDown + rescale row pass as its own
+    // launch instead of fused into the key-switch rows kernel (KsRowsArgs::tt)
     bool use_f64 = false;
     u32 f64_off = 0;
     // kernel variant of the fused conversion column kernel (FHEA_CC_VARIANT)
@@ -130,6 +132,15 @@ struct KsRowsArgs {
     const TensorArgs *ta = nullptr;
     const double *pmodq = nullptr;  // DevTables::pmodqf
     u32 gather = 0;  // the items' own limbs are read through sigma_g (KsIpArgs::galois)
+    // the ModDown + rescale's last transform fused in (tt != null; with ta, Q slots
+    // only): the sums acc_s are not stored; the kernel runs the forward row pass of
+    // the ModDown conversion's column-pass output of item z, polynomial s, slot i at
+    // tt + (2z+s)*tt_pstride + i*N and stores acc_s (P q_l)^-1 + that row pass,
+    // canonical, to ro[2z+s].out + i*N (cst_pq: ((P q_l)^-1 mod q_i, / q_i))
+    const double *tt = nullptr;
+    u64 tt_pstride = 0;
+    const RescaleOut *ro = nullptr;
+    const double *cst_pq = nullptr;
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);

@@ -737,9 +737,37 @@ __device__ __forceinline__ void rows_to_lds(const double (&x)[16], double *lds, 
     __syncthreads();
 }
 
+// Forward row pass (LOGRF stages) of the 4096-word tile at src + (tile base), the
+// result left in lds in position order (element e at lds_pad(e)).  src holds
+// first-pass intermediates (|x| <= q/2 + 2).
+template <int LOGRF>
+__device__ __forceinline__ void fwd_rows_to_lds(const __attribute__((address_space(1))) double *src, double *lds,
+                                                const TileGeom &g, u32 tid, u32 base) {
+    using RF0 = Round<false, LOGRF, 0>;
+    double x[16];
+    const u32 eb0 = (tid & ((1u << RF0::P) - 1)) | ((tid >> RF0::P) << (RF0::P + 4));
+    if (RF0::P >= 3) {
+        const u32 jb = g.gidx(eb0, LOGRF);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = src[jb + ((u32)u << RF0::P)];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lds[lds_pad(tid + (u32)k * F_TPB)] = src[base + tid + (u32)k * F_TPB];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0) + ((u32)u << RF0::P) + (((u32)u << RF0::P) >> 4)];
+    }
+    round_arith<false, LOGRF, 0, false>(x, g, g.gidx(eb0, LOGRF));
+    later_rounds<false, LOGRF, false>(x, lds, g, tid);
+    rows_to_lds<false, LOGRF>(x, lds, tid);
+}
+
 // OWN: how an item's own limbs are read: 0 as words of c, 1 formed from the HMult
 // factors (ta: d2 = a1 b1, with P d0 / P d1 added), 2 through sigma_g (c read at
-// galois_src); one instance each, so each keeps its own register budget.
+// galois_src); 3 as 1, with the ModDown + rescale's last row pass and its epilogue
+// in place of the stores of the sums (KsRowsArgs::tt); one instance each, so each
+// keeps its own register budget.
 template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
 __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
@@ -751,8 +779,6 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
     const u32 N = 1u << logN;
     const u32 lk = a.l + a.K;
     const u64 kstride = (u64)(a.key_limbs + a.K) << logN;
-    using RF0 = Round<false, LOGRF, 0>;
-    constexpr bool DIRECT_F = RF0::P >= 3;
     for (u32 w = blockIdx.x; w < n_work; w += gridDim.x) {
         const u32 tid = opaque_tid(tid0);
         const u32 tI = w % T;
@@ -773,7 +799,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
             const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);
             const auto *kb = ka + (size_t)a.dnum * kstride;
-            if (OWN == 1 && i >= start && i < start + sz) {
+            if ((OWN == 1 || OWN == 3) && i >= start && i < start + sz) {
                 // HMult with the tensor folded in: the own limb d2 = a1 b1, and
                 // acc0 += P d0, acc1 += P d1 (d0 = a0 b0, d1 = a0 b1 + a1 b0), every product
                 // of canonical words (mulmod_vv_f64, <= 0.875q); the sums are centred after
@@ -832,25 +858,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             g.q = q;
             g.qinv = qinv;
             g.tw = twf + ((size_t)pidx << (logN + 1));
-            {
-                double x[16];
-                const u32 eb0 = (tid & ((1u << RF0::P) - 1)) | ((tid >> RF0::P) << (RF0::P + 4));
-                if (DIRECT_F) {
-                    const u32 jb = g.gidx(eb0, LOGRF);
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = src[jb + ((u32)u << RF0::P)];
-                } else {
-                    __syncthreads();
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) lds[lds_pad(tid + (u32)k * F_TPB)] = src[base + tid + (u32)k * F_TPB];
-                    __syncthreads();
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) x[u] = lds[lds_pad(eb0) + ((u32)u << RF0::P) + (((u32)u << RF0::P) >> 4)];
-                }
-                round_arith<false, LOGRF, 0, false>(x, g, g.gidx(eb0, LOGRF));
-                later_rounds<false, LOGRF, false>(x, lds, g, tid);
-                rows_to_lds<false, LOGRF>(x, lds, tid);
-            }
+            fwd_rows_to_lds<LOGRF>(src, lds, g, tid, base);
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 if ((k & 3) == 0) __builtin_amdgcn_sched_barrier(0);
@@ -860,7 +868,38 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
                 s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
             }
         }
-        if (!PSLOT) {
+        if (OWN == 3) {
+            // the ModDown + rescale: out_s = acc_s (P q_l)^-1 + NTT_rows(tt_s), where tt_s
+            // is the column-pass output of c T_i - conv_i (P q_l)^-1 (FUSE_MDRS_OUT);
+            // |acc (P q_l)^-1| <= q/2 + 1, |row pass| <= q/2 + 2
+            const u32 tid = opaque_tid(tid0);
+            TileGeom g;
+            g.logN = logN;
+            g.logT = F_LOG_TILE;
+            g.logC = 0;
+            g.logS = 0;
+            g.c0 = 0;
+            g.b0 = tI << (F_LOG_TILE - LOGRF);
+            g.q = q;
+            g.qinv = qinv;
+            g.tw = twf + ((size_t)pidx << (logN + 1));
+            // (item and slot through opaque_tid: their addresses are formed here, not
+            // kept live across the digit loop)
+            const u32 zo = opaque_tid(z), io = opaque_tid(i);
+            const double cp = a.cst_pq[2 * io], cpq = a.cst_pq[2 * io + 1];
+#pragma unroll
+            for (u32 sp = 0; sp < 2; ++sp) {
+                const auto *src = gptr(a.tt + (size_t)(2 * zo + sp) * a.tt_pstride + ((size_t)io << logN));
+                fwd_rows_to_lds<LOGRF>(src, lds, g, tid, base);
+                auto *o = gptr_w(a.ro[2 * zo + sp].out + ((size_t)io << logN) + base + tid);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const double v = lds[lds_pad(tid + (u32)k * F_TPB)];
+                    const double r = mulmod_f64(reduce_f64(sp ? s1[k] : s0[k], q, qinv), cp, cpq, q) + v;
+                    o[(size_t)k * F_TPB] = f64_to_canon(reduce_f64(r, q, qinv), q);
+                }
+            }
+        } else if (!PSLOT) {
             auto *o0 = gptr_w(a.acc + (size_t)z * a.az + ((size_t)i << logN) + base + tid);
             auto *o1 = o0 + ((size_t)lk << logN);
 #pragma unroll
@@ -935,6 +974,12 @@ static void launch_ks_rows_own(const DevTables &t, const KsRowsArgs &a, u32 slot
 
 template <int LOGRF, int LOGRI, bool PSLOT>
 static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
+    if constexpr (!PSLOT) {
+        if (a.ta && a.tt) {
+            launch_ks_rows_own<LOGRF, LOGRI, false, 3>(t, a, slot0, nslots, s);
+            return;
+        }
+    }
     if (a.ta)
         launch_ks_rows_own<LOGRF, LOGRI, PSLOT, 1>(t, a, slot0, nslots, s);
     else if (a.gather)

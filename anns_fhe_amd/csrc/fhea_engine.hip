@@ -518,7 +518,8 @@ static u64 ks_words(const fhea_ctx *c, u32 l) {
 static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2 * (l - 1)); }
 
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
-                      const std::vector<CombineArgs> &comb, bool pb_mid = false);
+                      const std::vector<CombineArgs> &comb, bool pb_mid = false,
+                      const KsRowsArgs *late = nullptr);
 
 // The rescale that follows a key switch in eval_mult_relin_rescale: the switch's
 // add operand d_z = (d_base + z*dz, + ds) is also its output (l limbs, limb l-1
@@ -632,7 +633,7 @@ static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
                    u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr,
-                   const std::vector<GatherArgs> *gather = nullptr) {
+                   const std::vector<GatherArgs> *gather = nullptr, bool gather_late = false) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -689,12 +690,15 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     if (!ip_d) return FHEA_ERR_DEVICE;
     // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
     const bool fused = t.use_f64 && !(t.f64_off & (1 | 4 | 64)) && np <= 8 && ks_rows_fusable(t);
-    // HMult (the tensor folded in, the rescale after, both conversions fused): the
-    // rows kernel runs twice, first for slot l-1 and the P slots, whose sums the
-    // ModDown + rescale transforms and converts, then for slots 0 .. l-2 with the
-    // last row pass of the ModDown + rescale and its epilogue fused in, so the Q
-    // slots' sums never go through HBM (FHEA_F64_OFF bit 2048 keeps one launch)
-    const bool late = fused && cc && fold && ra && l >= 2 && !(t.f64_off & (16 | 128 | 2048)) && moddown_cc(c);
+    // HMult (the tensor folded in, the rescale after) and rotations (sigma_g read
+    // while loading, gather_late: no output overlaps c1, which the second launch
+    // reads), both conversions fused: the rows kernel runs twice, first for the P
+    // slots (and slot l-1 before a rescale), whose sums the ModDown transforms and
+    // converts, then for the other Q slots with the last row pass of the ModDown
+    // (+ rescale) and its epilogue fused in, so the Q slots' sums never go through
+    // HBM (FHEA_F64_OFF bit 2048 keeps one launch)
+    const bool late_ok = fused && cc && !(t.f64_off & (16 | 128 | 2048)) && moddown_cc(c);
+    const bool late = late_ok && ((fold && ra && l >= 2) || (gather && gather_late && !ra));
     KsRowsArgs ka_late{};
     if (fused) {
         // first forward pass of the converted digit limbs; the second runs inside the
@@ -714,7 +718,8 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         }
         ka.gather = gather ? 1 : 0;
         if (late) {
-            launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
+            if (ra) launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
+            else launch_ks_rows_f64_q(t, ka, l, (u32)K, s);
             ka_late = ka;
         } else {
             launch_ks_rows_f64(t, ka, s);
@@ -765,13 +770,13 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     if (ra)
         return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid, fold != nullptr,
                                   late ? &ka_late : nullptr);
-    return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
+    return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid, late ? &ka_late : nullptr);
 }
 
 // ModDown of nb accumulator pairs acc_z = [2][l+K][N] (key_switch.cpp:385-427) and the
 // composite's add: out_z = add_z + ModDown(acc_z).  pb: 2*nb*K*N words, tt: 2*nb*l*N.
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
-                      const std::vector<CombineArgs> &comb, bool pb_mid) {
+                      const std::vector<CombineArgs> &comb, bool pb_mid, const KsRowsArgs *late) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -787,6 +792,17 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
     }
     auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
+    if (late && cc) {
+        // the Q slots' rows kernel with the last row pass and its epilogue in place of
+        // the stores of their sums
+        KsRowsArgs ka = *late;
+        ka.tt = (const double *)tt;
+        ka.tt_pstride = l * N;
+        ka.cb = cb_d;
+        ka.cst = lt.pinvf;
+        launch_ks_rows_f64_q(t, ka, 0, l, s);
+        return FHEA_OK;
+    }
     if (t.use_f64 && !(t.f64_off & 17)) {
         // NTT of the converted P part with (acc - t) * P^-1 [+ add] fused into its last pass
         NttFuse fz;
@@ -936,7 +952,7 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
             ka.tt = (const double *)tt;
             ka.tt_pstride = l * N;
             ka.ro = ro_d;
-            ka.cst_pq = lt.rs_pqf;
+            ka.cst = lt.rs_pqf;
             launch_ks_rows_f64_q(t, ka, 0, l - 1, s);
             return FHEA_OK;
         }
@@ -1268,12 +1284,15 @@ static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<
         u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
         // sigma_g while loading, unless an item writes over its own c0 (its last kernel
         // reads c0 through sigma_g while it writes the outputs)
-        bool gather = gok;
+        // (and with late: the rows kernel that writes the outputs also reads c1 through
+        // sigma_g, so no output may overlap c1 either)
+        bool gather = gok, late = true;
         for (u32 z = 0; z < nb && gather; ++z) {
             const AutoItem &it = items[off + z];
             const u64 W = (u64)l * N;
             auto ov = [W](const u64 *a, const u64 *b) { return a < b + W && b < a + W; };
             gather = !ov(it.d0, it.in0) && !ov(it.d1, it.in0);
+            late = late && !ov(it.d0, it.in1) && !ov(it.d1, it.in1);
         }
         if (gather) {
             int st = ensure_ws(c, nb * ks_words(c, l));
@@ -1287,7 +1306,7 @@ static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<
                 cb[z] = {it.d0, it.d1, it.in0, nullptr, it.galois};
                 keys[z] = it.key;
             }
-            st = ks_core(c, l, key_limbs, nb, nullptr, 0, keys, cb, c->ws, nullptr, nullptr, &ga);
+            st = ks_core(c, l, key_limbs, nb, nullptr, 0, keys, cb, c->ws, nullptr, nullptr, &ga, late);
             if (st) return st;
             continue;
         }

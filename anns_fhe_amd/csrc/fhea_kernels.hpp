@@ -132,15 +132,19 @@ struct KsRowsArgs {
     const TensorArgs *ta = nullptr;
     const double *pmodq = nullptr;  // DevTables::pmodqf
     u32 gather = 0;  // the items' own limbs are read through sigma_g (KsIpArgs::galois)
-    // the ModDown + rescale's last transform fused in (tt != null; with ta, Q slots
-    // only): the sums acc_s are not stored; the kernel runs the forward row pass of
-    // the ModDown conversion's column-pass output of item z, polynomial s, slot i at
-    // tt + (2z+s)*tt_pstride + i*N and stores acc_s (P q_l)^-1 + that row pass,
-    // canonical, to ro[2z+s].out + i*N (cst_pq: ((P q_l)^-1 mod q_i, / q_i))
+    // the ModDown's last transform fused in (tt != null, Q slots only): the sums
+    // acc_s are not stored; the kernel runs the forward row pass x of the ModDown
+    // conversion's column-pass output of item z, polynomial s, slot i at
+    // tt + (2z+s)*tt_pstride + i*N and stores, canonical,
+    //   with ta (ModDown + rescale): acc_s (P q_l)^-1 + x to ro[2z+s].out + i*N,
+    //     cst = ((P q_l)^-1 mod q_i, / q_i);
+    //   with gather (ModDown): (acc_s - x) P^-1 [+ add_s] to cb[z].out_s + i*N,
+    //     add0 read through sigma_g when cb[z].add0_galois, cst = (P^-1 mod q_i, / q_i)
     const double *tt = nullptr;
     u64 tt_pstride = 0;
     const RescaleOut *ro = nullptr;
-    const double *cst_pq = nullptr;
+    const CombineArgs *cb = nullptr;
+    const double *cst = nullptr;
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);

@@ -765,9 +765,9 @@ __device__ __forceinline__ void fwd_rows_to_lds(const __attribute__((address_spa
 
 // OWN: how an item's own limbs are read: 0 as words of c, 1 formed from the HMult
 // factors (ta: d2 = a1 b1, with P d0 / P d1 added), 2 through sigma_g (c read at
-// galois_src); 3 as 1, with the ModDown + rescale's last row pass and its epilogue
-// in place of the stores of the sums (KsRowsArgs::tt); one instance each, so each
-// keeps its own register budget.
+// galois_src); 3 as 1 and 4 as 2, with the last row pass of the ModDown + rescale
+// (3) or of the ModDown (4) and its epilogue in place of the stores of the sums
+// (KsRowsArgs::tt); one instance each, so each keeps its own register budget.
 template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
 __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
                     // (fewer live registers than all 48 at once)
                     if ((k & 3) == 0) __builtin_amdgcn_sched_barrier(0);
                     const u32 pos = base + tid + (u32)k * F_TPB;
-                    const double d = u64_to_f64(c[OWN == 2 ? galois_src(pos, A.galois, logN) : pos]);
+                    const double d = u64_to_f64(c[(OWN == 2 || OWN == 4) ? galois_src(pos, A.galois, logN) : pos]);
                     s0[k] += mulmod_vv_f64(d, u64_to_f64(kb[(size_t)k * F_TPB]), q, qinv);
                     s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
                 }
@@ -868,10 +868,12 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
                 s1[k] += mulmod_vv_f64(d, u64_to_f64(ka[(size_t)k * F_TPB]), q, qinv);
             }
         }
-        if (OWN == 3) {
-            // the ModDown + rescale: out_s = acc_s (P q_l)^-1 + NTT_rows(tt_s), where tt_s
-            // is the column-pass output of c T_i - conv_i (P q_l)^-1 (FUSE_MDRS_OUT);
-            // |acc (P q_l)^-1| <= q/2 + 1, |row pass| <= q/2 + 2
+        if (OWN == 3 || OWN == 4) {
+            // OWN 3, the ModDown + rescale: out_s = acc_s (P q_l)^-1 + x_s, x_s the row
+            // pass of tt_s, the column-pass output of c T_i - conv_i (P q_l)^-1
+            // (FUSE_MDRS_OUT); |acc (P q_l)^-1| <= q/2 + 1, |x| <= q/2 + 2.
+            // OWN 4, the ModDown: out_s = (acc_s - x_s) P^-1 [+ add_s], x_s the row pass
+            // of conv(P part) (FUSE_MODDOWN_OUT); |acc - x| <= q + 3
             const u32 tid = opaque_tid(tid0);
             TileGeom g;
             g.logN = logN;
@@ -886,19 +888,52 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             // (item and slot through opaque_tid: their addresses are formed here, not
             // kept live across the digit loop)
             const u32 zo = opaque_tid(z), io = opaque_tid(i);
-            const double cp = a.cst_pq[2 * io], cpq = a.cst_pq[2 * io + 1];
-#pragma unroll
-            for (u32 sp = 0; sp < 2; ++sp) {
+            const double cp = a.cst[2 * io], cpq = a.cst[2 * io + 1];
+            const size_t so = ((size_t)io << logN) + base + tid;
+            // one polynomial's row pass and epilogue (called for s = 0 and s = 1)
+            auto finish = [&](const double (&acc)[16], u32 sp) {
                 const auto *src = gptr(a.tt + (size_t)(2 * zo + sp) * a.tt_pstride + ((size_t)io << logN));
                 fwd_rows_to_lds<LOGRF>(src, lds, g, tid, base);
-                auto *o = gptr_w(a.ro[2 * zo + sp].out + ((size_t)io << logN) + base + tid);
+                if (OWN == 3) {
+                    auto *o = gptr_w(a.ro[2 * zo + sp].out + so);
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const double v = lds[lds_pad(tid + (u32)k * F_TPB)];
-                    const double r = mulmod_f64(reduce_f64(sp ? s1[k] : s0[k], q, qinv), cp, cpq, q) + v;
-                    o[(size_t)k * F_TPB] = f64_to_canon(reduce_f64(r, q, qinv), q);
+                    for (int k = 0; k < 16; ++k) {
+                        const double v = lds[lds_pad(tid + (u32)k * F_TPB)];
+                        const double r = mulmod_f64(reduce_f64(acc[k], q, qinv), cp, cpq, q) + v;
+                        o[(size_t)k * F_TPB] = f64_to_canon(reduce_f64(r, q, qinv), q);
+                    }
+                } else {
+                    const CombineArgs C = a.cb[zo];
+                    auto *o = gptr_w((sp ? C.out1 : C.out0) + so);
+                    const u64 *add = sp ? C.add1 : C.add0;
+                    const u64 ag = sp ? 0 : C.add0_galois;
+                    const auto *ad = add ? gptr(add + ((size_t)io << logN)) : nullptr;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        // the add's words of eight elements first (one wait), through
+                        // sigma_g's source index when ag
+                        u64 w[8];
+                        if (ad && ag) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                w[k] = ad[galois_src(base + tid + (u32)(8 * h + k) * F_TPB, ag, logN)];
+                        } else if (ad) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) w[k] = ad[base + tid + (u32)(8 * h + k) * F_TPB];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int e = 8 * h + k;
+                            const double v = lds[lds_pad(tid + (u32)e * F_TPB)];
+                            double r = mulmod_f64(reduce_f64(acc[e], q, qinv) - v, cp, cpq, q);
+                            if (ad) r += u64_to_f64(w[k]);
+                            o[(size_t)e * F_TPB] = f64_to_canon(reduce_f64(r, q, qinv), q);
+                        }
+                    }
                 }
-            }
+            };
+            finish(s0, 0);
+            finish(s1, 1);
         } else if (!PSLOT) {
             auto *o0 = gptr_w(a.acc + (size_t)z * a.az + ((size_t)i << logN) + base + tid);
             auto *o1 = o0 + ((size_t)lk << logN);
@@ -975,8 +1010,12 @@ static void launch_ks_rows_own(const DevTables &t, const KsRowsArgs &a, u32 slot
 template <int LOGRF, int LOGRI, bool PSLOT>
 static void launch_ks_rows_inst(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
     if constexpr (!PSLOT) {
-        if (a.ta && a.tt) {
+        if (a.tt && a.ta) {
             launch_ks_rows_own<LOGRF, LOGRI, false, 3>(t, a, slot0, nslots, s);
+            return;
+        }
+        if (a.tt && a.gather) {
+            launch_ks_rows_own<LOGRF, LOGRI, false, 4>(t, a, slot0, nslots, s);
             return;
         }
     }

@@ -242,6 +242,12 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
         drv.eval_automorphism_ks(g1, ksz, L, 20 * W, 21 * W, 20 * W, 21 * W, L)
         assert o.eval_automorphism_ks(pm, km, g1, ksz, L, 20 * W, 21 * W, 20 * W, 21 * W, L) == 0
         assert np.array_equal(ctx.read_poly(20 * W, 2 * W), pm[20 * W:22 * W])
+        # c1 overwritten only: sigma_g read while loading, but the Q-slot sums stored
+        # (the kernel that writes the outputs would read c1 through sigma_g)
+        say("switch", switch, "rotate over c1")
+        drv.eval_automorphism_ks(g1, ksz, L, 20 * W, 21 * W, 22 * W, 21 * W, L)
+        assert o.eval_automorphism_ks(pm, km, g1, ksz, L, 20 * W, 21 * W, 22 * W, 21 * W, L) == 0
+        assert np.array_equal(ctx.read_poly(20 * W, 3 * W), pm[20 * W:23 * W])
     finally:
         ctx.close()
 

@@ -24,8 +24,9 @@
 // Arithmetic: the FP64 exact products of fhea_f64.hpp.  Input limb i is scaled by
 // N^-1 qhat_i^-1 mod q_i (the INTT's N^-1 and the conversion's first factor in one
 // product), centred, and moved into [0, q_i): the conversion's input integer.  The
-// conversion is k_fbc_f64's sum (terms mulmod_f64(y_i, qhat_i mod p_j), centred
-// every four terms), then the forward column pass starts from the centred sum.
+// conversion is k_fbc_f64's sum of the terms mulmod_f64(y_i, qhat_i mod p_j), each
+// |term| <= q/2 + 1, so at most CC_MAX_IN = 8 of them stay below 4q + 8 < 2^52 and
+// add exactly; the forward column pass starts from the centred sum.
 // Every step is exact mod p_j, so the words equal INTT -> fast_bases_conv -> NTT.
 #include "fhea_f64.hpp"
 #include "fhea_kernels.hpp"
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     constexpr u32 PO = LOGC + RoundW<false, LOGR, W, NR - 1>::P;  // output layout (forward last round)
     static_assert(RoundW<true, LOGR, W, NR - 1>::P == RoundW<false, LOGR, W, 0>::P, "windows must meet");
     static_assert(W <= 3, "forward values are centred every 2W <= 6 stages (bound 7.53q < 8q)");
+    static_assert(CC_MAX_IN <= 8, "conversion sums of at most 8 terms add exactly without centring");
     __shared__ double lds[2 * PADT];
     // the unit's constants, staged once per digit: conversion pairs (qhat_i mod p_j,
     // / p_j) as [i][j] with zero pairs for i >= n_in, per output j its prime index and
@@ -324,10 +326,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                 if ((u32)i < n_in) {
                     const double h = hv[i].x, hq = hv[i].y;
 #pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        x[u] += mulmod_f64(y[i][u], h, hq, q);
-                        if ((i & 3) == 3) x[u] = reduce_f64(x[u], q, qinv);
-                    }
+                    for (int u = 0; u < E; ++u) x[u] += mulmod_f64(y[i][u], h, hq, q);
                 }
             }
             const double hq_l = 0.5 * q;
@@ -362,10 +361,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                 if ((u32)i < n_in) {
                     const double h = hv[i].x, hq = hv[i].y;
 #pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        x[u] += mulmod_f64(y[i][u], h, hq, q);
-                        if ((i & 3) == 3) x[u] = reduce_f64(x[u], q, qinv);
-                    }
+                    for (int u = 0; u < E; ++u) x[u] += mulmod_f64(y[i][u], h, hq, q);
                 }
             }
 #pragma unroll

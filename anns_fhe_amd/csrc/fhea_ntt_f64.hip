@@ -289,8 +289,10 @@ __device__ __forceinline__ void store_fused16(const double *lds, u32 tid, const 
 // (FUSE_GATHER_IN, sigma_g applied while loading).
 // FZ false: an instance for plain transforms (no fused prologue or epilogue compiled
 // in, so it keeps fewer registers); the tile walk fields of fz still apply.
+// Register budgets: plain instances 4 waves per SIMD, the product prologue 3 (168
+// registers instead of the compiler's 185 at 2 waves: HMult +0.35%).
 template <bool INV, int LOGR, bool COLS, u32 TPB = F_TPB, bool PF = FHEA_NTT_PREFETCH, int PRO = 0, bool FZ = true>
-__global__ __launch_bounds__(TPB) FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu(!FZ ? 4 : 1))) void k_ntt64f(const void *src, void *dst, u64 src_stride,
+__global__ __launch_bounds__(TPB) FHEA_NTT_ATTR __attribute__((amdgpu_waves_per_eu(!FZ ? 4 : (PRO == 1 ? 3 : 1)))) void k_ntt64f(const void *src, void *dst, u64 src_stride,
                                                   u64 dst_stride, u64 src_slot, u64 dst_slot,
                                                   u32 limbs, const u32 *__restrict__ prime_map,
                                                   const double *__restrict__ twf,

@@ -10,7 +10,7 @@ The knobs:
   * the number of resident NTT workgroups per CU;
   * the number of batch items per key-switch pipeline chunk, on a caller-given
     list of work items (for example the HMult batch being served);
-  * one or two streams for a batched HMult (its halves overlapping on the device).
+  * one to four streams for a batched HMult (its parts overlapping on the device).
 Every NTT candidate's output (first and last polynomial of the batch) is checked
 against the default schedule's word for word before it can be chosen.  Choices can be cached in a JSON file keyed by the
 parameter set.

@@ -70,7 +70,8 @@ int retire_buffer(fhea_ctx *c, void *p) {
         return FHEA_OK;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));
+    for (u32 k = 1; k < fhea_ctx::MAX_LANES; ++k)
+        if (c->lane_stream[k]) HIPCHK(hipStreamSynchronize(c->lane_stream[k]));
     HIPCHK(hipFree(p));
     return FHEA_OK;
 }
@@ -110,7 +111,8 @@ static int stage_reserve(fhea_ctx *c, size_t bytes, char **h, char **d) {
         // every earlier copy (on either lane) must have landed before the staging
         // area is reused
         if (hipStreamSynchronize(c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
-        if (c->stream2 && hipStreamSynchronize(c->stream2) != hipSuccess) return FHEA_ERR_DEVICE;
+        for (u32 k = 1; k < fhea_ctx::MAX_LANES; ++k)
+            if (c->lane_stream[k] && hipStreamSynchronize(c->lane_stream[k]) != hipSuccess) return FHEA_ERR_DEVICE;
         c->args_used = 0;
         if (need > c->args_bytes) return FHEA_ERR_NO_MEM;
     }
@@ -1055,32 +1057,39 @@ static bool ks_gather_ok(const fhea_ctx *c, u32 l) {
            modup_cc(c, l) && moddown_cc(c);
 }
 
-// Swaps the context's stream and workspace with the second lane's.
-static void swap_lane(fhea_ctx *c) {
-    std::swap(c->stream, c->stream2);
-    std::swap(c->ws, c->ws2);
-    std::swap(c->ws_words, c->ws2_words);
+// Swaps the context's stream and workspace with lane k's.
+static void swap_lane(fhea_ctx *c, u32 k) {
+    std::swap(c->stream, c->lane_stream[k]);
+    std::swap(c->ws, c->lane_ws[k]);
+    std::swap(c->ws_words, c->lane_ws_words[k]);
 }
 
 static int mult_relin_rescale(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
                               const std::vector<MultItem> &items) {
-    if (c->lanes < 2 || c->capturing || items.size() < 2 || !c->stream2)
-        return mult_relin_rescale_lane(c, l, key, key_limbs, items);
-    // second half on the second lane: it starts after everything issued so far on
-    // the stream, and the stream continues only after it
-    const size_t half = items.size() / 2;
-    std::vector<MultItem> a(items.begin(), items.begin() + half), b(items.begin() + half, items.end());
-    if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess ||
-        hipStreamWaitEvent(c->stream2, c->ev_fork, 0) != hipSuccess)
-        return FHEA_ERR_DEVICE;
-    swap_lane(c);
-    int st = mult_relin_rescale_lane(c, l, key, key_limbs, b);
-    swap_lane(c);
-    const int st2 = mult_relin_rescale_lane(c, l, key, key_limbs, a);
-    if (hipEventRecord(c->ev_join, c->stream2) != hipSuccess ||
-        hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess)
-        return FHEA_ERR_DEVICE;
-    return st ? st : st2;
+    const u32 nl = (u32)std::min<size_t>(c->lanes, items.size());
+    if (nl < 2 || c->capturing) return mult_relin_rescale_lane(c, l, key, key_limbs, items);
+    // parts k = 1 .. nl-1 on their lanes: each starts after everything issued so far
+    // on the stream, and the stream continues only after all of them; part 0 on the
+    // stream itself, issued last
+    if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return FHEA_ERR_DEVICE;
+    const size_t n = items.size();
+    int st = FHEA_OK;
+    for (u32 k = nl - 1; k >= 1; --k) {
+        if (hipStreamWaitEvent(c->lane_stream[k], c->ev_fork, 0) != hipSuccess) return FHEA_ERR_DEVICE;
+        std::vector<MultItem> part(items.begin() + k * n / nl, items.begin() + (k + 1) * n / nl);
+        swap_lane(c, k);
+        const int sk = mult_relin_rescale_lane(c, l, key, key_limbs, part);
+        swap_lane(c, k);
+        if (!st) st = sk;
+    }
+    std::vector<MultItem> part0(items.begin(), items.begin() + n / nl);
+    const int s0 = mult_relin_rescale_lane(c, l, key, key_limbs, part0);
+    if (!st) st = s0;
+    for (u32 k = 1; k < nl; ++k)
+        if (hipEventRecord(c->lane_join[k], c->lane_stream[k]) != hipSuccess ||
+            hipStreamWaitEvent(c->stream, c->lane_join[k], 0) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+    return st;
 }
 
 static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb) {
@@ -1487,13 +1496,15 @@ int fhea_ctx_destroy(fhea_ctx *c) {
     if (c->args_host) hipHostFree(c->args_host);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream2) {
-        hipStreamSynchronize(c->stream2);
-        hipStreamDestroy(c->stream2);
+    for (u32 k = 1; k < fhea_ctx::MAX_LANES; ++k) {
+        if (c->lane_stream[k]) {
+            hipStreamSynchronize(c->lane_stream[k]);
+            hipStreamDestroy(c->lane_stream[k]);
+        }
+        if (c->lane_ws[k]) hipFree(c->lane_ws[k]);
+        if (c->lane_join[k]) hipEventDestroy(c->lane_join[k]);
     }
-    if (c->ws2) hipFree(c->ws2);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return FHEA_OK;
@@ -1671,11 +1682,13 @@ int fhea_ctx_set_tuning(fhea_ctx *c, int knob, uint32_t v) {
         c->max_chunk = v;
         return FHEA_OK;
     case FHEA_TUNE_STREAMS:
-        if (v < 1 || v > 2) return FHEA_ERR_ARG;
-        if (v == 2 && !c->stream2) {
-            if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return FHEA_ERR_DEVICE;
-            if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        if (v < 1 || v > fhea_ctx::MAX_LANES) return FHEA_ERR_ARG;
+        if (v > 1 && !c->ev_fork && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+            return FHEA_ERR_DEVICE;
+        for (u32 k = 1; k < v; ++k) {
+            if (!c->lane_stream[k] && hipStreamCreateWithFlags(&c->lane_stream[k], hipStreamNonBlocking) != hipSuccess)
+                return FHEA_ERR_DEVICE;
+            if (!c->lane_join[k] && hipEventCreateWithFlags(&c->lane_join[k], hipEventDisableTiming) != hipSuccess)
                 return FHEA_ERR_DEVICE;
         }
         c->lanes = v;

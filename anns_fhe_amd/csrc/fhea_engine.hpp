@@ -66,14 +66,17 @@ struct fhea_ctx {
     // last graph is destroyed (or with the context)
     std::vector<void *> retired;
 
-    // second lane (FHEA_TUNE_STREAMS = 2): a batched HMult runs half of its items on
-    // a second stream with its own workspace, forked from and joined back into
-    // `stream` with events, so that the two halves' kernels overlap on the device
+    // lanes (FHEA_TUNE_STREAMS = n <= MAX_LANES): a batched HMult runs its items in n
+    // parts, part k > 0 on lane_stream[k] with its own workspace lane_ws[k], forked
+    // from and joined back into `stream` with events, so that the parts' kernels
+    // overlap on the device (part 0 runs on `stream` in `ws`)
+    static constexpr u32 MAX_LANES = 4;
     u32 lanes = 1;
-    hipStream_t stream2 = nullptr;
-    u64 *ws2 = nullptr;
-    u64 ws2_words = 0;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t lane_stream[MAX_LANES] = {};
+    u64 *lane_ws[MAX_LANES] = {};
+    u64 lane_ws_words[MAX_LANES] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t lane_join[MAX_LANES] = {};
 
     // reads in flight (fhea_read_polys_async): copies on their own stream, after an
     // event of `stream`, each ticket's completion event in a ring

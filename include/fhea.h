@@ -225,7 +225,7 @@ typedef struct {
  * one pipeline; groups run in order.  Within a group of kinds 1, 2, 5 and 6 an item
  * may write over its own inputs, but no output may overlap another output or an
  * input of another item of the group (FHEA_ERR_ARG): the group's items run in
- * chunks and, with FHEA_TUNE_STREAMS = 2, on two streams at once. */
+ * chunks and, with FHEA_TUNE_STREAMS = n > 1, on n streams at once. */
 int fhea_submit(fhea_ctx *ctx, const fhea_work *items, uint32_t n_items);
 /* Batched forward (inverse != 0: inverse) NTT of npolys contiguous polynomials
  * of `limbs` limbs each (poly z at src + z*limbs*N), limb i mod prime i
@@ -263,8 +263,8 @@ int fhea_graph_destroy(fhea_ctx *ctx, uint32_t graph_id);
 #define FHEA_TUNE_NTT_SPLIT_INV 2 /* the same for the inverse (its second pass)      */
 #define FHEA_TUNE_NTT_WG_PER_CU 3 /* resident NTT workgroups per CU, 1..4, 0 = by occupancy */
 #define FHEA_TUNE_CHUNK 4         /* batch items per key-switch pipeline chunk, 1..64 */
-#define FHEA_TUNE_STREAMS 5       /* 1 or 2: a batched HMult runs its two halves on two
-                                     streams that overlap on the device              */
+#define FHEA_TUNE_STREAMS 5       /* 1..4: a batched HMult runs its items in that many
+                                     parts on as many streams, overlapping on the device */
 int fhea_ctx_set_tuning(fhea_ctx *ctx, int knob, uint32_t value);
 int fhea_ctx_get_tuning(const fhea_ctx *ctx, int knob, uint32_t *value);
 

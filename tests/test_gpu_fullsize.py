@@ -2,7 +2,8 @@
 
 * The benchmark's own configuration: batched eval_mult_relin_rescale (work kind 1)
   at N=2^16, L=24, K=8, dnum=3, B=64 on two streams with key-switch chunks of 16
-  and of 64; items at the chunk and lane boundaries against the oracle.
+  and of 64 and on three streams; items at the chunk and lane boundaries against
+  the oracle.
 * eval_mult_plain_rescale (fhe_host_driver.cpp:308-329) at the C1 shape and at
   N=2^16, L=24.
 * The integer kernels, which run when a prime is >= 2^50 or the chain spans a
@@ -38,8 +39,9 @@ def rand_key(rng, ctx, key_limbs):
 
 def test_bench_config_batched_hmult_vs_oracle(orc):
     """B=64 HMults sharing one relin key at N=2^16/L=24/K=8/dnum=3, two lanes
-    (items 32..63 on the second stream), chunks of 16 and of 64: items 0, 15, 16,
-    31, 32, 63 equal the oracle's eval_mult_relin_rescale."""
+    (items 32..63 on the second stream) with chunks of 16 and of 64, and three lanes
+    (items 0..20, 21..41, 42..63) with chunks of 16: items 0, 15, 16, 31, 32, 63 equal
+    the oracle's eval_mult_relin_rescale."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
     N, L, K, dnum, B = 65536, 24, 8, 3, 64
@@ -65,14 +67,15 @@ def test_bench_config_batched_hmult_vs_oracle(orc):
                       b1=(4 * z + 3) * W, d0=out_base + 2 * z * W1, d1=out_base + (2 * z + 1) * W1)
                  for z in range(B)]
         got = {}
-        for chunk in (16, 64):
-            ctx.set_tuning(_lib.TUNE_STREAMS, 2)
+        schedules = ((2, 16), (2, 64), (3, 16))
+        for streams, chunk in schedules:
+            ctx.set_tuning(_lib.TUNE_STREAMS, streams)
             ctx.set_tuning(_lib.TUNE_CHUNK, chunk)
-            say("submit chunk", chunk)
+            say("submit streams", streams, "chunk", chunk)
             ctx.write_poly(out_base, np.zeros(2 * B * W1, np.uint64))
             ctx.submit(works)
             ctx.sync()
-            got[chunk] = {z: ctx.read_poly(out_base + 2 * z * W1, 2 * W1) for z in sample}
+            got[streams, chunk] = {z: ctx.read_poly(out_base + 2 * z * W1, 2 * W1) for z in sample}
         ctx.set_tuning(_lib.TUNE_STREAMS, 1)
         ctx.set_tuning(_lib.TUNE_CHUNK, 16)
         o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
@@ -81,10 +84,10 @@ def test_bench_config_batched_hmult_vs_oracle(orc):
             pm = np.zeros(6 * W, np.uint64)
             pm[:4 * W] = inputs[z]
             assert o.eval_mult_relin_rescale(pm, key, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
-            for chunk in (16, 64):
-                g = got[chunk][z]
-                assert np.array_equal(g[:W1], pm[4 * W:4 * W + W1]), (chunk, z, "c0")
-                assert np.array_equal(g[W1:], pm[5 * W:5 * W + W1]), (chunk, z, "c1")
+            for sc in schedules:
+                g = got[sc][z]
+                assert np.array_equal(g[:W1], pm[4 * W:4 * W + W1]), (sc, z, "c0")
+                assert np.array_equal(g[W1:], pm[5 * W:5 * W + W1]), (sc, z, "c1")
     finally:
         ctx.close()
 

@@ -519,9 +519,20 @@ static u64 ks_words(const fhea_ctx *c, u32 l) {
 }
 static u64 rescale_words(const fhea_ctx *c, u32 l) { return (u64)c->h.N * (2 + 2 * (l - 1)); }
 
+// Per-item argument arrays of one key-switch chunk already on the device (staged
+// by the caller together with its other chunks', one copy for all): the pipeline
+// then stages nothing itself.  Null members are staged as usual.
+struct PreArgs {
+    const TensorArgs *fold = nullptr;
+    const GatherArgs *gather = nullptr;
+    const KsIpArgs *ip = nullptr;
+    const CombineArgs *cb = nullptr;
+    const RescaleOut *ro = nullptr;
+};
+
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
                       const std::vector<CombineArgs> &comb, bool pb_mid = false,
-                      const KsRowsArgs *late = nullptr);
+                      const KsRowsArgs *late = nullptr, const PreArgs *pre = nullptr);
 
 // The rescale that follows a key switch in eval_mult_relin_rescale: the switch's
 // add operand d_z = (d_base + z*dz, + ds) is also its output (l limbs, limb l-1
@@ -533,7 +544,8 @@ struct RescaleAfter {
 };
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
                               const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
-                              bool folded = false, const KsRowsArgs *late = nullptr);
+                              bool folded = false, const KsRowsArgs *late = nullptr,
+                              const PreArgs *pre = nullptr);
 
 // Mixed chain whose Q primes past a leading run of >= 2^50 primes are all below
 // 2^50: the ModDown / rescale transforms take the FP64 fused ends on those limbs.
@@ -635,7 +647,8 @@ static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
                    u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr,
-                   const std::vector<GatherArgs> *gather = nullptr, bool gather_late = false) {
+                   const std::vector<GatherArgs> *gather = nullptr, bool gather_late = false,
+                   const PreArgs *pre = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -655,7 +668,8 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // and (with cc) for the ModUp INTT's first pass, which then forms d2 = a1 b1 itself
     const TensorArgs *fold_d = nullptr;
     if (fold) {
-        fold_d = (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
+        fold_d = (pre && pre->fold) ? pre->fold
+                                    : (const TensorArgs *)stage_args(c, fold->data(), nb * sizeof(TensorArgs));
         if (!fold_d) return FHEA_ERR_DEVICE;
     }
     // gather (rotations): input z is sigma_g of (*gather)[z].in, formed while loading
@@ -663,7 +677,8 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // fused rows kernel (ks_gather_ok)
     const GatherArgs *gather_d = nullptr;
     if (gather) {
-        gather_d = (const GatherArgs *)stage_args(c, gather->data(), nb * sizeof(GatherArgs));
+        gather_d = (pre && pre->gather) ? pre->gather
+                                        : (const GatherArgs *)stage_args(c, gather->data(), nb * sizeof(GatherArgs));
         if (!gather_d) return FHEA_ERR_DEVICE;
     }
     if (cc) {
@@ -684,12 +699,15 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         launch_fbc(t, c->modup_desc_dev[l], c->modup_first_dev[l], np, std::min(l, c->h.alpha), l - 1 + K,
                    coef, l * N, compact, (l + K) * N, cz, nb, s);
     }
-    std::vector<KsIpArgs> ip(nb);
-    for (u32 z = 0; z < nb; ++z)
-        ip[z] = gather ? KsIpArgs{(*gather)[z].in, keys[z], (*gather)[z].galois}
-                       : KsIpArgs{c_base + (u64)z * c_stride, keys[z], 0};
-    auto *ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
-    if (!ip_d) return FHEA_ERR_DEVICE;
+    const KsIpArgs *ip_d = pre ? pre->ip : nullptr;
+    if (!ip_d) {
+        std::vector<KsIpArgs> ip(nb);
+        for (u32 z = 0; z < nb; ++z)
+            ip[z] = gather ? KsIpArgs{(*gather)[z].in, keys[z], (*gather)[z].galois}
+                           : KsIpArgs{c_base + (u64)z * c_stride, keys[z], 0};
+        ip_d = (const KsIpArgs *)stage_args(c, ip.data(), nb * sizeof(KsIpArgs));
+        if (!ip_d) return FHEA_ERR_DEVICE;
+    }
     // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
     const bool fused = t.use_f64 && !(t.f64_off & (1 | 4 | 64)) && np <= 8 && ks_rows_fusable(t);
     // HMult (the tensor folded in, the rescale after) and rotations (sigma_g read
@@ -771,14 +789,15 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     const bool pb_mid = fused && (t.f64_off & 128);
     if (ra)
         return ks_moddown_rescale(c, l, nb, acc, pb, tt, coef, comb, *ra, pb_mid, fold != nullptr,
-                                  late ? &ka_late : nullptr);
-    return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid, late ? &ka_late : nullptr);
+                                  late ? &ka_late : nullptr, pre);
+    return ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid, late ? &ka_late : nullptr, pre);
 }
 
 // ModDown of nb accumulator pairs acc_z = [2][l+K][N] (key_switch.cpp:385-427) and the
 // composite's add: out_z = add_z + ModDown(acc_z).  pb: 2*nb*K*N words, tt: 2*nb*l*N.
 static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
-                      const std::vector<CombineArgs> &comb, bool pb_mid, const KsRowsArgs *late) {
+                      const std::vector<CombineArgs> &comb, bool pb_mid, const KsRowsArgs *late,
+                      const PreArgs *pre) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -792,7 +811,8 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
         moddown_intt(c, l, nb, acc, pb, pb_mid);
         launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
     }
-    auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
+    auto *cb_d = (pre && pre->cb) ? pre->cb
+                                  : (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     if (late && cc) {
         // the Q slots' rows kernel with the last row pass and its epilogue in place of
@@ -909,7 +929,7 @@ static int rescale_core(fhea_ctx *c, u32 l, u32 nb, const u64 *a_base, u64 az, u
 // linearity of the NTT mod q_i: the same words, one transform of l-1 limbs fewer.
 static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt, u64 *v,
                               const std::vector<CombineArgs> &comb, const RescaleAfter &ra, bool pb_mid,
-                              bool folded, const KsRowsArgs *late) {
+                              bool folded, const KsRowsArgs *late, const PreArgs *pre) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
     const LevelTables &lt = c->lv[l];
@@ -919,7 +939,7 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     // integer kernels on the prefix
     const bool mix = !t.use_f64 && mixed_fused(t) && l >= 2 && t.q_big_prefix < l - 1;
     if ((!t.use_f64 && !mix) || (t.f64_off & 17) || l < 2) {
-        int st = ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid);
+        int st = ks_moddown(c, l, nb, acc, pb, tt, comb, pb_mid, nullptr, pre);
         if (st) return st;
         return rescale_core(c, l, nb, ra.d_base, ra.dz, ra.ds, *ra.outs, v);
     }
@@ -945,7 +965,8 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
             launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2,
                            c->rs_last_map[l], r1, s);
         moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift, !folded);
-        auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
+        auto *ro_d = (pre && pre->ro) ? pre->ro
+                                      : (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
         if (!ro_d) return FHEA_ERR_DEVICE;
         if (late) {
             // the Q slots' rows kernel with this row pass and epilogue in place of
@@ -971,7 +992,8 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     }
     moddown_intt(c, l, nb, acc, pb, pb_mid);
     launch_fbc(t, c->moddown_desc_dev[l], c->zero_u32, 1, K, l, pb, K * N, tt, 0, l * N, 2 * nb, s);
-    auto *cb_d = (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
+    auto *cb_d = (pre && pre->cb) ? pre->cb
+                                  : (const CombineArgs *)stage_args(c, comb.data(), nb * sizeof(CombineArgs));
     if (!cb_d) return FHEA_ERR_DEVICE;
     // limb l-1 of the ModDown result, (acc - NTT(conv)) * P^-1 + d, in place in d
     NttFuse f1;
@@ -986,7 +1008,8 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
     launch_ntt_f64(t, false, tt, tt, l * N, l * N, N, N, 2 * nb, l, t.ident, f1, s);
     // its coefficients
     launch_ntt(t, true, ra.d_base + (l - 1) * N, v, ra.dz, 2 * N, nb, 2, c->rs_last_map[l], s, ra.ds, N);
-    auto *ro_d = (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
+    auto *ro_d = (pre && pre->ro) ? pre->ro
+                                  : (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));
     if (!ro_d) return FHEA_ERR_DEVICE;
     NttFuse f2;
     f2.pro = FUSE_MDRS_IN;
@@ -1096,6 +1119,18 @@ static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb) {
     return (u64)nb * 3 * l * c->h.N + std::max<u64>(nb * ks_words(c, l), nb * rescale_words(c, l));
 }
 
+// Host-side argument blocks of several chunks, staged with one copy.
+struct ArgPack {
+    std::vector<char> h;
+    template <class T>
+    size_t put(const std::vector<T> &v) {
+        const size_t o = (h.size() + 63) & ~(size_t)63;
+        h.resize(o + std::max<size_t>(v.size() * sizeof(T), 1));
+        if (!v.empty()) memcpy(h.data() + o, v.data(), v.size() * sizeof(T));
+        return o;
+    }
+};
+
 static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_limbs,
                                    const std::vector<MultItem> &items, const FixedWs *fixed) {
     const u64 N = c->h.N;
@@ -1104,41 +1139,72 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         while (chunk > 1 && mult_relin_rescale_words(c, l, chunk) > fixed->words) --chunk;
         if (mult_relin_rescale_words(c, l, chunk) > fixed->words) return FHEA_ERR_SCRATCH;
     }
+    if (items.empty()) return FHEA_OK;
+    // one workspace for every chunk (sized for the largest), so that all chunks'
+    // argument arrays can be formed, and staged with one copy, before any launch
+    u64 *D = nullptr;
+    if (fixed) {
+        D = fixed->p;
+    } else {
+        int st = ensure_ws(c, mult_relin_rescale_words(c, l, (u32)std::min<size_t>(chunk, items.size())));
+        if (st) return st;
+        D = c->ws;
+    }
+    const bool fold = tensor_folded(c, l);
+    // with the fused ModUp as well, the ModUp INTT forms d2 itself: no tensor launch
+    const bool prod = fold && modup_cc(c, l);
+    struct Chunk {
+        u32 off, nb;
+        std::vector<TensorArgs> ta;
+        std::vector<CombineArgs> cb;
+        std::vector<u64 *> outs;
+        size_t o_ta, o_ip, o_cb, o_ro;
+    };
+    std::vector<Chunk> chunks;
+    ArgPack pk;
     for (size_t off = 0; off < items.size(); off += chunk) {
-        u32 nb = (u32)std::min<size_t>(chunk, items.size() - off);
-        u64 dwords = (u64)nb * 3 * l * N;
-        u64 *D = nullptr;
-        if (fixed) {
-            D = fixed->p;
-        } else {
-            int st = ensure_ws(c, mult_relin_rescale_words(c, l, nb));
-            if (st) return st;
-            D = c->ws;
-        }
-        u64 *rest = D + dwords;
-        int st = FHEA_OK;
-        std::vector<TensorArgs> ta(nb);
-        std::vector<CombineArgs> cb(nb);
-        std::vector<const u64 *> keys(nb, key);
-        std::vector<u64 *> outs(2 * nb);
-        const bool fold = tensor_folded(c, l);
-        // with the fused ModUp as well, the ModUp INTT forms d2 itself: no tensor launch
-        const bool prod = fold && modup_cc(c, l);
+        Chunk ch;
+        ch.off = (u32)off;
+        ch.nb = (u32)std::min<size_t>(chunk, items.size() - off);
+        const u32 nb = ch.nb;
+        ch.ta.resize(nb);
+        ch.cb.resize(nb);
+        ch.outs.resize(2 * nb);
+        std::vector<KsIpArgs> ip(nb);
+        std::vector<RescaleOut> ro(2 * nb);
         for (u32 z = 0; z < nb; ++z) {
             const MultItem &it = items[off + z];
             u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
-            ta[z] = {it.a0, it.a1, it.b0, it.b1, fold ? nullptr : d0, fold ? nullptr : d1, d2};
-            cb[z] = {d0, d1, d0, d1};
-            outs[2 * z] = it.d0;
-            outs[2 * z + 1] = it.d1;
+            ch.ta[z] = {it.a0, it.a1, it.b0, it.b1, fold ? nullptr : d0, fold ? nullptr : d1, d2};
+            ch.cb[z] = {d0, d1, d0, d1};
+            ch.outs[2 * z] = it.d0;
+            ch.outs[2 * z + 1] = it.d1;
+            // as ks_core and ks_moddown_rescale would form them: the key switch's input
+            // d2 (c_base + z*c_stride), the rescale's (d, out) pairs
+            ip[z] = KsIpArgs{d2, key, 0};
+            for (u32 k = 0; k < 2; ++k) ro[2 * z + k] = {fold ? nullptr : d0 + (u64)k * l * N, ch.outs[2 * z + k]};
         }
-        if (!prod) {
-            auto *ta_d = (const TensorArgs *)stage_args(c, ta.data(), nb * sizeof(TensorArgs));
-            if (!ta_d) return FHEA_ERR_DEVICE;
-            launch_tensor(c->dt, ta_d, nb, l, c->stream);
-        }
-        const RescaleAfter ra{D, 3 * l * N, l * N, &outs};
-        st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, cb, rest, &ra, fold ? &ta : nullptr);
+        ch.o_ta = pk.put(ch.ta);
+        ch.o_ip = pk.put(ip);
+        ch.o_cb = pk.put(ch.cb);
+        ch.o_ro = pk.put(ro);
+        chunks.push_back(std::move(ch));
+    }
+    char *dev = (char *)stage_args(c, pk.h.data(), pk.h.size());
+    if (!dev) return FHEA_ERR_DEVICE;
+    for (const Chunk &ch : chunks) {
+        const u32 nb = ch.nb;
+        u64 *rest = D + (u64)nb * 3 * l * N;
+        std::vector<const u64 *> keys(nb, key);
+        PreArgs pa;
+        pa.fold = (const TensorArgs *)(dev + ch.o_ta);
+        pa.ip = (const KsIpArgs *)(dev + ch.o_ip);
+        pa.cb = (const CombineArgs *)(dev + ch.o_cb);
+        pa.ro = (const RescaleOut *)(dev + ch.o_ro);
+        if (!prod) launch_tensor(c->dt, pa.fold, nb, l, c->stream);
+        const RescaleAfter ra{D, 3 * l * N, l * N, &ch.outs};
+        const int st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, ch.cb, rest, &ra,
+                               fold ? &ch.ta : nullptr, nullptr, false, &pa);
         if (st) return st;
     }
     return check_hip(hipGetLastError());
@@ -1289,33 +1355,74 @@ struct AutoItem {
 static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<AutoItem> &items) {
     const u64 N = c->h.N;
     const bool gok = ks_gather_ok(c, l);
+    // the chunks that read sigma_g while loading: their argument arrays first, all
+    // staged with one copy
+    struct Chunk {
+        u32 off, nb;
+        bool gather, late;
+        std::vector<GatherArgs> ga;
+        std::vector<CombineArgs> cb;
+        std::vector<const u64 *> keys;
+        size_t o_ga, o_ip, o_cb;
+    };
+    std::vector<Chunk> chunks;
+    ArgPack pk;
+    u32 gather_nb = 0;
     for (size_t off = 0; off < items.size(); off += c->max_chunk) {
-        u32 nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        Chunk ch;
+        ch.off = (u32)off;
+        ch.nb = (u32)std::min<size_t>(c->max_chunk, items.size() - off);
+        const u32 nb = ch.nb;
         // sigma_g while loading, unless an item writes over its own c0 (its last kernel
-        // reads c0 through sigma_g while it writes the outputs)
-        // (and with late: the rows kernel that writes the outputs also reads c1 through
-        // sigma_g, so no output may overlap c1 either)
-        bool gather = gok, late = true;
-        for (u32 z = 0; z < nb && gather; ++z) {
+        // reads c0 through sigma_g while it writes the outputs); late (the rows kernel
+        // that writes the outputs also reads c1 through sigma_g) unless an output
+        // overlaps c1 as well
+        ch.gather = gok;
+        ch.late = true;
+        for (u32 z = 0; z < nb && ch.gather; ++z) {
             const AutoItem &it = items[off + z];
             const u64 W = (u64)l * N;
             auto ov = [W](const u64 *a, const u64 *b) { return a < b + W && b < a + W; };
-            gather = !ov(it.d0, it.in0) && !ov(it.d1, it.in0);
-            late = late && !ov(it.d0, it.in1) && !ov(it.d1, it.in1);
+            ch.gather = !ov(it.d0, it.in0) && !ov(it.d1, it.in0);
+            ch.late = ch.late && !ov(it.d0, it.in1) && !ov(it.d1, it.in1);
         }
-        if (gather) {
-            int st = ensure_ws(c, nb * ks_words(c, l));
-            if (st) return st;
-            std::vector<GatherArgs> ga(nb);
-            std::vector<CombineArgs> cb(nb);
-            std::vector<const u64 *> keys(nb);
+        if (ch.gather) {
+            ch.ga.resize(nb);
+            ch.cb.resize(nb);
+            ch.keys.resize(nb);
+            std::vector<KsIpArgs> ip(nb);
             for (u32 z = 0; z < nb; ++z) {
                 const AutoItem &it = items[off + z];
-                ga[z] = {it.in1, it.galois};
-                cb[z] = {it.d0, it.d1, it.in0, nullptr, it.galois};
-                keys[z] = it.key;
+                ch.ga[z] = {it.in1, it.galois};
+                ch.cb[z] = {it.d0, it.d1, it.in0, nullptr, it.galois};
+                ch.keys[z] = it.key;
+                ip[z] = KsIpArgs{it.in1, it.key, it.galois};  // as ks_core forms them
             }
-            st = ks_core(c, l, key_limbs, nb, nullptr, 0, keys, cb, c->ws, nullptr, nullptr, &ga, late);
+            ch.o_ga = pk.put(ch.ga);
+            ch.o_ip = pk.put(ip);
+            ch.o_cb = pk.put(ch.cb);
+            gather_nb = std::max(gather_nb, nb);
+        }
+        chunks.push_back(std::move(ch));
+    }
+    char *dev = nullptr;
+    if (gather_nb) {
+        int st = ensure_ws(c, gather_nb * ks_words(c, l));
+        if (st) return st;
+        dev = (char *)stage_args(c, pk.h.data(), pk.h.size());
+        if (!dev) return FHEA_ERR_DEVICE;
+    }
+    for (const Chunk &ch : chunks) {
+        const u32 nb = ch.nb;
+        if (ch.gather) {
+            int st = ensure_ws(c, nb * ks_words(c, l));
+            if (st) return st;
+            PreArgs pa;
+            pa.gather = (const GatherArgs *)(dev + ch.o_ga);
+            pa.ip = (const KsIpArgs *)(dev + ch.o_ip);
+            pa.cb = (const CombineArgs *)(dev + ch.o_cb);
+            st = ks_core(c, l, key_limbs, nb, nullptr, 0, ch.keys, ch.cb, c->ws, nullptr, nullptr, &ch.ga, ch.late,
+                         &pa);
             if (st) return st;
             continue;
         }
@@ -1327,7 +1434,7 @@ static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<
         std::vector<CombineArgs> cb(nb);
         std::vector<const u64 *> keys(nb);
         for (u32 z = 0; z < nb; ++z) {
-            const AutoItem &it = items[off + z];
+            const AutoItem &it = items[ch.off + z];
             u64 *s0 = S + (u64)z * 2 * l * N, *s1 = s0 + l * N;
             aa[z] = {it.in0, it.in1, s0, s1, it.galois};
             cb[z] = {it.d0, it.d1, s0, nullptr};

@@ -1173,7 +1173,7 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         std::vector<KsIpArgs> ip(nb);
         std::vector<RescaleOut> ro(2 * nb);
         for (u32 z = 0; z < nb; ++z) {
-            const MultItem &it = items[off + z];
+            const MultItem &it = items[ch.off + z];
             u64 *d0 = D + (u64)z * 3 * l * N, *d1 = d0 + l * N, *d2 = d1 + l * N;
             ch.ta[z] = {it.a0, it.a1, it.b0, it.b1, fold ? nullptr : d0, fold ? nullptr : d1, d2};
             ch.cb[z] = {d0, d1, d0, d1};

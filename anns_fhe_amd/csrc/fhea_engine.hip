@@ -738,9 +738,17 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         }
         ka.gather = gather ? 1 : 0;
         if (late) {
-            if (ra) launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
-            else launch_ks_rows_f64_q(t, ka, l, (u32)K, s);
             ka_late = ka;
+            if (ra && !(t.f64_off & 8192)) {
+                // HMult: slot l-1 and the P slots also run the ModDown INTT's row pass
+                // on their sums (the lift of limb l-1 into the dead ModUp region)
+                ka_late.lift = (double *)coef;
+                launch_ks_rows_f64_early(t, ka_late, s);
+            } else if (ra) {
+                launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
+            } else {
+                launch_ks_rows_f64_q(t, ka, l, (u32)K, s);
+            }
         } else {
             launch_ks_rows_f64(t, ka, s);
         }
@@ -956,14 +964,18 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         // ModUp region at v, which extends into the dead digit region for l < 4.
         NttFuse r1;
         r1.only_pass = 1;
-        if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
         // (folded: the tensor's d is already in acc, the lift needs acc_{l-1} alone)
         u64 *lift = v;
-        const u64 ls = folded ? N : 2 * N;
-        launch_ntt_f64(t, true, acc + (l - 1) * N, lift, (l + K) * N, ls, N, N, 2 * nb, 1, t.ident + (l - 1), r1, s);
-        if (!folded)
-            launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2,
-                           c->rs_last_map[l], r1, s);
+        if (!(late && late->lift)) {
+            // (else the rows kernel has run these row passes on its sums)
+            if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
+            const u64 ls = folded ? N : 2 * N;
+            launch_ntt_f64(t, true, acc + (l - 1) * N, lift, (l + K) * N, ls, N, N, 2 * nb, 1, t.ident + (l - 1), r1,
+                           s);
+            if (!folded)
+                launch_ntt_f64(t, true, ra.d_base + (l - 1) * N, lift + N, ra.dz, 4 * N, ra.ds, 2 * N, nb, 2,
+                               c->rs_last_map[l], r1, s);
+        }
         moddown_cols(c, l, nb, pb, tt, 0, l - 1, lift, !folded);
         auto *ro_d = (pre && pre->ro) ? pre->ro
                                       : (const RescaleOut *)stage_args(c, ro.data(), ro.size() * sizeof(RescaleOut));

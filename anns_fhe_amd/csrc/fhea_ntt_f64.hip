@@ -769,7 +769,8 @@ __device__ __forceinline__ void fwd_rows_to_lds(const __attribute__((address_spa
 // factors (ta: d2 = a1 b1, with P d0 / P d1 added), 2 through sigma_g (c read at
 // galois_src); 3 as 1 and 4 as 2, with the last row pass of the ModDown + rescale
 // (3) or of the ModDown (4) and its epilogue in place of the stores of the sums
-// (KsRowsArgs::tt); one instance each, so each keeps its own register budget.
+// (KsRowsArgs::tt); 5 as 1, with the ModDown INTT's first (row) pass in place of the
+// stores (KsRowsArgs::lift); one instance each, so each keeps its own register budget.
 template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
 __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
@@ -801,7 +802,7 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             const u32 sz = (p == a.nparts - 1) ? (a.l - start) : a.alpha;
             const auto *ka = gptr(A.key + (size_t)p * kstride + ((size_t)kl << logN) + base + tid);
             const auto *kb = ka + (size_t)a.dnum * kstride;
-            if ((OWN == 1 || OWN == 3) && i >= start && i < start + sz) {
+            if ((OWN == 1 || OWN == 3 || OWN == 5) && i >= start && i < start + sz) {
                 // HMult with the tensor folded in: the own limb d2 = a1 b1, and
                 // acc0 += P d0, acc1 += P d1 (d0 = a0 b0, d1 = a0 b1 + a1 b0), every product
                 // of canonical words (mulmod_vv_f64, <= 0.875q); the sums are centred after
@@ -936,6 +937,55 @@ __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_wave
             };
             finish(s0, 0);
             finish(s1, 1);
+        } else if (OWN == 5) {
+            // the ModDown INTT's first pass on each sum (input: the canonical words):
+            // P slot i to pb, the rescaled limb l-1 to lift
+            using RI0 = Round<true, LOGRI, 0>;
+            using RIL = Round<true, LOGRI, (LOGRI + 3) / 4 - 1>;
+            const u32 tid = opaque_tid(tid0);
+            TileGeom g;
+            g.logN = logN;
+            g.logT = F_LOG_TILE;
+            g.logC = 0;
+            g.logS = 0;
+            g.c0 = 0;
+            g.b0 = tI << (F_LOG_TILE - LOGRI);
+            g.q = q;
+            g.qinv = qinv;
+            g.tw = itwf + ((size_t)pidx << (logN + 1));
+            const u32 zo = opaque_tid(z), io = opaque_tid(i);
+            auto finish = [&](const double (&acc)[16], u32 sp) {
+                double *dp = (io >= a.l) ? (double *)a.pb + ((size_t)(2 * zo + sp) * a.K + (io - a.l)) * N
+                                         : a.lift + (size_t)(2 * zo + sp) * N;
+                auto *dst = gptr_w(dp);
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const double v = reduce_f64(acc[k], q, qinv);
+                    lds[lds_pad(tid + (u32)k * F_TPB)] = v < 0.0 ? v + q : v;
+                }
+                __syncthreads();
+                double x[16];
+                const u32 eb0 = (tid & ((1u << RI0::P) - 1)) | ((tid >> RI0::P) << (RI0::P + 4));
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    x[u] = lds[lds_pad(eb0) + ((u32)u << RI0::P) + (((u32)u << RI0::P) >> 4)];
+                round_arith<true, LOGRI, 0, false>(x, g, g.gidx(eb0, LOGRI));
+                later_rounds<true, LOGRI, false>(x, lds, g, tid);
+                if (RIL::P >= 3) {
+                    const u32 eb = (tid & ((1u << RIL::P) - 1)) | ((tid >> RIL::P) << (RIL::P + 4));
+                    const u32 jb = g.gidx(eb, LOGRI);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) dst[jb + ((u32)u << RIL::P)] = reduce_f64(x[u], q, qinv);
+                } else {
+                    rows_to_lds<true, LOGRI>(x, lds, tid);
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                        dst[base + tid + (u32)k * F_TPB] = reduce_f64(lds[lds_pad(tid + (u32)k * F_TPB)], q, qinv);
+                }
+            };
+            finish(s0, 0);
+            finish(s1, 1);
         } else if (!PSLOT) {
             auto *o0 = gptr_w(a.acc + (size_t)z * a.az + ((size_t)i << logN) + base + tid);
             auto *o1 = o0 + ((size_t)lk << logN);
@@ -1065,6 +1115,23 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s) 
     FHEA_KS_ROWS_F(9)
 #undef FHEA_KS_ROWS_F
 #undef FHEA_KS_ROWS
+}
+
+void launch_ks_rows_f64_early(const DevTables &t, const KsRowsArgs &a, hipStream_t s) {
+    u32 lf, li;
+    rows_logs(t, lf, li);
+#define FHEA_KS_EARLY(LF, LI)                                                          \
+    if (lf == LF && li == LI) {                                                        \
+        launch_ks_rows_own<LF, LI, false, 5>(t, a, a.l - 1, a.K + 1, s);               \
+        return;                                                                        \
+    }
+#define FHEA_KS_EARLY_F(LF) FHEA_KS_EARLY(LF, 6) FHEA_KS_EARLY(LF, 7) FHEA_KS_EARLY(LF, 8) FHEA_KS_EARLY(LF, 9)
+    FHEA_KS_EARLY_F(6)
+    FHEA_KS_EARLY_F(7)
+    FHEA_KS_EARLY_F(8)
+    FHEA_KS_EARLY_F(9)
+#undef FHEA_KS_EARLY_F
+#undef FHEA_KS_EARLY
 }
 
 void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {

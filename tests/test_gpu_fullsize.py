@@ -185,7 +185,7 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
 
 
 @pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (0, 3), (128, 0), (256, 0), (64, 0), (512, 0),
-                                               (512 | 128, 0), (1024, 0), (1024 | 128, 0), (2048, 0), (4096, 0),
+                                               (512 | 128, 0), (1024, 0), (1024 | 128, 0), (2048, 0), (8192, 0), (4096, 0),
                                                (1 | 2 | 4 | 8 | 16 | 32, 0)])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     """FHEA_F64_OFF selects alternative kernel pipelines of the FP64 path (128: the
@@ -195,7 +195,9 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     pass, basis conversion and NTT column pass as three launches instead of the
     fused conversion kernel; 1024: the HMult tensor in full instead of folded into
     the key switch; 2048: the HMult's Q-slot sums stored and the ModDown + rescale
-    row pass as its own launch instead of fused into the rows kernel; 4096: rotations
+    row pass as its own launch instead of fused into the rows kernel; 8192: the
+    HMult's P-slot sums stored and the ModDown INTT row pass as its own launch;
+    4096: rotations
     through the automorphism kernel instead of sigma_g applied while loading; 1..32: each component on the integer kernels), and
     FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting
     gives the oracle's words for a batched HMult (3 items, one lane and two) and a

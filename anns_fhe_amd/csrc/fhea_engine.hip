@@ -739,11 +739,14 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
         ka.gather = gather ? 1 : 0;
         if (late) {
             ka_late = ka;
-            if (ra && !(t.f64_off & 8192)) {
-                // HMult: slot l-1 and the P slots also run the ModDown INTT's row pass
-                // on their sums (the lift of limb l-1 into the dead ModUp region)
+            if (!(t.f64_off & 8192)) {
+                // the P slots (and before the rescale slot l-1) also run the ModDown
+                // INTT's row pass on their sums (the lift of limb l-1 into the dead
+                // ModUp region)
                 ka_late.lift = (double *)coef;
-                launch_ks_rows_f64_early(t, ka_late, s);
+                ka_late.early = 1;
+                if (ra) launch_ks_rows_f64_early(t, ka_late, l - 1, (u32)K + 1, s);
+                else launch_ks_rows_f64_early(t, ka_late, l, (u32)K, s);
             } else if (ra) {
                 launch_ks_rows_f64_q(t, ka, l - 1, (u32)K + 1, s);
             } else {
@@ -812,8 +815,9 @@ static int ks_moddown(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64 *tt,
     hipStream_t s = c->stream;
     const bool cc = t.use_f64 && !(t.f64_off & 17) && moddown_cc(c);
     if (cc) {
-        // INTT rows pass, then its column pass + conversion + the NTT column pass fused
-        if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
+        // INTT rows pass (unless the rows kernel ran it on its sums), then its column
+        // pass + conversion + the NTT column pass fused
+        if (!pb_mid && !(late && late->early)) moddown_rows(c, l, nb, acc, pb);
         moddown_cols(c, l, nb, pb, tt, 0, l);
     } else {
         moddown_intt(c, l, nb, acc, pb, pb_mid);
@@ -966,7 +970,7 @@ static int ks_moddown_rescale(fhea_ctx *c, u32 l, u32 nb, u64 *acc, u64 *pb, u64
         r1.only_pass = 1;
         // (folded: the tensor's d is already in acc, the lift needs acc_{l-1} alone)
         u64 *lift = v;
-        if (!(late && late->lift)) {
+        if (!(late && late->early)) {
             // (else the rows kernel has run these row passes on its sums)
             if (!pb_mid) moddown_rows(c, l, nb, acc, pb);
             const u64 ls = folded ? N : 2 * N;

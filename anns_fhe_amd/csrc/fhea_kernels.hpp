@@ -27,7 +27,7 @@ struct DevTables {
     // rotations through the automorphism kernel instead of sigma_g applied while loading;
     // 2048: the HMult's Q-slot sums stored and the ModDown + rescale row pass as its own
     // launch instead of fused into the key-switch rows kernel (KsRowsArgs::tt); 8192:
-    // the HMult's P-slot and limb l-1 sums stored and the ModDown INTT's row pass as its
+    // the P-slot (and HMult limb l-1) sums stored and the ModDown INTT's row pass as its
     // own launch instead of fused into the rows kernel (KsRowsArgs::lift)
     bool use_f64 = false;
     u32 f64_off = 0;
@@ -147,11 +147,13 @@ struct KsRowsArgs {
     const RescaleOut *ro = nullptr;
     const CombineArgs *cb = nullptr;
     const double *cst = nullptr;
-    // the ModDown's INTT first (row) pass fused in (lift != null; with ta, slot l-1
-    // and the P slots): the sums are not stored; the kernel runs the INTT's row pass
-    // on each and stores it, P slot i of polynomial 2z+s to pb + ((2z+s)*K + i-l)*N
-    // (as doubles), the rescaled limb l-1 to lift + (2z+s)*N
+    // the ModDown's INTT first (row) pass fused in (launch_ks_rows_f64_early; the P
+    // slots, and with ta slot l-1): the sums are not stored; the kernel runs the
+    // INTT's row pass on each and stores it, P slot i of polynomial 2z+s to
+    // pb + ((2z+s)*K + i-l)*N (as doubles), the rescaled limb l-1 to lift + (2z+s)*N;
+    // early = 1 tells the ModDown that these row passes are done
     double *lift = nullptr;
+    u32 early = 0;
 };
 // true when launch_ks_rows_f64 has kernels for the context's current splits
 bool ks_rows_fusable(const DevTables &t);
@@ -159,8 +161,9 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
 // the same over Q slots slot0 .. slot0+nslots-1 only (mixed chains: the Q slots
 // whose primes are below 2^50)
 void launch_ks_rows_f64_q(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s);
-// slots l-1 .. l+K-1 with the ModDown INTT's row pass fused in (KsRowsArgs::lift)
-void launch_ks_rows_f64_early(const DevTables &t, const KsRowsArgs &a, hipStream_t s);
+// slots slot0 .. slot0+nslots-1 (the P slots, and slot l-1 with ta) with the ModDown
+// INTT's row pass fused in (KsRowsArgs::lift)
+void launch_ks_rows_f64_early(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s);
 
 // Basis conversion fused between the column passes of the transforms around it
 // (fhea_conv_cols.hip; FP64 path, two-pass N, forward and inverse column passes

@@ -770,7 +770,8 @@ __device__ __forceinline__ void fwd_rows_to_lds(const __attribute__((address_spa
 // galois_src); 3 as 1 and 4 as 2, with the last row pass of the ModDown + rescale
 // (3) or of the ModDown (4) and its epilogue in place of the stores of the sums
 // (KsRowsArgs::tt); 5 as 1, with the ModDown INTT's first (row) pass in place of the
-// stores (KsRowsArgs::lift); one instance each, so each keeps its own register budget.
+// stores (KsRowsArgs::lift; P slots have no own limb, so rotations use it too); one
+// instance each, so each keeps its own register budget.
 template <int LOGRF, int LOGRI, bool PSLOT, int OWN>
 __global__ __launch_bounds__(F_TPB) FHEA_KS_ROWS_ATTR __attribute__((amdgpu_waves_per_eu(PSLOT ? 1 : 3))) void k_ks_rows_f64(KsRowsArgs a, const double *__restrict__ twf,
                                                      const double *__restrict__ itwf,
@@ -1117,12 +1118,12 @@ void launch_ks_rows_f64(const DevTables &t, const KsRowsArgs &a, hipStream_t s) 
 #undef FHEA_KS_ROWS
 }
 
-void launch_ks_rows_f64_early(const DevTables &t, const KsRowsArgs &a, hipStream_t s) {
+void launch_ks_rows_f64_early(const DevTables &t, const KsRowsArgs &a, u32 slot0, u32 nslots, hipStream_t s) {
     u32 lf, li;
     rows_logs(t, lf, li);
 #define FHEA_KS_EARLY(LF, LI)                                                          \
     if (lf == LF && li == LI) {                                                        \
-        launch_ks_rows_own<LF, LI, false, 5>(t, a, a.l - 1, a.K + 1, s);               \
+        launch_ks_rows_own<LF, LI, false, 5>(t, a, slot0, nslots, s);                  \
         return;                                                                        \
     }
 #define FHEA_KS_EARLY_F(LF) FHEA_KS_EARLY(LF, 6) FHEA_KS_EARLY(LF, 7) FHEA_KS_EARLY(LF, 8) FHEA_KS_EARLY(LF, 9)

@@ -196,7 +196,8 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     fused conversion kernel; 1024: the HMult tensor in full instead of folded into
     the key switch; 2048: the HMult's Q-slot sums stored and the ModDown + rescale
     row pass as its own launch instead of fused into the rows kernel; 8192: the
-    HMult's P-slot sums stored and the ModDown INTT row pass as its own launch;
+    P-slot sums (HMult and rotations) stored and the ModDown INTT row pass as its
+    own launch;
     4096: rotations
     through the automorphism kernel instead of sigma_g applied while loading; 1..32: each component on the integer kernels), and
     FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting

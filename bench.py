@@ -349,10 +349,20 @@ def bench_ntt256(args):
     from anns_fhe_amd.driver import FheContext
     P = 256
     T = P * L * N
-    ctx = FheContext(N, L, 1, 1, q_bits=50, poly_words=T + 8, key_words=8)
+    PT = 64  # polynomials the split tuner times (its outputs go past the batch)
+    ctx = FheContext(N, L, 1, 1, q_bits=50, poly_words=T + PT * L * N + 8, key_words=8)
     rng = np.random.default_rng(42)
     for p in range(P):
         ctx.write_poly(p * L * N, rand_limbs(rng, ctx.q, N))
+    schedule = None
+    if not args.no_tune:
+        # the splits per direction and the workgroups per CU, chosen by measurement on
+        # the first 64 polynomials (every candidate's words checked against the default's)
+        from anns_fhe_amd import autotune
+        t0 = time.perf_counter()
+        autotune.tune_ntt(ctx, T, PT, L, src=0, log=log)
+        schedule = autotune.current(ctx)
+        log(f"ntt256: autotune {schedule} in {time.perf_counter() - t0:.1f} s")
     for _ in range(args.warmup):
         ctx.ntt_batch(0, 0, P, L, inverse=False)
         ctx.ntt_batch(0, 0, P, L, inverse=True)
@@ -377,7 +387,8 @@ def bench_ntt256(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (uniform residues)",
         "config": {"workload": "batched forward then inverse negacyclic NTT, 256 polys x 24 limbs, N=2^16, "
-                               "50-bit primes, in place in HBM", "N": N, "L": L, "polys": P},
+                               "50-bit primes, in place in HBM", "N": N, "L": L, "polys": P,
+                   "schedule": schedule},
         "roofline": {"bound": "hbm", "kernel": "k_ntt64f (forward and inverse, COLS + ROWS passes)",
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,

@@ -29,8 +29,12 @@
 //            <= q/2 + 0.375 B_s (twiddles formed as products, see stage_twiddles),
 //            so the four stages reach at most 7.01q < 8q; then every value is
 //            centred (|x| <= q/2 + 2);
-//   inverse: the sum branch is centred every stage, the difference (<= 3q) is
-//            multiplied back to <= 1.25q.
+//   inverse: in the rounds whose twiddles are formed as products (the first
+//            round along rows) the sum branch is centred every stage and the
+//            difference (<= 4q) is multiplied back to <= 2q; in the rounds that load
+//            canonical twiddles (a product then stays <= q/2 + 1) the sums are centred
+//            at every second stage and at the round's last: from |x| <= 2q at the
+//            round's start they stay below 8q < 2^53, so they add exactly.
 //
 // Fused ends (used by the rescale and ModDown pipelines to save HBM passes):
 //   FUSE_RESCALE_IN   forward COLS prologue: x = centred(v[j]) * T_i (rescale.cpp:43-56)
@@ -148,7 +152,8 @@ __device__ __forceinline__ void round_arith(double (&x)[16], const TileGeom &g, 
                 x[u2] = a - rr;
             } else {
                 const double a = x[u], bb = x[u2];
-                x[u] = reduce_f64(a + bb, g.q, g.qinv);
+                const bool centre = FACT || (st & 1) || st == R::NS - 1;
+                x[u] = centre ? reduce_f64(a + bb, g.q, g.qinv) : a + bb;
                 x[u2] = mulmod_f64(a - bb, ww, wwq, g.q);
             }
         }

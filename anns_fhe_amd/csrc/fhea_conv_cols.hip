@@ -155,20 +155,24 @@ __device__ __forceinline__ u32 col_gidx(u32 e, u32 logS, u32 c0) {
 }
 
 // All rounds of a column transform after round 0 (exchanges included); `xb` is the
-// running LDS buffer index.
-template <bool INV, int LOGR, int W, int LOGC, int RD>
+// running LDS buffer index.  The transform's view of the tile: LOGR row bits above
+// LOGCV column bits, over a limb of 2^logSV columns (its twiddles); the words' places
+// in the limb follow the tile's own geometry (LOGCG column bits, 2^logSG columns),
+// which differs from the view when the inverse column pass is one stage shorter
+// than the forward one (then element bit LOGCG is a column bit of the view).
+template <bool INV, int LOGR, int W, int LOGCV, int LOGCG, int RD>
 __device__ __forceinline__ void rounds_from(double (&x)[1 << W], double *lds, u32 &xb, u32 tid,
                                             const double *__restrict__ tw, double q, double qinv, u32 logN,
-                                            u32 logS, u32 c0) {
+                                            u32 logSV, u32 logSG, u32 c0) {
     constexpr int NR = RoundW<INV, LOGR, W, 0>::NR;
     if constexpr (RD < NR) {
-        constexpr u32 TILE = 1u << (LOGC + LOGR);
+        constexpr u32 TILE = 1u << (LOGCV + LOGR);
         constexpr u32 PADT = TILE + TILE / 4;  // cc_pad's span
-        exch<INV, LOGR, W, LOGC, RD - 1, RD>(x, lds + xb * PADT, tid);
+        exch<INV, LOGR, W, LOGCV, RD - 1, RD>(x, lds + xb * PADT, tid);
         xb ^= 1u;
-        constexpr u32 PN = LOGC + RoundW<INV, LOGR, W, RD>::P;
-        round_w<INV, LOGR, W, RD>(x, tw, q, qinv, logN, logS, col_gidx<LOGC>(elem_base<W>(tid, PN), logS, c0));
-        rounds_from<INV, LOGR, W, LOGC, RD + 1>(x, lds, xb, tid, tw, q, qinv, logN, logS, c0);
+        constexpr u32 PN = LOGCV + RoundW<INV, LOGR, W, RD>::P;
+        round_w<INV, LOGR, W, RD>(x, tw, q, qinv, logN, logSV, col_gidx<LOGCG>(elem_base<W>(tid, PN), logSG, c0));
+        rounds_from<INV, LOGR, W, LOGCV, LOGCG, RD + 1>(x, lds, xb, tid, tw, q, qinv, logN, logSV, logSG, c0);
     }
 }
 
@@ -177,18 +181,24 @@ __device__ __forceinline__ void rounds_from(double (&x)[1 << W], double *lds, u3
 // 8 * 2^(4-LOGC) workgroups (dispatched round-robin over the 8 XCDs) they run at the
 // same time on one XCD and share its L2 lines (128-byte rows).
 // WV > 0: register budget of WV waves per SIMD (the compiler's choice otherwise)
-template <int LOGR, int W, int LOGC, bool MDRS, int WV>
+// LOGRI: stages of the inverse column pass (LOGR, or LOGR - 1 when the inverse's
+// first pass is the longer one): the tile's 2^LOGC columns of 2^LOGR rows are then,
+// for the inverse, 2^(LOGC+1) columns of 2^LOGRI rows (columns c0.. and c0 + 2^logS..)
+template <int LOGR, int LOGRI, int W, int LOGC, bool MDRS, int WV>
 __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu_waves_per_eu(WV > 0 ? WV : 1))) void k_conv_cols_f64(
     ConvColsArgs a, const double *__restrict__ twf, const double *__restrict__ itwf,
     const ModParams *__restrict__ mp, u32 logN, u32 n_work) {
     constexpr int E = 1 << W;
     constexpr u32 TILE = 1u << (LOGC + LOGR);
     constexpr u32 PADT = TILE + TILE / 4;  // cc_pad's span
-    constexpr int NR = RoundW<true, LOGR, W, 0>::NR;
-    constexpr u32 PI0 = LOGC + RoundW<true, LOGR, W, 0>::P;       // input layout (inverse round 0)
+    static_assert(LOGRI == LOGR || LOGRI == LOGR - 1, "inverse column pass of LOGR or LOGR - 1 stages");
+    constexpr int LOGCI = LOGC + (LOGR - LOGRI);                     // the inverse's column bits
+    constexpr int NR = RoundW<false, LOGR, W, 0>::NR;
+    constexpr int NRI = RoundW<true, LOGRI, W, 0>::NR;
+    constexpr u32 PI0 = LOGCI + RoundW<true, LOGRI, W, 0>::P;     // input layout (inverse round 0)
     constexpr u32 PC = LOGC + RoundW<false, LOGR, W, 0>::P;       // conversion layout
     constexpr u32 PO = LOGC + RoundW<false, LOGR, W, NR - 1>::P;  // output layout (forward last round)
-    static_assert(RoundW<true, LOGR, W, NR - 1>::P == RoundW<false, LOGR, W, 0>::P, "windows must meet");
+    static_assert(LOGCI + RoundW<true, LOGRI, W, NRI - 1>::P == PC, "windows must meet");
     static_assert(W <= 3, "forward values are centred every 2W <= 6 stages (bound 7.53q < 8q)");
     static_assert(CC_MAX_IN <= 8, "conversion sums of at most 8 terms add exactly without centring");
     __shared__ double lds[2 * PADT];
@@ -204,6 +214,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     __shared__ u32 s_oprime[CC_MAX_OUT], s_iprime[CC_MAX_IN];
     const u32 tid = threadIdx.x;
     const u32 logS = logN - LOGR;
+    const u32 logSI = logN - LOGRI;  // the inverse view's columns
     const u32 ntiles = 1u << (logS - LOGC);
     const u32 ebi = elem_base<W>(tid, PI0);
     constexpr u32 LG = (LOGC < 4) ? 4 - LOGC : 0;  // log2 tiles per 16-column block
@@ -267,8 +278,8 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
                 const double2 qq = s_iq[2 * i], cc = s_iq[2 * i + 1];
                 const double q = uniform_f64(qq.x), qinv = uniform_f64(qq.y);
                 const double *tw = itwf + ((size_t)pi << (logN + 1));
-                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jl);
-                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
+                round_w<true, LOGRI, W, 0>(x, tw, q, qinv, logN, logSI, jl);
+                rounds_from<true, LOGRI, W, LOGCI, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logSI, logS, c0);
                 const double cs = uniform_f64(cc.x), csq = uniform_f64(cc.y);
 #pragma unroll
                 for (int u = 0; u < E; ++u) {
@@ -299,15 +310,16 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             double vs[E], x[E];
 #pragma unroll
             for (int u = 0; u < E; ++u) x[u] = lp[jin + (((u32)u * rstep_in) << logS)];
-            round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
-            rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
+            round_w<true, LOGRI, W, 0>(x, tw, q, qinv, logN, logSI, jin);
+            rounds_from<true, LOGRI, W, LOGCI, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logSI, logS, c0);
 #pragma unroll
             for (int u = 0; u < E; ++u) vs[u] = mulmod_f64(x[u], pni, pniq, q);
             if (a.lift_d) {
 #pragma unroll
                 for (int u = 0; u < E; ++u) x[u] = lp[(1u << logN) + jin + (((u32)u * rstep_in) << logS)];
-                round_w<true, LOGR, W, 0>(x, tw, q, qinv, logN, logS, jin);
-                rounds_from<true, LOGR, W, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logS, c0);
+                round_w<true, LOGRI, W, 0>(x, tw, q, qinv, logN, logSI, jin);
+                rounds_from<true, LOGRI, W, LOGCI, LOGC, 1>(x, lds, xb, opaque_tid(tid), tw, q, qinv, logN, logSI, logS,
+                                                           c0);
 #pragma unroll
                 for (int u = 0; u < E; ++u) vs[u] += mulmod_f64(x[u], ni, niq, q);
             }
@@ -378,7 +390,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
             const double *tw = twf + ((size_t)pj << (logN + 1));
             const u32 tl = opaque_tid(tid);
             round_w<false, LOGR, W, 0>(x, tw, q, qinv, logN, logS, col_gidx<LOGC>(elem_base<W>(tl, PC), logS, c0));
-            rounds_from<false, LOGR, W, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, c0);
+            rounds_from<false, LOGR, W, LOGC, LOGC, 1>(x, lds, xb, tl, tw, q, qinv, logN, logS, logS, c0);
             const u32 jo = col_gidx<LOGC>(elem_base<W>(tl, PO), logS, c0);
             auto *dst = gptr_w(dst_base + ((size_t)j << logN));
 #pragma unroll
@@ -387,7 +399,7 @@ __global__ __launch_bounds__(((1u << (LOGC + LOGR)) >> W)) __attribute__((amdgpu
     }
 }
 
-template <int LOGR, int W, int LOGC, bool MDRS, int WV>
+template <int LOGR, int LOGRI, int W, int LOGC, bool MDRS, int WV>
 static void launch_cc_inst(const DevTables &t, const ConvColsArgs &a, hipStream_t s) {
     constexpr u32 TPB = (1u << (LOGC + LOGR)) >> W;
     const u32 ntiles = 1u << (t.logN - LOGR - LOGC);
@@ -395,8 +407,8 @@ static void launch_cc_inst(const DevTables &t, const ConvColsArgs &a, hipStream_
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_cols_f64<LOGR, W, LOGC, MDRS, WV>, TPB, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_cols_f64<LOGR, LOGRI, W, LOGC, MDRS, WV>, TPB,
+                                                         0) != hipSuccess ||
             n <= 0)
             n = 1;
         per_cu = n;
@@ -407,23 +419,29 @@ static void launch_cc_inst(const DevTables &t, const ConvColsArgs &a, hipStream_
     const u32 need = (n_work + GR - 1) & ~(GR - 1);
     if (nwg > need) nwg = need;
     nwg = nwg >= GR ? (nwg & ~(GR - 1)) : nwg;
-    hipLaunchKernelGGL((k_conv_cols_f64<LOGR, W, LOGC, MDRS, WV>), dim3(nwg), dim3(TPB), 0, s, a, t.twf, t.itwf,
-                       t.mp, t.logN, n_work);
+    hipLaunchKernelGGL((k_conv_cols_f64<LOGR, LOGRI, W, LOGC, MDRS, WV>), dim3(nwg), dim3(TPB), 0, s, a, t.twf,
+                       t.itwf, t.mp, t.logN, n_work);
 }
 
-// column-pass length of the two-pass FP64 transforms when forward and inverse agree
-static u32 cols_log(const DevTables &t) {
+// Column-pass lengths of the two-pass FP64 transforms (the first pass of the forward
+// NTT, the second of the inverse), lr forward and lri inverse; lr = 0 where the
+// kernel does not apply (one-pass transforms).
+static void cols_logs(const DevTables &t, u32 &lr, u32 &lri) {
     const u32 lg = t.logN;
     const u32 kf = t.k1_fwd ? t.k1_fwd : lg / 2, ki = t.k1_inv ? t.k1_inv : lg / 2;
-    if ((lg == 13 || lg == 14) && (kf == lg || ki == lg)) return 0;  // one-pass transforms
-    return kf == ki ? kf : 0;
+    lr = lri = 0;
+    if ((lg == 13 || lg == 14) && (kf == lg || ki == lg)) return;  // one-pass transforms
+    lr = kf;
+    lri = ki;
 }
 
 bool conv_cols_fusable(const DevTables &t, u32 max_in) {
     if (!t.use_f64 || (t.f64_off & (1 | 2 | 512)) || t.logN <= 12 || max_in > CC_MAX_IN) return false;
     if (t.L + t.K > CC_MAX_OUT) return false;  // outputs per digit staged in LDS
-    const u32 lr = cols_log(t);
-    return lr >= 6 && lr <= 9 && t.logN - lr >= 4;
+    u32 lr, lri;
+    cols_logs(t, lr, lri);
+    // the inverse column pass as long as the forward one, or one stage shorter
+    return lr >= 6 && lr <= 9 && t.logN - lr >= 4 && (lri == lr || (lri + 1 == lr && lri >= 6));
 }
 
 // Variants (DevTables::cc_var, FHEA_CC_VARIANT): 0 radix-4 rounds on 4-column
@@ -435,20 +453,30 @@ bool conv_cols_fusable(const DevTables &t, u32 max_in) {
 // waves or the compiler's default (127 registers each) 8,537-8,595.  The budget
 // changes the instruction schedule more than the occupancy: in the build before
 // cc_pad the lift instance took 127 registers under both 3 and 4 and ran at 8,627
-// and 8,443 ops/s.
+// and 8,443 ops/s.  An inverse column pass one stage shorter than the forward one
+// takes variant 0's shape.
 template <bool MDRS>
-static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hipStream_t s) {
+static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, u32 lri, hipStream_t s) {
 #define FHEA_CC(LR)                                                                   \
     case LR:                                                                          \
         if (t.cc_var == 3)                                                            \
-            launch_cc_inst<LR, 2, 2, MDRS, 4>(t, a, s);                               \
+            launch_cc_inst<LR, LR, 2, 2, MDRS, 4>(t, a, s);                           \
         else if (t.cc_var == 1)                                                       \
-            launch_cc_inst<LR, 2, 3, MDRS, 0>(t, a, s);                               \
+            launch_cc_inst<LR, LR, 2, 3, MDRS, 0>(t, a, s);                           \
         else if (t.cc_var == 2)                                                       \
-            launch_cc_inst<LR, 2, 3, MDRS, 4>(t, a, s);                               \
+            launch_cc_inst<LR, LR, 2, 3, MDRS, 4>(t, a, s);                           \
         else                                                                          \
-            launch_cc_inst<LR, 2, 2, MDRS, 3>(t, a, s);                               \
+            launch_cc_inst<LR, LR, 2, 2, MDRS, 3>(t, a, s);                           \
         break;
+    if (lri + 1 == lr) {
+        switch (lr) {
+        case 7: launch_cc_inst<7, 6, 2, 2, MDRS, 3>(t, a, s); break;
+        case 8: launch_cc_inst<8, 7, 2, 2, MDRS, 3>(t, a, s); break;
+        case 9: launch_cc_inst<9, 8, 2, 2, MDRS, 3>(t, a, s); break;
+        default: break;
+        }
+        return;
+    }
     switch (lr) {
         FHEA_CC(6)
         FHEA_CC(7)
@@ -461,11 +489,12 @@ static void launch_cc_var(const DevTables &t, const ConvColsArgs &a, u32 lr, hip
 }
 
 void launch_conv_cols_f64(const DevTables &t, const ConvColsArgs &a, hipStream_t s) {
-    const u32 lr = cols_log(t);
+    u32 lr, lri;
+    cols_logs(t, lr, lri);
     if (a.lift_in)
-        launch_cc_var<true>(t, a, lr, s);
+        launch_cc_var<true>(t, a, lr, lri, s);
     else
-        launch_cc_var<false>(t, a, lr, s);
+        launch_cc_var<false>(t, a, lr, lri, s);
 }
 
 }  // namespace fhea

@@ -39,9 +39,11 @@ def rand_key(rng, ctx, key_limbs):
 
 def test_bench_config_batched_hmult_vs_oracle(orc):
     """B=64 HMults sharing one relin key at N=2^16/L=24/K=8/dnum=3, two lanes
-    (items 32..63 on the second stream) with chunks of 16 and of 64, and three lanes
-    (items 0..20, 21..41, 42..63) with chunks of 16: items 0, 15, 16, 31, 32, 63 equal
-    the oracle's eval_mult_relin_rescale."""
+    (items 32..63 on the second stream) with chunks of 16 and of 64, three lanes
+    (items 0..20, 21..41, 42..63) with chunks of 16, and two lanes with the inverse
+    NTT split at 7 (the conversion kernels' inverse column pass one stage shorter than
+    the forward one): items 0, 15, 16, 31, 32, 63 equal the oracle's
+    eval_mult_relin_rescale."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
     N, L, K, dnum, B = 65536, 24, 8, 3, 64
@@ -67,17 +69,21 @@ def test_bench_config_batched_hmult_vs_oracle(orc):
                       b1=(4 * z + 3) * W, d0=out_base + 2 * z * W1, d1=out_base + (2 * z + 1) * W1)
                  for z in range(B)]
         got = {}
-        schedules = ((2, 16), (2, 64), (3, 16))
-        for streams, chunk in schedules:
+        # (streams, chunk, inverse NTT split): split 7 makes the inverse column pass one
+        # stage shorter than the forward one inside the fused conversion kernels
+        schedules = ((2, 16, 0), (2, 64, 0), (3, 16, 0), (2, 16, 7))
+        for streams, chunk, split_inv in schedules:
             ctx.set_tuning(_lib.TUNE_STREAMS, streams)
             ctx.set_tuning(_lib.TUNE_CHUNK, chunk)
-            say("submit streams", streams, "chunk", chunk)
+            ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, split_inv)
+            say("submit streams", streams, "chunk", chunk, "split_inv", split_inv)
             ctx.write_poly(out_base, np.zeros(2 * B * W1, np.uint64))
             ctx.submit(works)
             ctx.sync()
-            got[streams, chunk] = {z: ctx.read_poly(out_base + 2 * z * W1, 2 * W1) for z in sample}
+            got[streams, chunk, split_inv] = {z: ctx.read_poly(out_base + 2 * z * W1, 2 * W1) for z in sample}
         ctx.set_tuning(_lib.TUNE_STREAMS, 1)
         ctx.set_tuning(_lib.TUNE_CHUNK, 16)
+        ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, 0)
         o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
         for z in sample:
             say("oracle item", z)
@@ -184,7 +190,7 @@ def test_integer_path_vs_oracle(orc, N, L, K, dnum, kind):
         ctx.close()
 
 
-@pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (0, 3), (128, 0), (256, 0), (64, 0), (512, 0),
+@pytest.mark.parametrize("switch,cc_variant", [(0, 0), (0, 1), (0, 2), (0, 3), (0, "inv6"), (0, "inv8"), (128, 0), (256, 0), (64, 0), (512, 0),
                                                (512 | 128, 0), (1024, 0), (1024 | 128, 0), (2048, 0), (8192, 0), (4096, 0),
                                                (1 | 2 | 4 | 8 | 16 | 32, 0)])
 def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
@@ -200,11 +206,16 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     own launch;
     4096: rotations
     through the automorphism kernel instead of sigma_g applied while loading; 1..32: each component on the integer kernels), and
-    FHEA_CC_VARIANT the fused conversion kernel's tile shape.  Every setting
-    gives the oracle's words for a batched HMult (3 items, one lane and two) and a
-    rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
+    FHEA_CC_VARIANT the fused conversion kernel's tile shape; "invK" sets the inverse
+    NTT split to K (inv6: the conversion kernels' inverse column pass one stage shorter
+    than the forward one; inv8: one longer, which leaves the conversions unfused).
+    Every setting gives the oracle's words for a batched HMult (3 items, one lane and
+    two) and a rotation at N=2^14, L=8, K=2, dnum=3 (two-pass transforms, 3 digits)."""
     from anns_fhe_amd import _lib
     from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    split_inv = 0
+    if isinstance(cc_variant, str):
+        split_inv, cc_variant = int(cc_variant[3:]), 0
     monkeypatch.setenv("FHEA_F64_OFF", str(switch))
     monkeypatch.setenv("FHEA_CC_VARIANT", str(cc_variant))
     N, L, K, dnum = 16384, 8, 2, 3
@@ -212,9 +223,11 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
     ksz = 2 * dnum * (L + K) * N
     ctx = FheContext(N, L, K, dnum, poly_words=24 * W, key_words=2 * ksz + 8)
     try:
+        if split_inv:
+            ctx.set_tuning(_lib.TUNE_NTT_SPLIT_INV, split_inv)
         drv = FHEHostDriver(ctx)
         o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
-        rng = np.random.default_rng(300 + switch + 7 * cc_variant)
+        rng = np.random.default_rng(300 + switch + 7 * cc_variant + 13 * split_inv)
         relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
         km = np.concatenate([relin, rot])
         ctx.write_key(0, km)
@@ -230,7 +243,7 @@ def test_pipeline_switches_keep_words(orc, monkeypatch, switch, cc_variant):
                       b1=(4 * z + 3) * W, d0=(12 + 2 * z) * W, d1=(13 + 2 * z) * W) for z in range(B)]
         for streams in (1, 2):
             ctx.set_tuning(_lib.TUNE_STREAMS, streams)
-            say("switch", switch, "cc_variant", cc_variant, "streams", streams)
+            say("switch", switch, "cc_variant", cc_variant, "split_inv", split_inv, "streams", streams)
             ctx.submit(works)
             for z in range(B):
                 assert o.eval_mult_relin_rescale(pm, km, 0, L, 4 * z * W, (4 * z + 1) * W, (4 * z + 2) * W,

@@ -26,13 +26,19 @@ KNOBS = {"ntt_split_fwd": _lib.TUNE_NTT_SPLIT_FWD, "ntt_split_inv": _lib.TUNE_NT
          "ntt_wg_per_cu": _lib.TUNE_NTT_WG_PER_CU, "chunk": _lib.TUNE_CHUNK, "streams": _lib.TUNE_STREAMS}
 
 
-def _time(ctx, fn, reps):
+def _time(ctx, fn, reps, rounds=2):
+    """Milliseconds per run of fn: the best of `rounds` timings of `reps` runs each
+    (the minimum is the estimate least disturbed by other activity on the device)."""
     fn()
     ctx.sync()
-    ctx.timer_start()
-    for _ in range(reps):
-        fn()
-    return ctx.timer_stop() / reps
+    best = None
+    for _ in range(rounds):
+        ctx.timer_start()
+        for _ in range(reps):
+            fn()
+        ms = ctx.timer_stop() / reps
+        best = ms if best is None else min(best, ms)
+    return best
 
 
 def ntt_splits(logN):

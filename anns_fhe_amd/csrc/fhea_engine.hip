@@ -1135,7 +1135,10 @@ static u64 mult_relin_rescale_words(const fhea_ctx *c, u32 l, u32 nb) {
     return (u64)nb * 3 * l * c->h.N + std::max<u64>(nb * ks_words(c, l), nb * rescale_words(c, l));
 }
 
-// Host-side argument blocks of several chunks, staged with one copy.
+// Host-side argument blocks of several chunks, staged with one copy.  Batches of
+// more than ARGPACK_ITEMS items run in segments of that many, one copy each, so a
+// block stays well inside the context's staging area.
+static constexpr size_t ARGPACK_ITEMS = 4096;
 struct ArgPack {
     std::vector<char> h;
     template <class T>
@@ -1156,6 +1159,15 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         if (mult_relin_rescale_words(c, l, chunk) > fixed->words) return FHEA_ERR_SCRATCH;
     }
     if (items.empty()) return FHEA_OK;
+    if (items.size() > ARGPACK_ITEMS) {
+        for (size_t off = 0; off < items.size(); off += ARGPACK_ITEMS) {
+            const std::vector<MultItem> seg(items.begin() + off,
+                                            items.begin() + std::min(items.size(), off + ARGPACK_ITEMS));
+            const int st = mult_relin_rescale_lane(c, l, key, key_limbs, seg, fixed);
+            if (st) return st;
+        }
+        return FHEA_OK;
+    }
     // one workspace for every chunk (sized for the largest), so that all chunks'
     // argument arrays can be formed, and staged with one copy, before any launch
     u64 *D = nullptr;
@@ -1369,6 +1381,15 @@ struct AutoItem {
 // eval_rotate_single_step (fhe_host_driver.cpp:331-359) with explicit Galois
 // elements: sigma(c0), sigma(c1) -> KS(sigma(c1)) -> (sigma(c0) + ks0, ks1).
 static int automorphism_ks(fhea_ctx *c, u32 l, u32 key_limbs, const std::vector<AutoItem> &items) {
+    if (items.size() > ARGPACK_ITEMS) {
+        for (size_t off = 0; off < items.size(); off += ARGPACK_ITEMS) {
+            const std::vector<AutoItem> seg(items.begin() + off,
+                                            items.begin() + std::min(items.size(), off + ARGPACK_ITEMS));
+            const int st = automorphism_ks(c, l, key_limbs, seg);
+            if (st) return st;
+        }
+        return FHEA_OK;
+    }
     const u64 N = c->h.N;
     const bool gok = ks_gather_ok(c, l);
     // the chunks that read sigma_g while loading: their argument arrays first, all

@@ -422,3 +422,48 @@ def test_smallest_n_vs_oracle(orc, N, chain):
         assert np.array_equal(ctx.read_poly(12 * W, W), pm[12 * W:13 * W])
     finally:
         ctx.close()
+
+
+def test_batches_beyond_one_argument_block(orc):
+    """A submit of more items than one staged argument block holds (4,096) runs in
+    segments: 4,100 HMults and 4,100 rotations at N=2^13, L=3 sharing their inputs;
+    items 0, 4095, 4096 and 4099 (either side of the segment boundary) equal the
+    oracle's words."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    N, L, K, dnum, B = 8192, 3, 1, 3, 4100
+    W, W1 = L * N, (L - 1) * N
+    ksz = 2 * dnum * (L + K) * N
+    out_m = 6 * W                 # HMult outputs: B pairs of L-1 limbs
+    out_r = out_m + 2 * B * W1    # rotation outputs: B pairs of L limbs
+    ctx = FheContext(N, L, K, dnum, poly_words=out_r + 2 * B * W + 8, key_words=2 * ksz + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
+        rng = np.random.default_rng(4100)
+        relin, rot = rand_key(rng, ctx, L), rand_key(rng, ctx, L)
+        km = np.concatenate([relin, rot])
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        g1 = ctx.galois_for_step(1)
+        data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+        ctx.write_poly(0, data)
+        pm = np.zeros(6 * W + 2 * W, np.uint64)
+        pm[:4 * W] = data
+        say("submit", B, "HMults")
+        ctx.submit([dict(kind=1, limbs=L, a0=0, a1=W, b0=2 * W, b1=3 * W,
+                         d0=out_m + 2 * z * W1, d1=out_m + (2 * z + 1) * W1) for z in range(B)])
+        say("submit", B, "rotations")
+        ctx.submit([dict(kind=2, limbs=L, a0=0, a1=W, galois=g1, evk=ksz, key_limbs=L,
+                         d0=out_r + 2 * z * W, d1=out_r + (2 * z + 1) * W) for z in range(B)])
+        ctx.sync()
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, 0, W, 2 * W, 3 * W, 4 * W, 5 * W, L) == 0
+        assert o.eval_automorphism_ks(pm, km, g1, ksz, L, 0, W, 6 * W, 7 * W, L) == 0
+        for z in (0, 4095, 4096, B - 1):
+            say("item", z)
+            gm = ctx.read_poly(out_m + 2 * z * W1, 2 * W1)
+            assert np.array_equal(gm[:W1], pm[4 * W:4 * W + W1]), ("hmult c0", z)
+            assert np.array_equal(gm[W1:], pm[5 * W:5 * W + W1]), ("hmult c1", z)
+            gr = ctx.read_poly(out_r + 2 * z * W, 2 * W)
+            assert np.array_equal(gr, pm[6 * W:8 * W]), ("rotation", z)
+    finally:
+        ctx.close()

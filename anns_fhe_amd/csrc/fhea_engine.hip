@@ -647,7 +647,7 @@ static void moddown_rows(fhea_ctx *c, u32 l, u32 nb, const u64 *acc, u64 *pb) {
 static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base, u64 c_stride,
                    const std::vector<const u64 *> &keys, const std::vector<CombineArgs> &comb,
                    u64 *ws, const RescaleAfter *ra = nullptr, const std::vector<TensorArgs> *fold = nullptr,
-                   const std::vector<GatherArgs> *gather = nullptr, bool gather_late = false,
+                   const std::vector<GatherArgs> *gather = nullptr, bool late_allowed = false,
                    const PreArgs *pre = nullptr) {
     const DevTables &t = c->dt;
     const u64 N = t.N, K = t.K;
@@ -711,14 +711,15 @@ static int ks_core(fhea_ctx *c, u32 l, u32 key_limbs, u32 nb, const u64 *c_base,
     // FHEA_F64_OFF bit 64 keeps the inner product as its own kernel
     const bool fused = t.use_f64 && !(t.f64_off & (1 | 4 | 64)) && np <= 8 && ks_rows_fusable(t);
     // HMult (the tensor folded in, the rescale after) and rotations (sigma_g read
-    // while loading, gather_late: no output overlaps c1, which the second launch
-    // reads), both conversions fused: the rows kernel runs twice, first for the P
-    // slots (and slot l-1 before a rescale), whose sums the ModDown transforms and
-    // converts, then for the other Q slots with the last row pass of the ModDown
+    // while loading), both conversions fused: the rows kernel runs twice, first for
+    // the P slots (and slot l-1 before a rescale), whose sums the ModDown transforms
+    // and converts, then for the other Q slots with the last row pass of the ModDown
     // (+ rescale) and its epilogue fused in, so the Q slots' sums never go through
-    // HBM (FHEA_F64_OFF bit 2048 keeps one launch)
-    const bool late_ok = fused && cc && !(t.f64_off & (16 | 128 | 2048)) && moddown_cc(c);
-    const bool late = late_ok && ((fold && ra && l >= 2) || (gather && gather_late && !ra));
+    // HBM (FHEA_F64_OFF bit 2048 keeps one launch).  The second launch reads the
+    // items' inputs while it writes their outputs: late_allowed, from the caller,
+    // says that no output overlaps an input other than at the same words.
+    const bool late_ok = fused && cc && late_allowed && !(t.f64_off & (16 | 128 | 2048)) && moddown_cc(c);
+    const bool late = late_ok && ((fold && ra && l >= 2) || (gather && !ra));
     KsRowsArgs ka_late{};
     if (fused) {
         // first forward pass of the converted digit limbs; the second runs inside the
@@ -1183,6 +1184,7 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
     const bool prod = fold && modup_cc(c, l);
     struct Chunk {
         u32 off, nb;
+        bool late;  // no output overlaps an input other than at the same words
         std::vector<TensorArgs> ta;
         std::vector<CombineArgs> cb;
         std::vector<u64 *> outs;
@@ -1195,6 +1197,14 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         ch.off = (u32)off;
         ch.nb = (u32)std::min<size_t>(chunk, items.size() - off);
         const u32 nb = ch.nb;
+        ch.late = true;
+        for (u32 z = 0; z < nb && ch.late; ++z) {
+            const MultItem &it = items[off + z];
+            const u64 Wi = (u64)l * N, Wo = (u64)(l - 1) * N;
+            for (const u64 *o : {(const u64 *)it.d0, (const u64 *)it.d1})
+                for (const u64 *in : {it.a0, it.a1, it.b0, it.b1})
+                    if (o != in && o < in + Wi && in < o + Wo) ch.late = false;
+        }
         ch.ta.resize(nb);
         ch.cb.resize(nb);
         ch.outs.resize(2 * nb);
@@ -1232,7 +1242,7 @@ static int mult_relin_rescale_lane(fhea_ctx *c, u32 l, const u64 *key, u32 key_l
         if (!prod) launch_tensor(c->dt, pa.fold, nb, l, c->stream);
         const RescaleAfter ra{D, 3 * l * N, l * N, &ch.outs};
         const int st = ks_core(c, l, key_limbs, nb, D + 2 * l * N, 3 * l * N, keys, ch.cb, rest, &ra,
-                               fold ? &ch.ta : nullptr, nullptr, false, &pa);
+                               fold ? &ch.ta : nullptr, nullptr, ch.late, &pa);
         if (st) return st;
     }
     return check_hip(hipGetLastError());

@@ -467,3 +467,35 @@ def test_batches_beyond_one_argument_block(orc):
             assert np.array_equal(gr, pm[6 * W:8 * W]), ("rotation", z)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("case", ["same_words", "shifted_d0", "shifted_d1"])
+def test_hmult_outputs_over_own_inputs(orc, case):
+    """An item may write over its own inputs (fhea.h).  The Q-slot rows kernel reads
+    an item's inputs while it writes its outputs, which is safe when an output lies
+    on an input's own words (same_words: d0 = a0, d1 = a1) and not when it is shifted
+    against it; then the pipeline stores the sums and runs the ModDown + rescale row
+    pass on its own.  N=2^14, L=8, K=2, dnum=3; every case = oracle."""
+    from anns_fhe_amd.driver import FheContext, FHEHostDriver
+    N, L, K, dnum = 16384, 8, 2, 3
+    W, W1 = L * N, (L - 1) * N
+    ctx = FheContext(N, L, K, dnum, poly_words=8 * W, key_words=2 * dnum * (L + K) * N + 8)
+    try:
+        drv = FHEHostDriver(ctx)
+        o = orc.OracleCtx(N, ctx.q, ctx.p, dnum)
+        rng = np.random.default_rng({"same_words": 1, "shifted_d0": 2, "shifted_d1": 3}[case])
+        km = rand_key(rng, ctx, L)
+        ctx.write_key(0, km)
+        drv.set_relin_key(0, L)
+        data = np.concatenate([rand_poly(rng, ctx.q, N).reshape(-1) for _ in range(4)])
+        ctx.write_poly(0, data)
+        pm = np.zeros(8 * W, np.uint64)
+        pm[:4 * W] = data
+        a0, a1, b0, b1 = 0, W, 2 * W, 3 * W
+        d0, d1 = {"same_words": (a0, a1), "shifted_d0": (a0 + N, 5 * W), "shifted_d1": (5 * W, b1 + 3 * N)}[case]
+        ctx.submit([dict(kind=1, limbs=L, a0=a0, a1=a1, b0=b0, b1=b1, d0=d0, d1=d1)])
+        assert o.eval_mult_relin_rescale(pm, km, 0, L, a0, a1, b0, b1, d0, d1, L) == 0
+        for d in (d0, d1):
+            assert np.array_equal(ctx.read_poly(d, W1), pm[d:d + W1]), (case, d)
+    finally:
+        ctx.close()
